@@ -1,0 +1,196 @@
+"""ctypes binding of libavbd_hip.so (the C ABI in include/avbd_hip.h).
+
+``HipWorld`` is a thin owner of one ``avbd_handle``: upload a State, step it
+on the GPU, download it. There is no CPU fallback here: if the library or a
+GPU is missing, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .state import (BODY, CONFIG, JOINT, MANIFOLD, SPRING, STATS, State, check_layouts)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libavbd_hip.so")
+
+_p = C.c_void_p
+_LIB = None
+
+AVBD_OK = 0
+ERRORS = {-1: "invalid argument", -2: "capacity", -3: "hip", -4: "no device", -5: "unsupported", -6: "injected"}
+
+
+class AvbdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"avbd error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return None if a is None or a.size == 0 else a.ctypes.data_as(_p)
+
+
+def load(path: str | None = None):
+    """Load the HIP library (raises OSError if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        lib = C.CDLL(path or LIB_PATH)
+        lib.avbd_last_error.restype = C.c_char_p
+        lib.avbd_create.argtypes = [_p, C.POINTER(_p)]
+        lib.avbd_destroy.argtypes = [_p]
+        lib.avbd_set_params.argtypes = [_p, _p]
+        lib.avbd_upload_state.argtypes = [_p, _p, C.c_int32, _p, C.c_int32, _p, C.c_int32, _p, C.c_int32, C.c_int64]
+        lib.avbd_state_counts.argtypes = [_p, _p, _p, _p, _p, _p]
+        lib.avbd_download_state.argtypes = [_p, _p, _p, _p, _p]
+        for n in ("avbd_synchronize", "avbd_prepare", "avbd_iterate", "avbd_finish"):
+            getattr(lib, n).argtypes = [_p]
+        lib.avbd_step.argtypes = [_p, C.c_int32]
+        lib.avbd_get_stats.argtypes = [_p, _p]
+        lib.avbd_get_colors.argtypes = [_p, _p, C.c_int32]
+        lib.avbd_get_hits.argtypes = [_p, _p, C.c_int32, _p]
+        lib.avbd_debug_inject_fault.argtypes = [_p, C.c_int32]
+        lib.avbd_default_params.argtypes = [_p]
+        lib.avbd_make_box_body.argtypes = [_p, C.c_float, C.c_float, _p, _p, _p, _p, _p]
+        lib.avbd_make_joint.argtypes = [_p, _p, C.c_int32, C.c_int32, C.c_int64, _p, _p, C.c_float, C.c_float,
+                                        C.c_float, _p]
+        lib.avbd_make_spring.argtypes = [_p, _p, C.c_int32, C.c_int32, C.c_int64, _p, _p, C.c_float, C.c_float, _p]
+        lib.avbd_sizeof.argtypes = [C.c_int]
+        check_layouts(lib.avbd_sizeof)
+        _LIB = lib
+    return _LIB
+
+
+def _check(code: int):
+    if code != AVBD_OK:
+        raise AvbdError(code, load().avbd_last_error().decode())
+
+
+def _f32(x, n):
+    return np.ascontiguousarray(np.asarray(x, np.float32).reshape(n))
+
+
+class Maker:
+    """Record builders of the library (Rigid/Joint/Spring constructor fields)."""
+
+    def make_box_body(self, size, density, friction, pos, rot, vel, angvel):
+        lib = load()
+        out = np.zeros((), BODY)
+        _check(lib.avbd_make_box_body(_ptr(_f32(size, 3)), float(density), float(friction), _ptr(_f32(pos, 3)),
+                                      _ptr(_f32(rot, 4)), _ptr(_f32(vel, 3)), _ptr(_f32(angvel, 3)),
+                                      out.ctypes.data_as(_p)))
+        return out
+
+    def joint_spring_derived(self, rec_a, rec_b, ra, rb):
+        lib = load()
+        j = np.zeros((), JOINT)
+        a = None if rec_a is None else np.ascontiguousarray(rec_a)
+        b = np.ascontiguousarray(rec_b)
+        _check(lib.avbd_make_joint(_ptr(a), b.ctypes.data_as(_p), 0, 0, 0, _ptr(_f32(ra, 3)), _ptr(_f32(rb, 3)),
+                                   float("inf"), 0.0, float("inf"), j.ctypes.data_as(_p)))
+        rest = -1.0
+        if rec_a is not None:
+            s = np.zeros((), SPRING)
+            _check(lib.avbd_make_spring(a.ctypes.data_as(_p), b.ctypes.data_as(_p), 0, 0, 0, _ptr(_f32(ra, 3)),
+                                        _ptr(_f32(rb, 3)), 1.0, -1.0, s.ctypes.data_as(_p)))
+            rest = float(s["rest"])
+        return float(j["torque_arm"]), rest
+
+
+class HipWorld:
+    """One resident GPU solver instance."""
+
+    def __init__(self, params, max_bodies: int, max_joints: int = 0, max_springs: int = 0,
+                 max_manifolds: int = 0, max_pairs: int = 0, device: int = 0, timing: bool = False):
+        lib = load()
+        self._lib = lib
+        cfg = np.zeros((), CONFIG)
+        cfg["params"] = params
+        cfg["max_bodies"] = max_bodies
+        cfg["max_joints"] = max_joints
+        cfg["max_springs"] = max_springs
+        cfg["max_manifolds"] = max_manifolds
+        cfg["max_pairs"] = max_pairs
+        cfg["device"] = device
+        cfg["enable_timing"] = int(timing)
+        h = _p()
+        _check(lib.avbd_create(cfg.ctypes.data_as(_p), C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def for_state(cls, state: State, params, **kw) -> "HipWorld":
+        w = cls(params, max_bodies=max(len(state.bodies), 1), max_joints=len(state.joints),
+                max_springs=len(state.springs),
+                max_manifolds=kw.pop("max_manifolds", 0) or max(12 * len(state.bodies) + 1024,
+                                                                2 * len(state.manifolds)), **kw)
+        w.upload(state)
+        return w
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.avbd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, params):
+        p = np.ascontiguousarray(params)
+        _check(self._lib.avbd_set_params(self._h, p.ctypes.data_as(_p)))
+
+    def upload(self, state: State):
+        self._keep = state
+        _check(self._lib.avbd_upload_state(self._h, _ptr(state.bodies), len(state.bodies), _ptr(state.joints),
+                                           len(state.joints), _ptr(state.springs), len(state.springs),
+                                           _ptr(state.manifolds), len(state.manifolds),
+                                           C.c_int64(state.next_serial)))
+
+    def step(self, n: int = 1):
+        _check(self._lib.avbd_step(self._h, n))
+
+    def prepare(self):
+        _check(self._lib.avbd_prepare(self._h))
+
+    def iterate(self):
+        _check(self._lib.avbd_iterate(self._h))
+
+    def finish(self):
+        _check(self._lib.avbd_finish(self._h))
+
+    def synchronize(self):
+        _check(self._lib.avbd_synchronize(self._h))
+
+    def counts(self):
+        v = [C.c_int32() for _ in range(4)]
+        s = C.c_int64()
+        _check(self._lib.avbd_state_counts(self._h, *[C.byref(x) for x in v], C.byref(s)))
+        return [x.value for x in v] + [s.value]
+
+    def state(self) -> State:
+        nb, nj, ns, nm, serial = self.counts()
+        b, j, s, m = np.zeros(nb, BODY), np.zeros(nj, JOINT), np.zeros(ns, SPRING), np.zeros(nm, MANIFOLD)
+        _check(self._lib.avbd_download_state(self._h, _ptr(b), _ptr(j), _ptr(s), _ptr(m)))
+        return State(b, j, s, m, serial)
+
+    def stats(self) -> np.ndarray:
+        out = np.zeros((), STATS)
+        _check(self._lib.avbd_get_stats(self._h, out.ctypes.data_as(_p)))
+        return out
+
+    def colors(self) -> np.ndarray:
+        nb = self.counts()[0]
+        out = np.zeros(nb, np.int32)
+        _check(self._lib.avbd_get_colors(self._h, _ptr(out), nb))
+        return out
+
+    def hits(self) -> np.ndarray:
+        n = C.c_int32()
+        _check(self._lib.avbd_get_hits(self._h, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 3), np.int32)
+        _check(self._lib.avbd_get_hits(self._h, _ptr(out), n.value, C.byref(n)))
+        return out
+
+    def inject_fault(self, code: int = 1):
+        _check(self._lib.avbd_debug_inject_fault(self._h, code))

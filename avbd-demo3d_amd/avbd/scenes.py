@@ -1,0 +1,247 @@
+"""Seeded scene generators for the AVBD step workloads.
+
+A Scene lists bodies and forces in creation order, the way a reference scene
+function builds them through ``new Rigid`` / ``new Joint`` / ``new Spring``
+(scenes.h:59-136 show the pattern). ``build_state`` turns a scene into the
+resident state tables of include/avbd_hip.h:
+
+* bodies are stored in body-list order: the reference prepends each new body
+  (rigid.cpp:29-31), so creation index c becomes list index N-1-c;
+* each force's serial is its creation index, which fixes the per-body force
+  order (force.cpp:25-36 prepends).
+
+Workloads (gravity -9.81 along -z, the reference's up axis is +z):
+
+* ``stack``   10 unit boxes resting on a ground box, horizontal jitter +-0.01 m.
+* ``pyramid_chain``  a 20-wide pyramid of unit boxes (210) plus a 1000-link
+  chain of 0.2 m boxes joined by ball joints, first link anchored to the world,
+  with springs (rest 0.2 m, stiffness log-uniform in [1e3, 1e5]) on every other
+  adjacent link pair.
+* ``pile``    boxes on a jittered 3D grid (half-extents U[0.25, 1] m, uniform
+  random orientations, velocities N(0, 0.5) m/s) dropping onto a ground box
+  inside four static walls. 50,000 boxes is the headline workload; the
+  generator takes any count.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .state import BODY, JOINT, SPRING, State
+
+FORCE_JOINT = 1
+FORCE_SPRING = 2
+
+
+@dataclasses.dataclass
+class Scene:
+    name: str
+    size: np.ndarray      # (n,3) full widths
+    density: np.ndarray   # (n,)
+    friction: np.ndarray  # (n,)
+    pos: np.ndarray       # (n,3)
+    rot: np.ndarray       # (n,4) x,y,z,w
+    vel: np.ndarray       # (n,3)
+    angvel: np.ndarray    # (n,3)
+    ftype: np.ndarray     # (m,) 1 joint, 2 spring
+    fa: np.ndarray        # (m,) creation index of body A, -1 = world
+    fb: np.ndarray        # (m,)
+    ra: np.ndarray        # (m,3)
+    rb: np.ndarray        # (m,3)
+    p0: np.ndarray        # joint: stiffnessLin; spring: stiffness
+    p1: np.ndarray        # joint: stiffnessAng; spring: rest (-1 = current distance)
+    p2: np.ndarray        # joint: fracture
+    gravity: float = -9.81
+    steps: int = 300
+
+    @property
+    def num_bodies(self) -> int:
+        return int(self.size.shape[0])
+
+    @property
+    def num_dynamic(self) -> int:
+        return int((self.density > 0).sum())
+
+
+class _Builder:
+    def __init__(self):
+        self.b = {k: [] for k in ("size", "density", "friction", "pos", "rot", "vel", "angvel")}
+        self.f = {k: [] for k in ("ftype", "fa", "fb", "ra", "rb", "p0", "p1", "p2")}
+
+    def body(self, size, density, friction, pos, rot=(0, 0, 0, 1), vel=(0, 0, 0), angvel=(0, 0, 0)) -> int:
+        self.b["size"].append(size)
+        self.b["density"].append(density)
+        self.b["friction"].append(friction)
+        self.b["pos"].append(pos)
+        self.b["rot"].append(rot)
+        self.b["vel"].append(vel)
+        self.b["angvel"].append(angvel)
+        return len(self.b["size"]) - 1
+
+    def joint(self, a, b, ra, rb, k_lin=math.inf, k_ang=0.0, fracture=math.inf):
+        for k, v in zip(self.f, (FORCE_JOINT, a, b, ra, rb, k_lin, k_ang, fracture)):
+            self.f[k].append(v)
+
+    def spring(self, a, b, ra, rb, stiffness, rest=-1.0):
+        for k, v in zip(self.f, (FORCE_SPRING, a, b, ra, rb, stiffness, rest, 0.0)):
+            self.f[k].append(v)
+
+    def scene(self, name, gravity, steps) -> Scene:
+        f32 = np.float32
+        n = len(self.b["size"])
+        m = len(self.f["ftype"])
+        return Scene(
+            name=name,
+            size=np.asarray(self.b["size"], f32).reshape(n, 3),
+            density=np.asarray(self.b["density"], f32).reshape(n),
+            friction=np.asarray(self.b["friction"], f32).reshape(n),
+            pos=np.asarray(self.b["pos"], f32).reshape(n, 3),
+            rot=np.asarray(self.b["rot"], f32).reshape(n, 4),
+            vel=np.asarray(self.b["vel"], f32).reshape(n, 3),
+            angvel=np.asarray(self.b["angvel"], f32).reshape(n, 3),
+            ftype=np.asarray(self.f["ftype"], np.int32).reshape(m),
+            fa=np.asarray(self.f["fa"], np.int32).reshape(m),
+            fb=np.asarray(self.f["fb"], np.int32).reshape(m),
+            ra=np.asarray(self.f["ra"], f32).reshape(m, 3),
+            rb=np.asarray(self.f["rb"], f32).reshape(m, 3),
+            p0=np.asarray(self.f["p0"], f32).reshape(m),
+            p1=np.asarray(self.f["p1"], f32).reshape(m),
+            p2=np.asarray(self.f["p2"], f32).reshape(m),
+            gravity=gravity, steps=steps)
+
+
+def _rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def random_quaternions(rng: np.random.Generator, n: int) -> np.ndarray:
+    """Uniform random unit quaternions (Shoemake's subgroup algorithm), x,y,z,w."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    q = np.stack([a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2),
+                  b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)], axis=1)
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return q.astype(np.float32)
+
+
+def stack(seed: int = 1, count: int = 10, jitter: float = 0.01, steps: int = 300) -> Scene:
+    """Stack of unit boxes resting on a static ground box (ground top at z = 0.5)."""
+    rng = _rng(seed)
+    sb = _Builder()
+    sb.body((100, 100, 1), 0.0, 0.5, (0, 0, 0))
+    for i in range(count):
+        jx, jy = rng.uniform(-jitter, jitter, 2)
+        sb.body((1, 1, 1), 1.0, 0.5, (jx, jy, 1.0 + i))
+    return sb.scene(f"stack{count}", -9.81, steps)
+
+
+def pyramid_chain(seed: int = 2, base: int = 20, links: int = 1000, steps: int = 600) -> Scene:
+    """Pyramid of unit boxes plus a world-anchored chain with springs."""
+    rng = _rng(seed)
+    sb = _Builder()
+    sb.body((300, 300, 1), 0.0, 0.5, (0, 0, -0.5))
+    for row in range(base):
+        for col in range(base - row):
+            sb.body((1, 1, 1), 1.0, 0.5, (col * 1.01 + row * 0.505 - base / 2.0, 0.0, row * 1.0 + 0.5))
+    link = 0.2
+    x0, y0, z0 = -100.0, 10.0, 20.0
+    ids = [sb.body((link, link, link), 1.0, 0.5, (x0 + link * 0.5 + k * link, y0, z0)) for k in range(links)]
+    if links:
+        sb.joint(-1, ids[0], (x0, y0, z0), (-link * 0.5, 0, 0))
+    stiff = np.exp(rng.uniform(math.log(1e3), math.log(1e5), max(links - 1, 0)))
+    for k in range(links - 1):
+        sb.joint(ids[k], ids[k + 1], (link * 0.5, 0, 0), (-link * 0.5, 0, 0))
+        if k % 2 == 0:
+            sb.spring(ids[k], ids[k + 1], (0, 0, 0), (0, 0, 0), float(stiff[k]), link)
+    return sb.scene(f"pyramid{base}_chain{links}", -9.81, steps)
+
+
+def pile(count: int = 50000, seed: int = 3, grid=None, pitch: float = 3.6, steps: int = 1000,
+         jitter: float = 0.05) -> Scene:
+    """Random box pile on a jittered grid inside four static walls.
+
+    The default 50,000-box grid is 50 x 40 x 25 (x, y, z); other counts use a
+    grid with the same 5:4 footprint aspect that holds ``count`` boxes.
+    """
+    rng = _rng(seed)
+    if grid is None:
+        if count == 50000:
+            grid = (50, 40, 25)
+        else:
+            nz = max(1, int(round((count / 20.0) ** (1.0 / 3.0))))
+            per_layer = math.ceil(count / nz)
+            ny = max(1, int(math.floor(math.sqrt(per_layer * 4.0 / 5.0))))
+            nx = math.ceil(per_layer / ny)
+            grid = (nx, ny, nz)
+    nx, ny, nz = grid
+    assert nx * ny * nz >= count
+    half_x, half_y = nx * pitch / 2.0, ny * pitch / 2.0
+    sb = _Builder()
+    wall_h = 20.0
+    sb.body((2 * half_x + 20, 2 * half_y + 20, 2), 0.0, 0.5, (0, 0, -1.0))
+    sb.body((2, 2 * half_y + 4, wall_h), 0.0, 0.5, (-half_x - 1.0 - 1.0, 0, wall_h / 2))
+    sb.body((2, 2 * half_y + 4, wall_h), 0.0, 0.5, (half_x + 1.0 + 1.0, 0, wall_h / 2))
+    sb.body((2 * half_x + 4, 2, wall_h), 0.0, 0.5, (0, -half_y - 1.0 - 1.0, wall_h / 2))
+    sb.body((2 * half_x + 4, 2, wall_h), 0.0, 0.5, (0, half_y + 1.0 + 1.0, wall_h / 2))
+    half = rng.uniform(0.25, 1.0, (count, 3))
+    quats = random_quaternions(rng, count)
+    vel = rng.normal(0.0, 0.5, (count, 3))
+    jit = rng.uniform(-jitter, jitter, (count, 3))
+    k = np.arange(count)
+    ix, iy, iz = k % nx, (k // nx) % ny, k // (nx * ny)
+    cx = -half_x + pitch * (ix + 0.5) + jit[:, 0]
+    cy = -half_y + pitch * (iy + 0.5) + jit[:, 1]
+    cz = 2.0 + pitch * (iz + 0.5) + jit[:, 2]
+    for i in range(count):
+        sb.body(tuple(2.0 * half[i]), 1.0, 0.5, (cx[i], cy[i], cz[i]), tuple(quats[i]), tuple(vel[i]))
+    return sb.scene(f"pile{count}", -9.81, steps)
+
+
+def by_name(name: str) -> Scene:
+    if name == "stack":
+        return stack()
+    if name == "pyramid_chain":
+        return pyramid_chain()
+    if name == "pile50k":
+        return pile(50000)
+    raise KeyError(name)
+
+
+def build_state(scene: Scene, maker) -> State:
+    """Scene (creation order) -> State (list order, serials = creation index).
+
+    ``maker`` supplies ``make_box_body(size, density, friction, pos, rot, vel, angvel)``
+    returning a BODY record, and ``joint_spring_derived(rec_a_or_None, rec_b, ra, rb)``
+    returning (torque_arm, rest) -- the constructor-derived fields.
+    """
+    n = scene.num_bodies
+    created = np.zeros(n, BODY)
+    for c in range(n):
+        created[c] = maker.make_box_body(scene.size[c], scene.density[c], scene.friction[c], scene.pos[c],
+                                         scene.rot[c], scene.vel[c], scene.angvel[c])
+        created[c]["dense_id"] = c
+    bodies = created[::-1].copy()
+    to_list = lambda c: -1 if c < 0 else n - 1 - int(c)
+    joints, springs = [], []
+    for k in range(scene.ftype.shape[0]):
+        a, b = int(scene.fa[k]), int(scene.fb[k])
+        rec_a = created[a] if a >= 0 else None
+        arm, rest = maker.joint_spring_derived(rec_a, created[b], scene.ra[k], scene.rb[k])
+        if scene.ftype[k] == FORCE_JOINT:
+            j = np.zeros((), JOINT)
+            j["body_a"], j["body_b"], j["serial"] = to_list(a), to_list(b), k
+            j["r_a"], j["r_b"] = scene.ra[k], scene.rb[k]
+            j["stiffness_lin"], j["stiffness_ang"], j["fracture"] = scene.p0[k], scene.p1[k], scene.p2[k]
+            j["torque_arm"] = arm
+            joints.append(j)
+        else:
+            s = np.zeros((), SPRING)
+            s["body_a"], s["body_b"], s["serial"] = to_list(a), to_list(b), k
+            s["r_a"], s["r_b"] = scene.ra[k], scene.rb[k]
+            s["stiffness"] = scene.p0[k]
+            s["rest"] = scene.p1[k] if scene.p1[k] >= 0 else rest
+            springs.append(s)
+    return State(bodies, np.array(joints, JOINT) if joints else None,
+                 np.array(springs, SPRING) if springs else None, None, scene.ftype.shape[0])

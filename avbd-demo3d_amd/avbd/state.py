@@ -1,0 +1,111 @@
+"""NumPy mirrors of the POD records in include/avbd_hip.h.
+
+The state tables (bodies, joints, springs, manifolds) are the exchange format
+of the C ABI. Each dtype below is laid out exactly like its C struct
+(``align=True`` reproduces the C alignment rules); ``check_layouts`` compares
+the sizes against what a compiled library reports.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+MAX_CONTACTS = 8
+
+PARAMS = np.dtype(
+    [("dt", "<f4"), ("gravity", "<f4"), ("iterations", "<i4"), ("alpha", "<f4"),
+     ("beta_lin", "<f4"), ("beta_ang", "<f4"), ("gamma", "<f4")], align=True)
+
+BODY = np.dtype(
+    [("size", "<f4", 3), ("mass", "<f4"), ("moment", "<f4", 3), ("friction", "<f4"),
+     ("radius", "<f4"), ("pos", "<f4", 3), ("rot", "<f4", 4), ("initial_lin", "<f4", 3),
+     ("initial_ang", "<f4", 4), ("inertial_lin", "<f4", 3), ("inertial_ang", "<f4", 4),
+     ("vel", "<f4", 3), ("ang_vel", "<f4", 3), ("prev_vel", "<f4", 3), ("shape", "<i4"),
+     ("dense_id", "<i4")], align=True)
+
+CONTACT = np.dtype(
+    [("feature", "<i4"), ("r_a", "<f4", 3), ("r_b", "<f4", 3), ("c0", "<f4", 3),
+     ("penalty", "<f4", 3), ("lambda", "<f4", 3), ("stick", "<i4")], align=True)
+
+MANIFOLD = np.dtype(
+    [("body_a", "<i4"), ("body_b", "<i4"), ("serial", "<i8"), ("num_contacts", "<i4"),
+     ("friction", "<f4"), ("basis", "<f4", 9), ("contacts", CONTACT, MAX_CONTACTS)], align=True)
+
+JOINT = np.dtype(
+    [("body_a", "<i4"), ("body_b", "<i4"), ("serial", "<i8"), ("r_a", "<f4", 3), ("r_b", "<f4", 3),
+     ("c0_lin", "<f4", 3), ("c0_ang", "<f4", 3), ("penalty_lin", "<f4", 3), ("penalty_ang", "<f4", 3),
+     ("lambda_lin", "<f4", 3), ("lambda_ang", "<f4", 3), ("stiffness_lin", "<f4"),
+     ("stiffness_ang", "<f4"), ("fracture", "<f4"), ("torque_arm", "<f4"), ("broken", "<i4"),
+     ("pad_", "<i4")], align=True)
+
+SPRING = np.dtype(
+    [("body_a", "<i4"), ("body_b", "<i4"), ("serial", "<i8"), ("r_a", "<f4", 3), ("r_b", "<f4", 3),
+     ("rest", "<f4"), ("stiffness", "<f4")], align=True)
+
+STATS = np.dtype(
+    [("body_count", "<i4"), ("active_body_count", "<i4"), ("solved_body_count", "<i4"),
+     ("pair_checks", "<i4"), ("sphere_hits", "<i4"), ("constrained_hits", "<i4"),
+     ("manifolds_created", "<i4"), ("manifold_count", "<i4"), ("contact_count", "<i4"),
+     ("joint_count", "<i4"), ("spring_count", "<i4"), ("color_count", "<i4"),
+     ("coloring_rounds", "<i4"), ("sweep_axis", "<i4"), ("axis_fallbacks", "<i4"),
+     ("cpu_fallbacks", "<i4"), ("steps_taken", "<i8"), ("broadphase_ms", "<f4"),
+     ("narrowphase_ms", "<f4"), ("coloring_ms", "<f4"), ("iterate_ms", "<f4"),
+     ("finish_ms", "<f4"), ("step_ms", "<f4")], align=True)
+
+CONFIG = np.dtype(
+    [("params", PARAMS), ("max_bodies", "<i4"), ("max_joints", "<i4"), ("max_springs", "<i4"),
+     ("max_manifolds", "<i4"), ("max_pairs", "<i4"), ("device", "<i4"), ("enable_timing", "<i4")],
+    align=True)
+
+EXPECTED_SIZES = {"body": 164, "contact": 68, "manifold": 608, "joint": 136, "spring": 48, "params": 28}
+
+
+def default_params(gravity: float = -10.0, iterations: int = 10, dt: float = 1.0 / 60.0) -> np.ndarray:
+    """Solver::defaultParams (solver.cpp:909-932) with overridable gravity/iterations/dt."""
+    p = np.zeros((), PARAMS)
+    p["dt"] = np.float32(dt)
+    p["gravity"] = np.float32(gravity)
+    p["iterations"] = iterations
+    p["alpha"] = np.float32(0.99)
+    p["beta_lin"] = np.float32(10000.0)
+    p["beta_ang"] = np.float32(100.0)
+    p["gamma"] = np.float32(0.999)
+    return p
+
+
+class State:
+    """A full solver state: bodies in list order, forces in ascending serial."""
+
+    def __init__(self, bodies, joints=None, springs=None, manifolds=None, next_serial=0):
+        self.bodies = np.ascontiguousarray(bodies, dtype=BODY)
+        self.joints = np.ascontiguousarray(joints if joints is not None else np.zeros(0, JOINT), dtype=JOINT)
+        self.springs = np.ascontiguousarray(springs if springs is not None else np.zeros(0, SPRING), dtype=SPRING)
+        self.manifolds = np.ascontiguousarray(
+            manifolds if manifolds is not None else np.zeros(0, MANIFOLD), dtype=MANIFOLD)
+        self.next_serial = int(next_serial)
+
+    def copy(self) -> "State":
+        return State(self.bodies.copy(), self.joints.copy(), self.springs.copy(), self.manifolds.copy(),
+                     self.next_serial)
+
+    @property
+    def num_dynamic(self) -> int:
+        return int((self.bodies["mass"] > 0).sum())
+
+    def save(self, path) -> None:
+        np.savez_compressed(path, bodies=self.bodies, joints=self.joints, springs=self.springs,
+                            manifolds=self.manifolds, next_serial=np.int64(self.next_serial))
+
+    @staticmethod
+    def load(path) -> "State":
+        with np.load(path, allow_pickle=False) as z:
+            return State(z["bodies"], z["joints"], z["springs"], z["manifolds"], int(z["next_serial"]))
+
+
+def check_layouts(sizeof) -> None:
+    """Compare dtype sizes with a library's sizeof(which) (0 body .. 5 params)."""
+    names = ["body", "contact", "manifold", "joint", "spring", "params"]
+    dts = [BODY, CONTACT, MANIFOLD, JOINT, SPRING, PARAMS]
+    for i, (n, dt) in enumerate(zip(names, dts)):
+        got = sizeof(i)
+        if got != dt.itemsize:
+            raise RuntimeError(f"layout mismatch for {n}: library {got} B, numpy {dt.itemsize} B")

@@ -1,0 +1,1065 @@
+// avbd_capi.hip -- C ABI of the resident MI355X AVBD step (declared in include/avbd_hip.h).
+//
+// Owns the device buffers of one solver instance, converts the POD state tables
+// of the ABI to the structure-of-arrays layout the kernels use, and enqueues
+// the per-step launch sequence on the handle's stream. Every count produced
+// inside a step stays on the device, so a step is a fixed, host-sync-free
+// sequence of launches.
+#include "../../include/avbd_hip.h"
+#include "avbd_kernels.hip"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace avbd;
+
+namespace
+{
+
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string &msg)
+{
+    g_lastError = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                      \
+    do                                                                                                     \
+    {                                                                                                      \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return fail(AVBD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+} // namespace
+
+struct avbd_handle
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    avbd_config cfg{};
+    Params P{};
+    int nb = 0, nj = 0, ns = 0;
+    int maxBodies = 0, maxJoints = 0, maxSprings = 0, maxManifolds = 0, maxPairs = 0, maxChunks = 0, maxAdj = 0;
+    int hashSize = 0;
+    int coopIterate = 0, coopColor = 0;
+    int injectFault = 0;
+    Bodies B{};
+    Manifolds cur{}, tmp{};
+    Joint *joints = nullptr;
+    Spring *springs = nullptr;
+    Scratch S{};
+    Counters *ctr = nullptr;
+    GridBarrier *gb = nullptr;
+    int *sweepEnd = nullptr;
+    int *radixHistScan = nullptr;
+    int *adjTotal = nullptr;
+    std::vector<void *> allocs;
+    hipEvent_t ev[8] = {};
+    avbd_stats stats{};
+    long long stepsTaken = 0;
+    bool timing = false;
+
+    template <class T>
+    int alloc(T *&p, size_t count)
+    {
+        void *q = nullptr;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess)
+            return fail(AVBD_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        p = (T *)q;
+        return AVBD_OK;
+    }
+};
+
+namespace
+{
+
+Params to_params(const avbd_params &p)
+{
+    Params o;
+    o.dt = p.dt;
+    o.gravity = p.gravity;
+    o.alpha = p.alpha;
+    o.betaLin = p.beta_lin;
+    o.betaAng = p.beta_ang;
+    o.gamma = p.gamma;
+    o.iterations = p.iterations;
+    return o;
+}
+
+int alloc_manifolds(avbd_handle *h, Manifolds &m, int cap)
+{
+    int r;
+    size_t cs = (size_t)cap * kMaxContacts;
+    if ((r = h->alloc(m.ab, cap)) || (r = h->alloc(m.serial, cap)) || (r = h->alloc(m.count, cap)) ||
+        (r = h->alloc(m.basis0, cap)) || (r = h->alloc(m.basis1, cap)) || (r = h->alloc(m.basis2, cap)) ||
+        (r = h->alloc(m.feature, cs)) || (r = h->alloc(m.rA, cs)) || (r = h->alloc(m.rB, cs)) ||
+        (r = h->alloc(m.c0, cs)) || (r = h->alloc(m.pen, cs)) || (r = h->alloc(m.lam, cs)))
+        return r;
+    return AVBD_OK;
+}
+
+int grid_for(int n) { return std::max(1, std::min(ceil_div(n, 256), 4096)); }
+
+// Exclusive scan with the count either on the device (nptr) or given (nconst).
+void scan(avbd_handle *h, const int *in, int *out, const int *nptr, int nconst, int *total)
+{
+    hipStream_t s = h->stream;
+    k_scan_reduce<<<kScanBlocks, 256, 0, s>>>(in, nptr, nconst, h->S.partial);
+    k_scan_partials<<<1, kScanBlocks, 0, s>>>(h->S.partial, kScanBlocks, total);
+    k_scan_down<<<kScanBlocks, 256, 0, s>>>(in, out, nptr, nconst, h->S.partial);
+}
+
+// Stable LSD radix sort of (key, value); `passes` 8-bit digits from bit 0. Result in keys/vals.
+void radix_sort(avbd_handle *h, unsigned *&keys, int *&vals, unsigned *&keysAlt, int *&valsAlt, int n, int passes)
+{
+    int tiles = ceil_div(n, kRadixTile);
+    for (int p = 0; p < passes; ++p)
+    {
+        int shift = 8 * p;
+        k_radix_hist<<<tiles, 256, 0, h->stream>>>(keys, n, shift, h->S.radixHist, tiles);
+        scan(h, h->S.radixHist, h->radixHistScan, nullptr, 256 * tiles, nullptr);
+        k_radix_scatter<<<tiles, 256, 0, h->stream>>>(keys, vals, n, shift, h->radixHistScan, tiles, keysAlt, valsAlt);
+        std::swap(keys, keysAlt);
+        std::swap(vals, valsAlt);
+    }
+}
+
+__global__ void k_clamp_count(int *count, int cap, int *overflow, int bit)
+{
+    if (*count > cap)
+    {
+        *count = cap;
+        atomicOr(overflow, bit);
+    }
+}
+
+__global__ void k_barrier_reset(GridBarrier *gb) { gb->count = 0u; }
+
+int launch_coop(const void *fn, int blocks, void **args, hipStream_t s)
+{
+    HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(256), args, 0, s));
+    return AVBD_OK;
+}
+
+// Bodies at `dev` -> host records (fields the body table carries).
+void bodies_to_host(const std::vector<float4> *v, const std::vector<float> &fr, const std::vector<int> &ids, int n,
+                    avbd_body *out)
+{
+    // v: pos rot initLin initAng inerLin inerAng vel angVel prevVel massMoment sizeRad
+    for (int i = 0; i < n; ++i)
+    {
+        avbd_body &b = out[i];
+        std::memset(&b, 0, sizeof(b));
+        auto c3 = [&](float *d, const float4 &s) { d[0] = s.x; d[1] = s.y; d[2] = s.z; };
+        auto c4 = [&](float *d, const float4 &s) { d[0] = s.x; d[1] = s.y; d[2] = s.z; d[3] = s.w; };
+        c3(b.pos, v[0][i]);
+        c4(b.rot, v[1][i]);
+        c3(b.initial_lin, v[2][i]);
+        c4(b.initial_ang, v[3][i]);
+        c3(b.inertial_lin, v[4][i]);
+        c4(b.inertial_ang, v[5][i]);
+        c3(b.vel, v[6][i]);
+        c3(b.ang_vel, v[7][i]);
+        c3(b.prev_vel, v[8][i]);
+        b.mass = v[9][i].x;
+        b.moment[0] = v[9][i].y;
+        b.moment[1] = v[9][i].z;
+        b.moment[2] = v[9][i].w;
+        c3(b.size, v[10][i]);
+        b.radius = v[10][i].w;
+        b.friction = fr[i];
+        b.shape = AVBD_SHAPE_BOX;
+        b.dense_id = ids[i];
+    }
+}
+
+int check_device_errors(avbd_handle *h)
+{
+    Counters c;
+    HIP_TRY(hipMemcpyAsync(&c, h->ctr, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (c.barrierTimeout)
+        return fail(AVBD_ERR_HIP, "grid barrier timed out in a cooperative kernel");
+    if (c.overflow)
+    {
+        const char *what = (c.overflow & 1) ? "broadphase pair buffer (max_pairs)"
+                           : (c.overflow & 2) ? "manifold buffer (max_manifolds)"
+                           : (c.overflow & 4) ? "adjacency buffer"
+                           : (c.overflow & 8) ? "colour count (more than 254 colours)"
+                           : (c.overflow & 16) ? "constrained-pair hash"
+                                               : "sweep work-item buffer";
+        return fail((c.overflow & 8) ? AVBD_ERR_UNSUPPORTED : AVBD_ERR_CAPACITY,
+                    std::string("capacity exceeded: ") + what);
+    }
+    return AVBD_OK;
+}
+
+int enqueue_prepare(avbd_handle *h)
+{
+    hipStream_t s = h->stream;
+    const int N = h->nb;
+    Scratch &S = h->S;
+    Counters *ctr = h->ctr;
+    const int G = 1024;
+    if (h->timing)
+        HIP_TRY(hipEventRecord(h->ev[0], s));
+    k_step_begin<<<1, 1, 0, s>>>(ctr);
+    k_barrier_reset<<<1, 1, 0, s>>>(h->gb);
+
+    // ---- broadphase: sweep and prune (solver.cpp:488-577) ----
+    if (N >= 2)
+    {
+        k_axis<<<1, 256, 0, s>>>(h->B.pos, N, ctr);
+        k_axis_serial<<<1, 64, 0, s>>>(h->B.pos, N, ctr);
+        k_sap_keys<<<ceil_div(N, 256), 256, 0, s>>>(h->B.pos, h->B.sizeRad, N, ctr, S.keys, S.vals);
+        unsigned *keys = S.keys, *keysAlt = S.keysAlt;
+        int *vals = S.vals, *valsAlt = S.valsAlt;
+        radix_sort(h, keys, vals, keysAlt, valsAlt, N, 4);
+        k_sap_gather<<<ceil_div(N, 256), 256, 0, s>>>(vals, h->B.pos, h->B.sizeRad, N, ctr, S.sLo, S.sHi, S.sPos, S.sIdx);
+        k_sweep_len<<<ceil_div(N, 256), 256, 0, s>>>(S.sLo, S.sHi, N, h->sweepEnd, S.chunkCount, ctr);
+        scan(h, S.chunkCount, S.chunkStart, nullptr, N, &ctr->numChunks);
+        k_clamp_count<<<1, 1, 0, s>>>(&ctr->numChunks, h->maxChunks, &ctr->overflow, 32);
+        k_sweep_count<<<G, 256, 0, s>>>(S.chunkStart, h->sweepEnd, N, ctr, S.sPos, S.hitCount);
+        scan(h, S.hitCount, S.hitStart, &ctr->numChunks, 0, &ctr->numHits);
+        k_fill_u64<<<grid_for(h->hashSize), 256, 0, s>>>(S.hash, h->hashSize, ~0ull);
+        k_hash_insert<<<G, 256, 0, s>>>(h->cur.ab, ctr, h->joints, h->nj, h->springs, h->ns, S.hash, S.hashMask, ctr);
+        k_sweep_emit<<<G, 256, 0, s>>>(S.chunkStart, h->sweepEnd, N, ctr, S.sPos, S.sIdx, S.hitStart, S.hash, S.hashMask,
+                                       S.hits, S.hitNew, h->maxPairs);
+        k_clamp_count<<<1, 1, 0, s>>>(&ctr->numHits, h->maxPairs, &ctr->overflow, 1);
+        scan(h, S.hitNew, S.hitRank, &ctr->numHits, 0, &ctr->numCreated);
+        k_create<<<G, 256, 0, s>>>(S.hits, S.hitNew, S.hitRank, ctr, h->cur, h->maxManifolds, h->maxPairs);
+    }
+    k_create_finish<<<1, 1, 0, s>>>(ctr, h->maxManifolds, h->maxPairs);
+    if (h->timing)
+        HIP_TRY(hipEventRecord(h->ev[1], s));
+
+    // ---- force initialisation (manifold.cpp:100-174, joint.cpp:35-54) ----
+    k_narrowphase<<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep);
+    scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numKept);
+    k_compact_manifolds<<<G, 256, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, ctr);
+    k_compact_finish<<<1, 1, 0, s>>>(ctr);
+    if (h->nj > 0)
+        k_joint_init<<<ceil_div(h->nj, 256), 256, 0, s>>>(h->joints, h->nj, h->B, h->P);
+    if (h->timing)
+        HIP_TRY(hipEventRecord(h->ev[2], s));
+
+    // ---- per-body force lists, body warm start (solver.cpp:1134-1169) ----
+    k_fill_int<<<grid_for(N), 256, 0, s>>>(S.deg, N, 0);
+    k_fill_int<<<grid_for(N), 256, 0, s>>>(S.adjCursor, N, 0);
+    k_adj<false><<<G, 256, 0, s>>>(h->cur, ctr, h->joints, h->nj, h->springs, h->ns, h->B.massMoment, S.deg, S.adjStart,
+                                   S.adjCursor, S.adj, S.adjSerial, h->maxAdj, &ctr->overflow);
+    scan(h, S.deg, S.adjStart, nullptr, N, h->adjTotal);
+    k_adj<true><<<G, 256, 0, s>>>(h->cur, ctr, h->joints, h->nj, h->springs, h->ns, h->B.massMoment, S.deg, S.adjStart,
+                                  S.adjCursor, S.adj, S.adjSerial, h->maxAdj, &ctr->overflow);
+    k_adj_sort<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjSerial, N, h->maxAdj);
+    k_warmstart<<<ceil_div(N, 256), 256, 0, s>>>(h->B, h->P, S.deg, N, S.color, ctr);
+
+    // ---- colouring (avbd_gpu_solver.cpp:1812-1865) ----
+    {
+        const int *deg = S.deg, *adjStart = S.adjStart, *adj = S.adj;
+        const int2 *mab = h->cur.ab;
+        const Joint *jt = h->joints;
+        const Spring *sp = h->springs;
+        const float4 *mm = h->B.massMoment;
+        int n = N, maxRounds = N + 2;
+        int *color = S.color;
+        GridBarrier *gb = h->gb;
+        void *args[] = {(void *)&deg, (void *)&adjStart, (void *)&adj, (void *)&mab, (void *)&jt, (void *)&sp,
+                        (void *)&mm, (void *)&n, (void *)&color, (void *)&ctr, (void *)&gb, (void *)&maxRounds};
+        int r = launch_coop((const void *)k_color, h->coopColor, args, s);
+        if (r)
+            return r;
+    }
+    {
+        unsigned *keys = S.keys, *keysAlt = S.keysAlt;
+        int *vals = S.vals, *valsAlt = S.valsAlt;
+        k_color_keys<<<ceil_div(N, 256), 256, 0, s>>>(S.color, N, keys, vals, ctr);
+        int tiles = ceil_div(N, kRadixTile);
+        k_radix_hist<<<tiles, 256, 0, s>>>(keys, N, 0, S.radixHist, tiles);
+        scan(h, S.radixHist, h->radixHistScan, nullptr, 256 * tiles, nullptr);
+        k_radix_scatter<<<tiles, 256, 0, s>>>(keys, vals, N, 0, h->radixHistScan, tiles, keysAlt, S.colorBodies);
+        k_color_offsets<<<1, 256, 0, s>>>(h->radixHistScan, tiles, N, S.colorOffsets);
+    }
+    if (h->timing)
+        HIP_TRY(hipEventRecord(h->ev[3], s));
+    return AVBD_OK;
+}
+
+int enqueue_iterate(avbd_handle *h)
+{
+    Bodies B = h->B;
+    Manifolds M = h->cur;
+    Joint *jt = h->joints;
+    int nj = h->nj;
+    const Spring *sp = h->springs;
+    const int *deg = h->S.deg, *adjStart = h->S.adjStart, *adj = h->S.adj;
+    const int *co = h->S.colorOffsets, *cb = h->S.colorBodies;
+    Params P = h->P;
+    Counters *ctr = h->ctr;
+    GridBarrier *gb = h->gb;
+    void *args[] = {(void *)&B, (void *)&M, (void *)&jt, (void *)&nj, (void *)&sp, (void *)&deg, (void *)&adjStart,
+                    (void *)&adj, (void *)&co, (void *)&cb, (void *)&P, (void *)&ctr, (void *)&gb};
+    int r = launch_coop((const void *)k_iterate, h->coopIterate, args, h->stream);
+    if (r)
+        return r;
+    if (h->timing)
+        HIP_TRY(hipEventRecord(h->ev[4], h->stream));
+    return AVBD_OK;
+}
+
+int enqueue_finish(avbd_handle *h)
+{
+    if (h->nb > 0)
+        k_finish<<<ceil_div(h->nb, 256), 256, 0, h->stream>>>(h->B, h->P, h->nb);
+    if (h->timing)
+        HIP_TRY(hipEventRecord(h->ev[5], h->stream));
+    h->stepsTaken++;
+    return AVBD_OK;
+}
+
+int pick_device(avbd_handle *h) { HIP_TRY(hipSetDevice(h->device)); return AVBD_OK; }
+
+} // namespace
+
+extern "C" {
+
+const char *avbd_last_error(void) { return g_lastError.c_str(); }
+
+int avbd_default_params(avbd_params *out)
+{
+    if (!out)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_default_params: null output");
+    out->dt = 1.0f / 60.0f;
+    out->gravity = -10.0f;
+    out->iterations = 10;
+    out->alpha = 0.99f;
+    out->beta_lin = 10000.0f;
+    out->beta_ang = 100.0f;
+    out->gamma = 0.999f;
+    return AVBD_OK;
+}
+
+int avbd_make_box_body(const float size[3], float density, float friction, const float pos[3], const float rot[4],
+                       const float vel[3], const float ang_vel[3], avbd_body *out)
+{
+    if (!size || !pos || !out)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_make_box_body: null argument");
+    static const float qid[4] = {0, 0, 0, 1}, zero[3] = {0, 0, 0};
+    const float *q = rot ? rot : qid;
+    const float *v = vel ? vel : zero;
+    const float *w = ang_vel ? ang_vel : zero;
+    std::memset(out, 0, sizeof(*out));
+    float sx = size[0], sy = size[1], sz = size[2];
+    float mass = sx * sy * sz * density;
+    out->mass = mass;
+    out->moment[0] = (sy * sy + sz * sz) / 12.0f * mass;
+    out->moment[1] = (sx * sx + sz * sz) / 12.0f * mass;
+    out->moment[2] = (sx * sx + sy * sy) / 12.0f * mass;
+    float hx = sx * 0.5f, hy = sy * 0.5f, hz = sz * 0.5f;
+    out->radius = sqrtf(hx * hx + hy * hy + hz * hz);
+    out->friction = friction;
+    for (int k = 0; k < 3; ++k)
+    {
+        out->size[k] = size[k];
+        out->pos[k] = pos[k];
+        out->initial_lin[k] = pos[k];
+        out->inertial_lin[k] = pos[k];
+        out->vel[k] = v[k];
+        out->prev_vel[k] = v[k];
+        out->ang_vel[k] = w[k];
+    }
+    for (int k = 0; k < 4; ++k)
+        out->rot[k] = out->initial_ang[k] = out->inertial_ang[k] = q[k];
+    out->shape = AVBD_SHAPE_BOX;
+    return AVBD_OK;
+}
+
+int avbd_make_joint(const avbd_body *body_a, const avbd_body *body_b, int32_t index_a, int32_t index_b, int64_t serial,
+                    const float r_a[3], const float r_b[3], float stiffness_lin, float stiffness_ang, float fracture,
+                    avbd_joint *out)
+{
+    if (!body_b || !r_a || !r_b || !out)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_make_joint: null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->body_a = body_a ? index_a : -1;
+    out->body_b = index_b;
+    out->serial = serial;
+    for (int k = 0; k < 3; ++k)
+    {
+        out->r_a[k] = r_a[k];
+        out->r_b[k] = r_b[k];
+    }
+    out->stiffness_lin = stiffness_lin;
+    out->stiffness_ang = stiffness_ang;
+    out->fracture = fracture;
+    float s0 = (body_a ? body_a->size[0] : 0.0f) + body_b->size[0];
+    float s1 = (body_a ? body_a->size[1] : 0.0f) + body_b->size[1];
+    float s2 = (body_a ? body_a->size[2] : 0.0f) + body_b->size[2];
+    out->torque_arm = s0 * s0 + s1 * s1 + s2 * s2;
+    return AVBD_OK;
+}
+
+int avbd_make_spring(const avbd_body *body_a, const avbd_body *body_b, int32_t index_a, int32_t index_b, int64_t serial,
+                     const float r_a[3], const float r_b[3], float stiffness, float rest, avbd_spring *out)
+{
+    if (!body_a || !body_b || !r_a || !r_b || !out)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_make_spring: null argument");
+    std::memset(out, 0, sizeof(*out));
+    out->body_a = index_a;
+    out->body_b = index_b;
+    out->serial = serial;
+    for (int k = 0; k < 3; ++k)
+    {
+        out->r_a[k] = r_a[k];
+        out->r_b[k] = r_b[k];
+    }
+    out->stiffness = stiffness;
+    if (rest < 0.0f)
+    {
+        V3 pa = rotate(Q4{body_a->rot[0], body_a->rot[1], body_a->rot[2], body_a->rot[3]}, V3{r_a[0], r_a[1], r_a[2]}) +
+                V3{body_a->pos[0], body_a->pos[1], body_a->pos[2]};
+        V3 pb = rotate(Q4{body_b->rot[0], body_b->rot[1], body_b->rot[2], body_b->rot[3]}, V3{r_b[0], r_b[1], r_b[2]}) +
+                V3{body_b->pos[0], body_b->pos[1], body_b->pos[2]};
+        rest = len(pa - pb);
+    }
+    out->rest = rest;
+    return AVBD_OK;
+}
+
+int avbd_create(const avbd_config *config, avbd_handle **out)
+{
+    if (!config || !out)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_create: null argument");
+    if (config->max_bodies <= 0)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_create: max_bodies must be positive");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(AVBD_ERR_NO_DEVICE, "avbd_create: no HIP device");
+    if (config->device < 0 || config->device >= ndev)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_create: device ordinal out of range");
+    avbd_handle *h = new avbd_handle();
+    h->cfg = *config;
+    h->device = config->device;
+    h->P = to_params(config->params);
+    h->timing = config->enable_timing != 0;
+    int N = config->max_bodies;
+    h->maxBodies = N;
+    h->maxJoints = std::max(config->max_joints, 0);
+    h->maxSprings = std::max(config->max_springs, 0);
+    h->maxManifolds = config->max_manifolds > 0 ? config->max_manifolds : 12 * N + 1024;
+    h->maxPairs = config->max_pairs > 0 ? config->max_pairs : 16 * N + 4096;
+    h->maxChunks = 4 * N + h->maxPairs / 4 + 4096;
+    h->maxAdj = 2 * (h->maxManifolds + h->maxJoints + h->maxSprings) + 16;
+    int hs = 1024;
+    while (hs < 2 * (h->maxManifolds + h->maxJoints + h->maxSprings))
+        hs <<= 1;
+    h->hashSize = hs;
+    int r = AVBD_OK;
+    auto bail = [&](int code) {
+        avbd_destroy(h);
+        return code;
+    };
+    if ((r = pick_device(h)))
+        return bail(r);
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess)
+        return bail(fail(AVBD_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+    Bodies &B = h->B;
+    if ((r = h->alloc(B.pos, N)) || (r = h->alloc(B.rot, N)) || (r = h->alloc(B.initLin, N)) ||
+        (r = h->alloc(B.initAng, N)) || (r = h->alloc(B.inerLin, N)) || (r = h->alloc(B.inerAng, N)) ||
+        (r = h->alloc(B.vel, N)) || (r = h->alloc(B.angVel, N)) || (r = h->alloc(B.prevVel, N)) ||
+        (r = h->alloc(B.massMoment, N)) || (r = h->alloc(B.sizeRad, N)) || (r = h->alloc(B.friction, N)) ||
+        (r = h->alloc(B.denseId, N)))
+        return bail(r);
+    if ((r = alloc_manifolds(h, h->cur, h->maxManifolds)) || (r = alloc_manifolds(h, h->tmp, h->maxManifolds)))
+        return bail(r);
+    if ((r = h->alloc(h->joints, h->maxJoints)) || (r = h->alloc(h->springs, h->maxSprings)))
+        return bail(r);
+    Scratch &S = h->S;
+    int tiles = ceil_div(N, kRadixTile);
+    if ((r = h->alloc(S.axisSums, 8)) || (r = h->alloc(S.keys, N)) || (r = h->alloc(S.keysAlt, N)) ||
+        (r = h->alloc(S.vals, N)) || (r = h->alloc(S.valsAlt, N)) || (r = h->alloc(S.radixHist, 256 * tiles)) ||
+        (r = h->alloc(h->radixHistScan, 256 * tiles)) || (r = h->alloc(S.sLo, N)) || (r = h->alloc(S.sHi, N)) ||
+        (r = h->alloc(S.sPos, N)) || (r = h->alloc(S.sIdx, N)) || (r = h->alloc(S.chunkCount, N)) ||
+        (r = h->alloc(S.chunkStart, N)) || (r = h->alloc(h->sweepEnd, N)) || (r = h->alloc(S.hitCount, h->maxChunks)) ||
+        (r = h->alloc(S.hitStart, h->maxChunks)) || (r = h->alloc(S.hits, h->maxPairs)) ||
+        (r = h->alloc(S.hitNew, h->maxPairs)) || (r = h->alloc(S.hitRank, h->maxPairs)) ||
+        (r = h->alloc(S.hash, h->hashSize)) || (r = h->alloc(S.keep, h->maxManifolds)) ||
+        (r = h->alloc(S.keepRank, h->maxManifolds)) || (r = h->alloc(S.deg, N)) || (r = h->alloc(S.adjStart, N)) ||
+        (r = h->alloc(S.adjCursor, N)) || (r = h->alloc(S.adj, h->maxAdj)) || (r = h->alloc(S.adjSerial, h->maxAdj)) ||
+        (r = h->alloc(S.color, N)) || (r = h->alloc(S.colorOffsets, 257)) || (r = h->alloc(S.colorBodies, N)) ||
+        (r = h->alloc(S.partial, kScanBlocks + 1)) || (r = h->alloc(h->ctr, 1)) || (r = h->alloc(h->gb, 1)) ||
+        (r = h->alloc(h->adjTotal, 1)))
+        return bail(r);
+    S.hashMask = h->hashSize - 1;
+    if ((e = hipMemsetAsync(h->ctr, 0, sizeof(Counters), h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(h->gb, 0, sizeof(GridBarrier), h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(S.color, 0xFF, sizeof(int) * N, h->stream)) != hipSuccess)
+        return bail(fail(AVBD_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e)));
+    // cooperative grid sizes: co-resident blocks of 256 threads
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, h->device)) != hipSuccess)
+        return bail(fail(AVBD_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e)));
+    int perCuIt = 0, perCuCol = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCuIt, (const void *)k_iterate, 256, 0)) != hipSuccess ||
+        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCuCol, (const void *)k_color, 256, 0)) != hipSuccess)
+        return bail(fail(AVBD_ERR_HIP, std::string("hipOccupancy: ") + hipGetErrorString(e)));
+    if (perCuIt < 1 || perCuCol < 1)
+        return bail(fail(AVBD_ERR_HIP, "cooperative kernels do not fit on a compute unit"));
+    h->coopIterate = prop.multiProcessorCount * std::min(perCuIt, 1);
+    h->coopColor = prop.multiProcessorCount * std::min(perCuCol, 1);
+    for (int k = 0; k < 8; ++k)
+        if ((e = hipEventCreate(&h->ev[k])) != hipSuccess)
+            return bail(fail(AVBD_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e)));
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess)
+        return bail(fail(AVBD_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e)));
+    *out = h;
+    return AVBD_OK;
+}
+
+int avbd_destroy(avbd_handle *h)
+{
+    if (!h)
+        return AVBD_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs)
+        (void)hipFree(p);
+    for (int k = 0; k < 8; ++k)
+        if (h->ev[k])
+            (void)hipEventDestroy(h->ev[k]);
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+    return AVBD_OK;
+}
+
+int avbd_set_params(avbd_handle *h, const avbd_params *params)
+{
+    if (!h || !params)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_params: null argument");
+    if (params->iterations < 0 || !(params->dt > 0.0f))
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_params: iterations must be >= 0 and dt > 0");
+    h->cfg.params = *params;
+    h->P = to_params(*params);
+    return AVBD_OK;
+}
+
+int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodies, const avbd_joint *joints,
+                      int32_t num_joints, const avbd_spring *springs, int32_t num_springs,
+                      const avbd_manifold *manifolds, int32_t num_manifolds, int64_t next_serial)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_upload_state: null handle");
+    if (num_bodies < 0 || num_joints < 0 || num_springs < 0 || num_manifolds < 0 ||
+        (num_bodies && !bodies) || (num_joints && !joints) || (num_springs && !springs) ||
+        (num_manifolds && !manifolds))
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_upload_state: bad table pointer or count");
+    if (num_bodies > h->maxBodies || num_joints > h->maxJoints || num_springs > h->maxSprings ||
+        num_manifolds > h->maxManifolds)
+        return fail(AVBD_ERR_CAPACITY, "avbd_upload_state: state exceeds the handle's capacities");
+    for (int i = 0; i < num_bodies; ++i)
+        if (bodies[i].shape != AVBD_SHAPE_BOX)
+            return fail(AVBD_ERR_UNSUPPORTED, "avbd_upload_state: only box bodies are supported");
+    auto badBody = [&](int b, bool allowWorld) { return (b < 0 && !(allowWorld && b == -1)) || b >= num_bodies; };
+    for (int i = 0; i < num_joints; ++i)
+        if (badBody(joints[i].body_a, true) || badBody(joints[i].body_b, false))
+            return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_upload_state: joint body index out of range");
+    for (int i = 0; i < num_springs; ++i)
+        if (badBody(springs[i].body_a, false) || badBody(springs[i].body_b, false))
+            return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_upload_state: spring body index out of range");
+    for (int i = 0; i < num_manifolds; ++i)
+        if (badBody(manifolds[i].body_a, false) || badBody(manifolds[i].body_b, false) ||
+            manifolds[i].num_contacts < 0 || manifolds[i].num_contacts > AVBD_MAX_CONTACTS)
+            return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_upload_state: manifold out of range");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    int N = num_bodies;
+    std::vector<float4> pos(N), rot(N), il(N), ia(N), nl(N), na(N), vel(N), av(N), pv(N), mm(N), sr(N);
+    std::vector<float> fr(N);
+    std::vector<int> ids(N);
+    for (int i = 0; i < N; ++i)
+    {
+        const avbd_body &b = bodies[i];
+        pos[i] = make_float4(b.pos[0], b.pos[1], b.pos[2], 0.0f);
+        rot[i] = make_float4(b.rot[0], b.rot[1], b.rot[2], b.rot[3]);
+        il[i] = make_float4(b.initial_lin[0], b.initial_lin[1], b.initial_lin[2], 0.0f);
+        ia[i] = make_float4(b.initial_ang[0], b.initial_ang[1], b.initial_ang[2], b.initial_ang[3]);
+        nl[i] = make_float4(b.inertial_lin[0], b.inertial_lin[1], b.inertial_lin[2], 0.0f);
+        na[i] = make_float4(b.inertial_ang[0], b.inertial_ang[1], b.inertial_ang[2], b.inertial_ang[3]);
+        vel[i] = make_float4(b.vel[0], b.vel[1], b.vel[2], 0.0f);
+        av[i] = make_float4(b.ang_vel[0], b.ang_vel[1], b.ang_vel[2], 0.0f);
+        pv[i] = make_float4(b.prev_vel[0], b.prev_vel[1], b.prev_vel[2], 0.0f);
+        mm[i] = make_float4(b.mass, b.moment[0], b.moment[1], b.moment[2]);
+        sr[i] = make_float4(b.size[0], b.size[1], b.size[2], b.radius);
+        fr[i] = b.friction;
+        ids[i] = b.dense_id;
+    }
+    hipStream_t s = h->stream;
+    auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
+        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+    };
+    size_t f4 = sizeof(float4) * N;
+    hipError_t e;
+    if ((e = up(h->B.pos, pos.data(), f4)) || (e = up(h->B.rot, rot.data(), f4)) || (e = up(h->B.initLin, il.data(), f4)) ||
+        (e = up(h->B.initAng, ia.data(), f4)) || (e = up(h->B.inerLin, nl.data(), f4)) ||
+        (e = up(h->B.inerAng, na.data(), f4)) || (e = up(h->B.vel, vel.data(), f4)) || (e = up(h->B.angVel, av.data(), f4)) ||
+        (e = up(h->B.prevVel, pv.data(), f4)) || (e = up(h->B.massMoment, mm.data(), f4)) ||
+        (e = up(h->B.sizeRad, sr.data(), f4)) || (e = up(h->B.friction, fr.data(), sizeof(float) * N)) ||
+        (e = up(h->B.denseId, ids.data(), sizeof(int) * N)))
+        return fail(AVBD_ERR_HIP, std::string("upload bodies: ") + hipGetErrorString(e));
+    // joints / springs
+    std::vector<Joint> jt(num_joints);
+    for (int i = 0; i < num_joints; ++i)
+    {
+        const avbd_joint &j = joints[i];
+        Joint &o = jt[i];
+        std::memset(&o, 0, sizeof(o));
+        o.a = j.body_a;
+        o.b = j.body_b;
+        o.broken = j.broken;
+        o.dead = 0;
+        o.serial = j.serial;
+        o.rA = make_float4(j.r_a[0], j.r_a[1], j.r_a[2], j.stiffness_lin);
+        o.rB = make_float4(j.r_b[0], j.r_b[1], j.r_b[2], j.stiffness_ang);
+        o.c0Lin = make_float4(j.c0_lin[0], j.c0_lin[1], j.c0_lin[2], j.torque_arm);
+        o.c0Ang = make_float4(j.c0_ang[0], j.c0_ang[1], j.c0_ang[2], j.fracture);
+        o.penLin = make_float4(j.penalty_lin[0], j.penalty_lin[1], j.penalty_lin[2], 0.0f);
+        o.penAng = make_float4(j.penalty_ang[0], j.penalty_ang[1], j.penalty_ang[2], 0.0f);
+        o.lamLin = make_float4(j.lambda_lin[0], j.lambda_lin[1], j.lambda_lin[2], 0.0f);
+        o.lamAng = make_float4(j.lambda_ang[0], j.lambda_ang[1], j.lambda_ang[2], 0.0f);
+    }
+    std::vector<Spring> st(num_springs);
+    for (int i = 0; i < num_springs; ++i)
+    {
+        const avbd_spring &sp = springs[i];
+        Spring &o = st[i];
+        std::memset(&o, 0, sizeof(o));
+        o.a = sp.body_a;
+        o.b = sp.body_b;
+        o.serial = sp.serial;
+        o.rA = make_float4(sp.r_a[0], sp.r_a[1], sp.r_a[2], sp.rest);
+        o.rB = make_float4(sp.r_b[0], sp.r_b[1], sp.r_b[2], sp.stiffness);
+    }
+    if ((e = up(h->joints, jt.data(), sizeof(Joint) * num_joints)) ||
+        (e = up(h->springs, st.data(), sizeof(Spring) * num_springs)))
+        return fail(AVBD_ERR_HIP, std::string("upload forces: ") + hipGetErrorString(e));
+    // manifolds in ascending serial
+    std::vector<int> order(num_manifolds);
+    for (int i = 0; i < num_manifolds; ++i)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return manifolds[x].serial < manifolds[y].serial; });
+    int M = num_manifolds;
+    std::vector<int2> ab(M);
+    std::vector<long long> serial(M);
+    std::vector<int> cnt(M);
+    std::vector<float4> b0(M), b1(M), b2(M);
+    size_t CS = (size_t)M * kMaxContacts;
+    std::vector<int> feat(CS, 0);
+    std::vector<float4> rA(CS), rB(CS), c0(CS), pen(CS), lam(CS);
+    for (int k = 0; k < M; ++k)
+    {
+        const avbd_manifold &m = manifolds[order[k]];
+        ab[k] = make_int2(m.body_a, m.body_b);
+        serial[k] = m.serial;
+        cnt[k] = m.num_contacts;
+        b0[k] = make_float4(m.basis[0], m.basis[1], m.basis[2], m.friction);
+        b1[k] = make_float4(m.basis[3], m.basis[4], m.basis[5], 0.0f);
+        b2[k] = make_float4(m.basis[6], m.basis[7], m.basis[8], 0.0f);
+        for (int c = 0; c < kMaxContacts; ++c)
+        {
+            const avbd_contact &ct = m.contacts[c];
+            size_t sl = (size_t)k * kMaxContacts + c;
+            feat[sl] = ct.feature;
+            rA[sl] = make_float4(ct.r_a[0], ct.r_a[1], ct.r_a[2], 0.0f);
+            rB[sl] = make_float4(ct.r_b[0], ct.r_b[1], ct.r_b[2], 0.0f);
+            c0[sl] = make_float4(ct.c0[0], ct.c0[1], ct.c0[2], 0.0f);
+            pen[sl] = make_float4(ct.penalty[0], ct.penalty[1], ct.penalty[2], 0.0f);
+            lam[sl] = make_float4(ct.lambda[0], ct.lambda[1], ct.lambda[2], ct.stick ? 1.0f : 0.0f);
+        }
+    }
+    Manifolds &C = h->cur;
+    size_t cf4 = sizeof(float4) * CS;
+    if ((e = up(C.ab, ab.data(), sizeof(int2) * M)) || (e = up(C.serial, serial.data(), sizeof(long long) * M)) ||
+        (e = up(C.count, cnt.data(), sizeof(int) * M)) || (e = up(C.basis0, b0.data(), sizeof(float4) * M)) ||
+        (e = up(C.basis1, b1.data(), sizeof(float4) * M)) || (e = up(C.basis2, b2.data(), sizeof(float4) * M)) ||
+        (e = up(C.feature, feat.data(), sizeof(int) * CS)) || (e = up(C.rA, rA.data(), cf4)) ||
+        (e = up(C.rB, rB.data(), cf4)) || (e = up(C.c0, c0.data(), cf4)) || (e = up(C.pen, pen.data(), cf4)) ||
+        (e = up(C.lam, lam.data(), cf4)))
+        return fail(AVBD_ERR_HIP, std::string("upload manifolds: ") + hipGetErrorString(e));
+    Counters c;
+    std::memset(&c, 0, sizeof(c));
+    c.numManifolds = M;
+    c.nextSerial = next_serial;
+    if ((e = up(h->ctr, &c, sizeof(c))))
+        return fail(AVBD_ERR_HIP, std::string("upload counters: ") + hipGetErrorString(e));
+    std::vector<int> negs(std::max(N, 1), -1);
+    if ((e = up(h->S.color, negs.data(), sizeof(int) * N)))
+        return fail(AVBD_ERR_HIP, std::string("upload colours: ") + hipGetErrorString(e));
+    if ((e = hipStreamSynchronize(s)) != hipSuccess)
+        return fail(AVBD_ERR_HIP, std::string("upload sync: ") + hipGetErrorString(e));
+    h->nb = N;
+    h->nj = num_joints;
+    h->ns = num_springs;
+    return AVBD_OK;
+}
+
+int avbd_state_counts(avbd_handle *h, int32_t *num_bodies, int32_t *num_joints, int32_t *num_springs,
+                      int32_t *num_manifolds, int64_t *next_serial)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_state_counts: null handle");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    Counters c;
+    HIP_TRY(hipMemcpyAsync(&c, h->ctr, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<Joint> jt(h->nj);
+    if (h->nj)
+        HIP_TRY(hipMemcpy(jt.data(), h->joints, sizeof(Joint) * h->nj, hipMemcpyDeviceToHost));
+    int live = 0;
+    for (const Joint &j : jt)
+        live += j.dead ? 0 : 1;
+    if (num_bodies)
+        *num_bodies = h->nb;
+    if (num_joints)
+        *num_joints = live;
+    if (num_springs)
+        *num_springs = h->ns;
+    if (num_manifolds)
+        *num_manifolds = c.numManifolds;
+    if (next_serial)
+        *next_serial = c.nextSerial;
+    return AVBD_OK;
+}
+
+int avbd_download_state(avbd_handle *h, avbd_body *bodies, avbd_joint *joints, avbd_spring *springs,
+                        avbd_manifold *manifolds)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_download_state: null handle");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if ((r = check_device_errors(h)))
+        return r;
+    int N = h->nb;
+    if (bodies && N)
+    {
+        std::vector<float4> v[11];
+        float4 *src[11] = {h->B.pos, h->B.rot, h->B.initLin, h->B.initAng, h->B.inerLin, h->B.inerAng,
+                           h->B.vel, h->B.angVel, h->B.prevVel, h->B.massMoment, h->B.sizeRad};
+        for (int k = 0; k < 11; ++k)
+        {
+            v[k].resize(N);
+            HIP_TRY(hipMemcpy(v[k].data(), src[k], sizeof(float4) * N, hipMemcpyDeviceToHost));
+        }
+        std::vector<float> fr(N);
+        std::vector<int> ids(N);
+        HIP_TRY(hipMemcpy(fr.data(), h->B.friction, sizeof(float) * N, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(ids.data(), h->B.denseId, sizeof(int) * N, hipMemcpyDeviceToHost));
+        bodies_to_host(v, fr, ids, N, bodies);
+    }
+    if (joints && h->nj)
+    {
+        std::vector<Joint> jt(h->nj);
+        HIP_TRY(hipMemcpy(jt.data(), h->joints, sizeof(Joint) * h->nj, hipMemcpyDeviceToHost));
+        int k = 0;
+        for (const Joint &j : jt)
+        {
+            if (j.dead)
+                continue;
+            avbd_joint &o = joints[k++];
+            std::memset(&o, 0, sizeof(o));
+            o.body_a = j.a;
+            o.body_b = j.b;
+            o.serial = j.serial;
+            o.broken = j.broken;
+            const float4 *f[8] = {&j.rA, &j.rB, &j.c0Lin, &j.c0Ang, &j.penLin, &j.penAng, &j.lamLin, &j.lamAng};
+            float *d[8] = {o.r_a, o.r_b, o.c0_lin, o.c0_ang, o.penalty_lin, o.penalty_ang, o.lambda_lin, o.lambda_ang};
+            for (int q = 0; q < 8; ++q)
+            {
+                d[q][0] = f[q]->x;
+                d[q][1] = f[q]->y;
+                d[q][2] = f[q]->z;
+            }
+            o.stiffness_lin = j.rA.w;
+            o.stiffness_ang = j.rB.w;
+            o.torque_arm = j.c0Lin.w;
+            o.fracture = j.c0Ang.w;
+        }
+    }
+    if (springs && h->ns)
+    {
+        std::vector<Spring> st(h->ns);
+        HIP_TRY(hipMemcpy(st.data(), h->springs, sizeof(Spring) * h->ns, hipMemcpyDeviceToHost));
+        for (int i = 0; i < h->ns; ++i)
+        {
+            avbd_spring &o = springs[i];
+            std::memset(&o, 0, sizeof(o));
+            o.body_a = st[i].a;
+            o.body_b = st[i].b;
+            o.serial = st[i].serial;
+            o.r_a[0] = st[i].rA.x;
+            o.r_a[1] = st[i].rA.y;
+            o.r_a[2] = st[i].rA.z;
+            o.r_b[0] = st[i].rB.x;
+            o.r_b[1] = st[i].rB.y;
+            o.r_b[2] = st[i].rB.z;
+            o.rest = st[i].rA.w;
+            o.stiffness = st[i].rB.w;
+        }
+    }
+    if (manifolds)
+    {
+        Counters c;
+        HIP_TRY(hipMemcpy(&c, h->ctr, sizeof(c), hipMemcpyDeviceToHost));
+        int M = c.numManifolds;
+        if (M)
+        {
+            size_t CS = (size_t)M * kMaxContacts;
+            std::vector<int2> ab(M);
+            std::vector<long long> serial(M);
+            std::vector<int> cnt(M), feat(CS);
+            std::vector<float4> b0(M), b1(M), b2(M), rA(CS), rB(CS), c0(CS), pen(CS), lam(CS);
+            const Manifolds &C = h->cur;
+            HIP_TRY(hipMemcpy(ab.data(), C.ab, sizeof(int2) * M, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(serial.data(), C.serial, sizeof(long long) * M, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(cnt.data(), C.count, sizeof(int) * M, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(b0.data(), C.basis0, sizeof(float4) * M, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(b1.data(), C.basis1, sizeof(float4) * M, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(b2.data(), C.basis2, sizeof(float4) * M, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(feat.data(), C.feature, sizeof(int) * CS, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(rA.data(), C.rA, sizeof(float4) * CS, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(rB.data(), C.rB, sizeof(float4) * CS, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(c0.data(), C.c0, sizeof(float4) * CS, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(pen.data(), C.pen, sizeof(float4) * CS, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(lam.data(), C.lam, sizeof(float4) * CS, hipMemcpyDeviceToHost));
+            for (int k = 0; k < M; ++k)
+            {
+                avbd_manifold &o = manifolds[k];
+                std::memset(&o, 0, sizeof(o));
+                o.body_a = ab[k].x;
+                o.body_b = ab[k].y;
+                o.serial = serial[k];
+                o.num_contacts = cnt[k];
+                o.friction = b0[k].w;
+                float4 bs[3] = {b0[k], b1[k], b2[k]};
+                for (int q = 0; q < 3; ++q)
+                {
+                    o.basis[3 * q] = bs[q].x;
+                    o.basis[3 * q + 1] = bs[q].y;
+                    o.basis[3 * q + 2] = bs[q].z;
+                }
+                for (int q = 0; q < cnt[k]; ++q)
+                {
+                    size_t sl = (size_t)k * kMaxContacts + q;
+                    avbd_contact &ct = o.contacts[q];
+                    ct.feature = feat[sl];
+                    const float4 *f[5] = {&rA[sl], &rB[sl], &c0[sl], &pen[sl], &lam[sl]};
+                    float *d[5] = {ct.r_a, ct.r_b, ct.c0, ct.penalty, ct.lambda};
+                    for (int z = 0; z < 5; ++z)
+                    {
+                        d[z][0] = f[z]->x;
+                        d[z][1] = f[z]->y;
+                        d[z][2] = f[z]->z;
+                    }
+                    ct.stick = lam[sl].w != 0.0f ? 1 : 0;
+                }
+            }
+        }
+    }
+    return AVBD_OK;
+}
+
+int avbd_prepare(avbd_handle *h)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_prepare: null handle");
+    if (h->injectFault)
+        return fail(AVBD_ERR_INJECTED, "avbd_prepare: injected fault");
+    int r = pick_device(h);
+    return r ? r : enqueue_prepare(h);
+}
+
+int avbd_iterate(avbd_handle *h)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_iterate: null handle");
+    int r = pick_device(h);
+    return r ? r : enqueue_iterate(h);
+}
+
+int avbd_finish(avbd_handle *h)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_finish: null handle");
+    int r = pick_device(h);
+    return r ? r : enqueue_finish(h);
+}
+
+int avbd_step(avbd_handle *h, int32_t num_steps)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_step: null handle");
+    if (num_steps < 0)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_step: negative step count");
+    if (h->injectFault)
+        return fail(AVBD_ERR_INJECTED, "avbd_step: injected fault");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    for (int k = 0; k < num_steps; ++k)
+    {
+        if ((r = enqueue_prepare(h)) || (r = enqueue_iterate(h)) || (r = enqueue_finish(h)))
+            return r;
+    }
+    return AVBD_OK;
+}
+
+int avbd_synchronize(avbd_handle *h)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_synchronize: null handle");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return check_device_errors(h);
+}
+
+int avbd_get_stats(avbd_handle *h, avbd_stats *out)
+{
+    if (!h || !out)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_get_stats: null argument");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    Counters c;
+    HIP_TRY(hipMemcpyAsync(&c, h->ctr, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<float4> mm(h->nb);
+    if (h->nb)
+        HIP_TRY(hipMemcpy(mm.data(), h->B.massMoment, sizeof(float4) * h->nb, hipMemcpyDeviceToHost));
+    int active = 0;
+    for (const float4 &m : mm)
+        active += m.x > 0.0f ? 1 : 0;
+    std::memset(out, 0, sizeof(*out));
+    out->body_count = h->nb;
+    out->active_body_count = active;
+    out->solved_body_count = c.solvedCount;
+    out->pair_checks = c.pairChecks;
+    out->sphere_hits = c.numHits;
+    out->constrained_hits = c.numHits - c.numCreated;
+    out->manifolds_created = c.numCreated;
+    out->manifold_count = c.numManifolds;
+    out->contact_count = c.contactCount;
+    out->joint_count = h->nj;
+    out->spring_count = h->ns;
+    out->color_count = c.colorCount;
+    out->coloring_rounds = c.coloringRounds;
+    out->sweep_axis = c.sweepAxis;
+    out->axis_fallbacks = c.axisFallbacks;
+    out->steps_taken = h->stepsTaken;
+    if (h->timing && h->stepsTaken > 0)
+    {
+        float t[5] = {0, 0, 0, 0, 0};
+        for (int k = 0; k < 5; ++k)
+            if (hipEventElapsedTime(&t[k], h->ev[k], h->ev[k + 1]) != hipSuccess)
+                t[k] = 0.0f;
+        out->broadphase_ms = t[0];
+        out->narrowphase_ms = t[1];
+        out->coloring_ms = t[2];
+        out->iterate_ms = t[3];
+        out->finish_ms = t[4];
+        float total = 0.0f;
+        if (hipEventElapsedTime(&total, h->ev[0], h->ev[5]) == hipSuccess)
+            out->step_ms = total;
+    }
+    if (c.barrierTimeout || c.overflow)
+        return check_device_errors(h);
+    return AVBD_OK;
+}
+
+int avbd_get_colors(avbd_handle *h, int32_t *colors, int32_t capacity)
+{
+    if (!h || (!colors && capacity > 0))
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_get_colors: null argument");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int n = std::min(capacity, h->nb);
+    if (n > 0)
+        HIP_TRY(hipMemcpy(colors, h->S.color, sizeof(int) * n, hipMemcpyDeviceToHost));
+    return AVBD_OK;
+}
+
+int avbd_get_hits(avbd_handle *h, int32_t *triples, int32_t capacity, int32_t *count)
+{
+    if (!h || !count)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_get_hits: null argument");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    Counters c;
+    HIP_TRY(hipMemcpyAsync(&c, h->ctr, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int H = h->nb >= 2 ? std::min(c.numHits, h->maxPairs) : 0;
+    *count = H;
+    int n = std::min(H, capacity);
+    if (n > 0 && triples)
+    {
+        std::vector<int2> hits(n);
+        std::vector<int> fresh(n);
+        HIP_TRY(hipMemcpy(hits.data(), h->S.hits, sizeof(int2) * n, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(fresh.data(), h->S.hitNew, sizeof(int) * n, hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; ++k)
+        {
+            triples[3 * k] = hits[k].x;
+            triples[3 * k + 1] = hits[k].y;
+            triples[3 * k + 2] = fresh[k];
+        }
+    }
+    return AVBD_OK;
+}
+
+int avbd_debug_inject_fault(avbd_handle *h, int32_t code)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_debug_inject_fault: null handle");
+    h->injectFault = code;
+    return AVBD_OK;
+}
+
+int avbd_sizeof(int which)
+{
+    switch (which)
+    {
+    case 0: return (int)sizeof(avbd_body);
+    case 1: return (int)sizeof(avbd_contact);
+    case 2: return (int)sizeof(avbd_manifold);
+    case 3: return (int)sizeof(avbd_joint);
+    case 4: return (int)sizeof(avbd_spring);
+    case 5: return (int)sizeof(avbd_params);
+    default: return -1;
+    }
+}
+
+} // extern "C"

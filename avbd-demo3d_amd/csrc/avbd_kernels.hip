@@ -1,0 +1,1790 @@
+// avbd_kernels.hip -- gfx950 kernels of the resident AVBD step.
+//
+// One step (Solver::stepCpuReference, solver.cpp:1036-1041) is a fixed
+// sequence of launches with every count kept on the device:
+//   broadphase  axis choice -> 4-pass LSD radix sort of interval starts ->
+//               chunked sweep (count, scan, emit) -> constrained-pair hash ->
+//               ordered creation of new manifolds            (solver.cpp:488-577, 131-248)
+//   init        box-box narrowphase + contact merge + warm start, joint warm
+//               start, stable compaction                      (manifold.cpp:100-174, joint.cpp:35-54)
+//   bodies      per-body force lists in descending creation serial, body warm
+//               start                                         (solver.cpp:1134-1169)
+//   colouring   index-priority Jones-Plassmann fixpoint == sequential greedy (avbd_gpu_solver.cpp:1812-1865)
+//   iterate     persistent cooperative kernel: per colour primal, then dual (solver.cpp:1172-1273)
+//   finish      BDF1 velocities                               (solver.cpp:1275-1298)
+//
+// Arithmetic follows the reference operation by operation (see avbd_math.hpp);
+// the build uses -ffp-contract=off.
+#include "avbd_device.hpp"
+#include "avbd_math.hpp"
+
+#include <float.h>
+
+namespace avbd
+{
+
+constexpr float kPenaltyMin = 1.0f;
+constexpr float kPenaltyMax = 10000000000.0f;
+constexpr float kMargin = 0.01f;
+constexpr float kStickThresh = 0.00001f;
+constexpr float kSatEps = 1.0e-6f;
+constexpr float kPlaneEps = 1.0e-5f;
+constexpr float kMergeDistSq = 1.0e-6f;
+
+// ------------------------------------------------------------------------------------------
+// wave / block primitives (wave64, blockDim multiple of 64, <= 1024)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int wave_inclusive_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1)
+    {
+        int t = __shfl_up(v, off, 64);
+        if (lane >= off)
+            v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Exclusive scan over the block; lds needs blockDim/64 ints.
+__device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
+{
+    int incl = wave_inclusive_scan(v);
+    int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63)
+        lds[wave] = incl;
+    __syncthreads();
+    int off = 0, tot = 0;
+    int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; ++w)
+    {
+        int s = lds[w];
+        off += w < wave ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + incl - v;
+}
+
+// Grid-wide barrier for cooperative launches. A spin limit turns a would-be
+// hang into an error flag; once set, every later barrier returns at once.
+__device__ __forceinline__ void grid_sync(GridBarrier *gb, unsigned nblocks, int *timeout)
+{
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        __threadfence();
+        unsigned g = __hip_atomic_load(&gb->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned arrived = atomicAdd(&gb->count, 1u);
+        if (arrived == nblocks - 1)
+        {
+            atomicExch(&gb->count, 0u);
+            __threadfence();
+            atomicAdd(&gb->gen, 1u);
+        }
+        else
+        {
+            unsigned spins = 0;
+            while (__hip_atomic_load(&gb->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g)
+            {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > (1u << 22) || __hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                {
+                    atomicOr(timeout, 1);
+                    break;
+                }
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------
+// scans with a device-side count (reduce / partials / down-sweep, fixed block count)
+// ------------------------------------------------------------------------------------------
+__global__ void k_scan_reduce(const int *in, const int *nptr, int nconst, int *partial)
+{
+    __shared__ int lds[16];
+    int n = nptr ? *nptr : nconst;
+    int chunk = (n + gridDim.x - 1) / gridDim.x;
+    int begin = blockIdx.x * chunk;
+    int end = min(begin + chunk, n);
+    int s = 0;
+    for (int i = begin + threadIdx.x; i < end; i += blockDim.x)
+        s += in[i];
+    int tot;
+    block_exclusive_scan(s, lds, tot);
+    if (threadIdx.x == 0)
+        partial[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_partials(int *partial, int nparts, int *total)
+{
+    __shared__ int lds[16];
+    int v = (int)threadIdx.x < nparts ? partial[threadIdx.x] : 0;
+    int tot;
+    int ex = block_exclusive_scan(v, lds, tot);
+    if ((int)threadIdx.x < nparts)
+        partial[threadIdx.x] = ex;
+    if (threadIdx.x == 0 && total)
+        *total = tot;
+}
+
+__global__ void k_scan_down(const int *in, int *out, const int *nptr, int nconst, const int *partial)
+{
+    __shared__ int lds[16];
+    int n = nptr ? *nptr : nconst;
+    int chunk = (n + gridDim.x - 1) / gridDim.x;
+    int begin = blockIdx.x * chunk;
+    int end = min(begin + chunk, n);
+    int running = partial[blockIdx.x];
+    for (int base = begin; base < end; base += blockDim.x)
+    {
+        int i = base + threadIdx.x;
+        int v = i < end ? in[i] : 0;
+        int tot;
+        int ex = block_exclusive_scan(v, lds, tot);
+        if (i < end)
+            out[i] = running + ex;
+        running += tot;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// stable LSD radix sort, 8 bits per pass, wave-ballot ranking
+// ------------------------------------------------------------------------------------------
+__global__ void k_radix_hist(const unsigned *keys, int n, int shift, int *hist, int tiles)
+{
+    __shared__ int cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int base = blockIdx.x * kRadixTile;
+    for (int r = 0; r < kRadixTile / 256; ++r)
+    {
+        int i = base + r * 256 + threadIdx.x;
+        if (i < n)
+            atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ void k_radix_scatter(const unsigned *keys, const int *vals, int n, int shift, const int *offs, int tiles,
+                                unsigned *keysOut, int *valsOut)
+{
+    __shared__ int base[256];
+    __shared__ int waveCnt[4][256];
+    base[threadIdx.x] = offs[threadIdx.x * tiles + blockIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int r = 0; r < kRadixTile / 256; ++r)
+    {
+        for (int t = threadIdx.x; t < 4 * 256; t += 256)
+            (&waveCnt[0][0])[t] = 0;
+        __syncthreads();
+        int i = blockIdx.x * kRadixTile + r * 256 + threadIdx.x;
+        bool valid = i < n;
+        unsigned k = valid ? keys[i] : 0u;
+        int v = valid ? vals[i] : 0;
+        int d = (int)((k >> shift) & 255u);
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b)
+        {
+            int bit = (d >> b) & 1;
+            unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        int rank = __popcll(peers & lt);
+        if (valid && rank == 0)
+            waveCnt[wave][d] = __popcll(peers);
+        __syncthreads();
+        {
+            int dd = threadIdx.x;
+            int run = base[dd];
+            for (int w = 0; w < 4; ++w)
+            {
+                int c = waveCnt[w][dd];
+                waveCnt[w][dd] = run;
+                run += c;
+            }
+            base[dd] = run;
+        }
+        __syncthreads();
+        if (valid)
+        {
+            int dst = waveCnt[wave][d] + rank;
+            keysOut[dst] = k;
+            valsOut[dst] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// small utilities
+// ------------------------------------------------------------------------------------------
+__global__ void k_fill_int(int *p, int n, int v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Per-step counter reset (a kernel instead of a memset so the step can be graph-captured).
+__global__ void k_step_begin(Counters *c)
+{
+    c->numChunks = 0;
+    c->numHits = 0;
+    c->numCreated = 0;
+    c->pairChecks = 0;
+    c->axisAmbiguous = 0;
+    c->colorCount = 0;
+    c->coloringRounds = 0;
+    c->solvedCount = 0;
+    c->contactCount = 0;
+    c->colorChanged[0] = c->colorChanged[1] = c->colorChanged[2] = 0u;
+}
+
+// ------------------------------------------------------------------------------------------
+// sweep axis: the reference picks the axis of largest centre variance from serial float sums
+// (solver.cpp:499-515). The decision is taken in double; only when two variances lie within
+// the float sums' error bound of each other is the exact serial float pass run.
+// ------------------------------------------------------------------------------------------
+__global__ void k_axis(const float4 *pos, int n, Counters *ctr)
+{
+    __shared__ double red[3][256];
+    double sx = 0, sy = 0, sz = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+    {
+        float4 p = pos[i];
+        sx += p.x;
+        sy += p.y;
+        sz += p.z;
+    }
+    red[0][threadIdx.x] = sx;
+    red[1][threadIdx.x] = sy;
+    red[2][threadIdx.x] = sz;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1)
+    {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 3; ++k)
+                red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    double mx = red[0][0] / n, my = red[1][0] / n, mz = red[2][0] / n;
+    __syncthreads();
+    double vx = 0, vy = 0, vz = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+    {
+        float4 p = pos[i];
+        double dx = p.x - mx, dy = p.y - my, dz = p.z - mz;
+        vx += dx * dx;
+        vy += dy * dy;
+        vz += dz * dz;
+    }
+    red[0][threadIdx.x] = vx;
+    red[1][threadIdx.x] = vy;
+    red[2][threadIdx.x] = vz;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1)
+    {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 3; ++k)
+                red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        double v[3] = {red[0][0], red[1][0], red[2][0]};
+        int best = 0;
+        if (v[1] > v[best])
+            best = 1;
+        if (v[2] > v[best])
+            best = 2;
+        // relative error bound of an n-term serial float sum of positive terms, with margin
+        double tol = 4.0 * ((double)n + 16.0) * 5.960464477539063e-08;
+        int amb = 0;
+        for (int k = 0; k < 3; ++k)
+            if (k != best && v[k] >= v[best] * (1.0 - tol))
+                amb = 1;
+        ctr->sweepAxis = best;
+        ctr->axisAmbiguous = amb;
+    }
+}
+
+__global__ void k_axis_serial(const float4 *pos, int n, Counters *ctr)
+{
+    if (threadIdx.x != 0 || !ctr->axisAmbiguous)
+        return;
+    V3 mean{0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < n; ++i)
+        mean += ld3(pos[i]);
+    mean = mean / (float)n;
+    V3 var{0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < n; ++i)
+    {
+        V3 d = ld3(pos[i]) - mean;
+        var += V3{d.x * d.x, d.y * d.y, d.z * d.z};
+    }
+    int best = 0;
+    if (var.y > comp(var, best))
+        best = 1;
+    if (var.z > comp(var, best))
+        best = 2;
+    ctr->sweepAxis = best;
+    ctr->axisFallbacks += 1;
+}
+
+// Interval start keys: order-preserving float bits, -0 folded onto +0 so that ties fall to the
+// body index exactly as the reference comparator does (solver.cpp:525-531).
+__global__ void k_sap_keys(const float4 *pos, const float4 *sizeRad, int n, const Counters *ctr, unsigned *keys,
+                           int *vals)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int axis = ctr->sweepAxis;
+    float c = comp(ld3(pos[i]), axis);
+    float lo = c - sizeRad[i].w;
+    if (lo == 0.0f)
+        lo = 0.0f;
+    unsigned u = __float_as_uint(lo);
+    keys[i] = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    vals[i] = i;
+}
+
+__global__ void k_sap_gather(const int *sortedIdx, const float4 *pos, const float4 *sizeRad, int n, const Counters *ctr,
+                             float *sLo, float *sHi, float4 *sPos, int *sIdx)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    int i = sortedIdx[k];
+    float4 p = pos[i];
+    float r = sizeRad[i].w;
+    float c = comp(ld3(p), ctr->sweepAxis);
+    sLo[k] = c - r;
+    sHi[k] = c + r;
+    sPos[k] = make_float4(p.x, p.y, p.z, r);
+    sIdx[k] = i;
+}
+
+// Sweep extent of each sorted interval: the first later interval whose start exceeds this one's
+// end stops the reference's sweep (solver.cpp:555-556); starts are sorted, so a binary search finds it.
+__global__ void k_sweep_len(const float *sLo, const float *sHi, int n, int *sweepEnd, int *chunkCount, Counters *ctr)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int len = 0;
+    if (i < n)
+    {
+        float hi = sHi[i];
+        int lo_ = i + 1, hi_ = n;
+        while (lo_ < hi_)
+        {
+            int mid = (lo_ + hi_) >> 1;
+            if (sLo[mid] > hi)
+                hi_ = mid;
+            else
+                lo_ = mid + 1;
+        }
+        sweepEnd[i] = lo_;
+        len = lo_ - i - 1;
+        chunkCount[i] = (len + kSweepChunk - 1) / kSweepChunk;
+    }
+    int s = wave_sum(len);
+    if ((threadIdx.x & 63) == 0 && s)
+        atomicAdd(&ctr->pairChecks, s);
+}
+
+__device__ __forceinline__ int chunk_owner(const int *chunkStart, int n, int t)
+{
+    int lo = 0, hi = n;
+    while (lo < hi)
+    {
+        int mid = (lo + hi) >> 1;
+        if (chunkStart[mid] > t)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo - 1;
+}
+
+// Bounding-sphere test of the reference sweep (solver.cpp:558-561).
+__device__ __forceinline__ bool sphere_hit(float4 a, float4 b)
+{
+    V3 dp = V3{a.x, a.y, a.z} - V3{b.x, b.y, b.z};
+    float r = a.w + b.w;
+    return !(dot(dp, dp) > r * r);
+}
+
+__global__ void k_sweep_count(const int *chunkStart, const int *sweepEnd, int n, const Counters *ctr,
+                              const float4 *sPos, int *hitCount)
+{
+    int T = ctr->numChunks;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x)
+    {
+        int i = chunk_owner(chunkStart, n, t);
+        int j0 = i + 1 + (t - chunkStart[i]) * kSweepChunk;
+        int j1 = min(j0 + kSweepChunk, sweepEnd[i]);
+        float4 a = sPos[i];
+        int c = 0;
+        for (int j = j0; j < j1; ++j)
+            c += sphere_hit(a, sPos[j]) ? 1 : 0;
+        hitCount[t] = c;
+    }
+}
+
+__device__ __forceinline__ unsigned long long pair_key(int a, int b)
+{
+    unsigned lo = (unsigned)min(a, b), hi = (unsigned)max(a, b);
+    return ((unsigned long long)lo << 32) | hi;
+}
+
+__device__ __forceinline__ unsigned hash_slot(unsigned long long k, int mask)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return (unsigned)k & (unsigned)mask;
+}
+
+constexpr unsigned long long kHashEmpty = ~0ull;
+
+__device__ __forceinline__ void hash_insert(unsigned long long *h, int mask, unsigned long long key, int *overflow)
+{
+    unsigned s = hash_slot(key, mask);
+    for (int probe = 0; probe <= mask; ++probe)
+    {
+        unsigned long long prev = atomicCAS(&h[s], kHashEmpty, key);
+        if (prev == kHashEmpty || prev == key)
+            return;
+        s = (s + 1) & (unsigned)mask;
+    }
+    atomicOr(overflow, 16);
+}
+
+__device__ __forceinline__ bool hash_contains(const unsigned long long *h, int mask, unsigned long long key)
+{
+    unsigned s = hash_slot(key, mask);
+    for (int probe = 0; probe <= mask; ++probe)
+    {
+        unsigned long long v = h[s];
+        if (v == key)
+            return true;
+        if (v == kHashEmpty)
+            return false;
+        s = (s + 1) & (unsigned)mask;
+    }
+    return false;
+}
+
+// Pairs already joined by any force are skipped by the broadphase (Rigid::constrainedTo, rigid.cpp:105-116).
+__global__ void k_hash_insert(const int2 *mab, const Counters *ctr, const Joint *joints, int nj, const Spring *springs,
+                              int ns, unsigned long long *h, int mask, Counters *ctrw)
+{
+    int nm = ctr->numManifolds;
+    int total = nm + nj + ns;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x)
+    {
+        int a, b;
+        if (t < nm)
+        {
+            int2 p = mab[t];
+            a = p.x;
+            b = p.y;
+        }
+        else if (t < nm + nj)
+        {
+            const Joint &j = joints[t - nm];
+            if (j.dead || j.a < 0)
+                continue;
+            a = j.a;
+            b = j.b;
+        }
+        else
+        {
+            const Spring &s = springs[t - nm - nj];
+            a = s.a;
+            b = s.b;
+        }
+        hash_insert(h, mask, pair_key(a, b), &ctrw->overflow);
+    }
+}
+
+__global__ void k_sweep_emit(const int *chunkStart, const int *sweepEnd, int n, Counters *ctr, const float4 *sPos,
+                             const int *sIdx, const int *hitStart, const unsigned long long *h, int mask, int2 *hits,
+                             int *hitNew, int maxPairs)
+{
+    int T = ctr->numChunks;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x)
+    {
+        int i = chunk_owner(chunkStart, n, t);
+        int j0 = i + 1 + (t - chunkStart[i]) * kSweepChunk;
+        int j1 = min(j0 + kSweepChunk, sweepEnd[i]);
+        float4 a = sPos[i];
+        int ia = sIdx[i];
+        int out = hitStart[t];
+        for (int j = j0; j < j1; ++j)
+        {
+            if (!sphere_hit(a, sPos[j]))
+                continue;
+            if (out < maxPairs)
+            {
+                int ib = sIdx[j];
+                hits[out] = make_int2(ia, ib);
+                hitNew[out] = hash_contains(h, mask, pair_key(ia, ib)) ? 0 : 1;
+            }
+            else
+                atomicOr(&ctr->overflow, 1);
+            ++out;
+        }
+    }
+}
+
+// New manifolds are appended in sweep order with consecutive creation serials (solver.cpp:243-247).
+__global__ void k_create(const int2 *hits, const int *hitNew, const int *hitRank, Counters *ctr, Manifolds M,
+                         int maxManifolds, int maxPairs)
+{
+    int H = min(ctr->numHits, maxPairs);
+    int base = ctr->numManifolds;
+    long long serial0 = ctr->nextSerial;
+    for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < H; h += gridDim.x * blockDim.x)
+    {
+        if (!hitNew[h])
+            continue;
+        int m = base + hitRank[h];
+        if (m >= maxManifolds)
+        {
+            atomicOr(&ctr->overflow, 2);
+            continue;
+        }
+        M.ab[m] = hits[h];
+        M.serial[m] = serial0 + hitRank[h];
+        M.count[m] = 0;
+    }
+}
+
+__global__ void k_create_finish(Counters *ctr, int maxManifolds, int maxPairs)
+{
+    if (ctr->numHits > maxPairs)
+        ctr->overflow |= 1;
+    int total = ctr->numManifolds + ctr->numCreated;
+    if (total > maxManifolds)
+    {
+        ctr->overflow |= 2;
+        total = maxManifolds;
+    }
+    ctr->numTotal = total;
+    ctr->nextSerial += ctr->numCreated;
+}
+
+// ------------------------------------------------------------------------------------------
+// box-box narrowphase (collide.cpp:60-433, 898-955)
+// ------------------------------------------------------------------------------------------
+struct Obb
+{
+    V3 c, h, ax0, ax1, ax2;
+};
+
+__device__ __forceinline__ V3 obb_ax(const Obb &o, int k) { return k == 0 ? o.ax0 : (k == 1 ? o.ax1 : o.ax2); }
+
+__device__ __forceinline__ Obb make_obb(V3 pos, Q4 rot, V3 size)
+{
+    Obb o;
+    o.c = pos;
+    o.h = size * 0.5f;
+    o.ax0 = rotate(rot, V3{1.0f, 0.0f, 0.0f});
+    o.ax1 = rotate(rot, V3{0.0f, 1.0f, 0.0f});
+    o.ax2 = rotate(rot, V3{0.0f, 0.0f, 1.0f});
+    return o;
+}
+
+__device__ __forceinline__ V3 support_pt(const Obb &o, V3 d)
+{
+    float sx = dot(d, o.ax0) >= 0.0f ? 1.0f : -1.0f;
+    float sy = dot(d, o.ax1) >= 0.0f ? 1.0f : -1.0f;
+    float sz = dot(d, o.ax2) >= 0.0f ? 1.0f : -1.0f;
+    return o.c + o.ax0 * (o.h.x * sx) + o.ax1 * (o.h.y * sy) + o.ax2 * (o.h.z * sz);
+}
+
+struct SatBest
+{
+    int type, ia, ib;
+    float sep;
+    V3 n;
+    bool valid;
+};
+
+__device__ __forceinline__ bool sat_axis(const Obb &A, const Obb &B, V3 delta, V3 axis, int type, int ia, int ib,
+                                         SatBest &best)
+{
+    float l2 = lensq(axis);
+    if (l2 < kSatEps)
+        return true;
+    float inv = 1.0f / sqrtf(l2);
+    V3 n = axis * inv;
+    if (dot(n, delta) < 0.0f)
+        n = -n;
+    float dist = fabsf(dot(delta, n));
+    float rA = A.h.x * fabsf(dot(n, A.ax0)) + A.h.y * fabsf(dot(n, A.ax1)) + A.h.z * fabsf(dot(n, A.ax2));
+    float rB = B.h.x * fabsf(dot(n, B.ax0)) + B.h.y * fabsf(dot(n, B.ax1)) + B.h.z * fabsf(dot(n, B.ax2));
+    float sep = dist - (rA + rB);
+    if (sep > 0.0f)
+        return false;
+    if (!best.valid || sep > best.sep)
+    {
+        best.valid = true;
+        best.type = type;
+        best.ia = ia;
+        best.ib = ib;
+        best.sep = sep;
+        best.n = n;
+    }
+    return true;
+}
+
+struct NewContacts
+{
+    int count;
+    int feature[kMaxContacts];
+    V3 rA[kMaxContacts];
+    V3 rB[kMaxContacts];
+    V3 mid[kMaxContacts];
+};
+
+__device__ __forceinline__ void add_contact(NewContacts &nc, V3 posA, Q4 rotA, V3 posB, Q4 rotB, V3 xA, V3 xB, int key)
+{
+    V3 m = (xA + xB) * 0.5f;
+    for (int i = 0; i < nc.count; ++i)
+        if (lensq(m - nc.mid[i]) < kMergeDistSq)
+            return;
+    if (nc.count >= kMaxContacts)
+        return;
+    int c = nc.count;
+    nc.feature[c] = key;
+    nc.rA[c] = rotate(qconj(rotA), xA - posA);
+    nc.rB[c] = rotate(qconj(rotB), xB - posB);
+    nc.mid[c] = m;
+    nc.count = c + 1;
+}
+
+__device__ __forceinline__ int clip_plane(const V3 *in, int n, V3 pn, float off, V3 *out)
+{
+    if (n <= 0)
+        return 0;
+    int m = 0;
+    V3 a = in[n - 1];
+    float da = dot(pn, a) - off;
+    for (int i = 0; i < n; ++i)
+    {
+        V3 b = in[i];
+        float db = dot(pn, b) - off;
+        bool ain = da <= kPlaneEps;
+        bool bin = db <= kPlaneEps;
+        if (ain != bin)
+        {
+            float t = 0.0f;
+            float den = da - db;
+            if (fabsf(den) > kSatEps)
+                t = clampr(da / den, 0.0f, 1.0f);
+            if (m < 16)
+                out[m++] = a + (b - a) * t;
+        }
+        if (bin && m < 16)
+            out[m++] = b;
+        a = b;
+        da = db;
+    }
+    return m;
+}
+
+__device__ __forceinline__ void face_axes(const Obb &o, int k, V3 &u, V3 &v, float &eu, float &ev)
+{
+    if (k == 0)
+    {
+        u = o.ax1;
+        v = o.ax2;
+        eu = o.h.y;
+        ev = o.h.z;
+    }
+    else if (k == 1)
+    {
+        u = o.ax0;
+        v = o.ax2;
+        eu = o.h.x;
+        ev = o.h.z;
+    }
+    else
+    {
+        u = o.ax0;
+        v = o.ax1;
+        eu = o.h.x;
+        ev = o.h.y;
+    }
+}
+
+__device__ void face_manifold(NewContacts &nc, V3 posA, Q4 rotA, V3 posB, Q4 rotB, const Obb &A, const Obb &B,
+                              bool refA, int refAxis, V3 nAB)
+{
+    const Obb &ref = refA ? A : B;
+    const Obb &inc = refA ? B : A;
+    V3 outward = refA ? nAB : -nAB;
+    V3 rax = obb_ax(ref, refAxis);
+    float fs = dot(outward, rax) >= 0.0f ? 1.0f : -1.0f;
+    V3 fn = rax * fs;
+    V3 fc = ref.c + fn * comp(ref.h, refAxis);
+    V3 fu, fv;
+    float eu, ev;
+    face_axes(ref, refAxis, fu, fv, eu, ev);
+
+    int incAxis = 0;
+    float best = -FLT_MAX;
+    for (int i = 0; i < 3; ++i)
+    {
+        float d = fabsf(dot(obb_ax(inc, i), fn));
+        if (d > best)
+        {
+            best = d;
+            incAxis = i;
+        }
+    }
+    V3 p0[16], p1[16];
+    {
+        V3 iax = obb_ax(inc, incAxis);
+        float s = dot(iax, fn) > 0.0f ? -1.0f : 1.0f;
+        V3 n = iax * s;
+        V3 c = inc.c + n * comp(inc.h, incAxis);
+        V3 u, v;
+        float ue, ve;
+        face_axes(inc, incAxis, u, v, ue, ve);
+        p0[0] = c + u * ue + v * ve;
+        p0[1] = c - u * ue + v * ve;
+        p0[2] = c - u * ue - v * ve;
+        p0[3] = c + u * ue - v * ve;
+    }
+    int n = 4;
+    n = clip_plane(p0, n, fu, dot(fu, fc) + eu, p1);
+    if (!n)
+        return;
+    V3 nu = -fu;
+    n = clip_plane(p1, n, nu, dot(nu, fc) + eu, p0);
+    if (!n)
+        return;
+    n = clip_plane(p0, n, fv, dot(fv, fc) + ev, p1);
+    if (!n)
+        return;
+    V3 nv = -fv;
+    n = clip_plane(p1, n, nv, dot(nv, fc) + ev, p0);
+    if (!n)
+        return;
+    int prefix = ((refA ? 0 : 1) << 24) | ((refAxis & 0xFF) << 16) | ((incAxis & 0xFF) << 8);
+    for (int i = 0; i < n && nc.count < kMaxContacts; ++i)
+    {
+        V3 pi = p0[i];
+        float depth = dot(pi - fc, fn);
+        if (depth > kPlaneEps)
+            continue;
+        V3 pr = pi - fn * depth;
+        add_contact(nc, posA, rotA, posB, rotB, refA ? pr : pi, refA ? pi : pr, prefix | (i & 0xFF));
+    }
+    if (!nc.count)
+        add_contact(nc, posA, rotA, posB, rotB, support_pt(A, nAB), support_pt(B, -nAB), prefix);
+}
+
+__device__ __forceinline__ void edge_support(const Obb &o, int k, V3 d, V3 &e0, V3 &e1)
+{
+    int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+    V3 a1 = obb_ax(o, k1), a2 = obb_ax(o, k2), a0 = obb_ax(o, k);
+    float s1 = dot(d, a1) >= 0.0f ? 1.0f : -1.0f;
+    float s2 = dot(d, a2) >= 0.0f ? 1.0f : -1.0f;
+    V3 mid = o.c + a1 * (comp(o.h, k1) * s1) + a2 * (comp(o.h, k2) * s2);
+    e0 = mid - a0 * comp(o.h, k);
+    e1 = mid + a0 * comp(o.h, k);
+}
+
+__device__ __forceinline__ void closest_seg(V3 p0, V3 p1, V3 q0, V3 q1, V3 &c0, V3 &c1)
+{
+    V3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
+    float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    float s = 0.0f, t = 0.0f;
+    if (a <= kSatEps && e <= kSatEps)
+    {
+        c0 = p0;
+        c1 = q0;
+        return;
+    }
+    if (a <= kSatEps)
+        t = clampr(f / e, 0.0f, 1.0f);
+    else
+    {
+        float c = dot(d1, r);
+        if (e <= kSatEps)
+            s = clampr(-c / a, 0.0f, 1.0f);
+        else
+        {
+            float b = dot(d1, d2);
+            float den = a * e - b * b;
+            if (fabsf(den) > kSatEps)
+                s = clampr((b * f - c * e) / den, 0.0f, 1.0f);
+            t = (b * s + f) / e;
+            if (t < 0.0f)
+            {
+                t = 0.0f;
+                s = clampr(-c / a, 0.0f, 1.0f);
+            }
+            else if (t > 1.0f)
+            {
+                t = 1.0f;
+                s = clampr((b - c) / a, 0.0f, 1.0f);
+            }
+        }
+    }
+    c0 = p0 + d1 * s;
+    c1 = q0 + d2 * t;
+}
+
+// Returns the contact count; basis rows are written only when the boxes overlap on all 15 axes.
+__device__ int collide_boxes(V3 posA, Q4 rotA, V3 sizeA, V3 posB, Q4 rotB, V3 sizeB, NewContacts &nc, V3 &b0, V3 &b1,
+                             V3 &b2)
+{
+    nc.count = 0;
+    Obb A = make_obb(posA, rotA, sizeA), B = make_obb(posB, rotB, sizeB);
+    V3 delta = B.c - A.c;
+    SatBest face, edge;
+    face.valid = false;
+    face.sep = -FLT_MAX;
+    face.type = face.ia = face.ib = 0;
+    face.n = V3{0, 0, 0};
+    edge = face;
+    for (int i = 0; i < 3; ++i)
+        if (!sat_axis(A, B, delta, obb_ax(A, i), 0, i, -1, face))
+            return 0;
+    for (int i = 0; i < 3; ++i)
+        if (!sat_axis(A, B, delta, obb_ax(B, i), 1, -1, i, face))
+            return 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (!sat_axis(A, B, delta, cross(obb_ax(A, i), obb_ax(B, j)), 2, i, j, edge))
+                return 0;
+    if (!face.valid)
+        return 0;
+    SatBest best = face;
+    if (edge.valid && 0.95f * edge.sep > face.sep + 0.01f)
+        best = edge;
+    M3 basis = orthonormal(-best.n);
+    b0 = basis.r0;
+    b1 = basis.r1;
+    b2 = basis.r2;
+    if (best.type == 2)
+    {
+        V3 a0, a1, q0, q1, xA, xB;
+        edge_support(A, best.ia, best.n, a0, a1);
+        edge_support(B, best.ib, -best.n, q0, q1);
+        closest_seg(a0, a1, q0, q1, xA, xB);
+        int key = (2 << 24) | ((best.ia & 0xFF) << 8) | (best.ib & 0xFF);
+        add_contact(nc, posA, rotA, posB, rotB, xA, xB, key);
+        if (!nc.count)
+            add_contact(nc, posA, rotA, posB, rotB, support_pt(A, best.n), support_pt(B, -best.n), key);
+        return nc.count;
+    }
+    if (best.type == 0)
+        face_manifold(nc, posA, rotA, posB, rotB, A, B, true, best.ia, best.n);
+    else
+        face_manifold(nc, posA, rotA, posB, rotB, A, B, false, best.ib, best.n);
+    return nc.count;
+}
+
+// Manifold::initialize for box pairs (manifold.cpp:100-174). Reads the persistent buffer `cur`
+// and writes the initialised manifold to the same slot of `tmp`.
+__global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, Counters *ctr, int *keep)
+{
+    int total = ctr->numTotal;
+    int old = ctr->numManifolds;
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < total; m += gridDim.x * blockDim.x)
+    {
+        int2 ab = cur.ab[m];
+        int nOld = m < old ? cur.count[m] : 0;
+        V3 pA = ld3(B.pos[ab.x]), pB = ld3(B.pos[ab.y]);
+        Q4 qA = ldq(B.rot[ab.x]), qB = ldq(B.rot[ab.y]);
+        V3 sA = ld3(B.sizeRad[ab.x]), sB = ld3(B.sizeRad[ab.y]);
+        float friction = sqrtf(B.friction[ab.x] * B.friction[ab.y]);
+        NewContacts nc;
+        V3 b0{0, 0, 0}, b1{0, 0, 0}, b2{0, 0, 0};
+        int nn = collide_boxes(pA, qA, sA, pB, qB, sB, nc, b0, b1, b2);
+        if (nn == 0)
+        {
+            keep[m] = 0;
+            continue;
+        }
+        keep[m] = 1;
+        tmp.ab[m] = ab;
+        tmp.serial[m] = cur.serial[m];
+        tmp.count[m] = nn;
+        tmp.basis0[m] = st3(b0, friction);
+        tmp.basis1[m] = st3(b1, 0.0f);
+        tmp.basis2[m] = st3(b2, 0.0f);
+        for (int i = 0; i < nn; ++i)
+        {
+            V3 rA = nc.rA[i], rB = nc.rB[i];
+            float4 pen = make_float4(0, 0, 0, 0), lam = make_float4(0, 0, 0, 0);
+            for (int j = 0; j < nOld; ++j)
+            {
+                if (cur.feature[m * kMaxContacts + j] == nc.feature[i])
+                {
+                    int s = m * kMaxContacts + j;
+                    pen = cur.pen[s];
+                    lam = cur.lam[s];
+                    if (lam.w != 0.0f)
+                    {
+                        rA = ld3(cur.rA[s]);
+                        rB = ld3(cur.rB[s]);
+                    }
+                    break;
+                }
+            }
+            V3 xA = pA + rotate(qA, rA);
+            V3 xB = pB + rotate(qB, rB);
+            V3 d = xA - xB;
+            V3 c0 = V3{dot(b0, d), dot(b1, d), dot(b2, d)} + V3{kMargin, 0.0f, 0.0f};
+            V3 l = V3{lam.x, lam.y, lam.z} * P.alpha * P.gamma;
+            V3 k = V3{pen.x, pen.y, pen.z} * P.gamma;
+            k = V3{clampr(k.x, kPenaltyMin, kPenaltyMax), clampr(k.y, kPenaltyMin, kPenaltyMax),
+                   clampr(k.z, kPenaltyMin, kPenaltyMax)};
+            int s = m * kMaxContacts + i;
+            tmp.feature[s] = nc.feature[i];
+            tmp.rA[s] = st3(rA, 0.0f);
+            tmp.rB[s] = st3(rB, 0.0f);
+            tmp.c0[s] = st3(c0, 0.0f);
+            tmp.pen[s] = st3(k, 0.0f);
+            tmp.lam[s] = st3(l, lam.w);
+        }
+    }
+}
+
+__global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *keep, const int *keepRank, Counters *ctr)
+{
+    int total = ctr->numTotal;
+    int contacts = 0;
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < total; m += gridDim.x * blockDim.x)
+    {
+        if (!keep[m])
+            continue;
+        int d = keepRank[m];
+        int n = tmp.count[m];
+        contacts += n;
+        cur.ab[d] = tmp.ab[m];
+        cur.serial[d] = tmp.serial[m];
+        cur.count[d] = n;
+        cur.basis0[d] = tmp.basis0[m];
+        cur.basis1[d] = tmp.basis1[m];
+        cur.basis2[d] = tmp.basis2[m];
+        for (int i = 0; i < n; ++i)
+        {
+            int s = m * kMaxContacts + i, o = d * kMaxContacts + i;
+            cur.feature[o] = tmp.feature[s];
+            cur.rA[o] = tmp.rA[s];
+            cur.rB[o] = tmp.rB[s];
+            cur.c0[o] = tmp.c0[s];
+            cur.pen[o] = tmp.pen[s];
+            cur.lam[o] = tmp.lam[s];
+        }
+    }
+    contacts = wave_sum(contacts);
+    if ((threadIdx.x & 63) == 0 && contacts)
+        atomicAdd(&ctr->contactCount, contacts);
+}
+
+__global__ void k_compact_finish(Counters *ctr) { ctr->numManifolds = ctr->numKept; }
+
+// Joint::initialize (joint.cpp:35-54); a broken joint is removed (dead) here.
+__global__ void k_joint_init(Joint *joints, int nj, Bodies B, Params P)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nj)
+        return;
+    Joint j = joints[k];
+    if (j.dead)
+        return;
+    V3 pb = rotate(ldq(B.rot[j.b]), ld3(j.rB)) + ld3(B.pos[j.b]);
+    V3 pa = j.a >= 0 ? rotate(ldq(B.rot[j.a]), ld3(j.rA)) + ld3(B.pos[j.a]) : ld3(j.rA);
+    V3 c0l = pa - pb;
+    Q4 qa = j.a >= 0 ? ldq(B.rot[j.a]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
+    V3 c0a = qdiff(qa, ldq(B.rot[j.b])) * j.c0Lin.w;
+    V3 ll = ld3(j.lamLin) * P.alpha * P.gamma;
+    V3 la = ld3(j.lamAng) * P.alpha * P.gamma;
+    V3 pl = ld3(j.penLin) * P.gamma;
+    V3 pa2 = ld3(j.penAng) * P.gamma;
+    float kl = j.rA.w, ka = j.rB.w;
+    pl = V3{clampr(pl.x, kPenaltyMin, kPenaltyMax), clampr(pl.y, kPenaltyMin, kPenaltyMax),
+            clampr(pl.z, kPenaltyMin, kPenaltyMax)};
+    pa2 = V3{clampr(pa2.x, kPenaltyMin, kPenaltyMax), clampr(pa2.y, kPenaltyMin, kPenaltyMax),
+             clampr(pa2.z, kPenaltyMin, kPenaltyMax)};
+    pl = V3{fminr(pl.x, kl), fminr(pl.y, kl), fminr(pl.z, kl)};
+    pa2 = V3{fminr(pa2.x, ka), fminr(pa2.y, ka), fminr(pa2.z, ka)};
+    joints[k].c0Lin = st3(c0l, j.c0Lin.w);
+    joints[k].c0Ang = st3(c0a, j.c0Ang.w);
+    joints[k].lamLin = st3(ll, 0.0f);
+    joints[k].lamAng = st3(la, 0.0f);
+    joints[k].penLin = st3(pl, 0.0f);
+    joints[k].penAng = st3(pa2, 0.0f);
+    if (j.broken)
+        joints[k].dead = 1;
+}
+
+// ------------------------------------------------------------------------------------------
+// per-body force lists, descending creation serial (the order force.cpp:25-36 leaves them in)
+// ------------------------------------------------------------------------------------------
+template <bool FILL>
+__global__ void k_adj(Manifolds M, const Counters *ctr, const Joint *joints, int nj, const Spring *springs, int ns,
+                      const float4 *massMoment, int *deg, const int *adjStart, int *cursor, int *adj,
+                      long long *adjSerial, int maxAdj, int *overflow)
+{
+    int nm = ctr->numManifolds;
+    int total = nm + nj + ns;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x)
+    {
+        int a, b, kind, idx;
+        long long serial;
+        if (t < nm)
+        {
+            int2 p = M.ab[t];
+            a = p.x;
+            b = p.y;
+            kind = kAdjManifold;
+            idx = t;
+            serial = M.serial[t];
+        }
+        else if (t < nm + nj)
+        {
+            const Joint &j = joints[t - nm];
+            if (j.dead)
+                continue;
+            a = j.a;
+            b = j.b;
+            kind = kAdjJoint;
+            idx = t - nm;
+            serial = j.serial;
+        }
+        else
+        {
+            const Spring &s = springs[t - nm - nj];
+            a = s.a;
+            b = s.b;
+            kind = kAdjSpring;
+            idx = t - nm - nj;
+            serial = s.serial;
+        }
+        int ends[2] = {a, b};
+        for (int e = 0; e < 2; ++e)
+        {
+            int body = ends[e];
+            if (body < 0 || !(massMoment[body].x > 0.0f))
+                continue;
+            if (!FILL)
+                atomicAdd(&deg[body], 1);
+            else
+            {
+                int slot = adjStart[body] + atomicAdd(&cursor[body], 1);
+                if (slot < maxAdj)
+                {
+                    adj[slot] = adjEncode(kind, idx);
+                    adjSerial[slot] = serial;
+                }
+                else
+                    atomicOr(overflow, 4);
+            }
+        }
+    }
+}
+
+__global__ void k_adj_sort(const int *deg, const int *adjStart, int *adj, long long *adjSerial, int n, int maxAdj)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int b = adjStart[i], e = min(b + deg[i], maxAdj);
+    for (int x = b + 1; x < e; ++x)
+    {
+        long long s = adjSerial[x];
+        int v = adj[x];
+        int y = x - 1;
+        while (y >= b && adjSerial[y] < s)
+        {
+            adjSerial[y + 1] = adjSerial[y];
+            adj[y + 1] = adj[y];
+            --y;
+        }
+        adjSerial[y + 1] = s;
+        adj[y + 1] = v;
+    }
+}
+
+// Body warm start (solver.cpp:1134-1169).
+__global__ void k_warmstart(Bodies B, Params P, const int *deg, int n, int *color, Counters *ctr)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float mass = B.massMoment[i].x;
+    V3 pos = ld3(B.pos[i]), vel = ld3(B.vel[i]), w = ld3(B.angVel[i]);
+    Q4 rot = ldq(B.rot[i]);
+    float dt = P.dt, g = P.gravity;
+    V3 iner = pos + vel * dt;
+    if (mass > 0.0f)
+        iner += V3{0.0f, 0.0f, g} * (dt * dt);
+    Q4 inerA = qintegrate(rot, w * dt);
+    V3 accel = (vel - ld3(B.prevVel[i])) / dt;
+    float accelExt = accel.z * signr(g);
+    float aw = clampr(accelExt / fabsf(g), 0.0f, 1.0f);
+    if (!isfinite(aw))
+        aw = 0.0f;
+    B.initLin[i] = st3(pos, 0.0f);
+    B.initAng[i] = stq(rot);
+    B.inerLin[i] = st3(iner, 0.0f);
+    B.inerAng[i] = stq(inerA);
+    bool solved = mass > 0.0f && deg[i] > 0;
+    if (mass > 0.0f)
+    {
+        V3 np = pos + vel * dt + V3{0.0f, 0.0f, g} * (aw * dt * dt);
+        Q4 nq = qintegrate(rot, w * dt);
+        if (!solved)
+        {
+            np = iner;
+            nq = inerA;
+        }
+        B.pos[i] = st3(np, 0.0f);
+        B.rot[i] = stq(nq);
+    }
+    if (!solved)
+        color[i] = -1;
+    else
+        atomicAdd(&ctr->solvedCount, 1);
+}
+
+// ------------------------------------------------------------------------------------------
+// colouring: c(i) = mex{ c(j) : j < i adjacent, solved } is the sequential greedy colouring
+// (avbd_gpu_solver.cpp:1812-1865); iterating it from any start converges to that unique fixpoint
+// in at most (longest index-increasing path) rounds. Warm-started from the previous step's colours.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int partner_of(int e, int i, const int2 *mab, const Joint *joints, const Spring *springs)
+{
+    int kind = adjKind(e), idx = adjIndex(e);
+    int a, b;
+    if (kind == kAdjManifold)
+    {
+        int2 p = mab[idx];
+        a = p.x;
+        b = p.y;
+    }
+    else if (kind == kAdjJoint)
+    {
+        a = joints[idx].a;
+        b = joints[idx].b;
+    }
+    else
+    {
+        a = springs[idx].a;
+        b = springs[idx].b;
+    }
+    return a == i ? b : a;
+}
+
+__global__ void k_color(const int *deg, const int *adjStart, const int *adj, const int2 *mab, const Joint *joints,
+                        const Spring *springs, const float4 *massMoment, int n, int *color, Counters *ctr,
+                        GridBarrier *gb, int maxRounds)
+{
+    const unsigned nblocks = gridDim.x;
+    __shared__ int anyChange;
+    int rounds = 0;
+    for (int r = 0; r < maxRounds; ++r)
+    {
+        if (threadIdx.x == 0)
+        {
+            anyChange = 0;
+            if (blockIdx.x == 0)
+                ctr->colorChanged[(r + 1) % 3] = 0u;
+        }
+        __syncthreads();
+        int changed = 0;
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nblocks * blockDim.x)
+        {
+            if (!(massMoment[i].x > 0.0f) || deg[i] == 0)
+                continue;
+            unsigned long long used[4] = {0ull, 0ull, 0ull, 0ull};
+            int b = adjStart[i], e = b + deg[i];
+            for (int x = b; x < e; ++x)
+            {
+                int o = partner_of(adj[x], i, mab, joints, springs);
+                if (o < 0 || o >= i)
+                    continue;
+                int c = __hip_atomic_load(&color[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (c >= 0 && c < 256)
+                    used[c >> 6] |= 1ull << (c & 63);
+            }
+            int c = 0;
+            for (int w = 0; w < 4; ++w)
+            {
+                if (~used[w])
+                {
+                    c = w * 64 + __ffsll((long long)~used[w]) - 1;
+                    break;
+                }
+                c = 256;
+            }
+            if (c >= kMaxColors)
+                atomicOr(&ctr->overflow, 8);
+            if (color[i] != c)
+            {
+                __hip_atomic_store(&color[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                changed = 1;
+            }
+        }
+        if (changed)
+            anyChange = 1;
+        __syncthreads();
+        if (threadIdx.x == 0 && anyChange)
+            atomicOr(&ctr->colorChanged[r % 3], 1u);
+        grid_sync(gb, nblocks, &ctr->barrierTimeout);
+        rounds = r + 1;
+        unsigned any = __hip_atomic_load(&ctr->colorChanged[r % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!any || ctr->barrierTimeout)
+            break;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        ctr->coloringRounds = rounds;
+}
+
+__global__ void k_color_keys(const int *color, int n, unsigned *keys, int *vals, Counters *ctr)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int mx = -1;
+    if (i < n)
+    {
+        int c = color[i];
+        keys[i] = c < 0 ? 255u : (unsigned)min(c, kMaxColors - 1);
+        vals[i] = i;
+        mx = c;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        mx = max(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0 && mx >= 0)
+        atomicMax(&ctr->colorCount, min(mx + 1, kMaxColors));
+}
+
+__global__ void k_color_offsets(const int *scannedHist, int tiles, int n, int *colorOffsets)
+{
+    int d = threadIdx.x;
+    if (d < 256)
+        colorOffsets[d] = scannedHist[d * tiles];
+    if (d == 0)
+        colorOffsets[256] = n;
+}
+
+// ------------------------------------------------------------------------------------------
+// primal / dual (solver.cpp:1177-1271)
+// ------------------------------------------------------------------------------------------
+struct Lhs
+{
+    float l00, l10, l11, l20, l21, l22;
+    float a00, a10, a11, a20, a21, a22;
+    float c[3][3];
+    V3 rl, ra;
+};
+
+// Contact force with normal clamp and friction cone (manifold.cpp:203-216).
+struct ContactEval
+{
+    float C0, C1, C2, F0, F1, F2, fs, bounds;
+};
+
+__device__ __forceinline__ ContactEval contact_eval(V3 c0, V3 pen, V3 lam, float friction, float oma, V3 b0, V3 b1,
+                                                    V3 b2, V3 nb0, V3 nb1, V3 nb2, V3 ja0, V3 ja1, V3 ja2, V3 jb0,
+                                                    V3 jb1, V3 jb2, V3 dqAL, V3 dqBL, V3 dqAA, V3 dqBA)
+{
+    ContactEval r;
+    r.C0 = (((c0.x * oma + dot(b0, dqAL)) + dot(nb0, dqBL)) + dot(ja0, dqAA)) + dot(jb0, dqBA);
+    r.C1 = (((c0.y * oma + dot(b1, dqAL)) + dot(nb1, dqBL)) + dot(ja1, dqAA)) + dot(jb1, dqBA);
+    r.C2 = (((c0.z * oma + dot(b2, dqAL)) + dot(nb2, dqBL)) + dot(ja2, dqAA)) + dot(jb2, dqBA);
+    float F0 = pen.x * r.C0 + lam.x;
+    float F1 = pen.y * r.C1 + lam.y;
+    float F2 = pen.z * r.C2 + lam.z;
+    F0 = fminr(F0, 0.0f);
+    float bounds = fabsf(F0) * friction;
+    float fs = sqrtf(F1 * F1 + F2 * F2);
+    if (fs > bounds && fs > 0.0f)
+    {
+        float q = bounds / fs;
+        F1 *= q;
+        F2 *= q;
+    }
+    r.F0 = F0;
+    r.F1 = F1;
+    r.F2 = F2;
+    r.fs = fs;
+    r.bounds = bounds;
+    return r;
+}
+
+// One contact's Hessian/gradient stamp for one side: rows J_L (l0..l2), J_A (g0..g2), K = diag(k).
+__device__ __forceinline__ void stamp_rows(Lhs &s, V3 l0, V3 l1, V3 l2, V3 g0, V3 g1, V3 g2, V3 k, float F0, float F1,
+                                          float F2)
+{
+    // T = J_L^T K, TA = J_A^T K (row i = component i of the Jacobian rows, scaled per row)
+    float T[3][3] = {{l0.x * k.x, l1.x * k.y, l2.x * k.z}, {l0.y * k.x, l1.y * k.y, l2.y * k.z},
+                     {l0.z * k.x, l1.z * k.y, l2.z * k.z}};
+    float TA[3][3] = {{g0.x * k.x, g1.x * k.y, g2.x * k.z}, {g0.y * k.x, g1.y * k.y, g2.y * k.z},
+                      {g0.z * k.x, g1.z * k.y, g2.z * k.z}};
+    float L[3][3] = {{l0.x, l0.y, l0.z}, {l1.x, l1.y, l1.z}, {l2.x, l2.y, l2.z}};
+    float G[3][3] = {{g0.x, g0.y, g0.z}, {g1.x, g1.y, g1.z}, {g2.x, g2.y, g2.z}};
+#define AVBD_PROD(A, i, B, j) ((A[i][0] * B[0][j] + A[i][1] * B[1][j]) + A[i][2] * B[2][j])
+    s.l00 += AVBD_PROD(T, 0, L, 0);
+    s.l10 += AVBD_PROD(T, 1, L, 0);
+    s.l11 += AVBD_PROD(T, 1, L, 1);
+    s.l20 += AVBD_PROD(T, 2, L, 0);
+    s.l21 += AVBD_PROD(T, 2, L, 1);
+    s.l22 += AVBD_PROD(T, 2, L, 2);
+    s.a00 += AVBD_PROD(TA, 0, G, 0);
+    s.a10 += AVBD_PROD(TA, 1, G, 0);
+    s.a11 += AVBD_PROD(TA, 1, G, 1);
+    s.a20 += AVBD_PROD(TA, 2, G, 0);
+    s.a21 += AVBD_PROD(TA, 2, G, 1);
+    s.a22 += AVBD_PROD(TA, 2, G, 2);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            s.c[i][j] += AVBD_PROD(TA, i, L, j);
+#undef AVBD_PROD
+    s.rl += V3{(L[0][0] * F0 + L[1][0] * F1) + L[2][0] * F2, (L[0][1] * F0 + L[1][1] * F1) + L[2][1] * F2,
+               (L[0][2] * F0 + L[1][2] * F1) + L[2][2] * F2};
+    s.ra += V3{(G[0][0] * F0 + G[1][0] * F1) + G[2][0] * F2, (G[0][1] * F0 + G[1][1] * F1) + G[2][1] * F2,
+               (G[0][2] * F0 + G[1][2] * F1) + G[2][2] * F2};
+}
+
+__device__ __forceinline__ void stamp_manifold(Lhs &s, int i, int m, const Manifolds &M, const Bodies &B, float oma)
+{
+    int2 ab = M.ab[m];
+    bool isA = ab.x == i;
+    V3 pA = ld3(B.pos[ab.x]), pB = ld3(B.pos[ab.y]);
+    Q4 qA = ldq(B.rot[ab.x]), qB = ldq(B.rot[ab.y]);
+    V3 dqAL = pA - ld3(B.initLin[ab.x]);
+    V3 dqAA = qdiff(qA, ldq(B.initAng[ab.x]));
+    V3 dqBL = pB - ld3(B.initLin[ab.y]);
+    V3 dqBA = qdiff(qB, ldq(B.initAng[ab.y]));
+    float4 bb0 = M.basis0[m];
+    V3 b0 = ld3(bb0), b1 = ld3(M.basis1[m]), b2 = ld3(M.basis2[m]);
+    V3 nb0 = -b0, nb1 = -b1, nb2 = -b2;
+    float friction = bb0.w;
+    int n = M.count[m];
+    for (int c = 0; c < n; ++c)
+    {
+        int sl = m * kMaxContacts + c;
+        V3 rAW = rotate(qA, ld3(M.rA[sl]));
+        V3 rBW = rotate(qB, ld3(M.rB[sl]));
+        V3 ja0 = cross(rAW, b0), ja1 = cross(rAW, b1), ja2 = cross(rAW, b2);
+        V3 jb0 = cross(rBW, nb0), jb1 = cross(rBW, nb1), jb2 = cross(rBW, nb2);
+        V3 pen = ld3(M.pen[sl]);
+        ContactEval ev = contact_eval(ld3(M.c0[sl]), pen, ld3(M.lam[sl]), friction, oma, b0, b1, b2, nb0, nb1, nb2,
+                                      ja0, ja1, ja2, jb0, jb1, jb2, dqAL, dqBL, dqAA, dqBA);
+        if (isA)
+            stamp_rows(s, b0, b1, b2, ja0, ja1, ja2, pen, ev.F0, ev.F1, ev.F2);
+        else
+            stamp_rows(s, nb0, nb1, nb2, jb0, jb1, jb2, pen, ev.F0, ev.F1, ev.F2);
+    }
+}
+
+// Dense 3x3 helpers for the joint stamp (joint.cpp:56-123), kept literal.
+struct D3
+{
+    float m[3][3];
+};
+__device__ __forceinline__ D3 d3_diag(float a, float b, float c)
+{
+    D3 r = {{{a, 0.0f, 0.0f}, {0.0f, b, 0.0f}, {0.0f, 0.0f, c}}};
+    return r;
+}
+__device__ __forceinline__ D3 d3_mul(const D3 &a, const D3 &b)
+{
+    D3 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r.m[i][j] = (a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j]) + a.m[i][2] * b.m[2][j];
+    return r;
+}
+__device__ __forceinline__ D3 d3_t(const D3 &a)
+{
+    D3 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r.m[i][j] = a.m[j][i];
+    return r;
+}
+__device__ __forceinline__ D3 d3_scale(const D3 &a, float s)
+{
+    D3 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r.m[i][j] = a.m[i][j] * s;
+    return r;
+}
+__device__ __forceinline__ V3 d3_mv(const D3 &a, V3 v)
+{
+    return V3{(a.m[0][0] * v.x + a.m[0][1] * v.y) + a.m[0][2] * v.z, (a.m[1][0] * v.x + a.m[1][1] * v.y) + a.m[1][2] * v.z,
+              (a.m[2][0] * v.x + a.m[2][1] * v.y) + a.m[2][2] * v.z};
+}
+__device__ __forceinline__ D3 d3_skew(V3 r)
+{
+    D3 o = {{{0.0f, -r.z, r.y}, {r.z, 0.0f, -r.x}, {-r.y, r.x, 0.0f}}};
+    return o;
+}
+__device__ __forceinline__ D3 gs_ball(int k, V3 v)
+{
+    float vk = comp(v, k);
+    D3 m = d3_diag(-vk, -vk, -vk);
+    m.m[0][k] += v.x;
+    m.m[1][k] += v.y;
+    m.m[2][k] += v.z;
+    return m;
+}
+__device__ __forceinline__ void lhs_add_lin(Lhs &s, const D3 &a)
+{
+    s.l00 += a.m[0][0];
+    s.l10 += a.m[1][0];
+    s.l11 += a.m[1][1];
+    s.l20 += a.m[2][0];
+    s.l21 += a.m[2][1];
+    s.l22 += a.m[2][2];
+}
+__device__ __forceinline__ void lhs_add_ang(Lhs &s, const D3 &a)
+{
+    s.a00 += a.m[0][0];
+    s.a10 += a.m[1][0];
+    s.a11 += a.m[1][1];
+    s.a20 += a.m[2][0];
+    s.a21 += a.m[2][1];
+    s.a22 += a.m[2][2];
+}
+
+__device__ __forceinline__ void stamp_joint(Lhs &s, int i, const Joint &j, const Bodies &B, float alpha)
+{
+    bool isA = j.a == i;
+    Q4 qB = ldq(B.rot[j.b]);
+    V3 pB = ld3(B.pos[j.b]);
+    V3 penL = ld3(j.penLin), penA = ld3(j.penAng);
+    if (lensq(penL) > 0.0f)
+    {
+        V3 rAW = j.a >= 0 ? rotate(ldq(B.rot[j.a]), ld3(j.rA)) : V3{0.0f, 0.0f, 0.0f};
+        V3 rBW = rotate(qB, ld3(j.rB));
+        D3 K = d3_diag(penL.x, penL.y, penL.z);
+        V3 C = (j.a >= 0 ? ld3(B.pos[j.a]) + rAW : ld3(j.rA)) - (pB + rBW);
+        if (isinf(j.rA.w))
+            C -= ld3(j.c0Lin) * alpha;
+        V3 F = d3_mv(K, C) + ld3(j.lamLin);
+        D3 jL = isA ? d3_diag(1.0f, 1.0f, 1.0f) : d3_diag(-1.0f, -1.0f, -1.0f);
+        D3 jA = isA ? d3_skew(-rAW) : d3_skew(rBW);
+        D3 jLT = d3_t(jL), jAT = d3_t(jA);
+        D3 jATk = d3_mul(jAT, K);
+        lhs_add_lin(s, d3_mul(d3_mul(jLT, K), jL));
+        lhs_add_ang(s, d3_mul(jATk, jA));
+        D3 cr = d3_mul(jATk, jL);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                s.c[a][b] += cr.m[a][b];
+        V3 r = isA ? rAW : -rBW;
+        D3 g0 = d3_scale(gs_ball(0, r), F.x), g1 = d3_scale(gs_ball(1, r), F.y), g2 = d3_scale(gs_ball(2, r), F.z);
+        D3 H;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                H.m[a][b] = (g0.m[a][b] + g1.m[a][b]) + g2.m[a][b];
+        V3 c0{H.m[0][0], H.m[1][0], H.m[2][0]}, c1{H.m[0][1], H.m[1][1], H.m[2][1]}, c2{H.m[0][2], H.m[1][2], H.m[2][2]};
+        lhs_add_ang(s, d3_diag(len(c0), len(c1), len(c2)));
+        s.rl += d3_mv(jLT, F);
+        s.ra += d3_mv(jAT, F);
+    }
+    if (lensq(penA) > 0.0f)
+    {
+        D3 K = d3_diag(penA.x, penA.y, penA.z);
+        Q4 qa = j.a >= 0 ? ldq(B.rot[j.a]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
+        V3 C = qdiff(qa, qB) * j.c0Lin.w;
+        if (isinf(j.rB.w))
+            C -= ld3(j.c0Ang) * alpha;
+        V3 F = d3_mv(K, C) + ld3(j.lamAng);
+        D3 jA = d3_scale(isA ? d3_diag(1.0f, 1.0f, 1.0f) : d3_diag(-1.0f, -1.0f, -1.0f), j.c0Lin.w);
+        D3 jAT = d3_t(jA);
+        lhs_add_ang(s, d3_mul(d3_mul(jAT, K), jA));
+        s.ra += d3_mv(jAT, F);
+    }
+}
+
+// Spring::updatePrimal (spring.cpp:27-69)
+__device__ __forceinline__ void stamp_spring(Lhs &s, int i, const Spring &sp, const Bodies &B)
+{
+    Q4 qA = ldq(B.rot[sp.a]), qB = ldq(B.rot[sp.b]);
+    V3 d = (rotate(qA, ld3(sp.rA)) + ld3(B.pos[sp.a])) - (rotate(qB, ld3(sp.rB)) + ld3(B.pos[sp.b]));
+    float dl = len(d);
+    if (dl <= 1.0e-6f)
+        return;
+    V3 n = d / dl;
+    float k = sp.rB.w;
+    float f = k * (dl - sp.rA.w);
+    V3 jL, jA;
+    if (i == sp.a)
+    {
+        V3 rW = rotate(qA, ld3(sp.rA));
+        jL = n;
+        jA = cross(rW, n);
+    }
+    else
+    {
+        V3 rW = rotate(qB, ld3(sp.rB));
+        jL = -n;
+        jA = -cross(rW, n);
+    }
+    // outer(a, b) * k: row r = b * a[r], then * k
+    s.l00 += (jL.x * jL.x) * k;
+    s.l10 += (jL.x * jL.y) * k;
+    s.l11 += (jL.y * jL.y) * k;
+    s.l20 += (jL.x * jL.z) * k;
+    s.l21 += (jL.y * jL.z) * k;
+    s.l22 += (jL.z * jL.z) * k;
+    s.a00 += (jA.x * jA.x) * k;
+    s.a10 += (jA.x * jA.y) * k;
+    s.a11 += (jA.y * jA.y) * k;
+    s.a20 += (jA.x * jA.z) * k;
+    s.a21 += (jA.y * jA.z) * k;
+    s.a22 += (jA.z * jA.z) * k;
+    float ga[3] = {jA.x, jA.y, jA.z}, gl[3] = {jL.x, jL.y, jL.z};
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            s.c[a][b] += (gl[b] * ga[a]) * k;
+    s.rl += jL * f;
+    s.ra += jA * f;
+}
+
+__device__ void primal_body(int i, const Bodies &B, const Manifolds &M, const Joint *joints, const Spring *springs,
+                            const int *deg, const int *adjStart, const int *adj, const Params &P)
+{
+    float4 mm = B.massMoment[i];
+    float dt2 = P.dt * P.dt;
+    float ml = mm.x / dt2;
+    float ax = mm.y / dt2, ay = mm.z / dt2, az = mm.w / dt2;
+    Lhs s;
+    s.l00 = ml;
+    s.l11 = ml;
+    s.l22 = ml;
+    s.l10 = s.l20 = s.l21 = 0.0f;
+    s.a00 = ax;
+    s.a11 = ay;
+    s.a22 = az;
+    s.a10 = s.a20 = s.a21 = 0.0f;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            s.c[a][b] = 0.0f;
+    V3 pos = ld3(B.pos[i]);
+    Q4 rot = ldq(B.rot[i]);
+    V3 dl = pos - ld3(B.inerLin[i]);
+    V3 da = qdiff(rot, ldq(B.inerAng[i]));
+    s.rl = V3{ml * dl.x, ml * dl.y, ml * dl.z};
+    s.ra = V3{ax * da.x, ay * da.y, az * da.z};
+    float oma = 1.0f - P.alpha;
+    int b = adjStart[i], e = b + deg[i];
+    for (int x = b; x < e; ++x)
+    {
+        int en = adj[x];
+        int kind = adjKind(en), idx = adjIndex(en);
+        if (kind == kAdjManifold)
+            stamp_manifold(s, i, idx, M, B, oma);
+        else if (kind == kAdjJoint)
+            stamp_joint(s, i, joints[idx], B, P.alpha);
+        else
+            stamp_spring(s, i, springs[idx], B);
+    }
+    Sys6 sys;
+    sys.l00 = s.l00;
+    sys.l10 = s.l10;
+    sys.l11 = s.l11;
+    sys.l20 = s.l20;
+    sys.l21 = s.l21;
+    sys.l22 = s.l22;
+    sys.a00 = s.a00;
+    sys.a10 = s.a10;
+    sys.a11 = s.a11;
+    sys.a20 = s.a20;
+    sys.a21 = s.a21;
+    sys.a22 = s.a22;
+    sys.c00 = s.c[0][0];
+    sys.c01 = s.c[0][1];
+    sys.c02 = s.c[0][2];
+    sys.c10 = s.c[1][0];
+    sys.c11 = s.c[1][1];
+    sys.c12 = s.c[1][2];
+    sys.c20 = s.c[2][0];
+    sys.c21 = s.c[2][1];
+    sys.c22 = s.c[2][2];
+    V3 dxL, dxA;
+    solve6(sys, -s.rl, -s.ra, dxL, dxA);
+    B.pos[i] = st3(pos + dxL, 0.0f);
+    B.rot[i] = stq(qintegrate(rot, dxA));
+}
+
+// Manifold::updateDual (manifold.cpp:231-280)
+__device__ void dual_manifold(int m, const Manifolds &M, const Bodies &B, const Params &P)
+{
+    int2 ab = M.ab[m];
+    V3 pA = ld3(B.pos[ab.x]), pB = ld3(B.pos[ab.y]);
+    Q4 qA = ldq(B.rot[ab.x]), qB = ldq(B.rot[ab.y]);
+    V3 dqAL = pA - ld3(B.initLin[ab.x]);
+    V3 dqAA = qdiff(qA, ldq(B.initAng[ab.x]));
+    V3 dqBL = pB - ld3(B.initLin[ab.y]);
+    V3 dqBA = qdiff(qB, ldq(B.initAng[ab.y]));
+    float4 bb0 = M.basis0[m];
+    V3 b0 = ld3(bb0), b1 = ld3(M.basis1[m]), b2 = ld3(M.basis2[m]);
+    V3 nb0 = -b0, nb1 = -b1, nb2 = -b2;
+    float oma = 1.0f - P.alpha;
+    int n = M.count[m];
+    for (int c = 0; c < n; ++c)
+    {
+        int sl = m * kMaxContacts + c;
+        V3 rAW = rotate(qA, ld3(M.rA[sl]));
+        V3 rBW = rotate(qB, ld3(M.rB[sl]));
+        V3 ja0 = cross(rAW, b0), ja1 = cross(rAW, b1), ja2 = cross(rAW, b2);
+        V3 jb0 = cross(rBW, nb0), jb1 = cross(rBW, nb1), jb2 = cross(rBW, nb2);
+        float4 pen4 = M.pen[sl];
+        float4 lam4 = M.lam[sl];
+        V3 pen = ld3(pen4);
+        ContactEval ev = contact_eval(ld3(M.c0[sl]), pen, ld3(lam4), bb0.w, oma, b0, b1, b2, nb0, nb1, nb2, ja0, ja1,
+                                      ja2, jb0, jb1, jb2, dqAL, dqBL, dqAA, dqBA);
+        float stick = lam4.w;
+        if (ev.F0 < 0.0f)
+            pen.x = fminr(pen.x + P.betaLin * fabsf(ev.C0), kPenaltyMax);
+        if (ev.fs <= ev.bounds)
+        {
+            pen.y = fminr(pen.y + P.betaLin * fabsf(ev.C1), kPenaltyMax);
+            pen.z = fminr(pen.z + P.betaLin * fabsf(ev.C2), kPenaltyMax);
+            stick = sqrtf(ev.C1 * ev.C1 + ev.C2 * ev.C2) < kStickThresh ? 1.0f : 0.0f;
+        }
+        M.lam[sl] = make_float4(ev.F0, ev.F1, ev.F2, stick);
+        M.pen[sl] = st3(pen, 0.0f);
+    }
+}
+
+// Joint::updateDual (joint.cpp:125-182)
+__device__ void dual_joint(Joint *jp, const Bodies &B, const Params &P)
+{
+    Joint j = *jp;
+    if (j.dead)
+        return;
+    Q4 qB = ldq(B.rot[j.b]);
+    V3 pB = ld3(B.pos[j.b]);
+    V3 penL = ld3(j.penLin), penA = ld3(j.penAng), lamL = ld3(j.lamLin), lamA = ld3(j.lamAng);
+    if (lensq(penL) > 0.0f)
+    {
+        D3 K = d3_diag(penL.x, penL.y, penL.z);
+        V3 pa = j.a >= 0 ? rotate(ldq(B.rot[j.a]), ld3(j.rA)) + ld3(B.pos[j.a]) : ld3(j.rA);
+        V3 C = pa - (rotate(qB, ld3(j.rB)) + pB);
+        if (isinf(j.rA.w))
+        {
+            C -= ld3(j.c0Lin) * P.alpha;
+            lamL = d3_mv(K, C) + lamL;
+        }
+        float cap = fminr(j.rA.w, kPenaltyMax);
+        V3 t = penL + V3{fabsf(C.x), fabsf(C.y), fabsf(C.z)} * P.betaLin;
+        penL = V3{fminr(t.x, cap), fminr(t.y, cap), fminr(t.z, cap)};
+    }
+    if (lensq(penA) > 0.0f)
+    {
+        D3 K = d3_diag(penA.x, penA.y, penA.z);
+        Q4 qa = j.a >= 0 ? ldq(B.rot[j.a]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
+        V3 C = qdiff(qa, qB) * j.c0Lin.w;
+        if (isinf(j.rB.w))
+        {
+            C -= ld3(j.c0Ang) * P.alpha;
+            lamA = d3_mv(K, C) + lamA;
+        }
+        float cap = fminr(j.rB.w, kPenaltyMax);
+        V3 t = penA + V3{fabsf(C.x), fabsf(C.y), fabsf(C.z)} * P.betaAng;
+        penA = V3{fminr(t.x, cap), fminr(t.y, cap), fminr(t.z, cap)};
+    }
+    int broken = j.broken;
+    if (lensq(lamA) > j.c0Ang.w * j.c0Ang.w)
+    {
+        penL = penA = lamL = lamA = V3{0.0f, 0.0f, 0.0f};
+        broken = 1;
+    }
+    jp->penLin = st3(penL, 0.0f);
+    jp->penAng = st3(penA, 0.0f);
+    jp->lamLin = st3(lamL, 0.0f);
+    jp->lamAng = st3(lamA, 0.0f);
+    jp->broken = broken;
+}
+
+// All solver iterations of one step in one cooperative launch: per colour a primal sweep, then
+// one dual pass over manifolds and joints, with grid barriers between phases.
+__global__ void k_iterate(Bodies B, Manifolds M, Joint *joints, int nj, const Spring *springs, const int *deg,
+                          const int *adjStart, const int *adj, const int *colorOffsets, const int *colorBodies, Params P,
+                          Counters *ctr, GridBarrier *gb)
+{
+    const unsigned nblocks = gridDim.x;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = nblocks * blockDim.x;
+    int colors = ctr->colorCount;
+    int nm = ctr->numManifolds;
+    for (int it = 0; it < P.iterations; ++it)
+    {
+        for (int c = 0; c < colors; ++c)
+        {
+            int b0 = colorOffsets[c], b1 = colorOffsets[c + 1];
+            for (int k = b0 + tid; k < b1; k += stride)
+                primal_body(colorBodies[k], B, M, joints, springs, deg, adjStart, adj, P);
+            grid_sync(gb, nblocks, &ctr->barrierTimeout);
+        }
+        for (int m = tid; m < nm; m += stride)
+            dual_manifold(m, M, B, P);
+        for (int k = tid; k < nj; k += stride)
+            dual_joint(&joints[k], B, P);
+        grid_sync(gb, nblocks, &ctr->barrierTimeout);
+        if (__hip_atomic_load(&ctr->barrierTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            break;
+    }
+}
+
+// Solver::finishStep for boxes (solver.cpp:1275-1288).
+__global__ void k_finish(Bodies B, Params P, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float4 v = B.vel[i];
+    B.prevVel[i] = v;
+    if (B.massMoment[i].x > 0.0f)
+    {
+        V3 nv = (ld3(B.pos[i]) - ld3(B.initLin[i])) / P.dt;
+        V3 nw = qdiff(ldq(B.rot[i]), ldq(B.initAng[i])) / P.dt;
+        B.vel[i] = st3(nv, 0.0f);
+        B.angVel[i] = st3(nw, 0.0f);
+    }
+}
+
+} // namespace avbd

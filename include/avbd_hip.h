@@ -1,0 +1,261 @@
+/*
+ * avbd_hip.h -- C ABI of the MI355X (gfx950) AVBD rigid-body step.
+ *
+ * This is the drop-in boundary. It replaces, for the AVBD step, the reference's
+ * PhysicsBackend implementations:
+ *   - struct PhysicsBackend { name(); step(Solver&); }            solver.h:410-415
+ *   - makeCpuReferencePhysicsBackend()                             solver.h:417, solver.cpp:671-674
+ *   - makeWebGpuAvbdPhysicsBackend(WebGpuDevice*)                  avbd_gpu_solver.h:15
+ *   - AvbdGpuSolverStats / avbdGpuSolverStats()                    avbd_gpu_solver.h:19-41
+ * A C++ PhysicsBackend adapter over these entry points lives in
+ * avbd-demo3d_amd/host/hip_avbd_backend.{h,cpp} (registered as "hip-avbd").
+ *
+ * Conventions
+ *   - Every entry point returns an int status: AVBD_OK (0) or a negative error
+ *     code. Nothing throws across the ABI. avbd_last_error() returns a
+ *     thread-local message for the most recent failure on the calling thread.
+ *   - The caller owns every host array passed in; the library owns all device
+ *     buffers. State stays resident in HBM between calls.
+ *   - Body arrays are in the solver's body-list order (solver.h:357: the list
+ *     head, i.e. the most recently created body, is index 0). Joint/spring/
+ *     manifold body references are body-list indices; -1 is the world anchor
+ *     (a joint whose bodyA is null, joint.cpp:28).
+ *   - Forces carry a creation serial. A body's constraint order (the order in
+ *     which the reference accumulates its Hessian, force.cpp:25-36 prepends) is
+ *     descending serial.
+ *   - One handle drives one HIP stream. Calls on one handle must be serialised
+ *     by the caller; distinct handles may run concurrently.
+ */
+#ifndef AVBD_HIP_H
+#define AVBD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AVBD_MAX_CONTACTS 8
+
+enum avbd_status {
+    AVBD_OK = 0,
+    AVBD_ERR_INVALID_ARGUMENT = -1,
+    AVBD_ERR_CAPACITY = -2,      /* a buffer sized at avbd_create would overflow */
+    AVBD_ERR_HIP = -3,           /* a HIP runtime call failed */
+    AVBD_ERR_NO_DEVICE = -4,
+    AVBD_ERR_UNSUPPORTED = -5,   /* shape or feature outside the box-only step */
+    AVBD_ERR_INJECTED = -6       /* avbd_debug_inject_fault armed this failure */
+};
+
+enum avbd_shape { AVBD_SHAPE_BOX = 0 };
+
+/* Solver parameters: Solver::dt/gravity/iterations/alpha/betaLin/betaAng/gamma
+ * (solver.h:348-355), defaults in Solver::defaultParams (solver.cpp:909-932).
+ * Gravity acts along -z (the reference is Z-up, solver.cpp:1141). */
+typedef struct avbd_params {
+    float dt;
+    float gravity;
+    int32_t iterations;
+    float alpha;
+    float beta_lin;
+    float beta_ang;
+    float gamma;
+} avbd_params;
+
+/* One rigid body: the Rigid fields the step reads or writes (solver.h:197-228).
+ * Quaternions are (x, y, z, w). */
+typedef struct avbd_body {
+    float size[3];          /* full box widths */
+    float mass;             /* <= 0: static */
+    float moment[3];        /* principal moments of inertia */
+    float friction;
+    float radius;           /* bounding-sphere radius */
+    float pos[3];           /* positionLin */
+    float rot[4];           /* positionAng */
+    float initial_lin[3];   /* x- (solver.cpp:1151-1152) */
+    float initial_ang[4];
+    float inertial_lin[3];  /* x~ (solver.cpp:1138-1141) */
+    float inertial_ang[4];
+    float vel[3];           /* velocityLin */
+    float ang_vel[3];       /* velocityAng */
+    float prev_vel[3];      /* prevVelocityLin */
+    int32_t shape;          /* enum avbd_shape */
+    int32_t dense_id;       /* caller's id, carried through unchanged */
+} avbd_body;
+
+/* One contact point of a manifold: Manifold::Contact (solver.h:301-310). */
+typedef struct avbd_contact {
+    int32_t feature;        /* FeaturePair::key */
+    float r_a[3];           /* contact offset in A's local frame */
+    float r_b[3];           /* contact offset in B's local frame */
+    float c0[3];            /* constraint value at x- plus margin */
+    float penalty[3];
+    float lambda[3];
+    int32_t stick;
+} avbd_contact;
+
+/* A box-box contact manifold: Manifold (solver.h:284-318). */
+typedef struct avbd_manifold {
+    int32_t body_a;
+    int32_t body_b;
+    int64_t serial;
+    int32_t num_contacts;
+    float friction;
+    float basis[9];         /* row 0 normal (B to A), rows 1-2 tangents */
+    avbd_contact contacts[AVBD_MAX_CONTACTS];
+} avbd_manifold;
+
+/* Ball-socket joint with optional angular row and fracture: Joint (solver.h:251-264). */
+typedef struct avbd_joint {
+    int32_t body_a;         /* -1: anchored to the world point r_a */
+    int32_t body_b;
+    int64_t serial;
+    float r_a[3];
+    float r_b[3];
+    float c0_lin[3];
+    float c0_ang[3];
+    float penalty_lin[3];
+    float penalty_ang[3];
+    float lambda_lin[3];
+    float lambda_ang[3];
+    float stiffness_lin;    /* INFINITY: hard constraint */
+    float stiffness_ang;
+    float fracture;
+    float torque_arm;
+    int32_t broken;
+    int32_t pad_;
+} avbd_joint;
+
+/* Penalty spring: Spring (solver.h:267-278). */
+typedef struct avbd_spring {
+    int32_t body_a;
+    int32_t body_b;
+    int64_t serial;
+    float r_a[3];
+    float r_b[3];
+    float rest;
+    float stiffness;
+} avbd_spring;
+
+/* Capacities of the device buffers; fixed for the handle's lifetime. */
+typedef struct avbd_config {
+    avbd_params params;
+    int32_t max_bodies;
+    int32_t max_joints;
+    int32_t max_springs;
+    int32_t max_manifolds;   /* persistent manifolds + new candidates in one step */
+    int32_t max_pairs;       /* bounding-sphere hits per broadphase */
+    int32_t device;          /* HIP device ordinal */
+    int32_t enable_timing;   /* record per-phase hipEvent timings in avbd_stats */
+} avbd_config;
+
+/* Per-step counters and timings of the last avbd_step call's final step:
+ * the subset of SolverStats (solver.h:121-188) and AvbdGpuSolverStats
+ * (avbd_gpu_solver.h:19-39) that the resident GPU step produces. */
+typedef struct avbd_stats {
+    int32_t body_count;
+    int32_t active_body_count;
+    int32_t solved_body_count;
+    int32_t pair_checks;
+    int32_t sphere_hits;
+    int32_t constrained_hits;
+    int32_t manifolds_created;
+    int32_t manifold_count;
+    int32_t contact_count;
+    int32_t joint_count;
+    int32_t spring_count;
+    int32_t color_count;
+    int32_t coloring_rounds;
+    int32_t sweep_axis;
+    int32_t axis_fallbacks;      /* steps whose sweep axis needed the exact serial pass */
+    int32_t cpu_fallbacks;       /* set by host adapters that fell back to the CPU */
+    int64_t steps_taken;
+    float broadphase_ms;
+    float narrowphase_ms;
+    float coloring_ms;
+    float iterate_ms;
+    float finish_ms;
+    float step_ms;
+} avbd_stats;
+
+typedef struct avbd_handle avbd_handle;
+
+/* Thread-local description of the last failure on this thread ("" if none). */
+const char *avbd_last_error(void);
+
+/* Solver::defaultParams (solver.cpp:909-932), with the given gravity. */
+int avbd_default_params(avbd_params *out);
+
+/* Host-side record builders: the mass properties and warm-start defaults the
+ * reference constructors compute.
+ *   avbd_make_box_body  <- Rigid::Rigid   (rigid.cpp:24-42)
+ *   avbd_make_joint     <- Joint::Joint   (joint.cpp:25-33)
+ *   avbd_make_spring    <- Spring::Spring (spring.cpp:14-25)
+ * Bodies referenced by a joint/spring are passed by pointer (NULL = world). */
+int avbd_make_box_body(const float size[3], float density, float friction,
+                       const float pos[3], const float rot[4], const float vel[3],
+                       const float ang_vel[3], avbd_body *out);
+int avbd_make_joint(const avbd_body *body_a, const avbd_body *body_b,
+                    int32_t index_a, int32_t index_b, int64_t serial,
+                    const float r_a[3], const float r_b[3], float stiffness_lin,
+                    float stiffness_ang, float fracture, avbd_joint *out);
+int avbd_make_spring(const avbd_body *body_a, const avbd_body *body_b,
+                     int32_t index_a, int32_t index_b, int64_t serial,
+                     const float r_a[3], const float r_b[3], float stiffness,
+                     float rest, avbd_spring *out);
+
+/* Handle lifetime. The replacement for constructing a PhysicsBackend. */
+int avbd_create(const avbd_config *config, avbd_handle **out);
+int avbd_destroy(avbd_handle *h);
+int avbd_set_params(avbd_handle *h, const avbd_params *params);
+
+/* Replace the whole resident state (bodies in list order). */
+int avbd_upload_state(avbd_handle *h,
+                      const avbd_body *bodies, int32_t num_bodies,
+                      const avbd_joint *joints, int32_t num_joints,
+                      const avbd_spring *springs, int32_t num_springs,
+                      const avbd_manifold *manifolds, int32_t num_manifolds,
+                      int64_t next_serial);
+int avbd_state_counts(avbd_handle *h, int32_t *num_bodies, int32_t *num_joints,
+                      int32_t *num_springs, int32_t *num_manifolds, int64_t *next_serial);
+/* Any output pointer may be NULL to skip that table. Manifolds are returned in
+ * ascending serial order (the reverse of the solver's force-list order). */
+int avbd_download_state(avbd_handle *h, avbd_body *bodies, avbd_joint *joints,
+                        avbd_spring *springs, avbd_manifold *manifolds);
+
+/* Full resident step(s): Solver::stepCpuReference (solver.cpp:1036-1041) with
+ * sweep-and-prune broadphase (solver.cpp:488-577), box-box manifolds
+ * (manifold.cpp:100-174, collide.cpp:898-955), greedy body coloring
+ * (avbd_gpu_solver.cpp:1812-1865), colored primal/dual iterations
+ * (solver.cpp:1172-1273) and BDF1 velocities (solver.cpp:1275-1298).
+ * Asynchronous on the handle's stream unless avbd_synchronize is called. */
+int avbd_step(avbd_handle *h, int32_t num_steps);
+int avbd_synchronize(avbd_handle *h);
+
+/* Phased form, as Solver::prepareStep / iteratePrimalDualCpu / finishStep
+ * (solver.h:394-401). avbd_step == prepare + iterate + finish. */
+int avbd_prepare(avbd_handle *h);
+int avbd_iterate(avbd_handle *h);
+int avbd_finish(avbd_handle *h);
+
+/* Introspection of the last step (for parity tests and the host adapter). */
+int avbd_get_stats(avbd_handle *h, avbd_stats *out);
+/* Per body (list order): color of the last step, -1 for unsolved bodies. */
+int avbd_get_colors(avbd_handle *h, int32_t *colors, int32_t capacity);
+/* Bounding-sphere hits of the last broadphase, in sweep emission order, as
+ * (index_a, index_b, created) triples; created = 1 when a new manifold was made. */
+int avbd_get_hits(avbd_handle *h, int32_t *triples, int32_t capacity, int32_t *count);
+
+/* Test hook: make the next avbd_step on this handle fail with AVBD_ERR_INJECTED
+ * before any work is enqueued (code 1), or clear it (code 0). */
+int avbd_debug_inject_fault(avbd_handle *h, int32_t code);
+
+/* sizeof of the ABI records (0 body, 1 contact, 2 manifold, 3 joint, 4 spring,
+ * 5 params), so bindings can verify their layouts; -1 for other values. */
+int avbd_sizeof(int which);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AVBD_HIP_H */

@@ -1,0 +1,1201 @@
+/*
+ * avbd_oracle.hpp -- CPU oracle for the AVBD rigid-body step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This restatement is the checker the HIP path is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may load it. The product path never calls it.
+ *
+ * It restates, in its own code, the reference CPU solver of ts-jls/avbd-demo3d:
+ *   maths.h           vector/quaternion algebra (maths.h:255-277, 431-479), 6x6 LDL^T (maths.h:481-547)
+ *   rigid.cpp         box mass properties (rigid.cpp:24-42)
+ *   joint.cpp         ball-socket joint init/primal/dual (joint.cpp:14-182)
+ *   spring.cpp        penalty spring primal (spring.cpp:14-69)
+ *   manifold.cpp      contact merge, warm start, primal, dual (manifold.cpp:100-280)
+ *   collide.cpp       box-box SAT + clipping manifolds (collide.cpp:18-433, 898-955)
+ *   solver.cpp        sweep-and-prune broadphase (solver.cpp:488-577), hit classification
+ *                     (solver.cpp:131-164, 202-248), prepareStep (solver.cpp:1043-1170),
+ *                     iteratePrimalDualCpu (solver.cpp:1172-1273), finishStep (solver.cpp:1275-1298)
+ *   avbd_gpu_solver.cpp  greedy body coloring (avbd_gpu_solver.cpp:1812-1865)
+ *
+ * The scalar type is a template parameter: Real = float reproduces the
+ * reference's arithmetic operation by operation (build with -ffp-contract=off),
+ * Real = double gives the FP64 run used to state each FP32 result's error.
+ *
+ * Iteration order: mode ORDER_COLOR visits solved bodies color by color (list
+ * order inside a color), which is what a colored GPU sweep computes; mode
+ * ORDER_LIST visits them in body-list order exactly like solver.cpp:1181.
+ *
+ * Pinning (see tests/test_oracle_*.py): every function here is checked either
+ * bit-exactly against golden vectors written by the compiled reference
+ * (oracle/_ref) or against closed forms/invariants.
+ */
+#pragma once
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace avbd_oracle
+{
+
+template <class R>
+struct V3
+{
+    R x, y, z;
+    R &operator[](int i) { return (&x)[i]; }
+    const R &operator[](int i) const { return (&x)[i]; }
+};
+
+template <class R>
+struct Q
+{
+    R x, y, z, w;
+};
+
+template <class R>
+struct M3
+{
+    V3<R> r[3];
+    V3<R> &operator[](int i) { return r[i]; }
+    const V3<R> &operator[](int i) const { return r[i]; }
+    V3<R> col(int i) const { return V3<R>{r[0][i], r[1][i], r[2][i]}; }
+};
+
+// ---- scalar helpers (maths.h:281-315) ----
+template <class R> inline R fmin_(R a, R b) { return a < b ? a : b; }
+template <class R> inline R fmax_(R a, R b) { return a > b ? a : b; }
+template <class R> inline R clampv(R x, R a, R b) { return fmax_(a, fmin_(b, x)); }
+template <class R> inline R signv(R x) { return x < 0 ? R(-1) : x > 0 ? R(1) : R(0); }
+inline float sqrt_(float v) { return sqrtf(v); }
+inline double sqrt_(double v) { return std::sqrt(v); }
+inline float abs_(float v) { return fabsf(v); }
+inline double abs_(double v) { return std::fabs(v); }
+
+// ---- vector algebra (maths.h:117-165, 317-406) ----
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <class R> inline V3<R> operator*(V3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class R> inline V3<R> operator/(V3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class R> inline V3<R> &operator+=(V3<R> &a, V3<R> b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+template <class R> inline V3<R> &operator-=(V3<R> &a, V3<R> b) { a.x -= b.x; a.y -= b.y; a.z -= b.z; return a; }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline R lengthSq(V3<R> a) { return dot(a, a); }
+template <class R> inline R length(V3<R> a) { return sqrt_(lengthSq(a)); }
+template <class R> inline V3<R> normalize(V3<R> a) { return a / length(a); }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> inline V3<R> absv(V3<R> a) { return {abs_(a.x), abs_(a.y), abs_(a.z)}; }
+template <class R> inline V3<R> minv(V3<R> a, R b) { return {fmin_(a.x, b), fmin_(a.y, b), fmin_(a.z, b)}; }
+template <class R> inline V3<R> clampv(V3<R> v, R a, R b) { return {clampv(v.x, a, b), clampv(v.y, a, b), clampv(v.z, a, b)}; }
+
+// ---- 3x3 algebra (maths.h:196-237, 380-429, 476-479) ----
+template <class R> inline V3<R> operator*(const M3<R> &a, V3<R> b) { return {dot(a[0], b), dot(a[1], b), dot(a[2], b)}; }
+template <class R> inline M3<R> operator*(const M3<R> &a, const M3<R> &b)
+{
+    M3<R> o;
+    for (int i = 0; i < 3; ++i)
+        o.r[i] = V3<R>{dot(a.r[i], b.col(0)), dot(a.r[i], b.col(1)), dot(a.r[i], b.col(2))};
+    return o;
+}
+template <class R> inline M3<R> operator*(const M3<R> &a, R s) { return {a[0] * s, a[1] * s, a[2] * s}; }
+template <class R> inline M3<R> operator/(const M3<R> &a, R s) { return {a[0] / s, a[1] / s, a[2] / s}; }
+template <class R> inline M3<R> operator+(const M3<R> &a, const M3<R> &b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+template <class R> inline M3<R> operator-(const M3<R> &a) { return {-a[0], -a[1], -a[2]}; }
+template <class R> inline M3<R> &operator+=(M3<R> &a, const M3<R> &b) { a[0] += b[0]; a[1] += b[1]; a[2] += b[2]; return a; }
+template <class R> inline M3<R> transpose(const M3<R> &a) { return {a.col(0), a.col(1), a.col(2)}; }
+template <class R> inline M3<R> diag(R a, R b, R c) { return {V3<R>{a, 0, 0}, V3<R>{0, b, 0}, V3<R>{0, 0, c}}; }
+template <class R> inline M3<R> ident(R s) { return diag<R>(s, s, s); }
+template <class R> inline M3<R> skew(V3<R> r) { return {V3<R>{0, -r.z, r.y}, V3<R>{r.z, 0, -r.x}, V3<R>{-r.y, r.x, 0}}; }
+template <class R> inline M3<R> outer(V3<R> a, V3<R> b) { return {b * a.x, b * a.y, b * a.z}; }
+template <class R> inline M3<R> diagonalize(const M3<R> &m) { return diag<R>(length(m.col(0)), length(m.col(1)), length(m.col(2))); }
+
+// ---- quaternions (maths.h:239-277, 431-474) ----
+template <class R> inline Q<R> operator*(Q<R> a, R s) { return {a.x * s, a.y * s, a.z * s, a.w * s}; }
+template <class R> inline Q<R> operator/(Q<R> a, R s) { return {a.x / s, a.y / s, a.z / s, a.w / s}; }
+template <class R> inline Q<R> operator+(Q<R> a, Q<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+template <class R> inline Q<R> operator*(Q<R> a, Q<R> b)
+{
+    return {a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+            a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+            a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w,
+            a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+template <class R> inline R lengthSq(Q<R> q) { return lengthSq(V3<R>{q.x, q.y, q.z}) + q.w * q.w; }
+template <class R> inline Q<R> normalize(Q<R> q) { return q / sqrt_(lengthSq(q)); }
+template <class R> inline Q<R> conjugate(Q<R> q) { return {-q.x, -q.y, -q.z, q.w}; }
+template <class R> inline Q<R> inverse(Q<R> q) { return conjugate(q) / lengthSq(q); }
+// q - p = 2 vec(q p^-1)
+template <class R> inline V3<R> operator-(Q<R> a, Q<R> b)
+{
+    Q<R> d = a * inverse(b);
+    return V3<R>{d.x, d.y, d.z} * R(2);
+}
+// q + w = normalize(q + 1/2 (w,0) q)
+template <class R> inline Q<R> operator+(Q<R> a, V3<R> b)
+{
+    return normalize(a + (Q<R>{b.x, b.y, b.z, 0} * a) * R(0.5));
+}
+template <class R> inline V3<R> rotate(Q<R> q, V3<R> v)
+{
+    V3<R> u{q.x, q.y, q.z};
+    V3<R> t = cross(u, v) * R(2);
+    return v + t * q.w + cross(u, t);
+}
+template <class R> inline V3<R> transform(V3<R> lin, Q<R> ang, V3<R> v) { return rotate(ang, v) + lin; }
+template <class R> inline M3<R> orthonormal(V3<R> n)
+{
+    V3<R> t1 = abs_(n.x) > abs_(n.z) ? V3<R>{-n.y, n.x, 0} : V3<R>{0, -n.z, n.y};
+    t1 = normalize(t1);
+    V3<R> t2 = cross(n, t1);
+    return {n, t1, t2};
+}
+
+// ---- 6x6 LDL^T solve of [[Lin, Cross^T],[Cross, Ang]] x = b (maths.h:481-547) ----
+template <class R>
+inline void solve6(const M3<R> &aLin, const M3<R> &aAng, const M3<R> &aCross, V3<R> bLin, V3<R> bAng, V3<R> &xLin, V3<R> &xAng)
+{
+    R A11 = aLin[0][0];
+    R A21 = aLin[1][0], A22 = aLin[1][1];
+    R A31 = aLin[2][0], A32 = aLin[2][1], A33 = aLin[2][2];
+    R A41 = aCross[0][0], A42 = aCross[0][1], A43 = aCross[0][2], A44 = aAng[0][0];
+    R A51 = aCross[1][0], A52 = aCross[1][1], A53 = aCross[1][2], A54 = aAng[1][0], A55 = aAng[1][1];
+    R A61 = aCross[2][0], A62 = aCross[2][1], A63 = aCross[2][2], A64 = aAng[2][0], A65 = aAng[2][1], A66 = aAng[2][2];
+
+    R D1 = A11;
+    R L21 = A21 / D1, L31 = A31 / D1, L41 = A41 / D1, L51 = A51 / D1, L61 = A61 / D1;
+    R D2 = A22 - L21 * L21 * D1;
+    R L32 = (A32 - L21 * L31 * D1) / D2;
+    R L42 = (A42 - L21 * L41 * D1) / D2;
+    R L52 = (A52 - L21 * L51 * D1) / D2;
+    R L62 = (A62 - L21 * L61 * D1) / D2;
+    R D3 = A33 - (L31 * L31 * D1 + L32 * L32 * D2);
+    R L43 = (A43 - L31 * L41 * D1 - L32 * L42 * D2) / D3;
+    R L53 = (A53 - L31 * L51 * D1 - L32 * L52 * D2) / D3;
+    R L63 = (A63 - L31 * L61 * D1 - L32 * L62 * D2) / D3;
+    R D4 = A44 - (L41 * L41 * D1 + L42 * L42 * D2 + L43 * L43 * D3);
+    R L54 = (A54 - L41 * L51 * D1 - L42 * L52 * D2 - L43 * L53 * D3) / D4;
+    R L64 = (A64 - L41 * L61 * D1 - L42 * L62 * D2 - L43 * L63 * D3) / D4;
+    R D5 = A55 - (L51 * L51 * D1 + L52 * L52 * D2 + L53 * L53 * D3 + L54 * L54 * D4);
+    R L65 = (A65 - L51 * L61 * D1 - L52 * L62 * D2 - L53 * L63 * D3 - L54 * L64 * D4) / D5;
+    R D6 = A66 - (L61 * L61 * D1 + L62 * L62 * D2 + L63 * L63 * D3 + L64 * L64 * D4 + L65 * L65 * D5);
+
+    R y1 = bLin[0];
+    R y2 = bLin[1] - L21 * y1;
+    R y3 = bLin[2] - L31 * y1 - L32 * y2;
+    R y4 = bAng[0] - L41 * y1 - L42 * y2 - L43 * y3;
+    R y5 = bAng[1] - L51 * y1 - L52 * y2 - L53 * y3 - L54 * y4;
+    R y6 = bAng[2] - L61 * y1 - L62 * y2 - L63 * y3 - L64 * y4 - L65 * y5;
+
+    R z1 = y1 / D1, z2 = y2 / D2, z3 = y3 / D3, z4 = y4 / D4, z5 = y5 / D5, z6 = y6 / D6;
+
+    xAng[2] = z6;
+    xAng[1] = z5 - L65 * xAng[2];
+    xAng[0] = z4 - L54 * xAng[1] - L64 * xAng[2];
+    xLin[2] = z3 - L43 * xAng[0] - L53 * xAng[1] - L63 * xAng[2];
+    xLin[1] = z2 - L32 * xLin[2] - L42 * xAng[0] - L52 * xAng[1] - L62 * xAng[2];
+    xLin[0] = z1 - L21 * xLin[1] - L31 * xLin[2] - L41 * xAng[0] - L51 * xAng[1] - L61 * xAng[2];
+}
+
+// ---- constants (solver.h:21-24, collide.cpp:18-22) ----
+template <class R> struct K
+{
+    static constexpr R penaltyMin = R(1.0f);
+    static constexpr R penaltyMax = R(10000000000.0f);
+    static constexpr R margin = R(0.01f);
+    static constexpr R stickThresh = R(0.00001f);
+    static constexpr R satEps = R(1.0e-6f);
+    static constexpr R planeEps = R(1.0e-5f);
+    static constexpr R mergeDistSq = R(1.0e-6f);
+};
+
+// ---- state ----
+template <class R>
+struct Body
+{
+    V3<R> size;
+    R mass;
+    V3<R> moment;
+    R friction;
+    R radius;
+    V3<R> pos;
+    Q<R> rot;
+    V3<R> initLin;
+    Q<R> initAng;
+    V3<R> inerLin;
+    Q<R> inerAng;
+    V3<R> vel;
+    V3<R> angVel;
+    V3<R> prevVel;
+    int shape;
+    int denseId;
+};
+
+template <class R>
+struct Contact
+{
+    int32_t feature;
+    V3<R> rA, rB, C0, penalty, lambda;
+    bool stick;
+};
+
+template <class R>
+struct Manifold
+{
+    int a, b;
+    int64_t serial;
+    int n;
+    R friction;
+    M3<R> basis;
+    Contact<R> c[8];
+};
+
+template <class R>
+struct Joint
+{
+    int a, b; // a == -1: world anchor
+    int64_t serial;
+    V3<R> rA, rB, C0Lin, C0Ang, penLin, penAng, lamLin, lamAng;
+    R kLin, kAng, fracture, torqueArm;
+    bool broken;
+};
+
+template <class R>
+struct Spring
+{
+    int a, b;
+    int64_t serial;
+    V3<R> rA, rB;
+    R rest, k;
+};
+
+template <class R>
+struct Params
+{
+    R dt, gravity;
+    int iterations;
+    R alpha, betaLin, betaAng, gamma;
+};
+
+enum ForceKind
+{
+    KIND_MANIFOLD = 0,
+    KIND_JOINT = 1,
+    KIND_SPRING = 2
+};
+
+struct ForceRef
+{
+    int kind;
+    int index;
+    int64_t serial;
+};
+
+struct Hit
+{
+    int a, b;
+    int created;
+};
+
+enum OrderMode
+{
+    ORDER_COLOR = 0,
+    ORDER_LIST = 1
+};
+
+// ---- box-box narrowphase (collide.cpp:18-433, 898-955) ----
+template <class R>
+struct Obb
+{
+    V3<R> c;
+    V3<R> h;
+    V3<R> ax[3];
+};
+
+template <class R>
+inline Obb<R> makeObb(const Body<R> &b)
+{
+    Obb<R> o;
+    o.c = b.pos;
+    o.h = b.size * R(0.5);
+    o.ax[0] = rotate(b.rot, V3<R>{1, 0, 0});
+    o.ax[1] = rotate(b.rot, V3<R>{0, 1, 0});
+    o.ax[2] = rotate(b.rot, V3<R>{0, 0, 1});
+    return o;
+}
+
+template <class R>
+inline V3<R> supportPt(const Obb<R> &o, V3<R> d)
+{
+    R sx = dot(d, o.ax[0]) >= 0 ? R(1) : R(-1);
+    R sy = dot(d, o.ax[1]) >= 0 ? R(1) : R(-1);
+    R sz = dot(d, o.ax[2]) >= 0 ? R(1) : R(-1);
+    return o.c + o.ax[0] * (o.h.x * sx) + o.ax[1] * (o.h.y * sy) + o.ax[2] * (o.h.z * sz);
+}
+
+template <class R>
+inline void faceAxes(const Obb<R> &o, int k, V3<R> &u, V3<R> &v, R &eu, R &ev)
+{
+    int iu = k == 0 ? 1 : 0;
+    int iv = k == 2 ? 1 : 2;
+    u = o.ax[iu];
+    v = o.ax[iv];
+    eu = o.h[iu];
+    ev = o.h[iv];
+}
+
+template <class R>
+inline int clipPlane(const V3<R> *in, int n, V3<R> pn, R off, V3<R> *out)
+{
+    if (n <= 0)
+        return 0;
+    int m = 0;
+    V3<R> a = in[n - 1];
+    R da = dot(pn, a) - off;
+    for (int i = 0; i < n; ++i)
+    {
+        V3<R> b = in[i];
+        R db = dot(pn, b) - off;
+        bool ain = da <= K<R>::planeEps;
+        bool bin = db <= K<R>::planeEps;
+        if (ain != bin)
+        {
+            R t = 0;
+            R den = da - db;
+            if (abs_(den) > K<R>::satEps)
+                t = clampv(da / den, R(0), R(1));
+            if (m < 16)
+                out[m++] = a + (b - a) * t;
+        }
+        if (bin && m < 16)
+            out[m++] = b;
+        a = b;
+        da = db;
+    }
+    return m;
+}
+
+template <class R>
+struct ContactBuilder
+{
+    const Body<R> &A;
+    const Body<R> &B;
+    Contact<R> *out;
+    int count = 0;
+    V3<R> mids[8];
+
+    ContactBuilder(const Body<R> &a, const Body<R> &b, Contact<R> *o) : A(a), B(b), out(o) {}
+
+    void add(V3<R> xA, V3<R> xB, int32_t key)
+    {
+        V3<R> mid = (xA + xB) * R(0.5);
+        for (int i = 0; i < count; ++i)
+            if (lengthSq(mid - mids[i]) < K<R>::mergeDistSq)
+                return;
+        if (count >= 8)
+            return;
+        Contact<R> &c = out[count];
+        c.feature = key;
+        c.rA = rotate(conjugate(A.rot), xA - A.pos);
+        c.rB = rotate(conjugate(B.rot), xB - B.pos);
+        mids[count] = mid;
+        ++count;
+    }
+};
+
+template <class R>
+struct SatBest
+{
+    int type; // 0 face A, 1 face B, 2 edge
+    int ia, ib;
+    R sep;
+    V3<R> n;
+    bool valid;
+};
+
+// Returns false when the axis separates the boxes.
+template <class R>
+inline bool satAxis(const Obb<R> &A, const Obb<R> &B, V3<R> delta, V3<R> axis, int type, int ia, int ib, SatBest<R> &best)
+{
+    R l2 = lengthSq(axis);
+    if (l2 < K<R>::satEps)
+        return true;
+    R inv = R(1) / sqrt_(l2);
+    V3<R> n = axis * inv;
+    if (dot(n, delta) < 0)
+        n = -n;
+    R dist = abs_(dot(delta, n));
+    R rA = A.h.x * abs_(dot(n, A.ax[0])) + A.h.y * abs_(dot(n, A.ax[1])) + A.h.z * abs_(dot(n, A.ax[2]));
+    R rB = B.h.x * abs_(dot(n, B.ax[0])) + B.h.y * abs_(dot(n, B.ax[1])) + B.h.z * abs_(dot(n, B.ax[2]));
+    R sep = dist - (rA + rB);
+    if (sep > 0)
+        return false;
+    if (!best.valid || sep > best.sep)
+    {
+        best.valid = true;
+        best.type = type;
+        best.ia = ia;
+        best.ib = ib;
+        best.sep = sep;
+        best.n = n;
+    }
+    return true;
+}
+
+template <class R>
+inline void closestSegPts(V3<R> p0, V3<R> p1, V3<R> q0, V3<R> q1, V3<R> &c0, V3<R> &c1)
+{
+    V3<R> d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
+    R a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+    R s = 0, t = 0;
+    if (a <= K<R>::satEps && e <= K<R>::satEps)
+    {
+        c0 = p0;
+        c1 = q0;
+        return;
+    }
+    if (a <= K<R>::satEps)
+    {
+        t = clampv(f / e, R(0), R(1));
+    }
+    else
+    {
+        R c = dot(d1, r);
+        if (e <= K<R>::satEps)
+        {
+            s = clampv(-c / a, R(0), R(1));
+        }
+        else
+        {
+            R b = dot(d1, d2);
+            R den = a * e - b * b;
+            if (abs_(den) > K<R>::satEps)
+                s = clampv((b * f - c * e) / den, R(0), R(1));
+            t = (b * s + f) / e;
+            if (t < 0)
+            {
+                t = 0;
+                s = clampv(-c / a, R(0), R(1));
+            }
+            else if (t > 1)
+            {
+                t = 1;
+                s = clampv((b - c) / a, R(0), R(1));
+            }
+        }
+    }
+    c0 = p0 + d1 * s;
+    c1 = q0 + d2 * t;
+}
+
+template <class R>
+inline void edgeSupport(const Obb<R> &o, int k, V3<R> d, V3<R> &e0, V3<R> &e1)
+{
+    int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+    R s1 = dot(d, o.ax[k1]) >= 0 ? R(1) : R(-1);
+    R s2 = dot(d, o.ax[k2]) >= 0 ? R(1) : R(-1);
+    V3<R> mid = o.c + o.ax[k1] * (o.h[k1] * s1) + o.ax[k2] * (o.h[k2] * s2);
+    e0 = mid - o.ax[k] * o.h[k];
+    e1 = mid + o.ax[k] * o.h[k];
+}
+
+template <class R>
+inline int faceManifold(ContactBuilder<R> &cb, const Obb<R> &A, const Obb<R> &B, bool refA, int refAxis, V3<R> nAB)
+{
+    const Obb<R> &ref = refA ? A : B;
+    const Obb<R> &inc = refA ? B : A;
+    V3<R> outward = refA ? nAB : -nAB;
+
+    // reference face frame (collide.cpp:110-117)
+    R fs = dot(outward, ref.ax[refAxis]) >= 0 ? R(1) : R(-1);
+    V3<R> fn = ref.ax[refAxis] * fs;
+    V3<R> fc = ref.c + fn * ref.h[refAxis];
+    V3<R> fu, fv;
+    R eu, ev;
+    faceAxes(ref, refAxis, fu, fv, eu, ev);
+
+    // incident face: most anti-parallel axis (collide.cpp:119-135)
+    int incAxis = 0;
+    R best = -FLT_MAX;
+    for (int i = 0; i < 3; ++i)
+    {
+        R d = abs_(dot(inc.ax[i], fn));
+        if (d > best)
+        {
+            best = d;
+            incAxis = i;
+        }
+    }
+    V3<R> p0[16], p1[16];
+    {
+        R s = dot(inc.ax[incAxis], fn) > 0 ? R(-1) : R(1);
+        V3<R> n = inc.ax[incAxis] * s;
+        V3<R> c = inc.c + n * inc.h[incAxis];
+        V3<R> u, v;
+        R ueu, vev;
+        faceAxes(inc, incAxis, u, v, ueu, vev);
+        p0[0] = c + u * ueu + v * vev;
+        p0[1] = c - u * ueu + v * vev;
+        p0[2] = c - u * ueu - v * vev;
+        p0[3] = c + u * ueu - v * vev;
+    }
+    int n = 4;
+    n = clipPlane(p0, n, fu, dot(fu, fc) + eu, p1);
+    if (!n)
+        return 0;
+    V3<R> nu = -fu;
+    n = clipPlane(p1, n, nu, dot(nu, fc) + eu, p0);
+    if (!n)
+        return 0;
+    n = clipPlane(p0, n, fv, dot(fv, fc) + ev, p1);
+    if (!n)
+        return 0;
+    V3<R> nv = -fv;
+    n = clipPlane(p1, n, nv, dot(nv, fc) + ev, p0);
+    if (!n)
+        return 0;
+
+    int32_t prefix = ((refA ? 0 : 1) << 24) | ((refAxis & 0xFF) << 16) | ((incAxis & 0xFF) << 8);
+    for (int i = 0; i < n && cb.count < 8; ++i)
+    {
+        V3<R> pi = p0[i];
+        R depth = dot(pi - fc, fn);
+        if (depth > K<R>::planeEps)
+            continue;
+        V3<R> pr = pi - fn * depth;
+        cb.add(refA ? pr : pi, refA ? pi : pr, prefix | (i & 0xFF));
+    }
+    if (!cb.count)
+        cb.add(supportPt(A, nAB), supportPt(B, -nAB), prefix);
+    return cb.count;
+}
+
+template <class R>
+inline int collideBoxes(const Body<R> &bA, const Body<R> &bB, Contact<R> *out, M3<R> &basis)
+{
+    Obb<R> A = makeObb(bA), B = makeObb(bB);
+    V3<R> delta = B.c - A.c;
+    SatBest<R> face{}, edge{};
+    face.sep = -FLT_MAX;
+    face.valid = false;
+    edge.sep = -FLT_MAX;
+    edge.valid = false;
+    for (int i = 0; i < 3; ++i)
+        if (!satAxis(A, B, delta, A.ax[i], 0, i, -1, face))
+            return 0;
+    for (int i = 0; i < 3; ++i)
+        if (!satAxis(A, B, delta, B.ax[i], 1, -1, i, face))
+            return 0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (!satAxis(A, B, delta, cross(A.ax[i], B.ax[j]), 2, i, j, edge))
+                return 0;
+    if (!face.valid)
+        return 0;
+    SatBest<R> best = face;
+    if (edge.valid && R(0.95f) * edge.sep > face.sep + R(0.01f))
+        best = edge;
+    basis = orthonormal(-best.n);
+
+    ContactBuilder<R> cb(bA, bB, out);
+    if (best.type == 2)
+    {
+        V3<R> a0, a1, b0, b1, xA, xB;
+        edgeSupport(A, best.ia, best.n, a0, a1);
+        edgeSupport(B, best.ib, -best.n, b0, b1);
+        closestSegPts(a0, a1, b0, b1, xA, xB);
+        int32_t key = (2 << 24) | ((best.ia & 0xFF) << 8) | (best.ib & 0xFF);
+        cb.add(xA, xB, key);
+        if (!cb.count)
+            cb.add(supportPt(A, best.n), supportPt(B, -best.n), key);
+        return cb.count;
+    }
+    if (best.type == 0)
+        return faceManifold(cb, A, B, true, best.ia, best.n);
+    return faceManifold(cb, A, B, false, best.ib, best.n);
+}
+
+// ---- the world and its step ----
+template <class R>
+struct World
+{
+    Params<R> p;
+    std::vector<Body<R>> bodies;        // body-list order
+    std::vector<Joint<R>> joints;       // ascending serial
+    std::vector<Spring<R>> springs;     // ascending serial
+    std::vector<Manifold<R>> manifolds; // ascending serial
+    int64_t nextSerial = 0;
+    int orderMode = ORDER_COLOR;
+
+    // introspection of the last step
+    std::vector<int> colors;
+    std::vector<Hit> hits;
+    int sweepAxis = 0;
+    int pairChecks = 0;
+    int colorCount = 0;
+    std::vector<std::vector<ForceRef>> bodyForces; // descending serial
+
+    bool solved(int i) const { return bodies[i].mass > 0 && !bodyForces[i].empty(); }
+
+    // Existing force between two bodies (Rigid::constrainedTo, rigid.cpp:105-116)
+    bool constrained(int i, int j, const std::vector<std::vector<int>> &partners) const
+    {
+        const std::vector<int> &pi = partners[i];
+        return std::find(pi.begin(), pi.end(), j) != pi.end();
+    }
+
+    void partnerLists(std::vector<std::vector<int>> &partners) const
+    {
+        partners.assign(bodies.size(), {});
+        auto add = [&](int a, int b) {
+            if (a >= 0)
+                partners[a].push_back(b);
+            if (b >= 0)
+                partners[b].push_back(a);
+        };
+        for (const auto &m : manifolds)
+            add(m.a, m.b);
+        for (const auto &j : joints)
+            add(j.a, j.b);
+        for (const auto &s : springs)
+            add(s.a, s.b);
+    }
+
+    // Sweep and prune (solver.cpp:488-577) followed by the serial, ordered apply
+    // of new manifolds (solver.cpp:220-248).
+    void broadphase()
+    {
+        hits.clear();
+        pairChecks = 0;
+        size_t n = bodies.size();
+        if (n < 2)
+            return;
+        V3<R> mean{0, 0, 0};
+        for (const auto &b : bodies)
+            mean += b.pos;
+        mean = mean / (R)n;
+        V3<R> var{0, 0, 0};
+        for (const auto &b : bodies)
+        {
+            V3<R> d = b.pos - mean;
+            var += V3<R>{d.x * d.x, d.y * d.y, d.z * d.z};
+        }
+        int axis = 0;
+        if (var.y > var[axis])
+            axis = 1;
+        if (var.z > var[axis])
+            axis = 2;
+        sweepAxis = axis;
+
+        struct Iv
+        {
+            R lo, hi;
+            int idx;
+        };
+        std::vector<Iv> iv(n);
+        for (size_t i = 0; i < n; ++i)
+        {
+            R c = bodies[i].pos[axis];
+            iv[i] = Iv{c - bodies[i].radius, c + bodies[i].radius, (int)i};
+        }
+        std::sort(iv.begin(), iv.end(), [](const Iv &a, const Iv &b) {
+            if (a.lo == b.lo)
+                return a.idx < b.idx;
+            return a.lo < b.lo;
+        });
+
+        std::vector<std::vector<int>> partners;
+        partnerLists(partners);
+        std::vector<std::pair<int, int>> created;
+        for (size_t i = 0; i < n; ++i)
+        {
+            const Iv &a = iv[i];
+            for (size_t j = i + 1; j < n; ++j)
+            {
+                const Iv &b = iv[j];
+                if (b.lo > a.hi)
+                    break;
+                ++pairChecks;
+                V3<R> dp = bodies[a.idx].pos - bodies[b.idx].pos;
+                R r = bodies[a.idx].radius + bodies[b.idx].radius;
+                if (dot(dp, dp) > r * r)
+                    continue;
+                bool con = partners[a.idx].size() <= partners[b.idx].size()
+                               ? constrained(a.idx, b.idx, partners)
+                               : constrained(b.idx, a.idx, partners);
+                hits.push_back(Hit{a.idx, b.idx, con ? 0 : 1});
+                if (!con)
+                    created.push_back({a.idx, b.idx});
+            }
+        }
+        for (auto &pr : created)
+        {
+            Manifold<R> m{};
+            m.a = pr.first;
+            m.b = pr.second;
+            m.serial = nextSerial++;
+            m.n = 0;
+            manifolds.push_back(m);
+        }
+    }
+
+    // Manifold::initialize (manifold.cpp:100-174), box shapes only.
+    bool initManifold(Manifold<R> &m)
+    {
+        const Body<R> &A = bodies[m.a];
+        const Body<R> &B = bodies[m.b];
+        m.friction = sqrt_(A.friction * B.friction);
+        Contact<R> nc[8];
+        std::memset((void *)nc, 0, sizeof(nc));
+        int nn = collideBoxes(A, B, nc, m.basis);
+        for (int i = 0; i < nn; ++i)
+        {
+            for (int j = 0; j < m.n; ++j)
+            {
+                if (nc[i].feature == m.c[j].feature)
+                {
+                    V3<R> rA = nc[i].rA, rB = nc[i].rB;
+                    nc[i] = m.c[j];
+                    if (!m.c[j].stick)
+                    {
+                        nc[i].rA = rA;
+                        nc[i].rB = rB;
+                    }
+                    break;
+                }
+            }
+        }
+        m.n = nn;
+        for (int i = 0; i < nn; ++i)
+            m.c[i] = nc[i];
+        for (int i = 0; i < m.n; ++i)
+        {
+            Contact<R> &c = m.c[i];
+            V3<R> xA = A.pos + rotate(A.rot, c.rA);
+            V3<R> xB = B.pos + rotate(B.rot, c.rB);
+            c.C0 = m.basis * (xA - xB) + V3<R>{K<R>::margin, 0, 0};
+            c.lambda = c.lambda * p.alpha * p.gamma;
+            c.penalty = clampv(c.penalty * p.gamma, K<R>::penaltyMin, K<R>::penaltyMax);
+        }
+        return m.n > 0;
+    }
+
+    // Joint::initialize (joint.cpp:35-54)
+    bool initJoint(Joint<R> &j)
+    {
+        const Body<R> &B = bodies[j.b];
+        V3<R> pa = j.a >= 0 ? transform(bodies[j.a].pos, bodies[j.a].rot, j.rA) : j.rA;
+        j.C0Lin = pa - transform(B.pos, B.rot, j.rB);
+        Q<R> qa = j.a >= 0 ? bodies[j.a].rot : Q<R>{0, 0, 0, 1};
+        j.C0Ang = (qa - B.rot) * j.torqueArm;
+        j.lamLin = j.lamLin * p.alpha * p.gamma;
+        j.lamAng = j.lamAng * p.alpha * p.gamma;
+        j.penLin = clampv(j.penLin * p.gamma, K<R>::penaltyMin, K<R>::penaltyMax);
+        j.penAng = clampv(j.penAng * p.gamma, K<R>::penaltyMin, K<R>::penaltyMax);
+        j.penLin = minv(j.penLin, j.kLin);
+        j.penAng = minv(j.penAng, j.kAng);
+        return !j.broken;
+    }
+
+    void buildBodyForces()
+    {
+        bodyForces.assign(bodies.size(), {});
+        auto add = [&](int body, int kind, int idx, int64_t serial) {
+            if (body >= 0)
+                bodyForces[body].push_back(ForceRef{kind, idx, serial});
+        };
+        for (int k = 0; k < (int)manifolds.size(); ++k)
+        {
+            add(manifolds[k].a, KIND_MANIFOLD, k, manifolds[k].serial);
+            add(manifolds[k].b, KIND_MANIFOLD, k, manifolds[k].serial);
+        }
+        for (int k = 0; k < (int)joints.size(); ++k)
+        {
+            add(joints[k].a, KIND_JOINT, k, joints[k].serial);
+            add(joints[k].b, KIND_JOINT, k, joints[k].serial);
+        }
+        for (int k = 0; k < (int)springs.size(); ++k)
+        {
+            add(springs[k].a, KIND_SPRING, k, springs[k].serial);
+            add(springs[k].b, KIND_SPRING, k, springs[k].serial);
+        }
+        for (auto &l : bodyForces)
+            std::sort(l.begin(), l.end(), [](const ForceRef &x, const ForceRef &y) { return x.serial > y.serial; });
+    }
+
+    // Greedy coloring over solved bodies in list order (avbd_gpu_solver.cpp:1812-1865).
+    void color()
+    {
+        int n = (int)bodies.size();
+        colors.assign(n, -1);
+        int maxColor = -1;
+        std::vector<char> used;
+        for (int i = 0; i < n; ++i)
+        {
+            if (!solved(i))
+                continue;
+            used.assign((size_t)maxColor + 2, 0);
+            for (const ForceRef &f : bodyForces[i])
+            {
+                int o = partnerOf(f, i);
+                if (o >= 0)
+                {
+                    int c = colors[o];
+                    if (c >= 0 && c < (int)used.size())
+                        used[c] = 1;
+                }
+            }
+            int c = 0;
+            while (c < (int)used.size() && used[c])
+                ++c;
+            colors[i] = c;
+            maxColor = std::max(maxColor, c);
+        }
+        colorCount = maxColor + 1;
+    }
+
+    int partnerOf(const ForceRef &f, int i) const
+    {
+        int a, b;
+        if (f.kind == KIND_MANIFOLD)
+        {
+            a = manifolds[f.index].a;
+            b = manifolds[f.index].b;
+        }
+        else if (f.kind == KIND_JOINT)
+        {
+            a = joints[f.index].a;
+            b = joints[f.index].b;
+        }
+        else
+        {
+            a = springs[f.index].a;
+            b = springs[f.index].b;
+        }
+        return a == i ? b : a;
+    }
+
+    // Solver::prepareStep (solver.cpp:1043-1170)
+    void prepare()
+    {
+        broadphase();
+        // force initialization; inactive forces removed in order
+        std::vector<Manifold<R>> keptM;
+        keptM.reserve(manifolds.size());
+        for (auto &m : manifolds)
+            if (initManifold(m))
+                keptM.push_back(m);
+        manifolds.swap(keptM);
+        std::vector<Joint<R>> keptJ;
+        for (auto &j : joints)
+            if (initJoint(j))
+                keptJ.push_back(j);
+        joints.swap(keptJ);
+        buildBodyForces();
+
+        for (int i = 0; i < (int)bodies.size(); ++i)
+        {
+            Body<R> &b = bodies[i];
+            b.inerLin = b.pos + b.vel * p.dt;
+            if (b.mass > 0)
+                b.inerLin += V3<R>{0, 0, p.gravity} * (p.dt * p.dt);
+            b.inerAng = b.rot + b.angVel * p.dt;
+            V3<R> accel = (b.vel - b.prevVel) / p.dt;
+            R accelExt = accel.z * signv(p.gravity);
+            R w = clampv(accelExt / abs_(p.gravity), R(0), R(1));
+            if (!std::isfinite(w))
+                w = 0;
+            b.initLin = b.pos;
+            b.initAng = b.rot;
+            if (b.mass > 0)
+            {
+                b.pos = b.pos + b.vel * p.dt + V3<R>{0, 0, p.gravity} * (w * p.dt * p.dt);
+                b.rot = b.rot + b.angVel * p.dt;
+            }
+            if (b.mass > 0 && bodyForces[i].empty())
+            {
+                b.pos = b.inerLin;
+                b.rot = b.inerAng;
+            }
+        }
+        color();
+    }
+
+    // Manifold::updatePrimal contribution of one manifold (manifold.cpp:176-229)
+    void stampManifold(const Manifold<R> &m, int body, M3<R> &lLin, M3<R> &lAng, M3<R> &lCross, V3<R> &rLin, V3<R> &rAng) const
+    {
+        const Body<R> &A = bodies[m.a];
+        const Body<R> &B = bodies[m.b];
+        V3<R> dqAL = A.pos - A.initLin, dqAA = A.rot - A.initAng;
+        V3<R> dqBL = B.pos - B.initLin, dqBA = B.rot - B.initAng;
+        for (int i = 0; i < m.n; ++i)
+        {
+            const Contact<R> &c = m.c[i];
+            V3<R> rAW = rotate(A.rot, c.rA), rBW = rotate(B.rot, c.rB);
+            M3<R> jAL = m.basis, jBL = -m.basis;
+            M3<R> jAA{cross(rAW, jAL[0]), cross(rAW, jAL[1]), cross(rAW, jAL[2])};
+            M3<R> jBA{cross(rBW, jBL[0]), cross(rBW, jBL[1]), cross(rBW, jBL[2])};
+            M3<R> Km = diag(c.penalty.x, c.penalty.y, c.penalty.z);
+            V3<R> C = c.C0 * (R(1) - p.alpha) + jAL * dqAL + jBL * dqBL + jAA * dqAA + jBA * dqBA;
+            V3<R> F = Km * C + c.lambda;
+            F[0] = fmin_(F[0], R(0));
+            R bounds = abs_(F[0]) * m.friction;
+            R fs = sqrt_(F[1] * F[1] + F[2] * F[2]);
+            if (fs > bounds && fs > 0)
+            {
+                F[1] *= bounds / fs;
+                F[2] *= bounds / fs;
+            }
+            const M3<R> &jL = body == m.a ? jAL : jBL;
+            const M3<R> &jA = body == m.a ? jAA : jBA;
+            M3<R> jLT = transpose(jL), jAT = transpose(jA);
+            M3<R> jATk = jAT * Km;
+            lLin += jLT * Km * jL;
+            lAng += jATk * jA;
+            lCross += jATk * jL;
+            rLin += jLT * F;
+            rAng += jAT * F;
+        }
+    }
+
+    // Manifold::updateDual (manifold.cpp:231-280)
+    void dualManifold(Manifold<R> &m) const
+    {
+        const Body<R> &A = bodies[m.a];
+        const Body<R> &B = bodies[m.b];
+        V3<R> dqAL = A.pos - A.initLin, dqAA = A.rot - A.initAng;
+        V3<R> dqBL = B.pos - B.initLin, dqBA = B.rot - B.initAng;
+        for (int i = 0; i < m.n; ++i)
+        {
+            Contact<R> &c = m.c[i];
+            V3<R> rAW = rotate(A.rot, c.rA), rBW = rotate(B.rot, c.rB);
+            M3<R> jAL = m.basis, jBL = -m.basis;
+            M3<R> jAA{cross(rAW, jAL[0]), cross(rAW, jAL[1]), cross(rAW, jAL[2])};
+            M3<R> jBA{cross(rBW, jBL[0]), cross(rBW, jBL[1]), cross(rBW, jBL[2])};
+            M3<R> Km = diag(c.penalty.x, c.penalty.y, c.penalty.z);
+            V3<R> C = c.C0 * (R(1) - p.alpha) + jAL * dqAL + jBL * dqBL + jAA * dqAA + jBA * dqBA;
+            V3<R> F = Km * C + c.lambda;
+            F[0] = fmin_(F[0], R(0));
+            R bounds = abs_(F[0]) * m.friction;
+            R fs = sqrt_(F[1] * F[1] + F[2] * F[2]);
+            if (fs > bounds && fs > 0)
+            {
+                F[1] *= bounds / fs;
+                F[2] *= bounds / fs;
+            }
+            c.lambda = F;
+            if (F[0] < 0)
+                c.penalty[0] = fmin_(c.penalty[0] + p.betaLin * abs_(C[0]), K<R>::penaltyMax);
+            if (fs <= bounds)
+            {
+                c.penalty[1] = fmin_(c.penalty[1] + p.betaLin * abs_(C[1]), K<R>::penaltyMax);
+                c.penalty[2] = fmin_(c.penalty[2] + p.betaLin * abs_(C[2]), K<R>::penaltyMax);
+                c.stick = sqrt_(C[1] * C[1] + C[2] * C[2]) < K<R>::stickThresh;
+            }
+        }
+    }
+
+    // geometric stiffness helper (joint.cpp:14-23)
+    static M3<R> gsBall(int k, V3<R> v)
+    {
+        M3<R> m = diag(-v[k], -v[k], -v[k]);
+        m[0][k] += v[0];
+        m[1][k] += v[1];
+        m[2][k] += v[2];
+        return m;
+    }
+
+    // Joint::updatePrimal (joint.cpp:56-123)
+    void stampJoint(const Joint<R> &j, int body, M3<R> &lLin, M3<R> &lAng, M3<R> &lCross, V3<R> &rLin, V3<R> &rAng) const
+    {
+        const Body<R> &B = bodies[j.b];
+        bool isA = body == j.a;
+        if (lengthSq(j.penLin) > 0)
+        {
+            V3<R> rAW = j.a >= 0 ? rotate(bodies[j.a].rot, j.rA) : V3<R>{0, 0, 0};
+            V3<R> rBW = rotate(B.rot, j.rB);
+            M3<R> Km = diag(j.penLin.x, j.penLin.y, j.penLin.z);
+            V3<R> C = (j.a >= 0 ? bodies[j.a].pos + rAW : j.rA) - (B.pos + rBW);
+            if (std::isinf(j.kLin))
+                C -= j.C0Lin * p.alpha;
+            V3<R> F = Km * C + j.lamLin;
+            M3<R> jL = isA ? ident(R(1)) : ident(R(-1));
+            M3<R> jA = isA ? skew(-rAW) : skew(rBW);
+            M3<R> jLT = transpose(jL), jAT = transpose(jA);
+            M3<R> jATk = jAT * Km;
+            lLin += jLT * Km * jL;
+            lAng += jATk * jA;
+            lCross += jATk * jL;
+            V3<R> r = isA ? rAW : -rBW;
+            M3<R> H = gsBall(0, r) * F[0] + gsBall(1, r) * F[1] + gsBall(2, r) * F[2];
+            lAng += diagonalize(H);
+            rLin += jLT * F;
+            rAng += jAT * F;
+        }
+        if (lengthSq(j.penAng) > 0)
+        {
+            M3<R> Km = diag(j.penAng.x, j.penAng.y, j.penAng.z);
+            Q<R> qa = j.a >= 0 ? bodies[j.a].rot : Q<R>{0, 0, 0, 1};
+            V3<R> C = (qa - B.rot) * j.torqueArm;
+            if (std::isinf(j.kAng))
+                C -= j.C0Ang * p.alpha;
+            V3<R> F = Km * C + j.lamAng;
+            M3<R> jA = (isA ? ident(R(1)) : ident(R(-1))) * j.torqueArm;
+            lAng += transpose(jA) * Km * jA;
+            rAng += transpose(jA) * F;
+        }
+    }
+
+    // Joint::updateDual (joint.cpp:125-182)
+    void dualJoint(Joint<R> &j) const
+    {
+        const Body<R> &B = bodies[j.b];
+        if (lengthSq(j.penLin) > 0)
+        {
+            M3<R> Km = diag(j.penLin.x, j.penLin.y, j.penLin.z);
+            V3<R> pa = j.a >= 0 ? transform(bodies[j.a].pos, bodies[j.a].rot, j.rA) : j.rA;
+            V3<R> C = pa - transform(B.pos, B.rot, j.rB);
+            if (std::isinf(j.kLin))
+            {
+                C -= j.C0Lin * p.alpha;
+                j.lamLin = Km * C + j.lamLin;
+            }
+            j.penLin = minv(j.penLin + absv(C) * p.betaLin, fmin_(j.kLin, K<R>::penaltyMax));
+        }
+        if (lengthSq(j.penAng) > 0)
+        {
+            M3<R> Km = diag(j.penAng.x, j.penAng.y, j.penAng.z);
+            Q<R> qa = j.a >= 0 ? bodies[j.a].rot : Q<R>{0, 0, 0, 1};
+            V3<R> C = (qa - B.rot) * j.torqueArm;
+            if (std::isinf(j.kAng))
+            {
+                C -= j.C0Ang * p.alpha;
+                j.lamAng = Km * C + j.lamAng;
+            }
+            j.penAng = minv(j.penAng + absv(C) * p.betaAng, fmin_(j.kAng, K<R>::penaltyMax));
+        }
+        if (lengthSq(j.lamAng) > j.fracture * j.fracture)
+        {
+            j.penLin = j.penAng = j.lamLin = j.lamAng = V3<R>{0, 0, 0};
+            j.broken = true;
+        }
+    }
+
+    // Spring::updatePrimal (spring.cpp:27-69)
+    void stampSpring(const Spring<R> &s, int body, M3<R> &lLin, M3<R> &lAng, M3<R> &lCross, V3<R> &rLin, V3<R> &rAng) const
+    {
+        const Body<R> &A = bodies[s.a];
+        const Body<R> &B = bodies[s.b];
+        V3<R> d = transform(A.pos, A.rot, s.rA) - transform(B.pos, B.rot, s.rB);
+        R dl = length(d);
+        if (dl <= R(1.0e-6f))
+            return;
+        V3<R> n = d / dl;
+        R f = s.k * (dl - s.rest);
+        V3<R> jL, jA;
+        if (body == s.a)
+        {
+            V3<R> rW = rotate(A.rot, s.rA);
+            jL = n;
+            jA = cross(rW, n);
+        }
+        else
+        {
+            V3<R> rW = rotate(B.rot, s.rB);
+            jL = -n;
+            jA = -cross(rW, n);
+        }
+        lLin += outer(jL, jL) * s.k;
+        lAng += outer(jA, jA) * s.k;
+        lCross += outer(jA, jL) * s.k;
+        rLin += jL * f;
+        rAng += jA * f;
+    }
+
+    // One body's primal update (solver.cpp:1184-1240)
+    void primalBody(int i)
+    {
+        Body<R> &b = bodies[i];
+        R dt2 = p.dt * p.dt;
+        M3<R> ML = diag(b.mass, b.mass, b.mass);
+        M3<R> MA = diag(b.moment.x, b.moment.y, b.moment.z);
+        M3<R> lLin = ML / dt2, lAng = MA / dt2;
+        M3<R> lCross{V3<R>{0, 0, 0}, V3<R>{0, 0, 0}, V3<R>{0, 0, 0}};
+        V3<R> rLin = ML / dt2 * (b.pos - b.inerLin);
+        V3<R> rAng = MA / dt2 * (b.rot - b.inerAng);
+        for (const ForceRef &f : bodyForces[i])
+        {
+            if (f.kind == KIND_MANIFOLD)
+                stampManifold(manifolds[f.index], i, lLin, lAng, lCross, rLin, rAng);
+            else if (f.kind == KIND_JOINT)
+                stampJoint(joints[f.index], i, lLin, lAng, lCross, rLin, rAng);
+            else
+                stampSpring(springs[f.index], i, lLin, lAng, lCross, rLin, rAng);
+        }
+        V3<R> dxL, dxA;
+        solve6(lLin, lAng, lCross, -rLin, -rAng, dxL, dxA);
+        b.pos = b.pos + dxL;
+        b.rot = b.rot + dxA;
+    }
+
+    // Solver::iteratePrimalDualCpu (solver.cpp:1172-1273)
+    void iterate()
+    {
+        int n = (int)bodies.size();
+        std::vector<std::vector<int>> byColor;
+        if (orderMode == ORDER_COLOR)
+        {
+            byColor.assign(colorCount, {});
+            for (int i = 0; i < n; ++i)
+                if (colors[i] >= 0)
+                    byColor[colors[i]].push_back(i);
+        }
+        for (int it = 0; it < p.iterations; ++it)
+        {
+            if (orderMode == ORDER_COLOR)
+            {
+                for (auto &list : byColor)
+                    for (int i : list)
+                        primalBody(i);
+            }
+            else
+            {
+                for (int i = 0; i < n; ++i)
+                    if (solved(i))
+                        primalBody(i);
+            }
+            for (auto &m : manifolds)
+                dualManifold(m);
+            for (auto &j : joints)
+                dualJoint(j);
+        }
+    }
+
+    // Solver::finishStep (solver.cpp:1275-1298), box bodies only
+    void finish()
+    {
+        for (auto &b : bodies)
+        {
+            b.prevVel = b.vel;
+            if (b.mass > 0)
+            {
+                b.vel = (b.pos - b.initLin) / p.dt;
+                b.angVel = (b.rot - b.initAng) / p.dt;
+            }
+        }
+    }
+
+    void step()
+    {
+        prepare();
+        iterate();
+        finish();
+    }
+};
+
+} // namespace avbd_oracle

@@ -1,0 +1,174 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Run in a container where the reference tree exists (it builds oracle/_ref from
+the reference's own solver sources; see oracle/Makefile):
+
+    python tests/golden/make_golden.py
+
+Every array written here is produced by the reference solver itself
+(oracle/_ref/libavbdref.so, driven through its public Solver/Rigid/Joint/
+Spring/Manifold API with the sweep-and-prune broadphase, stepping in colour
+order). The fixtures are data only: scene inputs and the reference's outputs.
+
+Fixtures
+  collide_boxes.npz   Manifold::collide on seeded random box pairs (contacts, keys, basis)
+  stack.npz           10-box stack: bodies after every one of 300 steps
+  pyramid_chain.npz   210-box pyramid + 1000-link chain: bodies at selected steps,
+                      plus a full state at step 100 and the state after step 101
+  pile.npz            1000-box pile: full state before/after single steps with the
+                      SAP pair emission and the colours of the stepped state
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "avbd-demo3d_amd"))
+
+from avbd import scenes  # noqa: E402
+from avbd.state import BODY, default_params  # noqa: E402
+from oracle.bindings import RefMaker, RefWorld, ref_collide, load_ref  # noqa: E402
+
+BODY_FIELDS = ("pos", "rot", "vel", "ang_vel")
+
+
+def body_block(state):
+    b = state.bodies
+    return np.concatenate([b["pos"], b["rot"], b["vel"], b["ang_vel"]], axis=1).astype(np.float32)
+
+
+def random_box(rng, near=None):
+    rec = np.zeros((), BODY)
+    rec["size"] = rng.uniform(0.2, 2.5, 3)
+    if near is None:
+        rec["pos"] = rng.uniform(-0.5, 0.5, 3)
+    else:
+        rec["pos"] = near + rng.normal(0, 0.8, 3)
+    q = rng.normal(size=4)
+    if rng.random() < 0.25:  # axis-aligned and near-aligned cases exercise face clipping ties
+        q = np.array([0, 0, rng.normal() * 0.05, 1.0])
+    rec["rot"] = (q / np.linalg.norm(q)).astype(np.float32)
+    return rec
+
+
+def make_collide(path, count=4000, seed=11):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    A = np.zeros(count, BODY)
+    B = np.zeros(count, BODY)
+    n = np.zeros(count, np.int32)
+    feat = np.zeros((count, 8), np.int32)
+    ra = np.zeros((count, 8, 3), np.float32)
+    rb = np.zeros((count, 8, 3), np.float32)
+    basis = np.zeros((count, 9), np.float32)
+    for k in range(count):
+        A[k] = random_box(rng)
+        B[k] = random_box(rng, near=A[k]["pos"])
+        cnt, c, bs = ref_collide(A[k], B[k])
+        n[k] = cnt
+        if cnt:
+            feat[k, :cnt] = c["feature"]
+            ra[k, :cnt] = c["r_a"]
+            rb[k, :cnt] = c["r_b"]
+            basis[k] = bs
+    np.savez_compressed(path, a_size=A["size"], a_pos=A["pos"], a_rot=A["rot"], b_size=B["size"],
+                        b_pos=B["pos"], b_rot=B["rot"], n=n, feature=feat, r_a=ra, r_b=rb, basis=basis)
+    print(f"{path}: {count} pairs, {int((n > 0).sum())} touching, max contacts {int(n.max())}")
+
+
+def scene_arrays(sc):
+    return dict(s_size=sc.size, s_density=sc.density, s_friction=sc.friction, s_pos=sc.pos, s_rot=sc.rot,
+                s_vel=sc.vel, s_angvel=sc.angvel, f_type=sc.ftype, f_a=sc.fa, f_b=sc.fb, f_ra=sc.ra, f_rb=sc.rb,
+                f_p0=sc.p0, f_p1=sc.p1, f_p2=sc.p2, gravity=np.float32(sc.gravity))
+
+
+def state_arrays(prefix, st):
+    return {f"{prefix}_bodies": st.bodies, f"{prefix}_joints": st.joints, f"{prefix}_springs": st.springs,
+            f"{prefix}_manifolds": st.manifolds, f"{prefix}_next_serial": np.int64(st.next_serial)}
+
+
+def make_stack(path):
+    sc = scenes.stack()
+    prm = default_params(gravity=sc.gravity)
+    w = RefWorld(prm, scene=sc)
+    traj = []
+    colors = []
+    nman = []
+    for _ in range(300):
+        w.step(1, colored=True)
+        st = w.state()
+        traj.append(body_block(st))
+        colors.append(w.colors())
+        nman.append(len(st.manifolds))
+    np.savez_compressed(path, **scene_arrays(sc), traj=np.stack(traj), colors=np.stack(colors),
+                        manifold_count=np.array(nman, np.int32))
+    print(f"{path}: 300 steps, final manifolds {nman[-1]}")
+
+
+def make_pyramid_chain(path):
+    sc = scenes.pyramid_chain()
+    prm = default_params(gravity=sc.gravity)
+    w = RefWorld(prm, scene=sc)
+    keep_steps = [1, 2, 3, 5, 10, 50, 100, 200, 300, 600]
+    snaps = {}
+    pre = post = None
+    for k in range(1, 601):
+        w.step(1, colored=True)
+        if k in keep_steps:
+            snaps[k] = body_block(w.state())
+        if k == 100:
+            pre = w.state()
+            pre_colors = None
+        if k == 101:
+            post = w.state()
+            post_colors = w.colors()
+    out = dict(scene_arrays(sc), snap_steps=np.array(keep_steps, np.int32),
+               snaps=np.stack([snaps[k] for k in keep_steps]), post_colors=post_colors)
+    out.update(state_arrays("pre", pre))
+    out.update(state_arrays("post", post))
+    np.savez_compressed(path, **out)
+    print(f"{path}: {sc.num_bodies} bodies, {len(pre.joints)} joints, {len(pre.springs)} springs, "
+          f"{len(pre.manifolds)} manifolds at step 100")
+
+
+def make_pile(path, count=1000, pre_steps=(60, 140, 240)):
+    sc = scenes.pile(count, seed=5)
+    prm = default_params(gravity=sc.gravity)
+    w = RefWorld(prm, scene=sc)
+    out = dict(scene_arrays(sc), pre_steps=np.array(pre_steps, np.int32))
+    done = 0
+    for idx, s in enumerate(pre_steps):
+        while done < s:
+            w.step(1, colored=True)
+            done += 1
+        pre = w.state()
+        pairs, checks, hits = w.broadphase_pairs()
+        w.step(1, colored=True)
+        done += 1
+        post = w.state()
+        out.update(state_arrays(f"pre{idx}", pre))
+        out.update(state_arrays(f"post{idx}", post))
+        out[f"pairs{idx}"] = pairs
+        out[f"checks{idx}"] = np.int64(checks)
+        out[f"hits{idx}"] = np.int64(hits)
+        out[f"colors{idx}"] = w.colors()
+        print(f"  step {s}: {len(pre.manifolds)} manifolds, {len(pairs)} new-manifold pairs, "
+              f"{checks} checks, {hits} hits, {int(w.colors().max()) + 1} colours")
+    np.savez_compressed(path, **out)
+    print(f"{path}: {count} boxes")
+
+
+def main():
+    load_ref()
+    make_collide(os.path.join(HERE, "collide_boxes.npz"))
+    make_stack(os.path.join(HERE, "stack.npz"))
+    make_pyramid_chain(os.path.join(HERE, "pyramid_chain.npz"))
+    make_pile(os.path.join(HERE, "pile.npz"))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,119 @@
+"""The CPU oracle against golden fixtures written by the compiled reference.
+
+Every comparison is exact FP32 equality: the oracle restates solver.cpp,
+manifold.cpp, collide.cpp, joint.cpp, spring.cpp and maths.h operation by
+operation, and the fixtures come from those very sources (tests/golden/make_golden.py).
+"""
+import numpy as np
+import pytest
+
+from conftest import (assert_states_equal, body_block, load_golden, scene_from_golden, state_from_golden)
+
+
+def test_collide_boxes_matches_reference(oracle):
+    """collide.cpp:898-955 on 4000 seeded box pairs: counts, feature keys, offsets, basis."""
+    from avbd.state import BODY
+    z = load_golden("collide_boxes.npz")
+    n = len(z["n"])
+    mismatches = 0
+    for k in range(n):
+        a, b = np.zeros((), BODY), np.zeros((), BODY)
+        a["size"], a["pos"], a["rot"] = z["a_size"][k], z["a_pos"][k], z["a_rot"][k]
+        b["size"], b["pos"], b["rot"] = z["b_size"][k], z["b_pos"][k], z["b_rot"][k]
+        cnt, c, basis = oracle.oracle_collide(a, b)
+        ok = cnt == z["n"][k]
+        if ok and cnt:
+            ok = (np.array_equal(c["feature"], z["feature"][k, :cnt]) and np.array_equal(c["r_a"], z["r_a"][k, :cnt])
+                  and np.array_equal(c["r_b"], z["r_b"][k, :cnt]) and np.array_equal(basis, z["basis"][k]))
+        mismatches += 0 if ok else 1
+    assert mismatches == 0, f"{mismatches} of {n} box pairs differ from the reference"
+    assert (z["n"] > 0).sum() > n // 2 and z["n"].max() >= 4
+
+
+def test_stack_trajectory_bit_exact(oracle):
+    """10-box stack, 300 steps: every step's bodies and colours equal the reference's."""
+    z = load_golden("stack.npz")
+    sc = scene_from_golden(z)
+    from avbd.scenes import build_state
+    from avbd.state import default_params
+    w = oracle.OracleWorld(build_state(sc, oracle.OracleMaker()), default_params(gravity=sc.gravity))
+    for k in range(300):
+        w.step(1)
+        st = w.state()
+        np.testing.assert_array_equal(body_block(st), z["traj"][k], err_msg=f"step {k + 1}")
+        np.testing.assert_array_equal(w.colors(), z["colors"][k], err_msg=f"colours step {k + 1}")
+        assert len(st.manifolds) == z["manifold_count"][k]
+
+
+def test_pyramid_chain_snapshots_bit_exact(oracle):
+    """Pyramid + 1000-link chain with joints and springs, 600 steps."""
+    z = load_golden("pyramid_chain.npz")
+    sc = scene_from_golden(z)
+    from avbd.scenes import build_state
+    from avbd.state import default_params
+    w = oracle.OracleWorld(build_state(sc, oracle.OracleMaker()), default_params(gravity=sc.gravity))
+    steps = list(z["snap_steps"])
+    done = 0
+    for i, s in enumerate(steps):
+        w.step(int(s) - done)
+        done = int(s)
+        np.testing.assert_array_equal(body_block(w.state()), z["snaps"][i], err_msg=f"step {s}")
+
+
+def test_pyramid_chain_single_step_from_reference_state(oracle):
+    """Identical inputs (the reference's state at step 100): one step reproduces its step 101."""
+    z = load_golden("pyramid_chain.npz")
+    from avbd.state import default_params
+    pre, post = state_from_golden(z, "pre"), state_from_golden(z, "post")
+    w = oracle.OracleWorld(pre, default_params(gravity=float(z["gravity"])))
+    w.step(1)
+    assert_states_equal(w.state(), post, "pyramid_chain step 101")
+    np.testing.assert_array_equal(w.colors(), z["post_colors"])
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_pile_single_step_pairs_colors(oracle, idx):
+    """1000-box pile: SAP pair emission, colours and the stepped state equal the reference's."""
+    z = load_golden("pile.npz")
+    from avbd.state import default_params
+    pre, post = state_from_golden(z, f"pre{idx}"), state_from_golden(z, f"post{idx}")
+    w = oracle.OracleWorld(pre, default_params(gravity=float(z["gravity"])))
+    w.prepare()
+    hits = w.hits()
+    np.testing.assert_array_equal(hits[:, :2], z[f"pairs{idx}"], err_msg="SAP pair emission")
+    assert w.pair_checks == int(z[f"checks{idx}"])
+    assert len(hits) == int(z[f"hits{idx}"])
+    w.iterate()
+    w.finish()
+    assert_states_equal(w.state(), post, f"pile step {int(z['pre_steps'][idx]) + 1}")
+    np.testing.assert_array_equal(w.colors(), z[f"colors{idx}"])
+
+
+def test_oracle_list_order_matches_plain_reference_step(oracle):
+    """ORDER_LIST reproduces Solver::step itself (body-list Gauss-Seidel), when the reference is built."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scenes.pile(400, seed=9)
+    prm = default_params(gravity=sc.gravity)
+    ref = oracle.RefWorld(prm, scene=sc)
+    w = oracle.OracleWorld(scenes.build_state(sc, oracle.OracleMaker()), prm, order=oracle.OracleWorld.ORDER_LIST)
+    for k in range(90):
+        ref.step(1, colored=False)
+        w.step(1)
+    np.testing.assert_array_equal(body_block(w.state()), body_block(ref.state()))
+
+
+def test_record_builders_match_reference_constructors(oracle):
+    """Rigid/Joint/Spring constructor-derived fields equal the reference's (rigid.cpp:24-42)."""
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    from avbd import scenes
+    sc = scenes.pyramid_chain(base=6, links=40)
+    a = scenes.build_state(sc, oracle.OracleMaker())
+    b = scenes.build_state(sc, oracle.RefMaker())
+    for f in a.bodies.dtype.names:
+        np.testing.assert_array_equal(a.bodies[f], b.bodies[f], err_msg=f)
+    np.testing.assert_array_equal(a.joints, b.joints)
+    np.testing.assert_array_equal(a.springs, b.springs)
