@@ -49,6 +49,7 @@ def load(path: str | None = None):
             getattr(lib, n).argtypes = [_p]
         lib.avbd_step.argtypes = [_p, C.c_int32]
         lib.avbd_get_stats.argtypes = [_p, _p]
+        lib.avbd_reset_stats.argtypes = [_p]
         lib.avbd_get_colors.argtypes = [_p, _p, C.c_int32]
         lib.avbd_get_hits.argtypes = [_p, _p, C.c_int32, _p]
         lib.avbd_debug_inject_fault.argtypes = [_p, C.c_int32]
@@ -178,6 +179,9 @@ class HipWorld:
         out = np.zeros((), STATS)
         _check(self._lib.avbd_get_stats(self._h, out.ctypes.data_as(_p)))
         return out
+
+    def reset_stats(self):
+        _check(self._lib.avbd_reset_stats(self._h))
 
     def colors(self) -> np.ndarray:
         nb = self.counts()[0]

@@ -49,7 +49,9 @@ STATS = np.dtype(
      ("coloring_rounds", "<i4"), ("sweep_axis", "<i4"), ("axis_fallbacks", "<i4"),
      ("cpu_fallbacks", "<i4"), ("steps_taken", "<i8"), ("broadphase_ms", "<f4"),
      ("narrowphase_ms", "<f4"), ("coloring_ms", "<f4"), ("iterate_ms", "<f4"),
-     ("finish_ms", "<f4"), ("step_ms", "<f4")], align=True)
+     ("finish_ms", "<f4"), ("step_ms", "<f4"), ("iterate_bytes", "<f8"), ("step_bytes", "<f8"),
+     ("iterate_bytes_total", "<f8"), ("step_bytes_total", "<f8"), ("iterate_ms_total", "<f8"),
+     ("step_ms_total", "<f8"), ("counted_steps", "<i8"), ("timed_steps", "<i8")], align=True)
 
 CONFIG = np.dtype(
     [("params", PARAMS), ("max_bodies", "<i4"), ("max_joints", "<i4"), ("max_springs", "<i4"),

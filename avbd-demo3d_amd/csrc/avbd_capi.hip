@@ -63,6 +63,10 @@ struct avbd_handle
     int *adjTotal = nullptr;
     std::vector<void *> allocs;
     hipEvent_t ev[8] = {};
+    // timing ring: per step, events at step start / before iterate / after iterate / step end
+    static constexpr int kRing = 4096;
+    std::vector<hipEvent_t> ring;
+    long long ringSteps = 0;   // steps recorded into the ring since the last reset
     avbd_stats stats{};
     long long stepsTaken = 0;
     bool timing = false;
@@ -213,7 +217,10 @@ int enqueue_prepare(avbd_handle *h)
     Counters *ctr = h->ctr;
     const int G = 1024;
     if (h->timing)
+    {
         HIP_TRY(hipEventRecord(h->ev[0], s));
+        HIP_TRY(hipEventRecord(h->ring[4 * (h->ringSteps % avbd_handle::kRing) + 0], s));
+    }
     k_step_begin<<<1, 1, 0, s>>>(ctr);
     k_barrier_reset<<<1, 1, 0, s>>>(h->gb);
 
@@ -247,10 +254,10 @@ int enqueue_prepare(avbd_handle *h)
     // ---- force initialisation (manifold.cpp:100-174, joint.cpp:35-54) ----
     k_narrowphase<<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep);
     scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numKept);
-    k_compact_manifolds<<<G, 256, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, ctr);
+    k_compact_manifolds<<<G, 256, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr);
     k_compact_finish<<<1, 1, 0, s>>>(ctr);
     if (h->nj > 0)
-        k_joint_init<<<ceil_div(h->nj, 256), 256, 0, s>>>(h->joints, h->nj, h->B, h->P);
+        k_joint_init<<<ceil_div(h->nj, 256), 256, 0, s>>>(h->joints, h->nj, h->B, h->P, ctr);
     if (h->timing)
         HIP_TRY(hipEventRecord(h->ev[2], s));
 
@@ -264,6 +271,7 @@ int enqueue_prepare(avbd_handle *h)
                                   S.adjCursor, S.adj, S.adjSerial, h->maxAdj, &ctr->overflow);
     k_adj_sort<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjSerial, N, h->maxAdj);
     k_warmstart<<<ceil_div(N, 256), 256, 0, s>>>(h->B, h->P, S.deg, N, S.color, ctr);
+    k_step_bytes<<<1, 1, 0, s>>>(ctr, N, h->P.iterations);
 
     // ---- colouring (avbd_gpu_solver.cpp:1812-1865) ----
     {
@@ -292,7 +300,10 @@ int enqueue_prepare(avbd_handle *h)
         k_color_offsets<<<1, 256, 0, s>>>(h->radixHistScan, tiles, N, S.colorOffsets);
     }
     if (h->timing)
+    {
         HIP_TRY(hipEventRecord(h->ev[3], s));
+        HIP_TRY(hipEventRecord(h->ring[4 * (h->ringSteps % avbd_handle::kRing) + 1], s));
+    }
     return AVBD_OK;
 }
 
@@ -314,7 +325,10 @@ int enqueue_iterate(avbd_handle *h)
     if (r)
         return r;
     if (h->timing)
+    {
         HIP_TRY(hipEventRecord(h->ev[4], h->stream));
+        HIP_TRY(hipEventRecord(h->ring[4 * (h->ringSteps % avbd_handle::kRing) + 2], h->stream));
+    }
     return AVBD_OK;
 }
 
@@ -323,7 +337,11 @@ int enqueue_finish(avbd_handle *h)
     if (h->nb > 0)
         k_finish<<<ceil_div(h->nb, 256), 256, 0, h->stream>>>(h->B, h->P, h->nb);
     if (h->timing)
+    {
         HIP_TRY(hipEventRecord(h->ev[5], h->stream));
+        HIP_TRY(hipEventRecord(h->ring[4 * (h->ringSteps % avbd_handle::kRing) + 3], h->stream));
+        h->ringSteps++;
+    }
     h->stepsTaken++;
     return AVBD_OK;
 }
@@ -459,7 +477,8 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     h->maxSprings = std::max(config->max_springs, 0);
     h->maxManifolds = config->max_manifolds > 0 ? config->max_manifolds : 12 * N + 1024;
     h->maxPairs = config->max_pairs > 0 ? config->max_pairs : 16 * N + 4096;
-    h->maxChunks = 4 * N + h->maxPairs / 4 + 4096;
+    // one work item per 64 interval visits: room for ~40 * 64 = 2560 sweep checks per body
+    h->maxChunks = 41 * N + h->maxPairs / 8 + 65536;
     h->maxAdj = 2 * (h->maxManifolds + h->maxJoints + h->maxSprings) + 16;
     int hs = 1024;
     while (hs < 2 * (h->maxManifolds + h->maxJoints + h->maxSprings))
@@ -522,6 +541,13 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     for (int k = 0; k < 8; ++k)
         if ((e = hipEventCreate(&h->ev[k])) != hipSuccess)
             return bail(fail(AVBD_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e)));
+    if (h->timing)
+    {
+        h->ring.assign(4 * avbd_handle::kRing, nullptr);
+        for (hipEvent_t &ev : h->ring)
+            if ((e = hipEventCreate(&ev)) != hipSuccess)
+                return bail(fail(AVBD_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e)));
+    }
     if ((e = hipStreamSynchronize(h->stream)) != hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e)));
     *out = h;
@@ -540,6 +566,9 @@ int avbd_destroy(avbd_handle *h)
     for (int k = 0; k < 8; ++k)
         if (h->ev[k])
             (void)hipEventDestroy(h->ev[k]);
+    for (hipEvent_t ev : h->ring)
+        if (ev)
+            (void)hipEventDestroy(ev);
     if (h->stream)
         (void)hipStreamDestroy(h->stream);
     delete h;
@@ -705,6 +734,8 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     std::memset(&c, 0, sizeof(c));
     c.numManifolds = M;
     c.nextSerial = next_serial;
+    for (int i = 0; i < num_springs; ++i)
+        c.springEndpoints += (bodies[springs[i].body_a].mass > 0.0f ? 1 : 0) + (bodies[springs[i].body_b].mass > 0.0f ? 1 : 0);
     if ((e = up(h->ctr, &c, sizeof(c))))
         return fail(AVBD_ERR_HIP, std::string("upload counters: ") + hipGetErrorString(e));
     std::vector<int> negs(std::max(N, 1), -1);
@@ -992,8 +1023,50 @@ int avbd_get_stats(avbd_handle *h, avbd_stats *out)
         if (hipEventElapsedTime(&total, h->ev[0], h->ev[5]) == hipSuccess)
             out->step_ms = total;
     }
+    out->iterate_bytes = c.lastIterBytes;
+    out->step_bytes = c.lastStepBytes;
+    out->iterate_bytes_total = c.cumIterBytes;
+    out->step_bytes_total = c.cumStepBytes;
+    out->counted_steps = c.cumSteps;
+    if (h->timing && h->ringSteps > 0)
+    {
+        long long n = std::min<long long>(h->ringSteps, avbd_handle::kRing);
+        double itTot = 0.0, stTot = 0.0;
+        for (long long k = h->ringSteps - n; k < h->ringSteps; ++k)
+        {
+            hipEvent_t *e = &h->ring[4 * (k % avbd_handle::kRing)];
+            float a = 0.0f, b = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&a, e[1], e[2]));
+            HIP_TRY(hipEventElapsedTime(&b, e[0], e[3]));
+            itTot += a;
+            stTot += b;
+        }
+        out->iterate_ms_total = itTot;
+        out->step_ms_total = stTot;
+        out->timed_steps = n;
+    }
     if (c.barrierTimeout || c.overflow)
         return check_device_errors(h);
+    return AVBD_OK;
+}
+
+__global__ void k_reset_cumulative(Counters *c)
+{
+    c->cumSteps = 0;
+    c->cumIterBytes = 0.0;
+    c->cumStepBytes = 0.0;
+}
+
+int avbd_reset_stats(avbd_handle *h)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_reset_stats: null handle");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    k_reset_cumulative<<<1, 1, 0, h->stream>>>(h->ctr);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->ringSteps = 0;
     return AVBD_OK;
 }
 

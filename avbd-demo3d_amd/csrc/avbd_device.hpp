@@ -99,7 +99,16 @@ struct Counters
     int barrierTimeout;
     int solvedCount;
     int contactCount;
+    int contactEndpoints;   // sum over contacts of their dynamic endpoints
+    int jointLive;
+    int jointEndpoints;
+    int springEndpoints;    // fixed by the uploaded springs
     long long nextSerial;
+    long long cumSteps;
+    double lastIterBytes;
+    double lastStepBytes;
+    double cumIterBytes;
+    double cumStepBytes;
     unsigned int colorChanged[3];
     unsigned int pad_;
 };

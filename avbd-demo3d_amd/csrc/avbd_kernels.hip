@@ -259,6 +259,9 @@ __global__ void k_step_begin(Counters *c)
     c->coloringRounds = 0;
     c->solvedCount = 0;
     c->contactCount = 0;
+    c->contactEndpoints = 0;
+    c->jointLive = 0;
+    c->jointEndpoints = 0;
     c->colorChanged[0] = c->colorChanged[1] = c->colorChanged[2] = 0u;
 }
 
@@ -982,18 +985,21 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
     }
 }
 
-__global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *keep, const int *keepRank, Counters *ctr)
+__global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *keep, const int *keepRank,
+                                    const float4 *massMoment, Counters *ctr)
 {
     int total = ctr->numTotal;
-    int contacts = 0;
+    int contacts = 0, endpoints = 0;
     for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < total; m += gridDim.x * blockDim.x)
     {
         if (!keep[m])
             continue;
         int d = keepRank[m];
         int n = tmp.count[m];
+        int2 ab = tmp.ab[m];
         contacts += n;
-        cur.ab[d] = tmp.ab[m];
+        endpoints += n * ((massMoment[ab.x].x > 0.0f ? 1 : 0) + (massMoment[ab.y].x > 0.0f ? 1 : 0));
+        cur.ab[d] = ab;
         cur.serial[d] = tmp.serial[m];
         cur.count[d] = n;
         cur.basis0[d] = tmp.basis0[m];
@@ -1011,14 +1017,18 @@ __global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *kee
         }
     }
     contacts = wave_sum(contacts);
+    endpoints = wave_sum(endpoints);
     if ((threadIdx.x & 63) == 0 && contacts)
+    {
         atomicAdd(&ctr->contactCount, contacts);
+        atomicAdd(&ctr->contactEndpoints, endpoints);
+    }
 }
 
 __global__ void k_compact_finish(Counters *ctr) { ctr->numManifolds = ctr->numKept; }
 
 // Joint::initialize (joint.cpp:35-54); a broken joint is removed (dead) here.
-__global__ void k_joint_init(Joint *joints, int nj, Bodies B, Params P)
+__global__ void k_joint_init(Joint *joints, int nj, Bodies B, Params P, Counters *ctr)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nj)
@@ -1050,6 +1060,34 @@ __global__ void k_joint_init(Joint *joints, int nj, Bodies B, Params P)
     joints[k].penAng = st3(pa2, 0.0f);
     if (j.broken)
         joints[k].dead = 1;
+    else
+    {
+        int e = (j.a >= 0 && B.massMoment[j.a].x > 0.0f ? 1 : 0) + (B.massMoment[j.b].x > 0.0f ? 1 : 0);
+        atomicAdd(&ctr->jointLive, 1);
+        atomicAdd(&ctr->jointEndpoints, e);
+    }
+}
+
+// Algorithmic HBM bytes of this step from its own counts, with the reference's
+// logical record sizes (body pose 28 B, contact 108 B read / 28 B dual write,
+// joint 124 B, spring 40 B; per-endpoint reads of the partner poses):
+//   iteration = Ns*(100+28) + sum_c [e_c*(108+56) + (108+2*56+28)]
+//             + sum_j [e_j*(124+28) + (124+2*28+52)] + sum_s e_s*(40+28)
+//   step      = iters*iteration + N*(148+92+16+64+8) + hits*(80+8) + 36*M + 130*contacts
+__global__ void k_step_bytes(Counters *c, int nBodies, int iterations)
+{
+    double ns = c->solvedCount;
+    double iter = ns * 128.0 + (double)c->contactEndpoints * 164.0 + (double)c->contactCount * 248.0 +
+                  (double)c->jointEndpoints * 152.0 + (double)c->jointLive * 232.0 +
+                  (double)c->springEndpoints * 68.0;
+    double it = (double)iterations * iter;
+    double step = it + (double)nBodies * 328.0 + (double)c->numHits * 88.0 + 36.0 * (double)c->numManifolds +
+                  130.0 * (double)c->contactCount;
+    c->lastIterBytes = it;
+    c->lastStepBytes = step;
+    c->cumIterBytes += it;
+    c->cumStepBytes += step;
+    c->cumSteps += 1;
 }
 
 // ------------------------------------------------------------------------------------------

@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""AVBD step throughput on one MI355X: the 50,000-box pile at 10 iterations.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one full resident AVBD step (sweep-and-prune broadphase, box-box
+narrowphase and warm start, colouring, 10 primal/dual iterations, velocity
+update) over the whole pile, with the state resident in HBM. The timed region
+is K steps after W warm-up steps, bracketed by device synchronisation.
+
+Printed JSON line:
+  value        dynamic bodies x K / wall seconds (body-steps/s)
+  roofline     the iteration kernel (k_iterate): algorithmic bytes per launch
+               (from the step's own counts, see avbd_stats.iterate_bytes) over
+               its average HIP-event duration, against 8 TB/s HBM
+  cpu_baseline the reference CPU solver (oracle/_ref, its own WorkerPool) on a
+               bounded sample of the same workload: the GPU's settled state
+               after the timed region, stepped for ~15 s of CPU time
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "avbd-demo3d_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_iterate_pile50k_traffic.txt")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per k_iterate launch from the committed rocprofv3 PMC summary, if present."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(state, params, target_s):
+    import oracle.bindings as ob
+    n_dyn = state.num_dynamic
+    if ob.ref_available():
+        world = ob.RefWorld(params, state=state)
+        kind = "reference"
+        workers = min(7, max((os.cpu_count() or 1) - 1, 0))
+        cores = workers + 1
+        run = lambda n: world.time_steps(n, colored=False)
+        what = "reference Solver::step (oracle/_ref, SAP broadphase)"
+    else:
+        world = ob.OracleWorld(state, params, order=ob.OracleWorld.ORDER_LIST)
+        kind = "port"
+        cores = 1
+        run = world.time_steps
+        what = "CPU oracle (single thread)"
+    t1 = run(1)
+    n = max(1, min(200, int(target_s / max(t1, 1e-6))))
+    t = run(n)
+    return {"value": n_dyn * n / t, "unit": "body-steps/s", "cores": cores, "kind": kind,
+            "sample": f"{what}: {n} steps of the {len(state.bodies)}-body pile from the GPU state after the "
+                      f"timed region ({t:.1f} s; broadphase/force init on {cores} threads, primal/dual loops "
+                      f"single-threaded as in the reference)"}
+
+
+def calibrate_copy(mib):
+    """A plain device-to-device copy of known size (for scaling PMC FETCH/WRITE counters)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB x5")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--bodies", type=int, default=50000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
+    args = ap.parse_args()
+
+    world_size = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world_size != 1:
+        print("bench.py measures one GPU: the AVBD step here runs on a single MI355X "
+              "(worlds shard without communication; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.init()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
+
+    from avbd import capi, scenes
+    from avbd.state import default_params
+
+    sc = scenes.pile(args.bodies)
+    params = default_params(gravity=sc.gravity)
+    log(f"building {sc.name}: {sc.num_bodies} bodies")
+    state = scenes.build_state(sc, capi.Maker())
+    world = capi.HipWorld.for_state(state, params, timing=True)
+    n_dyn = state.num_dynamic
+
+    log(f"warmup {args.warmup} steps")
+    world.step(args.warmup)
+    world.synchronize()
+    torch.cuda.synchronize()
+    world.reset_stats()
+    log(f"timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    chunk = 100
+    while done < args.steps:
+        k = min(chunk, args.steps - done)
+        world.step(k)
+        done += k
+    world.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = world.stats()
+    log(f"timed region {wall:.3f} s; last step: {int(st['manifold_count'])} manifolds, "
+        f"{int(st['contact_count'])} contacts, {int(st['color_count'])} colours")
+
+    iter_ms = float(st["iterate_ms_total"]) / max(int(st["timed_steps"]), 1)
+    iter_bytes = float(st["iterate_bytes_total"]) / max(int(st["counted_steps"]), 1)
+    achieved = iter_bytes / (iter_ms * 1e-3) / 1e9
+    step_bytes = float(st["step_bytes_total"]) / max(int(st["counted_steps"]), 1)
+
+    result = {
+        "metric": "body-steps/s @50k boxes,10 iters; %HBM peak; batched & adjoint vs plain",
+        "value": n_dyn * args.steps / wall,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"pile{args.bodies}: {args.bodies} boxes (half-extents U[0.25,1] m, random orientations, "
+                        f"v~N(0,0.5) m/s) on a jittered 50x40x25 grid dropping onto ground inside 4 walls; "
+                        f"dt=1/60, 10 iterations, friction 0.5, gravity -9.81; steps {args.warmup + 1}-"
+                        f"{args.warmup + args.steps} timed",
+            "bodies": int(n_dyn),
+            "iterations": int(params["iterations"]),
+            "parallelism": "single GPU",
+            "step_bytes_per_step": step_bytes,
+            "iterate_ms_per_step": iter_ms,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_iterate",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(),
+            "bytes_per_launch": iter_bytes,
+            "launch_ms": iter_ms,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        log("CPU baseline")
+        final = world.state()
+        result["cpu_baseline"] = cpu_baseline(final, params, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -176,6 +176,19 @@ typedef struct avbd_stats {
     float iterate_ms;
     float finish_ms;
     float step_ms;
+    /* Algorithmic HBM bytes of the last step: the iteration loop alone and the
+     * whole step, from the step's own counts (solved bodies, contacts and their
+     * dynamic endpoints, joints, springs, candidate pairs, manifolds). */
+    double iterate_bytes;
+    double step_bytes;
+    /* Sums since avbd_create / avbd_reset_stats. The *_ms_total sums need
+     * enable_timing and cover the last min(timed_steps, 4096) steps. */
+    double iterate_bytes_total;
+    double step_bytes_total;
+    double iterate_ms_total;
+    double step_ms_total;
+    int64_t counted_steps;
+    int64_t timed_steps;
 } avbd_stats;
 
 typedef struct avbd_handle avbd_handle;
@@ -240,6 +253,8 @@ int avbd_finish(avbd_handle *h);
 
 /* Introspection of the last step (for parity tests and the host adapter). */
 int avbd_get_stats(avbd_handle *h, avbd_stats *out);
+/* Zero the *_total sums of avbd_stats (synchronises the handle's stream). */
+int avbd_reset_stats(avbd_handle *h);
 /* Per body (list order): color of the last step, -1 for unsolved bodies. */
 int avbd_get_colors(avbd_handle *h, int32_t *colors, int32_t capacity);
 /* Bounding-sphere hits of the last broadphase, in sweep emission order, as
