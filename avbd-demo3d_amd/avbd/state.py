@@ -51,7 +51,9 @@ STATS = np.dtype(
      ("narrowphase_ms", "<f4"), ("coloring_ms", "<f4"), ("iterate_ms", "<f4"),
      ("finish_ms", "<f4"), ("step_ms", "<f4"), ("iterate_bytes", "<f8"), ("step_bytes", "<f8"),
      ("iterate_bytes_total", "<f8"), ("step_bytes_total", "<f8"), ("iterate_ms_total", "<f8"),
-     ("step_ms_total", "<f8"), ("counted_steps", "<i8"), ("timed_steps", "<i8")], align=True)
+     ("step_ms_total", "<f8"), ("counted_steps", "<i8"), ("timed_steps", "<i8"),
+     ("primal_bytes_total", "<f8"), ("dual_bytes_total", "<f8"), ("primal_ms_total", "<f8"),
+     ("dual_ms_total", "<f8"), ("primal_launches", "<i8"), ("dual_launches", "<i8")], align=True)
 
 CONFIG = np.dtype(
     [("params", PARAMS), ("max_bodies", "<i4"), ("max_joints", "<i4"), ("max_springs", "<i4"),

@@ -61,11 +61,21 @@ struct avbd_handle
     int *sweepEnd = nullptr;
     int *radixHistScan = nullptr;
     int *adjTotal = nullptr;
+    int maxStamps = 0;
+    // pinned host copy of the counters and colour offsets, read once per step before the
+    // iteration launches (their grid sizes depend on the colour and contact counts)
+    Counters *hostCtr = nullptr;
+    int *hostOffsets = nullptr;
     std::vector<void *> allocs;
     hipEvent_t ev[8] = {};
     // timing ring: per step, events at step start / before iterate / after iterate / step end
-    static constexpr int kRing = 4096;
+    static constexpr int kRing = 2048;
+    static constexpr int kIterSlots = 32;          // per-iteration primal/dual marks kept per step
+    static constexpr int kSlots = 4 + 2 * kIterSlots;
     std::vector<hipEvent_t> ring;
+    std::vector<int> ringIters;                    // iterations marked in each ring step
+    std::vector<int> ringPrimalLaunches;           // primal launches in each ring step
+    long long primalLaunches = 0, dualLaunches = 0;
     long long ringSteps = 0;   // steps recorded into the ring since the last reset
     avbd_stats stats{};
     long long stepsTaken = 0;
@@ -202,6 +212,7 @@ int check_device_errors(avbd_handle *h)
                            : (c.overflow & 4) ? "adjacency buffer"
                            : (c.overflow & 8) ? "colour count (more than 254 colours)"
                            : (c.overflow & 16) ? "constrained-pair hash"
+                           : (c.overflow & 64) ? "primal stamp buffer"
                                                : "sweep work-item buffer";
         return fail((c.overflow & 8) ? AVBD_ERR_UNSUPPORTED : AVBD_ERR_CAPACITY,
                     std::string("capacity exceeded: ") + what);
@@ -219,7 +230,7 @@ int enqueue_prepare(avbd_handle *h)
     if (h->timing)
     {
         HIP_TRY(hipEventRecord(h->ev[0], s));
-        HIP_TRY(hipEventRecord(h->ring[4 * (h->ringSteps % avbd_handle::kRing) + 0], s));
+        HIP_TRY(hipEventRecord(h->ring[avbd_handle::kSlots * (h->ringSteps % avbd_handle::kRing) + 0], s));
     }
     k_step_begin<<<1, 1, 0, s>>>(ctr);
     k_barrier_reset<<<1, 1, 0, s>>>(h->gb);
@@ -227,7 +238,8 @@ int enqueue_prepare(avbd_handle *h)
     // ---- broadphase: sweep and prune (solver.cpp:488-577) ----
     if (N >= 2)
     {
-        k_axis<<<1, 256, 0, s>>>(h->B.pos, N, ctr);
+        k_axis_partial<<<64, 256, 0, s>>>(h->B.pos, N, S.axisSums);
+        k_axis_decide<<<1, 64, 0, s>>>(S.axisSums, 64, N, ctr);
         k_axis_serial<<<1, 64, 0, s>>>(h->B.pos, N, ctr);
         k_sap_keys<<<ceil_div(N, 256), 256, 0, s>>>(h->B.pos, h->B.sizeRad, N, ctr, S.keys, S.vals);
         unsigned *keys = S.keys, *keysAlt = S.keysAlt;
@@ -256,6 +268,8 @@ int enqueue_prepare(avbd_handle *h)
     scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numKept);
     k_compact_manifolds<<<G, 256, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr);
     k_compact_finish<<<1, 1, 0, s>>>(ctr);
+    scan(h, h->cur.count, S.contactStart, &ctr->numManifolds, 0, &ctr->contactCount);
+    k_contact_owner<<<G, 256, 0, s>>>(h->cur.count, S.contactStart, ctr, S.contactOwner);
     if (h->nj > 0)
         k_joint_init<<<ceil_div(h->nj, 256), 256, 0, s>>>(h->joints, h->nj, h->B, h->P, ctr);
     if (h->timing)
@@ -265,26 +279,27 @@ int enqueue_prepare(avbd_handle *h)
     k_fill_int<<<grid_for(N), 256, 0, s>>>(S.deg, N, 0);
     k_fill_int<<<grid_for(N), 256, 0, s>>>(S.adjCursor, N, 0);
     k_adj<false><<<G, 256, 0, s>>>(h->cur, ctr, h->joints, h->nj, h->springs, h->ns, h->B.massMoment, S.deg, S.adjStart,
-                                   S.adjCursor, S.adj, S.adjSerial, h->maxAdj, &ctr->overflow);
+                                   S.adjCursor, S.adj, S.adjSerial, S.adjPartner, h->maxAdj, &ctr->overflow);
     scan(h, S.deg, S.adjStart, nullptr, N, h->adjTotal);
     k_adj<true><<<G, 256, 0, s>>>(h->cur, ctr, h->joints, h->nj, h->springs, h->ns, h->B.massMoment, S.deg, S.adjStart,
-                                  S.adjCursor, S.adj, S.adjSerial, h->maxAdj, &ctr->overflow);
-    k_adj_sort<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjSerial, N, h->maxAdj);
+                                  S.adjCursor, S.adj, S.adjSerial, S.adjPartner, h->maxAdj, &ctr->overflow);
+    k_adj_sort<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjSerial, S.adjPartner, N, h->maxAdj);
+    k_stamp_count<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, h->cur.count, N, S.stampCount);
+    scan(h, S.stampCount, S.stampStart, nullptr, N, &ctr->numStamps);
+    k_stamp_fill<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, h->cur.count, N, S.stampStart, S.stamps,
+                                                  h->maxStamps, &ctr->overflow);
     k_warmstart<<<ceil_div(N, 256), 256, 0, s>>>(h->B, h->P, S.deg, N, S.color, ctr);
     k_step_bytes<<<1, 1, 0, s>>>(ctr, N, h->P.iterations);
 
     // ---- colouring (avbd_gpu_solver.cpp:1812-1865) ----
     {
-        const int *deg = S.deg, *adjStart = S.adjStart, *adj = S.adj;
-        const int2 *mab = h->cur.ab;
-        const Joint *jt = h->joints;
-        const Spring *sp = h->springs;
+        const int *deg = S.deg, *adjStart = S.adjStart, *adjPartner = S.adjPartner;
         const float4 *mm = h->B.massMoment;
         int n = N, maxRounds = N + 2;
         int *color = S.color;
         GridBarrier *gb = h->gb;
-        void *args[] = {(void *)&deg, (void *)&adjStart, (void *)&adj, (void *)&mab, (void *)&jt, (void *)&sp,
-                        (void *)&mm, (void *)&n, (void *)&color, (void *)&ctr, (void *)&gb, (void *)&maxRounds};
+        void *args[] = {(void *)&deg, (void *)&adjStart, (void *)&adjPartner, (void *)&mm, (void *)&n, (void *)&color,
+                        (void *)&ctr, (void *)&gb, (void *)&maxRounds};
         int r = launch_coop((const void *)k_color, h->coopColor, args, s);
         if (r)
             return r;
@@ -302,32 +317,62 @@ int enqueue_prepare(avbd_handle *h)
     if (h->timing)
     {
         HIP_TRY(hipEventRecord(h->ev[3], s));
-        HIP_TRY(hipEventRecord(h->ring[4 * (h->ringSteps % avbd_handle::kRing) + 1], s));
+        HIP_TRY(hipEventRecord(h->ring[avbd_handle::kSlots * (h->ringSteps % avbd_handle::kRing) + 1], s));
     }
     return AVBD_OK;
 }
 
+// Solver iterations: per colour one primal launch, then one dual launch. The colour and
+// contact counts are read back once per step to size the launches.
 int enqueue_iterate(avbd_handle *h)
 {
-    Bodies B = h->B;
-    Manifolds M = h->cur;
-    Joint *jt = h->joints;
-    int nj = h->nj;
-    const Spring *sp = h->springs;
-    const int *deg = h->S.deg, *adjStart = h->S.adjStart, *adj = h->S.adj;
-    const int *co = h->S.colorOffsets, *cb = h->S.colorBodies;
-    Params P = h->P;
-    Counters *ctr = h->ctr;
-    GridBarrier *gb = h->gb;
-    void *args[] = {(void *)&B, (void *)&M, (void *)&jt, (void *)&nj, (void *)&sp, (void *)&deg, (void *)&adjStart,
-                    (void *)&adj, (void *)&co, (void *)&cb, (void *)&P, (void *)&ctr, (void *)&gb};
-    int r = launch_coop((const void *)k_iterate, h->coopIterate, args, h->stream);
-    if (r)
-        return r;
+    hipStream_t s = h->stream;
+    HIP_TRY(hipMemcpyAsync(h->hostCtr, h->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h->hostOffsets, h->S.colorOffsets, sizeof(int) * 257, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    const Counters &c = *h->hostCtr;
+    if (c.overflow || c.barrierTimeout)
+        return check_device_errors(h);
+    const int colors = std::min(c.colorCount, kMaxColors);
+    const int dualLanes = c.contactCount + h->nj;
+    Scratch &S = h->S;
+    hipEvent_t *mark = h->timing ? &h->ring[avbd_handle::kSlots * (h->ringSteps % avbd_handle::kRing)] : nullptr;
+    int marked = 0, primalPerStep = 0;
+    for (int it = 0; it < h->P.iterations; ++it)
+    {
+        for (int col = 0; col < colors; ++col)
+        {
+            int n = h->hostOffsets[col + 1] - h->hostOffsets[col];
+            if (n > 0)
+            {
+                k_primal_color<<<ceil_div(n, kGroupsPerBlock), 256, 0, s>>>(h->B, h->cur, h->joints, h->springs,
+                                                                             S.stampStart, S.stampCount, S.stamps,
+                                                                             S.colorOffsets, S.colorBodies, h->P, col);
+                ++primalPerStep;
+            }
+        }
+        if (mark && it < avbd_handle::kIterSlots)
+            HIP_TRY(hipEventRecord(mark[4 + 2 * it], s));
+        if (dualLanes > 0)
+        {
+            k_dual<<<ceil_div(dualLanes, 256), 256, 0, s>>>(h->B, h->cur, h->joints, h->nj, S.contactOwner,
+                                                            S.contactStart, h->ctr, h->P);
+            h->dualLaunches++;
+        }
+        if (mark && it < avbd_handle::kIterSlots)
+        {
+            HIP_TRY(hipEventRecord(mark[5 + 2 * it], s));
+            marked = it + 1;
+        }
+    }
+    h->primalLaunches += primalPerStep;
     if (h->timing)
     {
-        HIP_TRY(hipEventRecord(h->ev[4], h->stream));
-        HIP_TRY(hipEventRecord(h->ring[4 * (h->ringSteps % avbd_handle::kRing) + 2], h->stream));
+        HIP_TRY(hipEventRecord(h->ev[4], s));
+        HIP_TRY(hipEventRecord(mark[2], s));
+        h->ringIters[h->ringSteps % avbd_handle::kRing] = marked;
+        h->ringPrimalLaunches[h->ringSteps % avbd_handle::kRing] =
+            marked == h->P.iterations ? primalPerStep : primalPerStep * marked / std::max(h->P.iterations, 1);
     }
     return AVBD_OK;
 }
@@ -339,7 +384,7 @@ int enqueue_finish(avbd_handle *h)
     if (h->timing)
     {
         HIP_TRY(hipEventRecord(h->ev[5], h->stream));
-        HIP_TRY(hipEventRecord(h->ring[4 * (h->ringSteps % avbd_handle::kRing) + 3], h->stream));
+        HIP_TRY(hipEventRecord(h->ring[avbd_handle::kSlots * (h->ringSteps % avbd_handle::kRing) + 3], h->stream));
         h->ringSteps++;
     }
     h->stepsTaken++;
@@ -480,6 +525,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     // one work item per 64 interval visits: room for ~40 * 64 = 2560 sweep checks per body
     h->maxChunks = 41 * N + h->maxPairs / 8 + 65536;
     h->maxAdj = 2 * (h->maxManifolds + h->maxJoints + h->maxSprings) + 16;
+    h->maxStamps = 2 * kMaxContacts * h->maxManifolds + 6 * h->maxJoints + 2 * h->maxSprings + 16;
     int hs = 1024;
     while (hs < 2 * (h->maxManifolds + h->maxJoints + h->maxSprings))
         hs <<= 1;
@@ -507,7 +553,10 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         return bail(r);
     Scratch &S = h->S;
     int tiles = ceil_div(N, kRadixTile);
-    if ((r = h->alloc(S.axisSums, 8)) || (r = h->alloc(S.keys, N)) || (r = h->alloc(S.keysAlt, N)) ||
+    if ((r = h->alloc(S.axisSums, 64 * 6)) || (r = h->alloc(S.adjPartner, h->maxAdj)) ||
+        (r = h->alloc(S.stampCount, N)) || (r = h->alloc(S.stampStart, N)) || (r = h->alloc(S.stamps, h->maxStamps)) ||
+        (r = h->alloc(S.contactStart, h->maxManifolds)) ||
+        (r = h->alloc(S.contactOwner, (size_t)kMaxContacts * h->maxManifolds)) || (r = h->alloc(S.keys, N)) || (r = h->alloc(S.keysAlt, N)) ||
         (r = h->alloc(S.vals, N)) || (r = h->alloc(S.valsAlt, N)) || (r = h->alloc(S.radixHist, 256 * tiles)) ||
         (r = h->alloc(h->radixHistScan, 256 * tiles)) || (r = h->alloc(S.sLo, N)) || (r = h->alloc(S.sHi, N)) ||
         (r = h->alloc(S.sPos, N)) || (r = h->alloc(S.sIdx, N)) || (r = h->alloc(S.chunkCount, N)) ||
@@ -522,6 +571,9 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         (r = h->alloc(h->adjTotal, 1)))
         return bail(r);
     S.hashMask = h->hashSize - 1;
+    if ((e = hipHostMalloc((void **)&h->hostCtr, sizeof(Counters))) != hipSuccess ||
+        (e = hipHostMalloc((void **)&h->hostOffsets, sizeof(int) * 257)) != hipSuccess)
+        return bail(fail(AVBD_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e)));
     if ((e = hipMemsetAsync(h->ctr, 0, sizeof(Counters), h->stream)) != hipSuccess ||
         (e = hipMemsetAsync(h->gb, 0, sizeof(GridBarrier), h->stream)) != hipSuccess ||
         (e = hipMemsetAsync(S.color, 0xFF, sizeof(int) * N, h->stream)) != hipSuccess)
@@ -530,20 +582,20 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     hipDeviceProp_t prop;
     if ((e = hipGetDeviceProperties(&prop, h->device)) != hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e)));
-    int perCuIt = 0, perCuCol = 0;
-    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCuIt, (const void *)k_iterate, 256, 0)) != hipSuccess ||
-        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCuCol, (const void *)k_color, 256, 0)) != hipSuccess)
+    int perCuCol = 0;
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCuCol, (const void *)k_color, 256, 0)) != hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipOccupancy: ") + hipGetErrorString(e)));
-    if (perCuIt < 1 || perCuCol < 1)
-        return bail(fail(AVBD_ERR_HIP, "cooperative kernels do not fit on a compute unit"));
-    h->coopIterate = prop.multiProcessorCount * std::min(perCuIt, 1);
-    h->coopColor = prop.multiProcessorCount * std::min(perCuCol, 1);
+    if (perCuCol < 1)
+        return bail(fail(AVBD_ERR_HIP, "the colouring kernel does not fit on a compute unit"));
+    h->coopColor = std::min(prop.multiProcessorCount, 64);
     for (int k = 0; k < 8; ++k)
         if ((e = hipEventCreate(&h->ev[k])) != hipSuccess)
             return bail(fail(AVBD_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e)));
     if (h->timing)
     {
-        h->ring.assign(4 * avbd_handle::kRing, nullptr);
+        h->ring.assign((size_t)avbd_handle::kSlots * avbd_handle::kRing, nullptr);
+        h->ringIters.assign(avbd_handle::kRing, 0);
+        h->ringPrimalLaunches.assign(avbd_handle::kRing, 0);
         for (hipEvent_t &ev : h->ring)
             if ((e = hipEventCreate(&ev)) != hipSuccess)
                 return bail(fail(AVBD_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e)));
@@ -563,6 +615,10 @@ int avbd_destroy(avbd_handle *h)
         (void)hipStreamSynchronize(h->stream);
     for (void *p : h->allocs)
         (void)hipFree(p);
+    if (h->hostCtr)
+        (void)hipHostFree(h->hostCtr);
+    if (h->hostOffsets)
+        (void)hipHostFree(h->hostOffsets);
     for (int k = 0; k < 8; ++k)
         if (h->ev[k])
             (void)hipEventDestroy(h->ev[k]);
@@ -1026,24 +1082,42 @@ int avbd_get_stats(avbd_handle *h, avbd_stats *out)
     out->iterate_bytes = c.lastIterBytes;
     out->step_bytes = c.lastStepBytes;
     out->iterate_bytes_total = c.cumIterBytes;
+    out->primal_bytes_total = c.cumPrimalBytes;
+    out->dual_bytes_total = c.cumDualBytes;
     out->step_bytes_total = c.cumStepBytes;
     out->counted_steps = c.cumSteps;
     if (h->timing && h->ringSteps > 0)
     {
         long long n = std::min<long long>(h->ringSteps, avbd_handle::kRing);
-        double itTot = 0.0, stTot = 0.0;
+        double itTot = 0.0, stTot = 0.0, prTot = 0.0, duTot = 0.0;
+        long long prLaunch = 0, duLaunch = 0;
         for (long long k = h->ringSteps - n; k < h->ringSteps; ++k)
         {
-            hipEvent_t *e = &h->ring[4 * (k % avbd_handle::kRing)];
+            hipEvent_t *e = &h->ring[avbd_handle::kSlots * (k % avbd_handle::kRing)];
             float a = 0.0f, b = 0.0f;
             HIP_TRY(hipEventElapsedTime(&a, e[1], e[2]));
             HIP_TRY(hipEventElapsedTime(&b, e[0], e[3]));
             itTot += a;
             stTot += b;
+            int iters = h->ringIters[k % avbd_handle::kRing];
+            for (int it = 0; it < iters; ++it)
+            {
+                float p = 0.0f, d = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&p, it == 0 ? e[1] : e[5 + 2 * (it - 1)], e[4 + 2 * it]));
+                HIP_TRY(hipEventElapsedTime(&d, e[4 + 2 * it], e[5 + 2 * it]));
+                prTot += p;
+                duTot += d;
+            }
+            prLaunch += h->ringPrimalLaunches[k % avbd_handle::kRing];
+            duLaunch += iters;
         }
         out->iterate_ms_total = itTot;
         out->step_ms_total = stTot;
         out->timed_steps = n;
+        out->primal_ms_total = prTot;
+        out->dual_ms_total = duTot;
+        out->primal_launches = prLaunch;
+        out->dual_launches = duLaunch;
     }
     if (c.barrierTimeout || c.overflow)
         return check_device_errors(h);
@@ -1055,6 +1129,8 @@ __global__ void k_reset_cumulative(Counters *c)
     c->cumSteps = 0;
     c->cumIterBytes = 0.0;
     c->cumStepBytes = 0.0;
+    c->cumPrimalBytes = 0.0;
+    c->cumDualBytes = 0.0;
 }
 
 int avbd_reset_stats(avbd_handle *h)

@@ -100,6 +100,7 @@ struct Counters
     int solvedCount;
     int contactCount;
     int contactEndpoints;   // sum over contacts of their dynamic endpoints
+    int numStamps;
     int jointLive;
     int jointEndpoints;
     int springEndpoints;    // fixed by the uploaded springs
@@ -109,6 +110,8 @@ struct Counters
     double lastStepBytes;
     double cumIterBytes;
     double cumStepBytes;
+    double cumPrimalBytes;
+    double cumDualBytes;
     unsigned int colorChanged[3];
     unsigned int pad_;
 };
@@ -152,6 +155,13 @@ struct Scratch
     int *adjCursor;         // [N]
     int *adj;               // [maxAdj] (kind << 30) | index
     long long *adjSerial;   // [maxAdj]
+    int *adjPartner;        // [maxAdj] the force's other body (-1: world)
+    // primal stamp lists and the per-contact dual index
+    int *stampCount;        // [N]
+    int *stampStart;        // [N]
+    int2 *stamps;           // [maxStamps] (adj entry, part)
+    int *contactStart;      // [maxManifolds]
+    int *contactOwner;      // [maxManifolds * 8]
     // colouring
     int *color;             // [N]
     int *colorOffsets;      // [256]

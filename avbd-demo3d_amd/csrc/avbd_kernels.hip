@@ -270,67 +270,62 @@ __global__ void k_step_begin(Counters *c)
 // (solver.cpp:499-515). The decision is taken in double; only when two variances lie within
 // the float sums' error bound of each other is the exact serial float pass run.
 // ------------------------------------------------------------------------------------------
-__global__ void k_axis(const float4 *pos, int n, Counters *ctr)
+__global__ void k_axis_partial(const float4 *pos, int n, double *partials)
 {
-    __shared__ double red[3][256];
-    double sx = 0, sy = 0, sz = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
+    __shared__ double red[6][256];
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
         float4 p = pos[i];
-        sx += p.x;
-        sy += p.y;
-        sz += p.z;
+        a[0] += p.x;
+        a[1] += p.y;
+        a[2] += p.z;
+        a[3] += (double)p.x * p.x;
+        a[4] += (double)p.y * p.y;
+        a[5] += (double)p.z * p.z;
     }
-    red[0][threadIdx.x] = sx;
-    red[1][threadIdx.x] = sy;
-    red[2][threadIdx.x] = sz;
+    for (int k = 0; k < 6; ++k)
+        red[k][threadIdx.x] = a[k];
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1)
+    for (int st = blockDim.x / 2; st > 0; st >>= 1)
     {
-        if ((int)threadIdx.x < s)
-            for (int k = 0; k < 3; ++k)
-                red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        if ((int)threadIdx.x < st)
+            for (int k = 0; k < 6; ++k)
+                red[k][threadIdx.x] += red[k][threadIdx.x + st];
         __syncthreads();
     }
-    double mx = red[0][0] / n, my = red[1][0] / n, mz = red[2][0] / n;
-    __syncthreads();
-    double vx = 0, vy = 0, vz = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
+    if (threadIdx.x < 6)
+        partials[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Decision on the summed statistics; flags the exact serial pass when two variances are within
+// the error bound of the reference's n-term serial float sums.
+__global__ void k_axis_decide(const double *partials, int nparts, int n, Counters *ctr)
+{
+    if (threadIdx.x != 0)
+        return;
+    double t[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nparts; ++b)
+        for (int k = 0; k < 6; ++k)
+            t[k] += partials[b * 6 + k];
+    double v[3];
+    for (int k = 0; k < 3; ++k)
     {
-        float4 p = pos[i];
-        double dx = p.x - mx, dy = p.y - my, dz = p.z - mz;
-        vx += dx * dx;
-        vy += dy * dy;
-        vz += dz * dz;
+        double mean = t[k] / n;
+        v[k] = t[k + 3] - n * mean * mean;
     }
-    red[0][threadIdx.x] = vx;
-    red[1][threadIdx.x] = vy;
-    red[2][threadIdx.x] = vz;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1)
-    {
-        if ((int)threadIdx.x < s)
-            for (int k = 0; k < 3; ++k)
-                red[k][threadIdx.x] += red[k][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-    {
-        double v[3] = {red[0][0], red[1][0], red[2][0]};
-        int best = 0;
-        if (v[1] > v[best])
-            best = 1;
-        if (v[2] > v[best])
-            best = 2;
-        // relative error bound of an n-term serial float sum of positive terms, with margin
-        double tol = 4.0 * ((double)n + 16.0) * 5.960464477539063e-08;
-        int amb = 0;
-        for (int k = 0; k < 3; ++k)
-            if (k != best && v[k] >= v[best] * (1.0 - tol))
-                amb = 1;
-        ctr->sweepAxis = best;
-        ctr->axisAmbiguous = amb;
-    }
+    int best = 0;
+    if (v[1] > v[best])
+        best = 1;
+    if (v[2] > v[best])
+        best = 2;
+    double tol = 4.0 * ((double)n + 16.0) * 5.960464477539063e-08 + 1e-9;
+    int amb = 0;
+    for (int k = 0; k < 3; ++k)
+        if (k != best && v[k] >= v[best] * (1.0 - tol))
+            amb = 1;
+    ctr->sweepAxis = best;
+    ctr->axisAmbiguous = amb;
 }
 
 __global__ void k_axis_serial(const float4 *pos, int n, Counters *ctr)
@@ -1018,11 +1013,9 @@ __global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *kee
     }
     contacts = wave_sum(contacts);
     endpoints = wave_sum(endpoints);
-    if ((threadIdx.x & 63) == 0 && contacts)
-    {
-        atomicAdd(&ctr->contactCount, contacts);
+    (void)contacts;
+    if ((threadIdx.x & 63) == 0 && endpoints)
         atomicAdd(&ctr->contactEndpoints, endpoints);
-    }
 }
 
 __global__ void k_compact_finish(Counters *ctr) { ctr->numManifolds = ctr->numKept; }
@@ -1077,10 +1070,13 @@ __global__ void k_joint_init(Joint *joints, int nj, Bodies B, Params P, Counters
 __global__ void k_step_bytes(Counters *c, int nBodies, int iterations)
 {
     double ns = c->solvedCount;
-    double iter = ns * 128.0 + (double)c->contactEndpoints * 164.0 + (double)c->contactCount * 248.0 +
-                  (double)c->jointEndpoints * 152.0 + (double)c->jointLive * 232.0 +
-                  (double)c->springEndpoints * 68.0;
+    double primal = ns * 128.0 + (double)c->contactEndpoints * 164.0 + (double)c->jointEndpoints * 152.0 +
+                    (double)c->springEndpoints * 68.0;
+    double dual = (double)c->contactCount * 248.0 + (double)c->jointLive * 232.0;
+    double iter = primal + dual;
     double it = (double)iterations * iter;
+    c->cumPrimalBytes += (double)iterations * primal;
+    c->cumDualBytes += (double)iterations * dual;
     double step = it + (double)nBodies * 328.0 + (double)c->numHits * 88.0 + 36.0 * (double)c->numManifolds +
                   130.0 * (double)c->contactCount;
     c->lastIterBytes = it;
@@ -1096,7 +1092,7 @@ __global__ void k_step_bytes(Counters *c, int nBodies, int iterations)
 template <bool FILL>
 __global__ void k_adj(Manifolds M, const Counters *ctr, const Joint *joints, int nj, const Spring *springs, int ns,
                       const float4 *massMoment, int *deg, const int *adjStart, int *cursor, int *adj,
-                      long long *adjSerial, int maxAdj, int *overflow)
+                      long long *adjSerial, int *adjPartner, int maxAdj, int *overflow)
 {
     int nm = ctr->numManifolds;
     int total = nm + nj + ns;
@@ -1148,6 +1144,7 @@ __global__ void k_adj(Manifolds M, const Counters *ctr, const Joint *joints, int
                 {
                     adj[slot] = adjEncode(kind, idx);
                     adjSerial[slot] = serial;
+                    adjPartner[slot] = ends[1 - e];
                 }
                 else
                     atomicOr(overflow, 4);
@@ -1156,7 +1153,8 @@ __global__ void k_adj(Manifolds M, const Counters *ctr, const Joint *joints, int
     }
 }
 
-__global__ void k_adj_sort(const int *deg, const int *adjStart, int *adj, long long *adjSerial, int n, int maxAdj)
+__global__ void k_adj_sort(const int *deg, const int *adjStart, int *adj, long long *adjSerial, int *adjPartner, int n,
+                           int maxAdj)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -1165,16 +1163,18 @@ __global__ void k_adj_sort(const int *deg, const int *adjStart, int *adj, long l
     for (int x = b + 1; x < e; ++x)
     {
         long long s = adjSerial[x];
-        int v = adj[x];
+        int v = adj[x], pv = adjPartner[x];
         int y = x - 1;
         while (y >= b && adjSerial[y] < s)
         {
             adjSerial[y + 1] = adjSerial[y];
             adj[y + 1] = adj[y];
+            adjPartner[y + 1] = adjPartner[y];
             --y;
         }
         adjSerial[y + 1] = s;
         adj[y + 1] = v;
+        adjPartner[y + 1] = pv;
     }
 }
 
@@ -1225,32 +1225,8 @@ __global__ void k_warmstart(Bodies B, Params P, const int *deg, int n, int *colo
 // (avbd_gpu_solver.cpp:1812-1865); iterating it from any start converges to that unique fixpoint
 // in at most (longest index-increasing path) rounds. Warm-started from the previous step's colours.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ int partner_of(int e, int i, const int2 *mab, const Joint *joints, const Spring *springs)
-{
-    int kind = adjKind(e), idx = adjIndex(e);
-    int a, b;
-    if (kind == kAdjManifold)
-    {
-        int2 p = mab[idx];
-        a = p.x;
-        b = p.y;
-    }
-    else if (kind == kAdjJoint)
-    {
-        a = joints[idx].a;
-        b = joints[idx].b;
-    }
-    else
-    {
-        a = springs[idx].a;
-        b = springs[idx].b;
-    }
-    return a == i ? b : a;
-}
-
-__global__ void k_color(const int *deg, const int *adjStart, const int *adj, const int2 *mab, const Joint *joints,
-                        const Spring *springs, const float4 *massMoment, int n, int *color, Counters *ctr,
-                        GridBarrier *gb, int maxRounds)
+__global__ void k_color(const int *deg, const int *adjStart, const int *adjPartner, const float4 *massMoment, int n,
+                        int *color, Counters *ctr, GridBarrier *gb, int maxRounds)
 {
     const unsigned nblocks = gridDim.x;
     __shared__ int anyChange;
@@ -1273,7 +1249,7 @@ __global__ void k_color(const int *deg, const int *adjStart, const int *adj, con
             int b = adjStart[i], e = b + deg[i];
             for (int x = b; x < e; ++x)
             {
-                int o = partner_of(adj[x], i, mab, joints, springs);
+                int o = adjPartner[x];
                 if (o < 0 || o >= i)
                     continue;
                 int c = __hip_atomic_load(&color[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1342,13 +1318,22 @@ __global__ void k_color_offsets(const int *scannedHist, int tiles, int n, int *c
 
 // ------------------------------------------------------------------------------------------
 // primal / dual (solver.cpp:1177-1271)
+//
+// A body's primal update is its inertia term plus the Hessian/gradient stamps of its forces,
+// taken in its force order, followed by a 6x6 LDL^T solve. The stamps of one body are
+// independent of each other, so a group of kGroupLanes lanes computes them in parallel (one
+// 27-float stamp record per lane, staged in LDS); then each lane owns one or two of the 27
+// system entries and adds the records in their sequential order. Every entry therefore sees
+// the same sequence of FP32 additions as the reference's single-threaded loop.
 // ------------------------------------------------------------------------------------------
-struct Lhs
+constexpr int kGroupLanes = 16;
+constexpr int kGroupsPerBlock = 256 / kGroupLanes;
+constexpr int kRec = 27; // lhsLin lower 6 | lhsAng lower 6 | lhsCross 9 | rhsLin 3 | rhsAng 3
+enum
 {
-    float l00, l10, l11, l20, l21, l22;
-    float a00, a10, a11, a20, a21, a22;
-    float c[3][3];
-    V3 rl, ra;
+    R_L00 = 0, R_L10, R_L11, R_L20, R_L21, R_L22,
+    R_A00, R_A10, R_A11, R_A20, R_A21, R_A22,
+    R_C = 12, R_RL = 21, R_RA = 24
 };
 
 // Contact force with normal clamp and friction cone (manifold.cpp:203-216).
@@ -1385,11 +1370,12 @@ __device__ __forceinline__ ContactEval contact_eval(V3 c0, V3 pen, V3 lam, float
     return r;
 }
 
-// One contact's Hessian/gradient stamp for one side: rows J_L (l0..l2), J_A (g0..g2), K = diag(k).
-__device__ __forceinline__ void stamp_rows(Lhs &s, V3 l0, V3 l1, V3 l2, V3 g0, V3 g1, V3 g2, V3 k, float F0, float F1,
-                                          float F2)
+// Stamp record of one Jacobian block: rows J_L (l0..l2), J_A (g0..g2), K = diag(k), force F.
+// lhsLin += (J_L^T K) J_L, lhsAng += (J_A^T K) J_A, lhsCross += (J_A^T K) J_L,
+// rhsLin += J_L^T F, rhsAng += J_A^T F  (manifold.cpp:218-228)
+__device__ __forceinline__ void rows_record(float *r, V3 l0, V3 l1, V3 l2, V3 g0, V3 g1, V3 g2, V3 k, float F0,
+                                            float F1, float F2)
 {
-    // T = J_L^T K, TA = J_A^T K (row i = component i of the Jacobian rows, scaled per row)
     float T[3][3] = {{l0.x * k.x, l1.x * k.y, l2.x * k.z}, {l0.y * k.x, l1.y * k.y, l2.y * k.z},
                      {l0.z * k.x, l1.z * k.y, l2.z * k.z}};
     float TA[3][3] = {{g0.x * k.x, g1.x * k.y, g2.x * k.z}, {g0.y * k.x, g1.y * k.y, g2.y * k.z},
@@ -1397,60 +1383,61 @@ __device__ __forceinline__ void stamp_rows(Lhs &s, V3 l0, V3 l1, V3 l2, V3 g0, V
     float L[3][3] = {{l0.x, l0.y, l0.z}, {l1.x, l1.y, l1.z}, {l2.x, l2.y, l2.z}};
     float G[3][3] = {{g0.x, g0.y, g0.z}, {g1.x, g1.y, g1.z}, {g2.x, g2.y, g2.z}};
 #define AVBD_PROD(A, i, B, j) ((A[i][0] * B[0][j] + A[i][1] * B[1][j]) + A[i][2] * B[2][j])
-    s.l00 += AVBD_PROD(T, 0, L, 0);
-    s.l10 += AVBD_PROD(T, 1, L, 0);
-    s.l11 += AVBD_PROD(T, 1, L, 1);
-    s.l20 += AVBD_PROD(T, 2, L, 0);
-    s.l21 += AVBD_PROD(T, 2, L, 1);
-    s.l22 += AVBD_PROD(T, 2, L, 2);
-    s.a00 += AVBD_PROD(TA, 0, G, 0);
-    s.a10 += AVBD_PROD(TA, 1, G, 0);
-    s.a11 += AVBD_PROD(TA, 1, G, 1);
-    s.a20 += AVBD_PROD(TA, 2, G, 0);
-    s.a21 += AVBD_PROD(TA, 2, G, 1);
-    s.a22 += AVBD_PROD(TA, 2, G, 2);
+    r[R_L00] = AVBD_PROD(T, 0, L, 0);
+    r[R_L10] = AVBD_PROD(T, 1, L, 0);
+    r[R_L11] = AVBD_PROD(T, 1, L, 1);
+    r[R_L20] = AVBD_PROD(T, 2, L, 0);
+    r[R_L21] = AVBD_PROD(T, 2, L, 1);
+    r[R_L22] = AVBD_PROD(T, 2, L, 2);
+    r[R_A00] = AVBD_PROD(TA, 0, G, 0);
+    r[R_A10] = AVBD_PROD(TA, 1, G, 0);
+    r[R_A11] = AVBD_PROD(TA, 1, G, 1);
+    r[R_A20] = AVBD_PROD(TA, 2, G, 0);
+    r[R_A21] = AVBD_PROD(TA, 2, G, 1);
+    r[R_A22] = AVBD_PROD(TA, 2, G, 2);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            s.c[i][j] += AVBD_PROD(TA, i, L, j);
+            r[R_C + 3 * i + j] = AVBD_PROD(TA, i, L, j);
 #undef AVBD_PROD
-    s.rl += V3{(L[0][0] * F0 + L[1][0] * F1) + L[2][0] * F2, (L[0][1] * F0 + L[1][1] * F1) + L[2][1] * F2,
-               (L[0][2] * F0 + L[1][2] * F1) + L[2][2] * F2};
-    s.ra += V3{(G[0][0] * F0 + G[1][0] * F1) + G[2][0] * F2, (G[0][1] * F0 + G[1][1] * F1) + G[2][1] * F2,
-               (G[0][2] * F0 + G[1][2] * F1) + G[2][2] * F2};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        r[R_RL + i] = (L[0][i] * F0 + L[1][i] * F1) + L[2][i] * F2;
+        r[R_RA + i] = (G[0][i] * F0 + G[1][i] * F1) + G[2][i] * F2;
+    }
 }
 
-__device__ __forceinline__ void stamp_manifold(Lhs &s, int i, int m, const Manifolds &M, const Bodies &B, float oma)
+// One contact of a manifold as seen by body i (Manifold::updatePrimal, manifold.cpp:176-229).
+__device__ __forceinline__ void contact_record(float *r, int i, int m, int c, const Manifolds &M, const Bodies &B,
+                                               float oma)
 {
     int2 ab = M.ab[m];
-    bool isA = ab.x == i;
-    V3 pA = ld3(B.pos[ab.x]), pB = ld3(B.pos[ab.y]);
-    Q4 qA = ldq(B.rot[ab.x]), qB = ldq(B.rot[ab.y]);
-    V3 dqAL = pA - ld3(B.initLin[ab.x]);
-    V3 dqAA = qdiff(qA, ldq(B.initAng[ab.x]));
-    V3 dqBL = pB - ld3(B.initLin[ab.y]);
-    V3 dqBA = qdiff(qB, ldq(B.initAng[ab.y]));
+    int sl = m * kMaxContacts + c;
     float4 bb0 = M.basis0[m];
-    V3 b0 = ld3(bb0), b1 = ld3(M.basis1[m]), b2 = ld3(M.basis2[m]);
+    float4 bb1 = M.basis1[m], bb2 = M.basis2[m];
+    float4 rA4 = M.rA[sl], rB4 = M.rB[sl], c04 = M.c0[sl], pen4 = M.pen[sl], lam4 = M.lam[sl];
+    float4 pA4 = B.pos[ab.x], pB4 = B.pos[ab.y], qA4 = B.rot[ab.x], qB4 = B.rot[ab.y];
+    float4 iA4 = B.initLin[ab.x], iB4 = B.initLin[ab.y], jA4 = B.initAng[ab.x], jB4 = B.initAng[ab.y];
+    Q4 qA = ldq(qA4), qB = ldq(qB4);
+    V3 dqAL = ld3(pA4) - ld3(iA4);
+    V3 dqAA = qdiff(qA, ldq(jA4));
+    V3 dqBL = ld3(pB4) - ld3(iB4);
+    V3 dqBA = qdiff(qB, ldq(jB4));
+    V3 b0 = ld3(bb0), b1 = ld3(bb1), b2 = ld3(bb2);
     V3 nb0 = -b0, nb1 = -b1, nb2 = -b2;
-    float friction = bb0.w;
-    int n = M.count[m];
-    for (int c = 0; c < n; ++c)
-    {
-        int sl = m * kMaxContacts + c;
-        V3 rAW = rotate(qA, ld3(M.rA[sl]));
-        V3 rBW = rotate(qB, ld3(M.rB[sl]));
-        V3 ja0 = cross(rAW, b0), ja1 = cross(rAW, b1), ja2 = cross(rAW, b2);
-        V3 jb0 = cross(rBW, nb0), jb1 = cross(rBW, nb1), jb2 = cross(rBW, nb2);
-        V3 pen = ld3(M.pen[sl]);
-        ContactEval ev = contact_eval(ld3(M.c0[sl]), pen, ld3(M.lam[sl]), friction, oma, b0, b1, b2, nb0, nb1, nb2,
-                                      ja0, ja1, ja2, jb0, jb1, jb2, dqAL, dqBL, dqAA, dqBA);
-        if (isA)
-            stamp_rows(s, b0, b1, b2, ja0, ja1, ja2, pen, ev.F0, ev.F1, ev.F2);
-        else
-            stamp_rows(s, nb0, nb1, nb2, jb0, jb1, jb2, pen, ev.F0, ev.F1, ev.F2);
-    }
+    V3 rAW = rotate(qA, ld3(rA4));
+    V3 rBW = rotate(qB, ld3(rB4));
+    V3 ja0 = cross(rAW, b0), ja1 = cross(rAW, b1), ja2 = cross(rAW, b2);
+    V3 jb0 = cross(rBW, nb0), jb1 = cross(rBW, nb1), jb2 = cross(rBW, nb2);
+    V3 pen = ld3(pen4);
+    ContactEval ev = contact_eval(ld3(c04), pen, ld3(lam4), bb0.w, oma, b0, b1, b2, nb0, nb1, nb2, ja0, ja1, ja2, jb0,
+                                  jb1, jb2, dqAL, dqBL, dqAA, dqBA);
+    if (ab.x == i)
+        rows_record(r, b0, b1, b2, ja0, ja1, ja2, pen, ev.F0, ev.F1, ev.F2);
+    else
+        rows_record(r, nb0, nb1, nb2, jb0, jb1, jb2, pen, ev.F0, ev.F1, ev.F2);
 }
 
 // Dense 3x3 helpers for the joint stamp (joint.cpp:56-123), kept literal.
@@ -1506,33 +1493,38 @@ __device__ __forceinline__ D3 gs_ball(int k, V3 v)
     m.m[2][k] += v.z;
     return m;
 }
-__device__ __forceinline__ void lhs_add_lin(Lhs &s, const D3 &a)
+__device__ __forceinline__ void rec_lin(float *r, const D3 &a)
 {
-    s.l00 += a.m[0][0];
-    s.l10 += a.m[1][0];
-    s.l11 += a.m[1][1];
-    s.l20 += a.m[2][0];
-    s.l21 += a.m[2][1];
-    s.l22 += a.m[2][2];
+    r[R_L00] = a.m[0][0];
+    r[R_L10] = a.m[1][0];
+    r[R_L11] = a.m[1][1];
+    r[R_L20] = a.m[2][0];
+    r[R_L21] = a.m[2][1];
+    r[R_L22] = a.m[2][2];
 }
-__device__ __forceinline__ void lhs_add_ang(Lhs &s, const D3 &a)
+__device__ __forceinline__ void rec_ang(float *r, const D3 &a)
 {
-    s.a00 += a.m[0][0];
-    s.a10 += a.m[1][0];
-    s.a11 += a.m[1][1];
-    s.a20 += a.m[2][0];
-    s.a21 += a.m[2][1];
-    s.a22 += a.m[2][2];
+    r[R_A00] = a.m[0][0];
+    r[R_A10] = a.m[1][0];
+    r[R_A11] = a.m[1][1];
+    r[R_A20] = a.m[2][0];
+    r[R_A21] = a.m[2][1];
+    r[R_A22] = a.m[2][2];
 }
 
-__device__ __forceinline__ void stamp_joint(Lhs &s, int i, const Joint &j, const Bodies &B, float alpha)
+// A joint adds to lhsAng three times in sequence (linear row K term, its geometric-stiffness
+// diagonal, angular row) and to rhsAng twice, so it is split into three ordered records:
+// part 0 linear row, part 1 the diagonal of H, part 2 the angular row (joint.cpp:58-122).
+__device__ __forceinline__ void joint_record(float *r, int i, const Joint &j, int part, const Bodies &B, float alpha)
 {
     bool isA = j.a == i;
     Q4 qB = ldq(B.rot[j.b]);
     V3 pB = ld3(B.pos[j.b]);
     V3 penL = ld3(j.penLin), penA = ld3(j.penAng);
-    if (lensq(penL) > 0.0f)
+    if (part < 2)
     {
+        if (!(lensq(penL) > 0.0f))
+            return;
         V3 rAW = j.a >= 0 ? rotate(ldq(B.rot[j.a]), ld3(j.rA)) : V3{0.0f, 0.0f, 0.0f};
         V3 rBW = rotate(qB, ld3(j.rB));
         D3 K = d3_diag(penL.x, penL.y, penL.z);
@@ -1540,44 +1532,59 @@ __device__ __forceinline__ void stamp_joint(Lhs &s, int i, const Joint &j, const
         if (isinf(j.rA.w))
             C -= ld3(j.c0Lin) * alpha;
         V3 F = d3_mv(K, C) + ld3(j.lamLin);
-        D3 jL = isA ? d3_diag(1.0f, 1.0f, 1.0f) : d3_diag(-1.0f, -1.0f, -1.0f);
-        D3 jA = isA ? d3_skew(-rAW) : d3_skew(rBW);
-        D3 jLT = d3_t(jL), jAT = d3_t(jA);
-        D3 jATk = d3_mul(jAT, K);
-        lhs_add_lin(s, d3_mul(d3_mul(jLT, K), jL));
-        lhs_add_ang(s, d3_mul(jATk, jA));
-        D3 cr = d3_mul(jATk, jL);
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b)
-                s.c[a][b] += cr.m[a][b];
-        V3 r = isA ? rAW : -rBW;
-        D3 g0 = d3_scale(gs_ball(0, r), F.x), g1 = d3_scale(gs_ball(1, r), F.y), g2 = d3_scale(gs_ball(2, r), F.z);
-        D3 H;
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b)
-                H.m[a][b] = (g0.m[a][b] + g1.m[a][b]) + g2.m[a][b];
-        V3 c0{H.m[0][0], H.m[1][0], H.m[2][0]}, c1{H.m[0][1], H.m[1][1], H.m[2][1]}, c2{H.m[0][2], H.m[1][2], H.m[2][2]};
-        lhs_add_ang(s, d3_diag(len(c0), len(c1), len(c2)));
-        s.rl += d3_mv(jLT, F);
-        s.ra += d3_mv(jAT, F);
+        if (part == 0)
+        {
+            D3 jL = isA ? d3_diag(1.0f, 1.0f, 1.0f) : d3_diag(-1.0f, -1.0f, -1.0f);
+            D3 jA = isA ? d3_skew(-rAW) : d3_skew(rBW);
+            D3 jLT = d3_t(jL), jAT = d3_t(jA);
+            D3 jATk = d3_mul(jAT, K);
+            rec_lin(r, d3_mul(d3_mul(jLT, K), jL));
+            rec_ang(r, d3_mul(jATk, jA));
+            D3 cr = d3_mul(jATk, jL);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    r[R_C + 3 * a + b] = cr.m[a][b];
+            V3 rl = d3_mv(jLT, F), ra = d3_mv(jAT, F);
+            r[R_RL] = rl.x;
+            r[R_RL + 1] = rl.y;
+            r[R_RL + 2] = rl.z;
+            r[R_RA] = ra.x;
+            r[R_RA + 1] = ra.y;
+            r[R_RA + 2] = ra.z;
+        }
+        else
+        {
+            V3 rr = isA ? rAW : -rBW;
+            D3 g0 = d3_scale(gs_ball(0, rr), F.x), g1 = d3_scale(gs_ball(1, rr), F.y), g2 = d3_scale(gs_ball(2, rr), F.z);
+            D3 H;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b)
+                    H.m[a][b] = (g0.m[a][b] + g1.m[a][b]) + g2.m[a][b];
+            V3 c0{H.m[0][0], H.m[1][0], H.m[2][0]}, c1{H.m[0][1], H.m[1][1], H.m[2][1]},
+                c2{H.m[0][2], H.m[1][2], H.m[2][2]};
+            rec_ang(r, d3_diag(len(c0), len(c1), len(c2)));
+        }
+        return;
     }
-    if (lensq(penA) > 0.0f)
-    {
-        D3 K = d3_diag(penA.x, penA.y, penA.z);
-        Q4 qa = j.a >= 0 ? ldq(B.rot[j.a]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
-        V3 C = qdiff(qa, qB) * j.c0Lin.w;
-        if (isinf(j.rB.w))
-            C -= ld3(j.c0Ang) * alpha;
-        V3 F = d3_mv(K, C) + ld3(j.lamAng);
-        D3 jA = d3_scale(isA ? d3_diag(1.0f, 1.0f, 1.0f) : d3_diag(-1.0f, -1.0f, -1.0f), j.c0Lin.w);
-        D3 jAT = d3_t(jA);
-        lhs_add_ang(s, d3_mul(d3_mul(jAT, K), jA));
-        s.ra += d3_mv(jAT, F);
-    }
+    if (!(lensq(penA) > 0.0f))
+        return;
+    D3 K = d3_diag(penA.x, penA.y, penA.z);
+    Q4 qa = j.a >= 0 ? ldq(B.rot[j.a]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
+    V3 C = qdiff(qa, qB) * j.c0Lin.w;
+    if (isinf(j.rB.w))
+        C -= ld3(j.c0Ang) * alpha;
+    V3 F = d3_mv(K, C) + ld3(j.lamAng);
+    D3 jA = d3_scale(isA ? d3_diag(1.0f, 1.0f, 1.0f) : d3_diag(-1.0f, -1.0f, -1.0f), j.c0Lin.w);
+    D3 jAT = d3_t(jA);
+    rec_ang(r, d3_mul(d3_mul(jAT, K), jA));
+    V3 ra = d3_mv(jAT, F);
+    r[R_RA] = ra.x;
+    r[R_RA + 1] = ra.y;
+    r[R_RA + 2] = ra.z;
 }
 
 // Spring::updatePrimal (spring.cpp:27-69)
-__device__ __forceinline__ void stamp_spring(Lhs &s, int i, const Spring &sp, const Bodies &B)
+__device__ __forceinline__ void spring_record(float *r, int i, const Spring &sp, const Bodies &B)
 {
     Q4 qA = ldq(B.rot[sp.a]), qB = ldq(B.rot[sp.b]);
     V3 d = (rotate(qA, ld3(sp.rA)) + ld3(B.pos[sp.a])) - (rotate(qB, ld3(sp.rB)) + ld3(B.pos[sp.b]));
@@ -1600,132 +1607,241 @@ __device__ __forceinline__ void stamp_spring(Lhs &s, int i, const Spring &sp, co
         jL = -n;
         jA = -cross(rW, n);
     }
-    // outer(a, b) * k: row r = b * a[r], then * k
-    s.l00 += (jL.x * jL.x) * k;
-    s.l10 += (jL.x * jL.y) * k;
-    s.l11 += (jL.y * jL.y) * k;
-    s.l20 += (jL.x * jL.z) * k;
-    s.l21 += (jL.y * jL.z) * k;
-    s.l22 += (jL.z * jL.z) * k;
-    s.a00 += (jA.x * jA.x) * k;
-    s.a10 += (jA.x * jA.y) * k;
-    s.a11 += (jA.y * jA.y) * k;
-    s.a20 += (jA.x * jA.z) * k;
-    s.a21 += (jA.y * jA.z) * k;
-    s.a22 += (jA.z * jA.z) * k;
-    float ga[3] = {jA.x, jA.y, jA.z}, gl[3] = {jL.x, jL.y, jL.z};
+    // outer(a, b) * k: entry (r, c) = (b[c] * a[r]) * k
+    float gl[3] = {jL.x, jL.y, jL.z}, ga[3] = {jA.x, jA.y, jA.z};
+    r[R_L00] = (gl[0] * gl[0]) * k;
+    r[R_L10] = (gl[0] * gl[1]) * k;
+    r[R_L11] = (gl[1] * gl[1]) * k;
+    r[R_L20] = (gl[0] * gl[2]) * k;
+    r[R_L21] = (gl[1] * gl[2]) * k;
+    r[R_L22] = (gl[2] * gl[2]) * k;
+    r[R_A00] = (ga[0] * ga[0]) * k;
+    r[R_A10] = (ga[0] * ga[1]) * k;
+    r[R_A11] = (ga[1] * ga[1]) * k;
+    r[R_A20] = (ga[0] * ga[2]) * k;
+    r[R_A21] = (ga[1] * ga[2]) * k;
+    r[R_A22] = (ga[2] * ga[2]) * k;
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b)
-            s.c[a][b] += (gl[b] * ga[a]) * k;
-    s.rl += jL * f;
-    s.ra += jA * f;
+            r[R_C + 3 * a + b] = (gl[b] * ga[a]) * k;
+    r[R_RL] = jL.x * f;
+    r[R_RL + 1] = jL.y * f;
+    r[R_RL + 2] = jL.z * f;
+    r[R_RA] = jA.x * f;
+    r[R_RA + 1] = jA.y * f;
+    r[R_RA + 2] = jA.z * f;
 }
 
-__device__ void primal_body(int i, const Bodies &B, const Manifolds &M, const Joint *joints, const Spring *springs,
-                            const int *deg, const int *adjStart, const int *adj, const Params &P)
+// Per-body stamp lists, built once per step from the force lists: each manifold contributes
+// one record per contact, each joint three, each spring one, in force order.
+__global__ void k_stamp_count(const int *deg, const int *adjStart, const int *adj, const int *mcount, int n,
+                              int *stampCount)
 {
-    float4 mm = B.massMoment[i];
-    float dt2 = P.dt * P.dt;
-    float ml = mm.x / dt2;
-    float ax = mm.y / dt2, ay = mm.z / dt2, az = mm.w / dt2;
-    Lhs s;
-    s.l00 = ml;
-    s.l11 = ml;
-    s.l22 = ml;
-    s.l10 = s.l20 = s.l21 = 0.0f;
-    s.a00 = ax;
-    s.a11 = ay;
-    s.a22 = az;
-    s.a10 = s.a20 = s.a21 = 0.0f;
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b)
-            s.c[a][b] = 0.0f;
-    V3 pos = ld3(B.pos[i]);
-    Q4 rot = ldq(B.rot[i]);
-    V3 dl = pos - ld3(B.inerLin[i]);
-    V3 da = qdiff(rot, ldq(B.inerAng[i]));
-    s.rl = V3{ml * dl.x, ml * dl.y, ml * dl.z};
-    s.ra = V3{ax * da.x, ay * da.y, az * da.z};
-    float oma = 1.0f - P.alpha;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int s = 0;
     int b = adjStart[i], e = b + deg[i];
     for (int x = b; x < e; ++x)
     {
         int en = adj[x];
-        int kind = adjKind(en), idx = adjIndex(en);
-        if (kind == kAdjManifold)
-            stamp_manifold(s, i, idx, M, B, oma);
-        else if (kind == kAdjJoint)
-            stamp_joint(s, i, joints[idx], B, P.alpha);
-        else
-            stamp_spring(s, i, springs[idx], B);
+        int kind = adjKind(en);
+        s += kind == kAdjManifold ? mcount[adjIndex(en)] : (kind == kAdjJoint ? 3 : 1);
     }
-    Sys6 sys;
-    sys.l00 = s.l00;
-    sys.l10 = s.l10;
-    sys.l11 = s.l11;
-    sys.l20 = s.l20;
-    sys.l21 = s.l21;
-    sys.l22 = s.l22;
-    sys.a00 = s.a00;
-    sys.a10 = s.a10;
-    sys.a11 = s.a11;
-    sys.a20 = s.a20;
-    sys.a21 = s.a21;
-    sys.a22 = s.a22;
-    sys.c00 = s.c[0][0];
-    sys.c01 = s.c[0][1];
-    sys.c02 = s.c[0][2];
-    sys.c10 = s.c[1][0];
-    sys.c11 = s.c[1][1];
-    sys.c12 = s.c[1][2];
-    sys.c20 = s.c[2][0];
-    sys.c21 = s.c[2][1];
-    sys.c22 = s.c[2][2];
-    V3 dxL, dxA;
-    solve6(sys, -s.rl, -s.ra, dxL, dxA);
-    B.pos[i] = st3(pos + dxL, 0.0f);
-    B.rot[i] = stq(qintegrate(rot, dxA));
+    stampCount[i] = s;
 }
 
-// Manifold::updateDual (manifold.cpp:231-280)
-__device__ void dual_manifold(int m, const Manifolds &M, const Bodies &B, const Params &P)
+__global__ void k_stamp_fill(const int *deg, const int *adjStart, const int *adj, const int *mcount, int n,
+                             const int *stampStart, int2 *stamps, int maxStamps, int *overflow)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int o = stampStart[i];
+    int b = adjStart[i], e = b + deg[i];
+    for (int x = b; x < e; ++x)
+    {
+        int en = adj[x];
+        int kind = adjKind(en);
+        int parts = kind == kAdjManifold ? mcount[adjIndex(en)] : (kind == kAdjJoint ? 3 : 1);
+        for (int p = 0; p < parts; ++p, ++o)
+        {
+            if (o < maxStamps)
+                stamps[o] = make_int2(en, p);
+            else
+                atomicOr(overflow, 64);
+        }
+    }
+}
+
+__device__ __forceinline__ float init_entry(int e, float ml, float ax, float ay, float az, V3 dl, V3 da)
+{
+    switch (e)
+    {
+    case R_L00:
+    case R_L11:
+    case R_L22: return ml;
+    case R_A00: return ax;
+    case R_A11: return ay;
+    case R_A22: return az;
+    case R_RL: return ml * dl.x;
+    case R_RL + 1: return ml * dl.y;
+    case R_RL + 2: return ml * dl.z;
+    case R_RA: return ax * da.x;
+    case R_RA + 1: return ay * da.y;
+    case R_RA + 2: return az * da.z;
+    default: return 0.0f;
+    }
+}
+
+// Primal update of the bodies of one colour (solver.cpp:1184-1240): kGroupLanes lanes per body.
+__global__ __launch_bounds__(256) void k_primal_color(Bodies B, Manifolds M, const Joint *joints,
+                                                      const Spring *springs, const int *stampStart,
+                                                      const int *stampCount, const int2 *stamps,
+                                                      const int *colorOffsets, const int *colorBodies, Params P,
+                                                      int color)
+{
+    __shared__ float rec[kGroupsPerBlock][kGroupLanes][kRec + 1];
+    __shared__ int roundsShared;
+    const int lane = threadIdx.x % kGroupLanes, g = threadIdx.x / kGroupLanes;
+    const int k = colorOffsets[color] + blockIdx.x * kGroupsPerBlock + g;
+    const bool valid = k < colorOffsets[color + 1];
+    const int i = valid ? colorBodies[k] : 0;
+    const int s0 = valid ? stampStart[i] : 0;
+    const int ns = valid ? stampCount[i] : 0;
+    if (threadIdx.x == 0)
+        roundsShared = 0;
+    __syncthreads();
+    if (lane == 0 && valid)
+        atomicMax(&roundsShared, (ns + kGroupLanes - 1) / kGroupLanes);
+    float4 mm = B.massMoment[i];
+    V3 pos = ld3(B.pos[i]);
+    Q4 rot = ldq(B.rot[i]);
+    float dt2 = P.dt * P.dt;
+    float ml = mm.x / dt2, ax = mm.y / dt2, ay = mm.z / dt2, az = mm.w / dt2;
+    V3 dl = pos - ld3(B.inerLin[i]);
+    V3 da = qdiff(rot, ldq(B.inerAng[i]));
+    const int e0 = lane, e1 = lane + kGroupLanes;
+    float acc0 = init_entry(e0, ml, ax, ay, az, dl, da);
+    float acc1 = e1 < kRec ? init_entry(e1, ml, ax, ay, az, dl, da) : 0.0f;
+    const float oma = 1.0f - P.alpha;
+    __syncthreads();
+    const int rounds = roundsShared;
+    for (int r = 0; r < rounds; ++r)
+    {
+        int t = r * kGroupLanes + lane;
+        float out[kRec];
+#pragma unroll
+        for (int q = 0; q < kRec; ++q)
+            out[q] = 0.0f;
+        if (valid && t < ns)
+        {
+            int2 st = stamps[s0 + t];
+            int kind = adjKind(st.x), idx = adjIndex(st.x);
+            if (kind == kAdjManifold)
+                contact_record(out, i, idx, st.y, M, B, oma);
+            else if (kind == kAdjJoint)
+                joint_record(out, i, joints[idx], st.y, B, P.alpha);
+            else
+                spring_record(out, i, springs[idx], B);
+        }
+#pragma unroll
+        for (int q = 0; q < kRec; ++q)
+            rec[g][lane][q] = out[q];
+        __syncthreads();
+        int cnt = valid ? min(kGroupLanes, ns - r * kGroupLanes) : 0;
+        for (int q = 0; q < cnt; ++q)
+        {
+            acc0 += rec[g][q][e0];
+            if (e1 < kRec)
+                acc1 += rec[g][q][e1];
+        }
+        __syncthreads();
+    }
+    rec[g][0][e0] = acc0;
+    if (e1 < kRec)
+        rec[g][0][e1] = acc1;
+    __syncthreads();
+    if (valid && lane == 0)
+    {
+        const float *a = rec[g][0];
+        Sys6 sys;
+        sys.l00 = a[R_L00];
+        sys.l10 = a[R_L10];
+        sys.l11 = a[R_L11];
+        sys.l20 = a[R_L20];
+        sys.l21 = a[R_L21];
+        sys.l22 = a[R_L22];
+        sys.a00 = a[R_A00];
+        sys.a10 = a[R_A10];
+        sys.a11 = a[R_A11];
+        sys.a20 = a[R_A20];
+        sys.a21 = a[R_A21];
+        sys.a22 = a[R_A22];
+        sys.c00 = a[R_C + 0];
+        sys.c01 = a[R_C + 1];
+        sys.c02 = a[R_C + 2];
+        sys.c10 = a[R_C + 3];
+        sys.c11 = a[R_C + 4];
+        sys.c12 = a[R_C + 5];
+        sys.c20 = a[R_C + 6];
+        sys.c21 = a[R_C + 7];
+        sys.c22 = a[R_C + 8];
+        V3 rl{a[R_RL], a[R_RL + 1], a[R_RL + 2]}, ra{a[R_RA], a[R_RA + 1], a[R_RA + 2]};
+        V3 dxL, dxA;
+        solve6(sys, -rl, -ra, dxL, dxA);
+        B.pos[i] = st3(pos + dxL, 0.0f);
+        B.rot[i] = stq(qintegrate(rot, dxA));
+    }
+}
+
+// Flattened contact index -> owning manifold (for the per-contact dual pass).
+__global__ void k_contact_owner(const int *mcount, const int *contactStart, const Counters *ctr, int *owner)
+{
+    int nm = ctr->numManifolds;
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += gridDim.x * blockDim.x)
+    {
+        int o = contactStart[m], n = mcount[m];
+        for (int c = 0; c < n; ++c)
+            owner[o + c] = m;
+    }
+}
+
+// Manifold::updateDual for one contact (manifold.cpp:231-280)
+__device__ __forceinline__ void dual_contact(int m, int c, const Manifolds &M, const Bodies &B, const Params &P)
 {
     int2 ab = M.ab[m];
-    V3 pA = ld3(B.pos[ab.x]), pB = ld3(B.pos[ab.y]);
-    Q4 qA = ldq(B.rot[ab.x]), qB = ldq(B.rot[ab.y]);
-    V3 dqAL = pA - ld3(B.initLin[ab.x]);
-    V3 dqAA = qdiff(qA, ldq(B.initAng[ab.x]));
-    V3 dqBL = pB - ld3(B.initLin[ab.y]);
-    V3 dqBA = qdiff(qB, ldq(B.initAng[ab.y]));
+    int sl = m * kMaxContacts + c;
     float4 bb0 = M.basis0[m];
-    V3 b0 = ld3(bb0), b1 = ld3(M.basis1[m]), b2 = ld3(M.basis2[m]);
+    float4 bb1 = M.basis1[m], bb2 = M.basis2[m];
+    float4 rA4 = M.rA[sl], rB4 = M.rB[sl], c04 = M.c0[sl], pen4 = M.pen[sl], lam4 = M.lam[sl];
+    float4 pA4 = B.pos[ab.x], pB4 = B.pos[ab.y], qA4 = B.rot[ab.x], qB4 = B.rot[ab.y];
+    float4 iA4 = B.initLin[ab.x], iB4 = B.initLin[ab.y], jA4 = B.initAng[ab.x], jB4 = B.initAng[ab.y];
+    Q4 qA = ldq(qA4), qB = ldq(qB4);
+    V3 dqAL = ld3(pA4) - ld3(iA4);
+    V3 dqAA = qdiff(qA, ldq(jA4));
+    V3 dqBL = ld3(pB4) - ld3(iB4);
+    V3 dqBA = qdiff(qB, ldq(jB4));
+    V3 b0 = ld3(bb0), b1 = ld3(bb1), b2 = ld3(bb2);
     V3 nb0 = -b0, nb1 = -b1, nb2 = -b2;
-    float oma = 1.0f - P.alpha;
-    int n = M.count[m];
-    for (int c = 0; c < n; ++c)
+    V3 rAW = rotate(qA, ld3(rA4));
+    V3 rBW = rotate(qB, ld3(rB4));
+    V3 ja0 = cross(rAW, b0), ja1 = cross(rAW, b1), ja2 = cross(rAW, b2);
+    V3 jb0 = cross(rBW, nb0), jb1 = cross(rBW, nb1), jb2 = cross(rBW, nb2);
+    V3 pen = ld3(pen4);
+    ContactEval ev = contact_eval(ld3(c04), pen, ld3(lam4), bb0.w, 1.0f - P.alpha, b0, b1, b2, nb0, nb1, nb2, ja0,
+                                  ja1, ja2, jb0, jb1, jb2, dqAL, dqBL, dqAA, dqBA);
+    float stick = lam4.w;
+    if (ev.F0 < 0.0f)
+        pen.x = fminr(pen.x + P.betaLin * fabsf(ev.C0), kPenaltyMax);
+    if (ev.fs <= ev.bounds)
     {
-        int sl = m * kMaxContacts + c;
-        V3 rAW = rotate(qA, ld3(M.rA[sl]));
-        V3 rBW = rotate(qB, ld3(M.rB[sl]));
-        V3 ja0 = cross(rAW, b0), ja1 = cross(rAW, b1), ja2 = cross(rAW, b2);
-        V3 jb0 = cross(rBW, nb0), jb1 = cross(rBW, nb1), jb2 = cross(rBW, nb2);
-        float4 pen4 = M.pen[sl];
-        float4 lam4 = M.lam[sl];
-        V3 pen = ld3(pen4);
-        ContactEval ev = contact_eval(ld3(M.c0[sl]), pen, ld3(lam4), bb0.w, oma, b0, b1, b2, nb0, nb1, nb2, ja0, ja1,
-                                      ja2, jb0, jb1, jb2, dqAL, dqBL, dqAA, dqBA);
-        float stick = lam4.w;
-        if (ev.F0 < 0.0f)
-            pen.x = fminr(pen.x + P.betaLin * fabsf(ev.C0), kPenaltyMax);
-        if (ev.fs <= ev.bounds)
-        {
-            pen.y = fminr(pen.y + P.betaLin * fabsf(ev.C1), kPenaltyMax);
-            pen.z = fminr(pen.z + P.betaLin * fabsf(ev.C2), kPenaltyMax);
-            stick = sqrtf(ev.C1 * ev.C1 + ev.C2 * ev.C2) < kStickThresh ? 1.0f : 0.0f;
-        }
-        M.lam[sl] = make_float4(ev.F0, ev.F1, ev.F2, stick);
-        M.pen[sl] = st3(pen, 0.0f);
+        pen.y = fminr(pen.y + P.betaLin * fabsf(ev.C1), kPenaltyMax);
+        pen.z = fminr(pen.z + P.betaLin * fabsf(ev.C2), kPenaltyMax);
+        stick = sqrtf(ev.C1 * ev.C1 + ev.C2 * ev.C2) < kStickThresh ? 1.0f : 0.0f;
     }
+    M.lam[sl] = make_float4(ev.F0, ev.F1, ev.F2, stick);
+    M.pen[sl] = st3(pen, 0.0f);
 }
 
 // Joint::updateDual (joint.cpp:125-182)
@@ -1778,34 +1894,19 @@ __device__ void dual_joint(Joint *jp, const Bodies &B, const Params &P)
     jp->broken = broken;
 }
 
-// All solver iterations of one step in one cooperative launch: per colour a primal sweep, then
-// one dual pass over manifolds and joints, with grid barriers between phases.
-__global__ void k_iterate(Bodies B, Manifolds M, Joint *joints, int nj, const Spring *springs, const int *deg,
-                          const int *adjStart, const int *adj, const int *colorOffsets, const int *colorBodies, Params P,
-                          Counters *ctr, GridBarrier *gb)
+// Dual pass of one iteration: one lane per contact, then one per joint.
+__global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int *contactOwner, const int *contactStart,
+                       const Counters *ctr, Params P)
 {
-    const unsigned nblocks = gridDim.x;
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int stride = nblocks * blockDim.x;
-    int colors = ctr->colorCount;
-    int nm = ctr->numManifolds;
-    for (int it = 0; it < P.iterations; ++it)
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int nc = ctr->contactCount;
+    if (t < nc)
     {
-        for (int c = 0; c < colors; ++c)
-        {
-            int b0 = colorOffsets[c], b1 = colorOffsets[c + 1];
-            for (int k = b0 + tid; k < b1; k += stride)
-                primal_body(colorBodies[k], B, M, joints, springs, deg, adjStart, adj, P);
-            grid_sync(gb, nblocks, &ctr->barrierTimeout);
-        }
-        for (int m = tid; m < nm; m += stride)
-            dual_manifold(m, M, B, P);
-        for (int k = tid; k < nj; k += stride)
-            dual_joint(&joints[k], B, P);
-        grid_sync(gb, nblocks, &ctr->barrierTimeout);
-        if (__hip_atomic_load(&ctr->barrierTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            break;
+        int m = contactOwner[t];
+        dual_contact(m, t - contactStart[m], M, B, P);
     }
+    else if (t - nc < nj)
+        dual_joint(&joints[t - nc], B, P);
 }
 
 // Solver::finishStep for boxes (solver.cpp:1275-1288).

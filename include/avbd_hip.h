@@ -189,6 +189,17 @@ typedef struct avbd_stats {
     double step_ms_total;
     int64_t counted_steps;
     int64_t timed_steps;
+    /* The iteration loop split into its two kernels: per iteration, one primal
+     * launch per colour (k_primal_color) then one dual launch (k_dual). Bytes
+     * are algorithmic (primal: solved bodies and the stamps of their forces;
+     * dual: contacts and joints); times are HIP-event spans around each
+     * iteration's primal launches and its dual launch. */
+    double primal_bytes_total;
+    double dual_bytes_total;
+    double primal_ms_total;
+    double dual_ms_total;
+    int64_t primal_launches;
+    int64_t dual_launches;
 } avbd_stats;
 
 typedef struct avbd_handle avbd_handle;

@@ -95,6 +95,7 @@ def load_ref():
         lib.ref_collide.argtypes = [_p, _p, _p, _p]
         lib.ref_make_box_body.argtypes = [_p, C.c_float, C.c_float, _p, _p, _p]
         lib.ref_joint_spring_derived.argtypes = [_p, _p, _p, _p, _p, _p]
+        lib.ref_use_hip.argtypes = [_p, _i]
         _REF = lib
     return _REF
 
@@ -292,6 +293,10 @@ class RefWorld:
         if getattr(self, "_h", None):
             self._lib.ref_destroy(self._h)
             self._h = None
+
+    def use_hip(self, device: int = 0) -> bool:
+        """Install the hip-avbd PhysicsBackend adapter (Solver::step then runs on the GPU)."""
+        return self._lib.ref_use_hip(self._h, device) == 0
 
     def step(self, n: int = 1, colored: bool = True):
         self._lib.ref_step(self._h, n, int(colored))

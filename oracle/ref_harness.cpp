@@ -21,6 +21,9 @@
  */
 #include "solver.h"
 #include "../include/avbd_hip.h"
+#ifdef AVBD_WITH_HIP_ADAPTER
+#include "hip_avbd_backend.h"
+#endif
 
 #include <algorithm>
 #include <chrono>
@@ -547,6 +550,34 @@ void ref_joint_spring_derived(const avbd_body *a, const avbd_body *b, const floa
     }
     else
         *rest = -1.0f;
+}
+
+} // extern "C"
+
+extern "C" {
+
+/* Install the "hip-avbd" PhysicsBackend (avbd-demo3d_amd/host) on this solver, so that
+ * ref_step(h, n, 0) runs Solver::step through the GPU drop-in. Returns 0, or -1 when the
+ * harness was built without the adapter. */
+int ref_use_hip(void *h, int device)
+{
+#ifdef AVBD_WITH_HIP_ADAPTER
+    ((RefWorld *)h)->solver.physicsBackend = makeHipAvbdPhysicsBackend(device);
+    return 0;
+#else
+    (void)h;
+    (void)device;
+    return -1;
+#endif
+}
+
+int ref_hip_fallbacks(void)
+{
+#ifdef AVBD_WITH_HIP_ADAPTER
+    return hipAvbdSolverStats().cpuIterateFallbacks;
+#else
+    return -1;
+#endif
 }
 
 } // extern "C"

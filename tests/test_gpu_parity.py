@@ -161,10 +161,11 @@ def test_injected_fault_and_capacity_errors(gpu):
     # a pair buffer far too small for the pile: the kernels bound their writes and report it
     small = gpu.HipWorld(default_params(), max_bodies=len(st.bodies), max_pairs=8)
     small.upload(st)
-    small.step(2)
     with pytest.raises(gpu.AvbdError) as e:
+        small.step(2)
         small.synchronize()
     assert e.value.code == -2
+    assert "max_pairs" in str(e.value)
 
 
 def test_headline_size_single_step_vs_oracle(gpu, oracle):
