@@ -116,14 +116,16 @@ int alloc_manifolds(avbd_handle *h, Manifolds &m, int cap)
     int r;
     size_t cs = (size_t)cap * kMaxContacts;
     if ((r = h->alloc(m.ab, cap)) || (r = h->alloc(m.serial, cap)) || (r = h->alloc(m.count, cap)) ||
-        (r = h->alloc(m.basis0, cap)) || (r = h->alloc(m.basis1, cap)) || (r = h->alloc(m.basis2, cap)) ||
-        (r = h->alloc(m.feature, cs)) || (r = h->alloc(m.rA, cs)) || (r = h->alloc(m.rB, cs)) ||
-        (r = h->alloc(m.c0, cs)) || (r = h->alloc(m.pen, cs)) || (r = h->alloc(m.lam, cs)))
+        (r = h->alloc(m.feature, cs)) || (r = h->alloc(m.cr, cs * kCR)))
         return r;
     return AVBD_OK;
 }
 
 int grid_for(int n) { return std::max(1, std::min(ceil_div(n, 256), 4096)); }
+
+// Colouring runs as one cooperative launch of a few large blocks: every round ends in a grid
+// barrier whose cost grows with the block count.
+constexpr int kColorThreads = 1024;
 
 // Exclusive scan with the count either on the device (nptr) or given (nconst).
 void scan(avbd_handle *h, const int *in, int *out, const int *nptr, int nconst, int *total)
@@ -160,9 +162,9 @@ __global__ void k_clamp_count(int *count, int cap, int *overflow, int bit)
 
 __global__ void k_barrier_reset(GridBarrier *gb) { gb->count = 0u; }
 
-int launch_coop(const void *fn, int blocks, void **args, hipStream_t s)
+int launch_coop(const void *fn, int blocks, int threads, void **args, hipStream_t s)
 {
-    HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(256), args, 0, s));
+    HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(threads), args, 0, s));
     return AVBD_OK;
 }
 
@@ -238,23 +240,23 @@ int enqueue_prepare(avbd_handle *h)
     // ---- broadphase: sweep and prune (solver.cpp:488-577) ----
     if (N >= 2)
     {
-        k_axis_partial<<<64, 256, 0, s>>>(h->B.pos, N, S.axisSums);
+        k_axis_partial<<<64, 256, 0, s>>>(h->B.pose, N, S.axisSums);
         k_axis_decide<<<1, 64, 0, s>>>(S.axisSums, 64, N, ctr);
-        k_axis_serial<<<1, 64, 0, s>>>(h->B.pos, N, ctr);
-        k_sap_keys<<<ceil_div(N, 256), 256, 0, s>>>(h->B.pos, h->B.sizeRad, N, ctr, S.keys, S.vals);
+        k_axis_serial<<<1, 64, 0, s>>>(h->B.pose, N, ctr);
+        k_sap_keys<<<ceil_div(N, 256), 256, 0, s>>>(h->B.pose, h->B.sizeRad, N, ctr, S.keys, S.vals);
         unsigned *keys = S.keys, *keysAlt = S.keysAlt;
         int *vals = S.vals, *valsAlt = S.valsAlt;
         radix_sort(h, keys, vals, keysAlt, valsAlt, N, 4);
-        k_sap_gather<<<ceil_div(N, 256), 256, 0, s>>>(vals, h->B.pos, h->B.sizeRad, N, ctr, S.sLo, S.sHi, S.sPos, S.sIdx);
+        k_sap_gather<<<ceil_div(N, 256), 256, 0, s>>>(vals, h->B.pose, h->B.sizeRad, N, ctr, S.sLo, S.sHi, S.sPos, S.sIdx);
         k_sweep_len<<<ceil_div(N, 256), 256, 0, s>>>(S.sLo, S.sHi, N, h->sweepEnd, S.chunkCount, ctr);
         scan(h, S.chunkCount, S.chunkStart, nullptr, N, &ctr->numChunks);
         k_clamp_count<<<1, 1, 0, s>>>(&ctr->numChunks, h->maxChunks, &ctr->overflow, 32);
-        k_sweep_count<<<G, 256, 0, s>>>(S.chunkStart, h->sweepEnd, N, ctr, S.sPos, S.hitCount);
+        k_sweep_count<<<G, 256, 0, s>>>(S.chunkStart, h->sweepEnd, N, ctr, S.sPos, S.hitCount, S.hitMask);
         scan(h, S.hitCount, S.hitStart, &ctr->numChunks, 0, &ctr->numHits);
         k_fill_u64<<<grid_for(h->hashSize), 256, 0, s>>>(S.hash, h->hashSize, ~0ull);
         k_hash_insert<<<G, 256, 0, s>>>(h->cur.ab, ctr, h->joints, h->nj, h->springs, h->ns, S.hash, S.hashMask, ctr);
-        k_sweep_emit<<<G, 256, 0, s>>>(S.chunkStart, h->sweepEnd, N, ctr, S.sPos, S.sIdx, S.hitStart, S.hash, S.hashMask,
-                                       S.hits, S.hitNew, h->maxPairs);
+        k_sweep_emit<<<G, 256, 0, s>>>(S.chunkStart, N, ctr, S.hitMask, S.sIdx, S.hitStart, S.hash, S.hashMask, S.hits,
+                                       S.hitNew, h->maxPairs);
         k_clamp_count<<<1, 1, 0, s>>>(&ctr->numHits, h->maxPairs, &ctr->overflow, 1);
         scan(h, S.hitNew, S.hitRank, &ctr->numHits, 0, &ctr->numCreated);
         k_create<<<G, 256, 0, s>>>(S.hits, S.hitNew, S.hitRank, ctr, h->cur, h->maxManifolds, h->maxPairs);
@@ -269,7 +271,7 @@ int enqueue_prepare(avbd_handle *h)
     k_compact_manifolds<<<G, 256, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr);
     k_compact_finish<<<1, 1, 0, s>>>(ctr);
     scan(h, h->cur.count, S.contactStart, &ctr->numManifolds, 0, &ctr->contactCount);
-    k_contact_owner<<<G, 256, 0, s>>>(h->cur.count, S.contactStart, ctr, S.contactOwner);
+    k_contact_records<<<G, 256, 0, s>>>(h->cur.count, h->cur.ab, S.contactStart, ctr, S.contactRec);
     if (h->nj > 0)
         k_joint_init<<<ceil_div(h->nj, 256), 256, 0, s>>>(h->joints, h->nj, h->B, h->P, ctr);
     if (h->timing)
@@ -286,8 +288,8 @@ int enqueue_prepare(avbd_handle *h)
     k_adj_sort<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjSerial, S.adjPartner, N, h->maxAdj);
     k_stamp_count<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, h->cur.count, N, S.stampCount);
     scan(h, S.stampCount, S.stampStart, nullptr, N, &ctr->numStamps);
-    k_stamp_fill<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, h->cur.count, N, S.stampStart, S.stamps,
-                                                  h->maxStamps, &ctr->overflow);
+    k_stamp_fill<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjPartner, h->cur.count, h->cur.ab, N,
+                                                  S.stampStart, S.stamps, h->maxStamps, &ctr->overflow);
     k_warmstart<<<ceil_div(N, 256), 256, 0, s>>>(h->B, h->P, S.deg, N, S.color, ctr);
     k_step_bytes<<<1, 1, 0, s>>>(ctr, N, h->P.iterations);
 
@@ -300,19 +302,23 @@ int enqueue_prepare(avbd_handle *h)
         GridBarrier *gb = h->gb;
         void *args[] = {(void *)&deg, (void *)&adjStart, (void *)&adjPartner, (void *)&mm, (void *)&n, (void *)&color,
                         (void *)&ctr, (void *)&gb, (void *)&maxRounds};
-        int r = launch_coop((const void *)k_color, h->coopColor, args, s);
+        int r = launch_coop((const void *)k_color, h->coopColor, kColorThreads, args, s);
         if (r)
             return r;
     }
     {
         unsigned *keys = S.keys, *keysAlt = S.keysAlt;
         int *vals = S.vals, *valsAlt = S.valsAlt;
-        k_color_keys<<<ceil_div(N, 256), 256, 0, s>>>(S.color, N, keys, vals, ctr);
+        k_color_keys<<<ceil_div(N, 256), 256, 0, s>>>(S.color, S.stampCount, N, keys, vals, ctr);
         int tiles = ceil_div(N, kRadixTile);
         k_radix_hist<<<tiles, 256, 0, s>>>(keys, N, 0, S.radixHist, tiles);
         scan(h, S.radixHist, h->radixHistScan, nullptr, 256 * tiles, nullptr);
-        k_radix_scatter<<<tiles, 256, 0, s>>>(keys, vals, N, 0, h->radixHistScan, tiles, keysAlt, S.colorBodies);
+        k_radix_scatter<<<tiles, 256, 0, s>>>(keys, vals, N, 0, h->radixHistScan, tiles, keysAlt, valsAlt);
+        k_radix_hist<<<tiles, 256, 0, s>>>(keysAlt, N, 8, S.radixHist, tiles);
+        scan(h, S.radixHist, h->radixHistScan, nullptr, 256 * tiles, nullptr);
+        k_radix_scatter<<<tiles, 256, 0, s>>>(keysAlt, valsAlt, N, 8, h->radixHistScan, tiles, keys, S.colorBodies);
         k_color_offsets<<<1, 256, 0, s>>>(h->radixHistScan, tiles, N, S.colorOffsets);
+        k_tasks<<<ceil_div(N, 256), 256, 0, s>>>(S.colorBodies, S.stampStart, S.stampCount, N, S.tasks);
     }
     if (h->timing)
     {
@@ -336,43 +342,38 @@ int enqueue_iterate(avbd_handle *h)
     const int colors = std::min(c.colorCount, kMaxColors);
     const int dualLanes = c.contactCount + h->nj;
     Scratch &S = h->S;
+    // Timing marks (when enabled) bracket the first iteration's primal launches and its dual
+    // launch only: back-to-back launches of one stream have no gap between them, while each
+    // event adds one, so a few marks per step keep the measured step close to the unmeasured one.
     hipEvent_t *mark = h->timing ? &h->ring[avbd_handle::kSlots * (h->ringSteps % avbd_handle::kRing)] : nullptr;
-    int marked = 0, primalPerStep = 0;
+    int primalPerIter = 0;
     for (int it = 0; it < h->P.iterations; ++it)
     {
         for (int col = 0; col < colors; ++col)
         {
-            int n = h->hostOffsets[col + 1] - h->hostOffsets[col];
+            int b = h->hostOffsets[col], n = h->hostOffsets[col + 1] - b;
             if (n > 0)
             {
                 k_primal_color<<<ceil_div(n, kGroupsPerBlock), 256, 0, s>>>(h->B, h->cur, h->joints, h->springs,
-                                                                             S.stampStart, S.stampCount, S.stamps,
-                                                                             S.colorOffsets, S.colorBodies, h->P, col);
-                ++primalPerStep;
+                                                                             S.stamps, S.tasks, b, b + n, h->P);
+                if (it == 0)
+                    ++primalPerIter;
             }
         }
-        if (mark && it < avbd_handle::kIterSlots)
-            HIP_TRY(hipEventRecord(mark[4 + 2 * it], s));
+        if (mark && it == 0)
+            HIP_TRY(hipEventRecord(mark[4], s));
         if (dualLanes > 0)
-        {
-            k_dual<<<ceil_div(dualLanes, 256), 256, 0, s>>>(h->B, h->cur, h->joints, h->nj, S.contactOwner,
-                                                            S.contactStart, h->ctr, h->P);
-            h->dualLaunches++;
-        }
-        if (mark && it < avbd_handle::kIterSlots)
-        {
-            HIP_TRY(hipEventRecord(mark[5 + 2 * it], s));
-            marked = it + 1;
-        }
+            k_dual<<<ceil_div(dualLanes, 256), 256, 0, s>>>(h->B, h->cur, h->joints, h->nj, S.contactRec,
+                                                            c.contactCount, h->P);
+        if (mark && it == 0)
+            HIP_TRY(hipEventRecord(mark[5], s));
     }
-    h->primalLaunches += primalPerStep;
     if (h->timing)
     {
         HIP_TRY(hipEventRecord(h->ev[4], s));
         HIP_TRY(hipEventRecord(mark[2], s));
-        h->ringIters[h->ringSteps % avbd_handle::kRing] = marked;
-        h->ringPrimalLaunches[h->ringSteps % avbd_handle::kRing] =
-            marked == h->P.iterations ? primalPerStep : primalPerStep * marked / std::max(h->P.iterations, 1);
+        h->ringIters[h->ringSteps % avbd_handle::kRing] = h->P.iterations > 0 ? 1 : 0;
+        h->ringPrimalLaunches[h->ringSteps % avbd_handle::kRing] = primalPerIter;
     }
     return AVBD_OK;
 }
@@ -541,8 +542,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     if (e != hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
     Bodies &B = h->B;
-    if ((r = h->alloc(B.pos, N)) || (r = h->alloc(B.rot, N)) || (r = h->alloc(B.initLin, N)) ||
-        (r = h->alloc(B.initAng, N)) || (r = h->alloc(B.inerLin, N)) || (r = h->alloc(B.inerAng, N)) ||
+    if ((r = h->alloc(B.pose, 4 * (size_t)N)) || (r = h->alloc(B.inerLin, N)) || (r = h->alloc(B.inerAng, N)) ||
         (r = h->alloc(B.vel, N)) || (r = h->alloc(B.angVel, N)) || (r = h->alloc(B.prevVel, N)) ||
         (r = h->alloc(B.massMoment, N)) || (r = h->alloc(B.sizeRad, N)) || (r = h->alloc(B.friction, N)) ||
         (r = h->alloc(B.denseId, N)))
@@ -555,13 +555,13 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     int tiles = ceil_div(N, kRadixTile);
     if ((r = h->alloc(S.axisSums, 64 * 6)) || (r = h->alloc(S.adjPartner, h->maxAdj)) ||
         (r = h->alloc(S.stampCount, N)) || (r = h->alloc(S.stampStart, N)) || (r = h->alloc(S.stamps, h->maxStamps)) ||
-        (r = h->alloc(S.contactStart, h->maxManifolds)) ||
-        (r = h->alloc(S.contactOwner, (size_t)kMaxContacts * h->maxManifolds)) || (r = h->alloc(S.keys, N)) || (r = h->alloc(S.keysAlt, N)) ||
+        (r = h->alloc(S.tasks, N)) || (r = h->alloc(S.contactStart, h->maxManifolds)) ||
+        (r = h->alloc(S.contactRec, (size_t)kMaxContacts * h->maxManifolds)) || (r = h->alloc(S.keys, N)) || (r = h->alloc(S.keysAlt, N)) ||
         (r = h->alloc(S.vals, N)) || (r = h->alloc(S.valsAlt, N)) || (r = h->alloc(S.radixHist, 256 * tiles)) ||
         (r = h->alloc(h->radixHistScan, 256 * tiles)) || (r = h->alloc(S.sLo, N)) || (r = h->alloc(S.sHi, N)) ||
         (r = h->alloc(S.sPos, N)) || (r = h->alloc(S.sIdx, N)) || (r = h->alloc(S.chunkCount, N)) ||
         (r = h->alloc(S.chunkStart, N)) || (r = h->alloc(h->sweepEnd, N)) || (r = h->alloc(S.hitCount, h->maxChunks)) ||
-        (r = h->alloc(S.hitStart, h->maxChunks)) || (r = h->alloc(S.hits, h->maxPairs)) ||
+        (r = h->alloc(S.hitStart, h->maxChunks)) || (r = h->alloc(S.hitMask, h->maxChunks)) || (r = h->alloc(S.hits, h->maxPairs)) ||
         (r = h->alloc(S.hitNew, h->maxPairs)) || (r = h->alloc(S.hitRank, h->maxPairs)) ||
         (r = h->alloc(S.hash, h->hashSize)) || (r = h->alloc(S.keep, h->maxManifolds)) ||
         (r = h->alloc(S.keepRank, h->maxManifolds)) || (r = h->alloc(S.deg, N)) || (r = h->alloc(S.adjStart, N)) ||
@@ -583,7 +583,8 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     if ((e = hipGetDeviceProperties(&prop, h->device)) != hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e)));
     int perCuCol = 0;
-    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCuCol, (const void *)k_color, 256, 0)) != hipSuccess)
+    if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCuCol, (const void *)k_color, kColorThreads, 0)) !=
+        hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipOccupancy: ") + hipGetErrorString(e)));
     if (perCuCol < 1)
         return bail(fail(AVBD_ERR_HIP, "the colouring kernel does not fit on a compute unit"));
@@ -698,9 +699,16 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
         return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
     };
     size_t f4 = sizeof(float4) * N;
+    std::vector<float4> pose(4 * (size_t)N);
+    for (int i = 0; i < N; ++i)
+    {
+        pose[4 * i + P_POS] = pos[i];
+        pose[4 * i + P_ROT] = rot[i];
+        pose[4 * i + P_ILIN] = il[i];
+        pose[4 * i + P_IANG] = ia[i];
+    }
     hipError_t e;
-    if ((e = up(h->B.pos, pos.data(), f4)) || (e = up(h->B.rot, rot.data(), f4)) || (e = up(h->B.initLin, il.data(), f4)) ||
-        (e = up(h->B.initAng, ia.data(), f4)) || (e = up(h->B.inerLin, nl.data(), f4)) ||
+    if ((e = up(h->B.pose, pose.data(), 4 * f4)) || (e = up(h->B.inerLin, nl.data(), f4)) ||
         (e = up(h->B.inerAng, na.data(), f4)) || (e = up(h->B.vel, vel.data(), f4)) || (e = up(h->B.angVel, av.data(), f4)) ||
         (e = up(h->B.prevVel, pv.data(), f4)) || (e = up(h->B.massMoment, mm.data(), f4)) ||
         (e = up(h->B.sizeRad, sr.data(), f4)) || (e = up(h->B.friction, fr.data(), sizeof(float) * N)) ||
@@ -752,39 +760,35 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     std::vector<int2> ab(M);
     std::vector<long long> serial(M);
     std::vector<int> cnt(M);
-    std::vector<float4> b0(M), b1(M), b2(M);
     size_t CS = (size_t)M * kMaxContacts;
     std::vector<int> feat(CS, 0);
-    std::vector<float4> rA(CS), rB(CS), c0(CS), pen(CS), lam(CS);
+    std::vector<float4> cr(CS * kCR, make_float4(0, 0, 0, 0));
     for (int k = 0; k < M; ++k)
     {
         const avbd_manifold &m = manifolds[order[k]];
         ab[k] = make_int2(m.body_a, m.body_b);
         serial[k] = m.serial;
         cnt[k] = m.num_contacts;
-        b0[k] = make_float4(m.basis[0], m.basis[1], m.basis[2], m.friction);
-        b1[k] = make_float4(m.basis[3], m.basis[4], m.basis[5], 0.0f);
-        b2[k] = make_float4(m.basis[6], m.basis[7], m.basis[8], 0.0f);
         for (int c = 0; c < kMaxContacts; ++c)
         {
             const avbd_contact &ct = m.contacts[c];
             size_t sl = (size_t)k * kMaxContacts + c;
             feat[sl] = ct.feature;
-            rA[sl] = make_float4(ct.r_a[0], ct.r_a[1], ct.r_a[2], 0.0f);
-            rB[sl] = make_float4(ct.r_b[0], ct.r_b[1], ct.r_b[2], 0.0f);
-            c0[sl] = make_float4(ct.c0[0], ct.c0[1], ct.c0[2], 0.0f);
-            pen[sl] = make_float4(ct.penalty[0], ct.penalty[1], ct.penalty[2], 0.0f);
-            lam[sl] = make_float4(ct.lambda[0], ct.lambda[1], ct.lambda[2], ct.stick ? 1.0f : 0.0f);
+            float4 *o = &cr[sl * kCR];
+            o[CR_RA] = make_float4(ct.r_a[0], ct.r_a[1], ct.r_a[2], m.friction);
+            o[CR_RB] = make_float4(ct.r_b[0], ct.r_b[1], ct.r_b[2], 0.0f);
+            o[CR_C0] = make_float4(ct.c0[0], ct.c0[1], ct.c0[2], 0.0f);
+            o[CR_N] = make_float4(m.basis[0], m.basis[1], m.basis[2], 0.0f);
+            o[CR_T1] = make_float4(m.basis[3], m.basis[4], m.basis[5], 0.0f);
+            o[CR_T2] = make_float4(m.basis[6], m.basis[7], m.basis[8], 0.0f);
+            o[CR_PEN] = make_float4(ct.penalty[0], ct.penalty[1], ct.penalty[2], 0.0f);
+            o[CR_LAM] = make_float4(ct.lambda[0], ct.lambda[1], ct.lambda[2], ct.stick ? 1.0f : 0.0f);
         }
     }
     Manifolds &C = h->cur;
-    size_t cf4 = sizeof(float4) * CS;
     if ((e = up(C.ab, ab.data(), sizeof(int2) * M)) || (e = up(C.serial, serial.data(), sizeof(long long) * M)) ||
-        (e = up(C.count, cnt.data(), sizeof(int) * M)) || (e = up(C.basis0, b0.data(), sizeof(float4) * M)) ||
-        (e = up(C.basis1, b1.data(), sizeof(float4) * M)) || (e = up(C.basis2, b2.data(), sizeof(float4) * M)) ||
-        (e = up(C.feature, feat.data(), sizeof(int) * CS)) || (e = up(C.rA, rA.data(), cf4)) ||
-        (e = up(C.rB, rB.data(), cf4)) || (e = up(C.c0, c0.data(), cf4)) || (e = up(C.pen, pen.data(), cf4)) ||
-        (e = up(C.lam, lam.data(), cf4)))
+        (e = up(C.count, cnt.data(), sizeof(int) * M)) || (e = up(C.feature, feat.data(), sizeof(int) * CS)) ||
+        (e = up(C.cr, cr.data(), sizeof(float4) * CS * kCR)))
         return fail(AVBD_ERR_HIP, std::string("upload manifolds: ") + hipGetErrorString(e));
     Counters c;
     std::memset(&c, 0, sizeof(c));
@@ -850,9 +854,17 @@ int avbd_download_state(avbd_handle *h, avbd_body *bodies, avbd_joint *joints, a
     if (bodies && N)
     {
         std::vector<float4> v[11];
-        float4 *src[11] = {h->B.pos, h->B.rot, h->B.initLin, h->B.initAng, h->B.inerLin, h->B.inerAng,
+        std::vector<float4> pose(4 * (size_t)N);
+        HIP_TRY(hipMemcpy(pose.data(), h->B.pose, sizeof(float4) * 4 * N, hipMemcpyDeviceToHost));
+        for (int k = 0; k < 4; ++k)
+        {
+            v[k].resize(N);
+            for (int i = 0; i < N; ++i)
+                v[k][i] = pose[4 * i + k];
+        }
+        float4 *src[11] = {nullptr, nullptr, nullptr, nullptr, h->B.inerLin, h->B.inerAng,
                            h->B.vel, h->B.angVel, h->B.prevVel, h->B.massMoment, h->B.sizeRad};
-        for (int k = 0; k < 11; ++k)
+        for (int k = 4; k < 11; ++k)
         {
             v[k].resize(N);
             HIP_TRY(hipMemcpy(v[k].data(), src[k], sizeof(float4) * N, hipMemcpyDeviceToHost));
@@ -924,20 +936,13 @@ int avbd_download_state(avbd_handle *h, avbd_body *bodies, avbd_joint *joints, a
             std::vector<int2> ab(M);
             std::vector<long long> serial(M);
             std::vector<int> cnt(M), feat(CS);
-            std::vector<float4> b0(M), b1(M), b2(M), rA(CS), rB(CS), c0(CS), pen(CS), lam(CS);
+            std::vector<float4> cr(CS * kCR);
             const Manifolds &C = h->cur;
             HIP_TRY(hipMemcpy(ab.data(), C.ab, sizeof(int2) * M, hipMemcpyDeviceToHost));
             HIP_TRY(hipMemcpy(serial.data(), C.serial, sizeof(long long) * M, hipMemcpyDeviceToHost));
             HIP_TRY(hipMemcpy(cnt.data(), C.count, sizeof(int) * M, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(b0.data(), C.basis0, sizeof(float4) * M, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(b1.data(), C.basis1, sizeof(float4) * M, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(b2.data(), C.basis2, sizeof(float4) * M, hipMemcpyDeviceToHost));
             HIP_TRY(hipMemcpy(feat.data(), C.feature, sizeof(int) * CS, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(rA.data(), C.rA, sizeof(float4) * CS, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(rB.data(), C.rB, sizeof(float4) * CS, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(c0.data(), C.c0, sizeof(float4) * CS, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(pen.data(), C.pen, sizeof(float4) * CS, hipMemcpyDeviceToHost));
-            HIP_TRY(hipMemcpy(lam.data(), C.lam, sizeof(float4) * CS, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(cr.data(), C.cr, sizeof(float4) * CS * kCR, hipMemcpyDeviceToHost));
             for (int k = 0; k < M; ++k)
             {
                 avbd_manifold &o = manifolds[k];
@@ -946,20 +951,25 @@ int avbd_download_state(avbd_handle *h, avbd_body *bodies, avbd_joint *joints, a
                 o.body_b = ab[k].y;
                 o.serial = serial[k];
                 o.num_contacts = cnt[k];
-                o.friction = b0[k].w;
-                float4 bs[3] = {b0[k], b1[k], b2[k]};
-                for (int q = 0; q < 3; ++q)
+                if (cnt[k] > 0)
                 {
-                    o.basis[3 * q] = bs[q].x;
-                    o.basis[3 * q + 1] = bs[q].y;
-                    o.basis[3 * q + 2] = bs[q].z;
+                    const float4 *r0 = &cr[(size_t)k * kMaxContacts * kCR];
+                    o.friction = r0[CR_RA].w;
+                    const float4 *bs[3] = {&r0[CR_N], &r0[CR_T1], &r0[CR_T2]};
+                    for (int q = 0; q < 3; ++q)
+                    {
+                        o.basis[3 * q] = bs[q]->x;
+                        o.basis[3 * q + 1] = bs[q]->y;
+                        o.basis[3 * q + 2] = bs[q]->z;
+                    }
                 }
                 for (int q = 0; q < cnt[k]; ++q)
                 {
                     size_t sl = (size_t)k * kMaxContacts + q;
+                    const float4 *r = &cr[sl * kCR];
                     avbd_contact &ct = o.contacts[q];
                     ct.feature = feat[sl];
-                    const float4 *f[5] = {&rA[sl], &rB[sl], &c0[sl], &pen[sl], &lam[sl]};
+                    const float4 *f[5] = {&r[CR_RA], &r[CR_RB], &r[CR_C0], &r[CR_PEN], &r[CR_LAM]};
                     float *d[5] = {ct.r_a, ct.r_b, ct.c0, ct.penalty, ct.lambda};
                     for (int z = 0; z < 5; ++z)
                     {
@@ -967,7 +977,7 @@ int avbd_download_state(avbd_handle *h, avbd_body *bodies, avbd_joint *joints, a
                         d[z][1] = f[z]->y;
                         d[z][2] = f[z]->z;
                     }
-                    ct.stick = lam[sl].w != 0.0f ? 1 : 0;
+                    ct.stick = r[CR_LAM].w != 0.0f ? 1 : 0;
                 }
             }
         }

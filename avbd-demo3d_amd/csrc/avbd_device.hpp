@@ -27,12 +27,19 @@ struct Params
     int iterations;
 };
 
+// Gathered per-body pose record: the four float4 the solver iterations read for a
+// neighbour sit in one 64-byte record, pose[4 i + k].
+enum
+{
+    P_POS = 0,   // positionLin
+    P_ROT = 1,   // positionAng (x, y, z, w)
+    P_ILIN = 2,  // x- linear
+    P_IANG = 3   // x- angular
+};
+
 struct Bodies
 {
-    float4 *pos;      // xyz
-    float4 *rot;      // quaternion xyzw
-    float4 *initLin;  // x-
-    float4 *initAng;
+    float4 *pose;     // [4 N] records of P_POS, P_ROT, P_ILIN, P_IANG
     float4 *inerLin;  // x~
     float4 *inerAng;
     float4 *vel;
@@ -44,20 +51,29 @@ struct Bodies
     int *denseId;
 };
 
+// One 128-byte record per contact slot (8 float4): everything a primal stamp or a
+// dual update of the contact reads, in one cache line. The manifold's basis and
+// friction are repeated in each of its contacts.
+constexpr int kCR = 8;
+enum
+{
+    CR_RA = 0,   // rA (A's local frame), w = manifold friction
+    CR_RB = 1,   // rB (B's local frame)
+    CR_C0 = 2,   // C0 = basis (xA - xB) + (margin, 0, 0) at x-
+    CR_N = 3,    // basis row 0: normal (B to A)
+    CR_T1 = 4,   // basis row 1
+    CR_T2 = 5,   // basis row 2
+    CR_PEN = 6,  // penalty
+    CR_LAM = 7   // lambda, w = stick (0/1)
+};
+
 struct Manifolds
 {
     int2 *ab;            // body A, body B
     long long *serial;
     int *count;          // live contacts
-    float4 *basis0;      // normal, w = friction
-    float4 *basis1;      // tangent 1
-    float4 *basis2;      // tangent 2
     int *feature;        // [cap * 8]
-    float4 *rA;          // [cap * 8] local offsets
-    float4 *rB;
-    float4 *c0;          // constraint at x- (+margin)
-    float4 *pen;         // penalty
-    float4 *lam;         // lambda xyz, w = stick (0/1)
+    float4 *cr;          // [cap * 8 * kCR] contact records
 };
 
 struct Joint
@@ -139,6 +155,7 @@ struct Scratch
     int *chunkCount;        // [N] work items per interval
     int *chunkStart;        // [N]
     int *hitCount;          // [maxChunks] hits per work item
+    unsigned long long *hitMask; // [maxChunks] bit k: the k-th interval visited by the work item is a hit
     int *hitStart;
     int2 *hits;             // [maxPairs]
     int *hitNew;            // [maxPairs] 1 = unconstrained
@@ -159,9 +176,10 @@ struct Scratch
     // primal stamp lists and the per-contact dual index
     int *stampCount;        // [N]
     int *stampStart;        // [N]
-    int2 *stamps;           // [maxStamps] (adj entry, part)
+    int4 *stamps;           // [maxStamps] (adj entry, part | isA << 16, partner body, contact slot)
+    int4 *tasks;            // [N] colour-sorted (body, first stamp, stamp count, 0)
     int *contactStart;      // [maxManifolds]
-    int *contactOwner;      // [maxManifolds * 8]
+    int4 *contactRec;       // [maxManifolds * 8] (manifold, contact, body A, body B)
     // colouring
     int *color;             // [N]
     int *colorOffsets;      // [256]

@@ -270,13 +270,13 @@ __global__ void k_step_begin(Counters *c)
 // (solver.cpp:499-515). The decision is taken in double; only when two variances lie within
 // the float sums' error bound of each other is the exact serial float pass run.
 // ------------------------------------------------------------------------------------------
-__global__ void k_axis_partial(const float4 *pos, int n, double *partials)
+__global__ void k_axis_partial(const float4 *pose, int n, double *partials)
 {
     __shared__ double red[6][256];
     double a[6] = {0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     {
-        float4 p = pos[i];
+        float4 p = pose[4 * i + P_POS];
         a[0] += p.x;
         a[1] += p.y;
         a[2] += p.z;
@@ -328,18 +328,18 @@ __global__ void k_axis_decide(const double *partials, int nparts, int n, Counter
     ctr->axisAmbiguous = amb;
 }
 
-__global__ void k_axis_serial(const float4 *pos, int n, Counters *ctr)
+__global__ void k_axis_serial(const float4 *pose, int n, Counters *ctr)
 {
     if (threadIdx.x != 0 || !ctr->axisAmbiguous)
         return;
     V3 mean{0.0f, 0.0f, 0.0f};
     for (int i = 0; i < n; ++i)
-        mean += ld3(pos[i]);
+        mean += ld3(pose[4 * i + P_POS]);
     mean = mean / (float)n;
     V3 var{0.0f, 0.0f, 0.0f};
     for (int i = 0; i < n; ++i)
     {
-        V3 d = ld3(pos[i]) - mean;
+        V3 d = ld3(pose[4 * i + P_POS]) - mean;
         var += V3{d.x * d.x, d.y * d.y, d.z * d.z};
     }
     int best = 0;
@@ -353,14 +353,14 @@ __global__ void k_axis_serial(const float4 *pos, int n, Counters *ctr)
 
 // Interval start keys: order-preserving float bits, -0 folded onto +0 so that ties fall to the
 // body index exactly as the reference comparator does (solver.cpp:525-531).
-__global__ void k_sap_keys(const float4 *pos, const float4 *sizeRad, int n, const Counters *ctr, unsigned *keys,
+__global__ void k_sap_keys(const float4 *pose, const float4 *sizeRad, int n, const Counters *ctr, unsigned *keys,
                            int *vals)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     int axis = ctr->sweepAxis;
-    float c = comp(ld3(pos[i]), axis);
+    float c = comp(ld3(pose[4 * i + P_POS]), axis);
     float lo = c - sizeRad[i].w;
     if (lo == 0.0f)
         lo = 0.0f;
@@ -369,14 +369,14 @@ __global__ void k_sap_keys(const float4 *pos, const float4 *sizeRad, int n, cons
     vals[i] = i;
 }
 
-__global__ void k_sap_gather(const int *sortedIdx, const float4 *pos, const float4 *sizeRad, int n, const Counters *ctr,
+__global__ void k_sap_gather(const int *sortedIdx, const float4 *pose, const float4 *sizeRad, int n, const Counters *ctr,
                              float *sLo, float *sHi, float4 *sPos, int *sIdx)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n)
         return;
     int i = sortedIdx[k];
-    float4 p = pos[i];
+    float4 p = pose[4 * i + P_POS];
     float r = sizeRad[i].w;
     float c = comp(ld3(p), ctr->sweepAxis);
     sLo[k] = c - r;
@@ -435,7 +435,7 @@ __device__ __forceinline__ bool sphere_hit(float4 a, float4 b)
 }
 
 __global__ void k_sweep_count(const int *chunkStart, const int *sweepEnd, int n, const Counters *ctr,
-                              const float4 *sPos, int *hitCount)
+                              const float4 *sPos, int *hitCount, unsigned long long *hitMask)
 {
     int T = ctr->numChunks;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x)
@@ -444,10 +444,12 @@ __global__ void k_sweep_count(const int *chunkStart, const int *sweepEnd, int n,
         int j0 = i + 1 + (t - chunkStart[i]) * kSweepChunk;
         int j1 = min(j0 + kSweepChunk, sweepEnd[i]);
         float4 a = sPos[i];
-        int c = 0;
+        unsigned long long mask = 0ull;
         for (int j = j0; j < j1; ++j)
-            c += sphere_hit(a, sPos[j]) ? 1 : 0;
-        hitCount[t] = c;
+            if (sphere_hit(a, sPos[j]))
+                mask |= 1ull << (j - j0);
+        hitCount[t] = __popcll(mask);
+        hitMask[t] = mask;
     }
 }
 
@@ -530,23 +532,24 @@ __global__ void k_hash_insert(const int2 *mab, const Counters *ctr, const Joint 
     }
 }
 
-__global__ void k_sweep_emit(const int *chunkStart, const int *sweepEnd, int n, Counters *ctr, const float4 *sPos,
+__global__ void k_sweep_emit(const int *chunkStart, int n, Counters *ctr, const unsigned long long *hitMask,
                              const int *sIdx, const int *hitStart, const unsigned long long *h, int mask, int2 *hits,
                              int *hitNew, int maxPairs)
 {
     int T = ctr->numChunks;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x)
     {
+        unsigned long long bits = hitMask[t];
+        if (!bits)
+            continue;
         int i = chunk_owner(chunkStart, n, t);
         int j0 = i + 1 + (t - chunkStart[i]) * kSweepChunk;
-        int j1 = min(j0 + kSweepChunk, sweepEnd[i]);
-        float4 a = sPos[i];
         int ia = sIdx[i];
         int out = hitStart[t];
-        for (int j = j0; j < j1; ++j)
+        while (bits)
         {
-            if (!sphere_hit(a, sPos[j]))
-                continue;
+            int j = j0 + __ffsll((long long)bits) - 1;
+            bits &= bits - 1ull;
             if (out < maxPairs)
             {
                 int ib = sIdx[j];
@@ -923,8 +926,8 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
     {
         int2 ab = cur.ab[m];
         int nOld = m < old ? cur.count[m] : 0;
-        V3 pA = ld3(B.pos[ab.x]), pB = ld3(B.pos[ab.y]);
-        Q4 qA = ldq(B.rot[ab.x]), qB = ldq(B.rot[ab.y]);
+        V3 pA = ld3(B.pose[4 * (ab.x) + P_POS]), pB = ld3(B.pose[4 * (ab.y) + P_POS]);
+        Q4 qA = ldq(B.pose[4 * (ab.x) + P_ROT]), qB = ldq(B.pose[4 * (ab.y) + P_ROT]);
         V3 sA = ld3(B.sizeRad[ab.x]), sB = ld3(B.sizeRad[ab.y]);
         float friction = sqrtf(B.friction[ab.x] * B.friction[ab.y]);
         NewContacts nc;
@@ -939,9 +942,6 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
         tmp.ab[m] = ab;
         tmp.serial[m] = cur.serial[m];
         tmp.count[m] = nn;
-        tmp.basis0[m] = st3(b0, friction);
-        tmp.basis1[m] = st3(b1, 0.0f);
-        tmp.basis2[m] = st3(b2, 0.0f);
         for (int i = 0; i < nn; ++i)
         {
             V3 rA = nc.rA[i], rB = nc.rB[i];
@@ -950,13 +950,13 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
             {
                 if (cur.feature[m * kMaxContacts + j] == nc.feature[i])
                 {
-                    int s = m * kMaxContacts + j;
-                    pen = cur.pen[s];
-                    lam = cur.lam[s];
+                    const float4 *o = cur.cr + (size_t)(m * kMaxContacts + j) * kCR;
+                    pen = o[CR_PEN];
+                    lam = o[CR_LAM];
                     if (lam.w != 0.0f)
                     {
-                        rA = ld3(cur.rA[s]);
-                        rB = ld3(cur.rB[s]);
+                        rA = ld3(o[CR_RA]);
+                        rB = ld3(o[CR_RB]);
                     }
                     break;
                 }
@@ -969,13 +969,17 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
             V3 k = V3{pen.x, pen.y, pen.z} * P.gamma;
             k = V3{clampr(k.x, kPenaltyMin, kPenaltyMax), clampr(k.y, kPenaltyMin, kPenaltyMax),
                    clampr(k.z, kPenaltyMin, kPenaltyMax)};
-            int s = m * kMaxContacts + i;
-            tmp.feature[s] = nc.feature[i];
-            tmp.rA[s] = st3(rA, 0.0f);
-            tmp.rB[s] = st3(rB, 0.0f);
-            tmp.c0[s] = st3(c0, 0.0f);
-            tmp.pen[s] = st3(k, 0.0f);
-            tmp.lam[s] = st3(l, lam.w);
+            int sl = m * kMaxContacts + i;
+            tmp.feature[sl] = nc.feature[i];
+            float4 *o = tmp.cr + (size_t)sl * kCR;
+            o[CR_RA] = st3(rA, friction);
+            o[CR_RB] = st3(rB, 0.0f);
+            o[CR_C0] = st3(c0, 0.0f);
+            o[CR_N] = st3(b0, 0.0f);
+            o[CR_T1] = st3(b1, 0.0f);
+            o[CR_T2] = st3(b2, 0.0f);
+            o[CR_PEN] = st3(k, 0.0f);
+            o[CR_LAM] = st3(l, lam.w);
         }
     }
 }
@@ -997,18 +1001,15 @@ __global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *kee
         cur.ab[d] = ab;
         cur.serial[d] = tmp.serial[m];
         cur.count[d] = n;
-        cur.basis0[d] = tmp.basis0[m];
-        cur.basis1[d] = tmp.basis1[m];
-        cur.basis2[d] = tmp.basis2[m];
         for (int i = 0; i < n; ++i)
         {
             int s = m * kMaxContacts + i, o = d * kMaxContacts + i;
             cur.feature[o] = tmp.feature[s];
-            cur.rA[o] = tmp.rA[s];
-            cur.rB[o] = tmp.rB[s];
-            cur.c0[o] = tmp.c0[s];
-            cur.pen[o] = tmp.pen[s];
-            cur.lam[o] = tmp.lam[s];
+            const float4 *src = tmp.cr + (size_t)s * kCR;
+            float4 *dst = cur.cr + (size_t)o * kCR;
+#pragma unroll
+            for (int q = 0; q < kCR; ++q)
+                dst[q] = src[q];
         }
     }
     contacts = wave_sum(contacts);
@@ -1029,11 +1030,11 @@ __global__ void k_joint_init(Joint *joints, int nj, Bodies B, Params P, Counters
     Joint j = joints[k];
     if (j.dead)
         return;
-    V3 pb = rotate(ldq(B.rot[j.b]), ld3(j.rB)) + ld3(B.pos[j.b]);
-    V3 pa = j.a >= 0 ? rotate(ldq(B.rot[j.a]), ld3(j.rA)) + ld3(B.pos[j.a]) : ld3(j.rA);
+    V3 pb = rotate(ldq(B.pose[4 * (j.b) + P_ROT]), ld3(j.rB)) + ld3(B.pose[4 * (j.b) + P_POS]);
+    V3 pa = j.a >= 0 ? rotate(ldq(B.pose[4 * (j.a) + P_ROT]), ld3(j.rA)) + ld3(B.pose[4 * (j.a) + P_POS]) : ld3(j.rA);
     V3 c0l = pa - pb;
-    Q4 qa = j.a >= 0 ? ldq(B.rot[j.a]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
-    V3 c0a = qdiff(qa, ldq(B.rot[j.b])) * j.c0Lin.w;
+    Q4 qa = j.a >= 0 ? ldq(B.pose[4 * (j.a) + P_ROT]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
+    V3 c0a = qdiff(qa, ldq(B.pose[4 * (j.b) + P_ROT])) * j.c0Lin.w;
     V3 ll = ld3(j.lamLin) * P.alpha * P.gamma;
     V3 la = ld3(j.lamAng) * P.alpha * P.gamma;
     V3 pl = ld3(j.penLin) * P.gamma;
@@ -1185,8 +1186,8 @@ __global__ void k_warmstart(Bodies B, Params P, const int *deg, int n, int *colo
     if (i >= n)
         return;
     float mass = B.massMoment[i].x;
-    V3 pos = ld3(B.pos[i]), vel = ld3(B.vel[i]), w = ld3(B.angVel[i]);
-    Q4 rot = ldq(B.rot[i]);
+    V3 pos = ld3(B.pose[4 * (i) + P_POS]), vel = ld3(B.vel[i]), w = ld3(B.angVel[i]);
+    Q4 rot = ldq(B.pose[4 * (i) + P_ROT]);
     float dt = P.dt, g = P.gravity;
     V3 iner = pos + vel * dt;
     if (mass > 0.0f)
@@ -1197,8 +1198,8 @@ __global__ void k_warmstart(Bodies B, Params P, const int *deg, int n, int *colo
     float aw = clampr(accelExt / fabsf(g), 0.0f, 1.0f);
     if (!isfinite(aw))
         aw = 0.0f;
-    B.initLin[i] = st3(pos, 0.0f);
-    B.initAng[i] = stq(rot);
+    B.pose[4 * (i) + P_ILIN] = st3(pos, 0.0f);
+    B.pose[4 * (i) + P_IANG] = stq(rot);
     B.inerLin[i] = st3(iner, 0.0f);
     B.inerAng[i] = stq(inerA);
     bool solved = mass > 0.0f && deg[i] > 0;
@@ -1211,8 +1212,8 @@ __global__ void k_warmstart(Bodies B, Params P, const int *deg, int n, int *colo
             np = iner;
             nq = inerA;
         }
-        B.pos[i] = st3(np, 0.0f);
-        B.rot[i] = stq(nq);
+        B.pose[4 * (i) + P_POS] = st3(np, 0.0f);
+        B.pose[4 * (i) + P_ROT] = stq(nq);
     }
     if (!solved)
         color[i] = -1;
@@ -1289,14 +1290,18 @@ __global__ void k_color(const int *deg, const int *adjStart, const int *adjPartn
         ctr->coloringRounds = rounds;
 }
 
-__global__ void k_color_keys(const int *color, int n, unsigned *keys, int *vals, Counters *ctr)
+// Sort keys of the colour buckets: colour in bits 8-15, and 255 - stamp count in bits 0-7, so
+// that inside a colour the bodies with the most stamps come first (the order inside a colour
+// does not change any result: its bodies share no force).
+__global__ void k_color_keys(const int *color, const int *stampCount, int n, unsigned *keys, int *vals, Counters *ctr)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int mx = -1;
     if (i < n)
     {
         int c = color[i];
-        keys[i] = c < 0 ? 255u : (unsigned)min(c, kMaxColors - 1);
+        unsigned col = c < 0 ? 255u : (unsigned)min(c, kMaxColors - 1);
+        keys[i] = (col << 8) | (unsigned)(255 - min(stampCount[i], 255));
         vals[i] = i;
         mx = c;
     }
@@ -1410,16 +1415,15 @@ __device__ __forceinline__ void rows_record(float *r, V3 l0, V3 l1, V3 l2, V3 g0
 }
 
 // One contact of a manifold as seen by body i (Manifold::updatePrimal, manifold.cpp:176-229).
-__device__ __forceinline__ void contact_record(float *r, int i, int m, int c, const Manifolds &M, const Bodies &B,
-                                               float oma)
+__device__ __forceinline__ void contact_record(float *r, int i, int partner, bool isA, int sl, const Manifolds &M,
+                                               const Bodies &B, float oma)
 {
-    int2 ab = M.ab[m];
-    int sl = m * kMaxContacts + c;
-    float4 bb0 = M.basis0[m];
-    float4 bb1 = M.basis1[m], bb2 = M.basis2[m];
-    float4 rA4 = M.rA[sl], rB4 = M.rB[sl], c04 = M.c0[sl], pen4 = M.pen[sl], lam4 = M.lam[sl];
-    float4 pA4 = B.pos[ab.x], pB4 = B.pos[ab.y], qA4 = B.rot[ab.x], qB4 = B.rot[ab.y];
-    float4 iA4 = B.initLin[ab.x], iB4 = B.initLin[ab.y], jA4 = B.initAng[ab.x], jB4 = B.initAng[ab.y];
+    int2 ab = isA ? make_int2(i, partner) : make_int2(partner, i);
+    const float4 *cr = M.cr + (size_t)sl * kCR;
+    float4 rA4 = cr[CR_RA], rB4 = cr[CR_RB], c04 = cr[CR_C0], bb0 = cr[CR_N], bb1 = cr[CR_T1], bb2 = cr[CR_T2];
+    float4 pen4 = cr[CR_PEN], lam4 = cr[CR_LAM];
+    float4 pA4 = B.pose[4 * (ab.x) + P_POS], pB4 = B.pose[4 * (ab.y) + P_POS], qA4 = B.pose[4 * (ab.x) + P_ROT], qB4 = B.pose[4 * (ab.y) + P_ROT];
+    float4 iA4 = B.pose[4 * (ab.x) + P_ILIN], iB4 = B.pose[4 * (ab.y) + P_ILIN], jA4 = B.pose[4 * (ab.x) + P_IANG], jB4 = B.pose[4 * (ab.y) + P_IANG];
     Q4 qA = ldq(qA4), qB = ldq(qB4);
     V3 dqAL = ld3(pA4) - ld3(iA4);
     V3 dqAA = qdiff(qA, ldq(jA4));
@@ -1432,9 +1436,9 @@ __device__ __forceinline__ void contact_record(float *r, int i, int m, int c, co
     V3 ja0 = cross(rAW, b0), ja1 = cross(rAW, b1), ja2 = cross(rAW, b2);
     V3 jb0 = cross(rBW, nb0), jb1 = cross(rBW, nb1), jb2 = cross(rBW, nb2);
     V3 pen = ld3(pen4);
-    ContactEval ev = contact_eval(ld3(c04), pen, ld3(lam4), bb0.w, oma, b0, b1, b2, nb0, nb1, nb2, ja0, ja1, ja2, jb0,
+    ContactEval ev = contact_eval(ld3(c04), pen, ld3(lam4), rA4.w, oma, b0, b1, b2, nb0, nb1, nb2, ja0, ja1, ja2, jb0,
                                   jb1, jb2, dqAL, dqBL, dqAA, dqBA);
-    if (ab.x == i)
+    if (isA)
         rows_record(r, b0, b1, b2, ja0, ja1, ja2, pen, ev.F0, ev.F1, ev.F2);
     else
         rows_record(r, nb0, nb1, nb2, jb0, jb1, jb2, pen, ev.F0, ev.F1, ev.F2);
@@ -1518,17 +1522,17 @@ __device__ __forceinline__ void rec_ang(float *r, const D3 &a)
 __device__ __forceinline__ void joint_record(float *r, int i, const Joint &j, int part, const Bodies &B, float alpha)
 {
     bool isA = j.a == i;
-    Q4 qB = ldq(B.rot[j.b]);
-    V3 pB = ld3(B.pos[j.b]);
+    Q4 qB = ldq(B.pose[4 * (j.b) + P_ROT]);
+    V3 pB = ld3(B.pose[4 * (j.b) + P_POS]);
     V3 penL = ld3(j.penLin), penA = ld3(j.penAng);
     if (part < 2)
     {
         if (!(lensq(penL) > 0.0f))
             return;
-        V3 rAW = j.a >= 0 ? rotate(ldq(B.rot[j.a]), ld3(j.rA)) : V3{0.0f, 0.0f, 0.0f};
+        V3 rAW = j.a >= 0 ? rotate(ldq(B.pose[4 * (j.a) + P_ROT]), ld3(j.rA)) : V3{0.0f, 0.0f, 0.0f};
         V3 rBW = rotate(qB, ld3(j.rB));
         D3 K = d3_diag(penL.x, penL.y, penL.z);
-        V3 C = (j.a >= 0 ? ld3(B.pos[j.a]) + rAW : ld3(j.rA)) - (pB + rBW);
+        V3 C = (j.a >= 0 ? ld3(B.pose[4 * (j.a) + P_POS]) + rAW : ld3(j.rA)) - (pB + rBW);
         if (isinf(j.rA.w))
             C -= ld3(j.c0Lin) * alpha;
         V3 F = d3_mv(K, C) + ld3(j.lamLin);
@@ -1569,7 +1573,7 @@ __device__ __forceinline__ void joint_record(float *r, int i, const Joint &j, in
     if (!(lensq(penA) > 0.0f))
         return;
     D3 K = d3_diag(penA.x, penA.y, penA.z);
-    Q4 qa = j.a >= 0 ? ldq(B.rot[j.a]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
+    Q4 qa = j.a >= 0 ? ldq(B.pose[4 * (j.a) + P_ROT]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
     V3 C = qdiff(qa, qB) * j.c0Lin.w;
     if (isinf(j.rB.w))
         C -= ld3(j.c0Ang) * alpha;
@@ -1586,8 +1590,8 @@ __device__ __forceinline__ void joint_record(float *r, int i, const Joint &j, in
 // Spring::updatePrimal (spring.cpp:27-69)
 __device__ __forceinline__ void spring_record(float *r, int i, const Spring &sp, const Bodies &B)
 {
-    Q4 qA = ldq(B.rot[sp.a]), qB = ldq(B.rot[sp.b]);
-    V3 d = (rotate(qA, ld3(sp.rA)) + ld3(B.pos[sp.a])) - (rotate(qB, ld3(sp.rB)) + ld3(B.pos[sp.b]));
+    Q4 qA = ldq(B.pose[4 * (sp.a) + P_ROT]), qB = ldq(B.pose[4 * (sp.b) + P_ROT]);
+    V3 d = (rotate(qA, ld3(sp.rA)) + ld3(B.pose[4 * (sp.a) + P_POS])) - (rotate(qB, ld3(sp.rB)) + ld3(B.pose[4 * (sp.b) + P_POS]));
     float dl = len(d);
     if (dl <= 1.0e-6f)
         return;
@@ -1651,8 +1655,9 @@ __global__ void k_stamp_count(const int *deg, const int *adjStart, const int *ad
     stampCount[i] = s;
 }
 
-__global__ void k_stamp_fill(const int *deg, const int *adjStart, const int *adj, const int *mcount, int n,
-                             const int *stampStart, int2 *stamps, int maxStamps, int *overflow)
+__global__ void k_stamp_fill(const int *deg, const int *adjStart, const int *adj, const int *adjPartner,
+                             const int *mcount, const int2 *mab, int n, const int *stampStart, int4 *stamps,
+                             int maxStamps, int *overflow)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -1662,16 +1667,29 @@ __global__ void k_stamp_fill(const int *deg, const int *adjStart, const int *adj
     for (int x = b; x < e; ++x)
     {
         int en = adj[x];
-        int kind = adjKind(en);
-        int parts = kind == kAdjManifold ? mcount[adjIndex(en)] : (kind == kAdjJoint ? 3 : 1);
+        int kind = adjKind(en), idx = adjIndex(en);
+        int parts = kind == kAdjManifold ? mcount[idx] : (kind == kAdjJoint ? 3 : 1);
+        int isA = kind == kAdjManifold ? (mab[idx].x == i ? 1 : 0) : 0;
+        int partner = adjPartner[x];
         for (int p = 0; p < parts; ++p, ++o)
         {
             if (o < maxStamps)
-                stamps[o] = make_int2(en, p);
+                stamps[o] = make_int4(en, p | (isA << 16), partner,
+                                      kind == kAdjManifold ? idx * kMaxContacts + p : -1);
             else
                 atomicOr(overflow, 64);
         }
     }
+}
+
+// Colour-sorted primal tasks: (body, first stamp, stamp count).
+__global__ void k_tasks(const int *colorBodies, const int *stampStart, const int *stampCount, int n, int4 *tasks)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    int i = colorBodies[k];
+    tasks[k] = make_int4(i, stampStart[i], stampCount[i], 0);
 }
 
 __device__ __forceinline__ float init_entry(int e, float ml, float ax, float ay, float az, V3 dl, V3 da)
@@ -1694,29 +1712,36 @@ __device__ __forceinline__ float init_entry(int e, float ml, float ax, float ay,
     }
 }
 
+// LDS hand-off between the lanes of one wave: a group's 16 lanes always sit in one wave, so
+// ordering the wave's own LDS writes before its reads is enough (no block barrier).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Primal update of the bodies of one colour (solver.cpp:1184-1240): kGroupLanes lanes per body.
+// Bodies of a colour are sorted by stamp count, so the groups of a wave need similar round counts.
 __global__ __launch_bounds__(256) void k_primal_color(Bodies B, Manifolds M, const Joint *joints,
-                                                      const Spring *springs, const int *stampStart,
-                                                      const int *stampCount, const int2 *stamps,
-                                                      const int *colorOffsets, const int *colorBodies, Params P,
-                                                      int color)
+                                                      const Spring *springs, const int4 *stamps, const int4 *tasks,
+                                                      int begin, int end, Params P)
 {
     __shared__ float rec[kGroupsPerBlock][kGroupLanes][kRec + 1];
-    __shared__ int roundsShared;
     const int lane = threadIdx.x % kGroupLanes, g = threadIdx.x / kGroupLanes;
-    const int k = colorOffsets[color] + blockIdx.x * kGroupsPerBlock + g;
-    const bool valid = k < colorOffsets[color + 1];
-    const int i = valid ? colorBodies[k] : 0;
-    const int s0 = valid ? stampStart[i] : 0;
-    const int ns = valid ? stampCount[i] : 0;
-    if (threadIdx.x == 0)
-        roundsShared = 0;
-    __syncthreads();
-    if (lane == 0 && valid)
-        atomicMax(&roundsShared, (ns + kGroupLanes - 1) / kGroupLanes);
+    const int k = begin + blockIdx.x * kGroupsPerBlock + g;
+    const bool valid = k < end;
+    const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
+    const int i = task.x;
+    const int s0 = task.y;
+    const int ns = valid ? task.z : 0;
+    int rounds = (ns + kGroupLanes - 1) / kGroupLanes;
+#pragma unroll
+    for (int off = 32; off >= kGroupLanes; off >>= 1)
+        rounds = max(rounds, __shfl_xor(rounds, off, 64));
     float4 mm = B.massMoment[i];
-    V3 pos = ld3(B.pos[i]);
-    Q4 rot = ldq(B.rot[i]);
+    V3 pos = ld3(B.pose[4 * i + P_POS]);
+    Q4 rot = ldq(B.pose[4 * i + P_ROT]);
     float dt2 = P.dt * P.dt;
     float ml = mm.x / dt2, ax = mm.y / dt2, ay = mm.z / dt2, az = mm.w / dt2;
     V3 dl = pos - ld3(B.inerLin[i]);
@@ -1725,8 +1750,6 @@ __global__ __launch_bounds__(256) void k_primal_color(Bodies B, Manifolds M, con
     float acc0 = init_entry(e0, ml, ax, ay, az, dl, da);
     float acc1 = e1 < kRec ? init_entry(e1, ml, ax, ay, az, dl, da) : 0.0f;
     const float oma = 1.0f - P.alpha;
-    __syncthreads();
-    const int rounds = roundsShared;
     for (int r = 0; r < rounds; ++r)
     {
         int t = r * kGroupLanes + lane;
@@ -1736,32 +1759,43 @@ __global__ __launch_bounds__(256) void k_primal_color(Bodies B, Manifolds M, con
             out[q] = 0.0f;
         if (valid && t < ns)
         {
-            int2 st = stamps[s0 + t];
+            int4 st = stamps[s0 + t];
             int kind = adjKind(st.x), idx = adjIndex(st.x);
             if (kind == kAdjManifold)
-                contact_record(out, i, idx, st.y, M, B, oma);
+                contact_record(out, i, st.z, (st.y >> 16) != 0, st.w, M, B, oma);
             else if (kind == kAdjJoint)
-                joint_record(out, i, joints[idx], st.y, B, P.alpha);
+                joint_record(out, i, joints[idx], st.y & 0xFFFF, B, P.alpha);
             else
                 spring_record(out, i, springs[idx], B);
         }
 #pragma unroll
         for (int q = 0; q < kRec; ++q)
             rec[g][lane][q] = out[q];
-        __syncthreads();
+        wave_lds_sync();
+        // the records of this round, added entry by entry in their sequential order
         int cnt = valid ? min(kGroupLanes, ns - r * kGroupLanes) : 0;
-        for (int q = 0; q < cnt; ++q)
+        float v0[kGroupLanes], v1[kGroupLanes];
+#pragma unroll
+        for (int q = 0; q < kGroupLanes; ++q)
         {
-            acc0 += rec[g][q][e0];
-            if (e1 < kRec)
-                acc1 += rec[g][q][e1];
+            v0[q] = rec[g][q][e0];
+            v1[q] = e1 < kRec ? rec[g][q][e1] : 0.0f;
         }
-        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kGroupLanes; ++q)
+        {
+            if (q < cnt)
+            {
+                acc0 += v0[q];
+                acc1 += v1[q];
+            }
+        }
+        wave_lds_sync();
     }
     rec[g][0][e0] = acc0;
     if (e1 < kRec)
         rec[g][0][e1] = acc1;
-    __syncthreads();
+    wave_lds_sync();
     if (valid && lane == 0)
     {
         const float *a = rec[g][0];
@@ -1790,33 +1824,35 @@ __global__ __launch_bounds__(256) void k_primal_color(Bodies B, Manifolds M, con
         V3 rl{a[R_RL], a[R_RL + 1], a[R_RL + 2]}, ra{a[R_RA], a[R_RA + 1], a[R_RA + 2]};
         V3 dxL, dxA;
         solve6(sys, -rl, -ra, dxL, dxA);
-        B.pos[i] = st3(pos + dxL, 0.0f);
-        B.rot[i] = stq(qintegrate(rot, dxA));
+        B.pose[4 * i + P_POS] = st3(pos + dxL, 0.0f);
+        B.pose[4 * i + P_ROT] = stq(qintegrate(rot, dxA));
     }
 }
 
-// Flattened contact index -> owning manifold (for the per-contact dual pass).
-__global__ void k_contact_owner(const int *mcount, const int *contactStart, const Counters *ctr, int *owner)
+// Flattened contact records (manifold, contact, body A, body B) for the per-contact dual pass.
+__global__ void k_contact_records(const int *mcount, const int2 *mab, const int *contactStart, const Counters *ctr,
+                                  int4 *rec)
 {
     int nm = ctr->numManifolds;
     for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < nm; m += gridDim.x * blockDim.x)
     {
         int o = contactStart[m], n = mcount[m];
+        int2 ab = mab[m];
         for (int c = 0; c < n; ++c)
-            owner[o + c] = m;
+            rec[o + c] = make_int4(m, c, ab.x, ab.y);
     }
 }
 
 // Manifold::updateDual for one contact (manifold.cpp:231-280)
-__device__ __forceinline__ void dual_contact(int m, int c, const Manifolds &M, const Bodies &B, const Params &P)
+__device__ __forceinline__ void dual_contact(int4 cr, const Manifolds &M, const Bodies &B, const Params &P)
 {
-    int2 ab = M.ab[m];
-    int sl = m * kMaxContacts + c;
-    float4 bb0 = M.basis0[m];
-    float4 bb1 = M.basis1[m], bb2 = M.basis2[m];
-    float4 rA4 = M.rA[sl], rB4 = M.rB[sl], c04 = M.c0[sl], pen4 = M.pen[sl], lam4 = M.lam[sl];
-    float4 pA4 = B.pos[ab.x], pB4 = B.pos[ab.y], qA4 = B.rot[ab.x], qB4 = B.rot[ab.y];
-    float4 iA4 = B.initLin[ab.x], iB4 = B.initLin[ab.y], jA4 = B.initAng[ab.x], jB4 = B.initAng[ab.y];
+    int2 ab = make_int2(cr.z, cr.w);
+    int sl = cr.x * kMaxContacts + cr.y;
+    float4 *rec = M.cr + (size_t)sl * kCR;
+    float4 rA4 = rec[CR_RA], rB4 = rec[CR_RB], c04 = rec[CR_C0], bb0 = rec[CR_N], bb1 = rec[CR_T1], bb2 = rec[CR_T2];
+    float4 pen4 = rec[CR_PEN], lam4 = rec[CR_LAM];
+    float4 pA4 = B.pose[4 * (ab.x) + P_POS], pB4 = B.pose[4 * (ab.y) + P_POS], qA4 = B.pose[4 * (ab.x) + P_ROT], qB4 = B.pose[4 * (ab.y) + P_ROT];
+    float4 iA4 = B.pose[4 * (ab.x) + P_ILIN], iB4 = B.pose[4 * (ab.y) + P_ILIN], jA4 = B.pose[4 * (ab.x) + P_IANG], jB4 = B.pose[4 * (ab.y) + P_IANG];
     Q4 qA = ldq(qA4), qB = ldq(qB4);
     V3 dqAL = ld3(pA4) - ld3(iA4);
     V3 dqAA = qdiff(qA, ldq(jA4));
@@ -1829,7 +1865,7 @@ __device__ __forceinline__ void dual_contact(int m, int c, const Manifolds &M, c
     V3 ja0 = cross(rAW, b0), ja1 = cross(rAW, b1), ja2 = cross(rAW, b2);
     V3 jb0 = cross(rBW, nb0), jb1 = cross(rBW, nb1), jb2 = cross(rBW, nb2);
     V3 pen = ld3(pen4);
-    ContactEval ev = contact_eval(ld3(c04), pen, ld3(lam4), bb0.w, 1.0f - P.alpha, b0, b1, b2, nb0, nb1, nb2, ja0,
+    ContactEval ev = contact_eval(ld3(c04), pen, ld3(lam4), rA4.w, 1.0f - P.alpha, b0, b1, b2, nb0, nb1, nb2, ja0,
                                   ja1, ja2, jb0, jb1, jb2, dqAL, dqBL, dqAA, dqBA);
     float stick = lam4.w;
     if (ev.F0 < 0.0f)
@@ -1840,8 +1876,8 @@ __device__ __forceinline__ void dual_contact(int m, int c, const Manifolds &M, c
         pen.z = fminr(pen.z + P.betaLin * fabsf(ev.C2), kPenaltyMax);
         stick = sqrtf(ev.C1 * ev.C1 + ev.C2 * ev.C2) < kStickThresh ? 1.0f : 0.0f;
     }
-    M.lam[sl] = make_float4(ev.F0, ev.F1, ev.F2, stick);
-    M.pen[sl] = st3(pen, 0.0f);
+    rec[CR_LAM] = make_float4(ev.F0, ev.F1, ev.F2, stick);
+    rec[CR_PEN] = st3(pen, 0.0f);
 }
 
 // Joint::updateDual (joint.cpp:125-182)
@@ -1850,13 +1886,13 @@ __device__ void dual_joint(Joint *jp, const Bodies &B, const Params &P)
     Joint j = *jp;
     if (j.dead)
         return;
-    Q4 qB = ldq(B.rot[j.b]);
-    V3 pB = ld3(B.pos[j.b]);
+    Q4 qB = ldq(B.pose[4 * (j.b) + P_ROT]);
+    V3 pB = ld3(B.pose[4 * (j.b) + P_POS]);
     V3 penL = ld3(j.penLin), penA = ld3(j.penAng), lamL = ld3(j.lamLin), lamA = ld3(j.lamAng);
     if (lensq(penL) > 0.0f)
     {
         D3 K = d3_diag(penL.x, penL.y, penL.z);
-        V3 pa = j.a >= 0 ? rotate(ldq(B.rot[j.a]), ld3(j.rA)) + ld3(B.pos[j.a]) : ld3(j.rA);
+        V3 pa = j.a >= 0 ? rotate(ldq(B.pose[4 * (j.a) + P_ROT]), ld3(j.rA)) + ld3(B.pose[4 * (j.a) + P_POS]) : ld3(j.rA);
         V3 C = pa - (rotate(qB, ld3(j.rB)) + pB);
         if (isinf(j.rA.w))
         {
@@ -1870,7 +1906,7 @@ __device__ void dual_joint(Joint *jp, const Bodies &B, const Params &P)
     if (lensq(penA) > 0.0f)
     {
         D3 K = d3_diag(penA.x, penA.y, penA.z);
-        Q4 qa = j.a >= 0 ? ldq(B.rot[j.a]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
+        Q4 qa = j.a >= 0 ? ldq(B.pose[4 * (j.a) + P_ROT]) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
         V3 C = qdiff(qa, qB) * j.c0Lin.w;
         if (isinf(j.rB.w))
         {
@@ -1895,16 +1931,11 @@ __device__ void dual_joint(Joint *jp, const Bodies &B, const Params &P)
 }
 
 // Dual pass of one iteration: one lane per contact, then one per joint.
-__global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int *contactOwner, const int *contactStart,
-                       const Counters *ctr, Params P)
+__global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int4 *contactRec, int nc, Params P)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int nc = ctr->contactCount;
     if (t < nc)
-    {
-        int m = contactOwner[t];
-        dual_contact(m, t - contactStart[m], M, B, P);
-    }
+        dual_contact(contactRec[t], M, B, P);
     else if (t - nc < nj)
         dual_joint(&joints[t - nc], B, P);
 }
@@ -1919,8 +1950,8 @@ __global__ void k_finish(Bodies B, Params P, int n)
     B.prevVel[i] = v;
     if (B.massMoment[i].x > 0.0f)
     {
-        V3 nv = (ld3(B.pos[i]) - ld3(B.initLin[i])) / P.dt;
-        V3 nw = qdiff(ldq(B.rot[i]), ldq(B.initAng[i])) / P.dt;
+        V3 nv = (ld3(B.pose[4 * (i) + P_POS]) - ld3(B.pose[4 * (i) + P_ILIN])) / P.dt;
+        V3 nw = qdiff(ldq(B.pose[4 * (i) + P_ROT]), ldq(B.pose[4 * (i) + P_IANG])) / P.dt;
         B.vel[i] = st3(nv, 0.0f);
         B.angVel[i] = st3(nw, 0.0f);
     }

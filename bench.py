@@ -153,13 +153,15 @@ def main():
     iter_bytes = float(st["iterate_bytes_total"]) / steps_counted
     step_bytes = float(st["step_bytes_total"]) / steps_counted
     # dominant kernel: k_primal_color (one launch per colour per iteration)
+    iters = int(params["iterations"])
+    timed = max(int(st["timed_steps"]), 1)
     pr_launches = max(int(st["primal_launches"]), 1)
     pr_ms = float(st["primal_ms_total"]) / pr_launches
-    pr_bytes = float(st["primal_bytes_total"]) / steps_counted / (pr_launches / max(int(st["timed_steps"]), 1))
+    pr_bytes = float(st["primal_bytes_total"]) / steps_counted / iters / (pr_launches / timed)
     achieved = pr_bytes / (pr_ms * 1e-3) / 1e9
     du_launches = max(int(st["dual_launches"]), 1)
     du_ms = float(st["dual_ms_total"]) / du_launches
-    du_bytes = float(st["dual_bytes_total"]) / steps_counted / (du_launches / max(int(st["timed_steps"]), 1))
+    du_bytes = float(st["dual_bytes_total"]) / steps_counted / iters / (du_launches / timed)
     loop_gbs = iter_bytes / (iter_ms * 1e-3) / 1e9
     log(f"k_primal_color {pr_ms * 1e3:.2f} us/launch {pr_bytes / 1e6:.2f} MB/launch -> {achieved:.0f} GB/s; "
         f"k_dual {du_ms * 1e3:.2f} us/launch {du_bytes / 1e6:.2f} MB/launch -> "

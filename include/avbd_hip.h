@@ -190,10 +190,12 @@ typedef struct avbd_stats {
     int64_t counted_steps;
     int64_t timed_steps;
     /* The iteration loop split into its two kernels: per iteration, one primal
-     * launch per colour (k_primal_color) then one dual launch (k_dual). Bytes
-     * are algorithmic (primal: solved bodies and the stamps of their forces;
-     * dual: contacts and joints); times are HIP-event spans around each
-     * iteration's primal launches and its dual launch. */
+     * launch per colour (k_primal_color) then one dual launch (k_dual).
+     * *_bytes_total: algorithmic bytes of all iterations of the counted steps
+     * (primal: solved bodies and the stamps of their forces; dual: contacts
+     * and joints). *_ms_total and *_launches: HIP-event spans around the first
+     * iteration's primal launches and its dual launch of each timed step, and
+     * the number of launches those spans cover. */
     double primal_bytes_total;
     double dual_bytes_total;
     double primal_ms_total;
