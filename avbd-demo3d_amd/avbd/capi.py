@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from .state import (BODY, CONFIG, JOINT, MANIFOLD, SPRING, STATS, State, check_layouts)
+from .state import (BODY, CONFIG, JOINT, LAUNCH, MANIFOLD, SPRING, STATS, State, batch_states, check_layouts)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libavbd_hip.so")
@@ -59,6 +59,12 @@ def load(path: str | None = None):
                                         C.c_float, _p]
         lib.avbd_make_spring.argtypes = [_p, _p, C.c_int32, C.c_int32, C.c_int64, _p, _p, C.c_float, C.c_float, _p]
         lib.avbd_sizeof.argtypes = [C.c_int]
+        lib.avbd_batch_upload.argtypes = [_p, _p, C.c_int32, _p, C.c_int32, _p, C.c_int32, _p, C.c_int32, _p,
+                                          C.c_int32, C.c_int64]
+        lib.avbd_batch_num_worlds.argtypes = [_p, _p]
+        lib.avbd_set_launch_config.argtypes = [_p, _p]
+        lib.avbd_get_launch_config.argtypes = [_p, _p]
+        lib.avbd_autotune.argtypes = [_p, C.c_int32, C.c_int32, _p]
         check_layouts(lib.avbd_sizeof)
         _LIB = lib
     return _LIB
@@ -128,6 +134,48 @@ class HipWorld:
                                                                 2 * len(state.manifolds)), **kw)
         w.upload(state)
         return w
+
+    @classmethod
+    def for_batch(cls, states, params, **kw) -> "HipWorld":
+        """One handle stepping many independent worlds (avbd_batch_upload)."""
+        batch, offsets = batch_states(states)
+        nb = len(batch.bodies)
+        w = cls(params, max_bodies=max(nb, 1), max_joints=len(batch.joints), max_springs=len(batch.springs),
+                max_manifolds=kw.pop("max_manifolds", 0) or max(12 * nb + 1024, 2 * len(batch.manifolds)), **kw)
+        w.upload_batch(batch, offsets)
+        return w
+
+    def upload_batch(self, state: State, offsets):
+        offsets = np.ascontiguousarray(offsets, np.int32)
+        self._keep = (state, offsets)
+        _check(self._lib.avbd_batch_upload(self._h, _ptr(offsets), len(offsets) - 1, _ptr(state.bodies),
+                                           len(state.bodies), _ptr(state.joints), len(state.joints),
+                                           _ptr(state.springs), len(state.springs), _ptr(state.manifolds),
+                                           len(state.manifolds), C.c_int64(state.next_serial)))
+        self.world_offsets = offsets
+
+    @property
+    def num_worlds(self) -> int:
+        n = C.c_int32()
+        _check(self._lib.avbd_batch_num_worlds(self._h, C.byref(n)))
+        return n.value
+
+    def launch_config(self) -> dict:
+        out = np.zeros((), LAUNCH)
+        _check(self._lib.avbd_get_launch_config(self._h, out.ctypes.data_as(_p)))
+        return {k: int(out[k]) for k in LAUNCH.names}
+
+    def set_launch_config(self, primal_lanes=16, primal_threads=256, dual_threads=256, use_graph=0):
+        c = np.zeros((), LAUNCH)
+        c["primal_lanes"], c["primal_threads"] = primal_lanes, primal_threads
+        c["dual_threads"], c["use_graph"] = dual_threads, int(use_graph)
+        _check(self._lib.avbd_set_launch_config(self._h, c.ctypes.data_as(_p)))
+
+    def autotune(self, trial_steps: int = 3, allow_graph: bool = True) -> dict:
+        """Time every launch shape on the resident state and keep the fastest (state unchanged)."""
+        out = np.zeros((), LAUNCH)
+        _check(self._lib.avbd_autotune(self._h, trial_steps, 0 if allow_graph else 1, out.ctypes.data_as(_p)))
+        return {k: int(out[k]) for k in LAUNCH.names}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -21,6 +21,9 @@ Workloads (gravity -9.81 along -z, the reference's up axis is +z):
   random orientations, velocities N(0, 0.5) m/s) dropping onto a ground box
   inside four static walls. 50,000 boxes is the headline workload; the
   generator takes any count.
+* ``batch``   independent worlds (4096 by default) with body counts
+  log-uniform in [8, 512]: box-stack fields, piles and chains, for the
+  batched step (one handle, every launch covers all worlds).
 """
 from __future__ import annotations
 
@@ -197,6 +200,49 @@ def pile(count: int = 50000, seed: int = 3, grid=None, pitch: float = 3.6, steps
     for i in range(count):
         sb.body(tuple(2.0 * half[i]), 1.0, 0.5, (cx[i], cy[i], cz[i]), tuple(quats[i]), tuple(vel[i]))
     return sb.scene(f"pile{count}", -9.81, steps)
+
+
+def stacks(count: int, seed: int = 1, height: int = 10, jitter: float = 0.01, spacing: float = 1.5,
+           steps: int = 300) -> Scene:
+    """Columns of up to ``height`` unit boxes on a static ground box, horizontal jitter +-0.01 m."""
+    rng = _rng(seed)
+    sb = _Builder()
+    cols = max(1, math.ceil(count / height))
+    side = math.ceil(math.sqrt(cols))
+    sb.body((side * spacing + 10, side * spacing + 10, 1), 0.0, 0.5, (0, 0, 0))
+    for i in range(count):
+        c, h = divmod(i, height)
+        cx = (c % side - (side - 1) / 2.0) * spacing
+        cy = (c // side - (side - 1) / 2.0) * spacing
+        jx, jy = rng.uniform(-jitter, jitter, 2)
+        sb.body((1, 1, 1), 1.0, 0.5, (cx + jx, cy + jy, 1.0 + h))
+    return sb.scene(f"stacks{count}", -9.81, steps)
+
+
+BATCH_KINDS = ("stacks", "pile", "chain")
+
+
+def batch(num_worlds: int = 4096, seed: int = 4, lo: int = 8, hi: int = 512, steps: int = 300,
+          kinds=BATCH_KINDS) -> list:
+    """Independent worlds for the batched step: body counts log-uniform in [lo, hi]
+    (statics included), each world a box-stack field, a jittered pile (ground + 4
+    walls) or a world-anchored chain with springs over a ground box, drawn with the
+    same distributions as the single-world workloads."""
+    rng = _rng(seed)
+    sizes = np.rint(np.exp(rng.uniform(math.log(lo), math.log(hi), num_worlds))).astype(np.int64)
+    which = rng.integers(0, len(kinds), num_worlds)
+    seeds = rng.integers(1, 2 ** 31 - 1, num_worlds)
+    worlds = []
+    for n, k, sd in zip(sizes, which, seeds):
+        n, kind, sd = int(n), kinds[int(k)], int(sd)
+        if kind == "pile" and n >= 6:
+            sc = pile(n - 5, seed=sd, steps=steps)
+        elif kind == "chain" and n >= 3:
+            sc = pyramid_chain(seed=sd, base=0, links=n - 1, steps=steps)
+        else:
+            sc = stacks(n - 1, seed=sd, steps=steps)
+        worlds.append(sc)
+    return worlds
 
 
 def by_name(name: str) -> Scene:

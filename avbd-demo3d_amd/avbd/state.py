@@ -105,10 +105,71 @@ class State:
             return State(z["bodies"], z["joints"], z["springs"], z["manifolds"], int(z["next_serial"]))
 
 
-def check_layouts(sizeof) -> None:
-    """Compare dtype sizes with a library's sizeof(which) (0 body .. 5 params)."""
-    names = ["body", "contact", "manifold", "joint", "spring", "params"]
-    dts = [BODY, CONTACT, MANIFOLD, JOINT, SPRING, PARAMS]
+def batch_states(states) -> tuple[State, np.ndarray]:
+    """Concatenate per-world states into one batched state (avbd_batch_upload).
+
+    Body indices of forces are shifted by the world's body offset (-1, the world
+    anchor, stays -1); serials are shifted by the preceding worlds' next_serial,
+    so they stay unique over the batch and keep each world's order.
+    Returns (state, world_body_offsets).
+    """
+    offsets = np.zeros(len(states) + 1, np.int32)
+    bodies, joints, springs, manifolds = [], [], [], []
+    serial0 = 0
+    for w, st in enumerate(states):
+        b0 = int(offsets[w])
+        offsets[w + 1] = b0 + len(st.bodies)
+        bodies.append(st.bodies)
+        j = st.joints.copy()
+        j["body_a"] = np.where(j["body_a"] >= 0, j["body_a"] + b0, -1)
+        j["body_b"] += b0
+        j["serial"] += serial0
+        sp = st.springs.copy()
+        sp["body_a"] += b0
+        sp["body_b"] += b0
+        sp["serial"] += serial0
+        m = st.manifolds.copy()
+        m["body_a"] += b0
+        m["body_b"] += b0
+        m["serial"] += serial0
+        joints.append(j)
+        springs.append(sp)
+        manifolds.append(m)
+        serial0 += st.next_serial
+    cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt)
+    return (State(cat(bodies, BODY), cat(joints, JOINT), cat(springs, SPRING), cat(manifolds, MANIFOLD), serial0),
+            offsets)
+
+
+def split_state(state: State, offsets) -> list:
+    """Inverse of batch_states up to serial values: per-world states with local
+    body indices. Serials keep their batch values (each world's order is kept)."""
+    out = []
+    for w in range(len(offsets) - 1):
+        b0, b1 = int(offsets[w]), int(offsets[w + 1])
+        inw = lambda idx: (idx >= b0) & (idx < b1)
+        j = state.joints[inw(state.joints["body_b"])].copy()
+        j["body_a"] = np.where(j["body_a"] >= 0, j["body_a"] - b0, -1)
+        j["body_b"] -= b0
+        sp = state.springs[inw(state.springs["body_b"])].copy()
+        sp["body_a"] -= b0
+        sp["body_b"] -= b0
+        m = state.manifolds[inw(state.manifolds["body_a"])].copy()
+        m["body_a"] -= b0
+        m["body_b"] -= b0
+        out.append(State(state.bodies[b0:b1].copy(), j, sp, m, state.next_serial))
+    return out
+
+
+LAUNCH = np.dtype([("primal_lanes", "<i4"), ("primal_threads", "<i4"), ("dual_threads", "<i4"),
+                   ("use_graph", "<i4")], align=True)
+
+
+def check_layouts(sizeof, records: int = 7) -> None:
+    """Compare dtype sizes with a library's sizeof(which) for which in [0, records)
+    (0 body, 1 contact, 2 manifold, 3 joint, 4 spring, 5 params, 6 launch config)."""
+    names = ["body", "contact", "manifold", "joint", "spring", "params", "launch config"][:records]
+    dts = [BODY, CONTACT, MANIFOLD, JOINT, SPRING, PARAMS, LAUNCH][:records]
     for i, (n, dt) in enumerate(zip(names, dts)):
         got = sizeof(i)
         if got != dt.itemsize:

@@ -70,8 +70,9 @@ struct avbd_handle
     hipEvent_t ev[8] = {};
     // timing ring: per step, events at step start / before iterate / after iterate / step end
     static constexpr int kRing = 2048;
-    static constexpr int kIterSlots = 32;          // per-iteration primal/dual marks kept per step
-    static constexpr int kSlots = 4 + 2 * kIterSlots;
+    // per step: 0 step start, 1 iterate start, 2 iterate end, 3 step end, 4-6 the last
+    // iteration's primal start / dual start / dual end
+    static constexpr int kSlots = 7;
     std::vector<hipEvent_t> ring;
     std::vector<int> ringIters;                    // iterations marked in each ring step
     std::vector<int> ringPrimalLaunches;           // primal launches in each ring step
@@ -80,6 +81,17 @@ struct avbd_handle
     avbd_stats stats{};
     long long stepsTaken = 0;
     bool timing = false;
+    // batched worlds: world w owns bodies [worldStart[w], worldStart[w + 1])
+    int numWorlds = 1;
+    int *worldStart = nullptr;
+    std::vector<int> worldStartHost;
+    // launch shapes of the iteration kernels, and the iteration graph built from them
+    avbd_launch_config launch{16, 256, 256, 0};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    std::vector<hipGraphNode_t> graphPrimal;
+    hipGraphNode_t graphDual = nullptr;
+    avbd_launch_config graphLaunch{};
 
     template <class T>
     int alloc(T *&p, size_t count)
@@ -122,6 +134,63 @@ int alloc_manifolds(avbd_handle *h, Manifolds &m, int cap)
 }
 
 int grid_for(int n) { return std::max(1, std::min(ceil_div(n, 256), 4096)); }
+
+using PrimalKernel = void (*)(Bodies, Manifolds, const Joint *, const Spring *, const int4 *, const int4 *, int, int,
+                              Params);
+
+// The primal kernel's launch shapes: lanes per body x threads per block.
+PrimalKernel primal_kernel(int lanes, int threads)
+{
+#define AVBD_PRIMAL_SHAPE(l, t)                                                                            \
+    if (lanes == l && threads == t)                                                                        \
+        return &k_primal_color<l, t>;
+    AVBD_PRIMAL_SHAPE(8, 64)
+    AVBD_PRIMAL_SHAPE(8, 128)
+    AVBD_PRIMAL_SHAPE(8, 256)
+    AVBD_PRIMAL_SHAPE(16, 64)
+    AVBD_PRIMAL_SHAPE(16, 128)
+    AVBD_PRIMAL_SHAPE(16, 256)
+    AVBD_PRIMAL_SHAPE(32, 64)
+    AVBD_PRIMAL_SHAPE(32, 128)
+    AVBD_PRIMAL_SHAPE(32, 256)
+#undef AVBD_PRIMAL_SHAPE
+    return nullptr;
+}
+
+bool valid_launch(const avbd_launch_config &c)
+{
+    auto threads = [](int t) { return t == 64 || t == 128 || t == 256; };
+    return primal_kernel(c.primal_lanes, c.primal_threads) != nullptr && threads(c.dual_threads) &&
+           (c.use_graph == 0 || c.use_graph == 1);
+}
+
+// Kernel argument blocks (hipLaunchKernel / graph kernel nodes take an array of argument pointers).
+struct PrimalArgs
+{
+    Bodies B;
+    Manifolds M;
+    const Joint *joints;
+    const Spring *springs;
+    const int4 *stamps;
+    const int4 *tasks;
+    int begin, end;
+    Params P;
+    void *ptr[9];
+    PrimalArgs(const avbd_handle *h, int b, int e);
+};
+
+struct DualArgs
+{
+    Bodies B;
+    Manifolds M;
+    Joint *joints;
+    int nj;
+    const int4 *contactRec;
+    int nc;
+    Params P;
+    void *ptr[7];
+    DualArgs(const avbd_handle *h, int nc_);
+};
 
 // Colouring runs as one cooperative launch of a few large blocks: every round ends in a grid
 // barrier whose cost grows with the block count.
@@ -240,15 +309,25 @@ int enqueue_prepare(avbd_handle *h)
     // ---- broadphase: sweep and prune (solver.cpp:488-577) ----
     if (N >= 2)
     {
-        k_axis_partial<<<64, 256, 0, s>>>(h->B.pose, N, S.axisSums);
-        k_axis_decide<<<1, 64, 0, s>>>(S.axisSums, 64, N, ctr);
-        k_axis_serial<<<1, 64, 0, s>>>(h->B.pose, N, ctr);
-        k_sap_keys<<<ceil_div(N, 256), 256, 0, s>>>(h->B.pose, h->B.sizeRad, N, ctr, S.keys, S.vals);
-        unsigned *keys = S.keys, *keysAlt = S.keysAlt;
-        int *vals = S.vals, *valsAlt = S.valsAlt;
-        radix_sort(h, keys, vals, keysAlt, valsAlt, N, 4);
-        k_sap_gather<<<ceil_div(N, 256), 256, 0, s>>>(vals, h->B.pose, h->B.sizeRad, N, ctr, S.sLo, S.sHi, S.sPos, S.sIdx);
-        k_sweep_len<<<ceil_div(N, 256), 256, 0, s>>>(S.sLo, S.sHi, N, h->sweepEnd, S.chunkCount, ctr);
+        if (h->numWorlds > 1)
+        {
+            // batched worlds: per-world axis, sort and sweep extents in one launch
+            k_world_sap<<<h->numWorlds, 256, 0, s>>>(h->B.pose, h->B.sizeRad, h->worldStart, S.sLo, S.sHi, S.sPos,
+                                                     S.sIdx, h->sweepEnd, S.chunkCount, ctr);
+        }
+        else
+        {
+            k_axis_partial<<<64, 256, 0, s>>>(h->B.pose, N, S.axisSums);
+            k_axis_decide<<<1, 64, 0, s>>>(S.axisSums, 64, N, ctr);
+            k_axis_serial<<<1, 64, 0, s>>>(h->B.pose, N, ctr);
+            k_sap_keys<<<ceil_div(N, 256), 256, 0, s>>>(h->B.pose, h->B.sizeRad, N, ctr, S.keys, S.vals);
+            unsigned *keys = S.keys, *keysAlt = S.keysAlt;
+            int *vals = S.vals, *valsAlt = S.valsAlt;
+            radix_sort(h, keys, vals, keysAlt, valsAlt, N, 4);
+            k_sap_gather<<<ceil_div(N, 256), 256, 0, s>>>(vals, h->B.pose, h->B.sizeRad, N, ctr, S.sLo, S.sHi, S.sPos,
+                                                          S.sIdx);
+            k_sweep_len<<<ceil_div(N, 256), 256, 0, s>>>(S.sLo, S.sHi, N, h->sweepEnd, S.chunkCount, ctr);
+        }
         scan(h, S.chunkCount, S.chunkStart, nullptr, N, &ctr->numChunks);
         k_clamp_count<<<1, 1, 0, s>>>(&ctr->numChunks, h->maxChunks, &ctr->overflow, 32);
         k_sweep_count<<<G, 256, 0, s>>>(S.chunkStart, h->sweepEnd, N, ctr, S.sPos, S.hitCount, S.hitMask);
@@ -328,6 +407,101 @@ int enqueue_prepare(avbd_handle *h)
     return AVBD_OK;
 }
 
+PrimalArgs::PrimalArgs(const avbd_handle *h, int b, int e)
+    : B(h->B), M(h->cur), joints(h->joints), springs(h->springs), stamps(h->S.stamps), tasks(h->S.tasks), begin(b),
+      end(e), P(h->P)
+{
+    void *p[9] = {&B, &M, &joints, &springs, &stamps, &tasks, &begin, &end, &P};
+    std::memcpy(ptr, p, sizeof(p));
+}
+
+DualArgs::DualArgs(const avbd_handle *h, int nc_)
+    : B(h->B), M(h->cur), joints(h->joints), nj(h->nj), contactRec(h->S.contactRec), nc(nc_), P(h->P)
+{
+    void *p[7] = {&B, &M, &joints, &nj, &contactRec, &nc, &P};
+    std::memcpy(ptr, p, sizeof(p));
+}
+
+void destroy_graph(avbd_handle *h)
+{
+    if (h->graphExec)
+        (void)hipGraphExecDestroy(h->graphExec);
+    if (h->graph)
+        (void)hipGraphDestroy(h->graph);
+    h->graphExec = nullptr;
+    h->graph = nullptr;
+    h->graphPrimal.clear();
+    h->graphDual = nullptr;
+}
+
+hipKernelNodeParams node_params(const void *fn, int grid, int threads, void **args)
+{
+    hipKernelNodeParams kp{};
+    kp.func = const_cast<void *>(fn);
+    kp.gridDim = dim3(std::max(grid, 1));
+    kp.blockDim = dim3(threads);
+    kp.sharedMemBytes = 0;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    return kp;
+}
+
+// One solver iteration as a graph: the colour launches in order, then the dual launch. The
+// graph is rebuilt when its node count or launch shape changes; otherwise only the nodes'
+// grids and ranges are updated for the step.
+int iterate_graph(avbd_handle *h, const std::vector<std::pair<int, int>> &ranges, int nc, int dualLanes)
+{
+    const avbd_launch_config &lc = h->launch;
+    const void *pfn = (const void *)primal_kernel(lc.primal_lanes, lc.primal_threads);
+    const int groups = lc.primal_threads / lc.primal_lanes;
+    const bool dual = dualLanes > 0;
+    bool rebuild = !h->graphExec || h->graphPrimal.size() != ranges.size() || (h->graphDual != nullptr) != dual ||
+                   h->graphLaunch.primal_lanes != lc.primal_lanes ||
+                   h->graphLaunch.primal_threads != lc.primal_threads || h->graphLaunch.dual_threads != lc.dual_threads;
+    if (rebuild)
+    {
+        destroy_graph(h);
+        HIP_TRY(hipGraphCreate(&h->graph, 0));
+        hipGraphNode_t prev = nullptr;
+        for (const auto &r : ranges)
+        {
+            PrimalArgs a(h, r.first, r.second);
+            hipKernelNodeParams kp = node_params(pfn, ceil_div(r.second - r.first, groups), lc.primal_threads, a.ptr);
+            hipGraphNode_t node;
+            HIP_TRY(hipGraphAddKernelNode(&node, h->graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
+            h->graphPrimal.push_back(node);
+            prev = node;
+        }
+        if (dual)
+        {
+            DualArgs a(h, nc);
+            hipKernelNodeParams kp = node_params((const void *)k_dual, ceil_div(dualLanes, lc.dual_threads),
+                                                 lc.dual_threads, a.ptr);
+            HIP_TRY(hipGraphAddKernelNode(&h->graphDual, h->graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
+        }
+        HIP_TRY(hipGraphInstantiate(&h->graphExec, h->graph, nullptr, nullptr, 0));
+        h->graphLaunch = lc;
+    }
+    else
+    {
+        for (size_t k = 0; k < ranges.size(); ++k)
+        {
+            PrimalArgs a(h, ranges[k].first, ranges[k].second);
+            hipKernelNodeParams kp = node_params(pfn, ceil_div(ranges[k].second - ranges[k].first, groups),
+                                                 lc.primal_threads, a.ptr);
+            HIP_TRY(hipGraphExecKernelNodeSetParams(h->graphExec, h->graphPrimal[k], &kp));
+        }
+        if (dual)
+        {
+            DualArgs a(h, nc);
+            hipKernelNodeParams kp = node_params((const void *)k_dual, ceil_div(dualLanes, lc.dual_threads),
+                                                 lc.dual_threads, a.ptr);
+            HIP_TRY(hipGraphExecKernelNodeSetParams(h->graphExec, h->graphDual, &kp));
+        }
+    }
+    return AVBD_OK;
+}
+
 // Solver iterations: per colour one primal launch, then one dual launch. The colour and
 // contact counts are read back once per step to size the launches.
 int enqueue_iterate(avbd_handle *h)
@@ -341,32 +515,63 @@ int enqueue_iterate(avbd_handle *h)
         return check_device_errors(h);
     const int colors = std::min(c.colorCount, kMaxColors);
     const int dualLanes = c.contactCount + h->nj;
-    Scratch &S = h->S;
-    // Timing marks (when enabled) bracket the first iteration's primal launches and its dual
-    // launch only: back-to-back launches of one stream have no gap between them, while each
-    // event adds one, so a few marks per step keep the measured step close to the unmeasured one.
+    // Timing marks (when enabled) bracket the last iteration's primal launches and its dual
+    // launch only. By then the queue holds the earlier iterations' launches, so the marks see
+    // the kernels back to back (the first iteration after the per-step readback would also
+    // time the host's enqueueing), and a few marks per step keep the measured step close to
+    // the unmeasured one.
     hipEvent_t *mark = h->timing ? &h->ring[avbd_handle::kSlots * (h->ringSteps % avbd_handle::kRing)] : nullptr;
-    int primalPerIter = 0;
-    for (int it = 0; it < h->P.iterations; ++it)
+    std::vector<std::pair<int, int>> ranges;
+    for (int col = 0; col < colors; ++col)
+        if (h->hostOffsets[col + 1] > h->hostOffsets[col])
+            ranges.emplace_back(h->hostOffsets[col], h->hostOffsets[col + 1]);
+    const int primalPerIter = (int)ranges.size();
+    const avbd_launch_config &lc = h->launch;
+    if (lc.use_graph && h->P.iterations > 0 && (primalPerIter > 0 || dualLanes > 0))
     {
-        for (int col = 0; col < colors; ++col)
+        int r = iterate_graph(h, ranges, c.contactCount, dualLanes);
+        if (r)
+            return r;
+        // in graph form the last iteration is timed as a whole (its primal figure includes the dual)
+        const int last = h->P.iterations - 1;
+        for (int it = 0; it < h->P.iterations; ++it)
         {
-            int b = h->hostOffsets[col], n = h->hostOffsets[col + 1] - b;
-            if (n > 0)
+            if (mark && it == last)
+                HIP_TRY(hipEventRecord(mark[4], s));
+            HIP_TRY(hipGraphLaunch(h->graphExec, s));
+            if (mark && it == last)
             {
-                k_primal_color<<<ceil_div(n, kGroupsPerBlock), 256, 0, s>>>(h->B, h->cur, h->joints, h->springs,
-                                                                             S.stamps, S.tasks, b, b + n, h->P);
-                if (it == 0)
-                    ++primalPerIter;
+                HIP_TRY(hipEventRecord(mark[5], s));
+                HIP_TRY(hipEventRecord(mark[6], s));
             }
         }
-        if (mark && it == 0)
-            HIP_TRY(hipEventRecord(mark[4], s));
-        if (dualLanes > 0)
-            k_dual<<<ceil_div(dualLanes, 256), 256, 0, s>>>(h->B, h->cur, h->joints, h->nj, S.contactRec,
-                                                            c.contactCount, h->P);
-        if (mark && it == 0)
-            HIP_TRY(hipEventRecord(mark[5], s));
+    }
+    else
+    {
+        const void *pfn = (const void *)primal_kernel(lc.primal_lanes, lc.primal_threads);
+        const int groups = lc.primal_threads / lc.primal_lanes;
+        const int last = h->P.iterations - 1;
+        for (int it = 0; it < h->P.iterations; ++it)
+        {
+            if (mark && it == last)
+                HIP_TRY(hipEventRecord(mark[4], s));
+            for (const auto &rg : ranges)
+            {
+                PrimalArgs a(h, rg.first, rg.second);
+                HIP_TRY(hipLaunchKernel(pfn, dim3(ceil_div(rg.second - rg.first, groups)), dim3(lc.primal_threads),
+                                        a.ptr, 0, s));
+            }
+            if (mark && it == last)
+                HIP_TRY(hipEventRecord(mark[5], s));
+            if (dualLanes > 0)
+            {
+                DualArgs a(h, c.contactCount);
+                HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(dualLanes, lc.dual_threads)),
+                                        dim3(lc.dual_threads), a.ptr, 0, s));
+            }
+            if (mark && it == last)
+                HIP_TRY(hipEventRecord(mark[6], s));
+        }
     }
     if (h->timing)
     {
@@ -568,7 +773,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         (r = h->alloc(S.adjCursor, N)) || (r = h->alloc(S.adj, h->maxAdj)) || (r = h->alloc(S.adjSerial, h->maxAdj)) ||
         (r = h->alloc(S.color, N)) || (r = h->alloc(S.colorOffsets, 257)) || (r = h->alloc(S.colorBodies, N)) ||
         (r = h->alloc(S.partial, kScanBlocks + 1)) || (r = h->alloc(h->ctr, 1)) || (r = h->alloc(h->gb, 1)) ||
-        (r = h->alloc(h->adjTotal, 1)))
+        (r = h->alloc(h->adjTotal, 1)) || (r = h->alloc(h->worldStart, (size_t)N + 1)))
         return bail(r);
     S.hashMask = h->hashSize - 1;
     if ((e = hipHostMalloc((void **)&h->hostCtr, sizeof(Counters))) != hipSuccess ||
@@ -614,6 +819,7 @@ int avbd_destroy(avbd_handle *h)
     (void)hipSetDevice(h->device);
     if (h->stream)
         (void)hipStreamSynchronize(h->stream);
+    destroy_graph(h);
     for (void *p : h->allocs)
         (void)hipFree(p);
     if (h->hostCtr)
@@ -806,6 +1012,8 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     h->nb = N;
     h->nj = num_joints;
     h->ns = num_springs;
+    h->numWorlds = 1;
+    h->worldStartHost.assign({0, N});
     return AVBD_OK;
 }
 
@@ -1110,11 +1318,12 @@ int avbd_get_stats(avbd_handle *h, avbd_stats *out)
             itTot += a;
             stTot += b;
             int iters = h->ringIters[k % avbd_handle::kRing];
-            for (int it = 0; it < iters; ++it)
+            if (iters > 0)
             {
+                // the last iteration: e[4] before its primal launches, e[5] before its dual, e[6] after
                 float p = 0.0f, d = 0.0f;
-                HIP_TRY(hipEventElapsedTime(&p, it == 0 ? e[1] : e[5 + 2 * (it - 1)], e[4 + 2 * it]));
-                HIP_TRY(hipEventElapsedTime(&d, e[4 + 2 * it], e[5 + 2 * it]));
+                HIP_TRY(hipEventElapsedTime(&p, e[4], e[5]));
+                HIP_TRY(hipEventElapsedTime(&d, e[5], e[6]));
                 prTot += p;
                 duTot += d;
             }
@@ -1199,6 +1408,190 @@ int avbd_get_hits(avbd_handle *h, int32_t *triples, int32_t capacity, int32_t *c
     return AVBD_OK;
 }
 
+int avbd_batch_upload(avbd_handle *h, const int32_t *world_body_offsets, int32_t num_worlds,
+                      const avbd_body *bodies, int32_t num_bodies, const avbd_joint *joints, int32_t num_joints,
+                      const avbd_spring *springs, int32_t num_springs, const avbd_manifold *manifolds,
+                      int32_t num_manifolds, int64_t next_serial)
+{
+    if (!h || !world_body_offsets)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_batch_upload: null argument");
+    if (num_worlds < 1 || num_worlds > std::max(num_bodies, 1) || num_bodies < 0)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_batch_upload: world count must be in [1, num_bodies]");
+    if (world_body_offsets[0] != 0 || world_body_offsets[num_worlds] != num_bodies)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_batch_upload: offsets must run from 0 to num_bodies");
+    std::vector<int> worldOf(std::max(num_bodies, 1), 0);
+    for (int w = 0; w < num_worlds; ++w)
+    {
+        int b = world_body_offsets[w], e = world_body_offsets[w + 1];
+        if (e <= b)
+            return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_batch_upload: every world needs at least one body");
+        if (num_worlds > 1 && e - b > AVBD_MAX_WORLD_BODIES)
+            return fail(AVBD_ERR_UNSUPPORTED, "avbd_batch_upload: a batched world holds at most "
+                                              "AVBD_MAX_WORLD_BODIES bodies");
+        for (int i = b; i < e; ++i)
+            worldOf[i] = w;
+    }
+    auto inRange = [&](int b) { return b >= 0 && b < num_bodies; };
+    for (int i = 0; i < num_joints; ++i)
+    {
+        int a = joints ? joints[i].body_a : -1, b = joints ? joints[i].body_b : -1;
+        if (inRange(a) && inRange(b) && worldOf[a] != worldOf[b])
+            return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_batch_upload: joint spans two worlds");
+    }
+    for (int i = 0; i < num_springs; ++i)
+    {
+        int a = springs ? springs[i].body_a : -1, b = springs ? springs[i].body_b : -1;
+        if (inRange(a) && inRange(b) && worldOf[a] != worldOf[b])
+            return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_batch_upload: spring spans two worlds");
+    }
+    for (int i = 0; i < num_manifolds; ++i)
+    {
+        int a = manifolds ? manifolds[i].body_a : -1, b = manifolds ? manifolds[i].body_b : -1;
+        if (inRange(a) && inRange(b) && worldOf[a] != worldOf[b])
+            return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_batch_upload: manifold spans two worlds");
+    }
+    int r = avbd_upload_state(h, bodies, num_bodies, joints, num_joints, springs, num_springs, manifolds,
+                              num_manifolds, next_serial);
+    if (r)
+        return r;
+    HIP_TRY(hipMemcpy(h->worldStart, world_body_offsets, sizeof(int) * ((size_t)num_worlds + 1),
+                      hipMemcpyHostToDevice));
+    h->numWorlds = num_worlds;
+    h->worldStartHost.assign(world_body_offsets, world_body_offsets + num_worlds + 1);
+    return AVBD_OK;
+}
+
+int avbd_batch_num_worlds(avbd_handle *h, int32_t *num_worlds)
+{
+    if (!h || !num_worlds)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_batch_num_worlds: null argument");
+    *num_worlds = h->numWorlds;
+    return AVBD_OK;
+}
+
+int avbd_set_launch_config(avbd_handle *h, const avbd_launch_config *cfg)
+{
+    if (!h || !cfg)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_launch_config: null argument");
+    if (!valid_launch(*cfg))
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_launch_config: primal lanes must be 8/16/32, thread "
+                                               "counts 64/128/256 and use_graph 0 or 1");
+    h->launch = *cfg;
+    return AVBD_OK;
+}
+
+int avbd_get_launch_config(avbd_handle *h, avbd_launch_config *out)
+{
+    if (!h || !out)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_get_launch_config: null argument");
+    *out = h->launch;
+    return AVBD_OK;
+}
+
+int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launch_config *out)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_autotune: null handle");
+    if (trial_steps < 1)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_autotune: trial_steps must be at least 1");
+    int r = avbd_synchronize(h);
+    if (r)
+        return r;
+    // snapshot of the resident state through the public tables, restored before every trial
+    int32_t nb = 0, nj = 0, ns = 0, nm = 0;
+    int64_t next = 0;
+    if ((r = avbd_state_counts(h, &nb, &nj, &ns, &nm, &next)))
+        return r;
+    std::vector<avbd_body> bodies(std::max(nb, 1));
+    std::vector<avbd_joint> joints(std::max(nj, 1));
+    std::vector<avbd_spring> springs(std::max(ns, 1));
+    std::vector<avbd_manifold> manifolds(std::max(nm, 1));
+    if ((r = avbd_download_state(h, bodies.data(), joints.data(), springs.data(), manifolds.data())))
+        return r;
+    const std::vector<int> worlds = h->worldStartHost;
+    const int numWorlds = h->numWorlds;
+    auto restore = [&]() {
+        return avbd_batch_upload(h, worlds.data(), numWorlds, bodies.data(), nb, joints.data(), nj, springs.data(), ns,
+                                 manifolds.data(), nm, next);
+    };
+    const avbd_launch_config start = h->launch;
+    hipEvent_t t0 = h->ev[6], t1 = h->ev[7];
+    auto trial = [&](const avbd_launch_config &c, float &ms) -> int {
+        int rr;
+        if ((rr = restore()))
+            return rr;
+        h->launch = c;
+        if ((rr = avbd_step(h, 1)) || (rr = avbd_synchronize(h)))
+            return rr;
+        HIP_TRY(hipEventRecord(t0, h->stream));
+        if ((rr = avbd_step(h, trial_steps)))
+            return rr;
+        HIP_TRY(hipEventRecord(t1, h->stream));
+        if ((rr = avbd_synchronize(h)))
+            return rr;
+        HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+        return AVBD_OK;
+    };
+    const bool allowGraph = (flags & AVBD_TUNE_NO_GRAPH) == 0;
+    avbd_launch_config best = start;
+    float bestMs = 0.0f;
+    if ((r = trial(best, bestMs)))
+        return r;
+    // primal shape first (with the dual at the current shape), then the dual, then the graph form
+    for (int lanes : {8, 16, 32})
+        for (int threads : {64, 128, 256})
+        {
+            avbd_launch_config c = best;
+            c.primal_lanes = lanes;
+            c.primal_threads = threads;
+            float ms = 0.0f;
+            if ((r = trial(c, ms)))
+                return r;
+            if (ms < bestMs)
+            {
+                bestMs = ms;
+                best = c;
+            }
+        }
+    for (int threads : {64, 128, 256})
+    {
+        avbd_launch_config c = best;
+        c.dual_threads = threads;
+        float ms = 0.0f;
+        if ((r = trial(c, ms)))
+            return r;
+        if (ms < bestMs)
+        {
+            bestMs = ms;
+            best = c;
+        }
+    }
+    for (int g : {0, 1})
+    {
+        if (g == 1 && !allowGraph)
+            continue;
+        avbd_launch_config c = best;
+        c.use_graph = g;
+        float ms = 0.0f;
+        if ((r = trial(c, ms)))
+            return r;
+        if (ms < bestMs)
+        {
+            bestMs = ms;
+            best = c;
+        }
+    }
+    if ((r = restore()))
+        return r;
+    h->launch = best;
+    h->stepsTaken = 0;
+    if ((r = avbd_reset_stats(h)))
+        return r;
+    if (out)
+        *out = best;
+    return AVBD_OK;
+}
+
 int avbd_debug_inject_fault(avbd_handle *h, int32_t code)
 {
     if (!h)
@@ -1217,6 +1610,7 @@ int avbd_sizeof(int which)
     case 3: return (int)sizeof(avbd_joint);
     case 4: return (int)sizeof(avbd_spring);
     case 5: return (int)sizeof(avbd_params);
+    case 6: return (int)sizeof(avbd_launch_config);
     default: return -1;
     }
 }

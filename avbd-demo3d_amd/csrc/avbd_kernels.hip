@@ -412,6 +412,159 @@ __global__ void k_sweep_len(const float *sLo, const float *sHi, int n, int *swee
         atomicAdd(&ctr->pairChecks, s);
 }
 
+// Batched worlds (each an independent reference Solver, bodies stored world by world): one
+// block per world picks the world's sweep axis (solver.cpp:499-515, decided as k_axis_decide /
+// k_axis_serial do), sorts the world's interval starts by (start, list index) in LDS and finds
+// each interval's sweep extent inside the world (solver.cpp:525-556). Sorted arrays are
+// world-major, so the sweep stages after it never pair bodies of two worlds.
+constexpr int kWorldSapMax = 2048;
+
+__global__ __launch_bounds__(256) void k_world_sap(const float4 *pose, const float4 *sizeRad, const int *worldStart,
+                                                   float *sLo, float *sHi, float4 *sPos, int *sIdx, int *sweepEnd,
+                                                   int *chunkCount, Counters *ctr)
+{
+    __shared__ unsigned long long key[kWorldSapMax];
+    __shared__ float lo_s[kWorldSapMax];
+    __shared__ float hi_s[kWorldSapMax];
+    __shared__ double red[6][256];
+    __shared__ int axis_s;
+    const int w = blockIdx.x, t = threadIdx.x;
+    const int ws = worldStart[w], n = worldStart[w + 1] - ws;
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int l = t; l < n; l += 256)
+    {
+        float4 p = pose[4 * (ws + l) + P_POS];
+        a[0] += p.x;
+        a[1] += p.y;
+        a[2] += p.z;
+        a[3] += (double)p.x * p.x;
+        a[4] += (double)p.y * p.y;
+        a[5] += (double)p.z * p.z;
+    }
+    for (int k = 0; k < 6; ++k)
+        red[k][t] = a[k];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1)
+    {
+        if (t < st)
+            for (int k = 0; k < 6; ++k)
+                red[k][t] += red[k][t + st];
+        __syncthreads();
+    }
+    if (t == 0)
+    {
+        double v[3];
+        for (int k = 0; k < 3; ++k)
+        {
+            double mean = red[k][0] / n;
+            v[k] = red[k + 3][0] - n * mean * mean;
+        }
+        int best = 0;
+        if (v[1] > v[best])
+            best = 1;
+        if (v[2] > v[best])
+            best = 2;
+        double tol = 4.0 * ((double)n + 16.0) * 5.960464477539063e-08 + 1e-9;
+        bool amb = false;
+        for (int k = 0; k < 3; ++k)
+            if (k != best && v[k] >= v[best] * (1.0 - tol))
+                amb = true;
+        if (amb && n > 0)
+        {
+            V3 mean{0.0f, 0.0f, 0.0f};
+            for (int l = 0; l < n; ++l)
+                mean += ld3(pose[4 * (ws + l) + P_POS]);
+            mean = mean / (float)n;
+            V3 var{0.0f, 0.0f, 0.0f};
+            for (int l = 0; l < n; ++l)
+            {
+                V3 d = ld3(pose[4 * (ws + l) + P_POS]) - mean;
+                var += V3{d.x * d.x, d.y * d.y, d.z * d.z};
+            }
+            best = 0;
+            if (var.y > comp(var, best))
+                best = 1;
+            if (var.z > comp(var, best))
+                best = 2;
+            atomicAdd(&ctr->axisFallbacks, 1);
+        }
+        axis_s = best;
+    }
+    __syncthreads();
+    const int axis = axis_s;
+    int m = 1;
+    while (m < n)
+        m <<= 1;
+    for (int l = t; l < m; l += 256)
+    {
+        unsigned long long k = ~0ull;
+        if (l < n)
+        {
+            int i = ws + l;
+            float lo = comp(ld3(pose[4 * i + P_POS]), axis) - sizeRad[i].w;
+            if (lo == 0.0f)
+                lo = 0.0f;
+            unsigned u = __float_as_uint(lo);
+            unsigned ok = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+            k = ((unsigned long long)ok << 32) | (unsigned)l;
+        }
+        key[l] = k;
+    }
+    __syncthreads();
+    // bitonic sort of the m (power of two) composite keys, ascending
+    for (int size = 2; size <= m; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1)
+        {
+            for (int x = t; x < (m >> 1); x += 256)
+            {
+                int i0 = 2 * x - (x & (stride - 1)), i1 = i0 + stride;
+                bool asc = (i0 & size) == 0;
+                unsigned long long k0 = key[i0], k1 = key[i1];
+                if ((k0 > k1) == asc)
+                {
+                    key[i0] = k1;
+                    key[i1] = k0;
+                }
+            }
+            __syncthreads();
+        }
+    for (int k = t; k < n; k += 256)
+    {
+        int i = ws + (int)(key[k] & 0xFFFFFFFFull);
+        float4 p = pose[4 * i + P_POS];
+        float r = sizeRad[i].w;
+        float c = comp(ld3(p), axis);
+        lo_s[k] = c - r;
+        hi_s[k] = c + r;
+        sLo[ws + k] = c - r;
+        sHi[ws + k] = c + r;
+        sPos[ws + k] = make_float4(p.x, p.y, p.z, r);
+        sIdx[ws + k] = i;
+    }
+    __syncthreads();
+    int checks = 0;
+    for (int k = t; k < n; k += 256)
+    {
+        float hi = hi_s[k];
+        int lo_ = k + 1, hi_ = n;
+        while (lo_ < hi_)
+        {
+            int mid = (lo_ + hi_) >> 1;
+            if (lo_s[mid] > hi)
+                hi_ = mid;
+            else
+                lo_ = mid + 1;
+        }
+        sweepEnd[ws + k] = ws + lo_;
+        int len = lo_ - k - 1;
+        chunkCount[ws + k] = (len + kSweepChunk - 1) / kSweepChunk;
+        checks += len;
+    }
+    int sum = wave_sum(checks);
+    if ((t & 63) == 0 && sum)
+        atomicAdd(&ctr->pairChecks, sum);
+}
+
 __device__ __forceinline__ int chunk_owner(const int *chunkStart, int n, int t)
 {
     int lo = 0, hi = n;
@@ -1331,8 +1484,6 @@ __global__ void k_color_offsets(const int *scannedHist, int tiles, int n, int *c
 // system entries and adds the records in their sequential order. Every entry therefore sees
 // the same sequence of FP32 additions as the reference's single-threaded loop.
 // ------------------------------------------------------------------------------------------
-constexpr int kGroupLanes = 16;
-constexpr int kGroupsPerBlock = 256 / kGroupLanes;
 constexpr int kRec = 27; // lhsLin lower 6 | lhsAng lower 6 | lhsCross 9 | rhsLin 3 | rhsAng 3
 enum
 {
@@ -1721,23 +1872,28 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Primal update of the bodies of one colour (solver.cpp:1184-1240): kGroupLanes lanes per body.
+// Primal update of the bodies of one colour (solver.cpp:1184-1240): L lanes per body, T threads
+// per block. Lane l computes the stamp records l, l+L, l+2L, ... of its body; the records of one
+// round are then added entry by entry in stamp order, so every (L, T) gives the same sums.
 // Bodies of a colour are sorted by stamp count, so the groups of a wave need similar round counts.
-__global__ __launch_bounds__(256) void k_primal_color(Bodies B, Manifolds M, const Joint *joints,
-                                                      const Spring *springs, const int4 *stamps, const int4 *tasks,
-                                                      int begin, int end, Params P)
+template <int L, int T>
+__global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const Joint *joints, const Spring *springs,
+                                                    const int4 *stamps, const int4 *tasks, int begin, int end, Params P)
 {
-    __shared__ float rec[kGroupsPerBlock][kGroupLanes][kRec + 1];
-    const int lane = threadIdx.x % kGroupLanes, g = threadIdx.x / kGroupLanes;
-    const int k = begin + blockIdx.x * kGroupsPerBlock + g;
+    static_assert(L >= 4 && L <= 64 && (64 % L) == 0 && (T % 64) == 0, "a body's lanes must sit in one wave");
+    constexpr int G = T / L;               // bodies per block
+    constexpr int E = (kRec + L - 1) / L;  // accumulated entries per lane
+    __shared__ float rec[G][L][kRec + 1];
+    const int lane = threadIdx.x % L, g = threadIdx.x / L;
+    const int k = begin + blockIdx.x * G + g;
     const bool valid = k < end;
     const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
     const int i = task.x;
     const int s0 = task.y;
     const int ns = valid ? task.z : 0;
-    int rounds = (ns + kGroupLanes - 1) / kGroupLanes;
+    int rounds = (ns + L - 1) / L;
 #pragma unroll
-    for (int off = 32; off >= kGroupLanes; off >>= 1)
+    for (int off = 32; off >= L; off >>= 1)
         rounds = max(rounds, __shfl_xor(rounds, off, 64));
     float4 mm = B.massMoment[i];
     V3 pos = ld3(B.pose[4 * i + P_POS]);
@@ -1746,13 +1902,14 @@ __global__ __launch_bounds__(256) void k_primal_color(Bodies B, Manifolds M, con
     float ml = mm.x / dt2, ax = mm.y / dt2, ay = mm.z / dt2, az = mm.w / dt2;
     V3 dl = pos - ld3(B.inerLin[i]);
     V3 da = qdiff(rot, ldq(B.inerAng[i]));
-    const int e0 = lane, e1 = lane + kGroupLanes;
-    float acc0 = init_entry(e0, ml, ax, ay, az, dl, da);
-    float acc1 = e1 < kRec ? init_entry(e1, ml, ax, ay, az, dl, da) : 0.0f;
+    float acc[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q)
+        acc[q] = lane + q * L < kRec ? init_entry(lane + q * L, ml, ax, ay, az, dl, da) : 0.0f;
     const float oma = 1.0f - P.alpha;
     for (int r = 0; r < rounds; ++r)
     {
-        int t = r * kGroupLanes + lane;
+        int t = r * L + lane;
         float out[kRec];
 #pragma unroll
         for (int q = 0; q < kRec; ++q)
@@ -1772,29 +1929,29 @@ __global__ __launch_bounds__(256) void k_primal_color(Bodies B, Manifolds M, con
         for (int q = 0; q < kRec; ++q)
             rec[g][lane][q] = out[q];
         wave_lds_sync();
-        // the records of this round, added entry by entry in their sequential order
-        int cnt = valid ? min(kGroupLanes, ns - r * kGroupLanes) : 0;
-        float v0[kGroupLanes], v1[kGroupLanes];
+        int cnt = valid ? min(L, ns - r * L) : 0;
+        float v[E][L];
 #pragma unroll
-        for (int q = 0; q < kGroupLanes; ++q)
-        {
-            v0[q] = rec[g][q][e0];
-            v1[q] = e1 < kRec ? rec[g][q][e1] : 0.0f;
-        }
+        for (int e = 0; e < E; ++e)
 #pragma unroll
-        for (int q = 0; q < kGroupLanes; ++q)
+            for (int q = 0; q < L; ++q)
+                v[e][q] = lane + e * L < kRec ? rec[g][q][lane + e * L] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < L; ++q)
         {
             if (q < cnt)
             {
-                acc0 += v0[q];
-                acc1 += v1[q];
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    acc[e] += v[e][q];
             }
         }
         wave_lds_sync();
     }
-    rec[g][0][e0] = acc0;
-    if (e1 < kRec)
-        rec[g][0][e1] = acc1;
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        if (lane + e * L < kRec)
+            rec[g][0][lane + e * L] = acc[e];
     wave_lds_sync();
     if (valid && lane == 0)
     {

@@ -31,7 +31,7 @@ for _p in (ROOT, os.path.join(ROOT, "avbd-demo3d_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_iterate_pile50k_traffic.txt")
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_primal_color_pile50k_traffic.txt")
 
 
 def log(msg):
@@ -39,7 +39,7 @@ def log(msg):
 
 
 def measured_traffic():
-    """HBM bytes per k_iterate launch from the committed rocprofv3 PMC summary, if present."""
+    """HBM bytes per k_primal_color launch from the committed rocprofv3 PMC summary, if present."""
     try:
         with open(TRAFFIC_SUMMARY) as f:
             for line in f:
@@ -76,7 +76,8 @@ def cpu_baseline(state, params, target_s):
 
 
 def calibrate_copy(mib):
-    """A plain device-to-device copy of known size (for scaling PMC FETCH/WRITE counters)."""
+    """A plain device-to-device copy of known size, run before the bench in the same process so
+    that one rocprofv3 PMC pass reads FETCH_SIZE/WRITE_SIZE for both the copy and the step kernels."""
     import torch
     n = mib * (1 << 20) // 4
     x = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -96,6 +97,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--batch-worlds", type=int, default=4096, help="0 skips the batched measurement")
+    ap.add_argument("--batch-steps", type=int, default=300)
+    ap.add_argument("--batch-warmup", type=int, default=20)
+    ap.add_argument("--batch-sample", type=int, default=64,
+                    help="worlds stepped one handle at a time for the batched-vs-separate comparison")
     args = ap.parse_args()
 
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
@@ -112,7 +119,6 @@ def main():
     torch.cuda.init()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
-        return
 
     from avbd import capi, scenes
     from avbd.state import default_params
@@ -127,6 +133,13 @@ def main():
     log(f"warmup {args.warmup} steps")
     world.step(args.warmup)
     world.synchronize()
+    launch = world.launch_config()
+    if not args.no_tune:
+        # direct launches only: the per-kernel timing marks need them
+        launch = world.autotune(trial_steps=3, allow_graph=False)
+        log(f"autotuned launch shape {launch}")
+        world.step(2)
+        world.synchronize()
     torch.cuda.synchronize()
     world.reset_stats()
     log(f"timing {args.steps} steps")
@@ -191,6 +204,7 @@ def main():
             "step_bytes_per_step": step_bytes,
             "iterate_ms_per_step": iter_ms,
             "step_gbs_algorithmic": step_bytes / (wall / args.steps) / 1e9,
+            "launch": launch,
         },
         "roofline": {
             "bound": "hbm",
@@ -216,7 +230,74 @@ def main():
         result["cpu_baseline"] = cpu_baseline(final, params, args.cpu_seconds)
     world.close()
     torch.cuda.synchronize()
+    if args.batch_worlds > 0:
+        result["batched"] = batched(args, params)
     print(json.dumps(result), flush=True)
+
+
+def batched(args, params):
+    """4096 independent worlds (body counts log-uniform in [8, 512]) in one handle, against
+    the same worlds stepped one handle (one world) at a time."""
+    import numpy as np
+    import torch
+    from avbd import capi, scenes
+    log(f"batched: building {args.batch_worlds} worlds")
+    worlds = scenes.batch(args.batch_worlds, seed=4)
+    maker = capi.Maker()
+    states = [scenes.build_state(w, maker) for w in worlds]
+    n_bodies = sum(len(s.bodies) for s in states)
+    n_dyn = sum(s.num_dynamic for s in states)
+    world = capi.HipWorld.for_batch(states, params)
+    world.step(2)
+    world.synchronize()
+    launch = world.autotune(trial_steps=3, allow_graph=True)
+    log(f"batched: {n_bodies} bodies ({n_dyn} dynamic); autotuned launch shape {launch}")
+    world.step(args.batch_warmup)
+    world.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    world.step(args.batch_steps)
+    world.synchronize()
+    wall = time.perf_counter() - t0
+    st = world.stats()
+    world.close()
+    ms = wall * 1e3 / args.batch_steps
+    log(f"batched: {ms:.3f} ms/step, {int(st['color_count'])} colours, {int(st['contact_count'])} contacts")
+    # one world at a time: every k-th world on its own handle (same warm-up), stepped in turn
+    k = max(1, args.batch_worlds // max(args.batch_sample, 1))
+    sample = list(range(0, args.batch_worlds, k))[:args.batch_sample]
+    steps_each = 10
+    per_world = []
+    for w in sample:
+        one = capi.HipWorld.for_state(states[w], params)
+        one.step(2 + args.batch_warmup)
+        one.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps_each):
+            one.step(1)
+            one.synchronize()
+        per_world.append((time.perf_counter() - t1) * 1e3 / steps_each)
+        one.close()
+    sep_ms = float(np.mean(per_world)) * args.batch_worlds
+    log(f"one world at a time: {np.mean(per_world):.3f} ms per world-step over {len(sample)} sampled worlds "
+        f"-> {sep_ms:.1f} ms per step of all {args.batch_worlds} worlds")
+    return {
+        "workload": f"{args.batch_worlds} worlds, body counts log-uniform in [8, 512] (stack fields, piles, "
+                    f"chains); dt=1/60, 10 iterations; steps {args.batch_warmup + 3}-"
+                    f"{args.batch_warmup + 2 + args.batch_steps} timed",
+        "worlds": args.batch_worlds,
+        "bodies": int(n_bodies),
+        "dynamic_bodies": int(n_dyn),
+        "steps": args.batch_steps,
+        "ms_per_step": ms,
+        "value": n_dyn * args.batch_steps / wall,
+        "unit": "body-steps/s",
+        "launch": launch,
+        "one_at_a_time": {"sample_worlds": len(sample), "steps_each": steps_each,
+                          "ms_per_world_step": float(np.mean(per_world)),
+                          "ms_per_step_all_worlds": sep_ms},
+        "speedup_vs_one_at_a_time": sep_ms / ms,
+    }
 
 
 if __name__ == "__main__":

@@ -274,12 +274,53 @@ int avbd_get_colors(avbd_handle *h, int32_t *colors, int32_t capacity);
  * (index_a, index_b, created) triples; created = 1 when a new manifold was made. */
 int avbd_get_hits(avbd_handle *h, int32_t *triples, int32_t capacity, int32_t *count);
 
+/* ---- Batched worlds ----
+ * Many independent solvers in one handle. The reference runs one Solver per
+ * world (class Solver, solver.h:345-420, stepped by Solver::step,
+ * solver.cpp:934-937); here the worlds' bodies are stored one world after
+ * another and every launch of a step covers all of them. World w owns bodies
+ * [world_body_offsets[w], world_body_offsets[w + 1]), each world in its own
+ * list order. Joint, spring and manifold body indices are batch-wide and must
+ * stay inside one world; serials must be unique over the batch (any renumbering
+ * that keeps each world's order works). With more than one world, a world holds
+ * at most AVBD_MAX_WORLD_BODIES bodies. Stepping a batch gives every world
+ * exactly the result of stepping it alone. avbd_upload_state resets the handle
+ * to one world. */
+#define AVBD_MAX_WORLD_BODIES 2048
+int avbd_batch_upload(avbd_handle *h, const int32_t *world_body_offsets, int32_t num_worlds,
+                      const avbd_body *bodies, int32_t num_bodies,
+                      const avbd_joint *joints, int32_t num_joints,
+                      const avbd_spring *springs, int32_t num_springs,
+                      const avbd_manifold *manifolds, int32_t num_manifolds,
+                      int64_t next_serial);
+int avbd_batch_num_worlds(avbd_handle *h, int32_t *num_worlds);
+
+/* ---- Launch shapes and the autotuner ----
+ * No counterpart in the reference (its WebGPU lane uses fixed 64-thread
+ * workgroups, avbd_gpu_solver.cpp:502-714). Every shape gives bit-identical
+ * results; shapes differ only in speed. Defaults: 16 lanes, 256 threads,
+ * dual 256 threads, no graph. */
+typedef struct avbd_launch_config {
+    int32_t primal_lanes;    /* lanes per body in the primal kernel: 8, 16 or 32 */
+    int32_t primal_threads;  /* threads per primal block: 64, 128 or 256 */
+    int32_t dual_threads;    /* threads per dual block: 64, 128 or 256 */
+    int32_t use_graph;       /* 1: each solver iteration is one hipGraph launch */
+} avbd_launch_config;
+int avbd_set_launch_config(avbd_handle *h, const avbd_launch_config *cfg);
+int avbd_get_launch_config(avbd_handle *h, avbd_launch_config *out);
+#define AVBD_TUNE_NO_GRAPH 1   /* keep direct launches (per-kernel timing marks stay exact) */
+/* Times every launch shape on the resident state for trial_steps steps each and
+ * keeps the fastest. The state is restored before each trial and at the end,
+ * so tuning changes speed only; stats sums are reset. out may be NULL. */
+int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launch_config *out);
+
 /* Test hook: make the next avbd_step on this handle fail with AVBD_ERR_INJECTED
  * before any work is enqueued (code 1), or clear it (code 0). */
 int avbd_debug_inject_fault(avbd_handle *h, int32_t code);
 
 /* sizeof of the ABI records (0 body, 1 contact, 2 manifold, 3 joint, 4 spring,
- * 5 params), so bindings can verify their layouts; -1 for other values. */
+ * 5 params, 6 launch config), so bindings can verify their layouts; -1 for
+ * other values. */
 int avbd_sizeof(int which);
 
 #ifdef __cplusplus

@@ -66,7 +66,7 @@ def load_oracle():
         lib.orc_quat_ops.argtypes = [_p] * 7
         lib.orc_joint_spring_derived.argtypes = [_p, _p, _p, _p, _p, _p]
         lib.orc_sizeof.argtypes = [_i]
-        check_layouts(lib.orc_sizeof)
+        check_layouts(lib.orc_sizeof, records=6)
         _ORC = lib
     return _ORC
 

@@ -1,0 +1,136 @@
+"""Batched worlds and launch shapes on the GPU.
+
+* A batch of independent worlds stepped in one handle (avbd_batch_upload) equals
+  every world stepped alone: against the CPU oracle per world (colour order) and
+  against one GPU handle per world, exact FP32 (value equality).
+* Every launch shape of the iteration kernels (lanes per body, block sizes, graph
+  launch) gives the same state, and the autotuner leaves the state unchanged.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_states_equal
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from avbd import capi
+    capi.load()
+    return capi
+
+
+def _batch_scenes(num, seed, hi):
+    from avbd import scenes
+    ws = scenes.batch(num, seed=seed, hi=hi)
+    # edge worlds: a single static body, one box on the ground, a two-link chain
+    ws.append(scenes.stacks(0, seed=5))
+    ws.append(scenes.stacks(1, seed=6))
+    ws.append(scenes.pyramid_chain(seed=7, base=0, links=2))
+    return ws
+
+
+def test_batch_equals_each_world_on_oracle(gpu, oracle):
+    from avbd import scenes
+    from avbd.state import default_params, split_state
+    ws = _batch_scenes(20, seed=21, hi=160)
+    prm = default_params()
+    g = gpu.HipWorld.for_batch([scenes.build_state(w, gpu.Maker()) for w in ws], prm)
+    assert g.num_worlds == len(ws)
+    cpu = [oracle.OracleWorld(scenes.build_state(w, oracle.OracleMaker()), prm) for w in ws]
+    done = 0
+    for s in (1, 4, 15):
+        g.step(s - done)
+        for c in cpu:
+            c.step(s - done)
+        done = s
+        for w, (a, c) in enumerate(zip(split_state(g.state(), g.world_offsets), cpu)):
+            assert_states_equal(a, c.state(), f"world {w} ({ws[w].name}) step {s}")
+
+
+def test_batch_equals_one_handle_per_world(gpu):
+    from avbd import scenes
+    from avbd.state import default_params, split_state
+    ws = _batch_scenes(60, seed=22, hi=512)
+    prm = default_params()
+    states = [scenes.build_state(w, gpu.Maker()) for w in ws]
+    g = gpu.HipWorld.for_batch(states, prm)
+    g.set_launch_config(16, 256, 256, use_graph=1)
+    g.step(12)
+    got = split_state(g.state(), g.world_offsets)
+    for w, st in enumerate(states):
+        one = gpu.HipWorld.for_state(st, prm)
+        one.step(12)
+        assert_states_equal(got[w], one.state(), f"world {w} ({ws[w].name})")
+        one.close()
+
+
+def test_every_launch_shape_gives_the_same_state(gpu):
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scenes.pile(1500, seed=31)
+    prm = default_params(gravity=sc.gravity)
+    st = scenes.build_state(sc, gpu.Maker())
+    ref = gpu.HipWorld.for_state(st, prm)
+    ref.step(40)
+    base = ref.state()
+    ref.close()
+    shapes = [(l, t, d, gr) for l in (8, 16, 32) for t in (64, 128, 256) for d, gr in ((64, 0), (256, 1))]
+    for l, t, d, gr in shapes:
+        w = gpu.HipWorld.for_state(st, prm)
+        w.set_launch_config(l, t, d, use_graph=gr)
+        assert w.launch_config() == {"primal_lanes": l, "primal_threads": t, "dual_threads": d, "use_graph": gr}
+        w.step(40)
+        assert_states_equal(w.state(), base, f"lanes {l} threads {t} dual {d} graph {gr}")
+        w.close()
+
+
+def test_autotune_keeps_state_and_picks_a_valid_shape(gpu):
+    from avbd import scenes
+    from avbd.state import default_params, split_state
+    ws = _batch_scenes(40, seed=23, hi=256)
+    prm = default_params()
+    states = [scenes.build_state(w, gpu.Maker()) for w in ws]
+    plain = gpu.HipWorld.for_batch(states, prm)
+    plain.step(5)
+    tuned = gpu.HipWorld.for_batch(states, prm)
+    tuned.step(2)
+    cfg = tuned.autotune(trial_steps=2)
+    assert cfg["primal_lanes"] in (8, 16, 32) and cfg["primal_threads"] in (64, 128, 256)
+    assert cfg["dual_threads"] in (64, 128, 256) and cfg["use_graph"] in (0, 1)
+    assert tuned.launch_config() == cfg
+    tuned.step(3)
+    for w, (a, b) in enumerate(zip(split_state(tuned.state(), tuned.world_offsets),
+                                   split_state(plain.state(), plain.world_offsets))):
+        assert_states_equal(a, b, f"world {w}")
+
+
+def test_batch_upload_rejects_bad_worlds(gpu):
+    from avbd import scenes
+    from avbd.state import batch_states, default_params
+    prm = default_params()
+    states = [scenes.build_state(scenes.pyramid_chain(seed=3, base=0, links=6), gpu.Maker()),
+              scenes.build_state(scenes.stacks(4, seed=4), gpu.Maker())]
+    batch, off = batch_states(states)
+    w = gpu.HipWorld(prm, max_bodies=4000, max_joints=16, max_springs=16)
+    bad = batch.copy()
+    bad.joints["body_b"][-1] = off[1]          # a joint reaching into the next world
+    with pytest.raises(gpu.AvbdError, match="spans two worlds"):
+        w.upload_batch(bad, off)
+    with pytest.raises(gpu.AvbdError, match="offsets"):
+        w.upload_batch(batch, np.array([0, 3, len(batch.bodies) + 1], np.int32))
+    with pytest.raises(gpu.AvbdError, match="at least one body"):
+        w.upload_batch(batch, np.array([0, 0, len(batch.bodies)], np.int32))
+    big = scenes.build_state(scenes.stacks(2100, seed=8), gpu.Maker())
+    two, off2 = batch_states([big, states[1]])
+    w2 = gpu.HipWorld(prm, max_bodies=len(two.bodies))
+    with pytest.raises(gpu.AvbdError, match="AVBD_MAX_WORLD_BODIES"):
+        w2.upload_batch(two, off2)
+    with pytest.raises(gpu.AvbdError):
+        w.set_launch_config(12, 256, 256, 0)
+    w.upload_batch(batch, off)
+    w.step(2)
+    w.synchronize()
