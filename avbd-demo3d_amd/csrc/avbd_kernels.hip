@@ -17,6 +17,7 @@
 // the build uses -ffp-contract=off.
 #include "avbd_device.hpp"
 #include "avbd_math.hpp"
+#include "avbd_collide.hpp"
 
 #include <float.h>
 
@@ -27,9 +28,6 @@ constexpr float kPenaltyMin = 1.0f;
 constexpr float kPenaltyMax = 10000000000.0f;
 constexpr float kMargin = 0.01f;
 constexpr float kStickThresh = 0.00001f;
-constexpr float kSatEps = 1.0e-6f;
-constexpr float kPlaneEps = 1.0e-5f;
-constexpr float kMergeDistSq = 1.0e-6f;
 
 // ------------------------------------------------------------------------------------------
 // wave / block primitives (wave64, blockDim multiple of 64, <= 1024)
@@ -756,318 +754,7 @@ __global__ void k_create_finish(Counters *ctr, int maxManifolds, int maxPairs)
 // ------------------------------------------------------------------------------------------
 // box-box narrowphase (collide.cpp:60-433, 898-955)
 // ------------------------------------------------------------------------------------------
-struct Obb
-{
-    V3 c, h, ax0, ax1, ax2;
-};
-
-__device__ __forceinline__ V3 obb_ax(const Obb &o, int k) { return k == 0 ? o.ax0 : (k == 1 ? o.ax1 : o.ax2); }
-
-__device__ __forceinline__ Obb make_obb(V3 pos, Q4 rot, V3 size)
-{
-    Obb o;
-    o.c = pos;
-    o.h = size * 0.5f;
-    o.ax0 = rotate(rot, V3{1.0f, 0.0f, 0.0f});
-    o.ax1 = rotate(rot, V3{0.0f, 1.0f, 0.0f});
-    o.ax2 = rotate(rot, V3{0.0f, 0.0f, 1.0f});
-    return o;
-}
-
-__device__ __forceinline__ V3 support_pt(const Obb &o, V3 d)
-{
-    float sx = dot(d, o.ax0) >= 0.0f ? 1.0f : -1.0f;
-    float sy = dot(d, o.ax1) >= 0.0f ? 1.0f : -1.0f;
-    float sz = dot(d, o.ax2) >= 0.0f ? 1.0f : -1.0f;
-    return o.c + o.ax0 * (o.h.x * sx) + o.ax1 * (o.h.y * sy) + o.ax2 * (o.h.z * sz);
-}
-
-struct SatBest
-{
-    int type, ia, ib;
-    float sep;
-    V3 n;
-    bool valid;
-};
-
-__device__ __forceinline__ bool sat_axis(const Obb &A, const Obb &B, V3 delta, V3 axis, int type, int ia, int ib,
-                                         SatBest &best)
-{
-    float l2 = lensq(axis);
-    if (l2 < kSatEps)
-        return true;
-    float inv = 1.0f / sqrtf(l2);
-    V3 n = axis * inv;
-    if (dot(n, delta) < 0.0f)
-        n = -n;
-    float dist = fabsf(dot(delta, n));
-    float rA = A.h.x * fabsf(dot(n, A.ax0)) + A.h.y * fabsf(dot(n, A.ax1)) + A.h.z * fabsf(dot(n, A.ax2));
-    float rB = B.h.x * fabsf(dot(n, B.ax0)) + B.h.y * fabsf(dot(n, B.ax1)) + B.h.z * fabsf(dot(n, B.ax2));
-    float sep = dist - (rA + rB);
-    if (sep > 0.0f)
-        return false;
-    if (!best.valid || sep > best.sep)
-    {
-        best.valid = true;
-        best.type = type;
-        best.ia = ia;
-        best.ib = ib;
-        best.sep = sep;
-        best.n = n;
-    }
-    return true;
-}
-
-struct NewContacts
-{
-    int count;
-    int feature[kMaxContacts];
-    V3 rA[kMaxContacts];
-    V3 rB[kMaxContacts];
-    V3 mid[kMaxContacts];
-};
-
-__device__ __forceinline__ void add_contact(NewContacts &nc, V3 posA, Q4 rotA, V3 posB, Q4 rotB, V3 xA, V3 xB, int key)
-{
-    V3 m = (xA + xB) * 0.5f;
-    for (int i = 0; i < nc.count; ++i)
-        if (lensq(m - nc.mid[i]) < kMergeDistSq)
-            return;
-    if (nc.count >= kMaxContacts)
-        return;
-    int c = nc.count;
-    nc.feature[c] = key;
-    nc.rA[c] = rotate(qconj(rotA), xA - posA);
-    nc.rB[c] = rotate(qconj(rotB), xB - posB);
-    nc.mid[c] = m;
-    nc.count = c + 1;
-}
-
-__device__ __forceinline__ int clip_plane(const V3 *in, int n, V3 pn, float off, V3 *out)
-{
-    if (n <= 0)
-        return 0;
-    int m = 0;
-    V3 a = in[n - 1];
-    float da = dot(pn, a) - off;
-    for (int i = 0; i < n; ++i)
-    {
-        V3 b = in[i];
-        float db = dot(pn, b) - off;
-        bool ain = da <= kPlaneEps;
-        bool bin = db <= kPlaneEps;
-        if (ain != bin)
-        {
-            float t = 0.0f;
-            float den = da - db;
-            if (fabsf(den) > kSatEps)
-                t = clampr(da / den, 0.0f, 1.0f);
-            if (m < 16)
-                out[m++] = a + (b - a) * t;
-        }
-        if (bin && m < 16)
-            out[m++] = b;
-        a = b;
-        da = db;
-    }
-    return m;
-}
-
-__device__ __forceinline__ void face_axes(const Obb &o, int k, V3 &u, V3 &v, float &eu, float &ev)
-{
-    if (k == 0)
-    {
-        u = o.ax1;
-        v = o.ax2;
-        eu = o.h.y;
-        ev = o.h.z;
-    }
-    else if (k == 1)
-    {
-        u = o.ax0;
-        v = o.ax2;
-        eu = o.h.x;
-        ev = o.h.z;
-    }
-    else
-    {
-        u = o.ax0;
-        v = o.ax1;
-        eu = o.h.x;
-        ev = o.h.y;
-    }
-}
-
-__device__ void face_manifold(NewContacts &nc, V3 posA, Q4 rotA, V3 posB, Q4 rotB, const Obb &A, const Obb &B,
-                              bool refA, int refAxis, V3 nAB)
-{
-    const Obb &ref = refA ? A : B;
-    const Obb &inc = refA ? B : A;
-    V3 outward = refA ? nAB : -nAB;
-    V3 rax = obb_ax(ref, refAxis);
-    float fs = dot(outward, rax) >= 0.0f ? 1.0f : -1.0f;
-    V3 fn = rax * fs;
-    V3 fc = ref.c + fn * comp(ref.h, refAxis);
-    V3 fu, fv;
-    float eu, ev;
-    face_axes(ref, refAxis, fu, fv, eu, ev);
-
-    int incAxis = 0;
-    float best = -FLT_MAX;
-    for (int i = 0; i < 3; ++i)
-    {
-        float d = fabsf(dot(obb_ax(inc, i), fn));
-        if (d > best)
-        {
-            best = d;
-            incAxis = i;
-        }
-    }
-    V3 p0[16], p1[16];
-    {
-        V3 iax = obb_ax(inc, incAxis);
-        float s = dot(iax, fn) > 0.0f ? -1.0f : 1.0f;
-        V3 n = iax * s;
-        V3 c = inc.c + n * comp(inc.h, incAxis);
-        V3 u, v;
-        float ue, ve;
-        face_axes(inc, incAxis, u, v, ue, ve);
-        p0[0] = c + u * ue + v * ve;
-        p0[1] = c - u * ue + v * ve;
-        p0[2] = c - u * ue - v * ve;
-        p0[3] = c + u * ue - v * ve;
-    }
-    int n = 4;
-    n = clip_plane(p0, n, fu, dot(fu, fc) + eu, p1);
-    if (!n)
-        return;
-    V3 nu = -fu;
-    n = clip_plane(p1, n, nu, dot(nu, fc) + eu, p0);
-    if (!n)
-        return;
-    n = clip_plane(p0, n, fv, dot(fv, fc) + ev, p1);
-    if (!n)
-        return;
-    V3 nv = -fv;
-    n = clip_plane(p1, n, nv, dot(nv, fc) + ev, p0);
-    if (!n)
-        return;
-    int prefix = ((refA ? 0 : 1) << 24) | ((refAxis & 0xFF) << 16) | ((incAxis & 0xFF) << 8);
-    for (int i = 0; i < n && nc.count < kMaxContacts; ++i)
-    {
-        V3 pi = p0[i];
-        float depth = dot(pi - fc, fn);
-        if (depth > kPlaneEps)
-            continue;
-        V3 pr = pi - fn * depth;
-        add_contact(nc, posA, rotA, posB, rotB, refA ? pr : pi, refA ? pi : pr, prefix | (i & 0xFF));
-    }
-    if (!nc.count)
-        add_contact(nc, posA, rotA, posB, rotB, support_pt(A, nAB), support_pt(B, -nAB), prefix);
-}
-
-__device__ __forceinline__ void edge_support(const Obb &o, int k, V3 d, V3 &e0, V3 &e1)
-{
-    int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
-    V3 a1 = obb_ax(o, k1), a2 = obb_ax(o, k2), a0 = obb_ax(o, k);
-    float s1 = dot(d, a1) >= 0.0f ? 1.0f : -1.0f;
-    float s2 = dot(d, a2) >= 0.0f ? 1.0f : -1.0f;
-    V3 mid = o.c + a1 * (comp(o.h, k1) * s1) + a2 * (comp(o.h, k2) * s2);
-    e0 = mid - a0 * comp(o.h, k);
-    e1 = mid + a0 * comp(o.h, k);
-}
-
-__device__ __forceinline__ void closest_seg(V3 p0, V3 p1, V3 q0, V3 q1, V3 &c0, V3 &c1)
-{
-    V3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
-    float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
-    float s = 0.0f, t = 0.0f;
-    if (a <= kSatEps && e <= kSatEps)
-    {
-        c0 = p0;
-        c1 = q0;
-        return;
-    }
-    if (a <= kSatEps)
-        t = clampr(f / e, 0.0f, 1.0f);
-    else
-    {
-        float c = dot(d1, r);
-        if (e <= kSatEps)
-            s = clampr(-c / a, 0.0f, 1.0f);
-        else
-        {
-            float b = dot(d1, d2);
-            float den = a * e - b * b;
-            if (fabsf(den) > kSatEps)
-                s = clampr((b * f - c * e) / den, 0.0f, 1.0f);
-            t = (b * s + f) / e;
-            if (t < 0.0f)
-            {
-                t = 0.0f;
-                s = clampr(-c / a, 0.0f, 1.0f);
-            }
-            else if (t > 1.0f)
-            {
-                t = 1.0f;
-                s = clampr((b - c) / a, 0.0f, 1.0f);
-            }
-        }
-    }
-    c0 = p0 + d1 * s;
-    c1 = q0 + d2 * t;
-}
-
-// Returns the contact count; basis rows are written only when the boxes overlap on all 15 axes.
-__device__ int collide_boxes(V3 posA, Q4 rotA, V3 sizeA, V3 posB, Q4 rotB, V3 sizeB, NewContacts &nc, V3 &b0, V3 &b1,
-                             V3 &b2)
-{
-    nc.count = 0;
-    Obb A = make_obb(posA, rotA, sizeA), B = make_obb(posB, rotB, sizeB);
-    V3 delta = B.c - A.c;
-    SatBest face, edge;
-    face.valid = false;
-    face.sep = -FLT_MAX;
-    face.type = face.ia = face.ib = 0;
-    face.n = V3{0, 0, 0};
-    edge = face;
-    for (int i = 0; i < 3; ++i)
-        if (!sat_axis(A, B, delta, obb_ax(A, i), 0, i, -1, face))
-            return 0;
-    for (int i = 0; i < 3; ++i)
-        if (!sat_axis(A, B, delta, obb_ax(B, i), 1, -1, i, face))
-            return 0;
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j)
-            if (!sat_axis(A, B, delta, cross(obb_ax(A, i), obb_ax(B, j)), 2, i, j, edge))
-                return 0;
-    if (!face.valid)
-        return 0;
-    SatBest best = face;
-    if (edge.valid && 0.95f * edge.sep > face.sep + 0.01f)
-        best = edge;
-    M3 basis = orthonormal(-best.n);
-    b0 = basis.r0;
-    b1 = basis.r1;
-    b2 = basis.r2;
-    if (best.type == 2)
-    {
-        V3 a0, a1, q0, q1, xA, xB;
-        edge_support(A, best.ia, best.n, a0, a1);
-        edge_support(B, best.ib, -best.n, q0, q1);
-        closest_seg(a0, a1, q0, q1, xA, xB);
-        int key = (2 << 24) | ((best.ia & 0xFF) << 8) | (best.ib & 0xFF);
-        add_contact(nc, posA, rotA, posB, rotB, xA, xB, key);
-        if (!nc.count)
-            add_contact(nc, posA, rotA, posB, rotB, support_pt(A, best.n), support_pt(B, -best.n), key);
-        return nc.count;
-    }
-    if (best.type == 0)
-        face_manifold(nc, posA, rotA, posB, rotB, A, B, true, best.ia, best.n);
-    else
-        face_manifold(nc, posA, rotA, posB, rotB, A, B, false, best.ib, best.n);
-    return nc.count;
-}
+// (collide_boxes<T> in avbd_collide.hpp)
 
 // Manifold::initialize for box pairs (manifold.cpp:100-174). Reads the persistent buffer `cur`
 // and writes the initialised manifold to the same slot of `tmp`.
@@ -1083,9 +770,10 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
         Q4 qA = ldq(B.pose[4 * (ab.x) + P_ROT]), qB = ldq(B.pose[4 * (ab.y) + P_ROT]);
         V3 sA = ld3(B.sizeRad[ab.x]), sB = ld3(B.sizeRad[ab.y]);
         float friction = sqrtf(B.friction[ab.x] * B.friction[ab.y]);
-        NewContacts nc;
-        V3 b0{0, 0, 0}, b1{0, 0, 0}, b2{0, 0, 0};
-        int nn = collide_boxes(pA, qA, sA, pB, qB, sB, nc, b0, b1, b2);
+        NewContactsT<float> nc;
+        V3T<float> t0{0, 0, 0}, t1{0, 0, 0}, t2{0, 0, 0};
+        int nn = collide_boxes<float>(tv(pA), tq(qA), tv(sA), tv(pB), tq(qB), tv(sB), nc, t0, t1, t2);
+        V3 b0 = fv(t0), b1 = fv(t1), b2 = fv(t2);
         if (nn == 0)
         {
             keep[m] = 0;
@@ -1097,7 +785,7 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
         tmp.count[m] = nn;
         for (int i = 0; i < nn; ++i)
         {
-            V3 rA = nc.rA[i], rB = nc.rB[i];
+            V3 rA = fv(nc.rA[i]), rB = fv(nc.rB[i]);
             float4 pen = make_float4(0, 0, 0, 0), lam = make_float4(0, 0, 0, 0);
             for (int j = 0; j < nOld; ++j)
             {
@@ -1608,7 +1296,9 @@ __device__ __forceinline__ D3 d3_diag(float a, float b, float c)
 __device__ __forceinline__ D3 d3_mul(const D3 &a, const D3 &b)
 {
     D3 r;
+#pragma unroll
     for (int i = 0; i < 3; ++i)
+#pragma unroll
         for (int j = 0; j < 3; ++j)
             r.m[i][j] = (a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j]) + a.m[i][2] * b.m[2][j];
     return r;
@@ -1616,7 +1306,9 @@ __device__ __forceinline__ D3 d3_mul(const D3 &a, const D3 &b)
 __device__ __forceinline__ D3 d3_t(const D3 &a)
 {
     D3 r;
+#pragma unroll
     for (int i = 0; i < 3; ++i)
+#pragma unroll
         for (int j = 0; j < 3; ++j)
             r.m[i][j] = a.m[j][i];
     return r;
@@ -1624,7 +1316,9 @@ __device__ __forceinline__ D3 d3_t(const D3 &a)
 __device__ __forceinline__ D3 d3_scale(const D3 &a, float s)
 {
     D3 r;
+#pragma unroll
     for (int i = 0; i < 3; ++i)
+#pragma unroll
         for (int j = 0; j < 3; ++j)
             r.m[i][j] = a.m[i][j] * s;
     return r;
@@ -1639,14 +1333,12 @@ __device__ __forceinline__ D3 d3_skew(V3 r)
     D3 o = {{{0.0f, -r.z, r.y}, {r.z, 0.0f, -r.x}, {-r.y, r.x, 0.0f}}};
     return o;
 }
-__device__ __forceinline__ D3 gs_ball(int k, V3 v)
+// Entry (a, b) of the ball joint's geometric-stiffness block G_k(v) (joint.cpp:96-104): the
+// diagonal -v_k is written first, then v is added to column k.
+template <int k, int a, int b> __device__ __forceinline__ float gs_entry(V3 v)
 {
-    float vk = comp(v, k);
-    D3 m = d3_diag(-vk, -vk, -vk);
-    m.m[0][k] += v.x;
-    m.m[1][k] += v.y;
-    m.m[2][k] += v.z;
-    return m;
+    float d = a == b ? -comp(v, k) : 0.0f;
+    return b == k ? d + comp(v, a) : d;
 }
 __device__ __forceinline__ void rec_lin(float *r, const D3 &a)
 {
@@ -1696,7 +1388,9 @@ __device__ __forceinline__ void joint_record(float *r, int i, const Joint &j, in
             rec_lin(r, d3_mul(d3_mul(jLT, K), jL));
             rec_ang(r, d3_mul(jATk, jA));
             D3 cr = d3_mul(jATk, jL);
+#pragma unroll
             for (int a = 0; a < 3; ++a)
+#pragma unroll
                 for (int b = 0; b < 3; ++b)
                     r[R_C + 3 * a + b] = cr.m[a][b];
             V3 rl = d3_mv(jLT, F), ra = d3_mv(jAT, F);
@@ -1710,13 +1404,12 @@ __device__ __forceinline__ void joint_record(float *r, int i, const Joint &j, in
         else
         {
             V3 rr = isA ? rAW : -rBW;
-            D3 g0 = d3_scale(gs_ball(0, rr), F.x), g1 = d3_scale(gs_ball(1, rr), F.y), g2 = d3_scale(gs_ball(2, rr), F.z);
-            D3 H;
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b)
-                    H.m[a][b] = (g0.m[a][b] + g1.m[a][b]) + g2.m[a][b];
-            V3 c0{H.m[0][0], H.m[1][0], H.m[2][0]}, c1{H.m[0][1], H.m[1][1], H.m[2][1]},
-                c2{H.m[0][2], H.m[1][2], H.m[2][2]};
+            // H = sum_k F_k G_k(rr), entry by entry (straight-line, so the record stays in registers)
+#define AVBD_GS_H(a, b) ((gs_entry<0, a, b>(rr) * F.x + gs_entry<1, a, b>(rr) * F.y) + gs_entry<2, a, b>(rr) * F.z)
+            V3 c0{AVBD_GS_H(0, 0), AVBD_GS_H(1, 0), AVBD_GS_H(2, 0)};
+            V3 c1{AVBD_GS_H(0, 1), AVBD_GS_H(1, 1), AVBD_GS_H(2, 1)};
+            V3 c2{AVBD_GS_H(0, 2), AVBD_GS_H(1, 2), AVBD_GS_H(2, 2)};
+#undef AVBD_GS_H
             rec_ang(r, d3_diag(len(c0), len(c1), len(c2)));
         }
         return;
@@ -1883,7 +1576,8 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
     static_assert(L >= 4 && L <= 64 && (64 % L) == 0 && (T % 64) == 0, "a body's lanes must sit in one wave");
     constexpr int G = T / L;               // bodies per block
     constexpr int E = (kRec + L - 1) / L;  // accumulated entries per lane
-    __shared__ float rec[G][L][kRec + 1];
+    constexpr int kRow = kRec + 2;         // odd row stride: a wave's row writes hit distinct banks
+    __shared__ float rec[T * kRow];
     const int lane = threadIdx.x % L, g = threadIdx.x / L;
     const int k = begin + blockIdx.x * G + g;
     const bool valid = k < end;
@@ -1910,24 +1604,32 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
     for (int r = 0; r < rounds; ++r)
     {
         int t = r * L + lane;
-        float out[kRec];
-#pragma unroll
-        for (int q = 0; q < kRec; ++q)
-            out[q] = 0.0f;
+        // the record is written straight into this lane's LDS row; a contact writes all of it,
+        // the other kinds write part of it and the row is cleared first
+        float *row = rec + (g * L + lane) * kRow;
         if (valid && t < ns)
         {
             int4 st = stamps[s0 + t];
             int kind = adjKind(st.x), idx = adjIndex(st.x);
             if (kind == kAdjManifold)
-                contact_record(out, i, st.z, (st.y >> 16) != 0, st.w, M, B, oma);
-            else if (kind == kAdjJoint)
-                joint_record(out, i, joints[idx], st.y & 0xFFFF, B, P.alpha);
+                contact_record(row, i, st.z, (st.y >> 16) != 0, st.w, M, B, oma);
             else
-                spring_record(out, i, springs[idx], B);
-        }
+            {
 #pragma unroll
-        for (int q = 0; q < kRec; ++q)
-            rec[g][lane][q] = out[q];
+                for (int q = 0; q < kRec; ++q)
+                    row[q] = 0.0f;
+                if (kind == kAdjJoint)
+                    joint_record(row, i, joints[idx], st.y & 0xFFFF, B, P.alpha);
+                else
+                    spring_record(row, i, springs[idx], B);
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < kRec; ++q)
+                row[q] = 0.0f;
+        }
         wave_lds_sync();
         int cnt = valid ? min(L, ns - r * L) : 0;
         float v[E][L];
@@ -1935,7 +1637,7 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
         for (int e = 0; e < E; ++e)
 #pragma unroll
             for (int q = 0; q < L; ++q)
-                v[e][q] = lane + e * L < kRec ? rec[g][q][lane + e * L] : 0.0f;
+                v[e][q] = lane + e * L < kRec ? rec[(g * L + q) * kRow + lane + e * L] : 0.0f;
 #pragma unroll
         for (int q = 0; q < L; ++q)
         {
@@ -1951,11 +1653,11 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
 #pragma unroll
     for (int e = 0; e < E; ++e)
         if (lane + e * L < kRec)
-            rec[g][0][lane + e * L] = acc[e];
+            rec[g * L * kRow + lane + e * L] = acc[e];
     wave_lds_sync();
     if (valid && lane == 0)
     {
-        const float *a = rec[g][0];
+        const float *a = rec + g * L * kRow;
         Sys6 sys;
         sys.l00 = a[R_L00];
         sys.l10 = a[R_L10];

@@ -36,6 +36,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace avbd_oracle
@@ -73,6 +74,82 @@ inline float sqrt_(float v) { return sqrtf(v); }
 inline double sqrt_(double v) { return std::sqrt(v); }
 inline float abs_(float v) { return fabsf(v); }
 inline double abs_(double v) { return std::fabs(v); }
+inline bool isfinite_(float v) { return std::isfinite(v); }
+inline bool isfinite_(double v) { return std::isfinite(v); }
+inline bool isinf_(float v) { return std::isinf(v); }
+inline bool isinf_(double v) { return std::isinf(v); }
+
+// ---- forward-mode scalars: a value with an FP64 tangent ----
+// With an FP32 value (DualFD) the value half performs exactly the FP32 operations of Real =
+// float, so a World<DualFD> follows the FP32 trajectory bit for bit (every branch included)
+// while the tangent carries the FP64 directional derivative of that trajectory: the
+// forward-mode reference for the adjoint. With an FP64 value (DualDD) it follows the FP64
+// trajectory, the one central finite differences in FP64 see.
+template <class V>
+struct DualT
+{
+    V v;
+    double d;
+    DualT() = default;
+    constexpr DualT(float x) : v((V)x), d(0.0) {}
+    constexpr DualT(double x) : v((V)x), d(0.0) {}
+    constexpr DualT(int x) : v((V)x), d(0.0) {}
+    constexpr DualT(size_t x) : v((V)x), d(0.0) {}
+    constexpr DualT(V x, double dx, int) : v(x), d(dx) {}
+};
+using DualFD = DualT<float>;
+using DualDD = DualT<double>;
+template <class V> inline DualT<V> mk(V v, double d) { return DualT<V>(v, d, 0); }
+template <class V> inline DualT<V> operator+(DualT<V> a, DualT<V> b) { return mk<V>(a.v + b.v, a.d + b.d); }
+template <class V> inline DualT<V> operator-(DualT<V> a, DualT<V> b) { return mk<V>(a.v - b.v, a.d - b.d); }
+template <class V> inline DualT<V> operator-(DualT<V> a) { return mk<V>(-a.v, -a.d); }
+template <class V> inline DualT<V> operator*(DualT<V> a, DualT<V> b)
+{
+    return mk<V>(a.v * b.v, a.d * (double)b.v + (double)a.v * b.d);
+}
+template <class V> inline DualT<V> operator/(DualT<V> a, DualT<V> b)
+{
+    double q = (double)a.v / (double)b.v;
+    return mk<V>(a.v / b.v, (a.d - q * b.d) / (double)b.v);
+}
+template <class V> inline DualT<V> &operator+=(DualT<V> &a, DualT<V> b) { return a = a + b; }
+template <class V> inline DualT<V> &operator-=(DualT<V> &a, DualT<V> b) { return a = a - b; }
+template <class V> inline DualT<V> &operator*=(DualT<V> &a, DualT<V> b) { return a = a * b; }
+template <class V> inline DualT<V> &operator/=(DualT<V> &a, DualT<V> b) { return a = a / b; }
+template <class V> inline bool operator<(DualT<V> a, DualT<V> b) { return a.v < b.v; }
+template <class V> inline bool operator>(DualT<V> a, DualT<V> b) { return a.v > b.v; }
+template <class V> inline bool operator<=(DualT<V> a, DualT<V> b) { return a.v <= b.v; }
+template <class V> inline bool operator>=(DualT<V> a, DualT<V> b) { return a.v >= b.v; }
+template <class V> inline bool operator==(DualT<V> a, DualT<V> b) { return a.v == b.v; }
+template <class V> inline bool operator!=(DualT<V> a, DualT<V> b) { return a.v != b.v; }
+// mixed forms with plain literals (the templates above do not convert their operands)
+#define AVBD_DUAL_MIXED(OP)                                                                                \
+    template <class V, class S, class = typename std::enable_if<std::is_arithmetic<S>::value>::type>         \
+    inline auto operator OP(DualT<V> a, S b)->decltype(a OP DualT<V>(b))                                   \
+    {                                                                                                      \
+        return a OP DualT<V>(b);                                                                           \
+    }                                                                                                      \
+    template <class V, class S, class = typename std::enable_if<std::is_arithmetic<S>::value>::type>         \
+    inline auto operator OP(S a, DualT<V> b)->decltype(DualT<V>(a) OP b)                                   \
+    {                                                                                                      \
+        return DualT<V>(a) OP b;                                                                           \
+    }
+AVBD_DUAL_MIXED(+)
+AVBD_DUAL_MIXED(-)
+AVBD_DUAL_MIXED(*)
+AVBD_DUAL_MIXED(/)
+AVBD_DUAL_MIXED(<)
+AVBD_DUAL_MIXED(>)
+AVBD_DUAL_MIXED(<=)
+AVBD_DUAL_MIXED(>=)
+#undef AVBD_DUAL_MIXED
+template <class V> inline DualT<V> sqrt_(DualT<V> x)
+{
+    return mk<V>(sqrt_(x.v), x.d / (2.0 * std::sqrt((double)x.v)));
+}
+template <class V> inline DualT<V> abs_(DualT<V> x) { return x.v < V(0) ? -x : x; }
+template <class V> inline bool isfinite_(DualT<V> x) { return std::isfinite(x.v); }
+template <class V> inline bool isinf_(DualT<V> x) { return std::isinf(x.v); }
 
 // ---- vector algebra (maths.h:117-165, 317-406) ----
 template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
@@ -908,7 +985,7 @@ struct World
             V3<R> accel = (b.vel - b.prevVel) / p.dt;
             R accelExt = accel.z * signv(p.gravity);
             R w = clampv(accelExt / abs_(p.gravity), R(0), R(1));
-            if (!std::isfinite(w))
+            if (!isfinite_(w))
                 w = 0;
             b.initLin = b.pos;
             b.initAng = b.rot;
@@ -1021,7 +1098,7 @@ struct World
             V3<R> rBW = rotate(B.rot, j.rB);
             M3<R> Km = diag(j.penLin.x, j.penLin.y, j.penLin.z);
             V3<R> C = (j.a >= 0 ? bodies[j.a].pos + rAW : j.rA) - (B.pos + rBW);
-            if (std::isinf(j.kLin))
+            if (isinf_(j.kLin))
                 C -= j.C0Lin * p.alpha;
             V3<R> F = Km * C + j.lamLin;
             M3<R> jL = isA ? ident(R(1)) : ident(R(-1));
@@ -1042,7 +1119,7 @@ struct World
             M3<R> Km = diag(j.penAng.x, j.penAng.y, j.penAng.z);
             Q<R> qa = j.a >= 0 ? bodies[j.a].rot : Q<R>{0, 0, 0, 1};
             V3<R> C = (qa - B.rot) * j.torqueArm;
-            if (std::isinf(j.kAng))
+            if (isinf_(j.kAng))
                 C -= j.C0Ang * p.alpha;
             V3<R> F = Km * C + j.lamAng;
             M3<R> jA = (isA ? ident(R(1)) : ident(R(-1))) * j.torqueArm;
@@ -1060,7 +1137,7 @@ struct World
             M3<R> Km = diag(j.penLin.x, j.penLin.y, j.penLin.z);
             V3<R> pa = j.a >= 0 ? transform(bodies[j.a].pos, bodies[j.a].rot, j.rA) : j.rA;
             V3<R> C = pa - transform(B.pos, B.rot, j.rB);
-            if (std::isinf(j.kLin))
+            if (isinf_(j.kLin))
             {
                 C -= j.C0Lin * p.alpha;
                 j.lamLin = Km * C + j.lamLin;
@@ -1072,7 +1149,7 @@ struct World
             M3<R> Km = diag(j.penAng.x, j.penAng.y, j.penAng.z);
             Q<R> qa = j.a >= 0 ? bodies[j.a].rot : Q<R>{0, 0, 0, 1};
             V3<R> C = (qa - B.rot) * j.torqueArm;
-            if (std::isinf(j.kAng))
+            if (isinf_(j.kAng))
             {
                 C -= j.C0Ang * p.alpha;
                 j.lamAng = Km * C + j.lamAng;

@@ -214,11 +214,9 @@ struct WorldImpl : WorldBase
 };
 
 template <class R>
-WorldBase *build(const avbd_params *p, const avbd_body *bodies, int nb, const avbd_joint *joints, int nj,
-                 const avbd_spring *springs, int ns, const avbd_manifold *manifolds, int nm, int64_t nextSerial)
+void fill(World<R> &w, const avbd_params *p, const avbd_body *bodies, int nb, const avbd_joint *joints, int nj,
+          const avbd_spring *springs, int ns, const avbd_manifold *manifolds, int nm, int64_t nextSerial)
 {
-    auto *impl = new WorldImpl<R>();
-    World<R> &w = impl->w;
     w.p = Params<R>{(R)p->dt, (R)p->gravity, p->iterations, (R)p->alpha, (R)p->beta_lin, (R)p->beta_ang, (R)p->gamma};
     w.bodies.resize(nb);
     for (int i = 0; i < nb; ++i)
@@ -308,9 +306,65 @@ WorldBase *build(const avbd_params *p, const avbd_body *bodies, int nb, const av
     std::stable_sort(w.manifolds.begin(), w.manifolds.end(), bySerial);
     w.nextSerial = nextSerial;
     w.colors.assign(nb, -1);
+}
+
+template <class R>
+WorldBase *build(const avbd_params *p, const avbd_body *bodies, int nb, const avbd_joint *joints, int nj,
+                 const avbd_spring *springs, int ns, const avbd_manifold *manifolds, int nm, int64_t nextSerial)
+{
+    auto *impl = new WorldImpl<R>();
+    fill(impl->w, p, bodies, nb, joints, nj, springs, ns, manifolds, nm, nextSerial);
     return impl;
 }
 
+// Forward-mode reference gradient. Seeds FP64 directions on the initial linear and angular
+// velocities, the masses (each moment scales with its body's mass) and the friction
+// coefficients (any seed pointer may be NULL), steps a World<DualFD> in colour order and returns
+// per body the values and tangents of pos(3) rot(4) vel(3) angvel(3).
+template <class D>
+void tangent_run(const avbd_params *p, const avbd_body *bodies, int nb, const avbd_manifold *manifolds, int nm,
+                 int64_t nextSerial, int steps, const double *dvel, const double *dangvel, const double *dmass,
+                 const double *dfriction, double *val13, double *tan13)
+{
+    World<D> w;
+    fill(w, p, bodies, nb, nullptr, 0, nullptr, 0, manifolds, nm, nextSerial);
+    for (int i = 0; i < nb; ++i)
+    {
+        Body<D> &b = w.bodies[i];
+        for (int k = 0; k < 3; ++k)
+        {
+            if (dvel)
+                b.vel[k].d = dvel[3 * i + k];
+            if (dangvel)
+                b.angVel[k].d = dangvel[3 * i + k];
+        }
+        if (dmass && b.mass.v > 0.0f)
+        {
+            b.mass.d = dmass[i];
+            for (int k = 0; k < 3; ++k)
+                b.moment[k].d = dmass[i] * (double)b.moment[k].v / (double)b.mass.v;
+        }
+        if (dfriction)
+            b.friction.d = dfriction[i];
+    }
+    for (int s = 0; s < steps; ++s)
+        w.step();
+    for (int i = 0; i < nb; ++i)
+    {
+        const Body<D> &b = w.bodies[i];
+        const D f[13] = {b.pos.x, b.pos.y, b.pos.z, b.rot.x, b.rot.y, b.rot.z, b.rot.w,
+                              b.vel.x, b.vel.y, b.vel.z, b.angVel.x, b.angVel.y, b.angVel.z};
+        for (int k = 0; k < 13; ++k)
+        {
+            val13[13 * i + k] = f[k].v;
+            tan13[13 * i + k] = f[k].d;
+        }
+    }
+}
+
+// FP64 run with FP64 overrides of the initial velocities, angular velocities, masses (moments
+// scale with the mass) and friction coefficients (NULL keeps the table's value): the central
+// finite differences of the adjoint checks perturb these. Output per body as orc_get_bodies_f64.
 } // namespace
 
 extern "C" {
@@ -324,6 +378,48 @@ void *orc_world_create(int fp64, const avbd_params *p, const avbd_body *bodies, 
 }
 
 void orc_world_destroy(void *w) { delete (WorldBase *)w; }
+
+void orc_tangent_run(int fp64, const avbd_params *p, const avbd_body *bodies, int nb, const avbd_manifold *manifolds,
+                     int nm, int64_t nextSerial, int steps, const double *dvel, const double *dangvel,
+                     const double *dmass, const double *dfriction, double *val13, double *tan13)
+{
+    if (fp64)
+        tangent_run<DualDD>(p, bodies, nb, manifolds, nm, nextSerial, steps, dvel, dangvel, dmass, dfriction, val13,
+                            tan13);
+    else
+        tangent_run<DualFD>(p, bodies, nb, manifolds, nm, nextSerial, steps, dvel, dangvel, dmass, dfriction, val13,
+                            tan13);
+}
+
+void orc_f64_run(const avbd_params *p, const avbd_body *bodies, int nb, const avbd_manifold *manifolds, int nm,
+                 int64_t nextSerial, int steps, const double *vel, const double *angvel, const double *mass,
+                 const double *friction, double *out13)
+{
+    WorldImpl<double> impl;
+    World<double> &w = impl.w;
+    fill(w, p, bodies, nb, nullptr, 0, nullptr, 0, manifolds, nm, nextSerial);
+    for (int i = 0; i < nb; ++i)
+    {
+        Body<double> &b = w.bodies[i];
+        for (int k = 0; k < 3; ++k)
+        {
+            if (vel)
+                b.vel[k] = vel[3 * i + k];
+            if (angvel)
+                b.angVel[k] = angvel[3 * i + k];
+        }
+        if (mass && bodies[i].mass > 0.0f)
+        {
+            b.mass = mass[i];
+            for (int k = 0; k < 3; ++k)
+                b.moment[k] = mass[i] * ((double)bodies[i].moment[k] / (double)bodies[i].mass);
+        }
+        if (friction)
+            b.friction = friction[i];
+    }
+    impl.step(steps);
+    impl.getBodiesF64(out13);
+}
 void orc_set_order(void *w, int mode) { ((WorldBase *)w)->setOrder(mode); }
 void orc_step(void *w, int n) { ((WorldBase *)w)->step(n); }
 void orc_prepare(void *w) { ((WorldBase *)w)->prepare(); }
