@@ -65,6 +65,8 @@ def load(path: str | None = None):
         lib.avbd_set_launch_config.argtypes = [_p, _p]
         lib.avbd_get_launch_config.argtypes = [_p, _p]
         lib.avbd_autotune.argtypes = [_p, C.c_int32, C.c_int32, _p]
+        lib.avbd_adjoint_forward.argtypes = [_p, C.c_int32, C.c_int32]
+        lib.avbd_adjoint_backward.argtypes = [_p] + [_p] * 10
         check_layouts(lib.avbd_sizeof)
         _LIB = lib
     return _LIB
@@ -176,6 +178,26 @@ class HipWorld:
         out = np.zeros((), LAUNCH)
         _check(self._lib.avbd_autotune(self._h, trial_steps, 0 if allow_graph else 1, out.ctypes.data_as(_p)))
         return {k: int(out[k]) for k in LAUNCH.names}
+
+    def adjoint_forward(self, steps: int, checkpoint_every: int = 10):
+        """Step `steps` times keeping checkpoints for avbd_adjoint_backward."""
+        _check(self._lib.avbd_adjoint_forward(self._h, steps, checkpoint_every))
+
+    def adjoint_backward(self, d_pos=None, d_rot=None, d_vel=None, d_ang_vel=None) -> dict:
+        """Gradients w.r.t. the recorded forward's initial pose, velocities, masses (moments
+        scale with the mass) and friction coefficients, given dL/d(final state) (FP64 arrays,
+        None = 0). Returns a dict of FP64 arrays."""
+        nb = self.counts()[0]
+        def inp(x, k):
+            return None if x is None else np.ascontiguousarray(np.asarray(x, np.float64).reshape(nb, k))
+        ins = [inp(d_pos, 3), inp(d_rot, 4), inp(d_vel, 3), inp(d_ang_vel, 3)]
+        out = {"pos": np.zeros((nb, 3)), "rot": np.zeros((nb, 4)), "vel": np.zeros((nb, 3)),
+               "ang_vel": np.zeros((nb, 3)), "mass": np.zeros(nb), "friction": np.zeros(nb)}
+        ptr = lambda a: None if a is None else a.ctypes.data_as(_p)
+        _check(self._lib.avbd_adjoint_backward(self._h, *[ptr(a) for a in ins],
+                                               *[out[k].ctypes.data_as(_p) for k in
+                                                 ("pos", "rot", "vel", "ang_vel", "mass", "friction")]))
+        return out
 
     def close(self):
         if getattr(self, "_h", None):

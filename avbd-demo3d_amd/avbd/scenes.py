@@ -21,6 +21,8 @@ Workloads (gravity -9.81 along -z, the reference's up axis is +z):
   random orientations, velocities N(0, 0.5) m/s) dropping onto a ground box
   inside four static walls. 50,000 boxes is the headline workload; the
   generator takes any count.
+* ``projectile_pile``  the differentiable workload: a 1024-box block and 64
+  projectiles, with randomised masses and friction (``projectile_state``).
 * ``batch``   independent worlds (4096 by default) with body counts
   log-uniform in [8, 512]: box-stack fields, piles and chains, for the
   batched step (one handle, every launch covers all worlds).
@@ -243,6 +245,64 @@ def batch(num_worlds: int = 4096, seed: int = 4, lo: int = 8, hi: int = 512, ste
             sc = stacks(n - 1, seed=sd, steps=steps)
         worlds.append(sc)
     return worlds
+
+
+def projectile_pile(count: int = 1024, projectiles: int = 64, seed: int = 5, steps: int = 120) -> Scene:
+    """The differentiable workload: a jittered block of unit boxes resting on a ground box
+    (16 x 16 x 4 for 1024) and 0.5 m projectile boxes launched from a ring of radius 20 m so
+    that they reach the block after about one second. Creation order: ground, the block, then
+    the projectiles (masses and friction are randomised on the state, see ``projectile_state``)."""
+    rng = _rng(seed)
+    sb = _Builder()
+    side = max(1, int(math.ceil(math.sqrt(count / 4.0))))
+    layers = max(1, int(math.ceil(count / float(side * side))))
+    pitch = 1.02
+    sb.body((side * pitch + 60, side * pitch + 60, 1), 0.0, 0.5, (0, 0, 0))
+    jit = rng.uniform(-0.01, 0.01, (count, 2))
+    for k in range(count):
+        ix, iy, iz = k % side, (k // side) % side, k // (side * side)
+        x = (ix - (side - 1) / 2.0) * pitch + jit[k, 0]
+        y = (iy - (side - 1) / 2.0) * pitch + jit[k, 1]
+        sb.body((1, 1, 1), 1.0, 0.5, (x, y, 1.0 + iz))
+    ang = rng.uniform(0.0, 2.0 * math.pi, projectiles)
+    height = rng.uniform(1.0, layers + 1.0, projectiles)
+    t_hit = 1.0
+    for k in range(projectiles):
+        r = 20.0
+        px, py = r * math.cos(ang[k]), r * math.sin(ang[k])
+        pz = 3.0 + 0.5 * layers
+        tz = height[k]
+        vx, vy = -px / t_hit, -py / t_hit
+        vz = (tz - pz + 0.5 * 9.81 * t_hit * t_hit) / t_hit
+        w = rng.normal(0.0, 1.0, 3)
+        sb.body((0.5, 0.5, 0.5), 1.0, 0.5, (px, py, pz), tuple(random_quaternions(rng, 1)[0]), (vx, vy, vz),
+                tuple(w))
+    return sb.scene(f"pile{count}_proj{projectiles}", -9.81, steps)
+
+
+def projectile_state(scene: Scene, maker, seed: int = 5):
+    """State of ``projectile_pile`` with per-body masses U[0.5, 2] (moments scaled with the mass)
+    and friction coefficients U[0.3, 0.8] for the dynamic bodies. Returns (state, pile, proj,
+    target): body-list indices of the block and of the projectiles, and a target point for the
+    block's centre of mass (its start COM plus N(0, 0.5) m per axis)."""
+    st = build_state(scene, maker)
+    rng = _rng(seed + 1000)
+    n = len(st.bodies)
+    created = n - 1 - np.arange(n)                 # list index -> creation index
+    dyn = st.bodies["mass"] > 0
+    m_new = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    mu = rng.uniform(0.3, 0.8, n).astype(np.float32)
+    scale = np.where(dyn, m_new / np.where(dyn, st.bodies["mass"], 1.0), 1.0).astype(np.float32)
+    st.bodies["moment"] = (st.bodies["moment"] * scale[:, None]).astype(np.float32)
+    st.bodies["mass"] = np.where(dyn, m_new, st.bodies["mass"]).astype(np.float32)
+    st.bodies["friction"] = np.where(dyn, mu, st.bodies["friction"]).astype(np.float32)
+    n_block = int(scene.name.split("_")[0][4:])
+    pile = np.where((created >= 1) & (created <= n_block))[0]
+    proj = np.where(created > n_block)[0]
+    w = st.bodies["mass"][pile].astype(np.float64)
+    com = (st.bodies["pos"][pile].astype(np.float64) * w[:, None]).sum(0) / w.sum()
+    target = com + rng.normal(0.0, 0.5, 3)
+    return st, pile, proj, target
 
 
 def by_name(name: str) -> Scene:

@@ -1,0 +1,1307 @@
+// avbd_adjoint.hip -- hand-written adjoint (reverse mode) of the resident AVBD step for
+// contact-only box worlds. Included by avbd_capi.hip after the handle definition.
+//
+// The forward step runs in FP32 (bit-exact with the reference, solver.cpp:1036-1041); the
+// adjoint differentiates that FP32 trajectory, with every branch the step took (contact set,
+// features, friction-cone and normal clamps, penalty saturation, stick flags) held fixed, in
+// FP64 arithmetic. Pieces, in reverse order of the step:
+//   finishStep (solver.cpp:1275-1288)            v = (x - x0)/dt, w = (q - q0)/dt
+//   iterations (solver.cpp:1172-1273), reversed iteration by iteration:
+//     dual (manifold.cpp:231-280)                 lambda = F(C), k += beta |C| (unsaturated rows)
+//     primal colours in reverse (manifold.cpp:176-229, solver.cpp:1184-1240): the 6x6 solve's
+//       adjoint is one more LDL^T solve with the same (symmetric) matrix, w = H^-1 dx_bar, then
+//       r_bar = -w, H_bar = -w dx^T are pushed through every contact stamp and the inertia terms
+//   warm start (solver.cpp:1134-1169)             inertial targets, adaptive gravity weight
+//   force init (manifold.cpp:100-174)             C0 from the start poses, lambda*alpha*gamma,
+//       clamp(k*gamma); anchors kept by the stick rule pass their adjoint to the previous step,
+//       fresh anchors and the basis go through the collide pose Jacobian, evaluated by running
+//       collide_boxes<DualF> once per pose component (14 per manifold) on the step's branch.
+// Checkpoints: the state at every K-th step start is kept in HBM; a segment is replayed to
+// recover each step's start state, and each step is recomputed once more with its iterations
+// taped (poses and lambda/penalty after every iteration) right before it is reversed.
+#pragma once
+
+namespace avbd
+{
+
+// ---- FP64 vector / quaternion helpers ----
+struct DV
+{
+    double x, y, z;
+};
+struct DQ
+{
+    double x, y, z, w;
+};
+__device__ __forceinline__ DV dv(V3 v) { return DV{v.x, v.y, v.z}; }
+__device__ __forceinline__ DV dv4(float4 v) { return DV{v.x, v.y, v.z}; }
+__device__ __forceinline__ DQ dq4(float4 q) { return DQ{q.x, q.y, q.z, q.w}; }
+__device__ __forceinline__ DV operator+(DV a, DV b) { return DV{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ DV operator-(DV a, DV b) { return DV{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ DV operator-(DV a) { return DV{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ DV operator*(DV a, double s) { return DV{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ void operator+=(DV &a, DV b) { a.x += b.x; a.y += b.y; a.z += b.z; }
+__device__ __forceinline__ void operator-=(DV &a, DV b) { a.x -= b.x; a.y -= b.y; a.z -= b.z; }
+__device__ __forceinline__ double ddot(DV a, DV b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ DV dcross(DV a, DV b) { return DV{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+__device__ __forceinline__ double dcomp(DV v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+__device__ __forceinline__ DQ operator+(DQ a, DQ b) { return DQ{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; }
+__device__ __forceinline__ DQ operator*(DQ a, double s) { return DQ{a.x * s, a.y * s, a.z * s, a.w * s}; }
+__device__ __forceinline__ void operator+=(DQ &a, DQ b) { a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w; }
+__device__ __forceinline__ DQ dqmul(DQ a, DQ b)
+{
+    return DQ{a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+              a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+__device__ __forceinline__ DQ dqconj(DQ q) { return DQ{-q.x, -q.y, -q.z, q.w}; }
+__device__ __forceinline__ double dqdot(DQ a, DQ b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+__device__ __forceinline__ DV drot(DQ q, DV v)
+{
+    DV u{q.x, q.y, q.z};
+    DV t = dcross(u, v) * 2.0;
+    return v + t * q.w + dcross(u, t);
+}
+// out = rotate(q, v): accumulate q_bar, v_bar from out_bar
+__device__ __forceinline__ void drot_adj(DQ q, DV v, DV ob, DQ &qb, DV &vb)
+{
+    DV u{q.x, q.y, q.z};
+    DV t = dcross(u, v) * 2.0;
+    double wb = ddot(ob, t);
+    DV tb = ob * q.w + dcross(ob, u);
+    DV ub = dcross(t, ob) + dcross(v, tb) * 2.0;
+    vb += ob + dcross(tb, u) * 2.0;
+    qb += DQ{ub.x, ub.y, ub.z, wb};
+}
+// d = a - b = 2 vec(a b^-1), b^-1 = conj(b)/|b|^2
+__device__ __forceinline__ DV dqdiff(DQ a, DQ b)
+{
+    double s = dqdot(b, b);
+    DQ d = dqmul(a, dqconj(b) * (1.0 / s));
+    return DV{2.0 * d.x, 2.0 * d.y, 2.0 * d.z};
+}
+__device__ __forceinline__ void dqdiff_adj(DQ a, DQ b, DV db, DQ &ab, DQ &bb)
+{
+    double s = dqdot(b, b);
+    DQ inv = dqconj(b) * (1.0 / s);
+    DQ pb{2.0 * db.x, 2.0 * db.y, 2.0 * db.z, 0.0};
+    ab += dqmul(pb, dqconj(inv));
+    DQ ib = dqmul(dqconj(a), pb);
+    // inv = c o b / s with c = (-1,-1,-1,1)
+    double cib = -b.x * ib.x - b.y * ib.y - b.z * ib.z + b.w * ib.w;
+    double k2 = 2.0 * cib / (s * s);
+    bb += DQ{-ib.x / s - k2 * b.x, -ib.y / s - k2 * b.y, -ib.z / s - k2 * b.z, ib.w / s - k2 * b.w};
+}
+// q' = normalize(u), u = q + ((w,0) q) / 2: accumulate q_bar, w_bar from q'_bar
+__device__ __forceinline__ void dqint_adj(DQ q, DV w, DQ qpb, DQ &qb, DV &wb)
+{
+    DQ t = dqmul(DQ{w.x, w.y, w.z, 0.0}, q);
+    DQ u = q + t * 0.5;
+    double n = sqrt(dqdot(u, u));
+    DQ qp = u * (1.0 / n);
+    double pr = dqdot(qp, qpb);
+    DQ ub = DQ{qpb.x - qp.x * pr, qpb.y - qp.y * pr, qpb.z - qp.z * pr, qpb.w - qp.w * pr} * (1.0 / n);
+    DQ tb = ub * 0.5;
+    DQ wq = dqmul(tb, dqconj(q));
+    wb += DV{wq.x, wq.y, wq.z};
+    qb += ub + dqmul(DQ{-w.x, -w.y, -w.z, 0.0}, tb);
+}
+
+// FP64 adds into adjoint accumulators: the hardware float atomic (the buffers are ordinary
+// hipMalloc device memory), no compare-and-swap loop.
+__device__ __forceinline__ void aadd(double *p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add3(double *p, DV v)
+{
+    aadd(p, v.x);
+    aadd(p + 1, v.y);
+    aadd(p + 2, v.z);
+}
+__device__ __forceinline__ void atomic_add4(double *p, DQ v)
+{
+    aadd(p, v.x);
+    aadd(p + 1, v.y);
+    aadd(p + 2, v.z);
+    aadd(p + 3, v.w);
+}
+__device__ __forceinline__ DV ld_dv(const double *p) { return DV{p[0], p[1], p[2]}; }
+__device__ __forceinline__ DQ ld_dq(const double *p) { return DQ{p[0], p[1], p[2], p[3]}; }
+__device__ __forceinline__ void st_dv(double *p, DV v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+__device__ __forceinline__ void st_dq(double *p, DQ v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = v.w; }
+__device__ __forceinline__ void add_dv(double *p, DV v) { p[0] += v.x; p[1] += v.y; p[2] += v.z; }
+
+// Adjoint buffers (device, FP64). Bodies: pose / velocity / init pose / inertial target adjoints,
+// parameter adjoints. Contacts by slot (manifold * 8 + contact): penalty, lambda, C0, anchors.
+// Carry: contact adjoints handed to the previous step's final manifolds, by their slot.
+struct AdjBuf
+{
+    double *xb, *qb, *vb, *wb, *pvb, *ilb, *iab, *nlb, *nab, *mb, *mub;
+    double *kb, *lb, *c0b, *rAb, *rBb, *basisb, *mumb;
+    double *ckb, *clb, *crAb, *crBb;
+};
+
+// ---- one contact's force and its adjoint (manifold.cpp:176-229 / 231-280) ----
+struct CIn
+{
+    DV pA, pB, iLA, iLB, rA, rB, C0, n[3], K, lam;
+    DQ qA, qB, iAA, iAB;
+    double mu, oma;
+    bool clamp0, scaled;
+};
+struct CFwd
+{
+    DV rAW, rBW, dqAL, dqBL, dqAA, dqBA;
+    double C[3], F0[3], F[3];
+    double fs, bounds;
+};
+struct COut
+{
+    DV pA, pB, iLA, iLB, rA, rB, C0, n[3], K, lam;
+    DQ qA, qB, iAA, iAB;
+    double mu;
+};
+
+__device__ __forceinline__ void cfwd(const CIn &in, CFwd &f)
+{
+    f.rAW = drot(in.qA, in.rA);
+    f.rBW = drot(in.qB, in.rB);
+    f.dqAL = in.pA - in.iLA;
+    f.dqBL = in.pB - in.iLB;
+    f.dqAA = dqdiff(in.qA, in.iAA);
+    f.dqBA = dqdiff(in.qB, in.iAB);
+    for (int k = 0; k < 3; ++k)
+    {
+        DV n = in.n[k];
+        f.C[k] = dcomp(in.C0, k) * in.oma + ddot(n, f.dqAL) - ddot(n, f.dqBL) + ddot(dcross(f.rAW, n), f.dqAA) -
+                 ddot(dcross(f.rBW, n), f.dqBA);
+        f.F0[k] = dcomp(in.K, k) * f.C[k] + dcomp(in.lam, k);
+    }
+    f.F[0] = in.clamp0 ? 0.0 : f.F0[0];
+    f.bounds = fabs(f.F[0]) * in.mu;
+    f.fs = sqrt(f.F0[1] * f.F0[1] + f.F0[2] * f.F0[2]);
+    double sc = in.scaled ? f.bounds / f.fs : 1.0;
+    f.F[1] = f.F0[1] * sc;
+    f.F[2] = f.F0[2] * sc;
+}
+
+// Given F_bar, an extra C_bar and pre-accumulated rAW_bar / rBW_bar / n_bar (Jacobian terms of a
+// stamp), accumulate every input adjoint into o.
+__device__ __forceinline__ void cadj(const CIn &in, const CFwd &f, const double Fb[3], const double Cx[3], DV rAWb,
+                                     DV rBWb, COut &o)
+{
+    double F0b[3];
+    F0b[0] = Fb[0];
+    F0b[1] = Fb[1];
+    F0b[2] = Fb[2];
+    if (in.scaled)
+    {
+        double s = f.bounds / f.fs;
+        double ux = f.F0[1] / f.fs, uy = f.F0[2] / f.fs;
+        double bb = Fb[1] * ux + Fb[2] * uy;
+        F0b[1] = s * (Fb[1] - ux * bb);
+        F0b[2] = s * (Fb[2] - uy * bb);
+        // bounds = |F_0| mu
+        double sg = f.F[0] < 0.0 ? -1.0 : (f.F[0] > 0.0 ? 1.0 : 0.0);
+        F0b[0] += bb * in.mu * sg;
+        o.mu += bb * fabs(f.F[0]);
+    }
+    if (in.clamp0)
+        F0b[0] = 0.0;
+    double Cb[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        Cb[k] = dcomp(in.K, k) * F0b[k] + Cx[k];
+        double kb = f.C[k] * F0b[k];
+        if (k == 0) { o.K.x += kb; o.lam.x += F0b[0]; o.C0.x += in.oma * Cb[0]; }
+        if (k == 1) { o.K.y += kb; o.lam.y += F0b[1]; o.C0.y += in.oma * Cb[1]; }
+        if (k == 2) { o.K.z += kb; o.lam.z += F0b[2]; o.C0.z += in.oma * Cb[2]; }
+    }
+    DV dALb{0, 0, 0}, dAAb{0, 0, 0}, dBAb{0, 0, 0};
+    for (int k = 0; k < 3; ++k)
+    {
+        DV n = in.n[k];
+        dALb += n * Cb[k];
+        o.n[k] += (f.dqAL - f.dqBL) * Cb[k] + dcross(f.dqAA, f.rAW) * Cb[k] - dcross(f.dqBA, f.rBW) * Cb[k];
+        rAWb += dcross(n, f.dqAA) * Cb[k];
+        rBWb -= dcross(n, f.dqBA) * Cb[k];
+        dAAb += dcross(f.rAW, n) * Cb[k];
+        dBAb -= dcross(f.rBW, n) * Cb[k];
+    }
+    o.pA += dALb;
+    o.iLA -= dALb;
+    o.pB -= dALb;
+    o.iLB += dALb;
+    dqdiff_adj(in.qA, in.iAA, dAAb, o.qA, o.iAA);
+    dqdiff_adj(in.qB, in.iAB, dBAb, o.qB, o.iAB);
+    drot_adj(in.qA, in.rA, rAWb, o.qA, o.rA);
+    drot_adj(in.qB, in.rB, rBWb, o.qB, o.rB);
+}
+
+__device__ __forceinline__ void cout_zero(COut &o)
+{
+    DV z{0, 0, 0};
+    DQ zq{0, 0, 0, 0};
+    o.pA = o.pB = o.iLA = o.iLB = o.rA = o.rB = o.C0 = o.K = o.lam = z;
+    o.n[0] = o.n[1] = o.n[2] = z;
+    o.qA = o.qB = o.iAA = o.iAB = zq;
+    o.mu = 0.0;
+}
+
+// Loads one contact as seen at a point of the step: poses given, penalty/lambda from the tape.
+__device__ __forceinline__ void cin_load(CIn &in, const float4 *cr, float4 pen, float4 lam, float4 pA, float4 qA,
+                                         float4 pB, float4 qB, float4 iLA, float4 iAA, float4 iLB, float4 iAB,
+                                         float oma)
+{
+    in.pA = dv4(pA);
+    in.qA = dq4(qA);
+    in.pB = dv4(pB);
+    in.qB = dq4(qB);
+    in.iLA = dv4(iLA);
+    in.iAA = dq4(iAA);
+    in.iLB = dv4(iLB);
+    in.iAB = dq4(iAB);
+    in.rA = dv4(cr[CR_RA]);
+    in.rB = dv4(cr[CR_RB]);
+    in.C0 = dv4(cr[CR_C0]);
+    in.n[0] = dv4(cr[CR_N]);
+    in.n[1] = dv4(cr[CR_T1]);
+    in.n[2] = dv4(cr[CR_T2]);
+    in.K = dv4(pen);
+    in.lam = dv4(lam);
+    in.mu = cr[CR_RA].w;
+    in.oma = (double)oma;
+}
+
+// FP32 evaluation of the contact exactly as the step did it: branch flags and the stamp record.
+__device__ __forceinline__ ContactEval c_eval32(const float4 *cr, float4 pen4, float4 lam4, float4 pA4, float4 qA4,
+                                               float4 pB4, float4 qB4, float4 iA4, float4 jA4, float4 iB4,
+                                               float4 jB4, float oma, V3 &b0, V3 &b1, V3 &b2, V3 &ja0, V3 &ja1,
+                                               V3 &ja2, V3 &jb0, V3 &jb1, V3 &jb2)
+{
+    Q4 qA = ldq(qA4), qB = ldq(qB4);
+    V3 dqAL = ld3(pA4) - ld3(iA4);
+    V3 dqAA = qdiff(qA, ldq(jA4));
+    V3 dqBL = ld3(pB4) - ld3(iB4);
+    V3 dqBA = qdiff(qB, ldq(jB4));
+    b0 = ld3(cr[CR_N]);
+    b1 = ld3(cr[CR_T1]);
+    b2 = ld3(cr[CR_T2]);
+    V3 nb0 = -b0, nb1 = -b1, nb2 = -b2;
+    V3 rAW = rotate(qA, ld3(cr[CR_RA]));
+    V3 rBW = rotate(qB, ld3(cr[CR_RB]));
+    ja0 = cross(rAW, b0);
+    ja1 = cross(rAW, b1);
+    ja2 = cross(rAW, b2);
+    jb0 = cross(rBW, nb0);
+    jb1 = cross(rBW, nb1);
+    jb2 = cross(rBW, nb2);
+    return contact_eval(ld3(cr[CR_C0]), ld3(pen4), ld3(lam4), cr[CR_RA].w, oma, b0, b1, b2, nb0, nb1, nb2, ja0, ja1,
+                        ja2, jb0, jb1, jb2, dqAL, dqBL, dqAA, dqBA);
+}
+
+__device__ __forceinline__ bool eval_clamp0(const ContactEval &ev, float4 pen, float4 lam)
+{
+    // F0 = min(k C0 + lambda, 0): clamped when the unclamped value is not negative
+    float f = pen.x * ev.C0 + lam.x;
+    return !(f < 0.0f);
+}
+__device__ __forceinline__ bool eval_scaled(const ContactEval &ev) { return ev.fs > ev.bounds && ev.fs > 0.0f; }
+
+// ---- 6x6 symmetric solve in FP64 (the LDL^T of maths.h:481-547, unpivoted) ----
+__device__ __forceinline__ void solve6d(double H[6][6], const double b[6], double x[6])
+{
+    double L[6][6], D[6];
+    for (int i = 0; i < 6; ++i)
+    {
+        for (int j = 0; j < i; ++j)
+        {
+            double s = H[i][j];
+            for (int k = 0; k < j; ++k)
+                s -= L[i][k] * L[j][k] * D[k];
+            L[i][j] = s / D[j];
+        }
+        double d = H[i][i];
+        for (int k = 0; k < i; ++k)
+            d -= L[i][k] * L[i][k] * D[k];
+        D[i] = d;
+    }
+    double y[6];
+    for (int i = 0; i < 6; ++i)
+    {
+        double s = b[i];
+        for (int k = 0; k < i; ++k)
+            s -= L[i][k] * y[k];
+        y[i] = s;
+    }
+    for (int i = 5; i >= 0; --i)
+    {
+        double s = y[i] / D[i];
+        for (int k = i + 1; k < 6; ++k)
+            s -= L[k][i] * x[k];
+        x[i] = s;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// reverse kernels
+// ------------------------------------------------------------------------------------------
+
+// Start of a step's reversal: contact adjoints of the step's final contacts come from the carry
+// (written by the reversal of the next step); everything per step is cleared.
+__global__ void k_adj_begin(const int4 *contactRec, int nc, int nm, int nb, AdjBuf A)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nc)
+    {
+        int4 r = contactRec[t];
+        int sl = r.x * kMaxContacts + r.y;
+        for (int k = 0; k < 3; ++k)
+        {
+            A.kb[3 * sl + k] = A.ckb[3 * sl + k];
+            A.lb[3 * sl + k] = A.clb[3 * sl + k];
+            A.rAb[3 * sl + k] = A.crAb[3 * sl + k];
+            A.rBb[3 * sl + k] = A.crBb[3 * sl + k];
+            A.c0b[3 * sl + k] = 0.0;
+        }
+    }
+    if (t < nm)
+    {
+        for (int k = 0; k < 9; ++k)
+            A.basisb[9 * t + k] = 0.0;
+        A.mumb[t] = 0.0;
+    }
+    if (t < nb)
+    {
+        for (int k = 0; k < 3; ++k)
+        {
+            A.ilb[3 * t + k] = 0.0;
+            A.nlb[3 * t + k] = 0.0;
+        }
+        for (int k = 0; k < 4; ++k)
+        {
+            A.iab[4 * t + k] = 0.0;
+            A.nab[4 * t + k] = 0.0;
+        }
+    }
+}
+
+__global__ void k_adj_clear_carry(int n3, AdjBuf A)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n3)
+    {
+        A.ckb[t] = 0.0;
+        A.clb[t] = 0.0;
+        A.crAb[t] = 0.0;
+        A.crBb[t] = 0.0;
+    }
+}
+
+// finishStep (solver.cpp:1275-1288): prevVel' = v, and for dynamic bodies v' = (x - x0)/dt,
+// w' = (q - q0)/dt.
+__global__ void k_adj_finish(Bodies B, Params P, int n, AdjBuf A)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    DV vb = ld_dv(A.vb + 3 * i), wb = ld_dv(A.wb + 3 * i), pvb = ld_dv(A.pvb + 3 * i);
+    if (B.massMoment[i].x > 0.0f)
+    {
+        double idt = 1.0 / (double)P.dt;
+        add_dv(A.xb + 3 * i, vb * idt);
+        add_dv(A.ilb + 3 * i, vb * (-idt));
+        DQ qb = ld_dq(A.qb + 4 * i), iab = ld_dq(A.iab + 4 * i);
+        dqdiff_adj(dq4(B.pose[4 * i + P_ROT]), dq4(B.pose[4 * i + P_IANG]), wb * idt, qb, iab);
+        st_dq(A.qb + 4 * i, qb);
+        st_dq(A.iab + 4 * i, iab);
+        st_dv(A.vb + 3 * i, pvb);
+        st_dv(A.wb + 3 * i, DV{0, 0, 0});
+    }
+    else
+        st_dv(A.vb + 3 * i, vb + pvb);
+    st_dv(A.pvb + 3 * i, DV{0, 0, 0});
+}
+
+// Pushes a contact's input adjoints out: bodies and init poses (atomic), the contact's own slot
+// (exclusive) and its manifold (atomic unless `exclusive`).
+__device__ __forceinline__ void cout_store(const COut &o, int a, int b, int sl, int m, bool massA, bool massB,
+                                           bool exclusive, AdjBuf &A)
+{
+    if (massA)
+    {
+        atomic_add3(A.xb + 3 * a, o.pA);
+        atomic_add4(A.qb + 4 * a, o.qA);
+        atomic_add3(A.ilb + 3 * a, o.iLA);
+        atomic_add4(A.iab + 4 * a, o.iAA);
+    }
+    if (massB)
+    {
+        atomic_add3(A.xb + 3 * b, o.pB);
+        atomic_add4(A.qb + 4 * b, o.qB);
+        atomic_add3(A.ilb + 3 * b, o.iLB);
+        atomic_add4(A.iab + 4 * b, o.iAB);
+    }
+    add_dv(A.c0b + 3 * sl, o.C0);
+    add_dv(A.rAb + 3 * sl, o.rA);
+    add_dv(A.rBb + 3 * sl, o.rB);
+    if (exclusive)
+    {
+        for (int k = 0; k < 3; ++k)
+            add_dv(A.basisb + 9 * m + 3 * k, o.n[k]);
+        A.mumb[m] += o.mu;
+    }
+    else
+    {
+        for (int k = 0; k < 3; ++k)
+            atomic_add3(A.basisb + 9 * m + 3 * k, o.n[k]);
+        aadd(A.mumb + m, o.mu);
+    }
+}
+
+// Dual update of one contact, reversed (manifold.cpp:231-280). Poses after this iteration's
+// primal (tapeCur), penalty/lambda before the dual (tapeLamPrev).
+__global__ void k_adj_dual(Bodies B, Manifolds M, const int4 *contactRec, int nc, const float4 *tapeCur,
+                           const float4 *tapeLamPrev, Params P, AdjBuf A)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc)
+        return;
+    int4 r = contactRec[t];
+    int m = r.x, sl = r.x * kMaxContacts + r.y, a = r.z, b = r.w;
+    const float4 *cr = M.cr + (size_t)sl * kCR;
+    float4 pen4 = tapeLamPrev[2 * sl], lam4 = tapeLamPrev[2 * sl + 1];
+    float4 pA = tapeCur[2 * a], qA = tapeCur[2 * a + 1], pB = tapeCur[2 * b], qB = tapeCur[2 * b + 1];
+    float4 iA = B.pose[4 * a + P_ILIN], jA = B.pose[4 * a + P_IANG], iB = B.pose[4 * b + P_ILIN],
+           jB = B.pose[4 * b + P_IANG];
+    float oma = 1.0f - P.alpha;
+    V3 b0, b1, b2, ja0, ja1, ja2, jb0, jb1, jb2;
+    ContactEval ev = c_eval32(cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma, b0, b1, b2, ja0, ja1, ja2, jb0,
+                              jb1, jb2);
+    CIn in;
+    cin_load(in, cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma);
+    in.clamp0 = eval_clamp0(ev, pen4, lam4);
+    in.scaled = eval_scaled(ev);
+    CFwd f;
+    cfwd(in, f);
+    // adjoints of the updated lambda and penalty
+    DV lbn = ld_dv(A.lb + 3 * sl), kbn = ld_dv(A.kb + 3 * sl);
+    double Fb[3] = {lbn.x, lbn.y, lbn.z};
+    double Cx[3] = {0.0, 0.0, 0.0};
+    double kold[3] = {kbn.x, kbn.y, kbn.z};
+    const float cf[3] = {ev.C0, ev.C1, ev.C2};
+    const float pk[3] = {pen4.x, pen4.y, pen4.z};
+    const double beta = P.betaLin;
+    for (int k = 0; k < 3; ++k)
+    {
+        bool upd = k == 0 ? ev.F0 < 0.0f : ev.fs <= ev.bounds;
+        if (!upd)
+            continue;
+        bool sat = !(pk[k] + P.betaLin * fabsf(cf[k]) < kPenaltyMax);
+        double kb = k == 0 ? kbn.x : (k == 1 ? kbn.y : kbn.z);
+        if (sat)
+            kold[k] = 0.0;
+        else
+        {
+            double sg = f.C[k] < 0.0 ? -1.0 : (f.C[k] > 0.0 ? 1.0 : 0.0);
+            Cx[k] = kb * beta * sg;
+        }
+    }
+    COut o;
+    cout_zero(o);
+    cadj(in, f, Fb, Cx, DV{0, 0, 0}, DV{0, 0, 0}, o);
+    // the dual overwrote lambda and penalty: their old adjoints are what cadj produced, plus the
+    // penalty carried through the update
+    st_dv(A.lb + 3 * sl, o.lam);
+    st_dv(A.kb + 3 * sl, DV{kold[0], kold[1], kold[2]} + o.K);
+    o.lam = o.K = DV{0, 0, 0};
+    cout_store(o, a, b, sl, m, B.massMoment[a].x > 0.0f, B.massMoment[b].x > 0.0f, false, A);
+}
+
+// Primal update of the bodies of colour `col`, reversed (solver.cpp:1184-1240,
+// manifold.cpp:176-229); kAdjLanes lanes per body. tapePrev: poses before this iteration,
+// tapeCur: after it (lower colours already updated); tapeLamPrev: penalty/lambda the iteration's
+// primal read. Phase 1 rebuilds the body's FP32 system exactly as the step did (records of one
+// round in LDS rows, each entry added by its owner lane in stamp order) so the solve gives the
+// step's own dx; lane 0 then solves H w = dx_bar in FP64. Phase 2: every lane reverses its own
+// stamps; own-pose adjoints are summed over the lanes, partner ones go out atomically.
+constexpr int kAdjLanes = 16;
+constexpr int kAdjThreads = 64;
+
+struct AdjBodyShared
+{
+    double G[27];      // G_LL (9), G_AA (9), G_AL (9)
+    double rLb[3], rAb[3];
+};
+
+__device__ __forceinline__ void wave_sum_group(double &v)
+{
+#pragma unroll
+    for (int off = 1; off < kAdjLanes; off <<= 1)
+        v += __shfl_xor(v, off, kAdjLanes);
+}
+
+__global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds M, const int4 *stamps,
+                                                            const int4 *tasks, int begin, int end, const int *color,
+                                                            int col, const float4 *tapePrev, const float4 *tapeCur,
+                                                            const float4 *tapeLamPrev, Params P, AdjBuf A)
+{
+    constexpr int L = kAdjLanes, G = kAdjThreads / kAdjLanes, E = (kRec + L - 1) / L, kRow = kRec + 2;
+    __shared__ float rec[kAdjThreads * kRow];
+    __shared__ AdjBodyShared sh[G];
+    const int lane = threadIdx.x % L, g = threadIdx.x / L;
+    const int k = begin + blockIdx.x * G + g;
+    const bool valid = k < end;
+    const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
+    const int i = task.x, s0 = task.y, ns = valid ? task.z : 0;
+    int rounds = (ns + L - 1) / L;
+#pragma unroll
+    for (int off = 32; off >= L; off >>= 1)
+        rounds = max(rounds, __shfl_xor(rounds, off, 64));
+    auto seen = [&](int j) -> const float4 * {
+        int cj = color[j];
+        return (j != i && cj >= 0 && cj < col) ? tapeCur : tapePrev;
+    };
+    const float oma = 1.0f - P.alpha;
+    float4 mm = B.massMoment[i];
+    float4 pos4 = tapePrev[2 * i], rot4 = tapePrev[2 * i + 1];
+    V3 pos = ld3(pos4);
+    Q4 rot = ldq(rot4);
+    float dt2 = P.dt * P.dt;
+    float ml = mm.x / dt2, ax = mm.y / dt2, ay = mm.z / dt2, az = mm.w / dt2;
+    V3 dl = pos - ld3(B.inerLin[i]);
+    V3 da = qdiff(rot, ldq(B.inerAng[i]));
+    float acc[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        acc[e] = lane + e * L < kRec ? init_entry(lane + e * L, ml, ax, ay, az, dl, da) : 0.0f;
+    // ---- phase 1: the step's FP32 system ----
+    for (int r = 0; r < rounds; ++r)
+    {
+        int t = r * L + lane;
+        float *row = rec + (g * L + lane) * kRow;
+        if (valid && t < ns)
+        {
+            int4 st = stamps[s0 + t];
+            bool isA = (st.y >> 16) != 0;
+            int partner = st.z, sl = st.w;
+            int a = isA ? i : partner, b = isA ? partner : i;
+            const float4 *cr = M.cr + (size_t)sl * kCR;
+            const float4 *sa = a == i ? tapePrev : seen(a), *sb = b == i ? tapePrev : seen(b);
+            V3 b0, b1, b2, ja0, ja1, ja2, jb0, jb1, jb2;
+            float4 pen4 = tapeLamPrev[2 * sl];
+            ContactEval ev = c_eval32(cr, pen4, tapeLamPrev[2 * sl + 1], sa[2 * a], sa[2 * a + 1], sb[2 * b],
+                                      sb[2 * b + 1], B.pose[4 * a + P_ILIN], B.pose[4 * a + P_IANG],
+                                      B.pose[4 * b + P_ILIN], B.pose[4 * b + P_IANG], oma, b0, b1, b2, ja0, ja1, ja2,
+                                      jb0, jb1, jb2);
+            if (isA)
+                rows_record(row, b0, b1, b2, ja0, ja1, ja2, ld3(pen4), ev.F0, ev.F1, ev.F2);
+            else
+                rows_record(row, -b0, -b1, -b2, jb0, jb1, jb2, ld3(pen4), ev.F0, ev.F1, ev.F2);
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < kRec; ++q)
+                row[q] = 0.0f;
+        }
+        wave_lds_sync();
+        int cnt = valid ? min(L, ns - r * L) : 0;
+        for (int q = 0; q < cnt; ++q)
+        {
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if (lane + e * L < kRec)
+                    acc[e] += rec[(g * L + q) * kRow + lane + e * L];
+        }
+        wave_lds_sync();
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        if (lane + e * L < kRec)
+            rec[g * L * kRow + lane + e * L] = acc[e];
+    wave_lds_sync();
+    // ---- lane 0: the step's dx, the FP64 solve of H w = dx_bar, inertia terms ----
+    DV xb{0, 0, 0};
+    DQ qb{0, 0, 0, 0};
+    if (valid && lane == 0)
+    {
+        const float *a = rec + g * L * kRow;
+        Sys6 sys;
+        sys.l00 = a[R_L00];
+        sys.l10 = a[R_L10];
+        sys.l11 = a[R_L11];
+        sys.l20 = a[R_L20];
+        sys.l21 = a[R_L21];
+        sys.l22 = a[R_L22];
+        sys.a00 = a[R_A00];
+        sys.a10 = a[R_A10];
+        sys.a11 = a[R_A11];
+        sys.a20 = a[R_A20];
+        sys.a21 = a[R_A21];
+        sys.a22 = a[R_A22];
+        sys.c00 = a[R_C + 0];
+        sys.c01 = a[R_C + 1];
+        sys.c02 = a[R_C + 2];
+        sys.c10 = a[R_C + 3];
+        sys.c11 = a[R_C + 4];
+        sys.c12 = a[R_C + 5];
+        sys.c20 = a[R_C + 6];
+        sys.c21 = a[R_C + 7];
+        sys.c22 = a[R_C + 8];
+        V3 rl{a[R_RL], a[R_RL + 1], a[R_RL + 2]}, ra{a[R_RA], a[R_RA + 1], a[R_RA + 2]};
+        V3 dxL, dxA;
+        solve6(sys, -rl, -ra, dxL, dxA);
+        // x' = x + dxL, q' = q (+) dxA
+        DV xbo = ld_dv(A.xb + 3 * i);
+        DQ qbo = ld_dq(A.qb + 4 * i);
+        xb = xbo;
+        DV dxAb{0, 0, 0};
+        DQ q = dq4(rot4);
+        dqint_adj(q, dv(dxA), qbo, qb, dxAb);
+        double H[6][6];
+        const int lo[3][3] = {{R_L00, R_L10, R_L20}, {R_L10, R_L11, R_L21}, {R_L20, R_L21, R_L22}};
+        const int an[3][3] = {{R_A00, R_A10, R_A20}, {R_A10, R_A11, R_A21}, {R_A20, R_A21, R_A22}};
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+            {
+                H[r][c] = a[lo[r][c]];
+                H[3 + r][3 + c] = a[an[r][c]];
+                H[3 + r][c] = a[R_C + 3 * r + c];
+                H[c][3 + r] = a[R_C + 3 * r + c];
+            }
+        double g6[6] = {xbo.x, xbo.y, xbo.z, dxAb.x, dxAb.y, dxAb.z}, w6[6];
+        solve6d(H, g6, w6);
+        const double dx6[6] = {dxL.x, dxL.y, dxL.z, dxA.x, dxA.y, dxA.z};
+        AdjBodyShared &S = sh[g];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+            {
+                S.G[3 * r + c] = -(w6[r] * dx6[c] + w6[c] * dx6[r]);
+                S.G[9 + 3 * r + c] = -(w6[3 + r] * dx6[3 + c] + w6[3 + c] * dx6[3 + r]);
+                S.G[18 + 3 * r + c] = -(w6[3 + r] * dx6[c] + w6[c] * dx6[3 + r]);
+            }
+        DV rLb{-w6[0], -w6[1], -w6[2]}, rAb{-w6[3], -w6[4], -w6[5]};
+        S.rLb[0] = rLb.x;
+        S.rLb[1] = rLb.y;
+        S.rLb[2] = rLb.z;
+        S.rAb[0] = rAb.x;
+        S.rAb[1] = rAb.y;
+        S.rAb[2] = rAb.z;
+        // inertia: H_LL += m/dt^2 I, H_AA += diag(I)/dt^2, r_L += m/dt^2 (x - xt), r_A += I/dt^2 o (q - qt)
+        double idt2 = 1.0 / ((double)P.dt * (double)P.dt);
+        DV dld = dv(pos) - dv4(B.inerLin[i]);
+        DQ inA = dq4(B.inerAng[i]);
+        DV dad = dqdiff(q, inA);
+        double mbar = idt2 * (-(w6[0] * dx6[0] + w6[1] * dx6[1] + w6[2] * dx6[2]) + ddot(rLb, dld));
+        double Ib[3];
+        for (int c = 0; c < 3; ++c)
+            Ib[c] = idt2 * (-(w6[3 + c] * dx6[3 + c]) + dcomp(rAb, c) * dcomp(dad, c));
+        double mlD = (double)mm.x * idt2;
+        xb += rLb * mlD;
+        add_dv(A.nlb + 3 * i, rLb * (-mlD));
+        DV dab{dcomp(rAb, 0) * (double)mm.y * idt2, dcomp(rAb, 1) * (double)mm.z * idt2,
+               dcomp(rAb, 2) * (double)mm.w * idt2};
+        DQ nab{0, 0, 0, 0};
+        dqdiff_adj(q, inA, dab, qb, nab);
+        DQ old = ld_dq(A.nab + 4 * i);
+        st_dq(A.nab + 4 * i, old + nab);
+        if (mm.x > 0.0f)
+            mbar += (Ib[0] * (double)mm.y + Ib[1] * (double)mm.z + Ib[2] * (double)mm.w) / (double)mm.x;
+        A.mb[i] += mbar;
+    }
+    wave_lds_sync();
+    // ---- phase 2: every lane reverses its own stamps ----
+    const AdjBodyShared &S = sh[g];
+    DV rLb{S.rLb[0], S.rLb[1], S.rLb[2]}, rAb{S.rAb[0], S.rAb[1], S.rAb[2]};
+    DV oxb{0, 0, 0};
+    DQ oqb{0, 0, 0, 0};
+    for (int t = lane; valid && t < ns; t += L)
+    {
+        int4 st = stamps[s0 + t];
+        bool isA = (st.y >> 16) != 0;
+        int partner = st.z, sl = st.w, m = adjIndex(st.x);
+        int a = isA ? i : partner, b = isA ? partner : i;
+        const float4 *cr = M.cr + (size_t)sl * kCR;
+        const float4 *sa = a == i ? tapePrev : seen(a), *sb = b == i ? tapePrev : seen(b);
+        float4 pen4 = tapeLamPrev[2 * sl], lam4 = tapeLamPrev[2 * sl + 1];
+        float4 pA = sa[2 * a], qA = sa[2 * a + 1], pB = sb[2 * b], qB = sb[2 * b + 1];
+        float4 iA = B.pose[4 * a + P_ILIN], jA = B.pose[4 * a + P_IANG], iB = B.pose[4 * b + P_ILIN],
+               jB = B.pose[4 * b + P_IANG];
+        V3 b0, b1, b2, ja0, ja1, ja2, jb0, jb1, jb2;
+        ContactEval ev = c_eval32(cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma, b0, b1, b2, ja0, ja1, ja2,
+                                  jb0, jb1, jb2);
+        CIn in;
+        cin_load(in, cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma);
+        in.clamp0 = eval_clamp0(ev, pen4, lam4);
+        in.scaled = eval_scaled(ev);
+        CFwd f;
+        cfwd(in, f);
+        const double sgn = isA ? 1.0 : -1.0;
+        DV rW = isA ? f.rAW : f.rBW;
+        COut o;
+        cout_zero(o);
+        double Fb[3];
+        DV rWb{0, 0, 0};
+        for (int kk = 0; kk < 3; ++kk)
+        {
+            DV n = in.n[kk];
+            DV av = dcross(rW, n) * sgn;
+            double Kk = dcomp(in.K, kk);
+            const double *GLL = S.G, *GAA = S.G + 9, *GAL = S.G + 18;
+            DV GLLn{GLL[0] * n.x + GLL[1] * n.y + GLL[2] * n.z, GLL[3] * n.x + GLL[4] * n.y + GLL[5] * n.z,
+                    GLL[6] * n.x + GLL[7] * n.y + GLL[8] * n.z};
+            DV GAAa{GAA[0] * av.x + GAA[1] * av.y + GAA[2] * av.z, GAA[3] * av.x + GAA[4] * av.y + GAA[5] * av.z,
+                    GAA[6] * av.x + GAA[7] * av.y + GAA[8] * av.z};
+            DV GALn{GAL[0] * n.x + GAL[1] * n.y + GAL[2] * n.z, GAL[3] * n.x + GAL[4] * n.y + GAL[5] * n.z,
+                    GAL[6] * n.x + GAL[7] * n.y + GAL[8] * n.z};
+            DV GALta{GAL[0] * av.x + GAL[3] * av.y + GAL[6] * av.z, GAL[1] * av.x + GAL[4] * av.y + GAL[7] * av.z,
+                     GAL[2] * av.x + GAL[5] * av.y + GAL[8] * av.z};
+            // K adjoint: n^T Hb_LL n + a^T Hb_AA a + s a^T G_AL n (each Hb quadratic form is half its G form)
+            double kb = 0.5 * ddot(n, GLLn) + 0.5 * ddot(av, GAAa) + sgn * ddot(av, GALn);
+            if (kk == 0) o.K.x += kb;
+            if (kk == 1) o.K.y += kb;
+            if (kk == 2) o.K.z += kb;
+            Fb[kk] = sgn * ddot(n, rLb) + ddot(av, rAb);
+            DV nb = rLb * (f.F[kk] * sgn) + GLLn * Kk + GALta * (Kk * sgn);
+            DV ab = rAb * f.F[kk] + GAAa * Kk + GALn * (Kk * sgn);
+            // a = s (rW x n)
+            rWb += dcross(n, ab) * sgn;
+            nb += dcross(ab, rW) * sgn;
+            o.n[kk] += nb;
+        }
+        double Cx[3] = {0.0, 0.0, 0.0};
+        DV rAWb = isA ? rWb : DV{0, 0, 0}, rBWb = isA ? DV{0, 0, 0} : rWb;
+        cadj(in, f, Fb, Cx, rAWb, rBWb, o);
+        if (isA)
+        {
+            oxb += o.pA;
+            oqb += o.qA;
+            o.pA = DV{0, 0, 0};
+            o.qA = DQ{0, 0, 0, 0};
+        }
+        else
+        {
+            oxb += o.pB;
+            oqb += o.qB;
+            o.pB = DV{0, 0, 0};
+            o.qB = DQ{0, 0, 0, 0};
+        }
+        add_dv(A.kb + 3 * sl, o.K);
+        add_dv(A.lb + 3 * sl, o.lam);
+        o.K = o.lam = DV{0, 0, 0};
+        bool massA = B.massMoment[a].x > 0.0f, massB = B.massMoment[b].x > 0.0f;
+        // contacts of one manifold can sit on different lanes of this body: manifold sums are atomic
+        cout_store(o, a, b, sl, m, massA && a != i, massB && b != i, false, A);
+        if (isA)
+        {
+            atomic_add3(A.ilb + 3 * a, o.iLA);
+            atomic_add4(A.iab + 4 * a, o.iAA);
+        }
+        else
+        {
+            atomic_add3(A.ilb + 3 * b, o.iLB);
+            atomic_add4(A.iab + 4 * b, o.iAB);
+        }
+    }
+    wave_sum_group(oxb.x);
+    wave_sum_group(oxb.y);
+    wave_sum_group(oxb.z);
+    wave_sum_group(oqb.x);
+    wave_sum_group(oqb.y);
+    wave_sum_group(oqb.z);
+    wave_sum_group(oqb.w);
+    if (valid && lane == 0)
+    {
+        st_dv(A.xb + 3 * i, xb + oxb);
+        st_dq(A.qb + 4 * i, qb + oqb);
+    }
+}
+
+// Warm start (solver.cpp:1134-1169) reversed: the pose adjoints now refer to the step's start.
+__global__ void k_adj_warmstart(Bodies B, Params P, const int *deg, int n, AdjBuf A)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float mass = B.massMoment[i].x;
+    DV xwb = ld_dv(A.xb + 3 * i);        // adjoint of the warm-started pose
+    DQ qwb = ld_dq(A.qb + 4 * i);
+    DV nlb = ld_dv(A.nlb + 3 * i), ilb = ld_dv(A.ilb + 3 * i);
+    DQ nab = ld_dq(A.nab + 4 * i), iab = ld_dq(A.iab + 4 * i);
+    DV vb = ld_dv(A.vb + 3 * i), wb = ld_dv(A.wb + 3 * i), pvb = ld_dv(A.pvb + 3 * i);
+    double dt = P.dt;
+    DQ q = dq4(B.pose[4 * i + P_IANG]);  // the start pose (init = start)
+    DV w = dv4(B.angVel[i]);
+    DV xb = ilb, qbz{0, 0, 0};
+    DQ qb = iab;
+    bool solved = mass > 0.0f && deg[i] > 0;
+    if (mass > 0.0f && !solved)
+    {
+        // x' = inertial target, q' = inertial rotation target
+        nlb += xwb;
+        nab += qwb;
+        xwb = DV{0, 0, 0};
+        qwb = DQ{0, 0, 0, 0};
+    }
+    DQ qintb = nab;  // q (+) w dt feeds both the inertial target and (dynamic) the warm-started rotation
+    if (mass > 0.0f)
+    {
+        // x' = x + v dt + (0,0,g) aw dt^2
+        xb += xwb;
+        vb += xwb * dt;
+        qintb += qwb;
+        float4 v4 = B.vel[i], pv4 = B.prevVel[i];
+        float accelExt = ((v4.z - pv4.z) / P.dt) * signr(P.gravity);
+        float awr = accelExt / fabsf(P.gravity);
+        if (awr > 0.0f && awr < 1.0f && isfinite(awr))
+        {
+            double dawdvz = (double)signr(P.gravity) / ((double)P.dt * fabs((double)P.gravity));
+            double awb = xwb.z * (double)P.gravity * dt * dt;
+            vb.z += awb * dawdvz;
+            pvb.z -= awb * dawdvz;
+        }
+    }
+    else
+    {
+        xb += xwb;
+        qb += qwb;
+    }
+    // inertial target x + v dt (+ g dt^2)
+    xb += nlb;
+    vb += nlb * dt;
+    DV wdtb{0, 0, 0};
+    dqint_adj(q, w * dt, qintb, qb, wdtb);
+    wb += wdtb * dt;
+    (void)qbz;
+    st_dv(A.xb + 3 * i, xb);
+    st_dq(A.qb + 4 * i, qb);
+    st_dv(A.vb + 3 * i, vb);
+    st_dv(A.wb + 3 * i, wb);
+    st_dv(A.pvb + 3 * i, pvb);
+}
+
+// Force init of one contact reversed (manifold.cpp:100-174): C0 = basis (xA - xB) + margin,
+// lambda = lambda_old alpha gamma, k = clamp(k_old gamma); anchors kept by the stick rule hand
+// their adjoint to the previous step's contact.
+__global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc, const int *srcSlot,
+                           const int *matchOld, const int *stickUsed, const float4 *oldCr, int numOld, Params P,
+                           AdjBuf A)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nc)
+        return;
+    int4 r = contactRec[t];
+    int m = r.x, i = r.y, sl = m * kMaxContacts + i, a = r.z, b = r.w;
+    const float4 *cr = M.cr + (size_t)sl * kCR;
+    DQ qA = dq4(B.pose[4 * a + P_IANG]), qB = dq4(B.pose[4 * b + P_IANG]);
+    DV rA = dv4(cr[CR_RA]), rB = dv4(cr[CR_RB]);
+    DV pA = dv4(B.pose[4 * a + P_ILIN]), pB = dv4(B.pose[4 * b + P_ILIN]);
+    DV d = (pA + drot(qA, rA)) - (pB + drot(qB, rB));
+    DV c0b = ld_dv(A.c0b + 3 * sl);
+    DV db{0, 0, 0};
+    DV n[3] = {dv4(cr[CR_N]), dv4(cr[CR_T1]), dv4(cr[CR_T2])};
+    for (int k = 0; k < 3; ++k)
+    {
+        double ck = dcomp(c0b, k);
+        db += n[k] * ck;
+        atomic_add3(A.basisb + 9 * m + 3 * k, d * ck);
+    }
+    DQ qAb{0, 0, 0, 0}, qBb{0, 0, 0, 0};
+    DV rAb = ld_dv(A.rAb + 3 * sl), rBb = ld_dv(A.rBb + 3 * sl);
+    drot_adj(qA, rA, db, qAb, rAb);
+    drot_adj(qB, rB, -db, qBb, rBb);
+    bool massA = B.massMoment[a].x > 0.0f, massB = B.massMoment[b].x > 0.0f;
+    if (massA)
+    {
+        atomic_add3(A.xb + 3 * a, db);
+        atomic_add4(A.qb + 4 * a, qAb);
+    }
+    if (massB)
+    {
+        atomic_add3(A.xb + 3 * b, -db);
+        atomic_add4(A.qb + 4 * b, qBb);
+    }
+    int src = srcSlot[m];
+    int j = matchOld[src * kMaxContacts + i];
+    bool kept = stickUsed[src * kMaxContacts + i] != 0;
+    if (j >= 0 && src < numOld)
+    {
+        int os = src * kMaxContacts + j;
+        double ag = (double)P.alpha * (double)P.gamma;
+        DV lb = ld_dv(A.lb + 3 * sl), kb = ld_dv(A.kb + 3 * sl);
+        float4 kold = oldCr[(size_t)os * kCR + CR_PEN];
+        const float ko[3] = {kold.x, kold.y, kold.z};
+        double kbo[3] = {kb.x, kb.y, kb.z};
+        for (int k = 0; k < 3; ++k)
+        {
+            float kg = ko[k] * P.gamma;
+            if (!(kg > kPenaltyMin && kg < kPenaltyMax))
+                kbo[k] = 0.0;
+            else
+                kbo[k] *= (double)P.gamma;
+        }
+        st_dv(A.clb + 3 * os, lb * ag);
+        st_dv(A.ckb + 3 * os, DV{kbo[0], kbo[1], kbo[2]});
+        if (kept)
+        {
+            st_dv(A.crAb + 3 * os, rAb);
+            st_dv(A.crBb + 3 * os, rBb);
+            rAb = rBb = DV{0, 0, 0};
+        }
+    }
+    // fresh anchors: their adjoint goes through the collide Jacobian
+    st_dv(A.rAb + 3 * sl, kept && j >= 0 && src < numOld ? DV{0, 0, 0} : rAb);
+    st_dv(A.rBb + 3 * sl, kept && j >= 0 && src < numOld ? DV{0, 0, 0} : rBb);
+}
+
+// Collide pose Jacobian (collide.cpp:898-955), one thread per (manifold, pose component): the
+// tangent of the basis and fresh anchors along that component, dotted with their adjoints.
+__global__ void k_adj_collide(Bodies B, Manifolds M, int nm, AdjBuf A)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nm * 14)
+        return;
+    int m = t / 14, comp_ = t % 14;
+    int2 ab = M.ab[m];
+    int a = ab.x, b = ab.y;
+    bool onA = comp_ < 7;
+    if (!(B.massMoment[onA ? a : b].x > 0.0f))
+        return;
+    int c = onA ? comp_ : comp_ - 7;
+    float4 pA = B.pose[4 * a + P_ILIN], qA = B.pose[4 * a + P_IANG], pB = B.pose[4 * b + P_ILIN],
+           qB = B.pose[4 * b + P_IANG];
+    float4 sA = B.sizeRad[a], sB = B.sizeRad[b];
+    auto seedv = [&](float4 v, bool on, int base) {
+        return V3T<DualF>{DualF(v.x, on && c == base + 0 ? 1.0 : 0.0), DualF(v.y, on && c == base + 1 ? 1.0 : 0.0),
+                          DualF(v.z, on && c == base + 2 ? 1.0 : 0.0)};
+    };
+    auto seedq = [&](float4 q, bool on) {
+        return Q4T<DualF>{DualF(q.x, on && c == 3 ? 1.0 : 0.0), DualF(q.y, on && c == 4 ? 1.0 : 0.0),
+                          DualF(q.z, on && c == 5 ? 1.0 : 0.0), DualF(q.w, on && c == 6 ? 1.0 : 0.0)};
+    };
+    V3T<DualF> PA = seedv(pA, onA, 0), PB = seedv(pB, !onA, 0);
+    Q4T<DualF> QA = seedq(qA, onA), QB = seedq(qB, !onA);
+    V3T<DualF> SA{DualF(sA.x), DualF(sA.y), DualF(sA.z)}, SB{DualF(sB.x), DualF(sB.y), DualF(sB.z)};
+    NewContactsT<DualF> nc;
+    V3T<DualF> r0, r1, r2;
+    int nn = collide_boxes<DualF>(PA, QA, SA, PB, QB, SB, nc, r0, r1, r2);
+    int n = M.count[m];
+    if (nn != n)
+        return;  // a branch the step did not take cannot happen: values are the step's FP32 values
+    const double *bb = A.basisb + 9 * m;
+    double g = bb[0] * r0.x.d + bb[1] * r0.y.d + bb[2] * r0.z.d + bb[3] * r1.x.d + bb[4] * r1.y.d + bb[5] * r1.z.d +
+               bb[6] * r2.x.d + bb[7] * r2.y.d + bb[8] * r2.z.d;
+    for (int i = 0; i < n; ++i)
+    {
+        const double *ra = A.rAb + 3 * (m * kMaxContacts + i), *rb = A.rBb + 3 * (m * kMaxContacts + i);
+        g += ra[0] * nc.rA[i].x.d + ra[1] * nc.rA[i].y.d + ra[2] * nc.rA[i].z.d;
+        g += rb[0] * nc.rB[i].x.d + rb[1] * nc.rB[i].y.d + rb[2] * nc.rB[i].z.d;
+    }
+    int body = onA ? a : b;
+    if (c < 3)
+        aadd(A.xb + 3 * body + c, g);
+    else
+        aadd(A.qb + 4 * body + (c - 3), g);
+}
+
+// Manifold friction sqrt(mu_A mu_B) (manifold.cpp:104) reversed.
+__global__ void k_adj_friction(Bodies B, Manifolds M, int nm, AdjBuf A)
+{
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nm)
+        return;
+    int2 ab = M.ab[m];
+    double fa = B.friction[ab.x], fb = B.friction[ab.y];
+    double mub = A.mumb[m];
+    double s = sqrt(fa * fb);
+    if (s > 0.0)
+    {
+        aadd(A.mub + ab.x, mub * fb / (2.0 * s));
+        aadd(A.mub + ab.y, mub * fa / (2.0 * s));
+    }
+}
+
+
+} // namespace avbd
+
+// ------------------------------------------------------------------------------------------
+// host side: checkpoints, taped recompute, the reverse sweep
+// ------------------------------------------------------------------------------------------
+namespace
+{
+
+using namespace avbd;
+
+// A step-start state in HBM: body pose records and velocities, the persistent manifolds and the
+// counters (the next serial among them).
+struct Snapshot
+{
+    float4 *pose = nullptr, *vel = nullptr, *angVel = nullptr, *prevVel = nullptr, *cr = nullptr;
+    int2 *ab = nullptr;
+    long long *serial = nullptr;
+    int *count = nullptr, *feature = nullptr;
+    Counters *ctr = nullptr;
+    int numManifolds = 0;
+};
+
+} // namespace
+
+struct AdjointState
+{
+    int steps = 0, every = 1;
+    bool recorded = false;
+    std::vector<Snapshot> ck, seg;
+    Snapshot end;
+    AdjBuf A{};
+    int slotCap = 0;
+    // tape of the step being reversed and the narrowphase/compaction recording (the handle's
+    // pointers of the same names are set only while that step is recomputed)
+    float4 *tapePose = nullptr, *tapeContact = nullptr;
+    int *recMatch = nullptr, *recStick = nullptr, *recSrc = nullptr;
+};
+
+namespace
+{
+
+int snap_alloc(avbd_handle *h, Snapshot &s)
+{
+    int N = h->maxBodies, M = h->maxManifolds, r;
+    size_t C = (size_t)M * kMaxContacts;
+    if ((r = h->alloc(s.pose, 4 * (size_t)N)) || (r = h->alloc(s.vel, N)) || (r = h->alloc(s.angVel, N)) ||
+        (r = h->alloc(s.prevVel, N)) || (r = h->alloc(s.ab, M)) || (r = h->alloc(s.serial, M)) ||
+        (r = h->alloc(s.count, M)) || (r = h->alloc(s.feature, C)) || (r = h->alloc(s.cr, C * kCR)) ||
+        (r = h->alloc(s.ctr, 1)))
+        return r;
+    return AVBD_OK;
+}
+
+int snap_copy(avbd_handle *h, bool save, Snapshot &s)
+{
+    hipStream_t st = h->stream;
+    const size_t N = h->nb;
+    if (save)
+        s.numManifolds = h->lastNumManifolds;
+    const size_t M = s.numManifolds, C = M * kMaxContacts;
+    auto cp = [&](void *snap, void *live, size_t bytes) -> hipError_t {
+        if (!bytes)
+            return hipSuccess;
+        return save ? hipMemcpyAsync(snap, live, bytes, hipMemcpyDeviceToDevice, st)
+                    : hipMemcpyAsync(live, snap, bytes, hipMemcpyDeviceToDevice, st);
+    };
+    hipError_t e;
+    if ((e = cp(s.pose, h->B.pose, 4 * N * sizeof(float4))) || (e = cp(s.vel, h->B.vel, N * sizeof(float4))) ||
+        (e = cp(s.angVel, h->B.angVel, N * sizeof(float4))) || (e = cp(s.prevVel, h->B.prevVel, N * sizeof(float4))) ||
+        (e = cp(s.ab, h->cur.ab, M * sizeof(int2))) || (e = cp(s.serial, h->cur.serial, M * sizeof(long long))) ||
+        (e = cp(s.count, h->cur.count, M * sizeof(int))) || (e = cp(s.feature, h->cur.feature, C * sizeof(int))) ||
+        (e = cp(s.cr, h->cur.cr, C * kCR * sizeof(float4))) || (e = cp(s.ctr, h->ctr, sizeof(Counters))))
+        return fail(AVBD_ERR_HIP, std::string("adjoint snapshot copy: ") + hipGetErrorString(e));
+    if (!save)
+        h->lastNumManifolds = s.numManifolds;
+    return AVBD_OK;
+}
+
+int adj_alloc(avbd_handle *h, int steps, int every)
+{
+    if (!h->adj)
+        h->adj = new AdjointState();
+    AdjointState &S = *h->adj;
+    int r;
+    int nck = steps / every + 1;
+    while ((int)S.ck.size() < nck)
+    {
+        S.ck.emplace_back();
+        if ((r = snap_alloc(h, S.ck.back())))
+            return r;
+    }
+    while ((int)S.seg.size() < every)
+    {
+        S.seg.emplace_back();
+        if ((r = snap_alloc(h, S.seg.back())))
+            return r;
+    }
+    if (!S.end.pose && (r = snap_alloc(h, S.end)))
+        return r;
+    if (!S.tapePose)
+    {
+        int N = h->maxBodies, M = h->maxManifolds, I = 64;
+        S.slotCap = M * kMaxContacts;
+        size_t C = (size_t)S.slotCap;
+        AdjBuf &A = S.A;
+        if ((r = h->alloc(S.tapePose, (size_t)(I + 1) * 2 * N)) ||
+            (r = h->alloc(S.tapeContact, (size_t)(I + 1) * 2 * C)) || (r = h->alloc(S.recMatch, C)) ||
+            (r = h->alloc(S.recStick, C)) || (r = h->alloc(S.recSrc, M)) || (r = h->alloc(A.xb, 3 * (size_t)N)) ||
+            (r = h->alloc(A.qb, 4 * (size_t)N)) || (r = h->alloc(A.vb, 3 * (size_t)N)) ||
+            (r = h->alloc(A.wb, 3 * (size_t)N)) || (r = h->alloc(A.pvb, 3 * (size_t)N)) ||
+            (r = h->alloc(A.ilb, 3 * (size_t)N)) || (r = h->alloc(A.iab, 4 * (size_t)N)) ||
+            (r = h->alloc(A.nlb, 3 * (size_t)N)) || (r = h->alloc(A.nab, 4 * (size_t)N)) ||
+            (r = h->alloc(A.mb, N)) || (r = h->alloc(A.mub, N)) || (r = h->alloc(A.kb, 3 * C)) ||
+            (r = h->alloc(A.lb, 3 * C)) || (r = h->alloc(A.c0b, 3 * C)) || (r = h->alloc(A.rAb, 3 * C)) ||
+            (r = h->alloc(A.rBb, 3 * C)) || (r = h->alloc(A.basisb, 9 * (size_t)M)) || (r = h->alloc(A.mumb, M)) ||
+            (r = h->alloc(A.ckb, 3 * C)) || (r = h->alloc(A.clb, 3 * C)) || (r = h->alloc(A.crAb, 3 * C)) ||
+            (r = h->alloc(A.crBb, 3 * C)))
+        {
+            S.tapePose = nullptr;
+            return r;
+        }
+    }
+    return AVBD_OK;
+}
+
+// One step recomputed with its iterations taped (no finishStep: the reversal reads the step's
+// start velocities, which finishStep would overwrite).
+int taped_recompute(avbd_handle *h)
+{
+    AdjointState &S = *h->adj;
+    h->tapePose = S.tapePose;
+    h->tapeContact = S.tapeContact;
+    h->recMatch = S.recMatch;
+    h->recStick = S.recStick;
+    h->recSrc = S.recSrc;
+    int r = enqueue_prepare(h);
+    if (!r)
+        r = enqueue_iterate(h);
+    h->tapePose = h->tapeContact = nullptr;
+    h->recMatch = h->recStick = h->recSrc = nullptr;
+    return r;
+}
+
+int reverse_step(avbd_handle *h, const Snapshot &start)
+{
+    hipStream_t s = h->stream;
+    AdjointState &S = *h->adj;
+    AdjBuf &A = S.A;
+    const Counters &c = *h->hostCtr;
+    const int nb = h->nb, nc = c.contactCount, nm = c.numManifolds, I = h->P.iterations;
+    const int colors = std::min(c.colorCount, kMaxColors);
+    const size_t ps = 2 * (size_t)h->maxBodies, cs = 2 * (size_t)S.slotCap;
+    const float4 *tp = S.tapePose, *tc = S.tapeContact;
+    int n0 = std::max(std::max(nb, nc), std::max(nm, 1));
+    k_adj_begin<<<ceil_div(n0, 256), 256, 0, s>>>(h->S.contactRec, nc, nm, nb, A);
+    k_adj_clear_carry<<<ceil_div(3 * S.slotCap, 256), 256, 0, s>>>(3 * S.slotCap, A);
+    k_adj_finish<<<ceil_div(nb, 256), 256, 0, s>>>(h->B, h->P, nb, A);
+    for (int it = I - 1; it >= 0; --it)
+    {
+        const float4 *prev = tp + it * ps, *cur = tp + (it + 1) * ps, *lam = tc + it * cs;
+        if (nc > 0)
+            k_adj_dual<<<ceil_div(nc, 128), 128, 0, s>>>(h->B, h->cur, h->S.contactRec, nc, cur, lam, h->P, A);
+        for (int col = colors - 1; col >= 0; --col)
+        {
+            int b = h->hostOffsets[col], e = h->hostOffsets[col + 1];
+            if (e > b)
+                k_adj_primal<<<ceil_div(e - b, kAdjThreads / kAdjLanes), kAdjThreads, 0, s>>>(
+                    h->B, h->cur, h->S.stamps, h->S.tasks, b, e, h->S.color, col, prev, cur, lam, h->P, A);
+        }
+    }
+    k_adj_warmstart<<<ceil_div(nb, 256), 256, 0, s>>>(h->B, h->P, h->S.deg, nb, A);
+    if (nc > 0)
+        k_adj_init<<<ceil_div(nc, 256), 256, 0, s>>>(h->B, h->cur, h->S.contactRec, nc, S.recSrc, S.recMatch,
+                                                     S.recStick, start.cr, start.numManifolds, h->P, A);
+    if (nm > 0)
+    {
+        k_adj_collide<<<ceil_div(nm * 14, 64), 64, 0, s>>>(h->B, h->cur, nm, A);
+        k_adj_friction<<<ceil_div(nm, 256), 256, 0, s>>>(h->B, h->cur, nm, A);
+    }
+    HIP_TRY(hipGetLastError());
+    return AVBD_OK;
+}
+
+int check_adjoint_scene(avbd_handle *h)
+{
+    if (h->nj > 0 || h->ns > 0)
+        return fail(AVBD_ERR_UNSUPPORTED, "adjoint: contact-only worlds (no joints or springs)");
+    if (h->P.iterations > 64)
+        return fail(AVBD_ERR_UNSUPPORTED, "adjoint: at most 64 iterations per step");
+    return AVBD_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int avbd_adjoint_forward(avbd_handle *h, int32_t num_steps, int32_t checkpoint_every)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_forward: null handle");
+    if (num_steps < 1 || checkpoint_every < 1)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_forward: steps and checkpoint interval must be >= 1");
+    int r;
+    if ((r = pick_device(h)) || (r = check_adjoint_scene(h)) || (r = adj_alloc(h, num_steps, checkpoint_every)))
+        return r;
+    AdjointState &S = *h->adj;
+    S.recorded = false;
+    S.steps = num_steps;
+    S.every = checkpoint_every;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if ((r = snap_copy(h, true, S.ck[0])))
+        return r;
+    for (int k = 0; k < num_steps; ++k)
+    {
+        if ((r = enqueue_prepare(h)) || (r = enqueue_iterate(h)) || (r = enqueue_finish(h)))
+            return r;
+        if ((k + 1) % checkpoint_every == 0 && (k + 1) < num_steps && (r = snap_copy(h, true, S.ck[(k + 1) / checkpoint_every])))
+            return r;
+    }
+    if ((r = snap_copy(h, true, S.end)) || (r = avbd_synchronize(h)))
+        return r;
+    S.recorded = true;
+    return AVBD_OK;
+}
+
+int avbd_adjoint_backward(avbd_handle *h, const double *d_pos, const double *d_rot, const double *d_vel,
+                          const double *d_ang_vel, double *g_pos, double *g_rot, double *g_vel, double *g_ang_vel,
+                          double *g_mass, double *g_friction)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_backward: null handle");
+    if (!h->adj || !h->adj->recorded)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_backward: no recorded forward (avbd_adjoint_forward)");
+    int r;
+    if ((r = pick_device(h)) || (r = check_adjoint_scene(h)))
+        return r;
+    AdjointState &S = *h->adj;
+    AdjBuf &A = S.A;
+    hipStream_t s = h->stream;
+    const size_t N = h->nb;
+    auto zero = [&](double *p, size_t n) { return hipMemsetAsync(p, 0, n * sizeof(double), s); };
+    auto up = [&](double *dst, const double *src, size_t n) {
+        return src ? hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s) : zero(dst, n);
+    };
+    const size_t C3 = 3 * (size_t)S.slotCap;
+    hipError_t e;
+    if ((e = up(A.xb, d_pos, 3 * N)) || (e = up(A.qb, d_rot, 4 * N)) || (e = up(A.vb, d_vel, 3 * N)) ||
+        (e = up(A.wb, d_ang_vel, 3 * N)) || (e = zero(A.pvb, 3 * N)) || (e = zero(A.mb, N)) || (e = zero(A.mub, N)) ||
+        (e = zero(A.ckb, C3)) || (e = zero(A.clb, C3)) || (e = zero(A.crAb, C3)) || (e = zero(A.crBb, C3)))
+        return fail(AVBD_ERR_HIP, std::string("adjoint init: ") + hipGetErrorString(e));
+    const int K = S.every, steps = S.steps;
+    const int nseg = (steps + K - 1) / K;
+    for (int sg = nseg - 1; sg >= 0; --sg)
+    {
+        const int s0 = sg * K, s1 = std::min(s0 + K, steps);
+        // replay the segment from its checkpoint, keeping every step's start state
+        if ((r = snap_copy(h, false, S.ck[sg])) || (r = snap_copy(h, true, S.seg[0])))
+            return r;
+        for (int st = s0; st < s1 - 1; ++st)
+        {
+            if ((r = enqueue_prepare(h)) || (r = enqueue_iterate(h)) || (r = enqueue_finish(h)) ||
+                (r = snap_copy(h, true, S.seg[st - s0 + 1])))
+                return r;
+        }
+        for (int st = s1 - 1; st >= s0; --st)
+        {
+            Snapshot &start = S.seg[st - s0];
+            if ((r = snap_copy(h, false, start)) || (r = taped_recompute(h)) || (r = reverse_step(h, start)))
+                return r;
+        }
+    }
+    // leave the handle at the forward's final state
+    if ((r = snap_copy(h, false, S.end)))
+        return r;
+    auto down = [&](double *dst, const double *src, size_t n) {
+        return dst ? hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s) : hipSuccess;
+    };
+    if ((e = down(g_pos, A.xb, 3 * N)) || (e = down(g_rot, A.qb, 4 * N)) || (e = down(g_vel, A.vb, 3 * N)) ||
+        (e = down(g_ang_vel, A.wb, 3 * N)) || (e = down(g_mass, A.mb, N)) || (e = down(g_friction, A.mub, N)) ||
+        (e = hipStreamSynchronize(s)))
+        return fail(AVBD_ERR_HIP, std::string("adjoint results: ") + hipGetErrorString(e));
+    return avbd_synchronize(h);
+}
+
+} // extern "C"

@@ -92,6 +92,12 @@ struct avbd_handle
     std::vector<hipGraphNode_t> graphPrimal;
     hipGraphNode_t graphDual = nullptr;
     avbd_launch_config graphLaunch{};
+    // adjoint: recording outputs of the narrowphase/compaction (null unless a step is being taped),
+    // the tape of the step being reversed and the rest of the adjoint state (avbd_adjoint.hip)
+    int *recMatch = nullptr, *recStick = nullptr, *recSrc = nullptr;
+    float4 *tapePose = nullptr, *tapeContact = nullptr;
+    struct AdjointState *adj = nullptr;
+    int lastNumManifolds = 0;
 
     template <class T>
     int alloc(T *&p, size_t count)
@@ -345,9 +351,9 @@ int enqueue_prepare(avbd_handle *h)
         HIP_TRY(hipEventRecord(h->ev[1], s));
 
     // ---- force initialisation (manifold.cpp:100-174, joint.cpp:35-54) ----
-    k_narrowphase<<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep);
+    k_narrowphase<<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick);
     scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numKept);
-    k_compact_manifolds<<<G, 256, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr);
+    k_compact_manifolds<<<G, 256, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc);
     k_compact_finish<<<1, 1, 0, s>>>(ctr);
     scan(h, h->cur.count, S.contactStart, &ctr->numManifolds, 0, &ctr->contactCount);
     k_contact_records<<<G, 256, 0, s>>>(h->cur.count, h->cur.ab, S.contactStart, ctr, S.contactRec);
@@ -527,7 +533,19 @@ int enqueue_iterate(avbd_handle *h)
             ranges.emplace_back(h->hostOffsets[col], h->hostOffsets[col + 1]);
     const int primalPerIter = (int)ranges.size();
     const avbd_launch_config &lc = h->launch;
-    if (lc.use_graph && h->P.iterations > 0 && (primalPerIter > 0 || dualLanes > 0))
+    h->lastNumManifolds = c.numManifolds;
+    // adjoint tape (only while a step is being recomputed for its reversal): slot 0 holds the
+    // state after warm start and force init, slot it+1 the state after iteration it
+    const bool tape = h->tapePose != nullptr;
+    const size_t tapePoseStride = 2 * (size_t)h->maxBodies, tapeContactStride = 2 * (size_t)kMaxContacts * h->maxManifolds;
+    auto tapeAt = [&](int slot) {
+        k_tape_pose<<<ceil_div(std::max(h->nb, 1), 256), 256, 0, s>>>(h->B.pose, h->nb, h->tapePose + slot * tapePoseStride);
+        k_tape_contacts<<<grid_for(std::max(c.contactCount, 1)), 256, 0, s>>>(h->S.contactRec, h->ctr, h->cur.cr,
+                                                                            h->tapeContact + slot * tapeContactStride);
+    };
+    if (tape)
+        tapeAt(0);
+    if (lc.use_graph && !tape && h->P.iterations > 0 && (primalPerIter > 0 || dualLanes > 0))
     {
         int r = iterate_graph(h, ranges, c.contactCount, dualLanes);
         if (r)
@@ -571,6 +589,8 @@ int enqueue_iterate(avbd_handle *h)
             }
             if (mark && it == last)
                 HIP_TRY(hipEventRecord(mark[6], s));
+            if (tape)
+                tapeAt(it + 1);
         }
     }
     if (h->timing)
@@ -1012,6 +1032,7 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     h->nb = N;
     h->nj = num_joints;
     h->ns = num_springs;
+    h->lastNumManifolds = M;
     h->numWorlds = 1;
     h->worldStartHost.assign({0, N});
     return AVBD_OK;
@@ -1616,3 +1637,5 @@ int avbd_sizeof(int which)
 }
 
 } // extern "C"
+
+#include "avbd_adjoint.hip"

@@ -758,7 +758,10 @@ __global__ void k_create_finish(Counters *ctr, int maxManifolds, int maxPairs)
 
 // Manifold::initialize for box pairs (manifold.cpp:100-174). Reads the persistent buffer `cur`
 // and writes the initialised manifold to the same slot of `tmp`.
-__global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, Counters *ctr, int *keep)
+// matchOld / stickUsed (adjoint recording, may be null): per new contact slot, the old contact it
+// continues (-1: none) and whether its anchors were kept by the stick rule.
+__global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, Counters *ctr, int *keep,
+                              int *matchOld, int *stickUsed)
 {
     int total = ctr->numTotal;
     int old = ctr->numManifolds;
@@ -787,6 +790,7 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
         {
             V3 rA = fv(nc.rA[i]), rB = fv(nc.rB[i]);
             float4 pen = make_float4(0, 0, 0, 0), lam = make_float4(0, 0, 0, 0);
+            int match = -1, kept = 0;
             for (int j = 0; j < nOld; ++j)
             {
                 if (cur.feature[m * kMaxContacts + j] == nc.feature[i])
@@ -794,13 +798,20 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
                     const float4 *o = cur.cr + (size_t)(m * kMaxContacts + j) * kCR;
                     pen = o[CR_PEN];
                     lam = o[CR_LAM];
+                    match = j;
                     if (lam.w != 0.0f)
                     {
                         rA = ld3(o[CR_RA]);
                         rB = ld3(o[CR_RB]);
+                        kept = 1;
                     }
                     break;
                 }
+            }
+            if (matchOld)
+            {
+                matchOld[m * kMaxContacts + i] = match;
+                stickUsed[m * kMaxContacts + i] = kept;
             }
             V3 xA = pA + rotate(qA, rA);
             V3 xB = pB + rotate(qB, rB);
@@ -825,8 +836,9 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
     }
 }
 
+// srcSlot (adjoint recording, may be null): the pre-compaction slot of each kept manifold.
 __global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *keep, const int *keepRank,
-                                    const float4 *massMoment, Counters *ctr)
+                                    const float4 *massMoment, Counters *ctr, int *srcSlot)
 {
     int total = ctr->numTotal;
     int contacts = 0, endpoints = 0;
@@ -837,6 +849,8 @@ __global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *kee
         int d = keepRank[m];
         int n = tmp.count[m];
         int2 ab = tmp.ab[m];
+        if (srcSlot)
+            srcSlot[d] = m;
         contacts += n;
         endpoints += n * ((massMoment[ab.x].x > 0.0f ? 1 : 0) + (massMoment[ab.y].x > 0.0f ? 1 : 0));
         cur.ab[d] = ab;
@@ -1797,6 +1811,29 @@ __global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int4 
         dual_contact(contactRec[t], M, B, P);
     else if (t - nc < nj)
         dual_joint(&joints[t - nc], B, P);
+}
+
+// Adjoint tape of the step being reversed: poses (pos, rot per body) and contact penalty/lambda
+// (per contact slot) after the warm start / force init and after every iteration's primal and dual.
+__global__ void k_tape_pose(const float4 *pose, int n, float4 *dst)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    dst[2 * i] = pose[4 * i + P_POS];
+    dst[2 * i + 1] = pose[4 * i + P_ROT];
+}
+
+__global__ void k_tape_contacts(const int4 *contactRec, const Counters *ctr, const float4 *cr, float4 *dst)
+{
+    int nc = ctr->contactCount;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+    {
+        int4 r = contactRec[c];
+        int sl = r.x * kMaxContacts + r.y;
+        dst[2 * sl] = cr[(size_t)sl * kCR + CR_PEN];
+        dst[2 * sl + 1] = cr[(size_t)sl * kCR + CR_LAM];
+    }
 }
 
 // Solver::finishStep for boxes (solver.cpp:1275-1288).

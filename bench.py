@@ -103,6 +103,11 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=20)
     ap.add_argument("--batch-sample", type=int, default=64,
                     help="worlds stepped one handle at a time for the batched-vs-separate comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout measurement")
+    ap.add_argument("--adjoint-steps", type=int, default=120)
+    ap.add_argument("--adjoint-checkpoint", type=int, default=10)
+    ap.add_argument("--no-adjoint-check", action="store_true",
+                    help="skip the CPU forward-mode and FP64 finite-difference gradient checks")
     args = ap.parse_args()
 
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
@@ -232,7 +237,123 @@ def main():
     torch.cuda.synchronize()
     if args.batch_worlds > 0:
         result["batched"] = batched(args, params)
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_leg(args)
     print(json.dumps(result), flush=True)
+
+
+def adjoint_leg(args):
+    """The differentiable rollout: a 1024-box block plus 64 projectiles over 120 steps, loss =
+    |COM(block) - target|^2, gradients w.r.t. projectile velocities and angular velocities, all
+    masses and friction coefficients; cost against the plain forward, and the gradient checked
+    against the CPU forward-mode reference (same FP32 trajectory) and FP64 central differences."""
+    import numpy as np
+    import torch
+    from avbd import adjoint, capi, scenes
+    from avbd.state import default_params
+    sc = scenes.projectile_pile(1024, 64, seed=5)
+    st, pile, proj, target = scenes.projectile_state(sc, capi.Maker(), seed=5)
+    prm = default_params(gravity=sc.gravity)
+    steps, every = args.adjoint_steps, args.adjoint_checkpoint
+    n_dyn = int((st.bodies["mass"] > 0).sum())
+    # plain forward (same state, same steps)
+    plain = capi.HipWorld.for_state(st, prm)
+    plain.step(steps)
+    plain.synchronize()
+    times = []
+    for _ in range(3):
+        plain.upload(st)
+        plain.synchronize()
+        t0 = time.perf_counter()
+        plain.step(steps)
+        plain.synchronize()
+        times.append(time.perf_counter() - t0)
+    plain.close()
+    t_plain = min(times)
+    sim = adjoint.Rollout(st, prm, steps, checkpoint_every=every)
+    b = st.bodies
+    t = lambda a: torch.tensor(np.asarray(a, np.float64), requires_grad=True)
+    best = None
+    for _ in range(3):
+        vel, ang, mass, fric = t(b["vel"]), t(b["ang_vel"]), t(b["mass"]), t(b["friction"])
+        t0 = time.perf_counter()
+        pos = sim(vel, ang, mass, fric)
+        t1 = time.perf_counter()
+        w = mass[torch.as_tensor(pile)]
+        com = (pos[torch.as_tensor(pile)] * w[:, None]).sum(0) / w.sum()
+        loss = ((com - torch.as_tensor(target)) ** 2).sum()
+        loss.backward()
+        t2 = time.perf_counter()
+        if best is None or (t2 - t0) < best[0] + best[1]:
+            best = (t1 - t0, t2 - t1)
+    t_fwd, t_bwd = best
+    log(f"adjoint: plain {t_plain * 1e3:.1f} ms, forward with checkpoints {t_fwd * 1e3:.1f} ms, "
+        f"backward {t_bwd * 1e3:.1f} ms -> {(t_fwd + t_bwd) / t_plain:.2f}x the plain forward")
+    out = {
+        "workload": f"1024-box block + 64 projectiles (masses U[0.5,2], moments scaled, friction U[0.3,0.8]), "
+                    f"{steps} steps, dt=1/60, 10 iterations, checkpoints every {every} steps; loss "
+                    f"|COM(block) - target|^2",
+        "bodies": int(len(b)), "dynamic_bodies": n_dyn, "steps": steps, "checkpoint_every": every,
+        "plain_forward_ms": t_plain * 1e3, "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
+        "cost_vs_plain_forward": (t_fwd + t_bwd) / t_plain, "loss": float(loss.detach()),
+        "dtype": "f32 forward, f64 adjoint",
+    }
+    if not args.no_adjoint_check:
+        out["checks"] = adjoint_checks(sim, st, prm, steps, pile, proj, target, pos.detach().numpy(),
+                                       {"vel": vel.grad.numpy(), "ang_vel": ang.grad.numpy(),
+                                        "mass": mass.grad.numpy(), "friction": fric.grad.numpy()})
+    sim.close()
+    return out
+
+
+def adjoint_checks(sim, st, prm, steps, pile, proj, target, pos, grads):
+    """Directional derivatives of the loss against the CPU oracle: the forward-mode run on the
+    same FP32 trajectory, and FP64 central differences (h = 1e-4) on the FP64 trajectory."""
+    import numpy as np
+    import oracle.bindings as ob
+    b = st.bodies
+    nb = len(b)
+    rng = np.random.default_rng(11)
+    w = b["mass"][pile].astype(np.float64)
+    com = (pos[pile] * w[:, None]).sum(0) / w.sum()
+    dcom = 2.0 * (com - target)
+    dirs = {"vel": np.zeros((nb, 3)), "ang_vel": np.zeros((nb, 3))}
+    dirs["vel"][proj] = rng.normal(size=(len(proj), 3))
+    dirs["ang_vel"][proj] = rng.normal(size=(len(proj), 3))
+    dirs["mass"] = rng.normal(size=nb) * (b["mass"] > 0) * 0.1
+    dirs["friction"] = rng.normal(size=nb) * 0.05
+    seed = {"vel": "dvel", "ang_vel": "dangvel", "mass": "dmass", "friction": "dfriction"}
+    fdkey = {"vel": "vel", "ang_vel": "angvel", "mass": "mass", "friction": "friction"}
+    base = {"vel": b["vel"], "ang_vel": b["ang_vel"], "mass": b["mass"], "friction": b["friction"]}
+    init = sim.initial_state(b["vel"], b["ang_vel"], b["mass"], b["friction"])
+
+    def loss_of(p, m):
+        ww = m[pile]
+        c = (p[pile] * ww[:, None]).sum(0) / ww.sum()
+        return float(((c - target) ** 2).sum())
+
+    out = {}
+    t0 = time.perf_counter()
+    for k, u in dirs.items():
+        got = float((grads[k] * u).sum())
+        _, tan = ob.oracle_tangent_run(init, prm, steps, **{seed[k]: u})
+        ref = float(dcom @ ((tan[pile, 0:3] * w[:, None]).sum(0) / w.sum()))
+        if k == "mass":
+            ref += float(dcom @ (((pos[pile] - com) * u[pile, None]).sum(0) / w.sum()))
+        h = 1e-4
+        x0 = base[k].astype(np.float64)
+        lp, lm = [], []
+        for sgn, acc in ((1.0, lp), (-1.0, lm)):
+            xs = x0 + sgn * h * u
+            r = ob.oracle_f64_run(init, prm, steps, **{fdkey[k]: xs})
+            acc.append(loss_of(r[:, 0:3], xs if k == "mass" else b["mass"].astype(np.float64)))
+        fd = (lp[0] - lm[0]) / (2 * h)
+        out[k] = {"adjoint": got, "forward_mode_fp32_path": ref, "rel_vs_forward_mode": abs(got - ref) / max(abs(ref), 1e-30),
+                  "fd_fp64_h1e-4": fd, "rel_vs_fd": abs(got - fd) / max(abs(fd), 1e-30)}
+        log(f"adjoint check {k:8s}: adjoint {got: .6e} forward-mode {ref: .6e} "
+            f"(rel {out[k]['rel_vs_forward_mode']:.1e}) FD64 {fd: .6e} (rel {out[k]['rel_vs_fd']:.1e})")
+    out["seconds"] = time.perf_counter() - t0
+    return out
 
 
 def batched(args, params):

@@ -314,6 +314,27 @@ int avbd_get_launch_config(avbd_handle *h, avbd_launch_config *out);
  * so tuning changes speed only; stats sums are reset. out may be NULL. */
 int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launch_config *out);
 
+/* ---- Adjoint of the step (reverse-mode gradients) ----
+ * No counterpart in the reference (its solver is not differentiable). For
+ * contact-only box worlds: avbd_adjoint_forward steps num_steps times keeping
+ * the state at every checkpoint_every-th step start in HBM; avbd_adjoint_backward
+ * then returns, for a loss L of the final state, dL/d(initial pos, rot, vel,
+ * ang_vel) and dL/d(mass) (each body's moments scale with its mass) and
+ * dL/d(friction) per body. It replays each checkpoint segment, recomputes every
+ * step with its iterations taped, and sweeps the step in reverse: finishStep,
+ * then per iteration the dual update and the primal colours in reverse order
+ * (one transposed 6x6 LDL^T solve per body), the warm start and the force init
+ * (collide's pose Jacobian included). The gradient is that of the FP32
+ * trajectory with its branches (contact set, clamps, stick flags) held fixed,
+ * evaluated in FP64. d_* inputs (nb x 3, nb x 4 for rot) may be NULL (zero);
+ * g_* outputs (same shapes; nb for mass and friction) may be NULL. The handle
+ * is left at the forward's final state. */
+int avbd_adjoint_forward(avbd_handle *h, int32_t num_steps, int32_t checkpoint_every);
+int avbd_adjoint_backward(avbd_handle *h, const double *d_pos, const double *d_rot,
+                          const double *d_vel, const double *d_ang_vel,
+                          double *g_pos, double *g_rot, double *g_vel, double *g_ang_vel,
+                          double *g_mass, double *g_friction);
+
 /* Test hook: make the next avbd_step on this handle fail with AVBD_ERR_INJECTED
  * before any work is enqueued (code 1), or clear it (code 0). */
 int avbd_debug_inject_fault(avbd_handle *h, int32_t code);

@@ -66,7 +66,7 @@ def load_oracle():
         lib.orc_quat_ops.argtypes = [_p] * 7
         lib.orc_joint_spring_derived.argtypes = [_p, _p, _p, _p, _p, _p]
         lib.orc_sizeof.argtypes = [_i]
-        lib.orc_tangent_run.argtypes = [_p, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 6
+        lib.orc_tangent_run.argtypes = [_i, _p, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 6
         lib.orc_f64_run.argtypes = [_p, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 5
         check_layouts(lib.orc_sizeof, records=6)
         _ORC = lib
@@ -342,10 +342,12 @@ def _f64(x, n):
     return None if x is None else np.ascontiguousarray(np.asarray(x, np.float64).reshape(n))
 
 
-def oracle_tangent_run(state: State, params, steps: int, dvel=None, dangvel=None, dmass=None, dfriction=None):
-    """Forward-mode reference: the FP32 trajectory (colour order) with FP64 tangents seeded on
-    the initial velocities / angular velocities, masses (moments scale with mass) and friction
-    coefficients. Returns (values, tangents), each (bodies, 13): pos3 rot4 vel3 angvel3."""
+def oracle_tangent_run(state: State, params, steps: int, dvel=None, dangvel=None, dmass=None, dfriction=None,
+                       fp64: bool = False):
+    """Forward-mode reference: the FP32 trajectory (colour order; the FP64 one with fp64=True)
+    with FP64 tangents seeded on the initial velocities / angular velocities, masses (moments
+    scale with mass) and friction coefficients. Returns (values, tangents), each (bodies, 13):
+    pos3 rot4 vel3 angvel3."""
     lib = load_oracle()
     nb = len(state.bodies)
     if len(state.joints) or len(state.springs):
@@ -353,7 +355,7 @@ def oracle_tangent_run(state: State, params, steps: int, dvel=None, dangvel=None
     prm = np.ascontiguousarray(params)
     val = np.zeros((nb, 13)); tan = np.zeros((nb, 13))
     args = [_f64(dvel, 3 * nb), _f64(dangvel, 3 * nb), _f64(dmass, nb), _f64(dfriction, nb)]
-    lib.orc_tangent_run(prm.ctypes.data_as(_p), _ptr(state.bodies), nb, _ptr(state.manifolds), len(state.manifolds),
+    lib.orc_tangent_run(int(fp64), prm.ctypes.data_as(_p), _ptr(state.bodies), nb, _ptr(state.manifolds), len(state.manifolds),
                         C.c_int64(state.next_serial), steps, *[_ptr(a) for a in args], val.ctypes.data_as(_p),
                         tan.ctypes.data_as(_p))
     return val, tan

@@ -1,0 +1,171 @@
+"""The hand-written adjoint of the step (avbd_adjoint_forward / _backward) on the GPU.
+
+Reference: the oracle's forward-mode run (orc_tangent_run). Its value half performs the FP32
+operations of the step, so it follows the GPU's FP32 trajectory bit for bit, branches
+included, and its FP64 tangent is the directional derivative of that trajectory. For a loss
+L = sum c . pos_final, the adjoint's gradient g must satisfy g . u = dL[u] for directions u
+over each parameter kind (initial velocities, angular velocities, masses with moments scaling,
+friction coefficients). Bar: 1e-4 relative. Both are FP64 derivatives of the same FP32 path;
+they differ only in the point each linearises at (the forward-mode run uses the FP32-rounded
+intermediates, the adjoint recomputes them in FP64 from the FP32 state), which the stiff
+penalty rows amplify to ~1e-5 over tens of steps.
+The tangent itself is pinned against FP64 central differences in test_oracle_adjoint.py.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KINDS = ("vel", "ang_vel", "mass", "friction")
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from avbd import capi
+    capi.load()
+    return capi
+
+
+def _directions(state, seed):
+    rng = np.random.default_rng(seed)
+    nb = len(state.bodies)
+    dyn = (state.bodies["mass"] > 0).astype(np.float64)
+    return {"vel": rng.normal(size=(nb, 3)) * dyn[:, None], "ang_vel": rng.normal(size=(nb, 3)) * dyn[:, None],
+            "mass": rng.normal(size=nb) * dyn * state.bodies["mass"], "friction": rng.normal(size=nb) * 0.1}
+
+
+def _check(gpu, oracle, state, params, steps, every, seed=0, tol=1e-4, label=""):
+    nb = len(state.bodies)
+    rng = np.random.default_rng(seed + 100)
+    c = rng.normal(size=(nb, 3)) * (state.bodies["mass"] > 0)[:, None]
+    w = gpu.HipWorld.for_state(state, params)
+    w.adjoint_forward(steps, every)
+    g = w.adjoint_backward(d_pos=c)
+    final = w.state()
+    w.close()
+    u = _directions(state, seed)
+    seeds = {"vel": "dvel", "ang_vel": "dangvel", "mass": "dmass", "friction": "dfriction"}
+    errs = {}
+    for k in KINDS:
+        val, tan = oracle.oracle_tangent_run(state, params, steps, **{seeds[k]: u[k]})
+        # the forward-mode run and the GPU forward are the same FP32 trajectory
+        np.testing.assert_array_equal(val[:, 0:3].astype(np.float32), final.bodies["pos"], err_msg=f"{label} {k}")
+        ref = float((c * tan[:, 0:3]).sum())
+        got = float((g[k] * u[k]).sum())
+        errs[k] = abs(got - ref) / max(abs(ref), 1e-12)
+        print(f"{label} {k:8s} adjoint {got: .10e}  forward-mode {ref: .10e}  rel {errs[k]:.2e}", flush=True)
+    for k in KINDS:
+        assert errs[k] < tol, f"{label} {k}: rel error {errs[k]:.2e}"
+
+
+def test_adjoint_free_flight(gpu, oracle):
+    from avbd import capi, scenes
+    from avbd.state import default_params
+    sc = scenes.stacks(1, seed=2)
+    st = scenes.build_state(sc, oracle.OracleMaker())
+    st.bodies["pos"][0, 2] += 3.0          # the box starts in the air
+    st.bodies["vel"][0] = (0.3, -0.2, 1.0)
+    st.bodies["ang_vel"][0] = (0.5, 0.1, -0.7)
+    _check(gpu, oracle, st, default_params(), 12, 4, label="free flight")
+
+
+def _settled(oracle, sc, steps):
+    from avbd import scenes
+    from avbd.state import default_params
+    prm = default_params(gravity=sc.gravity)
+    o = oracle.OracleWorld(scenes.build_state(sc, oracle.OracleMaker()), prm)
+    o.step(steps)
+    return o.state(), prm
+
+
+def test_adjoint_stack(gpu, oracle):
+    from avbd import scenes
+    st, prm = _settled(oracle, scenes.stack(count=3), 5)
+    _check(gpu, oracle, st, prm, 6, 3, label="stack3")
+
+
+def test_adjoint_pile_with_checkpoints(gpu, oracle):
+    from avbd import scenes
+    st, prm = _settled(oracle, scenes.pile(24, seed=7), 70)
+    assert len(st.manifolds) > 0
+    _check(gpu, oracle, st, prm, 20, 5, label="pile24")
+
+
+def test_adjoint_checkpoint_interval_does_not_change_the_gradient(gpu, oracle):
+    from avbd import capi, scenes
+    st, prm = _settled(oracle, scenes.pile(24, seed=8), 70)
+    nb = len(st.bodies)
+    c = np.random.default_rng(3).normal(size=(nb, 3))
+    out = []
+    for every in (1, 4, 16):
+        w = gpu.HipWorld.for_state(st, prm)
+        w.adjoint_forward(16, every)
+        out.append(w.adjoint_backward(d_pos=c))
+        w.close()
+    for k in KINDS:
+        np.testing.assert_allclose(out[1][k], out[0][k], rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(out[2][k], out[0][k], rtol=1e-9, atol=1e-12)
+
+
+def test_adjoint_rejects_joints(gpu, oracle):
+    from avbd import capi, scenes
+    from avbd.state import default_params
+    st = scenes.build_state(scenes.pyramid_chain(base=0, links=4), gpu.Maker())
+    w = gpu.HipWorld.for_state(st, default_params())
+    with pytest.raises(gpu.AvbdError, match="contact-only"):
+        w.adjoint_forward(2, 1)
+    with pytest.raises(gpu.AvbdError, match="no recorded forward"):
+        w.adjoint_backward()
+
+
+def _com_loss_parts(pos, mass, pile, target):
+    w = mass[pile]
+    com = (pos[pile] * w[:, None]).sum(0) / w.sum()
+    return com, ((com - target) ** 2).sum()
+
+
+def test_rollout_autograd_function_matches_forward_mode(gpu, oracle):
+    """torch.autograd.Function over the C ABI: the projectile workload at small size, loss =
+    |COM(block) - target|^2 with the COM weighted by the (differentiable) masses."""
+    import torch
+    from avbd import adjoint, scenes
+    from avbd.state import default_params
+    sc = scenes.projectile_pile(count=64, projectiles=4, seed=9)
+    st, pile, proj, target = scenes.projectile_state(sc, gpu.Maker(), seed=9)
+    prm = default_params(gravity=sc.gravity)
+    steps = 90
+    sim = adjoint.Rollout(st, prm, steps, checkpoint_every=10)
+    b = st.bodies
+    t = lambda a: torch.tensor(np.asarray(a, np.float64), requires_grad=True)
+    vel, ang, mass, fric = t(b["vel"]), t(b["ang_vel"]), t(b["mass"]), t(b["friction"])
+    pos = sim(vel, ang, mass, fric)
+    com, loss = _com_loss_parts(pos, mass, torch.as_tensor(pile), torch.as_tensor(target))
+    loss.backward()
+    np.testing.assert_array_equal(pos.detach().numpy().astype(np.float32), sim.final.bodies["pos"])
+    # forward-mode reference of the same FP32 trajectory, plus the loss's explicit mass term
+    rng = np.random.default_rng(4)
+    nb = len(b)
+    com_np = com.detach().numpy()
+    dcom = 2.0 * (com_np - target)
+    wsum = b["mass"][pile].astype(np.float64).sum()
+    dirs = {"dvel": np.zeros((nb, 3)), "dangvel": np.zeros((nb, 3))}
+    dirs["dvel"][proj] = rng.normal(size=(len(proj), 3))
+    dirs["dangvel"][proj] = rng.normal(size=(len(proj), 3))
+    dirs["dmass"] = rng.normal(size=nb) * (b["mass"] > 0)
+    dirs["dfriction"] = rng.normal(size=nb) * 0.05
+    grads = {"dvel": vel.grad.numpy(), "dangvel": ang.grad.numpy(), "dmass": mass.grad.numpy(),
+             "dfriction": fric.grad.numpy()}
+    pos_np = pos.detach().numpy()
+    for k, u in dirs.items():
+        _, tan = oracle.oracle_tangent_run(sim.initial_state(b["vel"], b["ang_vel"], b["mass"], b["friction"]), prm,
+                                           steps, **{k: u})
+        ref = float(dcom @ ((tan[pile, 0:3] * b["mass"][pile, None]).sum(0) / wsum))
+        if k == "dmass":
+            ref += float(dcom @ (((pos_np[pile] - com_np) * u[pile, None]).sum(0) / wsum))
+        got = float((grads[k] * u).sum())
+        rel = abs(got - ref) / max(abs(ref), 1e-12)
+        print(f"rollout {k:9s} autograd {got: .10e}  forward-mode {ref: .10e}  rel {rel:.2e}", flush=True)
+        assert rel < 1e-4, f"{k}: {rel:.2e}"
+    sim.close()

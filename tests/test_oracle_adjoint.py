@@ -1,0 +1,92 @@
+"""The forward-mode reference gradient (oracle World<DualT>) against FP64 central differences.
+
+orc_tangent_run with an FP64 value half follows the FP64 trajectory, the one central finite
+differences of orc_f64_run see; where that trajectory is smooth in the perturbation (no
+contact, stick, clamp or feature change within +-h) the two must agree to FD accuracy. The
+FP32-value variant (the adjoint's reference) runs the same template code, so this pins the
+derivative rules (every DualT operation, sqrt, abs, min/max branches) it uses.
+
+Also pinned by closed form: a free box's ballistic step (x1 = x0 + v dt + g dt^2 w, w = 0 from
+rest-start acceleration, v1 = (x1 - x0)/dt) has d(z_final)/d(v0_z) = steps * dt.
+"""
+import numpy as np
+import pytest
+
+
+def _fd(oracle, st, prm, steps, key, base, u, h):
+    p = oracle.oracle_f64_run(st, prm, steps, **{key: base + h * u})
+    m = oracle.oracle_f64_run(st, prm, steps, **{key: base - h * u})
+    return (p - m) / (2 * h)
+
+
+def test_free_flight_closed_form(oracle):
+    from avbd import scenes
+    from avbd.state import default_params
+    st = scenes.build_state(scenes.stacks(1, seed=2), oracle.OracleMaker())
+    st.bodies["pos"][0, 2] += 5.0
+    prm = default_params()
+    u = np.zeros((2, 3))
+    u[0, 2] = 1.0
+    for fp64 in (False, True):
+        _, tan = oracle.oracle_tangent_run(st, prm, 10, dvel=u, fp64=fp64)
+        # warm start x1 = x0 + v dt (+ gravity weighted), finishStep v1 = (x1 - x0)/dt: z moves v dt per step
+        assert tan[0, 2] == pytest.approx(10 * float(prm["dt"]), rel=1e-9)
+        assert tan[0, 9] == pytest.approx(1.0, rel=1e-9)
+
+
+def _slider(oracle):
+    """One box sliding at 3 m/s on the ground with a spin: a contact that stays sliding."""
+    from avbd import scenes
+    from avbd.state import default_params
+    st = scenes.build_state(scenes.stacks(1, seed=4), oracle.OracleMaker())
+    st.bodies["vel"][0] = (3.0, 0.5, 0.0)
+    st.bodies["ang_vel"][0] = (0.0, 0.0, 0.8)
+    o = oracle.OracleWorld(st, default_params())
+    o.step(3)
+    return o.state(), default_params()
+
+
+def _pile(oracle):
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scenes.pile(30, seed=5)
+    return scenes.build_state(sc, oracle.OracleMaker()), default_params(gravity=sc.gravity)
+
+
+# (case, steps, kind): each a direction along which the FP64 trajectory is smooth within +-h.
+CASES = [("pile30", 40, k) for k in ("dvel", "dangvel", "dmass", "dfriction")] + \
+        [("pile30", 120, k) for k in ("dvel", "dmass", "dfriction")] + \
+        [("slider", 30, k) for k in ("dvel", "dangvel", "dmass", "dfriction")]
+
+
+@pytest.mark.parametrize("case,steps,kind", CASES)
+def test_fp64_tangent_matches_central_differences(oracle, case, steps, kind):
+    st, prm = _pile(oracle) if case == "pile30" else _slider(oracle)
+    nb = len(st.bodies)
+    dyn = st.bodies["mass"] > 0
+    rng = np.random.default_rng(1)
+    u = {"dvel": rng.normal(size=(nb, 3)) * dyn[:, None], "dangvel": rng.normal(size=(nb, 3)) * dyn[:, None],
+         "dmass": rng.normal(size=nb) * dyn * st.bodies["mass"], "dfriction": rng.normal(size=nb) * 0.1}[kind]
+    base = {"dvel": st.bodies["vel"], "dangvel": st.bodies["ang_vel"], "dmass": st.bodies["mass"],
+            "dfriction": st.bodies["friction"]}[kind].astype(np.float64)
+    _, t64 = oracle.oracle_tangent_run(st, prm, steps, fp64=True, **{kind: u})
+    fd = _fd(oracle, st, prm, steps, {"dvel": "vel", "dangvel": "angvel", "dmass": "mass",
+                                      "dfriction": "friction"}[kind], base, u, 1e-6)
+    a, b = t64[dyn, 0:7], fd[dyn, 0:7]
+    rel = np.abs(a - b).max() / max(np.abs(b).max(), 1e-12)
+    assert rel < 1e-4, f"{case} {steps} {kind}: FP64 tangent vs central differences {rel:.2e}"
+
+
+def test_fp32_branch_tangent_follows_the_fp32_trajectory(oracle):
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scenes.pile(30, seed=5)
+    st = scenes.build_state(sc, oracle.OracleMaker())
+    prm = default_params(gravity=sc.gravity)
+    val, _ = oracle.oracle_tangent_run(st, prm, 90, dvel=np.ones((len(st.bodies), 3)))
+    w = oracle.OracleWorld(st, prm)
+    w.step(90)
+    b = w.state().bodies
+    np.testing.assert_array_equal(val[:, 0:3].astype(np.float32), b["pos"])
+    np.testing.assert_array_equal(val[:, 3:7].astype(np.float32), b["rot"])
+    np.testing.assert_array_equal(val[:, 7:10].astype(np.float32), b["vel"])
