@@ -459,7 +459,7 @@ __device__ __forceinline__ void cout_store(const COut &o, int a, int b, int sl, 
 // Dual update of one contact, reversed (manifold.cpp:231-280). Poses after this iteration's
 // primal (tapeCur), penalty/lambda before the dual (tapeLamPrev).
 __global__ void k_adj_dual(Bodies B, Manifolds M, const int4 *contactRec, int nc, const float4 *tapeCur,
-                           const float4 *tapeLamPrev, Params P, AdjBuf A)
+                           const float4 *tapeLamPrev, const unsigned char *flags, Params P, AdjBuf A)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nc)
@@ -472,13 +472,11 @@ __global__ void k_adj_dual(Bodies B, Manifolds M, const int4 *contactRec, int nc
     float4 iA = B.pose[4 * a + P_ILIN], jA = B.pose[4 * a + P_IANG], iB = B.pose[4 * b + P_ILIN],
            jB = B.pose[4 * b + P_IANG];
     float oma = 1.0f - P.alpha;
-    V3 b0, b1, b2, ja0, ja1, ja2, jb0, jb1, jb2;
-    ContactEval ev = c_eval32(cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma, b0, b1, b2, ja0, ja1, ja2, jb0,
-                              jb1, jb2);
+    const int fl = flags[t];
     CIn in;
     cin_load(in, cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma);
-    in.clamp0 = eval_clamp0(ev, pen4, lam4);
-    in.scaled = eval_scaled(ev);
+    in.clamp0 = (fl & kFlagClamp0) != 0;
+    in.scaled = (fl & kFlagScaled) != 0;
     CFwd f;
     cfwd(in, f);
     // adjoints of the updated lambda and penalty
@@ -486,15 +484,13 @@ __global__ void k_adj_dual(Bodies B, Manifolds M, const int4 *contactRec, int nc
     double Fb[3] = {lbn.x, lbn.y, lbn.z};
     double Cx[3] = {0.0, 0.0, 0.0};
     double kold[3] = {kbn.x, kbn.y, kbn.z};
-    const float cf[3] = {ev.C0, ev.C1, ev.C2};
-    const float pk[3] = {pen4.x, pen4.y, pen4.z};
     const double beta = P.betaLin;
     for (int k = 0; k < 3; ++k)
     {
-        bool upd = k == 0 ? ev.F0 < 0.0f : ev.fs <= ev.bounds;
+        bool upd = (fl & (k == 0 ? kFlagUpd0 : kFlagUpd12)) != 0;
         if (!upd)
             continue;
-        bool sat = !(pk[k] + P.betaLin * fabsf(cf[k]) < kPenaltyMax);
+        bool sat = (fl & (kFlagSat0 << k)) != 0;
         double kb = k == 0 ? kbn.x : (k == 1 ? kbn.y : kbn.z);
         if (sat)
             kold[k] = 0.0;
@@ -541,20 +537,16 @@ __device__ __forceinline__ void wave_sum_group(double &v)
 __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds M, const int4 *stamps,
                                                             const int4 *tasks, int begin, int end, const int *color,
                                                             int col, const float4 *tapePrev, const float4 *tapeCur,
-                                                            const float4 *tapeLamPrev, Params P, AdjBuf A)
+                                                            const float4 *tapeLamPrev, const float4 *sysTape,
+                                                            const unsigned char *stampFlags, Params P, AdjBuf A)
 {
-    constexpr int L = kAdjLanes, G = kAdjThreads / kAdjLanes, E = (kRec + L - 1) / L, kRow = kRec + 2;
-    __shared__ float rec[kAdjThreads * kRow];
+    constexpr int L = kAdjLanes, G = kAdjThreads / kAdjLanes;
     __shared__ AdjBodyShared sh[G];
     const int lane = threadIdx.x % L, g = threadIdx.x / L;
     const int k = begin + blockIdx.x * G + g;
     const bool valid = k < end;
     const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
     const int i = task.x, s0 = task.y, ns = valid ? task.z : 0;
-    int rounds = (ns + L - 1) / L;
-#pragma unroll
-    for (int off = 32; off >= L; off >>= 1)
-        rounds = max(rounds, __shfl_xor(rounds, off, 64));
     auto seen = [&](int j) -> const float4 * {
         int cj = color[j];
         return (j != i && cj >= 0 && cj < col) ? tapeCur : tapePrev;
@@ -563,92 +555,24 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
     float4 mm = B.massMoment[i];
     float4 pos4 = tapePrev[2 * i], rot4 = tapePrev[2 * i + 1];
     V3 pos = ld3(pos4);
-    Q4 rot = ldq(rot4);
-    float dt2 = P.dt * P.dt;
-    float ml = mm.x / dt2, ax = mm.y / dt2, ay = mm.z / dt2, az = mm.w / dt2;
-    V3 dl = pos - ld3(B.inerLin[i]);
-    V3 da = qdiff(rot, ldq(B.inerAng[i]));
-    float acc[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-        acc[e] = lane + e * L < kRec ? init_entry(lane + e * L, ml, ax, ay, az, dl, da) : 0.0f;
-    // ---- phase 1: the step's FP32 system ----
-    for (int r = 0; r < rounds; ++r)
-    {
-        int t = r * L + lane;
-        float *row = rec + (g * L + lane) * kRow;
-        if (valid && t < ns)
-        {
-            int4 st = stamps[s0 + t];
-            bool isA = (st.y >> 16) != 0;
-            int partner = st.z, sl = st.w;
-            int a = isA ? i : partner, b = isA ? partner : i;
-            const float4 *cr = M.cr + (size_t)sl * kCR;
-            const float4 *sa = a == i ? tapePrev : seen(a), *sb = b == i ? tapePrev : seen(b);
-            V3 b0, b1, b2, ja0, ja1, ja2, jb0, jb1, jb2;
-            float4 pen4 = tapeLamPrev[2 * sl];
-            ContactEval ev = c_eval32(cr, pen4, tapeLamPrev[2 * sl + 1], sa[2 * a], sa[2 * a + 1], sb[2 * b],
-                                      sb[2 * b + 1], B.pose[4 * a + P_ILIN], B.pose[4 * a + P_IANG],
-                                      B.pose[4 * b + P_ILIN], B.pose[4 * b + P_IANG], oma, b0, b1, b2, ja0, ja1, ja2,
-                                      jb0, jb1, jb2);
-            if (isA)
-                rows_record(row, b0, b1, b2, ja0, ja1, ja2, ld3(pen4), ev.F0, ev.F1, ev.F2);
-            else
-                rows_record(row, -b0, -b1, -b2, jb0, jb1, jb2, ld3(pen4), ev.F0, ev.F1, ev.F2);
-        }
-        else
-        {
-#pragma unroll
-            for (int q = 0; q < kRec; ++q)
-                row[q] = 0.0f;
-        }
-        wave_lds_sync();
-        int cnt = valid ? min(L, ns - r * L) : 0;
-        for (int q = 0; q < cnt; ++q)
-        {
-#pragma unroll
-            for (int e = 0; e < E; ++e)
-                if (lane + e * L < kRec)
-                    acc[e] += rec[(g * L + q) * kRow + lane + e * L];
-        }
-        wave_lds_sync();
-    }
-#pragma unroll
-    for (int e = 0; e < E; ++e)
-        if (lane + e * L < kRec)
-            rec[g * L * kRow + lane + e * L] = acc[e];
-    wave_lds_sync();
     // ---- lane 0: the step's dx, the FP64 solve of H w = dx_bar, inertia terms ----
     DV xb{0, 0, 0};
     DQ qb{0, 0, 0, 0};
     if (valid && lane == 0)
     {
-        const float *a = rec + g * L * kRow;
-        Sys6 sys;
-        sys.l00 = a[R_L00];
-        sys.l10 = a[R_L10];
-        sys.l11 = a[R_L11];
-        sys.l20 = a[R_L20];
-        sys.l21 = a[R_L21];
-        sys.l22 = a[R_L22];
-        sys.a00 = a[R_A00];
-        sys.a10 = a[R_A10];
-        sys.a11 = a[R_A11];
-        sys.a20 = a[R_A20];
-        sys.a21 = a[R_A21];
-        sys.a22 = a[R_A22];
-        sys.c00 = a[R_C + 0];
-        sys.c01 = a[R_C + 1];
-        sys.c02 = a[R_C + 2];
-        sys.c10 = a[R_C + 3];
-        sys.c11 = a[R_C + 4];
-        sys.c12 = a[R_C + 5];
-        sys.c20 = a[R_C + 6];
-        sys.c21 = a[R_C + 7];
-        sys.c22 = a[R_C + 8];
-        V3 rl{a[R_RL], a[R_RL + 1], a[R_RL + 2]}, ra{a[R_RA], a[R_RA + 1], a[R_RA + 2]};
-        V3 dxL, dxA;
-        solve6(sys, -rl, -ra, dxL, dxA);
+        // the step's accumulated system and dx, as its primal kernel taped them
+        float a[36];
+        const float4 *src = sysTape + 9 * (size_t)i;
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+        {
+            float4 v = src[q];
+            a[4 * q] = v.x;
+            a[4 * q + 1] = v.y;
+            a[4 * q + 2] = v.z;
+            a[4 * q + 3] = v.w;
+        }
+        V3 dxL{a[27], a[28], a[29]}, dxA{a[30], a[31], a[32]};
         // x' = x + dxL, q' = q (+) dxA
         DV xbo = ld_dv(A.xb + 3 * i);
         DQ qbo = ld_dq(A.qb + 4 * i);
@@ -656,6 +580,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
         DV dxAb{0, 0, 0};
         DQ q = dq4(rot4);
         dqint_adj(q, dv(dxA), qbo, qb, dxAb);
+        (void)oma;
         double H[6][6];
         const int lo[3][3] = {{R_L00, R_L10, R_L20}, {R_L10, R_L11, R_L21}, {R_L20, R_L21, R_L22}};
         const int an[3][3] = {{R_A00, R_A10, R_A20}, {R_A10, R_A11, R_A21}, {R_A20, R_A21, R_A22}};
@@ -713,9 +638,12 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
     DV rLb{S.rLb[0], S.rLb[1], S.rLb[2]}, rAb{S.rAb[0], S.rAb[1], S.rAb[2]};
     DV oxb{0, 0, 0};
     DQ oqb{0, 0, 0, 0};
+    DV oil{0, 0, 0};
+    DQ oia{0, 0, 0, 0};
     for (int t = lane; valid && t < ns; t += L)
     {
         int4 st = stamps[s0 + t];
+        const int fl = stampFlags[s0 + t];
         bool isA = (st.y >> 16) != 0;
         int partner = st.z, sl = st.w, m = adjIndex(st.x);
         int a = isA ? i : partner, b = isA ? partner : i;
@@ -725,13 +653,10 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
         float4 pA = sa[2 * a], qA = sa[2 * a + 1], pB = sb[2 * b], qB = sb[2 * b + 1];
         float4 iA = B.pose[4 * a + P_ILIN], jA = B.pose[4 * a + P_IANG], iB = B.pose[4 * b + P_ILIN],
                jB = B.pose[4 * b + P_IANG];
-        V3 b0, b1, b2, ja0, ja1, ja2, jb0, jb1, jb2;
-        ContactEval ev = c_eval32(cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma, b0, b1, b2, ja0, ja1, ja2,
-                                  jb0, jb1, jb2);
         CIn in;
         cin_load(in, cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma);
-        in.clamp0 = eval_clamp0(ev, pen4, lam4);
-        in.scaled = eval_scaled(ev);
+        in.clamp0 = (fl & kFlagClamp0) != 0;
+        in.scaled = (fl & kFlagScaled) != 0;
         CFwd f;
         cfwd(in, f);
         const double sgn = isA ? 1.0 : -1.0;
@@ -788,18 +713,23 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
         add_dv(A.lb + 3 * sl, o.lam);
         o.K = o.lam = DV{0, 0, 0};
         bool massA = B.massMoment[a].x > 0.0f, massB = B.massMoment[b].x > 0.0f;
-        // contacts of one manifold can sit on different lanes of this body: manifold sums are atomic
-        cout_store(o, a, b, sl, m, massA && a != i, massB && b != i, false, A);
+        // own init-pose adjoints stay in registers
         if (isA)
         {
-            atomic_add3(A.ilb + 3 * a, o.iLA);
-            atomic_add4(A.iab + 4 * a, o.iAA);
+            oil += o.iLA;
+            oia += o.iAA;
+            o.iLA = DV{0, 0, 0};
+            o.iAA = DQ{0, 0, 0, 0};
         }
         else
         {
-            atomic_add3(A.ilb + 3 * b, o.iLB);
-            atomic_add4(A.iab + 4 * b, o.iAB);
+            oil += o.iLB;
+            oia += o.iAB;
+            o.iLB = DV{0, 0, 0};
+            o.iAB = DQ{0, 0, 0, 0};
         }
+        // contacts of one manifold can sit on different lanes of this body: manifold sums are atomic
+        cout_store(o, a, b, sl, m, massA && a != i, massB && b != i, false, A);
     }
     wave_sum_group(oxb.x);
     wave_sum_group(oxb.y);
@@ -808,10 +738,21 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
     wave_sum_group(oqb.y);
     wave_sum_group(oqb.z);
     wave_sum_group(oqb.w);
+    wave_sum_group(oil.x);
+    wave_sum_group(oil.y);
+    wave_sum_group(oil.z);
+    wave_sum_group(oia.x);
+    wave_sum_group(oia.y);
+    wave_sum_group(oia.z);
+    wave_sum_group(oia.w);
     if (valid && lane == 0)
     {
         st_dv(A.xb + 3 * i, xb + oxb);
         st_dq(A.qb + 4 * i, qb + oqb);
+        // no other thread of this launch touches body i (neighbours have other colours)
+        add_dv(A.ilb + 3 * i, oil);
+        DQ ia = ld_dq(A.iab + 4 * i);
+        st_dq(A.iab + 4 * i, ia + oia);
     }
 }
 
@@ -1054,7 +995,8 @@ struct AdjointState
     int slotCap = 0;
     // tape of the step being reversed and the narrowphase/compaction recording (the handle's
     // pointers of the same names are set only while that step is recomputed)
-    float4 *tapePose = nullptr, *tapeContact = nullptr;
+    float4 *tapePose = nullptr, *tapeContact = nullptr, *tapeSys = nullptr;
+    unsigned char *tapeStampFlags = nullptr, *tapeDualFlags = nullptr;
     int *recMatch = nullptr, *recStick = nullptr, *recSrc = nullptr;
 };
 
@@ -1125,7 +1067,9 @@ int adj_alloc(avbd_handle *h, int steps, int every)
         S.slotCap = M * kMaxContacts;
         size_t C = (size_t)S.slotCap;
         AdjBuf &A = S.A;
-        if ((r = h->alloc(S.tapePose, (size_t)(I + 1) * 2 * N)) ||
+        if ((r = h->alloc(S.tapePose, (size_t)(I + 1) * 2 * N)) || (r = h->alloc(S.tapeSys, (size_t)I * 9 * N)) ||
+            (r = h->alloc(S.tapeStampFlags, (size_t)I * h->maxStamps)) ||
+            (r = h->alloc(S.tapeDualFlags, (size_t)I * C)) ||
             (r = h->alloc(S.tapeContact, (size_t)(I + 1) * 2 * C)) || (r = h->alloc(S.recMatch, C)) ||
             (r = h->alloc(S.recStick, C)) || (r = h->alloc(S.recSrc, M)) || (r = h->alloc(A.xb, 3 * (size_t)N)) ||
             (r = h->alloc(A.qb, 4 * (size_t)N)) || (r = h->alloc(A.vb, 3 * (size_t)N)) ||
@@ -1152,13 +1096,17 @@ int taped_recompute(avbd_handle *h)
     AdjointState &S = *h->adj;
     h->tapePose = S.tapePose;
     h->tapeContact = S.tapeContact;
+    h->tapeSys = S.tapeSys;
+    h->tapeStampFlags = S.tapeStampFlags;
+    h->tapeDualFlags = S.tapeDualFlags;
     h->recMatch = S.recMatch;
     h->recStick = S.recStick;
     h->recSrc = S.recSrc;
     int r = enqueue_prepare(h);
     if (!r)
         r = enqueue_iterate(h);
-    h->tapePose = h->tapeContact = nullptr;
+    h->tapePose = h->tapeContact = h->tapeSys = nullptr;
+    h->tapeStampFlags = h->tapeDualFlags = nullptr;
     h->recMatch = h->recStick = h->recSrc = nullptr;
     return r;
 }
@@ -1180,14 +1128,17 @@ int reverse_step(avbd_handle *h, const Snapshot &start)
     for (int it = I - 1; it >= 0; --it)
     {
         const float4 *prev = tp + it * ps, *cur = tp + (it + 1) * ps, *lam = tc + it * cs;
+        const float4 *sys = S.tapeSys + (size_t)it * 9 * h->maxBodies;
+        const unsigned char *sfl = S.tapeStampFlags + (size_t)it * h->maxStamps;
+        const unsigned char *dfl = S.tapeDualFlags + (size_t)it * S.slotCap;
         if (nc > 0)
-            k_adj_dual<<<ceil_div(nc, 128), 128, 0, s>>>(h->B, h->cur, h->S.contactRec, nc, cur, lam, h->P, A);
+            k_adj_dual<<<ceil_div(nc, 64), 64, 0, s>>>(h->B, h->cur, h->S.contactRec, nc, cur, lam, dfl, h->P, A);
         for (int col = colors - 1; col >= 0; --col)
         {
             int b = h->hostOffsets[col], e = h->hostOffsets[col + 1];
             if (e > b)
                 k_adj_primal<<<ceil_div(e - b, kAdjThreads / kAdjLanes), kAdjThreads, 0, s>>>(
-                    h->B, h->cur, h->S.stamps, h->S.tasks, b, e, h->S.color, col, prev, cur, lam, h->P, A);
+                    h->B, h->cur, h->S.stamps, h->S.tasks, b, e, h->S.color, col, prev, cur, lam, sys, sfl, h->P, A);
         }
     }
     k_adj_warmstart<<<ceil_div(nb, 256), 256, 0, s>>>(h->B, h->P, h->S.deg, nb, A);

@@ -95,7 +95,8 @@ struct avbd_handle
     // adjoint: recording outputs of the narrowphase/compaction (null unless a step is being taped),
     // the tape of the step being reversed and the rest of the adjoint state (avbd_adjoint.hip)
     int *recMatch = nullptr, *recStick = nullptr, *recSrc = nullptr;
-    float4 *tapePose = nullptr, *tapeContact = nullptr;
+    float4 *tapePose = nullptr, *tapeContact = nullptr, *tapeSys = nullptr;
+    unsigned char *tapeStampFlags = nullptr, *tapeDualFlags = nullptr;
     struct AdjointState *adj = nullptr;
     int lastNumManifolds = 0;
 
@@ -142,7 +143,7 @@ int alloc_manifolds(avbd_handle *h, Manifolds &m, int cap)
 int grid_for(int n) { return std::max(1, std::min(ceil_div(n, 256), 4096)); }
 
 using PrimalKernel = void (*)(Bodies, Manifolds, const Joint *, const Spring *, const int4 *, const int4 *, int, int,
-                              Params);
+                              Params, float4 *, unsigned char *);
 
 // The primal kernel's launch shapes: lanes per body x threads per block.
 PrimalKernel primal_kernel(int lanes, int threads)
@@ -181,8 +182,10 @@ struct PrimalArgs
     const int4 *tasks;
     int begin, end;
     Params P;
-    void *ptr[9];
-    PrimalArgs(const avbd_handle *h, int b, int e);
+    float4 *sysTape;
+    unsigned char *flagTape;
+    void *ptr[11];
+    PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys = nullptr, unsigned char *flags = nullptr);
 };
 
 struct DualArgs
@@ -194,8 +197,9 @@ struct DualArgs
     const int4 *contactRec;
     int nc;
     Params P;
-    void *ptr[7];
-    DualArgs(const avbd_handle *h, int nc_);
+    unsigned char *flagTape;
+    void *ptr[8];
+    DualArgs(const avbd_handle *h, int nc_, unsigned char *flags = nullptr);
 };
 
 // Colouring runs as one cooperative launch of a few large blocks: every round ends in a grid
@@ -413,18 +417,19 @@ int enqueue_prepare(avbd_handle *h)
     return AVBD_OK;
 }
 
-PrimalArgs::PrimalArgs(const avbd_handle *h, int b, int e)
+PrimalArgs::PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys, unsigned char *flags)
     : B(h->B), M(h->cur), joints(h->joints), springs(h->springs), stamps(h->S.stamps), tasks(h->S.tasks), begin(b),
-      end(e), P(h->P)
+      end(e), P(h->P), sysTape(sys), flagTape(flags)
 {
-    void *p[9] = {&B, &M, &joints, &springs, &stamps, &tasks, &begin, &end, &P};
+    void *p[11] = {&B, &M, &joints, &springs, &stamps, &tasks, &begin, &end, &P, &sysTape, &flagTape};
     std::memcpy(ptr, p, sizeof(p));
 }
 
-DualArgs::DualArgs(const avbd_handle *h, int nc_)
-    : B(h->B), M(h->cur), joints(h->joints), nj(h->nj), contactRec(h->S.contactRec), nc(nc_), P(h->P)
+DualArgs::DualArgs(const avbd_handle *h, int nc_, unsigned char *flags)
+    : B(h->B), M(h->cur), joints(h->joints), nj(h->nj), contactRec(h->S.contactRec), nc(nc_), P(h->P),
+      flagTape(flags)
 {
-    void *p[7] = {&B, &M, &joints, &nj, &contactRec, &nc, &P};
+    void *p[8] = {&B, &M, &joints, &nj, &contactRec, &nc, &P, &flagTape};
     std::memcpy(ptr, p, sizeof(p));
 }
 
@@ -573,9 +578,12 @@ int enqueue_iterate(avbd_handle *h)
         {
             if (mark && it == last)
                 HIP_TRY(hipEventRecord(mark[4], s));
+            float4 *sys = tape ? h->tapeSys + (size_t)it * 9 * h->maxBodies : nullptr;
+            unsigned char *sfl = tape ? h->tapeStampFlags + (size_t)it * h->maxStamps : nullptr;
+            unsigned char *dfl = tape ? h->tapeDualFlags + (size_t)it * kMaxContacts * h->maxManifolds : nullptr;
             for (const auto &rg : ranges)
             {
-                PrimalArgs a(h, rg.first, rg.second);
+                PrimalArgs a(h, rg.first, rg.second, sys, sfl);
                 HIP_TRY(hipLaunchKernel(pfn, dim3(ceil_div(rg.second - rg.first, groups)), dim3(lc.primal_threads),
                                         a.ptr, 0, s));
             }
@@ -583,7 +591,7 @@ int enqueue_iterate(avbd_handle *h)
                 HIP_TRY(hipEventRecord(mark[5], s));
             if (dualLanes > 0)
             {
-                DualArgs a(h, c.contactCount);
+                DualArgs a(h, c.contactCount, dfl);
                 HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(dualLanes, lc.dual_threads)),
                                         dim3(lc.dual_threads), a.ptr, 0, s));
             }

@@ -1268,8 +1268,16 @@ __device__ __forceinline__ void rows_record(float *r, V3 l0, V3 l1, V3 l2, V3 g0
 }
 
 // One contact of a manifold as seen by body i (Manifold::updatePrimal, manifold.cpp:176-229).
-__device__ __forceinline__ void contact_record(float *r, int i, int partner, bool isA, int sl, const Manifolds &M,
-                                               const Bodies &B, float oma)
+// Returns the contact's branch flags (kFlag*), which the adjoint's tape keeps.
+constexpr int kFlagClamp0 = 1, kFlagScaled = 2, kFlagUpd0 = 4, kFlagUpd12 = 8, kFlagSat0 = 16;
+__device__ __forceinline__ int contact_flags(const ContactEval &ev, V3 pen, V3 lam)
+{
+    float f0 = pen.x * ev.C0 + lam.x;
+    return (!(f0 < 0.0f) ? kFlagClamp0 : 0) | ((ev.fs > ev.bounds && ev.fs > 0.0f) ? kFlagScaled : 0);
+}
+
+__device__ __forceinline__ int contact_record(float *r, int i, int partner, bool isA, int sl, const Manifolds &M,
+                                              const Bodies &B, float oma)
 {
     int2 ab = isA ? make_int2(i, partner) : make_int2(partner, i);
     const float4 *cr = M.cr + (size_t)sl * kCR;
@@ -1295,6 +1303,7 @@ __device__ __forceinline__ void contact_record(float *r, int i, int partner, boo
         rows_record(r, b0, b1, b2, ja0, ja1, ja2, pen, ev.F0, ev.F1, ev.F2);
     else
         rows_record(r, nb0, nb1, nb2, jb0, jb1, jb2, pen, ev.F0, ev.F1, ev.F2);
+    return contact_flags(ev, pen, ld3(lam4));
 }
 
 // Dense 3x3 helpers for the joint stamp (joint.cpp:56-123), kept literal.
@@ -1583,9 +1592,12 @@ __device__ __forceinline__ void wave_lds_sync()
 // per block. Lane l computes the stamp records l, l+L, l+2L, ... of its body; the records of one
 // round are then added entry by entry in stamp order, so every (L, T) gives the same sums.
 // Bodies of a colour are sorted by stamp count, so the groups of a wave need similar round counts.
+// sysTape / flagTape (adjoint recording, null otherwise): per body the accumulated system and
+// dx (9 float4), per stamp the contact's branch flags.
 template <int L, int T>
 __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const Joint *joints, const Spring *springs,
-                                                    const int4 *stamps, const int4 *tasks, int begin, int end, Params P)
+                                                    const int4 *stamps, const int4 *tasks, int begin, int end, Params P,
+                                                    float4 *sysTape, unsigned char *flagTape)
 {
     static_assert(L >= 4 && L <= 64 && (64 % L) == 0 && (T % 64) == 0, "a body's lanes must sit in one wave");
     constexpr int G = T / L;               // bodies per block
@@ -1626,7 +1638,11 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
             int4 st = stamps[s0 + t];
             int kind = adjKind(st.x), idx = adjIndex(st.x);
             if (kind == kAdjManifold)
-                contact_record(row, i, st.z, (st.y >> 16) != 0, st.w, M, B, oma);
+            {
+                int fl = contact_record(row, i, st.z, (st.y >> 16) != 0, st.w, M, B, oma);
+                if (flagTape)
+                    flagTape[s0 + t] = (unsigned char)fl;
+            }
             else
             {
 #pragma unroll
@@ -1699,6 +1715,16 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
         solve6(sys, -rl, -ra, dxL, dxA);
         B.pose[4 * i + P_POS] = st3(pos + dxL, 0.0f);
         B.pose[4 * i + P_ROT] = stq(qintegrate(rot, dxA));
+        if (sysTape)
+        {
+            float4 *o = sysTape + 9 * (size_t)i;
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+                o[q] = make_float4(a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]);
+            o[6] = make_float4(a[24], a[25], a[26], dxL.x);
+            o[7] = make_float4(dxL.y, dxL.z, dxA.x, dxA.y);
+            o[8] = make_float4(dxA.z, 0.0f, 0.0f, 0.0f);
+        }
     }
 }
 
@@ -1717,7 +1743,7 @@ __global__ void k_contact_records(const int *mcount, const int2 *mab, const int 
 }
 
 // Manifold::updateDual for one contact (manifold.cpp:231-280)
-__device__ __forceinline__ void dual_contact(int4 cr, const Manifolds &M, const Bodies &B, const Params &P)
+__device__ __forceinline__ int dual_contact(int4 cr, const Manifolds &M, const Bodies &B, const Params &P)
 {
     int2 ab = make_int2(cr.z, cr.w);
     int sl = cr.x * kMaxContacts + cr.y;
@@ -1741,16 +1767,24 @@ __device__ __forceinline__ void dual_contact(int4 cr, const Manifolds &M, const 
     ContactEval ev = contact_eval(ld3(c04), pen, ld3(lam4), rA4.w, 1.0f - P.alpha, b0, b1, b2, nb0, nb1, nb2, ja0,
                                   ja1, ja2, jb0, jb1, jb2, dqAL, dqBL, dqAA, dqBA);
     float stick = lam4.w;
+    int fl = contact_flags(ev, pen, ld3(lam4));
     if (ev.F0 < 0.0f)
-        pen.x = fminr(pen.x + P.betaLin * fabsf(ev.C0), kPenaltyMax);
+    {
+        float k0 = pen.x + P.betaLin * fabsf(ev.C0);
+        fl |= kFlagUpd0 | (!(k0 < kPenaltyMax) ? kFlagSat0 : 0);
+        pen.x = fminr(k0, kPenaltyMax);
+    }
     if (ev.fs <= ev.bounds)
     {
-        pen.y = fminr(pen.y + P.betaLin * fabsf(ev.C1), kPenaltyMax);
-        pen.z = fminr(pen.z + P.betaLin * fabsf(ev.C2), kPenaltyMax);
+        float k1 = pen.y + P.betaLin * fabsf(ev.C1), k2 = pen.z + P.betaLin * fabsf(ev.C2);
+        fl |= kFlagUpd12 | (!(k1 < kPenaltyMax) ? kFlagSat0 << 1 : 0) | (!(k2 < kPenaltyMax) ? kFlagSat0 << 2 : 0);
+        pen.y = fminr(k1, kPenaltyMax);
+        pen.z = fminr(k2, kPenaltyMax);
         stick = sqrtf(ev.C1 * ev.C1 + ev.C2 * ev.C2) < kStickThresh ? 1.0f : 0.0f;
     }
     rec[CR_LAM] = make_float4(ev.F0, ev.F1, ev.F2, stick);
     rec[CR_PEN] = st3(pen, 0.0f);
+    return fl;
 }
 
 // Joint::updateDual (joint.cpp:125-182)
@@ -1804,11 +1838,17 @@ __device__ void dual_joint(Joint *jp, const Bodies &B, const Params &P)
 }
 
 // Dual pass of one iteration: one lane per contact, then one per joint.
-__global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int4 *contactRec, int nc, Params P)
+// flagTape (adjoint recording, may be null): per contact the dual's branch flags.
+__global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int4 *contactRec, int nc, Params P,
+                       unsigned char *flagTape)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nc)
-        dual_contact(contactRec[t], M, B, P);
+    {
+        int fl = dual_contact(contactRec[t], M, B, P);
+        if (flagTape)
+            flagTape[t] = (unsigned char)fl;
+    }
     else if (t - nc < nj)
         dual_joint(&joints[t - nc], B, P);
 }
