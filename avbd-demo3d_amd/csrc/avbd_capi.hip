@@ -86,7 +86,10 @@ struct avbd_handle
     int *worldStart = nullptr;
     std::vector<int> worldStartHost;
     // launch shapes of the iteration kernels, and the iteration graph built from them
-    avbd_launch_config launch{16, 256, 256, 0};
+    avbd_launch_config launch{16, 256, 256, 0, 0};
+    int *dfDone = nullptr;  // dataflow primal: last iteration id each body finished
+    int4 *dfTasks = nullptr;  // colour-sorted tasks, each colour padded to kDataflowPad
+    int dfIter = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     std::vector<hipGraphNode_t> graphPrimal;
@@ -164,11 +167,34 @@ PrimalKernel primal_kernel(int lanes, int threads)
     return nullptr;
 }
 
+using DataflowKernel = void (*)(Bodies, Manifolds, const int4 *, const int4 *, int, const int *, int *, int, Params,
+                                Counters *);
+
+PrimalKernel primal_kernel(int lanes, int threads);
+
+DataflowKernel dataflow_kernel(int lanes, int threads)
+{
+#define AVBD_DATAFLOW_SHAPE(l, t)                                                                          \
+    if (lanes == l && threads == t)                                                                        \
+        return &k_primal_dataflow<l, t>;
+    AVBD_DATAFLOW_SHAPE(8, 64)
+    AVBD_DATAFLOW_SHAPE(8, 128)
+    AVBD_DATAFLOW_SHAPE(8, 256)
+    AVBD_DATAFLOW_SHAPE(16, 64)
+    AVBD_DATAFLOW_SHAPE(16, 128)
+    AVBD_DATAFLOW_SHAPE(16, 256)
+    AVBD_DATAFLOW_SHAPE(32, 64)
+    AVBD_DATAFLOW_SHAPE(32, 128)
+    AVBD_DATAFLOW_SHAPE(32, 256)
+#undef AVBD_DATAFLOW_SHAPE
+    return nullptr;
+}
+
 bool valid_launch(const avbd_launch_config &c)
 {
     auto threads = [](int t) { return t == 64 || t == 128 || t == 256; };
     return primal_kernel(c.primal_lanes, c.primal_threads) != nullptr && threads(c.dual_threads) &&
-           (c.use_graph == 0 || c.use_graph == 1);
+           (c.use_graph == 0 || c.use_graph == 1) && (c.dataflow == 0 || c.dataflow == 1);
 }
 
 // Kernel argument blocks (hipLaunchKernel / graph kernel nodes take an array of argument pointers).
@@ -198,8 +224,9 @@ struct DualArgs
     int nc;
     Params P;
     unsigned char *flagTape;
-    void *ptr[8];
-    DualArgs(const avbd_handle *h, int nc_, unsigned char *flags = nullptr);
+    unsigned *claimReset;
+    void *ptr[9];
+    DualArgs(const avbd_handle *h, int nc_, unsigned char *flags = nullptr, unsigned *claim = nullptr);
 };
 
 // Colouring runs as one cooperative launch of a few large blocks: every round ends in a grid
@@ -210,6 +237,11 @@ constexpr int kColorThreads = 1024;
 void scan(avbd_handle *h, const int *in, int *out, const int *nptr, int nconst, int *total)
 {
     hipStream_t s = h->stream;
+    if (!nptr && nconst <= kScanSmallMax)
+    {
+        k_scan_small<<<1, 1024, 0, s>>>(in, out, nconst, total);
+        return;
+    }
     k_scan_reduce<<<kScanBlocks, 256, 0, s>>>(in, nptr, nconst, h->S.partial);
     k_scan_partials<<<1, kScanBlocks, 0, s>>>(h->S.partial, kScanBlocks, total);
     k_scan_down<<<kScanBlocks, 256, 0, s>>>(in, out, nptr, nconst, h->S.partial);
@@ -425,11 +457,11 @@ PrimalArgs::PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys, unsigned
     std::memcpy(ptr, p, sizeof(p));
 }
 
-DualArgs::DualArgs(const avbd_handle *h, int nc_, unsigned char *flags)
+DualArgs::DualArgs(const avbd_handle *h, int nc_, unsigned char *flags, unsigned *claim)
     : B(h->B), M(h->cur), joints(h->joints), nj(h->nj), contactRec(h->S.contactRec), nc(nc_), P(h->P),
-      flagTape(flags)
+      flagTape(flags), claimReset(claim)
 {
-    void *p[8] = {&B, &M, &joints, &nj, &contactRec, &nc, &P, &flagTape};
+    void *p[9] = {&B, &M, &joints, &nj, &contactRec, &nc, &P, &flagTape, &claimReset};
     std::memcpy(ptr, p, sizeof(p));
 }
 
@@ -574,6 +606,19 @@ int enqueue_iterate(avbd_handle *h)
         const void *pfn = (const void *)primal_kernel(lc.primal_lanes, lc.primal_threads);
         const int groups = lc.primal_threads / lc.primal_lanes;
         const int last = h->P.iterations - 1;
+        // dataflow form: one primal launch per iteration (contact-only worlds, not while taping)
+        const bool dataflow = lc.dataflow && h->nj == 0 && h->ns == 0 && !tape && primalPerIter > 0;
+        const int numTasks = h->hostOffsets[colors];
+        int numPadded = 0;
+        for (int col = 0; col < colors; ++col)
+            numPadded += ceil_div(h->hostOffsets[col + 1] - h->hostOffsets[col], kDataflowPad) * kDataflowPad;
+        const int dfBlocks = std::max(1, std::min(ceil_div(numPadded, groups), 2048));
+        if (dataflow)
+        {
+            k_tasks_pad_clear<<<grid_for(numPadded), 256, 0, s>>>(h->dfTasks, numPadded);
+            k_tasks_padded_fill<<<ceil_div(numTasks, 256), 256, 0, s>>>(h->S.tasks, numTasks, h->S.colorOffsets,
+                                                                        colors, h->dfTasks);
+        }
         for (int it = 0; it < h->P.iterations; ++it)
         {
             if (mark && it == last)
@@ -581,17 +626,26 @@ int enqueue_iterate(avbd_handle *h)
             float4 *sys = tape ? h->tapeSys + (size_t)it * 9 * h->maxBodies : nullptr;
             unsigned char *sfl = tape ? h->tapeStampFlags + (size_t)it * h->maxStamps : nullptr;
             unsigned char *dfl = tape ? h->tapeDualFlags + (size_t)it * kMaxContacts * h->maxManifolds : nullptr;
-            for (const auto &rg : ranges)
+            if (dataflow)
             {
-                PrimalArgs a(h, rg.first, rg.second, sys, sfl);
-                HIP_TRY(hipLaunchKernel(pfn, dim3(ceil_div(rg.second - rg.first, groups)), dim3(lc.primal_threads),
-                                        a.ptr, 0, s));
+                const int iter = ++h->dfIter;
+                dataflow_kernel(lc.primal_lanes, lc.primal_threads)<<<dfBlocks, lc.primal_threads, 0, s>>>(
+                    h->B, h->cur, h->S.stamps, h->dfTasks, numPadded, h->S.color, h->dfDone, iter, h->P, h->ctr);
+                if (dualLanes == 0)
+                    k_claim_reset<<<1, 1, 0, s>>>(h->ctr);
             }
+            else
+                for (const auto &rg : ranges)
+                {
+                    PrimalArgs a(h, rg.first, rg.second, sys, sfl);
+                    HIP_TRY(hipLaunchKernel(pfn, dim3(ceil_div(rg.second - rg.first, groups)),
+                                            dim3(lc.primal_threads), a.ptr, 0, s));
+                }
             if (mark && it == last)
                 HIP_TRY(hipEventRecord(mark[5], s));
             if (dualLanes > 0)
             {
-                DualArgs a(h, c.contactCount, dfl);
+                DualArgs a(h, c.contactCount, dfl, dataflow ? &h->ctr->dfClaim : nullptr);
                 HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(dualLanes, lc.dual_threads)),
                                         dim3(lc.dual_threads), a.ptr, 0, s));
             }
@@ -801,7 +855,8 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         (r = h->alloc(S.adjCursor, N)) || (r = h->alloc(S.adj, h->maxAdj)) || (r = h->alloc(S.adjSerial, h->maxAdj)) ||
         (r = h->alloc(S.color, N)) || (r = h->alloc(S.colorOffsets, 257)) || (r = h->alloc(S.colorBodies, N)) ||
         (r = h->alloc(S.partial, kScanBlocks + 1)) || (r = h->alloc(h->ctr, 1)) || (r = h->alloc(h->gb, 1)) ||
-        (r = h->alloc(h->adjTotal, 1)) || (r = h->alloc(h->worldStart, (size_t)N + 1)))
+        (r = h->alloc(h->adjTotal, 1)) || (r = h->alloc(h->worldStart, (size_t)N + 1)) || (r = h->alloc(h->dfDone, N)) ||
+        (r = h->alloc(h->dfTasks, (size_t)N + (size_t)kDataflowPad * kMaxColors)))
         return bail(r);
     S.hashMask = h->hashSize - 1;
     if ((e = hipHostMalloc((void **)&h->hostCtr, sizeof(Counters))) != hipSuccess ||
@@ -809,7 +864,8 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         return bail(fail(AVBD_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e)));
     if ((e = hipMemsetAsync(h->ctr, 0, sizeof(Counters), h->stream)) != hipSuccess ||
         (e = hipMemsetAsync(h->gb, 0, sizeof(GridBarrier), h->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(S.color, 0xFF, sizeof(int) * N, h->stream)) != hipSuccess)
+        (e = hipMemsetAsync(S.color, 0xFF, sizeof(int) * N, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(h->dfDone, 0, sizeof(int) * N, h->stream)) != hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e)));
     // cooperative grid sizes: co-resident blocks of 256 threads
     hipDeviceProp_t prop;

@@ -159,6 +159,36 @@ __global__ void k_scan_down(const int *in, int *out, const int *nptr, int nconst
     }
 }
 
+// Exclusive scan of up to kScanSmallMax ints in one block of 1024 threads: tiles of 4096
+// (4 consecutive elements per thread, coalesced), each scanned across the block and carried
+// into the next. One launch instead of three for the small histogram scans.
+constexpr int kScanSmallMax = 8192;
+__global__ __launch_bounds__(1024) void k_scan_small(const int *in, int *out, int n, int *total)
+{
+    __shared__ int lds[16];
+    int carry = 0;
+    for (int base = 0; base < n; base += 4 * 1024)
+    {
+        int i0 = base + 4 * (int)threadIdx.x;
+        int v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            v[q] = i0 + q < n ? in[i0 + q] : 0;
+        int tot;
+        int run = carry + block_exclusive_scan((v[0] + v[1]) + (v[2] + v[3]), lds, tot);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+        {
+            if (i0 + q < n)
+                out[i0 + q] = run;
+            run += v[q];
+        }
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total)
+        *total = carry;
+}
+
 // ------------------------------------------------------------------------------------------
 // stable LSD radix sort, 8 bits per pass, wave-ballot ranking
 // ------------------------------------------------------------------------------------------
@@ -248,6 +278,7 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 // Per-step counter reset (a kernel instead of a memset so the step can be graph-captured).
 __global__ void k_step_begin(Counters *c)
 {
+    c->dfClaim = 0u;
     c->numChunks = 0;
     c->numHits = 0;
     c->numCreated = 0;
@@ -1276,6 +1307,32 @@ __device__ __forceinline__ int contact_flags(const ContactEval &ev, V3 pen, V3 l
     return (!(f0 < 0.0f) ? kFlagClamp0 : 0) | ((ev.fs > ev.bounds && ev.fs > 0.0f) ? kFlagScaled : 0);
 }
 
+// Agent-scope (device-coherent across the XCDs' L2s) 16-byte load / store of a pose row, as two
+// 64-bit relaxed atomics: the dataflow primal reads poses other workgroups wrote in the same launch.
+__device__ __forceinline__ float4 ld_coherent(const float4 *p)
+{
+    unsigned long long *q = reinterpret_cast<unsigned long long *>(const_cast<float4 *>(p));
+    unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    float4 r;
+    r.x = __uint_as_float((unsigned)a);
+    r.y = __uint_as_float((unsigned)(a >> 32));
+    r.z = __uint_as_float((unsigned)b);
+    r.w = __uint_as_float((unsigned)(b >> 32));
+    return r;
+}
+__device__ __forceinline__ void st_coherent(float4 *p, float4 v)
+{
+    unsigned long long *q = reinterpret_cast<unsigned long long *>(p);
+    unsigned long long a = (unsigned long long)__float_as_uint(v.x) | ((unsigned long long)__float_as_uint(v.y) << 32);
+    unsigned long long b = (unsigned long long)__float_as_uint(v.z) | ((unsigned long long)__float_as_uint(v.w) << 32);
+    __hip_atomic_store(q, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Co: the partner's current pose is read coherently (dataflow primal); the own body's rows and
+// the constant rows (x-) are ordinary loads.
+template <bool Co = false>
 __device__ __forceinline__ int contact_record(float *r, int i, int partner, bool isA, int sl, const Manifolds &M,
                                               const Bodies &B, float oma)
 {
@@ -1283,7 +1340,8 @@ __device__ __forceinline__ int contact_record(float *r, int i, int partner, bool
     const float4 *cr = M.cr + (size_t)sl * kCR;
     float4 rA4 = cr[CR_RA], rB4 = cr[CR_RB], c04 = cr[CR_C0], bb0 = cr[CR_N], bb1 = cr[CR_T1], bb2 = cr[CR_T2];
     float4 pen4 = cr[CR_PEN], lam4 = cr[CR_LAM];
-    float4 pA4 = B.pose[4 * (ab.x) + P_POS], pB4 = B.pose[4 * (ab.y) + P_POS], qA4 = B.pose[4 * (ab.x) + P_ROT], qB4 = B.pose[4 * (ab.y) + P_ROT];
+    auto cur = [&](int b, int row) { return (Co && b == partner) ? ld_coherent(&B.pose[4 * b + row]) : B.pose[4 * b + row]; };
+    float4 pA4 = cur(ab.x, P_POS), pB4 = cur(ab.y, P_POS), qA4 = cur(ab.x, P_ROT), qB4 = cur(ab.y, P_ROT);
     float4 iA4 = B.pose[4 * (ab.x) + P_ILIN], iB4 = B.pose[4 * (ab.y) + P_ILIN], jA4 = B.pose[4 * (ab.x) + P_IANG], jB4 = B.pose[4 * (ab.y) + P_IANG];
     Q4 qA = ldq(qA4), qB = ldq(qB4);
     V3 dqAL = ld3(pA4) - ld3(iA4);
@@ -1594,20 +1652,20 @@ __device__ __forceinline__ void wave_lds_sync()
 // Bodies of a colour are sorted by stamp count, so the groups of a wave need similar round counts.
 // sysTape / flagTape (adjoint recording, null otherwise): per body the accumulated system and
 // dx (9 float4), per stamp the contact's branch flags.
-template <int L, int T>
-__global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const Joint *joints, const Spring *springs,
-                                                    const int4 *stamps, const int4 *tasks, int begin, int end, Params P,
-                                                    float4 *sysTape, unsigned char *flagTape)
+// One colour-sorted task (a body and its stamps) of the primal, for L lanes per body and T
+// threads per block: the stamps' 27-float records go straight into the lanes' LDS rows (odd
+// stride), the owner lanes add each system entry in stamp order, lane 0 solves and stores.
+// Every (L, T) gives the same sums. Co: the dataflow form (coherent partner poses and pose
+// stores).
+template <int L, int T, bool Co>
+__device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool valid, int4 task, const Bodies &B,
+                                            const Manifolds &M, const Joint *joints, const Spring *springs,
+                                            const int4 *stamps, const Params &P, float4 *sysTape,
+                                            unsigned char *flagTape)
 {
     static_assert(L >= 4 && L <= 64 && (64 % L) == 0 && (T % 64) == 0, "a body's lanes must sit in one wave");
-    constexpr int G = T / L;               // bodies per block
     constexpr int E = (kRec + L - 1) / L;  // accumulated entries per lane
     constexpr int kRow = kRec + 2;         // odd row stride: a wave's row writes hit distinct banks
-    __shared__ float rec[T * kRow];
-    const int lane = threadIdx.x % L, g = threadIdx.x / L;
-    const int k = begin + blockIdx.x * G + g;
-    const bool valid = k < end;
-    const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
     const int i = task.x;
     const int s0 = task.y;
     const int ns = valid ? task.z : 0;
@@ -1639,7 +1697,7 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
             int kind = adjKind(st.x), idx = adjIndex(st.x);
             if (kind == kAdjManifold)
             {
-                int fl = contact_record(row, i, st.z, (st.y >> 16) != 0, st.w, M, B, oma);
+                int fl = contact_record<Co>(row, i, st.z, (st.y >> 16) != 0, st.w, M, B, oma);
                 if (flagTape)
                     flagTape[s0 + t] = (unsigned char)fl;
             }
@@ -1713,8 +1771,17 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
         V3 rl{a[R_RL], a[R_RL + 1], a[R_RL + 2]}, ra{a[R_RA], a[R_RA + 1], a[R_RA + 2]};
         V3 dxL, dxA;
         solve6(sys, -rl, -ra, dxL, dxA);
-        B.pose[4 * i + P_POS] = st3(pos + dxL, 0.0f);
-        B.pose[4 * i + P_ROT] = stq(qintegrate(rot, dxA));
+        float4 np = st3(pos + dxL, 0.0f), nq = stq(qintegrate(rot, dxA));
+        if (Co)
+        {
+            st_coherent(&B.pose[4 * i + P_POS], np);
+            st_coherent(&B.pose[4 * i + P_ROT], nq);
+        }
+        else
+        {
+            B.pose[4 * i + P_POS] = np;
+            B.pose[4 * i + P_ROT] = nq;
+        }
         if (sysTape)
         {
             float4 *o = sysTape + 9 * (size_t)i;
@@ -1725,6 +1792,114 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
             o[7] = make_float4(dxL.y, dxL.z, dxA.x, dxA.y);
             o[8] = make_float4(dxA.z, 0.0f, 0.0f, 0.0f);
         }
+    }
+}
+
+// Primal update of the bodies of one colour (solver.cpp:1184-1240): one launch per colour.
+// Bodies of a colour are sorted by stamp count, so the groups of a wave need similar round counts.
+// sysTape / flagTape (adjoint recording, null otherwise): per body the accumulated system and
+// dx (9 float4), per stamp the contact's branch flags.
+template <int L, int T>
+__global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const Joint *joints, const Spring *springs,
+                                                    const int4 *stamps, const int4 *tasks, int begin, int end, Params P,
+                                                    float4 *sysTape, unsigned char *flagTape)
+{
+    constexpr int G = T / L;
+    __shared__ float rec[T * (kRec + 2)];
+    const int lane = threadIdx.x % L, g = threadIdx.x / L;
+    const int k = begin + blockIdx.x * G + g;
+    const bool valid = k < end;
+    const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
+    primal_task<L, T, false>(rec, g, lane, valid, task, B, M, joints, springs, stamps, P, sysTape, flagTape);
+}
+
+// Dataflow form of one iteration's primal (contact-only worlds): one launch covers all colours.
+// Each wave claims the next 64/L colour-sorted tasks; a body starts once every lower-colour
+// neighbour has published this iteration (done[j] == iter), so colours overlap instead of
+// waiting for kernel boundaries. Claiming in order keeps the wait deadlock-free: a body waits
+// only on bodies claimed before it, and the earliest unfinished one has nothing left to wait on.
+// Partner poses are read coherently; a body's new pose is stored coherently and its done flag
+// is published with release order after it. A bounded spin turns a missing flag into the
+// timeout error instead of a hang. Results equal the per-colour launches bit for bit.
+constexpr int kDataflowSpin = 1 << 18;
+constexpr int kDataflowPad = 8;  // a colour's tasks start on a multiple of this (the most groups a wave claims)
+
+// The colour-sorted tasks with every colour padded to a multiple of kDataflowPad (padding tasks
+// have body -1), so that one wave's claim never holds two colours: a body never waits on a
+// lower-colour neighbour its own wave processes in lockstep with it.
+__global__ void k_tasks_pad_clear(int4 *padded, int numPadded)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < numPadded; k += gridDim.x * blockDim.x)
+        padded[k] = make_int4(-1, 0, 0, 0);
+}
+
+__global__ void k_tasks_padded_fill(const int4 *tasks, int numTasks, const int *colorOffsets, int colors,
+                                    int4 *padded)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= numTasks)
+        return;
+    int shift = 0, c = 0;
+    for (; c < colors; ++c)
+    {
+        int b = colorOffsets[c], e = colorOffsets[c + 1];
+        if (k < e)
+            break;
+        int n = e - b;
+        shift += (n + kDataflowPad - 1) / kDataflowPad * kDataflowPad - n;
+    }
+    padded[k + shift] = tasks[k];
+}
+template <int L, int T>
+__global__ __launch_bounds__(T) void k_primal_dataflow(Bodies B, Manifolds M, const int4 *stamps, const int4 *tasks,
+                                                       int numTasks, const int *color, int *done, int iter, Params P,
+                                                       Counters *ctr)
+{
+    constexpr int G = T / L, WG = 64 / L;  // groups per block, per wave
+    __shared__ float rec[T * (kRec + 2)];
+    const int lane = threadIdx.x % L, g = threadIdx.x / L;
+    const int wl = threadIdx.x & 63;
+    for (;;)
+    {
+        int k0 = 0;
+        if (wl == 0)
+            k0 = (int)atomicAdd(&ctr->dfClaim, (unsigned)WG);
+        k0 = __shfl(k0, 0, 64);
+        if (k0 >= numTasks)
+            break;
+        const int k = k0 + (g % WG);
+        int4 task = k < numTasks ? tasks[k] : make_int4(-1, 0, 0, 0);
+        const bool valid = task.x >= 0;
+        if (!valid)
+            task = make_int4(0, 0, 0, 0);
+        // wait for the lower-colour neighbours of this body
+        if (valid)
+        {
+            const int c = color[task.x];
+            for (int t = lane; t < task.z; t += L)
+            {
+                int j = stamps[task.y + t].z;
+                int cj = j >= 0 ? color[j] : -1;
+                if (cj < 0 || cj >= c)
+                    continue;
+                int spins = 0;
+                while (__hip_atomic_load(&done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != iter)
+                {
+                    __builtin_amdgcn_s_sleep(1);
+                    // once any wait has timed out the results are void: stop waiting at once
+                    if (++spins >= kDataflowSpin ||
+                        ((spins & 1023) == 0 &&
+                         __hip_atomic_load(&ctr->barrierTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+                    {
+                        atomicOr(&ctr->barrierTimeout, 1);
+                        break;
+                    }
+                }
+            }
+        }
+        primal_task<L, T, true>(rec, g, lane, valid, task, B, M, nullptr, nullptr, stamps, P, nullptr, nullptr);
+        if (valid && lane == 0)
+            __hip_atomic_store(&done[task.x], iter, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1839,10 +2014,19 @@ __device__ void dual_joint(Joint *jp, const Bodies &B, const Params &P)
 
 // Dual pass of one iteration: one lane per contact, then one per joint.
 // flagTape (adjoint recording, may be null): per contact the dual's branch flags.
+__device__ __forceinline__ void ctr_reset_claim(unsigned *claim)
+{
+    if (claim)
+        *claim = 0u;
+}
+
+// claimReset (may be null): the dataflow primal's claim counter, cleared for the next iteration.
 __global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int4 *contactRec, int nc, Params P,
-                       unsigned char *flagTape)
+                       unsigned char *flagTape, unsigned *claimReset)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0)
+        ctr_reset_claim(claimReset);
     if (t < nc)
     {
         int fl = dual_contact(contactRec[t], M, B, P);
@@ -1876,6 +2060,8 @@ __global__ void k_tape_contacts(const int4 *contactRec, const Counters *ctr, con
     }
 }
 
+__global__ void k_claim_reset(Counters *c) { c->dfClaim = 0u; }
+
 // Solver::finishStep for boxes (solver.cpp:1275-1288).
 __global__ void k_finish(Bodies B, Params P, int n)
 {
@@ -1894,3 +2080,4 @@ __global__ void k_finish(Bodies B, Params P, int n)
 }
 
 } // namespace avbd
+

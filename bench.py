@@ -113,7 +113,7 @@ def main():
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
         print("bench.py measures one GPU: the AVBD step here runs on a single MI355X "
-              "(worlds shard without communication; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+              "(independent worlds shard across GPUs without communication). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

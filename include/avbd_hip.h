@@ -299,12 +299,15 @@ int avbd_batch_num_worlds(avbd_handle *h, int32_t *num_worlds);
  * No counterpart in the reference (its WebGPU lane uses fixed 64-thread
  * workgroups, avbd_gpu_solver.cpp:502-714). Every shape gives bit-identical
  * results; shapes differ only in speed. Defaults: 16 lanes, 256 threads,
- * dual 256 threads, no graph. */
+ * dual 256 threads, no graph, no dataflow. */
 typedef struct avbd_launch_config {
     int32_t primal_lanes;    /* lanes per body in the primal kernel: 8, 16 or 32 */
     int32_t primal_threads;  /* threads per primal block: 64, 128 or 256 */
     int32_t dual_threads;    /* threads per dual block: 64, 128 or 256 */
     int32_t use_graph;       /* 1: each solver iteration is one hipGraph launch */
+    int32_t dataflow;        /* 1: one primal launch per iteration for all colours, bodies
+                                starting as their lower-colour neighbours finish
+                                (contact-only worlds; otherwise ignored) */
 } avbd_launch_config;
 int avbd_set_launch_config(avbd_handle *h, const avbd_launch_config *cfg);
 int avbd_get_launch_config(avbd_handle *h, avbd_launch_config *out);

@@ -78,14 +78,39 @@ def test_every_launch_shape_gives_the_same_state(gpu):
     ref.step(40)
     base = ref.state()
     ref.close()
-    shapes = [(l, t, d, gr) for l in (8, 16, 32) for t in (64, 128, 256) for d, gr in ((64, 0), (256, 1))]
-    for l, t, d, gr in shapes:
+    shapes = [(l, t, d, gr, 0) for l in (8, 16, 32) for t in (64, 128, 256) for d, gr in ((64, 0), (256, 1))]
+    shapes += [(l, t, 128, 0, 1) for l, t in ((8, 128), (16, 64), (16, 256), (32, 256))]
+    for l, t, d, gr, df in shapes:
         w = gpu.HipWorld.for_state(st, prm)
-        w.set_launch_config(l, t, d, use_graph=gr)
-        assert w.launch_config() == {"primal_lanes": l, "primal_threads": t, "dual_threads": d, "use_graph": gr}
+        w.set_launch_config(l, t, d, use_graph=gr, dataflow=df)
+        assert w.launch_config() == {"primal_lanes": l, "primal_threads": t, "dual_threads": d, "use_graph": gr,
+                                     "dataflow": df}
         w.step(40)
-        assert_states_equal(w.state(), base, f"lanes {l} threads {t} dual {d} graph {gr}")
+        w.synchronize()
+        assert_states_equal(w.state(), base, f"lanes {l} threads {t} dual {d} graph {gr} dataflow {df}")
         w.close()
+
+
+def test_dataflow_primal_at_headline_size(gpu):
+    """The dataflow primal (all colours in one launch per iteration, bodies waiting on their
+    lower-colour neighbours) equals the per-colour launches bit for bit on the 50,000-box pile."""
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scenes.pile(50000)
+    prm = default_params(gravity=sc.gravity)
+    st = scenes.build_state(sc, gpu.Maker())
+    a = gpu.HipWorld.for_state(st, prm)
+    a.step(80)
+    mid = a.state()
+    a.step(3)
+    ref = a.state()
+    a.close()
+    b = gpu.HipWorld.for_state(mid, prm)
+    b.set_launch_config(16, 64, 128, use_graph=0, dataflow=1)
+    b.step(3)
+    b.synchronize()
+    assert_states_equal(b.state(), ref, "dataflow, 50k pile, steps 81-83")
+    b.close()
 
 
 def test_autotune_keeps_state_and_picks_a_valid_shape(gpu):
