@@ -55,6 +55,7 @@ def load(path: str | None = None):
         lib.avbd_debug_inject_fault.argtypes = [_p, C.c_int32]
         lib.avbd_default_params.argtypes = [_p]
         lib.avbd_make_box_body.argtypes = [_p, C.c_float, C.c_float, _p, _p, _p, _p, _p]
+        lib.avbd_make_round_body.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, _p, _p, _p, _p, _p]
         lib.avbd_make_joint.argtypes = [_p, _p, C.c_int32, C.c_int32, C.c_int64, _p, _p, C.c_float, C.c_float,
                                         C.c_float, _p]
         lib.avbd_make_spring.argtypes = [_p, _p, C.c_int32, C.c_int32, C.c_int64, _p, _p, C.c_float, C.c_float, _p]
@@ -90,6 +91,14 @@ class Maker:
         _check(lib.avbd_make_box_body(_ptr(_f32(size, 3)), float(density), float(friction), _ptr(_f32(pos, 3)),
                                       _ptr(_f32(rot, 4)), _ptr(_f32(vel, 3)), _ptr(_f32(angvel, 3)),
                                       out.ctypes.data_as(_p)))
+        return out
+
+    def make_round_body(self, shape, radius, half_length, density, friction, pos, rot, vel, angvel):
+        lib = load()
+        out = np.zeros((), BODY)
+        _check(lib.avbd_make_round_body(int(shape), float(radius), float(half_length), float(density), float(friction),
+                                        _ptr(_f32(pos, 3)), _ptr(_f32(rot, 4)), _ptr(_f32(vel, 3)),
+                                        _ptr(_f32(angvel, 3)), out.ctypes.data_as(_p)))
         return out
 
     def joint_spring_derived(self, rec_a, rec_b, ra, rb):

@@ -34,7 +34,7 @@ import math
 
 import numpy as np
 
-from .state import BODY, JOINT, SPRING, State
+from .state import BODY, JOINT, SPRING, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER, SHAPE_SPHERE, State
 
 FORCE_JOINT = 1
 FORCE_SPRING = 2
@@ -60,6 +60,17 @@ class Scene:
     p2: np.ndarray        # joint: fracture
     gravity: float = -9.81
     steps: int = 300
+    shape: np.ndarray | None = None         # (n,) RigidShapeType; None: all boxes
+    shape_radius: np.ndarray | None = None  # (n,) sphere/capsule/cylinder radius
+    half_length: np.ndarray | None = None   # (n,) capsule/cylinder core half length
+
+    def shape_columns(self):
+        """(shape, radius, half length) per body, boxes filled in when the scene has none."""
+        n = self.num_bodies
+        if self.shape is None:
+            return np.zeros(n, np.int32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+        return (np.asarray(self.shape, np.int32), np.asarray(self.shape_radius, np.float32),
+                np.asarray(self.half_length, np.float32))
 
     @property
     def num_bodies(self) -> int:
@@ -72,10 +83,12 @@ class Scene:
 
 class _Builder:
     def __init__(self):
-        self.b = {k: [] for k in ("size", "density", "friction", "pos", "rot", "vel", "angvel")}
+        self.b = {k: [] for k in ("size", "density", "friction", "pos", "rot", "vel", "angvel", "shape", "srad",
+                                  "hlen")}
         self.f = {k: [] for k in ("ftype", "fa", "fb", "ra", "rb", "p0", "p1", "p2")}
 
-    def body(self, size, density, friction, pos, rot=(0, 0, 0, 1), vel=(0, 0, 0), angvel=(0, 0, 0)) -> int:
+    def body(self, size, density, friction, pos, rot=(0, 0, 0, 1), vel=(0, 0, 0), angvel=(0, 0, 0),
+             shape=SHAPE_BOX, radius=0.0, half_length=0.0) -> int:
         self.b["size"].append(size)
         self.b["density"].append(density)
         self.b["friction"].append(friction)
@@ -83,7 +96,28 @@ class _Builder:
         self.b["rot"].append(rot)
         self.b["vel"].append(vel)
         self.b["angvel"].append(angvel)
+        self.b["shape"].append(shape)
+        self.b["srad"].append(radius)
+        self.b["hlen"].append(half_length)
         return len(self.b["size"]) - 1
+
+    # Rigid::makeSphere / makeCapsule / makeCylinder (rigid.cpp:44-86): the recorded size is the
+    # bounding box the constructors pass on to Rigid::Rigid.
+    def sphere(self, radius, density, friction, pos, rot=(0, 0, 0, 1), vel=(0, 0, 0), angvel=(0, 0, 0)) -> int:
+        d = np.float32(radius) * np.float32(2.0)
+        return self.body((d, d, d), density, friction, pos, rot, vel, angvel, SHAPE_SPHERE, radius, 0.0)
+
+    def capsule(self, radius, half_length, density, friction, pos, rot=(0, 0, 0, 1), vel=(0, 0, 0),
+                angvel=(0, 0, 0)) -> int:
+        d = np.float32(radius) * np.float32(2.0)
+        h = np.float32(half_length) * np.float32(2.0) + d
+        return self.body((d, d, h), density, friction, pos, rot, vel, angvel, SHAPE_CAPSULE, radius, half_length)
+
+    def cylinder(self, radius, half_length, density, friction, pos, rot=(0, 0, 0, 1), vel=(0, 0, 0),
+                 angvel=(0, 0, 0)) -> int:
+        d = np.float32(radius) * np.float32(2.0)
+        h = np.float32(half_length) * np.float32(2.0)
+        return self.body((d, d, h), density, friction, pos, rot, vel, angvel, SHAPE_CYLINDER, radius, half_length)
 
     def joint(self, a, b, ra, rb, k_lin=math.inf, k_ang=0.0, fracture=math.inf):
         for k, v in zip(self.f, (FORCE_JOINT, a, b, ra, rb, k_lin, k_ang, fracture)):
@@ -114,7 +148,10 @@ class _Builder:
             p0=np.asarray(self.f["p0"], f32).reshape(m),
             p1=np.asarray(self.f["p1"], f32).reshape(m),
             p2=np.asarray(self.f["p2"], f32).reshape(m),
-            gravity=gravity, steps=steps)
+            gravity=gravity, steps=steps,
+            shape=np.asarray(self.b["shape"], np.int32).reshape(n),
+            shape_radius=np.asarray(self.b["srad"], f32).reshape(n),
+            half_length=np.asarray(self.b["hlen"], f32).reshape(n))
 
 
 def _rng(seed: int) -> np.random.Generator:
@@ -224,6 +261,216 @@ def stacks(count: int, seed: int = 1, height: int = 10, jitter: float = 0.01, sp
 BATCH_KINDS = ("stacks", "pile", "chain")
 
 
+# ---- round shapes (Rigid::makeSphere / makeCapsule / makeCylinder) ----
+
+def _axis_quat(axis, angle):
+    """Unit quaternion (x, y, z, w) of a rotation by ``angle`` about a unit axis."""
+    s, c = math.sin(angle * 0.5), math.cos(angle * 0.5)
+    return (axis[0] * s, axis[1] * s, axis[2] * s, c)
+
+
+def _rotate(q, v):
+    x, y, z, w = q
+    u = np.array([x, y, z])
+    v = np.asarray(v, np.float64)
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+def round_stack(kind: str = "sphere", count: int = 10, steps: int = 300) -> Scene:
+    """A column of spheres, crosswise-lying capsules or standing cylinders on a static ground slab
+    (the slab is a thin static box, so the shapes meet its top through the thin-top paths)."""
+    sb = _Builder()
+    sb.body((1000, 1000, 1), 0.0, 0.5, (0, 0, 0))
+    for i in range(count):
+        if kind == "sphere":
+            r = 0.5
+            sb.sphere(r, 1.0, 0.5, (0, 0, 0.5 + r + i * r * 2.05))
+        elif kind == "capsule":
+            r, hl = 0.35, 0.9
+            q = _axis_quat((0, 1, 0) if i % 2 == 0 else (1, 0, 0), math.pi / 2)
+            sb.capsule(r, hl, 1.0, 0.5, (0, 0, 0.5 + r + i * r * 2.4), q)
+        elif kind == "cylinder":
+            r, hl = 0.48, 0.35
+            sb.cylinder(r, hl, 1.0, 0.6, (0, 0, 0.5 + hl + i * (hl * 2 + 0.025)))
+        else:
+            raise KeyError(kind)
+    return sb.scene(f"{kind}_stack{count}", -9.81, steps)
+
+
+def round_ramp(kind: str = "sphere", count: int = 6, angle_deg: float = 20.0, steps: int = 240) -> Scene:
+    """Round shapes released on a tilted static slab with frictions from 0.1 up (spheres roll and
+    slide, capsules lie along the slope, cylinders lie across it)."""
+    sb = _Builder()
+    sb.body((1000, 1000, 1), 0.0, 0.5, (0, 0, 0))
+    q = _axis_quat((0, 1, 0), math.radians(angle_deg))
+    sb.body((36, 16, 1), 0.0, 0.8, (0, 0, 3), q)
+    tangent, normal = _rotate(q, (1, 0, 0)), _rotate(q, (0, 0, 1))
+    for i in range(count):
+        fr = 0.1 + 0.15 * i
+        if kind == "sphere":
+            r, rot = 0.55, (0, 0, 0, 1)
+        elif kind == "capsule":
+            r, rot = 0.35, _axis_quat((0, 1, 0), math.pi / 2)
+        else:
+            r, rot = 0.48, _axis_quat((1, 0, 0), -math.pi / 2)
+        pos = np.array([0, 0, 3.0]) + tangent * (-12.0 + i * 2.0) + np.array([0, -3.0 + i, 0]) + normal * (0.5 + r + 0.02)
+        if kind == "sphere":
+            sb.sphere(r, 1.0, fr, tuple(pos), rot)
+        elif kind == "capsule":
+            sb.capsule(r, 0.9, 1.0, fr, tuple(pos), rot)
+        else:
+            sb.cylinder(r, 0.65, 1.0, fr, tuple(pos), rot)
+    return sb.scene(f"{kind}_ramp{count}", -9.81, steps)
+
+
+def sphere_pour(count: int = 1000, width: int = 8, depth: int = 8, start_z: float = 10.0, seed: int = 7,
+                steps: int = 300) -> Scene:
+    """Small spheres poured in a jittered column, moving forward and down, onto seven standing
+    cylinders on a static ground slab."""
+    rng = _rng(seed)
+    sb = _Builder()
+    sb.body((1000, 1000, 1), 0.0, 0.7, (0, 0, 0))
+    cr, chl, sp = 0.45, 0.55, 1.7
+    for x, y in ((0, 0), (-sp, 0), (sp, 0), (-sp / 2, sp * 0.9), (sp / 2, sp * 0.9), (-sp / 2, -sp * 0.9),
+                 (sp / 2, -sp * 0.9)):
+        sb.cylinder(cr, chl, 1.0, 0.85, (x, y, 0.5 + chl))
+    r = 0.12
+    spacing = r * 2.35
+    jit = rng.uniform(-1.0, 1.0, (count, 3)) * np.array([0.35, 0.35, 0.2]) * r
+    vj = rng.uniform(-1.0, 1.0, (count, 3)) * np.array([0.18, 0.35, 0.18])
+    per_layer = width * depth
+    for k in range(count):
+        layer, cell = divmod(k, per_layer)
+        y, x = divmod(cell, width)
+        taper = 1.0 - min(layer / 18.0, 0.45)
+        px = (x - (width - 1) * 0.5) * spacing * taper + jit[k, 0] - 1.2
+        py = (y - (depth - 1) * 0.5) * spacing * taper + jit[k, 1]
+        pz = start_z + layer * spacing + jit[k, 2]
+        sb.sphere(r, 1.0, 0.45, (px, py, pz), vel=(1.1 + vj[k, 0], vj[k, 1], -1.5 + vj[k, 2]))
+    return sb.scene(f"sphere_pour{count}", -9.81, steps)
+
+
+def mixed_pile(count: int = 2000, seed: int = 8, pitch: float = 2.8, steps: int = 300) -> Scene:
+    """Boxes, spheres, capsules and cylinders in random sizes and orientations on a jittered grid
+    inside four static walls: every pair kind of the narrowphase meets every other."""
+    rng = _rng(seed)
+    side = max(1, int(math.ceil((count / 4.0) ** 0.5)))
+    half = side * pitch / 2.0
+    sb = _Builder()
+    sb.body((2 * half + 20, 2 * half + 20, 2), 0.0, 0.5, (0, 0, -1.0))
+    for sx, sy, px, py in ((2, 2 * half + 4, -half - 2, 0), (2, 2 * half + 4, half + 2, 0),
+                           (2 * half + 4, 2, 0, -half - 2), (2 * half + 4, 2, 0, half + 2)):
+        sb.body((sx, sy, 20), 0.0, 0.5, (px, py, 10.0))
+    kind = rng.integers(0, 4, count)
+    dims = rng.uniform(0.25, 0.9, (count, 3))
+    quats = random_quaternions(rng, count)
+    vel = rng.normal(0.0, 0.5, (count, 3))
+    fric = rng.uniform(0.3, 0.8, count)
+    jit = rng.uniform(-0.05, 0.05, (count, 3))
+    for k in range(count):
+        ix, iy, iz = k % side, (k // side) % side, k // (side * side)
+        pos = (-half + pitch * (ix + 0.5) + jit[k, 0], -half + pitch * (iy + 0.5) + jit[k, 1],
+               1.5 + pitch * (iz + 0.5) + jit[k, 2])
+        rot, v = tuple(quats[k]), tuple(vel[k])
+        if kind[k] == 0:
+            sb.body(tuple(2.0 * dims[k]), 1.0, fric[k], pos, rot, v)
+        elif kind[k] == 1:
+            sb.sphere(dims[k, 0], 1.0, fric[k], pos, rot, v)
+        elif kind[k] == 2:
+            sb.capsule(0.6 * dims[k, 0], 0.5 * dims[k, 1], 1.0, fric[k], pos, rot, v)
+        else:
+            sb.cylinder(dims[k, 0], 0.5 * dims[k, 1], 1.0, fric[k], pos, rot, v)
+    return sb.scene(f"mixed_pile{count}", -9.81, steps)
+
+
+def round_cases(steps: int = 90) -> list:
+    """Small scenes that each drive one branch of the round-shape narrowphase (collide.cpp:435-1181)
+    and of the contact offsets (manifold.cpp:29-90)."""
+    out = []
+
+    def case(name, build):
+        sb = _Builder()
+        build(sb)
+        out.append(sb.scene(name, -9.81, steps))
+
+    def ground(sb):
+        sb.body((100, 100, 1), 0.0, 0.6, (0, 0, 0))
+
+    # sphere resting on a dynamic box: outside, within the radius
+    case("sphere_on_box", lambda sb: (ground(sb), sb.body((2, 2, 1), 1.0, 0.5, (0, 0, 1.0)),
+                                      sb.sphere(0.4, 1.0, 0.5, (0.3, 0.1, 1.9))))
+    # sphere inside a static cube (not a thin slab): pushed out of the top face
+    case("sphere_in_static_cube", lambda sb: (sb.body((2, 2, 2), 0.0, 0.5, (0, 0, 0)),
+                                              sb.sphere(0.3, 1.0, 0.5, (0.2, -0.1, 0.7))))
+    # sphere inside a moving dynamic box: the face most against the relative motion
+    case("sphere_in_moving_box", lambda sb: (ground(sb), sb.body((2, 2, 2), 1.0, 0.5, (0, 0, 1.6), vel=(0.5, 0, 0)),
+                                             sb.sphere(0.3, 1.0, 0.5, (0.2, 0.1, 1.8), vel=(-1.0, 0.2, 0.0))))
+    # sphere inside a resting dynamic box with no relative motion: the nearest face
+    case("sphere_in_resting_box", lambda sb: (ground(sb), sb.body((2, 2, 2), 1.0, 0.5, (0, 0, 1.6)),
+                                              sb.sphere(0.3, 1.0, 0.5, (0.6, 0.1, 1.7))))
+    # fast sphere crossing a dynamic plate within one step: the swept test
+    case("sphere_swept_plate", lambda sb: (ground(sb), sb.body((3, 3, 0.2), 1.0, 0.5, (0, 0, 3.0)),
+                                           sb.sphere(0.1, 1.0, 0.5, (0, 0, 3.35), vel=(0, 0, -30.0))))
+    # fast sphere that ends a step inside a static cube: the swept test from inside
+    case("sphere_fast_into_cube", lambda sb: (sb.body((2, 2, 2), 0.0, 0.5, (0, 0, 0)),
+                                              sb.sphere(0.15, 1.0, 0.5, (0.1, 0.2, 1.5), vel=(0, 0, -40.0))))
+    # sphere that crosses a whole plate within one step: the swept test from outside
+    case("sphere_through_plate", lambda sb: (ground(sb), sb.body((3, 3, 0.2), 1.0, 0.5, (0, 0, 3.0)),
+                                             sb.sphere(0.1, 1.0, 0.5, (0, 0, 3.6), vel=(0, 0, -60.0))))
+    # capsule lying across a static slab narrower than its core: both end points miss the top
+    # face, the point under the centre does not
+    case("capsule_across_slab", lambda sb: (sb.body((1, 1, 0.1), 0.0, 0.5, (0, 0, 2.0)),
+                                            sb.capsule(0.2, 1.0, 1.0, 0.5, (0, 0, 2.24), _axis_quat((0, 1, 0), math.pi / 2))))
+    # capsule on a dynamic box, tilted so one end is nearer
+    case("capsule_on_box", lambda sb: (ground(sb), sb.body((3, 2, 1), 1.0, 0.5, (0, 0, 1.0)),
+                                       sb.capsule(0.3, 0.6, 1.0, 0.5, (0, 0, 2.1), _axis_quat((0, 1, 0), 1.2))))
+    # capsule lying flat on the ground slab (both ends and the near-parallel branch)
+    case("capsule_flat", lambda sb: (ground(sb), sb.capsule(0.3, 0.8, 1.0, 0.5, (0, 0, 0.85),
+                                                            _axis_quat((1, 0, 0), math.pi / 2))))
+    # capsule inside a static cube
+    case("capsule_in_static_cube", lambda sb: (sb.body((2, 2, 2), 0.0, 0.5, (0, 0, 0)),
+                                               sb.capsule(0.2, 0.3, 1.0, 0.5, (0.1, 0, 0.6))))
+    # cylinder on a dynamic box: collided as the box of its bounding size
+    case("cylinder_on_box", lambda sb: (ground(sb), sb.body((2, 2, 1), 1.0, 0.5, (0, 0, 1.0)),
+                                        sb.cylinder(0.4, 0.3, 1.0, 0.5, (0.2, 0, 1.9), _axis_quat((0, 1, 0), 0.3))))
+    # cylinder standing (cap rim) and lying (side lines) on the ground slab
+    case("cylinder_standing", lambda sb: (ground(sb), sb.cylinder(0.5, 0.4, 1.0, 0.6, (0, 0, 0.95),
+                                                                  _axis_quat((1, 0, 0), 0.2))))
+    case("cylinder_lying", lambda sb: (ground(sb), sb.cylinder(0.4, 0.6, 1.0, 0.6, (0, 0, 0.95),
+                                                               _axis_quat((1, 0, 0), math.pi / 2), angvel=(0, 0, 1.0))))
+    # two coaxial cylinders: caps facing
+    case("cylinders_caps", lambda sb: (ground(sb), sb.cylinder(0.5, 0.4, 1.0, 0.6, (0, 0, 0.95)),
+                                       sb.cylinder(0.4, 0.3, 1.0, 0.6, (0.05, 0, 1.7))))
+    # two parallel cylinders side by side
+    case("cylinders_side", lambda sb: (ground(sb),
+                                       sb.cylinder(0.4, 0.6, 1.0, 0.6, (0, 0, 0.95), _axis_quat((1, 0, 0), math.pi / 2)),
+                                       sb.cylinder(0.4, 0.5, 1.0, 0.6, (0.2, 0.1, 1.8), _axis_quat((1, 0, 0), math.pi / 2))))
+    # crossed cylinders: closest points of the two axes
+    case("cylinders_crossed", lambda sb: (ground(sb),
+                                          sb.cylinder(0.3, 0.8, 1.0, 0.6, (0, 0, 0.85), _axis_quat((1, 0, 0), math.pi / 2)),
+                                          sb.cylinder(0.3, 0.8, 1.0, 0.6, (0, 0.1, 1.5), _axis_quat((0, 1, 0), math.pi / 2))))
+    # cylinder against a sphere and against a capsule
+    case("cylinder_sphere", lambda sb: (ground(sb), sb.cylinder(0.5, 0.5, 1.0, 0.8, (0, 0, 1.05)),
+                                        sb.sphere(0.3, 1.0, 0.5, (0.1, 0.05, 1.95))))
+    case("capsule_cylinder", lambda sb: (ground(sb),
+                                         sb.capsule(0.3, 0.5, 1.0, 0.8, (0, 0, 1.9), _axis_quat((1, 0, 0), 1.3)),
+                                         sb.cylinder(0.5, 0.5, 1.0, 0.8, (0, 0, 1.05))))
+    # sphere-sphere, sphere-capsule, capsule-capsule
+    case("sphere_sphere", lambda sb: (ground(sb), sb.sphere(0.5, 1.0, 0.5, (0, 0, 1.0)),
+                                      sb.sphere(0.4, 1.0, 0.5, (0.3, 0.1, 1.85))))
+    case("sphere_capsule", lambda sb: (ground(sb), sb.capsule(0.3, 0.7, 1.0, 0.5, (0, 0, 0.8),
+                                                              _axis_quat((0, 1, 0), math.pi / 2)),
+                                       sb.sphere(0.3, 1.0, 0.5, (0.4, 0.05, 1.4))))
+    case("capsules_crossed", lambda sb: (ground(sb),
+                                         sb.capsule(0.3, 0.7, 1.0, 0.5, (0, 0, 0.8), _axis_quat((0, 1, 0), math.pi / 2)),
+                                         sb.capsule(0.3, 0.7, 1.0, 0.5, (0.05, 0, 1.45), _axis_quat((1, 0, 0), math.pi / 2))))
+    # static sphere under a dynamic sphere: no rolling pass for the dynamic one (the static A is chosen)
+    case("spheres_static_a", lambda sb: (sb.sphere(0.6, 0.0, 0.5, (0, 0, 0)),
+                                         sb.sphere(0.3, 1.0, 0.5, (0.05, 0, 0.95))))
+    return out
+
+
 def batch(num_worlds: int = 4096, seed: int = 4, lo: int = 8, hi: int = 512, steps: int = 300,
           kinds=BATCH_KINDS) -> list:
     """Independent worlds for the batched step: body counts log-uniform in [lo, hi]
@@ -318,15 +565,21 @@ def by_name(name: str) -> Scene:
 def build_state(scene: Scene, maker) -> State:
     """Scene (creation order) -> State (list order, serials = creation index).
 
-    ``maker`` supplies ``make_box_body(size, density, friction, pos, rot, vel, angvel)``
+    ``maker`` supplies ``make_box_body(size, density, friction, pos, rot, vel, angvel)`` and
+    ``make_round_body(shape, radius, half_length, density, friction, pos, rot, vel, angvel)``
     returning a BODY record, and ``joint_spring_derived(rec_a_or_None, rec_b, ra, rb)``
     returning (torque_arm, rest) -- the constructor-derived fields.
     """
     n = scene.num_bodies
+    shape, srad, hlen = scene.shape_columns()
     created = np.zeros(n, BODY)
     for c in range(n):
-        created[c] = maker.make_box_body(scene.size[c], scene.density[c], scene.friction[c], scene.pos[c],
-                                         scene.rot[c], scene.vel[c], scene.angvel[c])
+        if shape[c] == SHAPE_BOX:
+            created[c] = maker.make_box_body(scene.size[c], scene.density[c], scene.friction[c], scene.pos[c],
+                                             scene.rot[c], scene.vel[c], scene.angvel[c])
+        else:
+            created[c] = maker.make_round_body(int(shape[c]), srad[c], hlen[c], scene.density[c], scene.friction[c],
+                                               scene.pos[c], scene.rot[c], scene.vel[c], scene.angvel[c])
         created[c]["dense_id"] = c
     bodies = created[::-1].copy()
     to_list = lambda c: -1 if c < 0 else n - 1 - int(c)

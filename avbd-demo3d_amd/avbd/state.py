@@ -19,8 +19,11 @@ BODY = np.dtype(
     [("size", "<f4", 3), ("mass", "<f4"), ("moment", "<f4", 3), ("friction", "<f4"),
      ("radius", "<f4"), ("pos", "<f4", 3), ("rot", "<f4", 4), ("initial_lin", "<f4", 3),
      ("initial_ang", "<f4", 4), ("inertial_lin", "<f4", 3), ("inertial_ang", "<f4", 4),
-     ("vel", "<f4", 3), ("ang_vel", "<f4", 3), ("prev_vel", "<f4", 3), ("shape", "<i4"),
-     ("dense_id", "<i4")], align=True)
+     ("vel", "<f4", 3), ("ang_vel", "<f4", 3), ("prev_vel", "<f4", 3), ("shape_radius", "<f4"),
+     ("half_length", "<f4"), ("shape", "<i4"), ("dense_id", "<i4")], align=True)
+
+# RigidShapeType (solver.h:47-53)
+SHAPE_BOX, SHAPE_SPHERE, SHAPE_CAPSULE, SHAPE_CYLINDER = 0, 1, 2, 3
 
 CONTACT = np.dtype(
     [("feature", "<i4"), ("r_a", "<f4", 3), ("r_b", "<f4", 3), ("c0", "<f4", 3),
@@ -76,11 +79,26 @@ def default_params(gravity: float = -10.0, iterations: int = 10, dt: float = 1.0
     return p
 
 
+def as_bodies(arr) -> np.ndarray:
+    """Body records in the BODY layout, copied field by field. Records written before the shape
+    fields existed are boxes, whose RigidShape radius is their bounding radius."""
+    a = np.asarray(arr)
+    if a.dtype == BODY:
+        return np.ascontiguousarray(a)
+    out = np.zeros(a.shape, BODY)
+    for name in a.dtype.names:
+        if name in BODY.names:
+            out[name] = a[name]
+    if "shape_radius" not in a.dtype.names:
+        out["shape_radius"] = np.where(out["shape"] == SHAPE_BOX, out["radius"], np.float32(0))
+    return out
+
+
 class State:
     """A full solver state: bodies in list order, forces in ascending serial."""
 
     def __init__(self, bodies, joints=None, springs=None, manifolds=None, next_serial=0):
-        self.bodies = np.ascontiguousarray(bodies, dtype=BODY)
+        self.bodies = as_bodies(bodies)
         self.joints = np.ascontiguousarray(joints if joints is not None else np.zeros(0, JOINT), dtype=JOINT)
         self.springs = np.ascontiguousarray(springs if springs is not None else np.zeros(0, SPRING), dtype=SPRING)
         self.manifolds = np.ascontiguousarray(

@@ -50,6 +50,7 @@ struct avbd_handle
     int maxBodies = 0, maxJoints = 0, maxSprings = 0, maxManifolds = 0, maxPairs = 0, maxChunks = 0, maxAdj = 0;
     int hashSize = 0;
     int coopIterate = 0, coopColor = 0;
+    int numRound = 0, numSpheres = 0; // bodies of the uploaded state that are not boxes / are spheres
     int injectFault = 0;
     Bodies B{};
     Manifolds cur{}, tmp{};
@@ -280,10 +281,12 @@ int launch_coop(const void *fn, int blocks, int threads, void **args, hipStream_
 }
 
 // Bodies at `dev` -> host records (fields the body table carries).
+constexpr int kBodyRows = 12; // the rows bodies_to_host reads
+
 void bodies_to_host(const std::vector<float4> *v, const std::vector<float> &fr, const std::vector<int> &ids, int n,
                     avbd_body *out)
 {
-    // v: pos rot initLin initAng inerLin inerAng vel angVel prevVel massMoment sizeRad
+    // v[kBodyRows]: pos rot initLin initAng inerLin inerAng vel angVel prevVel massMoment sizeRad shape
     for (int i = 0; i < n; ++i)
     {
         avbd_body &b = out[i];
@@ -306,7 +309,9 @@ void bodies_to_host(const std::vector<float4> *v, const std::vector<float> &fr, 
         c3(b.size, v[10][i]);
         b.radius = v[10][i].w;
         b.friction = fr[i];
-        b.shape = AVBD_SHAPE_BOX;
+        b.shape_radius = v[11][i].x;
+        b.half_length = v[11][i].y;
+        b.shape = (int32_t)v[11][i].z;
         b.dense_id = ids[i];
     }
 }
@@ -669,6 +674,9 @@ int enqueue_finish(avbd_handle *h)
 {
     if (h->nb > 0)
         k_finish<<<ceil_div(h->nb, 256), 256, 0, h->stream>>>(h->B, h->P, h->nb);
+    if (h->numSpheres > 0)
+        k_rolling<<<ceil_div(h->nb, 256), 256, 0, h->stream>>>(h->B, h->cur, h->S.deg, h->S.adjStart, h->S.adj,
+                                                               h->S.adjPartner, h->nb, h->maxAdj);
     if (h->timing)
     {
         HIP_TRY(hipEventRecord(h->ev[5], h->stream));
@@ -733,6 +741,64 @@ int avbd_make_box_body(const float size[3], float density, float friction, const
     for (int k = 0; k < 4; ++k)
         out->rot[k] = out->initial_ang[k] = out->inertial_ang[k] = q[k];
     out->shape = AVBD_SHAPE_BOX;
+    out->shape_radius = out->radius;
+    return AVBD_OK;
+}
+
+int avbd_make_round_body(int32_t shape, float radius, float half_length, float density, float friction,
+                         const float pos[3], const float rot[4], const float vel[3], const float ang_vel[3],
+                         avbd_body *out)
+{
+    if (!pos || !out)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_make_round_body: null argument");
+    if (shape != AVBD_SHAPE_SPHERE && shape != AVBD_SHAPE_CAPSULE && shape != AVBD_SHAPE_CYLINDER)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_make_round_body: shape must be sphere, capsule or cylinder");
+    if (!(radius > 0.0f) || !(half_length >= 0.0f))
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_make_round_body: radius > 0 and half_length >= 0 required");
+    // the box constructor's record of the bounding size, then the shape's own mass properties
+    // (rigid.cpp:44-86: sphere 4/3 pi r^3 rho, I = 2/5 m r^2; capsule = cylinder + two caps;
+    // cylinder pi r^2 h rho, I = m r^2 / 2 about z and m (3 r^2 + h^2) / 12 across)
+    const float pi = 3.14159265358979323846f;
+    float d = radius * 2.0f;
+    if (shape == AVBD_SHAPE_SPHERE)
+        half_length = 0.0f;
+    float len = half_length * 2.0f;
+    float size[3] = {d, d, shape == AVBD_SHAPE_CAPSULE ? len + d : shape == AVBD_SHAPE_CYLINDER ? len : d};
+    int r = avbd_make_box_body(size, density, friction, pos, rot, vel, ang_vel, out);
+    if (r)
+        return r;
+    auto ball = [&](float rr) { return 4.0f / 3.0f * pi * rr * rr * rr * density; };
+    if (shape == AVBD_SHAPE_SPHERE)
+    {
+        out->mass = ball(radius);
+        float in = 2.0f / 5.0f * out->mass * radius * radius;
+        out->moment[0] = out->moment[1] = out->moment[2] = in;
+        out->radius = radius;
+    }
+    else if (shape == AVBD_SHAPE_CAPSULE)
+    {
+        float mc = pi * radius * radius * len * density;
+        float mb = ball(radius);
+        out->mass = mc + mb;
+        float axial = 0.5f * mc * radius * radius + 2.0f / 5.0f * mb * radius * radius;
+        float across = (3.0f * radius * radius + len * len) / 12.0f * mc + 2.0f / 5.0f * mb * radius * radius +
+                       mb * half_length * half_length;
+        out->moment[0] = out->moment[1] = across;
+        out->moment[2] = axial;
+        out->radius = half_length + radius;
+    }
+    else
+    {
+        out->mass = pi * radius * radius * len * density;
+        float axial = 0.5f * out->mass * radius * radius;
+        float across = (3.0f * radius * radius + len * len) / 12.0f * out->mass;
+        out->moment[0] = out->moment[1] = across;
+        out->moment[2] = axial;
+        out->radius = sqrtf(radius * radius + half_length * half_length);
+    }
+    out->shape = shape;
+    out->shape_radius = radius;
+    out->half_length = half_length;
     return AVBD_OK;
 }
 
@@ -831,7 +897,8 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     Bodies &B = h->B;
     if ((r = h->alloc(B.pose, 4 * (size_t)N)) || (r = h->alloc(B.inerLin, N)) || (r = h->alloc(B.inerAng, N)) ||
         (r = h->alloc(B.vel, N)) || (r = h->alloc(B.angVel, N)) || (r = h->alloc(B.prevVel, N)) ||
-        (r = h->alloc(B.massMoment, N)) || (r = h->alloc(B.sizeRad, N)) || (r = h->alloc(B.friction, N)) ||
+        (r = h->alloc(B.massMoment, N)) || (r = h->alloc(B.sizeRad, N)) || (r = h->alloc(B.shape, N)) ||
+        (r = h->alloc(B.friction, N)) ||
         (r = h->alloc(B.denseId, N)))
         return bail(r);
     if ((r = alloc_manifolds(h, h->cur, h->maxManifolds)) || (r = alloc_manifolds(h, h->tmp, h->maxManifolds)))
@@ -946,9 +1013,17 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     if (num_bodies > h->maxBodies || num_joints > h->maxJoints || num_springs > h->maxSprings ||
         num_manifolds > h->maxManifolds)
         return fail(AVBD_ERR_CAPACITY, "avbd_upload_state: state exceeds the handle's capacities");
+    int numRound = 0, numSpheres = 0;
     for (int i = 0; i < num_bodies; ++i)
-        if (bodies[i].shape != AVBD_SHAPE_BOX)
-            return fail(AVBD_ERR_UNSUPPORTED, "avbd_upload_state: only box bodies are supported");
+    {
+        int sh = bodies[i].shape;
+        if (sh < AVBD_SHAPE_BOX || sh > AVBD_SHAPE_CYLINDER)
+            return fail(AVBD_ERR_UNSUPPORTED, "avbd_upload_state: unknown shape code");
+        if (sh != AVBD_SHAPE_BOX && !(bodies[i].shape_radius > 0.0f && bodies[i].half_length >= 0.0f))
+            return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_upload_state: round shape needs radius > 0 and half_length >= 0");
+        numRound += sh != AVBD_SHAPE_BOX;
+        numSpheres += sh == AVBD_SHAPE_SPHERE;
+    }
     auto badBody = [&](int b, bool allowWorld) { return (b < 0 && !(allowWorld && b == -1)) || b >= num_bodies; };
     for (int i = 0; i < num_joints; ++i)
         if (badBody(joints[i].body_a, true) || badBody(joints[i].body_b, false))
@@ -964,7 +1039,7 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     if (r)
         return r;
     int N = num_bodies;
-    std::vector<float4> pos(N), rot(N), il(N), ia(N), nl(N), na(N), vel(N), av(N), pv(N), mm(N), sr(N);
+    std::vector<float4> pos(N), rot(N), il(N), ia(N), nl(N), na(N), vel(N), av(N), pv(N), mm(N), sr(N), sh(N);
     std::vector<float> fr(N);
     std::vector<int> ids(N);
     for (int i = 0; i < N; ++i)
@@ -981,6 +1056,7 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
         pv[i] = make_float4(b.prev_vel[0], b.prev_vel[1], b.prev_vel[2], 0.0f);
         mm[i] = make_float4(b.mass, b.moment[0], b.moment[1], b.moment[2]);
         sr[i] = make_float4(b.size[0], b.size[1], b.size[2], b.radius);
+        sh[i] = make_float4(b.shape_radius, b.half_length, (float)b.shape, 0.0f);
         fr[i] = b.friction;
         ids[i] = b.dense_id;
     }
@@ -1001,7 +1077,8 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     if ((e = up(h->B.pose, pose.data(), 4 * f4)) || (e = up(h->B.inerLin, nl.data(), f4)) ||
         (e = up(h->B.inerAng, na.data(), f4)) || (e = up(h->B.vel, vel.data(), f4)) || (e = up(h->B.angVel, av.data(), f4)) ||
         (e = up(h->B.prevVel, pv.data(), f4)) || (e = up(h->B.massMoment, mm.data(), f4)) ||
-        (e = up(h->B.sizeRad, sr.data(), f4)) || (e = up(h->B.friction, fr.data(), sizeof(float) * N)) ||
+        (e = up(h->B.sizeRad, sr.data(), f4)) || (e = up(h->B.shape, sh.data(), f4)) ||
+        (e = up(h->B.friction, fr.data(), sizeof(float) * N)) ||
         (e = up(h->B.denseId, ids.data(), sizeof(int) * N)))
         return fail(AVBD_ERR_HIP, std::string("upload bodies: ") + hipGetErrorString(e));
     // joints / springs
@@ -1065,12 +1142,19 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
             size_t sl = (size_t)k * kMaxContacts + c;
             feat[sl] = ct.feature;
             float4 *o = &cr[sl * kCR];
+            // the shape code and radii / half lengths of the pair, as k_narrowphase stores them
+            const avbd_body &ba = bodies[m.body_a], &bb = bodies[m.body_b];
+            int code = shape_code(ba.shape, bb.shape, ct.feature);
+            int codeBits;
+            std::memcpy(&codeBits, &code, sizeof(int));
+            float codeF;
+            std::memcpy(&codeF, &codeBits, sizeof(float));
             o[CR_RA] = make_float4(ct.r_a[0], ct.r_a[1], ct.r_a[2], m.friction);
-            o[CR_RB] = make_float4(ct.r_b[0], ct.r_b[1], ct.r_b[2], 0.0f);
-            o[CR_C0] = make_float4(ct.c0[0], ct.c0[1], ct.c0[2], 0.0f);
-            o[CR_N] = make_float4(m.basis[0], m.basis[1], m.basis[2], 0.0f);
-            o[CR_T1] = make_float4(m.basis[3], m.basis[4], m.basis[5], 0.0f);
-            o[CR_T2] = make_float4(m.basis[6], m.basis[7], m.basis[8], 0.0f);
+            o[CR_RB] = make_float4(ct.r_b[0], ct.r_b[1], ct.r_b[2], codeF);
+            o[CR_C0] = make_float4(ct.c0[0], ct.c0[1], ct.c0[2], bb.half_length);
+            o[CR_N] = make_float4(m.basis[0], m.basis[1], m.basis[2], ba.shape_radius);
+            o[CR_T1] = make_float4(m.basis[3], m.basis[4], m.basis[5], bb.shape_radius);
+            o[CR_T2] = make_float4(m.basis[6], m.basis[7], m.basis[8], ba.half_length);
             o[CR_PEN] = make_float4(ct.penalty[0], ct.penalty[1], ct.penalty[2], 0.0f);
             o[CR_LAM] = make_float4(ct.lambda[0], ct.lambda[1], ct.lambda[2], ct.stick ? 1.0f : 0.0f);
         }
@@ -1096,6 +1180,8 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     h->nb = N;
     h->nj = num_joints;
     h->ns = num_springs;
+    h->numRound = numRound;
+    h->numSpheres = numSpheres;
     h->lastNumManifolds = M;
     h->numWorlds = 1;
     h->worldStartHost.assign({0, N});
@@ -1146,7 +1232,7 @@ int avbd_download_state(avbd_handle *h, avbd_body *bodies, avbd_joint *joints, a
     int N = h->nb;
     if (bodies && N)
     {
-        std::vector<float4> v[11];
+        std::vector<float4> v[kBodyRows];
         std::vector<float4> pose(4 * (size_t)N);
         HIP_TRY(hipMemcpy(pose.data(), h->B.pose, sizeof(float4) * 4 * N, hipMemcpyDeviceToHost));
         for (int k = 0; k < 4; ++k)
@@ -1155,9 +1241,9 @@ int avbd_download_state(avbd_handle *h, avbd_body *bodies, avbd_joint *joints, a
             for (int i = 0; i < N; ++i)
                 v[k][i] = pose[4 * i + k];
         }
-        float4 *src[11] = {nullptr, nullptr, nullptr, nullptr, h->B.inerLin, h->B.inerAng,
-                           h->B.vel, h->B.angVel, h->B.prevVel, h->B.massMoment, h->B.sizeRad};
-        for (int k = 4; k < 11; ++k)
+        float4 *src[kBodyRows] = {nullptr, nullptr, nullptr, nullptr, h->B.inerLin, h->B.inerAng, h->B.vel,
+                           h->B.angVel, h->B.prevVel, h->B.massMoment, h->B.sizeRad, h->B.shape};
+        for (int k = 4; k < kBodyRows; ++k)
         {
             v[k].resize(N);
             HIP_TRY(hipMemcpy(v[k].data(), src[k], sizeof(float4) * N, hipMemcpyDeviceToHost));

@@ -47,6 +47,7 @@ struct Bodies
     float4 *prevVel;
     float4 *massMoment; // mass, moment xyz
     float4 *sizeRad;    // full widths xyz, bounding radius
+    float4 *shape;      // RigidShape radius, halfLength, type (as float), 0
     float *friction;
     int *denseId;
 };

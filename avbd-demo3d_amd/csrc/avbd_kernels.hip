@@ -18,6 +18,7 @@
 #include "avbd_device.hpp"
 #include "avbd_math.hpp"
 #include "avbd_collide.hpp"
+#include "avbd_round.hpp"
 
 #include <float.h>
 
@@ -787,8 +788,10 @@ __global__ void k_create_finish(Counters *ctr, int maxManifolds, int maxPairs)
 // ------------------------------------------------------------------------------------------
 // (collide_boxes<T> in avbd_collide.hpp)
 
-// Manifold::initialize for box pairs (manifold.cpp:100-174). Reads the persistent buffer `cur`
-// and writes the initialised manifold to the same slot of `tmp`.
+// Manifold::initialize (manifold.cpp:100-174). Reads the persistent buffer `cur` and writes the
+// initialised manifold to the same slot of `tmp`. Box pairs take the box-box path; a pair with a
+// sphere, capsule or cylinder goes through the shape dispatch, is frictionless when a sphere is
+// in it, and re-takes its anchors and friction rows at every match.
 // matchOld / stickUsed (adjoint recording, may be null): per new contact slot, the old contact it
 // continues (-1: none) and whether its anchors were kept by the stick rule.
 __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, Counters *ctr, int *keep,
@@ -802,11 +805,24 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
         int nOld = m < old ? cur.count[m] : 0;
         V3 pA = ld3(B.pose[4 * (ab.x) + P_POS]), pB = ld3(B.pose[4 * (ab.y) + P_POS]);
         Q4 qA = ldq(B.pose[4 * (ab.x) + P_ROT]), qB = ldq(B.pose[4 * (ab.y) + P_ROT]);
-        V3 sA = ld3(B.sizeRad[ab.x]), sB = ld3(B.sizeRad[ab.y]);
+        float4 sA4 = B.sizeRad[ab.x], sB4 = B.sizeRad[ab.y];
+        float4 hA = B.shape[ab.x], hB = B.shape[ab.y];
+        int shA = (int)hA.z, shB = (int)hB.z;
         float friction = sqrtf(B.friction[ab.x] * B.friction[ab.y]);
+        if (shA == kShapeSphere || shB == kShapeSphere)
+            friction = 0.0f;
+        bool refresh = any_round(shA) || any_round(shB);
         NewContactsT<float> nc;
         V3T<float> t0{0, 0, 0}, t1{0, 0, 0}, t2{0, 0, 0};
-        int nn = collide_boxes<float>(tv(pA), tq(qA), tv(sA), tv(pB), tq(qB), tv(sB), nc, t0, t1, t2);
+        int nn;
+        if (shA == kShapeBox && shB == kShapeBox)
+            nn = collide_boxes<float>(tv(pA), tq(qA), tv(ld3(sA4)), tv(pB), tq(qB), tv(ld3(sB4)), nc, t0, t1, t2);
+        else
+        {
+            ShapeBody A{tv(pA), tv(ld3(sA4)), tv(ld3(B.vel[ab.x])), tq(qA), B.massMoment[ab.x].x, hA.x, hA.y, shA};
+            ShapeBody Bb{tv(pB), tv(ld3(sB4)), tv(ld3(B.vel[ab.y])), tq(qB), B.massMoment[ab.y].x, hB.x, hB.y, shB};
+            nn = collide_pair(A, Bb, P.dt, nc, t0, t1, t2);
+        }
         V3 b0 = fv(t0), b1 = fv(t1), b2 = fv(t2);
         if (nn == 0)
         {
@@ -830,11 +846,16 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
                     pen = o[CR_PEN];
                     lam = o[CR_LAM];
                     match = j;
-                    if (lam.w != 0.0f)
+                    if (!refresh && lam.w != 0.0f)
                     {
                         rA = ld3(o[CR_RA]);
                         rB = ld3(o[CR_RB]);
                         kept = 1;
+                    }
+                    if (refresh)
+                    {
+                        lam = make_float4(lam.x, 0.0f, 0.0f, 0.0f);
+                        pen = make_float4(pen.x, kPenaltyMin, kPenaltyMin, pen.w);
                     }
                     break;
                 }
@@ -844,8 +865,19 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
                 matchOld[m * kMaxContacts + i] = match;
                 stickUsed[m * kMaxContacts + i] = kept;
             }
-            V3 xA = pA + rotate(qA, rA);
-            V3 xB = pB + rotate(qB, rB);
+            int code = shape_code(shA, shB, nc.feature[i]);
+            V3 xA, xB;
+            if (code == 0)
+            {
+                xA = pA + rotate(qA, rA);
+                xB = pB + rotate(qB, rB);
+            }
+            else
+            {
+                bool cap = (code & kCodeCap) != 0;
+                xA = pA + contact_offset(shA, hA.x, hA.y, cap, qA, rA, b0, true);
+                xB = pB + contact_offset(shB, hB.x, hB.y, cap, qB, rB, b0, false);
+            }
             V3 d = xA - xB;
             V3 c0 = V3{dot(b0, d), dot(b1, d), dot(b2, d)} + V3{kMargin, 0.0f, 0.0f};
             V3 l = V3{lam.x, lam.y, lam.z} * P.alpha * P.gamma;
@@ -855,12 +887,13 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
             int sl = m * kMaxContacts + i;
             tmp.feature[sl] = nc.feature[i];
             float4 *o = tmp.cr + (size_t)sl * kCR;
+            // shape code and the two radii / half lengths ride in the free w slots
             o[CR_RA] = st3(rA, friction);
-            o[CR_RB] = st3(rB, 0.0f);
-            o[CR_C0] = st3(c0, 0.0f);
-            o[CR_N] = st3(b0, 0.0f);
-            o[CR_T1] = st3(b1, 0.0f);
-            o[CR_T2] = st3(b2, 0.0f);
+            o[CR_RB] = st3(rB, __int_as_float(code));
+            o[CR_C0] = st3(c0, hB.y);
+            o[CR_N] = st3(b0, hA.x);
+            o[CR_T1] = st3(b1, hB.x);
+            o[CR_T2] = st3(b2, hA.y);
             o[CR_PEN] = st3(k, 0.0f);
             o[CR_LAM] = st3(l, lam.w);
         }
@@ -1330,6 +1363,25 @@ __device__ __forceinline__ void st_coherent(float4 *p, float4 v)
     __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// World offsets of a contact's two anchors (manifold.cpp:29-90). Box pairs carry shape code 0
+// in CR_RB.w and rotate their local anchors; other pairs read the shapes, radii (CR_N.w, CR_T1.w)
+// and half lengths (CR_T2.w, CR_C0.w) the narrowphase stored with the record.
+__device__ __forceinline__ void contact_anchors(float4 rA4, float4 rB4, float4 c04, float4 bb0, float4 bb1, float4 bb2,
+                                                Q4 qA, Q4 qB, V3 &rAW, V3 &rBW)
+{
+    int code = __float_as_int(rB4.w);
+    if (code == 0)
+    {
+        rAW = rotate(qA, ld3(rA4));
+        rBW = rotate(qB, ld3(rB4));
+        return;
+    }
+    bool cap = (code & kCodeCap) != 0;
+    V3 n = ld3(bb0);
+    rAW = contact_offset(code & 3, bb0.w, bb2.w, cap, qA, ld3(rA4), n, true);
+    rBW = contact_offset((code >> 2) & 3, bb1.w, c04.w, cap, qB, ld3(rB4), n, false);
+}
+
 // Co: the partner's current pose is read coherently (dataflow primal); the own body's rows and
 // the constant rows (x-) are ordinary loads.
 template <bool Co = false>
@@ -1350,8 +1402,8 @@ __device__ __forceinline__ int contact_record(float *r, int i, int partner, bool
     V3 dqBA = qdiff(qB, ldq(jB4));
     V3 b0 = ld3(bb0), b1 = ld3(bb1), b2 = ld3(bb2);
     V3 nb0 = -b0, nb1 = -b1, nb2 = -b2;
-    V3 rAW = rotate(qA, ld3(rA4));
-    V3 rBW = rotate(qB, ld3(rB4));
+    V3 rAW, rBW;
+    contact_anchors(rA4, rB4, c04, bb0, bb1, bb2, qA, qB, rAW, rBW);
     V3 ja0 = cross(rAW, b0), ja1 = cross(rAW, b1), ja2 = cross(rAW, b2);
     V3 jb0 = cross(rBW, nb0), jb1 = cross(rBW, nb1), jb2 = cross(rBW, nb2);
     V3 pen = ld3(pen4);
@@ -1934,8 +1986,8 @@ __device__ __forceinline__ int dual_contact(int4 cr, const Manifolds &M, const B
     V3 dqBA = qdiff(qB, ldq(jB4));
     V3 b0 = ld3(bb0), b1 = ld3(bb1), b2 = ld3(bb2);
     V3 nb0 = -b0, nb1 = -b1, nb2 = -b2;
-    V3 rAW = rotate(qA, ld3(rA4));
-    V3 rBW = rotate(qB, ld3(rB4));
+    V3 rAW, rBW;
+    contact_anchors(rA4, rB4, c04, bb0, bb1, bb2, qA, qB, rAW, rBW);
     V3 ja0 = cross(rAW, b0), ja1 = cross(rAW, b1), ja2 = cross(rAW, b2);
     V3 jb0 = cross(rBW, nb0), jb1 = cross(rBW, nb1), jb2 = cross(rBW, nb2);
     V3 pen = ld3(pen4);
@@ -2063,6 +2115,57 @@ __global__ void k_tape_contacts(const int4 *contactRec, const Counters *ctr, con
 __global__ void k_claim_reset(Counters *c) { c->dfClaim = 0u; }
 
 // Solver::finishStep for boxes (solver.cpp:1275-1288).
+// applySphereRollingFriction (solver.cpp:628-660), per dynamic sphere after k_finish: its
+// manifolds in force-list order (the adjacency is sorted newest first) against static bodies,
+// where this sphere is the pair's chosen sphere (A if A is a sphere), blend the contact-point
+// slip out of the velocity and pull the spin toward rolling, once per contact point.
+__global__ void k_rolling(Bodies B, Manifolds M, const int *deg, const int *adjStart, const int *adj,
+                          const int *adjPartner, int n, int maxAdj)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float4 hs = B.shape[i];
+    if ((int)hs.z != kShapeSphere || !(B.massMoment[i].x > 0.0f))
+        return;
+    int b = adjStart[i], e = min(b + deg[i], maxAdj);
+    if (b >= e)
+        return;
+    V3 v = ld3(B.vel[i]), w = ld3(B.angVel[i]);
+    float r = hs.x, fi = B.friction[i];
+    for (int x = b; x < e; ++x)
+    {
+        int ent = adj[x];
+        if (adjKind(ent) != kAdjManifold)
+            continue;
+        int m = adjIndex(ent), o = adjPartner[x];
+        int2 ab = M.ab[m];
+        bool isA = ab.x == i;
+        if (!isA && (int)B.shape[ab.x].z == kShapeSphere)
+            continue;
+        int nc = M.count[m];
+        if (B.massMoment[o].x > 0.0f || nc <= 0)
+            continue;
+        float mu = sqrtf(fi * B.friction[o]);
+        if (mu <= 0.0f)
+            continue;
+        float blend = clampr(mu * 0.08f, 0.0f, 0.18f);
+        V3 n0 = ld3(M.cr[(size_t)m * kMaxContacts * kCR + CR_N]);
+        V3 nrm = isA ? n0 : -n0;
+        for (int c = 0; c < nc; ++c)
+        {
+            V3 rW = -nrm * r;
+            V3 cv = v + cross(w, rW);
+            V3 slip = cv - nrm * dot(cv, nrm);
+            v = v - slip * blend;
+            V3 roll = cross(nrm, v) / r;
+            w = w + (roll - w) * blend;
+        }
+    }
+    B.vel[i] = st3(v, 0.0f);
+    B.angVel[i] = st3(w, 0.0f);
+}
+
 __global__ void k_finish(Bodies B, Params P, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

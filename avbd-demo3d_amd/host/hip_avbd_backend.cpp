@@ -111,8 +111,6 @@ struct HipAvbdBackend : PhysicsBackend
         for (size_t i = 0; i < bodies.size(); ++i)
         {
             const Rigid *r = bodies[i];
-            if (r->shape.type != RIGID_SHAPE_BOX)
-                return fallback(solver, "non-box body");
             avbd_body &b = tb[i];
             std::memset(&b, 0, sizeof(b));
             put3(b.size, r->size);
@@ -129,7 +127,10 @@ struct HipAvbdBackend : PhysicsBackend
             put3(b.vel, r->velocityLin);
             put3(b.ang_vel, r->velocityAng);
             put3(b.prev_vel, r->prevVelocityLin);
-            b.shape = AVBD_SHAPE_BOX;
+            // RigidShapeType and avbd_shape share their codes (box, sphere, capsule, cylinder)
+            b.shape = (int32_t)r->shape.type;
+            b.shape_radius = r->shape.radius;
+            b.half_length = r->shape.halfLength;
             b.dense_id = (int32_t)r->denseId;
         }
         std::vector<avbd_joint> tj;

@@ -43,11 +43,14 @@ enum avbd_status {
     AVBD_ERR_CAPACITY = -2,      /* a buffer sized at avbd_create would overflow */
     AVBD_ERR_HIP = -3,           /* a HIP runtime call failed */
     AVBD_ERR_NO_DEVICE = -4,
-    AVBD_ERR_UNSUPPORTED = -5,   /* shape or feature outside the box-only step */
+    AVBD_ERR_UNSUPPORTED = -5,   /* feature outside the step (shape code, adjoint of a non-box world) */
     AVBD_ERR_INJECTED = -6       /* avbd_debug_inject_fault armed this failure */
 };
 
-enum avbd_shape { AVBD_SHAPE_BOX = 0 };
+/* RigidShapeType (solver.h:47-53). Boxes are Rigid::Rigid (rigid.cpp:24-42); the round
+ * shapes are Rigid::makeSphere / makeCapsule / makeCylinder (rigid.cpp:44-86), whose
+ * core segment runs along the body's local z axis. */
+enum avbd_shape { AVBD_SHAPE_BOX = 0, AVBD_SHAPE_SPHERE = 1, AVBD_SHAPE_CAPSULE = 2, AVBD_SHAPE_CYLINDER = 3 };
 
 /* Solver parameters: Solver::dt/gravity/iterations/alpha/betaLin/betaAng/gamma
  * (solver.h:348-355), defaults in Solver::defaultParams (solver.cpp:909-932).
@@ -79,6 +82,8 @@ typedef struct avbd_body {
     float vel[3];           /* velocityLin */
     float ang_vel[3];       /* velocityAng */
     float prev_vel[3];      /* prevVelocityLin */
+    float shape_radius;     /* RigidShape::radius: sphere/capsule/cylinder radius (box: bounding radius) */
+    float half_length;      /* RigidShape::halfLength: capsule/cylinder core half length (0 otherwise) */
     int32_t shape;          /* enum avbd_shape */
     int32_t dense_id;       /* caller's id, carried through unchanged */
 } avbd_body;
@@ -215,12 +220,17 @@ int avbd_default_params(avbd_params *out);
 /* Host-side record builders: the mass properties and warm-start defaults the
  * reference constructors compute.
  *   avbd_make_box_body  <- Rigid::Rigid   (rigid.cpp:24-42)
+ *   avbd_make_round_body <- Rigid::makeSphere / makeCapsule / makeCylinder (rigid.cpp:44-86)
  *   avbd_make_joint     <- Joint::Joint   (joint.cpp:25-33)
  *   avbd_make_spring    <- Spring::Spring (spring.cpp:14-25)
  * Bodies referenced by a joint/spring are passed by pointer (NULL = world). */
 int avbd_make_box_body(const float size[3], float density, float friction,
                        const float pos[3], const float rot[4], const float vel[3],
                        const float ang_vel[3], avbd_body *out);
+/* shape: AVBD_SHAPE_SPHERE (half_length ignored), _CAPSULE or _CYLINDER. */
+int avbd_make_round_body(int32_t shape, float radius, float half_length, float density, float friction,
+                         const float pos[3], const float rot[4], const float vel[3],
+                         const float ang_vel[3], avbd_body *out);
 int avbd_make_joint(const avbd_body *body_a, const avbd_body *body_b,
                     int32_t index_a, int32_t index_b, int64_t serial,
                     const float r_a[3], const float r_b[3], float stiffness_lin,

@@ -309,6 +309,8 @@ struct Body
     V3<R> vel;
     V3<R> angVel;
     V3<R> prevVel;
+    R shapeRadius; // RigidShape::radius
+    R halfLength;  // RigidShape::halfLength
     int shape;
     int denseId;
 };
@@ -697,6 +699,650 @@ inline int collideBoxes(const Body<R> &bA, const Body<R> &bB, Contact<R> *out, M
     return faceManifold(cb, A, B, false, best.ib, best.n);
 }
 
+// ---- round shapes: spheres, capsules, cylinders (collide.cpp:435-1181) ----
+// Shape codes of RigidShapeType (solver.h:47-53). A capsule's and a cylinder's core segment
+// runs along the body's local z axis, halfLength either side of its centre.
+enum ShapeKind
+{
+    SHAPE_BOX = 0,
+    SHAPE_SPHERE = 1,
+    SHAPE_CAPSULE = 2,
+    SHAPE_CYLINDER = 3
+};
+
+// Branch-hit counters of the round-shape narrowphase and the rolling pass (test
+// instrumentation: which code paths a scene exercises).
+enum RoundBranch
+{
+    RB_THIN_TOP_ROUND = 0,   // sphere/capsule on a thin static top
+    RB_ROUND_BOX_NEAR,       // centre outside the box, within the radius
+    RB_ROUND_BOX_TOP_POINT,  // thin-top point after the top contacts missed
+    RB_INSIDE_SWEPT,         // centre inside, swept sphere test hit
+    RB_INSIDE_STATIC,        // centre inside a static box: its up face
+    RB_INSIDE_MOTION,        // centre inside: face against the relative motion
+    RB_INSIDE_NEAREST,       // centre inside, no relative motion: nearest face
+    RB_OUTSIDE_SWEPT,        // centre beyond the radius, swept sphere test hit
+    RB_CYL_CAPS,             // coaxial cylinders, caps facing
+    RB_CYL_SIDE,             // parallel cylinders side by side
+    RB_CYL_CROSSED,          // non-parallel cylinders
+    RB_CYL_TOP_RIM,          // standing cylinder on a thin static top
+    RB_CYL_TOP_SIDE,         // lying cylinder on a thin static top
+    RB_CYL_BOX,              // cylinder collided as its bounding box
+    RB_ROLLING,              // rolling friction applied to a sphere contact
+    RB_ROUND_ROUND,          // sphere/capsule pairs and cylinder-round pairs
+    RB_COUNT
+};
+inline long long roundBranchHits[RB_COUNT] = {};
+inline void hitBranch(int b) { ++roundBranchHits[b]; }
+
+// collide.cpp:458-466: the narrowphase's "round" family is spheres and capsules; cylinders are
+// dispatched on their own.
+template <class R> inline bool sphereOrCapsule(const Body<R> &b) { return b.shape == SHAPE_SPHERE || b.shape == SHAPE_CAPSULE; }
+template <class R> inline V3<R> localZ(const Body<R> &b) { return rotate(b.rot, V3<R>{0, 0, 1}); }
+
+// Core segment of a capsule (a sphere's is its centre twice), collide.cpp:468-481.
+template <class R> inline void roundCore(const Body<R> &b, V3<R> &s0, V3<R> &s1)
+{
+    if (b.shape == SHAPE_CAPSULE)
+    {
+        V3<R> z = localZ(b);
+        s0 = b.pos - z * b.halfLength;
+        s1 = b.pos + z * b.halfLength;
+        return;
+    }
+    s0 = b.pos;
+    s1 = b.pos;
+}
+// collide.cpp:483-488
+template <class R> inline void cylinderCore(const Body<R> &b, V3<R> &s0, V3<R> &s1)
+{
+    V3<R> z = localZ(b);
+    s0 = b.pos - z * b.halfLength;
+    s1 = b.pos + z * b.halfLength;
+}
+
+// Box-frame helpers (collide.cpp:435-456).
+template <class R> inline V3<R> obbLocal(const Obb<R> &o, V3<R> p)
+{
+    V3<R> d = p - o.c;
+    return V3<R>{dot(d, o.ax[0]), dot(d, o.ax[1]), dot(d, o.ax[2])};
+}
+template <class R> inline V3<R> obbWorld(const Obb<R> &o, V3<R> l) { return o.c + o.ax[0] * l.x + o.ax[1] * l.y + o.ax[2] * l.z; }
+template <class R> inline V3<R> obbClosest(const Obb<R> &o, V3<R> p)
+{
+    V3<R> d = p - o.c;
+    return o.c + o.ax[0] * clampv(dot(d, o.ax[0]), -o.h.x, o.h.x) + o.ax[1] * clampv(dot(d, o.ax[1]), -o.h.y, o.h.y) +
+           o.ax[2] * clampv(dot(d, o.ax[2]), -o.h.z, o.h.z);
+}
+
+// collide.cpp:490-498
+template <class R> inline V3<R> segClosest(V3<R> a, V3<R> b, V3<R> p)
+{
+    V3<R> ab = b - a;
+    R den = lengthSq(ab);
+    if (den < K<R>::satEps)
+        return a;
+    R t = clampv(dot(p - a, ab) / den, R(0), R(1));
+    return a + ab * t;
+}
+
+// Smallest half extent (ties to the lower axis) of a static box that is thin enough to count as a
+// floor/slab, and the face of that axis pointing up (+z); collide.cpp:957-976.
+template <class R> inline bool thinTop(const Obb<R> &o, const Body<R> &box, int &axis, R &sign, V3<R> &normal)
+{
+    if (box.mass > 0)
+        return false;
+    axis = 0;
+    if (o.h.y < o.h[axis])
+        axis = 1;
+    if (o.h.z < o.h[axis])
+        axis = 2;
+    int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
+    if (o.h[axis] * R(4.0f) > fmin_(o.h[a1], o.h[a2]))
+        return false;
+    sign = dot(o.ax[axis], V3<R>{0, 0, 1}) >= 0 ? R(1) : R(-1);
+    normal = o.ax[axis] * sign;
+    return true;
+}
+
+// Local point clamped onto the top face (all but the thin axis clamped), after a proximity test
+// against the face grown by `grow` in-plane; false if out of reach. collide.cpp:687-715.
+template <class R> inline bool thinTopFoot(const Obb<R> &o, int axis, V3<R> &local, R grow)
+{
+    for (int k = 0; k < 3; ++k)
+    {
+        if (k == axis)
+            continue;
+        if (local[k] < -o.h[k] - grow || local[k] > o.h[k] + grow)
+            return false;
+    }
+    return true;
+}
+template <class R> inline void clampOthers(const Obb<R> &o, int axis, V3<R> &local)
+{
+    for (int k = 0; k < 3; ++k)
+        if (k != axis)
+            local[k] = clampv(local[k], -o.h[k], o.h[k]);
+}
+
+// collide.cpp:500-517: contact between two spheres centred on the given points.
+template <class R>
+inline int roundPair(const Body<R> &bA, const Body<R> &bB, V3<R> cA, R rA, V3<R> cB, R rB, int32_t key, Contact<R> *out,
+                     M3<R> &basis)
+{
+    V3<R> delta = cB - cA;
+    R rs = rA + rB;
+    R d2 = lengthSq(delta);
+    if (d2 > rs * rs)
+        return 0;
+    V3<R> n = d2 > K<R>::satEps ? delta / sqrt_(d2) : V3<R>{1, 0, 0};
+    V3<R> xA = cA + n * rA;
+    V3<R> xB = cB - n * rB;
+    basis = orthonormal(-n);
+    ContactBuilder<R> cb(bA, bB, out);
+    cb.add(xA, xB, key);
+    return cb.count;
+}
+
+// Swept test of a sphere's step-start motion against a box grown by its radius (slab method),
+// collide.cpp:519-587. On a hit: the face normal and the clamped surface point.
+template <class R>
+inline bool sweptSphere(const Obb<R> &o, const Body<R> &sph, const Body<R> &box, R dt, V3<R> &n, V3<R> &xBox)
+{
+    V3<R> rel = sph.vel - box.vel;
+    if (dt <= 0 || lengthSq(rel) < K<R>::satEps)
+        return false;
+    V3<R> cur = sph.pos;
+    V3<R> prev = cur - rel * dt;
+    V3<R> l0 = obbLocal(o, prev), l1 = obbLocal(o, cur);
+    V3<R> dl = l1 - l0;
+    V3<R> grown = o.h + V3<R>{sph.shapeRadius, sph.shapeRadius, sph.shapeRadius};
+    R tIn = 0, tOut = 1;
+    int hit = -1;
+    R hitSign = 1;
+    for (int k = 0; k < 3; ++k)
+    {
+        if (abs_(dl[k]) < K<R>::satEps)
+        {
+            if (l0[k] < -grown[k] || l0[k] > grown[k])
+                return false;
+            continue;
+        }
+        R inv = R(1) / dl[k];
+        R t0 = (-grown[k] - l0[k]) * inv;
+        R t1 = (grown[k] - l0[k]) * inv;
+        R s = -1;
+        if (t0 > t1)
+        {
+            R tmp = t0;
+            t0 = t1;
+            t1 = tmp;
+            s = 1;
+        }
+        if (t0 > tIn)
+        {
+            tIn = t0;
+            hit = k;
+            hitSign = s;
+        }
+        tOut = fmin_(tOut, t1);
+        if (tIn > tOut)
+            return false;
+    }
+    if (hit < 0 || tIn < 0 || tIn > 1)
+        return false;
+    V3<R> surf = l0 + dl * tIn;
+    surf[hit] = o.h[hit] * hitSign;
+    clampOthers(o, hit, surf);
+    n = o.ax[hit] * hitSign;
+    xBox = obbWorld(o, surf);
+    return true;
+}
+
+// The point of a sphere/capsule nearest a thin static box's top, and its foot on the top face;
+// collide.cpp:632-685.
+template <class R>
+inline bool thinTopRoundPoint(const Obb<R> &o, const Body<R> &rb, const Body<R> &box, V3<R> &centre, V3<R> &n, V3<R> &xBox)
+{
+    int axis;
+    R sign;
+    V3<R> normal;
+    if (!thinTop(o, box, axis, sign, normal))
+        return false;
+    V3<R> s0, s1;
+    roundCore(rb, s0, s1);
+    R d0 = dot(s0, normal), d1 = dot(s1, normal);
+    if (rb.shape == SHAPE_CAPSULE && abs_(d0 - d1) < R(1.0e-4f))
+        centre = segClosest(s0, s1, rb.pos);
+    else
+        centre = d0 <= d1 ? s0 : s1;
+    V3<R> local = obbLocal(o, centre);
+    if (local[axis] * sign - o.h[axis] > rb.shapeRadius)
+        return false;
+    if (!thinTopFoot(o, axis, local, rb.shapeRadius))
+        return false;
+    local[axis] = o.h[axis] * sign;
+    clampOthers(o, axis, local);
+    n = normal;
+    xBox = obbWorld(o, local);
+    return true;
+}
+
+// One sphere-cap contact against a thin static box top; collide.cpp:687-715.
+template <class R>
+inline void thinTopRoundContact(ContactBuilder<R> &cb, const Obb<R> &o, const Body<R> &rb, bool roundIsA, int axis, R sign,
+                                V3<R> normal, V3<R> centre, int32_t key)
+{
+    V3<R> local = obbLocal(o, centre);
+    if (local[axis] * sign - o.h[axis] > rb.shapeRadius)
+        return;
+    if (!thinTopFoot(o, axis, local, rb.shapeRadius))
+        return;
+    local[axis] = o.h[axis] * sign;
+    clampOthers(o, axis, local);
+    V3<R> xRound = centre - normal * rb.shapeRadius;
+    V3<R> xBox = obbWorld(o, local);
+    cb.add(roundIsA ? xRound : xBox, roundIsA ? xBox : xRound, key);
+}
+
+// collide.cpp:717-754: -1 when the box is not a thin static top or nothing touches it.
+template <class R>
+inline int thinTopRound(const Obb<R> &o, const Body<R> &rb, const Body<R> &box, bool roundIsA, const Body<R> &bA,
+                        const Body<R> &bB, Contact<R> *out, M3<R> &basis)
+{
+    int axis;
+    R sign;
+    V3<R> normal;
+    if (!thinTop(o, box, axis, sign, normal))
+        return -1;
+    V3<R> nAB = roundIsA ? -normal : normal;
+    basis = orthonormal(-nAB);
+    ContactBuilder<R> cb(bA, bB, out);
+    if (rb.shape == SHAPE_CAPSULE)
+    {
+        V3<R> s0, s1;
+        roundCore(rb, s0, s1);
+        thinTopRoundContact(cb, o, rb, roundIsA, axis, sign, normal, s0, 0x04000000);
+        thinTopRoundContact(cb, o, rb, roundIsA, axis, sign, normal, s1, 0x04000001);
+    }
+    else
+    {
+        thinTopRoundContact(cb, o, rb, roundIsA, axis, sign, normal, rb.pos, 0x02000000);
+    }
+    if (cb.count > 0)
+        hitBranch(RB_THIN_TOP_ROUND);
+    return cb.count > 0 ? cb.count : -1;
+}
+
+// Sphere/capsule against a box; collide.cpp:767-896.
+template <class R>
+inline int collideRoundBox(const Body<R> &rb, const Body<R> &box, bool roundIsA, R dt, Contact<R> *out, M3<R> &basis)
+{
+    const Body<R> &bA = roundIsA ? rb : box;
+    const Body<R> &bB = roundIsA ? box : rb;
+    Obb<R> o = makeObb(box);
+    int top = thinTopRound(o, rb, box, roundIsA, bA, bB, out, basis);
+    if (top >= 0)
+        return top;
+    V3<R> centre = rb.pos;
+    if (rb.shape == SHAPE_CAPSULE)
+    {
+        V3<R> s0, s1;
+        roundCore(rb, s0, s1);
+        V3<R> c0 = obbClosest(o, s0), c1 = obbClosest(o, s1);
+        centre = lengthSq(s0 - c0) <= lengthSq(s1 - c1) ? s0 : s1;
+    }
+    R r = rb.shapeRadius;
+    V3<R> closest = obbClosest(o, centre);
+    V3<R> toRound = centre - closest;
+    R d2 = lengthSq(toRound);
+    V3<R> tc, tn, tp;
+    if (thinTopRoundPoint(o, rb, box, tc, tn, tp))
+    {
+        hitBranch(RB_ROUND_BOX_TOP_POINT);
+        centre = tc;
+        toRound = tn;
+        closest = tp;
+        d2 = 1;
+    }
+    else
+    {
+        V3<R> local = obbLocal(o, centre);
+        bool inside = abs_(local.x) <= o.h.x && abs_(local.y) <= o.h.y && abs_(local.z) <= o.h.z;
+        V3<R> sn, sp;
+        if (inside)
+        {
+            if (rb.shape == SHAPE_SPHERE && sweptSphere(o, rb, box, dt, sn, sp))
+            {
+                hitBranch(RB_INSIDE_SWEPT);
+                toRound = sn;
+                closest = sp;
+                d2 = 1;
+            }
+            else
+            {
+                // push-out face: the floor's top for a static box, else the face most against the
+                // relative motion, else the nearest face (collide.cpp:827-873)
+                int axis = 0;
+                R sign = local.x >= 0 ? R(1) : R(-1);
+                R best = o.h.x - abs_(local.x);
+                V3<R> rel = rb.vel - box.vel;
+                if (box.mass <= 0)
+                {
+                    hitBranch(RB_INSIDE_STATIC);
+                    axis = 0;
+                    if (o.h.y < o.h[axis])
+                        axis = 1;
+                    if (o.h.z < o.h[axis])
+                        axis = 2;
+                    sign = dot(o.ax[axis], V3<R>{0, 0, 1}) >= 0 ? R(1) : R(-1);
+                }
+                else if (lengthSq(rel) > K<R>::satEps)
+                {
+                    hitBranch(RB_INSIDE_MOTION);
+                    V3<R> against = -rel;
+                    R bestDot = R(-FLT_MAX);
+                    for (int k = 0; k < 3; ++k)
+                        for (int si = 0; si < 2; ++si)
+                        {
+                            R s = si == 0 ? R(-1) : R(1);
+                            R d = dot(o.ax[k] * s, against);
+                            if (d > bestDot)
+                            {
+                                bestDot = d;
+                                axis = k;
+                                sign = s;
+                            }
+                        }
+                }
+                else
+                {
+                    hitBranch(RB_INSIDE_NEAREST);
+                    R dy = o.h.y - abs_(local.y);
+                    if (dy < best)
+                    {
+                        axis = 1;
+                        sign = local.y >= 0 ? R(1) : R(-1);
+                        best = dy;
+                    }
+                    R dz = o.h.z - abs_(local.z);
+                    if (dz < best)
+                    {
+                        axis = 2;
+                        sign = local.z >= 0 ? R(1) : R(-1);
+                    }
+                }
+                V3<R> surf = local;
+                surf[axis] = o.h[axis] * sign;
+                closest = obbWorld(o, surf);
+                toRound = o.ax[axis] * sign;
+                d2 = 1;
+            }
+        }
+        else if (d2 <= r * r)
+            hitBranch(RB_ROUND_BOX_NEAR);
+        else if (d2 > r * r)
+        {
+            if (rb.shape != SHAPE_SPHERE || !sweptSphere(o, rb, box, dt, sn, sp))
+                return 0;
+            hitBranch(RB_OUTSIDE_SWEPT);
+            toRound = sn;
+            closest = sp;
+            d2 = 1;
+        }
+    }
+    V3<R> n = d2 > K<R>::satEps ? toRound / sqrt_(d2) : V3<R>{0, 0, 1};
+    V3<R> nAB = roundIsA ? -n : n;
+    V3<R> xRound = centre - n * r;
+    basis = orthonormal(-nAB);
+    ContactBuilder<R> cb(bA, bB, out);
+    cb.add(roundIsA ? xRound : closest, roundIsA ? closest : xRound, rb.shape == SHAPE_CAPSULE ? 0x04000000 : 0x02000000);
+    return cb.count;
+}
+
+// Rim point of a thin static box top against a cylinder; collide.cpp:978-1007.
+template <class R>
+inline void thinTopCylinderContact(ContactBuilder<R> &cb, const Obb<R> &o, const Body<R> &cyl, bool cylIsA, int axis, R sign,
+                                   V3<R> xCyl, int32_t key)
+{
+    V3<R> local = obbLocal(o, xCyl);
+    if (local[axis] * sign - o.h[axis] > K<R>::margin)
+        return;
+    if (!thinTopFoot(o, axis, local, cyl.shapeRadius))
+        return;
+    local[axis] = (o.h[axis] + K<R>::margin) * sign;
+    clampOthers(o, axis, local);
+    V3<R> xBox = obbWorld(o, local);
+    cb.add(cylIsA ? xCyl : xBox, cylIsA ? xBox : xCyl, key);
+}
+
+// Unit vector perpendicular to n (orthonormal's first tangent, normalised on its own).
+template <class R> inline V3<R> perpUnit(V3<R> n)
+{
+    return abs_(n.x) > abs_(n.z) ? normalize(V3<R>{-n.y, n.x, 0}) : normalize(V3<R>{0, -n.z, n.y});
+}
+
+// collide.cpp:1009-1053: a standing cylinder's cap rim (4 points) or a lying one's two base
+// rims against a thin static top; -1 when that does not apply.
+template <class R>
+inline int thinTopCylinder(const Obb<R> &o, const Body<R> &cyl, const Body<R> &box, bool cylIsA, Contact<R> *out, M3<R> &basis)
+{
+    int axis;
+    R sign;
+    V3<R> normal;
+    if (!thinTop(o, box, axis, sign, normal))
+        return -1;
+    V3<R> nAB = cylIsA ? -normal : normal;
+    basis = orthonormal(-nAB);
+    const Body<R> &bA = cylIsA ? cyl : box;
+    const Body<R> &bB = cylIsA ? box : cyl;
+    ContactBuilder<R> cb(bA, bB, out);
+    V3<R> z = localZ(cyl);
+    R zd = dot(z, normal);
+    R r = cyl.shapeRadius;
+    if (abs_(zd) > R(0.65f))
+    {
+        R zs = zd >= 0 ? R(1) : R(-1);
+        V3<R> cap = cyl.pos - z * (cyl.halfLength * zs);
+        V3<R> u = perpUnit(z);
+        V3<R> v = normalize(cross(z, u));
+        V3<R> rim[4] = {u * r, -u * r, v * r, -v * r};
+        for (int i = 0; i < 4; ++i)
+            thinTopCylinderContact(cb, o, cyl, cylIsA, axis, sign, cap + rim[i], 0x05000000 | i);
+        if (cb.count)
+            hitBranch(RB_CYL_TOP_RIM);
+    }
+    else
+    {
+        V3<R> radial = normal - z * zd;
+        if (lengthSq(radial) < K<R>::satEps)
+            radial = normal;
+        radial = normalize(radial);
+        V3<R> e0 = cyl.pos - z * cyl.halfLength;
+        V3<R> e1 = cyl.pos + z * cyl.halfLength;
+        thinTopCylinderContact(cb, o, cyl, cylIsA, axis, sign, e0 - radial * r, 0x05000100);
+        thinTopCylinderContact(cb, o, cyl, cylIsA, axis, sign, e1 - radial * r, 0x05000101);
+        if (cb.count)
+            hitBranch(RB_CYL_TOP_SIDE);
+    }
+    return cb.count > 0 ? cb.count : -1;
+}
+
+// Two cylinders; collide.cpp:1080-1152.
+template <class R>
+inline int collideCylinders(const Body<R> &bA, const Body<R> &bB, Contact<R> *out, M3<R> &basis)
+{
+    V3<R> zA = localZ(bA);
+    V3<R> zB = localZ(bB);
+    R par = dot(zA, zB);
+    V3<R> delta = bB.pos - bA.pos;
+    if (abs_(par) > R(0.95f))
+    {
+        R axial = dot(delta, zA);
+        V3<R> radial = delta - zA * axial;
+        R rl2 = lengthSq(radial);
+        R rs = bA.shapeRadius + bB.shapeRadius;
+        if (rl2 > rs * rs)
+            return 0;
+        if (bA.halfLength + bB.halfLength - abs_(axial) < -K<R>::margin)
+            return 0;
+        ContactBuilder<R> cb(bA, bB, out);
+        if (abs_(axial) > (bA.halfLength + bB.halfLength) * R(0.5f))
+        {
+            // end caps facing each other: four rim points of the smaller cap
+            R side = axial >= 0 ? R(1) : R(-1);
+            V3<R> n = zA * side;
+            V3<R> capA = bA.pos + n * bA.halfLength;
+            V3<R> capB = bB.pos - n * bB.halfLength;
+            hitBranch(RB_CYL_CAPS);
+            R rr = fmin_(bA.shapeRadius, bB.shapeRadius) * R(0.85f);
+            basis = orthonormal(-n);
+            V3<R> u = perpUnit(n);
+            V3<R> v = normalize(cross(n, u));
+            V3<R> rim[4] = {u * rr, -u * rr, v * rr, -v * rr};
+            for (int i = 0; i < 4; ++i)
+                cb.add(capA + rim[i], capB + rim[i], 0x06000000 | i);
+            return cb.count;
+        }
+        // side by side: the two ends of the axial overlap
+        hitBranch(RB_CYL_SIDE);
+        V3<R> n = rl2 > K<R>::satEps ? radial / sqrt_(rl2) : orthonormal(zA)[1];
+        basis = orthonormal(-n);
+        R ends[2] = {fmax_(-bA.halfLength, axial - bB.halfLength), fmin_(bA.halfLength, axial + bB.halfLength)};
+        for (int i = 0; i < 2; ++i)
+        {
+            R tA = ends[i];
+            R tB = tA - axial;
+            V3<R> xA = bA.pos + zA * tA + n * bA.shapeRadius;
+            V3<R> xB = bB.pos + zA * tB - n * bB.shapeRadius;
+            cb.add(xA, xB, 0x06000100 | i);
+        }
+        return cb.count;
+    }
+    V3<R> a0, a1, b0, b1, cA, cB;
+    cylinderCore(bA, a0, a1);
+    cylinderCore(bB, b0, b1);
+    closestSegPts(a0, a1, b0, b1, cA, cB);
+    hitBranch(RB_CYL_CROSSED);
+    return roundPair(bA, bB, cA, bA.shapeRadius, cB, bB.shapeRadius, 0x06000200, out, basis);
+}
+
+// Manifold::collide (collide.cpp:1154-1181): the pair dispatch. A cylinder against a box that is
+// not a thin static top is collided as the box of its bounding size (Rigid::size).
+template <class R>
+inline int collidePair(const Body<R> &bA, const Body<R> &bB, R dt, Contact<R> *out, M3<R> &basis)
+{
+    bool cylA = bA.shape == SHAPE_CYLINDER, cylB = bB.shape == SHAPE_CYLINDER;
+    bool rndA = sphereOrCapsule(bA), rndB = sphereOrCapsule(bB);
+    if (cylA && cylB)
+        return collideCylinders(bA, bB, out, basis);
+    if ((cylA && rndB) || (rndA && cylB))
+    {
+        // collide.cpp:1065-1078
+        const Body<R> &cyl = cylA ? bA : bB;
+        const Body<R> &rb = cylA ? bB : bA;
+        V3<R> c0, c1, r0, r1, cc, cr;
+        cylinderCore(cyl, c0, c1);
+        roundCore(rb, r0, r1);
+        closestSegPts(c0, c1, r0, r1, cc, cr);
+        hitBranch(RB_ROUND_ROUND);
+        if (cylA)
+            return roundPair(bA, bB, cc, cyl.shapeRadius, cr, rb.shapeRadius, 0x07000000, out, basis);
+        return roundPair(bA, bB, cr, rb.shapeRadius, cc, cyl.shapeRadius, 0x07000000, out, basis);
+    }
+    if ((cylA && bB.shape == SHAPE_BOX) || (bA.shape == SHAPE_BOX && cylB))
+    {
+        const Body<R> &cyl = cylA ? bA : bB;
+        const Body<R> &box = cylA ? bB : bA;
+        Obb<R> o = makeObb(box);
+        int top = thinTopCylinder(o, cyl, box, cylA, out, basis);
+        if (top >= 0)
+            return top;
+        hitBranch(RB_CYL_BOX);
+        return collideBoxes(bA, bB, out, basis);
+    }
+    if (rndA && rndB)
+    {
+        // collide.cpp:756-765
+        V3<R> a0, a1, b0, b1, cA, cB;
+        roundCore(bA, a0, a1);
+        roundCore(bB, b0, b1);
+        closestSegPts(a0, a1, b0, b1, cA, cB);
+        hitBranch(RB_ROUND_ROUND);
+        return roundPair(bA, bB, cA, bA.shapeRadius, cB, bB.shapeRadius, 0x03000000, out, basis);
+    }
+    if (rndA && bB.shape == SHAPE_BOX)
+        return collideRoundBox(bA, bB, true, dt, out, basis);
+    if (bA.shape == SHAPE_BOX && rndB)
+        return collideRoundBox(bB, bA, false, dt, out, basis);
+    return collideBoxes(bA, bB, out, basis);
+}
+
+// manifold.cpp:14-27: contact anchors of every round shape (cylinders included) are re-taken at
+// each initialisation; cylinder cap contacts keep their rim direction.
+template <class R> inline bool anyRound(const Body<R> &b)
+{
+    return b.shape == SHAPE_SPHERE || b.shape == SHAPE_CAPSULE || b.shape == SHAPE_CYLINDER;
+}
+inline bool cylinderCapKey(int32_t key)
+{
+    return (key & (int32_t)0xFFFFFF00) == 0x05000000 || (key & (int32_t)0xFFFFFF00) == 0x06000000;
+}
+
+// World-frame offset of a contact point from the body centre (manifold.cpp:29-90): a box rotates
+// its stored local offset; a sphere's point is its radius along the manifold normal; a capsule's
+// slides along its core to the stored height moved by the normal; a cylinder's is the rim point
+// in the support direction (or the stored cap direction for cap contacts).
+template <class R>
+inline V3<R> contactOffset(const Body<R> &b, V3<R> lo, const M3<R> &basis, bool isA, int32_t key)
+{
+    if (b.shape == SHAPE_SPHERE)
+        return basis[0] * (isA ? -b.shapeRadius : b.shapeRadius);
+    if (b.shape == SHAPE_CAPSULE)
+    {
+        V3<R> z = localZ(b);
+        V3<R> n = basis[0];
+        V3<R> nl = rotate(conjugate(b.rot), n);
+        R cz = isA ? lo.z + nl.z * b.shapeRadius : lo.z - nl.z * b.shapeRadius;
+        cz = clampv(cz, -b.halfLength, b.halfLength);
+        V3<R> along = z * cz;
+        return along + n * (isA ? -b.shapeRadius : b.shapeRadius);
+    }
+    if (b.shape == SHAPE_CYLINDER)
+    {
+        V3<R> dirW = basis[0] * (isA ? R(-1) : R(1));
+        V3<R> dirL = rotate(conjugate(b.rot), dirW);
+        R storedRad = sqrt_(lo.x * lo.x + lo.y * lo.y);
+        if (cylinderCapKey(key))
+        {
+            V3<R> l{0, 0, 0};
+            if (storedRad > R(1.0e-6f))
+            {
+                l.x = lo.x / storedRad * b.shapeRadius;
+                l.y = lo.y / storedRad * b.shapeRadius;
+            }
+            R capSign = abs_(lo.z) > R(1.0e-6f) ? (lo.z >= 0 ? R(1) : R(-1)) : (dirL.z >= 0 ? R(1) : R(-1));
+            l.z = b.halfLength * capSign;
+            return rotate(b.rot, l);
+        }
+        R dirRad = sqrt_(dirL.x * dirL.x + dirL.y * dirL.y);
+        R z = abs_(dirL.z) > R(0.15f) ? (dirL.z >= 0 ? b.halfLength : -b.halfLength) : clampv(lo.z, -b.halfLength, b.halfLength);
+        V3<R> l{0, 0, z};
+        if (dirRad > R(1.0e-6f))
+        {
+            l.x = dirL.x / dirRad * b.shapeRadius;
+            l.y = dirL.y / dirRad * b.shapeRadius;
+        }
+        else if (storedRad > R(1.0e-6f))
+        {
+            l.x = lo.x / storedRad * b.shapeRadius;
+            l.y = lo.y / storedRad * b.shapeRadius;
+        }
+        return rotate(b.rot, l);
+    }
+    return rotate(b.rot, lo);
+}
+
 // ---- the world and its step ----
 template <class R>
 struct World
@@ -821,15 +1467,20 @@ struct World
         }
     }
 
-    // Manifold::initialize (manifold.cpp:100-174), box shapes only.
+    // Manifold::initialize (manifold.cpp:100-174). A sphere in the pair makes the manifold
+    // frictionless; with any round shape (cylinders included) matched contacts always take the
+    // new anchors and restart their friction rows (lambda, penalty, stick).
     bool initManifold(Manifold<R> &m)
     {
         const Body<R> &A = bodies[m.a];
         const Body<R> &B = bodies[m.b];
         m.friction = sqrt_(A.friction * B.friction);
+        if (A.shape == SHAPE_SPHERE || B.shape == SHAPE_SPHERE)
+            m.friction = R(0);
         Contact<R> nc[8];
         std::memset((void *)nc, 0, sizeof(nc));
-        int nn = collideBoxes(A, B, nc, m.basis);
+        int nn = collidePair(A, B, p.dt, nc, m.basis);
+        bool refresh = anyRound(A) || anyRound(B);
         for (int i = 0; i < nn; ++i)
         {
             for (int j = 0; j < m.n; ++j)
@@ -838,10 +1489,18 @@ struct World
                 {
                     V3<R> rA = nc[i].rA, rB = nc[i].rB;
                     nc[i] = m.c[j];
-                    if (!m.c[j].stick)
+                    if (refresh || !m.c[j].stick)
                     {
                         nc[i].rA = rA;
                         nc[i].rB = rB;
+                    }
+                    if (refresh)
+                    {
+                        nc[i].lambda[1] = R(0);
+                        nc[i].lambda[2] = R(0);
+                        nc[i].penalty[1] = K<R>::penaltyMin;
+                        nc[i].penalty[2] = K<R>::penaltyMin;
+                        nc[i].stick = false;
                     }
                     break;
                 }
@@ -853,8 +1512,8 @@ struct World
         for (int i = 0; i < m.n; ++i)
         {
             Contact<R> &c = m.c[i];
-            V3<R> xA = A.pos + rotate(A.rot, c.rA);
-            V3<R> xB = B.pos + rotate(B.rot, c.rB);
+            V3<R> xA = A.pos + contactOffset(A, c.rA, m.basis, true, c.feature);
+            V3<R> xB = B.pos + contactOffset(B, c.rB, m.basis, false, c.feature);
             c.C0 = m.basis * (xA - xB) + V3<R>{K<R>::margin, 0, 0};
             c.lambda = c.lambda * p.alpha * p.gamma;
             c.penalty = clampv(c.penalty * p.gamma, K<R>::penaltyMin, K<R>::penaltyMax);
@@ -1013,7 +1672,8 @@ struct World
         for (int i = 0; i < m.n; ++i)
         {
             const Contact<R> &c = m.c[i];
-            V3<R> rAW = rotate(A.rot, c.rA), rBW = rotate(B.rot, c.rB);
+            V3<R> rAW = contactOffset(A, c.rA, m.basis, true, c.feature);
+            V3<R> rBW = contactOffset(B, c.rB, m.basis, false, c.feature);
             M3<R> jAL = m.basis, jBL = -m.basis;
             M3<R> jAA{cross(rAW, jAL[0]), cross(rAW, jAL[1]), cross(rAW, jAL[2])};
             M3<R> jBA{cross(rBW, jBL[0]), cross(rBW, jBL[1]), cross(rBW, jBL[2])};
@@ -1050,7 +1710,8 @@ struct World
         for (int i = 0; i < m.n; ++i)
         {
             Contact<R> &c = m.c[i];
-            V3<R> rAW = rotate(A.rot, c.rA), rBW = rotate(B.rot, c.rB);
+            V3<R> rAW = contactOffset(A, c.rA, m.basis, true, c.feature);
+            V3<R> rBW = contactOffset(B, c.rB, m.basis, false, c.feature);
             M3<R> jAL = m.basis, jBL = -m.basis;
             M3<R> jAA{cross(rAW, jAL[0]), cross(rAW, jAL[1]), cross(rAW, jAL[2])};
             M3<R> jBA{cross(rBW, jBL[0]), cross(rBW, jBL[1]), cross(rBW, jBL[2])};
@@ -1253,7 +1914,8 @@ struct World
         }
     }
 
-    // Solver::finishStep (solver.cpp:1275-1298), box bodies only
+    // Solver::finishStep (solver.cpp:1275-1298): BDF1 velocities, then the sphere rolling
+    // friction pass over the manifolds in force-list order (newest first).
     void finish()
     {
         for (auto &b : bodies)
@@ -1264,6 +1926,47 @@ struct World
                 b.vel = (b.pos - b.initLin) / p.dt;
                 b.angVel = (b.rot - b.initAng) / p.dt;
             }
+        }
+        for (size_t k = manifolds.size(); k-- > 0;)
+            rollingFriction(manifolds[k]);
+    }
+
+    // applySphereRollingFriction (solver.cpp:628-660): a dynamic sphere resting on a static body
+    // loses a friction-scaled share of its contact-point slip and is pulled toward rolling,
+    // once per contact point.
+    void rollingFriction(const Manifold<R> &m)
+    {
+        int si = -1, oi = -1;
+        if (bodies[m.a].shape == SHAPE_SPHERE)
+        {
+            si = m.a;
+            oi = m.b;
+        }
+        else if (bodies[m.b].shape == SHAPE_SPHERE)
+        {
+            si = m.b;
+            oi = m.a;
+        }
+        if (si < 0)
+            return;
+        Body<R> &s = bodies[si];
+        const Body<R> &o = bodies[oi];
+        if (s.mass <= 0 || o.mass > 0 || m.n <= 0)
+            return;
+        R mu = sqrt_(s.friction * o.friction);
+        if (mu <= 0)
+            return;
+        R blend = clampv(mu * R(0.08f), R(0), R(0.18f));
+        hitBranch(RB_ROLLING);
+        for (int i = 0; i < m.n; ++i)
+        {
+            V3<R> n = si == m.a ? m.basis[0] : -m.basis[0];
+            V3<R> rW = -n * s.shapeRadius;
+            V3<R> cv = s.vel + cross(s.angVel, rW);
+            V3<R> slip = cv - n * dot(cv, n);
+            s.vel -= slip * blend;
+            V3<R> roll = cross(n, s.vel) / s.shapeRadius;
+            s.angVel += (roll - s.angVel) * blend;
         }
     }
 

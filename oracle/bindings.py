@@ -61,6 +61,9 @@ def load_oracle():
         lib.orc_time_steps.argtypes = [_p, _i]
         lib.orc_time_steps.restype = C.c_double
         lib.orc_make_box_body.argtypes = [_p, C.c_float, C.c_float, _p, _p, _p, _p, _p]
+        lib.orc_make_round_body.argtypes = [_i, C.c_float, C.c_float, C.c_float, C.c_float, _p, _p, _p, _p, _p]
+        lib.orc_collide_pair.argtypes = [_p, _p, C.c_float, _p, _p]
+        lib.orc_round_branch_hits.argtypes = [_p, _i]
         lib.orc_collide.argtypes = [_i, _p, _p, _p, _p]
         lib.orc_solve6.argtypes = [_i, _p, _p, _p, _p, _p]
         lib.orc_quat_ops.argtypes = [_p] * 7
@@ -82,7 +85,7 @@ def load_ref():
     if _REF is None:
         lib = C.CDLL(ref_path())
         lib.ref_create.restype = _p
-        lib.ref_create.argtypes = [_p, _i] + [_p] * 7 + [_i] + [_p] * 8
+        lib.ref_create.argtypes = [_p, _i] + [_p] * 10 + [_i] + [_p] * 8
         lib.ref_create_from_state.restype = _p
         lib.ref_create_from_state.argtypes = [_p, _p, _i, _p, _i, _p, _i, _p, _i]
         lib.ref_destroy.argtypes = [_p]
@@ -96,6 +99,8 @@ def load_ref():
         lib.ref_broadphase_pairs.argtypes = [_p, _p, _i, _p, _p]
         lib.ref_collide.argtypes = [_p, _p, _p, _p]
         lib.ref_make_box_body.argtypes = [_p, C.c_float, C.c_float, _p, _p, _p]
+        lib.ref_make_round_body.argtypes = [_i, C.c_float, C.c_float, C.c_float, C.c_float, _p, _p, _p]
+        lib.ref_collide_pair.argtypes = [_p, _p, C.c_float, _p, _p]
         lib.ref_joint_spring_derived.argtypes = [_p, _p, _p, _p, _p, _p]
         lib.ref_use_hip.argtypes = [_p, _i]
         _REF = lib
@@ -108,6 +113,17 @@ def _f32(x, n):
 
 class OracleMaker:
     """Record builders of the oracle (Rigid/Joint/Spring constructors restated)."""
+
+    def make_round_body(self, shape, radius, half_length, density, friction, pos, rot, vel, angvel):
+        lib = load_oracle()
+        out = np.zeros((), BODY)
+        rc = lib.orc_make_round_body(int(shape), C.c_float(radius), C.c_float(half_length), C.c_float(density),
+                                     C.c_float(friction), _f32(pos, 3).ctypes.data_as(_p),
+                                     _f32(rot, 4).ctypes.data_as(_p), _f32(vel, 3).ctypes.data_as(_p),
+                                     _f32(angvel, 3).ctypes.data_as(_p), out.ctypes.data_as(_p))
+        if rc != 0:
+            raise ValueError(f"shape {shape} is not a round shape")
+        return out
 
     def make_box_body(self, size, density, friction, pos, rot, vel, angvel):
         lib = load_oracle()
@@ -224,6 +240,31 @@ def oracle_collide(a, b, fp64=False):
     return n, out[:n].copy(), basis
 
 
+ROUND_BRANCHES = ("thin_top_round", "round_box_near", "round_box_top_point", "inside_swept", "inside_static",
+                  "inside_motion", "inside_nearest", "outside_swept", "cyl_caps", "cyl_side", "cyl_crossed",
+                  "cyl_top_rim", "cyl_top_side", "cyl_box", "rolling", "round_round")
+
+
+def round_branch_hits(reset=False) -> dict:
+    """How often each round-shape narrowphase branch and the rolling pass ran in the oracle."""
+    lib = load_oracle()
+    out = np.zeros(len(ROUND_BRANCHES), np.int64)
+    n = lib.orc_round_branch_hits(_ptr(out), int(reset))
+    assert n == len(ROUND_BRANCHES)
+    return dict(zip(ROUND_BRANCHES, (int(v) for v in out)))
+
+
+def oracle_collide_pair(a, b, dt=1.0 / 60.0):
+    """Manifold::collide of two body records of any shapes: (count, contacts, basis)."""
+    lib = load_oracle()
+    out = np.zeros(8, CONTACT)
+    basis = np.zeros(9, np.float32)
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    n = lib.orc_collide_pair(a.ctypes.data_as(_p), b.ctypes.data_as(_p), C.c_float(dt), _ptr(out), _ptr(basis))
+    return n, out[:n].copy(), basis
+
+
 def oracle_solve6(lin, ang, cross, b, fp64=False):
     lib = load_oracle()
     x = np.zeros(6, np.float64)
@@ -242,6 +283,22 @@ def oracle_quat_ops(q, p, w, v):
 
 class RefMaker:
     """Record builders that call the reference constructors themselves."""
+
+    def make_round_body(self, shape, radius, half_length, density, friction, pos, rot, vel, angvel):
+        lib = load_ref()
+        out = np.zeros((), BODY)
+        rc = lib.ref_make_round_body(int(shape), C.c_float(radius), C.c_float(half_length), C.c_float(density),
+                                     C.c_float(friction), _f32(pos, 3).ctypes.data_as(_p),
+                                     _f32(vel, 3).ctypes.data_as(_p), out.ctypes.data_as(_p))
+        if rc != 0:
+            raise ValueError(f"shape {shape} is not a round shape")
+        out["rot"] = _f32(rot, 4)
+        out["initial_lin"] = _f32(pos, 3)
+        out["inertial_lin"] = _f32(pos, 3)
+        out["initial_ang"] = _f32(rot, 4)
+        out["inertial_ang"] = _f32(rot, 4)
+        out["ang_vel"] = _f32(angvel, 3)
+        return out
 
     def make_box_body(self, size, density, friction, pos, rot, vel, angvel):
         lib = load_ref()
@@ -278,8 +335,9 @@ class RefWorld:
         prm = np.ascontiguousarray(params)
         if scene is not None:
             s = scene
+            shape, srad, hlen = s.shape_columns()
             arrs = [np.ascontiguousarray(x) for x in (s.size, s.density, s.friction, s.pos, s.rot, s.vel,
-                                                       s.angvel)]
+                                                       s.angvel, shape, srad, hlen)]
             farrs = [np.ascontiguousarray(x) for x in (s.ftype, s.fa, s.fb, s.ra, s.rb, s.p0, s.p1, s.p2)]
             self._keep = (arrs, farrs)
             self._h = lib.ref_create(prm.ctypes.data_as(_p), s.num_bodies, *[_ptr(a) for a in arrs],
@@ -326,6 +384,17 @@ class RefWorld:
         out = np.zeros((n, 2), np.int32)
         self._lib.ref_broadphase_pairs(self._h, _ptr(out), n, C.byref(checks), C.byref(hits))
         return out, checks.value, hits.value
+
+
+def ref_collide_pair(a, b, dt=1.0 / 60.0):
+    """The reference's Manifold::collide on two full body records: (count, contacts, basis)."""
+    lib = load_ref()
+    out = np.zeros(8, CONTACT)
+    basis = np.zeros(9, np.float32)
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    n = lib.ref_collide_pair(a.ctypes.data_as(_p), b.ctypes.data_as(_p), C.c_float(dt), _ptr(out), _ptr(basis))
+    return n, out[:n].copy(), basis
 
 
 def ref_collide(a, b):

@@ -91,6 +91,8 @@ struct WorldImpl : WorldBase
                 put3(o.vel, b.vel);
                 put3(o.ang_vel, b.angVel);
                 put3(o.prev_vel, b.prevVel);
+                o.shape_radius = (float)b.shapeRadius;
+                o.half_length = (float)b.halfLength;
                 o.shape = b.shape;
                 o.dense_id = b.denseId;
             }
@@ -237,6 +239,8 @@ void fill(World<R> &w, const avbd_params *p, const avbd_body *bodies, int nb, co
         b.vel = v3<R>(s.vel);
         b.angVel = v3<R>(s.ang_vel);
         b.prevVel = v3<R>(s.prev_vel);
+        b.shapeRadius = s.shape_radius;
+        b.halfLength = s.half_length;
         b.shape = s.shape;
         b.denseId = s.dense_id;
     }
@@ -473,6 +477,106 @@ void orc_make_box_body(const float size[3], float density, float friction, const
     std::memcpy(out->inertial_lin, pos, 12);
     std::memcpy(out->inertial_ang, rot, 16);
     out->shape = AVBD_SHAPE_BOX;
+    out->shape_radius = out->radius;
+}
+
+/* Rigid::makeSphere / makeCapsule / makeCylinder mass properties (rigid.cpp:44-86): the box
+ * constructor's record of the bounding size, then the shape's own mass, moments and radii. */
+int orc_make_round_body(int shape, float radius, float halfLength, float density, float friction, const float pos[3],
+                        const float rot[4], const float vel[3], const float angVel[3], avbd_body *out)
+{
+    const float pi = 3.14159265358979323846f;
+    float d = radius * 2.0f;
+    if (shape == AVBD_SHAPE_SPHERE)
+        halfLength = 0.0f;
+    float len = halfLength * 2.0f;
+    float size[3] = {d, d, shape == AVBD_SHAPE_CAPSULE ? len + d : shape == AVBD_SHAPE_CYLINDER ? len : d};
+    orc_make_box_body(size, density, friction, pos, rot, vel, angVel, out);
+    auto ball = [&](float r) { return 4.0f / 3.0f * pi * r * r * r * density; };
+    if (shape == AVBD_SHAPE_SPHERE)
+    {
+        out->mass = ball(radius);
+        float in = 2.0f / 5.0f * out->mass * radius * radius;
+        out->moment[0] = out->moment[1] = out->moment[2] = in;
+        out->radius = radius;
+    }
+    else if (shape == AVBD_SHAPE_CAPSULE)
+    {
+        float mc = pi * radius * radius * len * density;
+        float mb = ball(radius);
+        out->mass = mc + mb;
+        float axial = 0.5f * mc * radius * radius + 2.0f / 5.0f * mb * radius * radius;
+        float across = (3.0f * radius * radius + len * len) / 12.0f * mc + 2.0f / 5.0f * mb * radius * radius +
+                       mb * halfLength * halfLength;
+        out->moment[0] = out->moment[1] = across;
+        out->moment[2] = axial;
+        out->radius = halfLength + radius;
+    }
+    else if (shape == AVBD_SHAPE_CYLINDER)
+    {
+        out->mass = pi * radius * radius * len * density;
+        float axial = 0.5f * out->mass * radius * radius;
+        float across = (3.0f * radius * radius + len * len) / 12.0f * out->mass;
+        out->moment[0] = out->moment[1] = across;
+        out->moment[2] = axial;
+        out->radius = sqrtf(radius * radius + halfLength * halfLength);
+    }
+    else
+        return -1;
+    out->shape = shape;
+    out->shape_radius = radius;
+    out->half_length = halfLength;
+    return 0;
+}
+
+/* Round-shape branch-hit counters (avbd_oracle::RoundBranch order); reset = 1 clears them. */
+int orc_round_branch_hits(long long *out, int reset)
+{
+    for (int i = 0; i < avbd_oracle::RB_COUNT; ++i)
+    {
+        if (out)
+            out[i] = avbd_oracle::roundBranchHits[i];
+        if (reset)
+            avbd_oracle::roundBranchHits[i] = 0;
+    }
+    return avbd_oracle::RB_COUNT;
+}
+
+/* Narrowphase of two bodies of any shapes (Manifold::collide, collide.cpp:1154-1181); dt is the
+ * solver step the swept sphere test reads. Returns the contact count. */
+int orc_collide_pair(const avbd_body *a, const avbd_body *b, float dt, avbd_contact *out, float basis[9])
+{
+    avbd_oracle::World<float> w;
+    for (const avbd_body *s : {a, b})
+    {
+        Body<float> o{};
+        o.size = v3<float>(s->size);
+        o.mass = s->mass;
+        o.friction = s->friction;
+        o.radius = s->radius;
+        o.pos = v3<float>(s->pos);
+        o.rot = q4<float>(s->rot);
+        o.vel = v3<float>(s->vel);
+        o.angVel = v3<float>(s->ang_vel);
+        o.shapeRadius = s->shape_radius;
+        o.halfLength = s->half_length;
+        o.shape = s->shape;
+        w.bodies.push_back(o);
+    }
+    Contact<float> c[8];
+    std::memset((void *)c, 0, sizeof(c));
+    M3<float> bs{};
+    int n = collidePair(w.bodies[0], w.bodies[1], dt, c, bs);
+    for (int r = 0; r < 3; ++r)
+        put3(basis + 3 * r, bs[r]);
+    for (int i = 0; i < n; ++i)
+    {
+        std::memset(&out[i], 0, sizeof(avbd_contact));
+        out[i].feature = c[i].feature;
+        put3(out[i].r_a, c[i].rA);
+        put3(out[i].r_b, c[i].rB);
+    }
+    return n;
 }
 
 /* Box-box narrowphase of two bodies (collide.cpp:898-955). Returns the contact count. */

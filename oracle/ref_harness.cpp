@@ -90,7 +90,9 @@ void setBodyState(Rigid *r, const avbd_body &b)
     r->moment = f3(b.moment);
     r->friction = b.friction;
     r->radius = b.radius;
-    r->shape.radius = b.radius;
+    r->shape.type = (RigidShapeType)b.shape;
+    r->shape.radius = b.shape_radius;
+    r->shape.halfLength = b.half_length;
     r->positionLin = f3(b.pos);
     r->positionAng = q4(b.rot);
     r->initialLin = f3(b.initial_lin);
@@ -119,7 +121,9 @@ void getBodyState(const Rigid *r, avbd_body &b)
     put3(b.vel, r->velocityLin);
     put3(b.ang_vel, r->velocityAng);
     put3(b.prev_vel, r->prevVelocityLin);
-    b.shape = AVBD_SHAPE_BOX;
+    b.shape = (int32_t)r->shape.type;
+    b.shape_radius = r->shape.radius;
+    b.half_length = r->shape.halfLength;
     b.dense_id = (int32_t)r->denseId;
 }
 
@@ -268,19 +272,32 @@ extern "C" {
 
 /* Build a scene through the reference constructors, in creation order.
  * Bodies: size3 density friction pos3 rot4 vel3 angvel3 (rot/angvel set after
- * construction, as a scene would). Forces: type 1 = Joint(a,b,rA,rB,p0=kLin,
- * p1=kAng,p2=fracture), 2 = Spring(a,b,rA,rB,p0=k,p1=rest); a = -1 is the world. */
+ * construction, as a scene would); shape/radius/halfLength (may be NULL: all
+ * boxes) select Rigid::makeSphere / makeCapsule / makeCylinder. Forces: type
+ * 1 = Joint(a,b,rA,rB,p0=kLin, p1=kAng,p2=fracture), 2 = Spring(a,b,rA,rB,
+ * p0=k,p1=rest); a = -1 is the world. */
 void *ref_create(const avbd_params *p, int nb, const float *size, const float *density, const float *friction,
-                 const float *pos, const float *rot, const float *vel, const float *angvel, int nf, const int *ftype,
-                 const int *fa, const int *fb, const float *rA, const float *rB, const float *p0, const float *p1,
-                 const float *p2)
+                 const float *pos, const float *rot, const float *vel, const float *angvel, const int *shape,
+                 const float *shapeRadius, const float *halfLength, int nf, const int *ftype, const int *fa,
+                 const int *fb, const float *rA, const float *rB, const float *p0, const float *p1, const float *p2)
 {
     RefWorld *w = new RefWorld();
     applyParams(w->solver, p);
     std::vector<Rigid *> created(nb);
     for (int i = 0; i < nb; ++i)
     {
-        Rigid *r = new Rigid(&w->solver, f3(size + 3 * i), density[i], friction[i], f3(pos + 3 * i), f3(vel + 3 * i));
+        int kind = shape ? shape[i] : AVBD_SHAPE_BOX;
+        Rigid *r;
+        if (kind == AVBD_SHAPE_SPHERE)
+            r = Rigid::makeSphere(&w->solver, shapeRadius[i], density[i], friction[i], f3(pos + 3 * i), f3(vel + 3 * i));
+        else if (kind == AVBD_SHAPE_CAPSULE)
+            r = Rigid::makeCapsule(&w->solver, shapeRadius[i], halfLength[i], density[i], friction[i], f3(pos + 3 * i),
+                                   f3(vel + 3 * i));
+        else if (kind == AVBD_SHAPE_CYLINDER)
+            r = Rigid::makeCylinder(&w->solver, shapeRadius[i], halfLength[i], density[i], friction[i], f3(pos + 3 * i),
+                                    f3(vel + 3 * i));
+        else
+            r = new Rigid(&w->solver, f3(size + 3 * i), density[i], friction[i], f3(pos + 3 * i), f3(vel + 3 * i));
         r->positionAng = q4(rot + 4 * i);
         r->velocityAng = f3(angvel + 3 * i);
         created[i] = r;
@@ -519,6 +536,33 @@ int ref_collide(const avbd_body *a, const avbd_body *b, avbd_contact *out, float
     return n;
 }
 
+/* Manifold::collide of two full body records (any shapes; velocities and the solver dt feed
+ * the swept sphere test). */
+int ref_collide_pair(const avbd_body *a, const avbd_body *b, float dt, avbd_contact *out, float basis[9])
+{
+    Solver s;
+    s.clear();
+    s.dt = dt;
+    Rigid *ra = new Rigid(&s, f3(a->size), 1.0f, 0.5f, f3(a->pos));
+    Rigid *rb = new Rigid(&s, f3(b->size), 1.0f, 0.5f, f3(b->pos));
+    setBodyState(ra, *a);
+    setBodyState(rb, *b);
+    Manifold::Contact c[8];
+    std::memset((void *)c, 0, sizeof(c));
+    float3x3 bs{};
+    int n = Manifold::collide(ra, rb, c, bs);
+    for (int r = 0; r < 3; ++r)
+        put3(basis + 3 * r, bs[r]);
+    for (int i = 0; i < n; ++i)
+    {
+        std::memset(&out[i], 0, sizeof(avbd_contact));
+        out[i].feature = c[i].feature.key;
+        put3(out[i].r_a, c[i].rA);
+        put3(out[i].r_b, c[i].rB);
+    }
+    return n;
+}
+
 /* Rigid::Rigid for a box: mass properties as the reference computes them. */
 void ref_make_box_body(const float size[3], float density, float friction, const float pos[3], const float vel[3],
                        avbd_body *out)
@@ -527,6 +571,25 @@ void ref_make_box_body(const float size[3], float density, float friction, const
     s.clear();
     Rigid *r = new Rigid(&s, f3(size), density, friction, f3(pos), f3(vel));
     getBodyState(r, *out);
+}
+
+/* Rigid::makeSphere / makeCapsule / makeCylinder (rigid.cpp:44-86) record. */
+int ref_make_round_body(int shape, float radius, float halfLength, float density, float friction, const float pos[3],
+                        const float vel[3], avbd_body *out)
+{
+    Solver s;
+    s.clear();
+    Rigid *r;
+    if (shape == AVBD_SHAPE_SPHERE)
+        r = Rigid::makeSphere(&s, radius, density, friction, f3(pos), f3(vel));
+    else if (shape == AVBD_SHAPE_CAPSULE)
+        r = Rigid::makeCapsule(&s, radius, halfLength, density, friction, f3(pos), f3(vel));
+    else if (shape == AVBD_SHAPE_CYLINDER)
+        r = Rigid::makeCylinder(&s, radius, halfLength, density, friction, f3(pos), f3(vel));
+    else
+        return -1;
+    getBodyState(r, *out);
+    return 0;
 }
 
 /* Joint/Spring constructors' derived fields: torque arm (joint.cpp:32) and

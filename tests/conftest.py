@@ -37,11 +37,18 @@ def load_golden(name):
     return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
 
 
-def scene_from_golden(z):
+def scene_from_golden(z, prefix=""):
+    """The Scene stored in a fixture (keys under `prefix`, e.g. "sphere_stack10.")."""
     from avbd.scenes import Scene
-    return Scene(name="golden", size=z["s_size"], density=z["s_density"], friction=z["s_friction"], pos=z["s_pos"],
-                 rot=z["s_rot"], vel=z["s_vel"], angvel=z["s_angvel"], ftype=z["f_type"], fa=z["f_a"], fb=z["f_b"],
-                 ra=z["f_ra"], rb=z["f_rb"], p0=z["f_p0"], p1=z["f_p1"], p2=z["f_p2"], gravity=float(z["gravity"]))
+    g = lambda k: z[prefix + k]
+    has_shape = (prefix + "s_shape") in z
+    return Scene(name=prefix.rstrip(".") or "golden", size=g("s_size"), density=g("s_density"),
+                 friction=g("s_friction"), pos=g("s_pos"), rot=g("s_rot"), vel=g("s_vel"), angvel=g("s_angvel"),
+                 ftype=g("f_type"), fa=g("f_a"), fb=g("f_b"), ra=g("f_ra"), rb=g("f_rb"), p0=g("f_p0"), p1=g("f_p1"),
+                 p2=g("f_p2"), gravity=float(g("gravity")),
+                 shape=g("s_shape") if has_shape else None,
+                 shape_radius=g("s_shape_radius") if has_shape else None,
+                 half_length=g("s_half_length") if has_shape else None)
 
 
 def state_from_golden(z, prefix):

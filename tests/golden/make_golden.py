@@ -17,6 +17,12 @@ Fixtures
                       plus a full state at step 100 and the state after step 101
   pile.npz            1000-box pile: full state before/after single steps with the
                       SAP pair emission and the colours of the stepped state
+  collide_round.npz   Manifold::collide on seeded random pairs over every shape pair
+                      (boxes, thin static slabs, spheres, capsules, cylinders; static
+                      bodies and fast relative motion for the swept sphere test)
+  round_scenes.npz    the round-shape scenes of avbd.scenes (branch cases, stacks, ramps,
+                      a 300-sphere pour, a 200-body mixed pile): bodies after every step
+                      (every 10th step for the pour and the mixed pile)
 """
 from __future__ import annotations
 
@@ -32,7 +38,7 @@ sys.path.insert(0, os.path.join(ROOT, "avbd-demo3d_amd"))
 
 from avbd import scenes  # noqa: E402
 from avbd.state import BODY, default_params  # noqa: E402
-from oracle.bindings import RefMaker, RefWorld, ref_collide, load_ref  # noqa: E402
+from oracle.bindings import RefMaker, RefWorld, ref_collide, ref_collide_pair, load_ref  # noqa: E402
 
 BODY_FIELDS = ("pos", "rot", "vel", "ang_vel")
 
@@ -80,10 +86,93 @@ def make_collide(path, count=4000, seed=11):
     print(f"{path}: {count} pairs, {int((n > 0).sum())} touching, max contacts {int(n.max())}")
 
 
+def random_shape_body(rng, maker, near=None):
+    """A body record of a random shape: box, thin static slab, sphere, capsule or cylinder."""
+    kind = int(rng.integers(0, 5))
+    q = rng.normal(size=4)
+    if rng.random() < 0.3:  # near-aligned poses exercise the flat / parallel branches
+        q = np.array([rng.normal() * 0.03, rng.normal() * 0.03, rng.normal() * 0.05, 1.0])
+    rot = (q / np.linalg.norm(q)).astype(np.float32)
+    pos = rng.uniform(-0.5, 0.5, 3) if near is None else near + rng.normal(0, 0.7, 3)
+    vel = rng.normal(0, 1.0, 3) * (8.0 if rng.random() < 0.3 else 1.0)
+    density = 0.0 if rng.random() < 0.25 else 1.0
+    if kind == 0:
+        return maker.make_box_body(rng.uniform(0.2, 2.5, 3), density, 0.5, pos, rot, vel, (0, 0, 0))
+    if kind == 1:
+        return maker.make_round_body(1, rng.uniform(0.1, 1.0), 0.0, density, 0.5, pos, rot, vel, (0, 0, 0))
+    if kind == 2:
+        return maker.make_round_body(2, rng.uniform(0.1, 0.6), rng.uniform(0.1, 1.0), density, 0.5, pos, rot, vel,
+                                     (0, 0, 0))
+    if kind == 3:
+        return maker.make_round_body(3, rng.uniform(0.1, 0.8), rng.uniform(0.1, 1.0), density, 0.5, pos, rot, vel,
+                                     (0, 0, 0))
+    slab = (rng.uniform(2.0, 6.0), rng.uniform(2.0, 6.0), rng.uniform(0.05, 0.4))
+    return maker.make_box_body(slab, 0.0, 0.5, pos, rot, (0, 0, 0), (0, 0, 0))
+
+
+def make_collide_round(path, count=6000, seed=12):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    maker = RefMaker()
+    A = np.zeros(count, BODY)
+    B = np.zeros(count, BODY)
+    n = np.zeros(count, np.int32)
+    feat = np.zeros((count, 8), np.int32)
+    ra = np.zeros((count, 8, 3), np.float32)
+    rb = np.zeros((count, 8, 3), np.float32)
+    basis = np.zeros((count, 9), np.float32)
+    dt = np.float32(1.0 / 60.0)
+    for k in range(count):
+        A[k] = random_shape_body(rng, maker)
+        B[k] = random_shape_body(rng, maker, near=A[k]["pos"])
+        cnt, c, bs = ref_collide_pair(A[k], B[k], dt)
+        n[k] = cnt
+        if cnt:
+            feat[k, :cnt] = c["feature"]
+            ra[k, :cnt] = c["r_a"]
+            rb[k, :cnt] = c["r_b"]
+            basis[k] = bs
+    np.savez_compressed(path, a=A, b=B, dt=dt, n=n, feature=feat, r_a=ra, r_b=rb, basis=basis)
+    print(f"{path}: {count} pairs, {int((n > 0).sum())} touching, max contacts {int(n.max())}")
+
+
+def round_scene_list():
+    out = scenes.round_cases(90)
+    out += [scenes.round_stack(k) for k in ("sphere", "capsule", "cylinder")]
+    out += [scenes.round_ramp(k) for k in ("sphere", "capsule", "cylinder")]
+    out += [scenes.sphere_pour(300, steps=200), scenes.mixed_pile(200, steps=200)]
+    return out
+
+
+def make_round_scenes(path):
+    out = {}
+    names = []
+    for sc in round_scene_list():
+        prm = default_params(gravity=sc.gravity)
+        w = RefWorld(prm, scene=sc)
+        every = 10 if sc.num_bodies > 100 else 1
+        traj = []
+        for k in range(1, sc.steps + 1):
+            w.step(1, colored=True)
+            if k % every == 0:
+                traj.append(body_block(w.state()))
+        names.append(sc.name)
+        for key, v in scene_arrays(sc).items():
+            out[f"{sc.name}.{key}"] = v
+        out[f"{sc.name}.every"] = np.int32(every)
+        out[f"{sc.name}.steps"] = np.int32(sc.steps)
+        out[f"{sc.name}.traj"] = np.stack(traj)
+        print(f"  {sc.name}: {sc.num_bodies} bodies, {sc.steps} steps, final manifolds {len(w.state().manifolds)}")
+    out["names"] = np.array(names)
+    np.savez_compressed(path, **out)
+    print(f"{path}: {len(names)} scenes")
+
+
 def scene_arrays(sc):
+    shape, srad, hlen = sc.shape_columns()
     return dict(s_size=sc.size, s_density=sc.density, s_friction=sc.friction, s_pos=sc.pos, s_rot=sc.rot,
                 s_vel=sc.vel, s_angvel=sc.angvel, f_type=sc.ftype, f_a=sc.fa, f_b=sc.fb, f_ra=sc.ra, f_rb=sc.rb,
-                f_p0=sc.p0, f_p1=sc.p1, f_p2=sc.p2, gravity=np.float32(sc.gravity))
+                f_p0=sc.p0, f_p1=sc.p1, f_p2=sc.p2, gravity=np.float32(sc.gravity), s_shape=shape,
+                s_shape_radius=srad, s_half_length=hlen)
 
 
 def state_arrays(prefix, st):
@@ -162,13 +251,14 @@ def make_pile(path, count=1000, pre_steps=(60, 140, 240)):
     print(f"{path}: {count} boxes")
 
 
-def main():
+def main(which=None):
     load_ref()
-    make_collide(os.path.join(HERE, "collide_boxes.npz"))
-    make_stack(os.path.join(HERE, "stack.npz"))
-    make_pyramid_chain(os.path.join(HERE, "pyramid_chain.npz"))
-    make_pile(os.path.join(HERE, "pile.npz"))
+    jobs = {"collide_boxes": make_collide, "stack": make_stack, "pyramid_chain": make_pyramid_chain, "pile": make_pile,
+            "collide_round": make_collide_round, "round_scenes": make_round_scenes}
+    for name, fn in jobs.items():
+        if not which or name in which:
+            fn(os.path.join(HERE, f"{name}.npz"))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -25,12 +25,13 @@ def ref():
     return ob
 
 
-@pytest.mark.parametrize("name", ["stack", "pile", "pyramid_chain"])
+@pytest.mark.parametrize("name", ["stack", "pile", "pyramid_chain", "mixed_round", "sphere_pour"])
 def test_hip_backend_steps_like_reference(ref, name):
     from avbd import scenes
     from avbd.state import default_params
     sc = {"stack": scenes.stack(), "pile": scenes.pile(600, seed=12),
-          "pyramid_chain": scenes.pyramid_chain(base=8, links=120)}[name]
+          "pyramid_chain": scenes.pyramid_chain(base=8, links=120),
+          "mixed_round": scenes.mixed_pile(300, seed=11), "sphere_pour": scenes.sphere_pour(400)}[name]
     prm = default_params(gravity=sc.gravity)
     gpu = ref.RefWorld(prm, scene=sc)
     assert gpu.use_hip(0), "harness built without the hip-avbd adapter"
