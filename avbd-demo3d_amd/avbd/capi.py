@@ -56,6 +56,10 @@ def load(path: str | None = None):
         lib.avbd_default_params.argtypes = [_p]
         lib.avbd_make_box_body.argtypes = [_p, C.c_float, C.c_float, _p, _p, _p, _p, _p]
         lib.avbd_make_round_body.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, _p, _p, _p, _p, _p]
+        lib.avbd_snapshot_avbs_size.restype = C.c_int64
+        lib.avbd_snapshot_avbs_size.argtypes = [_p, C.c_int32]
+        lib.avbd_snapshot_avbs.restype = C.c_int64
+        lib.avbd_snapshot_avbs.argtypes = [_p, C.c_uint64, C.c_char_p, _p, C.c_int64]
         lib.avbd_make_joint.argtypes = [_p, _p, C.c_int32, C.c_int32, C.c_int64, _p, _p, C.c_float, C.c_float,
                                         C.c_float, _p]
         lib.avbd_make_spring.argtypes = [_p, _p, C.c_int32, C.c_int32, C.c_int64, _p, _p, C.c_float, C.c_float, _p]
@@ -274,6 +278,27 @@ class HipWorld:
         out = np.zeros((n.value, 3), np.int32)
         _check(self._lib.avbd_get_hits(self._h, _ptr(out), n.value, C.byref(n)))
         return out
+
+    def snapshot_avbs(self, frame: int = 0, scene: str = "") -> bytes:
+        """AVBS v1 binary snapshot of the resident state (the viewer stream format), packed on the GPU."""
+        name = scene.encode("utf-8")
+        size = int(self._lib.avbd_snapshot_avbs_size(self._h, len(name)))
+        if size < 0:
+            _check(size)
+        buf = np.zeros(size, np.uint8)
+        n = int(self._lib.avbd_snapshot_avbs(self._h, int(frame), name, buf.ctypes.data_as(_p), size))
+        if n < 0:
+            _check(n)
+        self.synchronize()
+        return buf[:n].tobytes()
+
+    def stream_avbs(self, frames: int, every: int = 1, scene: str = "", first_frame: int = 0):
+        """Step the resident state and yield (frame, AVBS snapshot) every `every` steps."""
+        frame = first_frame
+        for _ in range(frames):
+            self.step(every)
+            frame += every
+            yield frame, self.snapshot_avbs(frame, scene)
 
     def inject_fault(self, code: int = 1):
         _check(self._lib.avbd_debug_inject_fault(self._h, code))

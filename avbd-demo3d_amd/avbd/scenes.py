@@ -325,9 +325,11 @@ def round_ramp(kind: str = "sphere", count: int = 6, angle_deg: float = 20.0, st
 
 
 def sphere_pour(count: int = 1000, width: int = 8, depth: int = 8, start_z: float = 10.0, seed: int = 7,
-                steps: int = 300) -> Scene:
-    """Small spheres poured in a jittered column, moving forward and down, onto seven standing
-    cylinders on a static ground slab."""
+                steps: int = 300, layer_scale: float = 1.0, forward: float = 1.1, down: float = -1.5,
+                taper: bool = True) -> Scene:
+    """Small spheres (r = 0.12) poured in a jittered column of width x depth per layer, moving
+    forward and down, onto seven standing cylinders on a static ground slab (the layout of the
+    reference's sphere-pour scenes, scenes.h:270-342, with this module's own random jitter)."""
     rng = _rng(seed)
     sb = _Builder()
     sb.body((1000, 1000, 1), 0.0, 0.7, (0, 0, 0))
@@ -343,12 +345,29 @@ def sphere_pour(count: int = 1000, width: int = 8, depth: int = 8, start_z: floa
     for k in range(count):
         layer, cell = divmod(k, per_layer)
         y, x = divmod(cell, width)
-        taper = 1.0 - min(layer / 18.0, 0.45)
-        px = (x - (width - 1) * 0.5) * spacing * taper + jit[k, 0] - 1.2
-        py = (y - (depth - 1) * 0.5) * spacing * taper + jit[k, 1]
-        pz = start_z + layer * spacing + jit[k, 2]
-        sb.sphere(r, 1.0, 0.45, (px, py, pz), vel=(1.1 + vj[k, 0], vj[k, 1], -1.5 + vj[k, 2]))
+        tf = 1.0 - min(layer / 18.0, 0.45) if taper else 1.0
+        px = (x - (width - 1) * 0.5) * spacing * tf + jit[k, 0] - 1.2
+        py = (y - (depth - 1) * 0.5) * spacing * tf + jit[k, 1]
+        pz = start_z + layer * spacing * layer_scale + jit[k, 2]
+        sb.sphere(r, 1.0, 0.45, (px, py, pz), vel=(forward + vj[k, 0], vj[k, 1], down + vj[k, 2]))
     return sb.scene(f"sphere_pour{count}", -9.81, steps)
+
+
+# The reference's benchmark pours (scenes.h:332-342) and their gates
+# (viewer/package.json:21-22): 600 frames; average physics step under 25 ms (5000 spheres) or
+# 30 ms (20000 spheres); max sphere-ground penetration 0.05 / 0.08; final max speed <= 25;
+# final dynamic max height <= 35.
+POUR_GATES = {
+    5000: dict(grid=(14, 8), avg_ms=25.0, penetration=0.05, max_speed=25.0, max_height=35.0),
+    20000: dict(grid=(20, 14), avg_ms=30.0, penetration=0.08, max_speed=25.0, max_height=35.0),
+}
+
+
+def sphere_pour_gate(count: int = 20000, steps: int = 600, seed: int = 7) -> Scene:
+    """The 5000- or 20000-sphere pour on cylinders the reference benchmarks."""
+    width, depth = POUR_GATES[count]["grid"]
+    return sphere_pour(count, width, depth, start_z=9.0, seed=seed, steps=steps, layer_scale=1.45, forward=0.35,
+                       down=-1.0, taper=False)
 
 
 def mixed_pile(count: int = 2000, seed: int = 8, pitch: float = 2.8, steps: int = 300) -> Scene:

@@ -51,6 +51,8 @@ struct avbd_handle
     int hashSize = 0;
     int coopIterate = 0, coopColor = 0;
     int numRound = 0, numSpheres = 0; // bodies of the uploaded state that are not boxes / are spheres
+    int *avbsRank = nullptr;          // per body: its rank in ascending dense id (AVBS snapshot order)
+    uint4 *avbsBuf = nullptr;         // AVBS body records (64 B each), allocated on first use
     int injectFault = 0;
     Bodies B{};
     Manifolds cur{}, tmp{};
@@ -898,7 +900,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     if ((r = h->alloc(B.pose, 4 * (size_t)N)) || (r = h->alloc(B.inerLin, N)) || (r = h->alloc(B.inerAng, N)) ||
         (r = h->alloc(B.vel, N)) || (r = h->alloc(B.angVel, N)) || (r = h->alloc(B.prevVel, N)) ||
         (r = h->alloc(B.massMoment, N)) || (r = h->alloc(B.sizeRad, N)) || (r = h->alloc(B.shape, N)) ||
-        (r = h->alloc(B.friction, N)) ||
+        (r = h->alloc(h->avbsRank, N)) || (r = h->alloc(B.friction, N)) ||
         (r = h->alloc(B.denseId, N)))
         return bail(r);
     if ((r = alloc_manifolds(h, h->cur, h->maxManifolds)) || (r = alloc_manifolds(h, h->tmp, h->maxManifolds)))
@@ -1060,6 +1062,14 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
         fr[i] = b.friction;
         ids[i] = b.dense_id;
     }
+    // AVBS snapshot order: bodies by ascending dense id (ties by list index); ids are fixed while
+    // the state is resident, so the ranks are computed once here
+    std::vector<int> byId(N), avbsRank(N);
+    for (int i = 0; i < N; ++i)
+        byId[i] = i;
+    std::stable_sort(byId.begin(), byId.end(), [&](int x, int y) { return ids[x] < ids[y]; });
+    for (int k = 0; k < N; ++k)
+        avbsRank[byId[k]] = k;
     hipStream_t s = h->stream;
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
@@ -1078,6 +1088,7 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
         (e = up(h->B.inerAng, na.data(), f4)) || (e = up(h->B.vel, vel.data(), f4)) || (e = up(h->B.angVel, av.data(), f4)) ||
         (e = up(h->B.prevVel, pv.data(), f4)) || (e = up(h->B.massMoment, mm.data(), f4)) ||
         (e = up(h->B.sizeRad, sr.data(), f4)) || (e = up(h->B.shape, sh.data(), f4)) ||
+        (e = up(h->avbsRank, avbsRank.data(), sizeof(int) * N)) ||
         (e = up(h->B.friction, fr.data(), sizeof(float) * N)) ||
         (e = up(h->B.denseId, ids.data(), sizeof(int) * N)))
         return fail(AVBD_ERR_HIP, std::string("upload bodies: ") + hipGetErrorString(e));
@@ -1548,6 +1559,47 @@ int avbd_get_colors(avbd_handle *h, int32_t *colors, int32_t capacity)
     if (n > 0)
         HIP_TRY(hipMemcpy(colors, h->S.color, sizeof(int) * n, hipMemcpyDeviceToHost));
     return AVBD_OK;
+}
+
+int64_t avbd_snapshot_avbs_size(avbd_handle *h, int32_t scene_bytes)
+{
+    if (!h || scene_bytes < 0)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_snapshot_avbs_size: null handle or negative scene length");
+    return AVBD_AVBS_HEADER_BYTES + (int64_t)scene_bytes + 64 * (int64_t)h->nb;
+}
+
+int64_t avbd_snapshot_avbs(avbd_handle *h, uint64_t frame, const char *scene, void *out, int64_t capacity)
+{
+    if (!h || !out)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_snapshot_avbs: null argument");
+    size_t sb = scene ? std::strlen(scene) : 0;
+    if (sb > 0xFFFFFFFFu)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_snapshot_avbs: scene name too long");
+    int64_t need = AVBD_AVBS_HEADER_BYTES + (int64_t)sb + 64 * (int64_t)h->nb;
+    if (capacity < need)
+        return fail(AVBD_ERR_CAPACITY, "avbd_snapshot_avbs: output buffer smaller than avbd_snapshot_avbs_size");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    if (!h->avbsBuf && (r = h->alloc(h->avbsBuf, 4 * (size_t)h->maxBodies)))
+        return r;
+    // header (viewer_bridge.cpp:807-817): magic, version, frame, scene bytes, body count, name
+    unsigned char *o = (unsigned char *)out;
+    uint32_t magic = AVBD_AVBS_MAGIC, version = 1, sceneBytes = (uint32_t)sb, bodies = (uint32_t)h->nb;
+    std::memcpy(o, &magic, 4);
+    std::memcpy(o + 4, &version, 4);
+    std::memcpy(o + 8, &frame, 8);
+    std::memcpy(o + 16, &sceneBytes, 4);
+    std::memcpy(o + 20, &bodies, 4);
+    if (sb)
+        std::memcpy(o + AVBD_AVBS_HEADER_BYTES, scene, sb);
+    if (h->nb > 0)
+    {
+        k_avbs_pack<<<ceil_div(h->nb, 256), 256, 0, h->stream>>>(h->B, h->avbsRank, h->nb, h->avbsBuf);
+        HIP_TRY(hipMemcpyAsync(o + AVBD_AVBS_HEADER_BYTES + sb, h->avbsBuf, 64 * (size_t)h->nb, hipMemcpyDeviceToHost,
+                               h->stream));
+    }
+    return need;
 }
 
 int avbd_get_hits(avbd_handle *h, int32_t *triples, int32_t capacity, int32_t *count)

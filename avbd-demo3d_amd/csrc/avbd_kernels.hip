@@ -2166,6 +2166,27 @@ __global__ void k_rolling(Bodies B, Manifolds M, const int *deg, const int *adjS
     B.angVel[i] = st3(w, 0.0f);
 }
 
+// AVBS body records (viewer_bridge.cpp:796-828): 64 bytes per body at its rank in ascending
+// dense id; id, shape, material (viewer_bridge.cpp:475-493), dynamic, position, rotation, size,
+// shape radius, half length. Four 16-byte stores per body.
+__global__ void k_avbs_pack(Bodies B, const int *rank, int n, uint4 *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float4 p = B.pose[4 * i + P_POS], q = B.pose[4 * i + P_ROT], sr = B.sizeRad[i], sh = B.shape[i];
+    float mass = B.massMoment[i].x;
+    int shape = (int)sh.z;
+    unsigned material = 0u;
+    if (mass > 0.0f)
+        material = shape == kShapeSphere ? 3u : shape == kShapeCylinder ? 4u : shape == kShapeCapsule ? 5u : 1u;
+    uint4 *o = out + 4 * (size_t)rank[i];
+    o[0] = make_uint4((unsigned)B.denseId[i], (unsigned)shape, material, mass > 0.0f ? 1u : 0u);
+    o[1] = make_uint4(__float_as_uint(p.x), __float_as_uint(p.y), __float_as_uint(p.z), __float_as_uint(q.x));
+    o[2] = make_uint4(__float_as_uint(q.y), __float_as_uint(q.z), __float_as_uint(q.w), __float_as_uint(sr.x));
+    o[3] = make_uint4(__float_as_uint(sr.y), __float_as_uint(sr.z), __float_as_uint(sh.x), __float_as_uint(sh.y));
+}
+
 __global__ void k_finish(Bodies B, Params P, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

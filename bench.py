@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--batch-warmup", type=int, default=20)
     ap.add_argument("--batch-sample", type=int, default=64,
                     help="worlds stepped one handle at a time for the batched-vs-separate comparison")
+    ap.add_argument("--pour-spheres", type=int, default=20000, choices=[0, 5000, 20000],
+                    help="the reference's benchmark sphere pour on cylinders; 0 skips it")
+    ap.add_argument("--pour-steps", type=int, default=600)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout measurement")
     ap.add_argument("--adjoint-steps", type=int, default=120)
     ap.add_argument("--adjoint-checkpoint", type=int, default=10)
@@ -237,6 +240,8 @@ def main():
     torch.cuda.synchronize()
     if args.batch_worlds > 0:
         result["batched"] = batched(args, params)
+    if args.pour_spheres > 0:
+        result["pour"] = pour_leg(args)
     if not args.no_adjoint:
         result["adjoint"] = adjoint_leg(args)
     print(json.dumps(result), flush=True)
@@ -354,6 +359,59 @@ def adjoint_checks(sim, st, prm, steps, pile, proj, target, pos, grads):
             f"(rel {out[k]['rel_vs_forward_mode']:.1e}) FD64 {fd: .6e} (rel {out[k]['rel_vs_fd']:.1e})")
     out["seconds"] = time.perf_counter() - t0
     return out
+
+
+def pour_leg(args):
+    """The reference's round-shape benchmark: spheres poured on seven cylinders
+    (sceneSpherePour{5000,20000}OnCylinders, scenes.h:332-342), 5 warm-up steps then the timed
+    steps (viewer/package.json:21-22: --frames 600 --warmup-frames 5), and its gates on the final
+    state. The launch shape is tuned on the pour's state at step 300, then the run restarts."""
+    import torch
+    from avbd import capi, gates, scenes
+    from avbd.state import default_params
+    sc = scenes.sphere_pour_gate(args.pour_spheres, steps=args.pour_steps)
+    gate = scenes.POUR_GATES[args.pour_spheres]
+    prm = default_params(gravity=sc.gravity)
+    start = scenes.build_state(sc, capi.Maker())
+    world = capi.HipWorld.for_state(start, prm)
+    world.step(300)
+    world.synchronize()
+    launch = world.autotune(trial_steps=3, allow_graph=False)
+    world.upload(start)
+    world.step(5)
+    world.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    world.step(args.pour_steps)
+    world.synchronize()
+    wall = time.perf_counter() - t0
+    st = world.stats()
+    final = world.state()
+    world.close()
+    m = gates.pour_metrics(final)
+    ms = wall * 1e3 / args.pour_steps
+    n_dyn = m["dynamic_bodies"]
+    checks = {"avg_step_ms": ms < gate["avg_ms"],
+              "max_sphere_ground_penetration": m.get("max_sphere_ground_penetration", 0.0) <= gate["penetration"],
+              "final_max_speed": m["final_max_speed"] <= gate["max_speed"],
+              "final_dynamic_max_height": m.get("final_dynamic_max_height", 0.0) <= gate["max_height"]}
+    log(f"pour: {args.pour_spheres} spheres, {ms:.3f} ms/step, {int(st['contact_count'])} contacts at the end, "
+        f"gates {checks}")
+    return {
+        "workload": f"sphere pour on cylinders, {args.pour_spheres} spheres (r 0.12) + 7 cylinders + ground slab; "
+                    f"dt=1/60, 10 iterations; steps 6-{5 + args.pour_steps} timed",
+        "bodies": int(len(final.bodies)),
+        "dynamic_bodies": n_dyn,
+        "steps": args.pour_steps,
+        "ms_per_step": ms,
+        "value": n_dyn * args.pour_steps / wall,
+        "unit": "body-steps/s",
+        "launch": launch,
+        "final_contacts": int(st["contact_count"]),
+        "metrics": m,
+        "reference_gate": gate,
+        "gate_passed": checks,
+    }
 
 
 def batched(args, params):

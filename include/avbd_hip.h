@@ -284,6 +284,27 @@ int avbd_get_colors(avbd_handle *h, int32_t *colors, int32_t capacity);
  * (index_a, index_b, created) triples; created = 1 when a new manifold was made. */
 int avbd_get_hits(avbd_handle *h, int32_t *triples, int32_t capacity, int32_t *count);
 
+/* ---- AVBS binary snapshots of the resident state ----
+ * The viewer stream format of makeSnapshotBinary (viewer_bridge.cpp:796-828),
+ * decoded by the viewer's decodeBinarySnapshot (viewer/src/main.js:986-1050):
+ * little-endian header { uint32 magic 0x53425641 ("AVBS"), uint32 version 1,
+ * uint64 frame, uint32 scene_bytes, uint32 body_count }, the scene name bytes,
+ * then 64 bytes per body in ascending dense_id: { uint32 id (dense_id), shape,
+ * material (materialIdForBody, viewer_bridge.cpp:475-493: 0 static, 1 box,
+ * 3 sphere, 4 cylinder, 5 capsule), dynamic; float pos[3], rot[4], size[3],
+ * shape_radius, half_length }. The body records are packed on the GPU from the
+ * resident state (no state download). */
+#define AVBD_AVBS_MAGIC 0x53425641u
+#define AVBD_AVBS_HEADER_BYTES 24
+/* Bytes of the snapshot of the current state with a scene name of scene_bytes. */
+int64_t avbd_snapshot_avbs_size(avbd_handle *h, int32_t scene_bytes);
+/* Enqueues the packing kernel and the copy into out (capacity bytes, at least
+ * avbd_snapshot_avbs_size) on the handle's stream and returns the byte count;
+ * the body records are in out once the stream reaches this point
+ * (avbd_synchronize). Pinned out (hipHostMalloc) makes the copy asynchronous,
+ * so a viewer can stream frames while the next steps run. scene may be NULL. */
+int64_t avbd_snapshot_avbs(avbd_handle *h, uint64_t frame, const char *scene, void *out, int64_t capacity);
+
 /* ---- Batched worlds ----
  * Many independent solvers in one handle. The reference runs one Solver per
  * world (class Solver, solver.h:345-420, stepped by Solver::step,
