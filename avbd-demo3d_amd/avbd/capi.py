@@ -43,6 +43,7 @@ def load(path: str | None = None):
         lib.avbd_destroy.argtypes = [_p]
         lib.avbd_set_params.argtypes = [_p, _p]
         lib.avbd_upload_state.argtypes = [_p, _p, C.c_int32, _p, C.c_int32, _p, C.c_int32, _p, C.c_int32, C.c_int64]
+        lib.avbd_set_ignore_pairs.argtypes = [_p, _p, C.c_int32]
         lib.avbd_state_counts.argtypes = [_p, _p, _p, _p, _p, _p]
         lib.avbd_download_state.argtypes = [_p, _p, _p, _p, _p]
         for n in ("avbd_synchronize", "avbd_prepare", "avbd_iterate", "avbd_finish"):
@@ -167,6 +168,7 @@ class HipWorld:
                                            len(state.bodies), _ptr(state.joints), len(state.joints),
                                            _ptr(state.springs), len(state.springs), _ptr(state.manifolds),
                                            len(state.manifolds), C.c_int64(state.next_serial)))
+        self.set_ignore_pairs(state.ignores)
         self.world_offsets = offsets
 
     @property
@@ -230,6 +232,13 @@ class HipWorld:
                                            len(state.joints), _ptr(state.springs), len(state.springs),
                                            _ptr(state.manifolds), len(state.manifolds),
                                            C.c_int64(state.next_serial)))
+        self.set_ignore_pairs(state.ignores)
+
+    def set_ignore_pairs(self, pairs):
+        """IgnoreCollision pairs (list indices) of the uploaded state."""
+        pairs = np.ascontiguousarray(np.asarray(pairs, np.int32).reshape(-1, 2))
+        self._ignores = pairs
+        _check(self._lib.avbd_set_ignore_pairs(self._h, _ptr(pairs), len(pairs)))
 
     def step(self, n: int = 1):
         _check(self._lib.avbd_step(self._h, n))

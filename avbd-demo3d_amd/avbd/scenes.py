@@ -38,6 +38,7 @@ from .state import BODY, JOINT, SPRING, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER
 
 FORCE_JOINT = 1
 FORCE_SPRING = 2
+FORCE_IGNORE = 3   # IgnoreCollision (force.cpp:75-79)
 
 
 @dataclasses.dataclass
@@ -125,6 +126,11 @@ class _Builder:
 
     def spring(self, a, b, ra, rb, stiffness, rest=-1.0):
         for k, v in zip(self.f, (FORCE_SPRING, a, b, ra, rb, stiffness, rest, 0.0)):
+            self.f[k].append(v)
+
+    def ignore(self, a, b):
+        """IgnoreCollision between bodies a and b: the broadphase never pairs them."""
+        for k, v in zip(self.f, (FORCE_IGNORE, a, b, (0, 0, 0), (0, 0, 0), 0.0, 0.0, 0.0)):
             self.f[k].append(v)
 
     def scene(self, name, gravity, steps) -> Scene:
@@ -602,9 +608,12 @@ def build_state(scene: Scene, maker) -> State:
         created[c]["dense_id"] = c
     bodies = created[::-1].copy()
     to_list = lambda c: -1 if c < 0 else n - 1 - int(c)
-    joints, springs = [], []
+    joints, springs, ignores = [], [], []
     for k in range(scene.ftype.shape[0]):
         a, b = int(scene.fa[k]), int(scene.fb[k])
+        if scene.ftype[k] == FORCE_IGNORE:
+            ignores.append((to_list(a), to_list(b)))
+            continue
         rec_a = created[a] if a >= 0 else None
         arm, rest = maker.joint_spring_derived(rec_a, created[b], scene.ra[k], scene.rb[k])
         if scene.ftype[k] == FORCE_JOINT:
@@ -622,4 +631,5 @@ def build_state(scene: Scene, maker) -> State:
             s["rest"] = scene.p1[k] if scene.p1[k] >= 0 else rest
             springs.append(s)
     return State(bodies, np.array(joints, JOINT) if joints else None,
-                 np.array(springs, SPRING) if springs else None, None, scene.ftype.shape[0])
+                 np.array(springs, SPRING) if springs else None, None, scene.ftype.shape[0],
+                 np.array(ignores, np.int32) if ignores else None)

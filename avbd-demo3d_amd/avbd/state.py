@@ -97,8 +97,11 @@ def as_bodies(arr) -> np.ndarray:
 class State:
     """A full solver state: bodies in list order, forces in ascending serial."""
 
-    def __init__(self, bodies, joints=None, springs=None, manifolds=None, next_serial=0):
+    def __init__(self, bodies, joints=None, springs=None, manifolds=None, next_serial=0, ignores=None):
         self.bodies = as_bodies(bodies)
+        # IgnoreCollision pairs (list indices), shape (n, 2)
+        self.ignores = np.ascontiguousarray(np.asarray(ignores if ignores is not None else np.zeros((0, 2)),
+                                                       np.int32).reshape(-1, 2))
         self.joints = np.ascontiguousarray(joints if joints is not None else np.zeros(0, JOINT), dtype=JOINT)
         self.springs = np.ascontiguousarray(springs if springs is not None else np.zeros(0, SPRING), dtype=SPRING)
         self.manifolds = np.ascontiguousarray(
@@ -107,7 +110,7 @@ class State:
 
     def copy(self) -> "State":
         return State(self.bodies.copy(), self.joints.copy(), self.springs.copy(), self.manifolds.copy(),
-                     self.next_serial)
+                     self.next_serial, self.ignores.copy())
 
     @property
     def num_dynamic(self) -> int:
@@ -115,12 +118,13 @@ class State:
 
     def save(self, path) -> None:
         np.savez_compressed(path, bodies=self.bodies, joints=self.joints, springs=self.springs,
-                            manifolds=self.manifolds, next_serial=np.int64(self.next_serial))
+                            manifolds=self.manifolds, next_serial=np.int64(self.next_serial), ignores=self.ignores)
 
     @staticmethod
     def load(path) -> "State":
         with np.load(path, allow_pickle=False) as z:
-            return State(z["bodies"], z["joints"], z["springs"], z["manifolds"], int(z["next_serial"]))
+            return State(z["bodies"], z["joints"], z["springs"], z["manifolds"], int(z["next_serial"]),
+                         z["ignores"] if "ignores" in z else None)
 
 
 def batch_states(states) -> tuple[State, np.ndarray]:
@@ -132,7 +136,7 @@ def batch_states(states) -> tuple[State, np.ndarray]:
     Returns (state, world_body_offsets).
     """
     offsets = np.zeros(len(states) + 1, np.int32)
-    bodies, joints, springs, manifolds = [], [], [], []
+    bodies, joints, springs, manifolds, ignores = [], [], [], [], []
     serial0 = 0
     for w, st in enumerate(states):
         b0 = int(offsets[w])
@@ -153,10 +157,11 @@ def batch_states(states) -> tuple[State, np.ndarray]:
         joints.append(j)
         springs.append(sp)
         manifolds.append(m)
+        ignores.append(st.ignores + b0)
         serial0 += st.next_serial
     cat = lambda xs, dt: np.concatenate(xs) if xs else np.zeros(0, dt)
-    return (State(cat(bodies, BODY), cat(joints, JOINT), cat(springs, SPRING), cat(manifolds, MANIFOLD), serial0),
-            offsets)
+    return (State(cat(bodies, BODY), cat(joints, JOINT), cat(springs, SPRING), cat(manifolds, MANIFOLD), serial0,
+                  np.concatenate(ignores) if ignores else None), offsets)
 
 
 def split_state(state: State, offsets) -> list:
@@ -175,7 +180,8 @@ def split_state(state: State, offsets) -> list:
         m = state.manifolds[inw(state.manifolds["body_a"])].copy()
         m["body_a"] -= b0
         m["body_b"] -= b0
-        out.append(State(state.bodies[b0:b1].copy(), j, sp, m, state.next_serial))
+        ig = state.ignores[inw(state.ignores[:, 0])] - b0
+        out.append(State(state.bodies[b0:b1].copy(), j, sp, m, state.next_serial, ig))
     return out
 
 

@@ -1158,6 +1158,8 @@ int check_adjoint_scene(avbd_handle *h)
 {
     if (h->nj > 0 || h->ns > 0)
         return fail(AVBD_ERR_UNSUPPORTED, "adjoint: contact-only worlds (no joints or springs)");
+    if (h->numIgnore > 0)
+        return fail(AVBD_ERR_UNSUPPORTED, "adjoint: worlds without IgnoreCollision pairs");
     if (h->numRound > 0)
         return fail(AVBD_ERR_UNSUPPORTED, "adjoint: box bodies only (the reverse sweep has no round-shape offsets)");
     if (h->P.iterations > 64)

@@ -53,6 +53,9 @@ struct avbd_handle
     int numRound = 0, numSpheres = 0; // bodies of the uploaded state that are not boxes / are spheres
     int *avbsRank = nullptr;          // per body: its rank in ascending dense id (AVBS snapshot order)
     uint4 *avbsBuf = nullptr;         // AVBS body records (64 B each), allocated on first use
+    int2 *ignorePairs = nullptr;      // IgnoreCollision pairs (list indices)
+    int numIgnore = 0, ignoreCap = 0;
+    int *ignoreDeg = nullptr;         // per body: its IgnoreCollision forces
     int injectFault = 0;
     Bodies B{};
     Manifolds cur{}, tmp{};
@@ -382,7 +385,8 @@ int enqueue_prepare(avbd_handle *h)
         k_sweep_count<<<G, 256, 0, s>>>(S.chunkStart, h->sweepEnd, N, ctr, S.sPos, S.hitCount, S.hitMask);
         scan(h, S.hitCount, S.hitStart, &ctr->numChunks, 0, &ctr->numHits);
         k_fill_u64<<<grid_for(h->hashSize), 256, 0, s>>>(S.hash, h->hashSize, ~0ull);
-        k_hash_insert<<<G, 256, 0, s>>>(h->cur.ab, ctr, h->joints, h->nj, h->springs, h->ns, S.hash, S.hashMask, ctr);
+        k_hash_insert<<<G, 256, 0, s>>>(h->cur.ab, ctr, h->joints, h->nj, h->springs, h->ns, h->ignorePairs,
+                                        h->numIgnore, S.hash, S.hashMask, ctr);
         k_sweep_emit<<<G, 256, 0, s>>>(S.chunkStart, N, ctr, S.hitMask, S.sIdx, S.hitStart, S.hash, S.hashMask, S.hits,
                                        S.hitNew, h->maxPairs);
         k_clamp_count<<<1, 1, 0, s>>>(&ctr->numHits, h->maxPairs, &ctr->overflow, 1);
@@ -418,7 +422,8 @@ int enqueue_prepare(avbd_handle *h)
     scan(h, S.stampCount, S.stampStart, nullptr, N, &ctr->numStamps);
     k_stamp_fill<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjPartner, h->cur.count, h->cur.ab, N,
                                                   S.stampStart, S.stamps, h->maxStamps, &ctr->overflow);
-    k_warmstart<<<ceil_div(N, 256), 256, 0, s>>>(h->B, h->P, S.deg, N, S.color, ctr);
+    const int *ignoreDeg = h->numIgnore > 0 ? h->ignoreDeg : nullptr;
+    k_warmstart<<<ceil_div(N, 256), 256, 0, s>>>(h->B, h->P, S.deg, ignoreDeg, N, S.color, ctr);
     k_step_bytes<<<1, 1, 0, s>>>(ctr, N, h->P.iterations);
 
     // ---- colouring (avbd_gpu_solver.cpp:1812-1865) ----
@@ -428,8 +433,8 @@ int enqueue_prepare(avbd_handle *h)
         int n = N, maxRounds = N + 2;
         int *color = S.color;
         GridBarrier *gb = h->gb;
-        void *args[] = {(void *)&deg, (void *)&adjStart, (void *)&adjPartner, (void *)&mm, (void *)&n, (void *)&color,
-                        (void *)&ctr, (void *)&gb, (void *)&maxRounds};
+        void *args[] = {(void *)&deg, (void *)&ignoreDeg, (void *)&adjStart, (void *)&adjPartner, (void *)&mm,
+                        (void *)&n, (void *)&color, (void *)&ctr, (void *)&gb, (void *)&maxRounds};
         int r = launch_coop((const void *)k_color, h->coopColor, kColorThreads, args, s);
         if (r)
             return r;
@@ -900,7 +905,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     if ((r = h->alloc(B.pose, 4 * (size_t)N)) || (r = h->alloc(B.inerLin, N)) || (r = h->alloc(B.inerAng, N)) ||
         (r = h->alloc(B.vel, N)) || (r = h->alloc(B.angVel, N)) || (r = h->alloc(B.prevVel, N)) ||
         (r = h->alloc(B.massMoment, N)) || (r = h->alloc(B.sizeRad, N)) || (r = h->alloc(B.shape, N)) ||
-        (r = h->alloc(h->avbsRank, N)) || (r = h->alloc(B.friction, N)) ||
+        (r = h->alloc(h->avbsRank, N)) || (r = h->alloc(h->ignoreDeg, N)) || (r = h->alloc(B.friction, N)) ||
         (r = h->alloc(B.denseId, N)))
         return bail(r);
     if ((r = alloc_manifolds(h, h->cur, h->maxManifolds)) || (r = alloc_manifolds(h, h->tmp, h->maxManifolds)))
@@ -1193,6 +1198,7 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     h->ns = num_springs;
     h->numRound = numRound;
     h->numSpheres = numSpheres;
+    h->numIgnore = 0;
     h->lastNumManifolds = M;
     h->numWorlds = 1;
     h->worldStartHost.assign({0, N});
@@ -1558,6 +1564,54 @@ int avbd_get_colors(avbd_handle *h, int32_t *colors, int32_t capacity)
     int n = std::min(capacity, h->nb);
     if (n > 0)
         HIP_TRY(hipMemcpy(colors, h->S.color, sizeof(int) * n, hipMemcpyDeviceToHost));
+    return AVBD_OK;
+}
+
+int avbd_set_ignore_pairs(avbd_handle *h, const int32_t *pairs, int32_t count)
+{
+    if (!h || count < 0 || (count > 0 && !pairs))
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_ignore_pairs: null handle or bad pair table");
+    for (int k = 0; k < count; ++k)
+    {
+        int a = pairs[2 * k], b = pairs[2 * k + 1];
+        if (a < 0 || b < 0 || a >= h->nb || b >= h->nb || a == b)
+            return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_ignore_pairs: body index out of range");
+    }
+    int r = pick_device(h);
+    if (r)
+        return r;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    // the constrained-pair hash holds manifolds, joints, springs and these pairs at most half full
+    int64_t need = 2 * ((int64_t)h->maxManifolds + h->maxJoints + h->maxSprings + count);
+    if (need > h->hashSize)
+    {
+        int hs = h->hashSize;
+        while (hs < need)
+            hs <<= 1;
+        unsigned long long *nh = nullptr;
+        if ((r = h->alloc(nh, (size_t)hs)))
+            return r;
+        h->S.hash = nh;
+        h->hashSize = hs;
+        h->S.hashMask = hs - 1;
+    }
+    if (count > h->ignoreCap)
+    {
+        if ((r = h->alloc(h->ignorePairs, (size_t)count)))
+            return r;
+        h->ignoreCap = count;
+    }
+    std::vector<int> degs(std::max(h->nb, 1), 0);
+    for (int k = 0; k < count; ++k)
+    {
+        ++degs[pairs[2 * k]];
+        ++degs[pairs[2 * k + 1]];
+    }
+    if (count > 0)
+        HIP_TRY(hipMemcpy(h->ignorePairs, pairs, sizeof(int2) * count, hipMemcpyHostToDevice));
+    if (h->nb > 0)
+        HIP_TRY(hipMemcpy(h->ignoreDeg, degs.data(), sizeof(int) * h->nb, hipMemcpyHostToDevice));
+    h->numIgnore = count;
     return AVBD_OK;
 }
 

@@ -684,10 +684,10 @@ __device__ __forceinline__ bool hash_contains(const unsigned long long *h, int m
 
 // Pairs already joined by any force are skipped by the broadphase (Rigid::constrainedTo, rigid.cpp:105-116).
 __global__ void k_hash_insert(const int2 *mab, const Counters *ctr, const Joint *joints, int nj, const Spring *springs,
-                              int ns, unsigned long long *h, int mask, Counters *ctrw)
+                              int ns, const int2 *ignores, int ni, unsigned long long *h, int mask, Counters *ctrw)
 {
     int nm = ctr->numManifolds;
-    int total = nm + nj + ns;
+    int total = nm + nj + ns + ni;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x)
     {
         int a, b;
@@ -705,11 +705,17 @@ __global__ void k_hash_insert(const int2 *mab, const Counters *ctr, const Joint 
             a = j.a;
             b = j.b;
         }
-        else
+        else if (t < nm + nj + ns)
         {
             const Spring &s = springs[t - nm - nj];
             a = s.a;
             b = s.b;
+        }
+        else
+        {
+            int2 p = ignores[t - nm - nj - ns];
+            a = p.x;
+            b = p.y;
         }
         hash_insert(h, mask, pair_key(a, b), &ctrw->overflow);
     }
@@ -1099,7 +1105,8 @@ __global__ void k_adj_sort(const int *deg, const int *adjStart, int *adj, long l
 }
 
 // Body warm start (solver.cpp:1134-1169).
-__global__ void k_warmstart(Bodies B, Params P, const int *deg, int n, int *color, Counters *ctr)
+// ignoreDeg (may be null): per body, its IgnoreCollision forces; they make a body constrained.
+__global__ void k_warmstart(Bodies B, Params P, const int *deg, const int *ignoreDeg, int n, int *color, Counters *ctr)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -1121,7 +1128,7 @@ __global__ void k_warmstart(Bodies B, Params P, const int *deg, int n, int *colo
     B.pose[4 * (i) + P_IANG] = stq(rot);
     B.inerLin[i] = st3(iner, 0.0f);
     B.inerAng[i] = stq(inerA);
-    bool solved = mass > 0.0f && deg[i] > 0;
+    bool solved = mass > 0.0f && (deg[i] > 0 || (ignoreDeg && ignoreDeg[i] > 0));
     if (mass > 0.0f)
     {
         V3 np = pos + vel * dt + V3{0.0f, 0.0f, g} * (aw * dt * dt);
@@ -1145,8 +1152,8 @@ __global__ void k_warmstart(Bodies B, Params P, const int *deg, int n, int *colo
 // (avbd_gpu_solver.cpp:1812-1865); iterating it from any start converges to that unique fixpoint
 // in at most (longest index-increasing path) rounds. Warm-started from the previous step's colours.
 // ------------------------------------------------------------------------------------------
-__global__ void k_color(const int *deg, const int *adjStart, const int *adjPartner, const float4 *massMoment, int n,
-                        int *color, Counters *ctr, GridBarrier *gb, int maxRounds)
+__global__ void k_color(const int *deg, const int *ignoreDeg, const int *adjStart, const int *adjPartner,
+                        const float4 *massMoment, int n, int *color, Counters *ctr, GridBarrier *gb, int maxRounds)
 {
     const unsigned nblocks = gridDim.x;
     __shared__ int anyChange;
@@ -1163,7 +1170,7 @@ __global__ void k_color(const int *deg, const int *adjStart, const int *adjPartn
         int changed = 0;
         for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nblocks * blockDim.x)
         {
-            if (!(massMoment[i].x > 0.0f) || deg[i] == 0)
+            if (!(massMoment[i].x > 0.0f) || (deg[i] == 0 && !(ignoreDeg && ignoreDeg[i] > 0)))
                 continue;
             unsigned long long used[4] = {0ull, 0ull, 0ull, 0ull};
             int b = adjStart[i], e = b + deg[i];

@@ -136,6 +136,7 @@ struct HipAvbdBackend : PhysicsBackend
         std::vector<avbd_joint> tj;
         std::vector<avbd_spring> ts;
         std::vector<avbd_manifold> tm;
+        std::vector<int32_t> ti; // IgnoreCollision pairs
         std::unordered_map<int64_t, Force *> bySerial;
         for (size_t k = 0; k < created.size(); ++k)
         {
@@ -207,8 +208,13 @@ struct HipAvbdBackend : PhysicsBackend
                 }
                 tm.push_back(o);
             }
+            else if (f->type == SIM_CONSTRAINT_IGNORE_COLLISION)
+            {
+                ti.push_back(ia);
+                ti.push_back(ib);
+            }
             else
-                return fallback(solver, "force type without a GPU path (ignore-collision)");
+                return fallback(solver, "force type without a GPU path");
         }
         avbd_params p{solver.dt, solver.gravity, solver.iterations, solver.alpha,
                       solver.betaLin, solver.betaAng, solver.gamma};
@@ -223,6 +229,7 @@ struct HipAvbdBackend : PhysicsBackend
         int64_t nextSerial = 0;
         if (avbd_upload_state(h, tb.data(), (int)tb.size(), tj.data(), (int)tj.size(), ts.data(), (int)ts.size(),
                               tm.data(), (int)tm.size(), (int64_t)created.size()) != AVBD_OK ||
+            avbd_set_ignore_pairs(h, ti.data(), (int32_t)(ti.size() / 2)) != AVBD_OK ||
             avbd_step(h, 1) != AVBD_OK || avbd_synchronize(h) != AVBD_OK ||
             avbd_state_counts(h, &nb, &nj, &ns, &nm, &nextSerial) != AVBD_OK)
         {

@@ -284,6 +284,15 @@ int avbd_get_colors(avbd_handle *h, int32_t *colors, int32_t capacity);
  * (index_a, index_b, created) triples; created = 1 when a new manifold was made. */
 int avbd_get_hits(avbd_handle *h, int32_t *triples, int32_t capacity, int32_t *count);
 
+/* IgnoreCollision forces (solver.h:283-290, force.cpp:75-79) of the uploaded
+ * state: body pairs (list indices, 2 * count values) the broadphase treats as
+ * already joined (Rigid::constrainedTo, rigid.cpp:105-116), so no manifold is
+ * made between them; they add nothing to the primal or dual, but a dynamic
+ * body holding one counts as constrained (solved every iteration rather than
+ * moved to its inertial target, solver.cpp:1161-1166, 1190-1191). Replaces the
+ * handle's previous set; avbd_upload_state clears it. */
+int avbd_set_ignore_pairs(avbd_handle *h, const int32_t *pairs, int32_t count);
+
 /* ---- AVBS binary snapshots of the resident state ----
  * The viewer stream format of makeSnapshotBinary (viewer_bridge.cpp:796-828),
  * decoded by the viewer's decodeBinarySnapshot (viewer/src/main.js:986-1050):

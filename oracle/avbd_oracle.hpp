@@ -1352,6 +1352,7 @@ struct World
     std::vector<Joint<R>> joints;       // ascending serial
     std::vector<Spring<R>> springs;     // ascending serial
     std::vector<Manifold<R>> manifolds; // ascending serial
+    std::vector<std::pair<int, int>> ignores; // IgnoreCollision pairs (force.cpp:75-79)
     int64_t nextSerial = 0;
     int orderMode = ORDER_COLOR;
 
@@ -1363,7 +1364,11 @@ struct World
     int colorCount = 0;
     std::vector<std::vector<ForceRef>> bodyForces; // descending serial
 
-    bool solved(int i) const { return bodies[i].mass > 0 && !bodyForces[i].empty(); }
+    // IgnoreCollision forces add nothing to a body's system but still make it constrained
+    // (solver.cpp:1161-1166, 1190-1191): body->forces counts them.
+    std::vector<int> ignoreCount;
+    bool hasForces(int i) const { return !bodyForces[i].empty() || (!ignoreCount.empty() && ignoreCount[i] > 0); }
+    bool solved(int i) const { return bodies[i].mass > 0 && hasForces(i); }
 
     // Existing force between two bodies (Rigid::constrainedTo, rigid.cpp:105-116)
     bool constrained(int i, int j, const std::vector<std::vector<int>> &partners) const
@@ -1387,6 +1392,8 @@ struct World
             add(j.a, j.b);
         for (const auto &s : springs)
             add(s.a, s.b);
+        for (const auto &g : ignores)
+            add(g.first, g.second);
     }
 
     // Sweep and prune (solver.cpp:488-577) followed by the serial, ordered apply
@@ -1633,6 +1640,12 @@ struct World
                 keptJ.push_back(j);
         joints.swap(keptJ);
         buildBodyForces();
+        ignoreCount.assign(bodies.size(), 0);
+        for (const auto &g : ignores)
+        {
+            ++ignoreCount[g.first];
+            ++ignoreCount[g.second];
+        }
 
         for (int i = 0; i < (int)bodies.size(); ++i)
         {
@@ -1653,7 +1666,7 @@ struct World
                 b.pos = b.pos + b.vel * p.dt + V3<R>{0, 0, p.gravity} * (w * p.dt * p.dt);
                 b.rot = b.rot + b.angVel * p.dt;
             }
-            if (b.mass > 0 && bodyForces[i].empty())
+            if (b.mass > 0 && !hasForces(i))
             {
                 b.pos = b.inerLin;
                 b.rot = b.inerAng;

@@ -48,6 +48,7 @@ def load_oracle():
         lib.orc_world_create.argtypes = [_i, _p, _p, _i, _p, _i, _p, _i, _p, _i, C.c_int64]
         for n in ("orc_world_destroy", "orc_prepare", "orc_iterate", "orc_finish"):
             getattr(lib, n).argtypes = [_p]
+        lib.orc_set_ignores.argtypes = [_p, _p, _i]
         lib.orc_set_order.argtypes = [_p, _i]
         lib.orc_step.argtypes = [_p, _i]
         lib.orc_counts.argtypes = [_p, _p, _p, _p, _p, _p]
@@ -88,6 +89,13 @@ def load_ref():
         lib.ref_create.argtypes = [_p, _i] + [_p] * 10 + [_i] + [_p] * 8
         lib.ref_create_from_state.restype = _p
         lib.ref_create_from_state.argtypes = [_p, _p, _i, _p, _i, _p, _i, _p, _i]
+        lib.ref_add_ignores.argtypes = [_p, _p, _i]
+        lib.ref_scene_count.restype = _i
+        lib.ref_scene_name.restype = C.c_char_p
+        lib.ref_scene_name.argtypes = [_i]
+        lib.ref_load_scene.restype = _p
+        lib.ref_load_scene.argtypes = [_p, _i]
+        lib.ref_ignore_pairs.argtypes = [_p, _p, _i]
         lib.ref_destroy.argtypes = [_p]
         lib.ref_step.argtypes = [_p, _i, _i]
         lib.ref_time_steps.argtypes = [_p, _i, _i]
@@ -160,6 +168,9 @@ class OracleWorld:
                                        _ptr(state.springs), len(state.springs), _ptr(state.manifolds),
                                        len(state.manifolds), C.c_int64(state.next_serial))
         lib.orc_set_order(self._h, order)
+        ig = np.ascontiguousarray(getattr(state, "ignores", np.zeros((0, 2), np.int32)), np.int32)
+        self._ignores = ig
+        lib.orc_set_ignores(self._h, _ptr(ig), len(ig))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -326,14 +337,25 @@ class RefMaker:
         return arm.value, rest.value
 
 
+def ref_scene_names() -> list:
+    """Names of the reference's own scenes (scenes.h sceneNames), in index order."""
+    lib = load_ref()
+    return [lib.ref_scene_name(i).decode() for i in range(lib.ref_scene_count())]
+
+
 class RefWorld:
     """The compiled reference solver (SAP broadphase), driven through its public API."""
 
-    def __init__(self, params, scene=None, state: State | None = None):
+    def __init__(self, params, scene=None, state: State | None = None, ref_scene: int | None = None):
         lib = load_ref()
         self._lib = lib
         prm = np.ascontiguousarray(params)
-        if scene is not None:
+        if ref_scene is not None:
+            self._keep = ()
+            self._h = lib.ref_load_scene(prm.ctypes.data_as(_p), int(ref_scene))
+            if not self._h:
+                raise ValueError(f"no reference scene {ref_scene}")
+        elif scene is not None:
             s = scene
             shape, srad, hlen = s.shape_columns()
             arrs = [np.ascontiguousarray(x) for x in (s.size, s.density, s.friction, s.pos, s.rot, s.vel,
@@ -348,6 +370,9 @@ class RefWorld:
                                                 _ptr(state.joints), len(state.joints), _ptr(state.springs),
                                                 len(state.springs), _ptr(state.manifolds),
                                                 len(state.manifolds))
+            ig = np.ascontiguousarray(state.ignores, np.int32)
+            if len(ig):
+                lib.ref_add_ignores(self._h, _ptr(ig), len(ig))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -370,7 +395,18 @@ class RefWorld:
         nb, nj, ns, nm = [x.value for x in v]
         b, j, s, m = np.zeros(nb, BODY), np.zeros(nj, JOINT), np.zeros(ns, SPRING), np.zeros(nm, MANIFOLD)
         self._lib.ref_get_state(self._h, _ptr(b), _ptr(j), _ptr(s), _ptr(m))
-        return State(b, j, s, m, nj + ns + nm)
+        # serials are ranks among all live forces (IgnoreCollision included); the next force
+        # created comes after every one of them
+        n_ignore = self._lib.ref_ignore_pairs(self._h, None, 0)
+        return State(b, j, s, m, nj + ns + nm + n_ignore, self.ignore_pairs() if n_ignore else None)
+
+    def ignore_pairs(self) -> np.ndarray:
+        """IgnoreCollision pairs (list indices) in creation order."""
+        n = self._lib.ref_ignore_pairs(self._h, None, 0)
+        out = np.zeros((max(n, 0), 2), np.int32)
+        if n:
+            self._lib.ref_ignore_pairs(self._h, _ptr(out), n)
+        return out
 
     def colors(self) -> np.ndarray:
         n = self._lib.ref_get_colors(self._h, None, 0)

@@ -36,6 +36,7 @@ struct WorldBase
     virtual void iterate() = 0;
     virtual void finish() = 0;
     virtual void setOrder(int mode) = 0;
+    virtual void setIgnores(const int32_t *pairs, int n) = 0;
     virtual void counts(int32_t *nb, int32_t *nj, int32_t *ns, int32_t *nm, int64_t *serial) = 0;
     virtual void get(avbd_body *b, avbd_joint *j, avbd_spring *s, avbd_manifold *m) = 0;
     virtual void getBodiesF64(double *out) = 0;
@@ -61,6 +62,12 @@ struct WorldImpl : WorldBase
     void iterate() override { w.iterate(); }
     void finish() override { w.finish(); }
     void setOrder(int mode) override { w.orderMode = mode; }
+    void setIgnores(const int32_t *pairs, int n) override
+    {
+        w.ignores.clear();
+        for (int k = 0; k < n; ++k)
+            w.ignores.push_back({pairs[2 * k], pairs[2 * k + 1]});
+    }
     void counts(int32_t *nb, int32_t *nj, int32_t *ns, int32_t *nm, int64_t *serial) override
     {
         if (nb) *nb = (int32_t)w.bodies.size();
@@ -425,6 +432,8 @@ void orc_f64_run(const avbd_params *p, const avbd_body *bodies, int nb, const av
     impl.getBodiesF64(out13);
 }
 void orc_set_order(void *w, int mode) { ((WorldBase *)w)->setOrder(mode); }
+/* IgnoreCollision pairs (list indices), replacing the previous set. */
+void orc_set_ignores(void *w, const int32_t *pairs, int n) { ((WorldBase *)w)->setIgnores(pairs, n); }
 void orc_step(void *w, int n) { ((WorldBase *)w)->step(n); }
 void orc_prepare(void *w) { ((WorldBase *)w)->prepare(); }
 void orc_iterate(void *w) { ((WorldBase *)w)->iterate(); }

@@ -20,6 +20,7 @@
  * computes.
  */
 #include "solver.h"
+#include "scenes.h"
 #include "../include/avbd_hip.h"
 #ifdef AVBD_WITH_HIP_ADAPTER
 #include "hip_avbd_backend.h"
@@ -275,7 +276,7 @@ extern "C" {
  * construction, as a scene would); shape/radius/halfLength (may be NULL: all
  * boxes) select Rigid::makeSphere / makeCapsule / makeCylinder. Forces: type
  * 1 = Joint(a,b,rA,rB,p0=kLin, p1=kAng,p2=fracture), 2 = Spring(a,b,rA,rB,
- * p0=k,p1=rest); a = -1 is the world. */
+ * p0=k,p1=rest), 3 = IgnoreCollision(a,b); a = -1 is the world. */
 void *ref_create(const avbd_params *p, int nb, const float *size, const float *density, const float *friction,
                  const float *pos, const float *rot, const float *vel, const float *angvel, const int *shape,
                  const float *shapeRadius, const float *halfLength, int nf, const int *ftype, const int *fa,
@@ -310,6 +311,8 @@ void *ref_create(const avbd_params *p, int nb, const float *size, const float *d
             new Joint(&w->solver, a, b, f3(rA + 3 * k), f3(rB + 3 * k), p0[k], p1[k], p2[k]);
         else if (ftype[k] == 2)
             new Spring(&w->solver, a, b, f3(rA + 3 * k), f3(rB + 3 * k), p0[k], p1[k]);
+        else if (ftype[k] == 3)
+            new IgnoreCollision(&w->solver, a, b);
     }
     w->colors.assign(nb, -1);
     return w;
@@ -322,6 +325,54 @@ void *ref_create_from_state(const avbd_params *p, const avbd_body *bodies, int n
 }
 
 void ref_destroy(void *h) { delete (RefWorld *)h; }
+
+/* The reference's own scenes (scenes.h:1146-1245), built by its scene functions into a
+ * solver with the given parameters. */
+int ref_scene_count(void) { return sceneCount; }
+const char *ref_scene_name(int index) { return index >= 0 && index < sceneCount ? sceneNames[index] : ""; }
+void *ref_load_scene(const avbd_params *p, int index)
+{
+    if (index < 0 || index >= sceneCount)
+        return nullptr;
+    RefWorld *w = new RefWorld();
+    applyParams(w->solver, p);
+    scenes[index](&w->solver);
+    applyParams(w->solver, p);
+    w->colors.assign(bodyList(w->solver).size(), -1);
+    return w;
+}
+
+/* IgnoreCollision pairs of the solver as list-index pairs, in creation order; returns the count. */
+int ref_ignore_pairs(void *h, int32_t *out, int cap)
+{
+    RefWorld *w = (RefWorld *)h;
+    std::vector<Rigid *> list = bodyList(w->solver);
+    std::unordered_map<Rigid *, int> index;
+    for (int i = 0; i < (int)list.size(); ++i)
+        index[list[i]] = i;
+    int n = 0;
+    for (Force *f : forceCreationOrder(w->solver))
+    {
+        if (f->type != SIM_CONSTRAINT_IGNORE_COLLISION)
+            continue;
+        if (n < cap && out)
+        {
+            out[2 * n] = index[f->bodyA];
+            out[2 * n + 1] = index[f->bodyB];
+        }
+        ++n;
+    }
+    return n;
+}
+
+/* IgnoreCollision forces between list-index body pairs (force.cpp:75-79). */
+void ref_add_ignores(void *h, const int32_t *pairs, int n)
+{
+    RefWorld *w = (RefWorld *)h;
+    std::vector<Rigid *> list = bodyList(w->solver);
+    for (int k = 0; k < n; ++k)
+        new IgnoreCollision(&w->solver, list[pairs[2 * k]], list[pairs[2 * k + 1]]);
+}
 
 /* colored = 1: reference step in colour order (see header); 0: Solver::step as is. */
 void ref_step(void *h, int n, int colored)
