@@ -265,7 +265,8 @@ class HipWorld:
         nb, nj, ns, nm, serial = self.counts()
         b, j, s, m = np.zeros(nb, BODY), np.zeros(nj, JOINT), np.zeros(ns, SPRING), np.zeros(nm, MANIFOLD)
         _check(self._lib.avbd_download_state(self._h, _ptr(b), _ptr(j), _ptr(s), _ptr(m)))
-        return State(b, j, s, m, serial)
+        ig = getattr(self, "_ignores", None)
+        return State(b, j, s, m, serial, ig if ig is not None and len(ig) else None)
 
     def stats(self) -> np.ndarray:
         out = np.zeros((), STATS)

@@ -63,7 +63,7 @@ CONFIG = np.dtype(
      ("max_manifolds", "<i4"), ("max_pairs", "<i4"), ("device", "<i4"), ("enable_timing", "<i4")],
     align=True)
 
-EXPECTED_SIZES = {"body": 164, "contact": 68, "manifold": 608, "joint": 136, "spring": 48, "params": 28}
+EXPECTED_SIZES = {"body": 172, "contact": 68, "manifold": 608, "joint": 136, "spring": 48, "params": 28}
 
 
 def default_params(gravity: float = -10.0, iterations: int = 10, dt: float = 1.0 / 60.0) -> np.ndarray:

@@ -49,7 +49,7 @@ struct avbd_handle
     int nb = 0, nj = 0, ns = 0;
     int maxBodies = 0, maxJoints = 0, maxSprings = 0, maxManifolds = 0, maxPairs = 0, maxChunks = 0, maxAdj = 0;
     int hashSize = 0;
-    int coopIterate = 0, coopColor = 0;
+    int coopColor = 0;
     int numRound = 0, numSpheres = 0; // bodies of the uploaded state that are not boxes / are spheres
     int *avbsRank = nullptr;          // per body: its rank in ascending dense id (AVBS snapshot order)
     uint4 *avbsBuf = nullptr;         // AVBS body records (64 B each), allocated on first use
@@ -235,8 +235,12 @@ struct DualArgs
     DualArgs(const avbd_handle *h, int nc_, unsigned char *flags = nullptr, unsigned *claim = nullptr);
 };
 
-// Colouring runs as one cooperative launch of a few large blocks: every round ends in a grid
-// barrier whose cost grows with the block count.
+// Colouring runs as one launch of a few large blocks (at most 64, one per compute unit, every
+// one of them resident at once on 256 CUs): every round ends in a grid barrier whose cost grows
+// with the block count, and whose spin limit turns a block that never arrives into an error.
+// The launch is an ordinary one: a cooperative launch (hipLaunchCooperativeKernel) is refused
+// with hipErrorIllegalState in a second process while another process uses cooperative launches
+// on the same GPU, which would keep two programs from stepping on one device.
 constexpr int kColorThreads = 1024;
 
 // Exclusive scan with the count either on the device (nptr) or given (nconst).
@@ -279,11 +283,6 @@ __global__ void k_clamp_count(int *count, int cap, int *overflow, int bit)
 
 __global__ void k_barrier_reset(GridBarrier *gb) { gb->count = 0u; }
 
-int launch_coop(const void *fn, int blocks, int threads, void **args, hipStream_t s)
-{
-    HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(threads), args, 0, s));
-    return AVBD_OK;
-}
 
 // Bodies at `dev` -> host records (fields the body table carries).
 constexpr int kBodyRows = 12; // the rows bodies_to_host reads
@@ -327,7 +326,7 @@ int check_device_errors(avbd_handle *h)
     HIP_TRY(hipMemcpyAsync(&c, h->ctr, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (c.barrierTimeout)
-        return fail(AVBD_ERR_HIP, "grid barrier timed out in a cooperative kernel");
+        return fail(AVBD_ERR_HIP, "grid barrier timed out in the colouring kernel");
     if (c.overflow)
     {
         const char *what = (c.overflow & 1) ? "broadphase pair buffer (max_pairs)"
@@ -428,16 +427,9 @@ int enqueue_prepare(avbd_handle *h)
 
     // ---- colouring (avbd_gpu_solver.cpp:1812-1865) ----
     {
-        const int *deg = S.deg, *adjStart = S.adjStart, *adjPartner = S.adjPartner;
-        const float4 *mm = h->B.massMoment;
-        int n = N, maxRounds = N + 2;
-        int *color = S.color;
-        GridBarrier *gb = h->gb;
-        void *args[] = {(void *)&deg, (void *)&ignoreDeg, (void *)&adjStart, (void *)&adjPartner, (void *)&mm,
-                        (void *)&n, (void *)&color, (void *)&ctr, (void *)&gb, (void *)&maxRounds};
-        int r = launch_coop((const void *)k_color, h->coopColor, kColorThreads, args, s);
-        if (r)
-            return r;
+        k_color<<<h->coopColor, kColorThreads, 0, s>>>(S.deg, ignoreDeg, S.adjStart, S.adjPartner, h->B.massMoment, N,
+                                                        S.color, ctr, h->gb, N + 2);
+        HIP_TRY(hipGetLastError());
     }
     {
         unsigned *keys = S.keys, *keysAlt = S.keysAlt;
@@ -941,7 +933,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         (e = hipMemsetAsync(S.color, 0xFF, sizeof(int) * N, h->stream)) != hipSuccess ||
         (e = hipMemsetAsync(h->dfDone, 0, sizeof(int) * N, h->stream)) != hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e)));
-    // cooperative grid sizes: co-resident blocks of 256 threads
+    // colouring grid: co-resident blocks of kColorThreads threads
     hipDeviceProp_t prop;
     if ((e = hipGetDeviceProperties(&prop, h->device)) != hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e)));

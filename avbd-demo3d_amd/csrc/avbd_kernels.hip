@@ -10,7 +10,7 @@
 //   bodies      per-body force lists in descending creation serial, body warm
 //               start                                         (solver.cpp:1134-1169)
 //   colouring   index-priority Jones-Plassmann fixpoint == sequential greedy (avbd_gpu_solver.cpp:1812-1865)
-//   iterate     persistent cooperative kernel: per colour primal, then dual (solver.cpp:1172-1273)
+//   iterate     per colour one primal launch, then one dual launch (solver.cpp:1172-1273)
 //   finish      BDF1 velocities                               (solver.cpp:1275-1298)
 //
 // Arithmetic follows the reference operation by operation (see avbd_math.hpp);
@@ -75,7 +75,7 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
     return off + incl - v;
 }
 
-// Grid-wide barrier for cooperative launches. A spin limit turns a would-be
+// Grid-wide barrier for a launch whose blocks are all resident. A spin limit turns a would-be
 // hang into an error flag; once set, every later barrier returns at once.
 __device__ __forceinline__ void grid_sync(GridBarrier *gb, unsigned nblocks, int *timeout)
 {

@@ -1,0 +1,171 @@
+"""Headless benchmark contract on the GPU (avbd/headless.py, headless_server.cpp:605-943).
+
+* The reference's own benchmark gates (viewer/package.json:21-23), run through run_benchmark with
+  the gates' arguments (warm-up 5 frames, no stream): the 5000- and 20000-sphere pours and the
+  8x8x8 soft body. The final state must equal the reference solver's (tests/golden/
+  headless_gates.json, written by make_headless_gates.py from oracle/_ref), the average physics
+  step must be under the gate's bound, and every metric gate the gate script reads
+  (headless-benchmark.mjs:544-744) must give the reference solver's verdict.
+* The backend comparison gate (package.json:24: Pyramid, Rope, Soft Body, 120 frames; position
+  error <= 0.05 max / 0.02 rms, rotation error <= 0.05) with the compiled reference solver
+  (oracle/_ref) as the reference backend: the GPU step is the reference's, so every error is 0
+  and the two final-state checksums are equal.
+* --reset-after-warmup restarts the scene: the final line equals a run without warm-up.
+* The command line prints one parseable headlessBenchmark line with the reference's keys, and
+  streams AVBS frames that decode to the final state.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+KEYS = json.load(open(os.path.join(ROOT, "tests", "golden", "headless_keys.json")))
+
+
+@pytest.fixture(scope="module")
+def hl():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from avbd import capi, headless
+    capi.load()
+    return headless
+
+
+GATE_REF = json.load(open(os.path.join(ROOT, "tests", "golden", "headless_gates.json")))
+
+
+def _digest(state, hl):
+    import hashlib
+    b = hl.dense_order(state)
+    h = hashlib.sha256()
+    for f in ("pos", "rot", "vel", "ang_vel"):
+        h.update(np.ascontiguousarray(b[f], np.float32).tobytes())
+    return h.hexdigest()
+
+
+# the gates of viewer/package.json:21-23: (field, bound, "max" | "min")
+GATES = {
+    "Sphere Pour 5000 on Cylinders": (25, [("finalMaxSphereGroundPenetration", 0.05, "max"),
+                                          ("finalMaxSpeed", 25, "max"), ("finalDynamicMaxHeight", 35, "max")]),
+    "Sphere Pour 20000 on Cylinders": (30, [("finalMaxSphereGroundPenetration", 0.08, "max"),
+                                           ("finalMaxSpeed", 25, "max"), ("finalDynamicMaxHeight", 35, "max")]),
+    "Soft Body 8x8x8": (30, [("finalMaxSpeed", 100, "max"), ("finalDynamicMaxHeight", 2.0, "min"),
+                             ("finalMaxGroundBodyPenetration", 0.05, "max")]),
+}
+
+
+@pytest.mark.parametrize("scene", list(GATES))
+def test_reference_benchmark_gates(hl, scene):
+    """The final state equals the reference solver's after the gate's frames (SHA-256 of every
+    body's pose and velocities), the average physics step is under the gate's bound, and each
+    metric gate gives the verdict it gives for the reference solver. The 20000-sphere pour's
+    penetration gate (0.08) is one the reference solver's own state does not meet (0.171): that
+    gate was set for the reference's WebGPU lane, which steps a different (non-colour-ordered)
+    solve."""
+    from avbd import named_scenes
+    want = GATE_REF[scene]
+    avg_bound, gates = GATES[scene]
+    print(f"headless gate {scene}: {want['warmup']} + {want['frames']} frames", flush=True)
+    host = hl.Host(named_scenes.index_of(scene))
+    f, line = hl.run_benchmark(scene, frames=want["frames"], warmup_frames=want["warmup"], no_stream=True, host=host)
+    state = host.world.state()
+    host.close()
+    print(line, flush=True)
+    assert _digest(state, hl) == want["digest"]
+    assert f["physicsAvgMs"] < avg_bound
+    assert f["webgpuRuntimeFallbacks"] == 0
+    for field, bound, kind in gates:
+        assert f[field] == want[field]
+        ok = f[field] <= bound if kind == "max" else f[field] >= bound
+        ref_ok = want[field] <= bound if kind == "max" else want[field] >= bound
+        print(f"  gate {field} {kind} {bound}: {f[field]:.6f} ({'pass' if ok else 'FAIL'}; reference "
+              f"{'pass' if ref_ok else 'FAIL'})", flush=True)
+        assert ok == ref_ok
+
+
+@pytest.fixture(scope="module")
+def ref():
+    import oracle.bindings as ob
+    if not ob.ref_available():
+        pytest.skip("oracle/_ref (the compiled reference harness) is not built")
+    ob.load_ref()
+    return ob
+
+
+class _RefBackend:
+    """The compiled reference solver stepping in colour order (the GPU step's order)."""
+
+    def __init__(self, ob, name):
+        from avbd import named_scenes
+        from avbd.state import default_params
+        self.w = ob.RefWorld(default_params(), ref_scene=named_scenes.index_of(name))
+
+    def step(self, n):
+        self.w.step(n, colored=True)
+
+    def state(self):
+        return self.w.state()
+
+
+@pytest.mark.parametrize("scene", ["Pyramid", "Rope", "Soft Body"])
+def test_backend_comparison_gate(hl, ref, scene):
+    print(f"backend comparison {scene}: 120 frames", flush=True)
+    f, line = hl.run_backend_comparison(scene, 120, _RefBackend(ref, scene))
+    print(line, flush=True)
+    assert list(json.loads(line)) == KEYS["backendComparison"]
+    assert f["bodyCountMismatch"] == 0 and f["comparedBodies"] == f["referenceBodies"] > 0
+    for k in ("Position", "Rotation", "LinearVelocity", "AngularVelocity"):
+        assert f[f"max{k}Error"] == 0.0 and f[f"rms{k}Error"] == 0.0
+    for k in ("Position", "Rotation", "LinearVelocity", "AngularVelocity"):
+        assert f[f"reference{k}Checksum"] == f[f"candidate{k}Checksum"]
+    assert f["candidateWebgpuRuntimeFallbacks"] == 0
+
+
+def test_reset_after_warmup_restarts_scene(hl):
+    print("reset after warm-up: Soft Body, 7 + 20 frames", flush=True)
+    a, _ = hl.run_benchmark("Soft Body", frames=20, warmup_frames=7, reset_after_warmup=True)
+    b, _ = hl.run_benchmark("Soft Body", frames=20)
+    c, _ = hl.run_benchmark("Soft Body", frames=20, warmup_frames=7)
+    for k in ("finalPositionChecksum", "finalRotationChecksum", "finalLinearVelocityChecksum",
+              "finalAngularVelocityChecksum"):
+        assert a[k] == b[k]
+        assert c[k] != b[k]
+
+
+def test_cli_line_and_avbs_stream(hl, tmp_path):
+    from avbd import named_scenes
+    from avbd.scenes import build_state
+    from avbd import capi
+    from avbd.state import default_params
+    import oracle.avbs as avbs
+    out = tmp_path / "frames.avbs"
+    cmd = [sys.executable, os.path.join(ROOT, "avbd-demo3d_amd", "avbd_headless.py"), "--benchmark-scene",
+           "Newton's Cradle", "--benchmark-frames", "30", "--warmup-frames", "3", "--reset-after-warmup",
+           "--physics-backend", "hip-avbd", "--avbs-out", str(out), "--phase-timing"]
+    print("headless CLI: Newton's Cradle, 30 frames, streaming to a file", flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    line = next(l for l in res.stdout.splitlines() if '"headlessBenchmark"' in l)
+    f = json.loads(line)
+    assert list(f) == KEYS["headlessBenchmark"]
+    assert f["scene"] == "Newton's Cradle" and f["frames"] == 30 and f["streaming"] is True
+    assert f["snapshotsSent"] == 30 and f["snapshotMode"] == "binary"
+    assert f["avbdGpuIterateAvgMs"] > 0 and f["webgpuRuntimeFallbacks"] == 0
+    data = out.read_bytes()
+    assert len(data) == 30 * f["snapshotBytes"]
+    # the last frame is the final state: compare with the same 30 steps stepped here
+    st = build_state(named_scenes.build("Newton's Cradle"), capi.Maker())
+    w = capi.HipWorld.for_state(st, default_params())
+    w.step(30)
+    w.synchronize()
+    expect = avbs.encode_avbs(w.state().bodies, 29, "Newton's Cradle")
+    w.close()
+    assert data[-f["snapshotBytes"]:] == expect
+    frames = [avbs.decode_avbs(data[i * f["snapshotBytes"]:(i + 1) * f["snapshotBytes"]]) for i in range(30)]
+    assert [fr["frame"] for fr in frames] == list(range(30))
