@@ -359,21 +359,14 @@ def sphere_pour(count: int = 1000, width: int = 8, depth: int = 8, start_z: floa
     return sb.scene(f"sphere_pour{count}", -9.81, steps)
 
 
-# The reference's benchmark pours (scenes.h:332-342) and their gates
-# (viewer/package.json:21-22): 600 frames; average physics step under 25 ms (5000 spheres) or
+# The gates of the reference's benchmark pours (scenes.h:332-342 built by avbd/named_scenes.py;
+# viewer/package.json:21-22): 600 frames; average physics step under 25 ms (5000 spheres) or
 # 30 ms (20000 spheres); max sphere-ground penetration 0.05 / 0.08; final max speed <= 25;
 # final dynamic max height <= 35.
 POUR_GATES = {
-    5000: dict(grid=(14, 8), avg_ms=25.0, penetration=0.05, max_speed=25.0, max_height=35.0),
-    20000: dict(grid=(20, 14), avg_ms=30.0, penetration=0.08, max_speed=25.0, max_height=35.0),
+    5000: dict(avg_ms=25.0, penetration=0.05, max_speed=25.0, max_height=35.0),
+    20000: dict(avg_ms=30.0, penetration=0.08, max_speed=25.0, max_height=35.0),
 }
-
-
-def sphere_pour_gate(count: int = 20000, steps: int = 600, seed: int = 7) -> Scene:
-    """The 5000- or 20000-sphere pour on cylinders the reference benchmarks."""
-    width, depth = POUR_GATES[count]["grid"]
-    return sphere_pour(count, width, depth, start_z=9.0, seed=seed, steps=steps, layer_scale=1.45, forward=0.35,
-                       down=-1.0, taper=False)
 
 
 def mixed_pile(count: int = 2000, seed: int = 8, pitch: float = 2.8, steps: int = 300) -> Scene:

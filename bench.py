@@ -367,9 +367,9 @@ def pour_leg(args):
     steps (viewer/package.json:21-22: --frames 600 --warmup-frames 5), and its gates on the final
     state. The launch shape is tuned on the pour's state at step 300, then the run restarts."""
     import torch
-    from avbd import capi, gates, scenes
+    from avbd import capi, gates, named_scenes, scenes
     from avbd.state import default_params
-    sc = scenes.sphere_pour_gate(args.pour_spheres, steps=args.pour_steps)
+    sc = named_scenes.build(f"Sphere Pour {args.pour_spheres} on Cylinders")
     gate = scenes.POUR_GATES[args.pour_spheres]
     prm = default_params(gravity=sc.gravity)
     start = scenes.build_state(sc, capi.Maker())
@@ -398,8 +398,8 @@ def pour_leg(args):
     log(f"pour: {args.pour_spheres} spheres, {ms:.3f} ms/step, {int(st['contact_count'])} contacts at the end, "
         f"gates {checks}")
     return {
-        "workload": f"sphere pour on cylinders, {args.pour_spheres} spheres (r 0.12) + 7 cylinders + ground slab; "
-                    f"dt=1/60, 10 iterations; steps 6-{5 + args.pour_steps} timed",
+        "workload": f"{sc.name}: {args.pour_spheres} spheres (r 0.12) + 7 cylinders + ground slab, the "
+                    f"reference's scene; dt=1/60, 10 iterations, gravity -10; steps 6-{5 + args.pour_steps} timed",
         "bodies": int(len(final.bodies)),
         "dynamic_bodies": n_dyn,
         "steps": args.pour_steps,
@@ -411,6 +411,9 @@ def pour_leg(args):
         "metrics": m,
         "reference_gate": gate,
         "gate_passed": checks,
+        "gate_note": ("the reference solver's own final state after these steps misses the penetration gate "
+                      "(0.171 > 0.08) and this run's final state equals it bit for bit; the gate was set for the "
+                      "reference's WebGPU lane") if args.pour_spheres == 20000 else "",
     }
 
 
