@@ -50,6 +50,10 @@ struct avbd_handle
     int maxBodies = 0, maxJoints = 0, maxSprings = 0, maxManifolds = 0, maxPairs = 0, maxChunks = 0, maxAdj = 0;
     int hashSize = 0;
     int coopColor = 0;
+    int4 *padStamps = nullptr;   // stamps in task order, kPadStampsMax per task at most
+    unsigned long long *scanStatus = nullptr;
+    unsigned *scanCtl = nullptr;
+    int scanMaxTiles = 0;
     int numRound = 0, numSpheres = 0; // bodies of the uploaded state that are not boxes / are spheres
     int *avbsRank = nullptr;          // per body: its rank in ascending dense id (AVBS snapshot order)
     uint4 *avbsBuf = nullptr;         // AVBS body records (64 B each), allocated on first use
@@ -152,7 +156,7 @@ int alloc_manifolds(avbd_handle *h, Manifolds &m, int cap)
 int grid_for(int n) { return std::max(1, std::min(ceil_div(n, 256), 4096)); }
 
 using PrimalKernel = void (*)(Bodies, Manifolds, const Joint *, const Spring *, const int4 *, const int4 *, int, int,
-                              Params, float4 *, unsigned char *);
+                              Params, float4 *, unsigned char *, const int4 *, int);
 
 // The primal kernel's launch shapes: lanes per body x threads per block.
 PrimalKernel primal_kernel(int lanes, int threads)
@@ -216,8 +220,11 @@ struct PrimalArgs
     Params P;
     float4 *sysTape;
     unsigned char *flagTape;
-    void *ptr[11];
-    PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys = nullptr, unsigned char *flags = nullptr);
+    const int4 *pad;
+    int padShift;
+    void *ptr[13];
+    PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys = nullptr, unsigned char *flags = nullptr,
+               const int4 *pad_ = nullptr, int padShift_ = 0);
 };
 
 struct DualArgs
@@ -252,9 +259,8 @@ void scan(avbd_handle *h, const int *in, int *out, const int *nptr, int nconst, 
         k_scan_small<<<1, 1024, 0, s>>>(in, out, nconst, total);
         return;
     }
-    k_scan_reduce<<<kScanBlocks, 256, 0, s>>>(in, nptr, nconst, h->S.partial);
-    k_scan_partials<<<1, kScanBlocks, 0, s>>>(h->S.partial, kScanBlocks, total);
-    k_scan_down<<<kScanBlocks, 256, 0, s>>>(in, out, nptr, nconst, h->S.partial);
+    k_scan_lookback<<<kLbBlocks, kLbThreads, 0, s>>>(in, out, nptr, nconst, total, h->scanStatus, h->scanCtl,
+                                                    h->scanMaxTiles, &h->ctr->barrierTimeout);
 }
 
 // Stable LSD radix sort of (key, value); `passes` 8-bit digits from bit 0. Result in keys/vals.
@@ -326,7 +332,8 @@ int check_device_errors(avbd_handle *h)
     HIP_TRY(hipMemcpyAsync(&c, h->ctr, sizeof(Counters), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (c.barrierTimeout)
-        return fail(AVBD_ERR_HIP, "grid barrier timed out in the colouring kernel");
+        return fail(AVBD_ERR_HIP, (c.barrierTimeout & 2) ? "a scan's look-back timed out"
+                                                           : "a grid barrier or dataflow wait timed out");
     if (c.overflow)
     {
         const char *what = (c.overflow & 1) ? "broadphase pair buffer (max_pairs)"
@@ -417,7 +424,8 @@ int enqueue_prepare(avbd_handle *h)
     k_adj<true><<<G, 256, 0, s>>>(h->cur, ctr, h->joints, h->nj, h->springs, h->ns, h->B.massMoment, S.deg, S.adjStart,
                                   S.adjCursor, S.adj, S.adjSerial, S.adjPartner, h->maxAdj, &ctr->overflow);
     k_adj_sort<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjSerial, S.adjPartner, N, h->maxAdj);
-    k_stamp_count<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, h->cur.count, N, S.stampCount);
+    k_stamp_count<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, h->cur.count, N, S.stampCount,
+                                                   &ctr->maxStampCount);
     scan(h, S.stampCount, S.stampStart, nullptr, N, &ctr->numStamps);
     k_stamp_fill<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjPartner, h->cur.count, h->cur.ab, N,
                                                   S.stampStart, S.stamps, h->maxStamps, &ctr->overflow);
@@ -453,11 +461,12 @@ int enqueue_prepare(avbd_handle *h)
     return AVBD_OK;
 }
 
-PrimalArgs::PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys, unsigned char *flags)
+PrimalArgs::PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys, unsigned char *flags, const int4 *pad_,
+                       int padShift_)
     : B(h->B), M(h->cur), joints(h->joints), springs(h->springs), stamps(h->S.stamps), tasks(h->S.tasks), begin(b),
-      end(e), P(h->P), sysTape(sys), flagTape(flags)
+      end(e), P(h->P), sysTape(sys), flagTape(flags), pad(pad_), padShift(padShift_)
 {
-    void *p[11] = {&B, &M, &joints, &springs, &stamps, &tasks, &begin, &end, &P, &sysTape, &flagTape};
+    void *p[13] = {&B, &M, &joints, &springs, &stamps, &tasks, &begin, &end, &P, &sysTape, &flagTape, &pad, &padShift};
     std::memcpy(ptr, p, sizeof(p));
 }
 
@@ -617,6 +626,17 @@ int enqueue_iterate(avbd_handle *h)
         for (int col = 0; col < colors; ++col)
             numPadded += ceil_div(h->hostOffsets[col + 1] - h->hostOffsets[col], kDataflowPad) * kDataflowPad;
         const int dfBlocks = std::max(1, std::min(ceil_div(numPadded, groups), 2048));
+        // task-ordered stamps at a power-of-two stride (direct per-colour launches, no adjoint tape)
+        int padShift = -1;
+        if (!dataflow && !tape && numTasks > 0 && c.maxStampCount <= kPadStampsMax)
+        {
+            padShift = 0;
+            while ((1 << padShift) < std::max(c.maxStampCount, lc.primal_lanes))
+                ++padShift;
+            const long long work = (long long)numTasks << padShift;
+            k_pad_stamps<<<(int)std::min<long long>(ceil_div(work, 256LL), 8192), 256, 0, s>>>(
+                h->S.tasks, h->S.stamps, numTasks, padShift, h->padStamps);
+        }
         if (dataflow)
         {
             k_tasks_pad_clear<<<grid_for(numPadded), 256, 0, s>>>(h->dfTasks, numPadded);
@@ -641,7 +661,8 @@ int enqueue_iterate(avbd_handle *h)
             else
                 for (const auto &rg : ranges)
                 {
-                    PrimalArgs a(h, rg.first, rg.second, sys, sfl);
+                    PrimalArgs a(h, rg.first, rg.second, sys, sfl, padShift >= 0 ? h->padStamps : nullptr,
+                                 std::max(padShift, 0));
                     HIP_TRY(hipLaunchKernel(pfn, dim3(ceil_div(rg.second - rg.first, groups)),
                                             dim3(lc.primal_threads), a.ptr, 0, s));
                 }
@@ -906,6 +927,8 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         return bail(r);
     Scratch &S = h->S;
     int tiles = ceil_div(N, kRadixTile);
+    // look-back scan status: one word per tile of the longest scanned array
+    h->scanMaxTiles = ceil_div(std::max({N, h->maxChunks, h->maxPairs, h->maxManifolds, 256 * tiles}), kLbTile);
     if ((r = h->alloc(S.axisSums, 64 * 6)) || (r = h->alloc(S.adjPartner, h->maxAdj)) ||
         (r = h->alloc(S.stampCount, N)) || (r = h->alloc(S.stampStart, N)) || (r = h->alloc(S.stamps, h->maxStamps)) ||
         (r = h->alloc(S.tasks, N)) || (r = h->alloc(S.contactStart, h->maxManifolds)) ||
@@ -920,7 +943,9 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         (r = h->alloc(S.keepRank, h->maxManifolds)) || (r = h->alloc(S.deg, N)) || (r = h->alloc(S.adjStart, N)) ||
         (r = h->alloc(S.adjCursor, N)) || (r = h->alloc(S.adj, h->maxAdj)) || (r = h->alloc(S.adjSerial, h->maxAdj)) ||
         (r = h->alloc(S.color, N)) || (r = h->alloc(S.colorOffsets, 257)) || (r = h->alloc(S.colorBodies, N)) ||
-        (r = h->alloc(S.partial, kScanBlocks + 1)) || (r = h->alloc(h->ctr, 1)) || (r = h->alloc(h->gb, 1)) ||
+        (r = h->alloc(h->ctr, 1)) || (r = h->alloc(h->gb, 1)) ||
+        (r = h->alloc(h->scanStatus, (size_t)h->scanMaxTiles)) || (r = h->alloc(h->scanCtl, 4)) ||
+        (r = h->alloc(h->padStamps, (size_t)N * kPadStampsMax)) ||
         (r = h->alloc(h->adjTotal, 1)) || (r = h->alloc(h->worldStart, (size_t)N + 1)) || (r = h->alloc(h->dfDone, N)) ||
         (r = h->alloc(h->dfTasks, (size_t)N + (size_t)kDataflowPad * kMaxColors)))
         return bail(r);
@@ -930,6 +955,8 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         return bail(fail(AVBD_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e)));
     if ((e = hipMemsetAsync(h->ctr, 0, sizeof(Counters), h->stream)) != hipSuccess ||
         (e = hipMemsetAsync(h->gb, 0, sizeof(GridBarrier), h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(h->scanStatus, 0, sizeof(unsigned long long) * h->scanMaxTiles, h->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(h->scanCtl, 0, sizeof(unsigned) * 4, h->stream)) != hipSuccess ||
         (e = hipMemsetAsync(S.color, 0xFF, sizeof(int) * N, h->stream)) != hipSuccess ||
         (e = hipMemsetAsync(h->dfDone, 0, sizeof(int) * N, h->stream)) != hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipMemsetAsync: ") + hipGetErrorString(e)));

@@ -14,7 +14,7 @@ namespace avbd
 constexpr int kMaxContacts = 8;
 constexpr int kMaxColors = 255;      // colour 255 is the "unsolved" bucket of the colour sort
 constexpr int kSweepChunk = 64;      // sorted intervals visited per sweep work item
-constexpr int kScanBlocks = 256;     // fixed block count of the device-count scans
+constexpr int kPadStampsMax = 64;  // largest per-task stride of the task-ordered stamp copy
 constexpr int kRadixTile = 2048;     // keys per radix-sort block (256 threads x 8)
 constexpr int kAdjManifold = 0;
 constexpr int kAdjJoint = 1;
@@ -118,6 +118,7 @@ struct Counters
     int contactCount;
     int contactEndpoints;   // sum over contacts of their dynamic endpoints
     int numStamps;
+    int maxStampCount;      // the most stamps of one body this step
     int jointLive;
     int jointEndpoints;
     int springEndpoints;    // fixed by the uploaded springs
@@ -187,7 +188,6 @@ struct Scratch
     int *colorOffsets;      // [256]
     int *colorBodies;       // [N]
     // scan partials
-    int *partial;           // [kScanBlocks + 1]
 };
 
 inline __host__ __device__ int adjKind(int e) { return (int)((unsigned)e >> 30); }

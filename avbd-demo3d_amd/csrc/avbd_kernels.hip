@@ -110,56 +110,8 @@ __device__ __forceinline__ void grid_sync(GridBarrier *gb, unsigned nblocks, int
 }
 
 // ------------------------------------------------------------------------------------------
-// scans with a device-side count (reduce / partials / down-sweep, fixed block count)
+// scans: one block for short arrays with a host-known length, single-pass look-back otherwise
 // ------------------------------------------------------------------------------------------
-__global__ void k_scan_reduce(const int *in, const int *nptr, int nconst, int *partial)
-{
-    __shared__ int lds[16];
-    int n = nptr ? *nptr : nconst;
-    int chunk = (n + gridDim.x - 1) / gridDim.x;
-    int begin = blockIdx.x * chunk;
-    int end = min(begin + chunk, n);
-    int s = 0;
-    for (int i = begin + threadIdx.x; i < end; i += blockDim.x)
-        s += in[i];
-    int tot;
-    block_exclusive_scan(s, lds, tot);
-    if (threadIdx.x == 0)
-        partial[blockIdx.x] = tot;
-}
-
-__global__ void k_scan_partials(int *partial, int nparts, int *total)
-{
-    __shared__ int lds[16];
-    int v = (int)threadIdx.x < nparts ? partial[threadIdx.x] : 0;
-    int tot;
-    int ex = block_exclusive_scan(v, lds, tot);
-    if ((int)threadIdx.x < nparts)
-        partial[threadIdx.x] = ex;
-    if (threadIdx.x == 0 && total)
-        *total = tot;
-}
-
-__global__ void k_scan_down(const int *in, int *out, const int *nptr, int nconst, const int *partial)
-{
-    __shared__ int lds[16];
-    int n = nptr ? *nptr : nconst;
-    int chunk = (n + gridDim.x - 1) / gridDim.x;
-    int begin = blockIdx.x * chunk;
-    int end = min(begin + chunk, n);
-    int running = partial[blockIdx.x];
-    for (int base = begin; base < end; base += blockDim.x)
-    {
-        int i = base + threadIdx.x;
-        int v = i < end ? in[i] : 0;
-        int tot;
-        int ex = block_exclusive_scan(v, lds, tot);
-        if (i < end)
-            out[i] = running + ex;
-        running += tot;
-    }
-}
-
 // Exclusive scan of up to kScanSmallMax ints in one block of 1024 threads: tiles of 4096
 // (4 consecutive elements per thread, coalesced), each scanned across the block and carried
 // into the next. One launch instead of three for the small histogram scans.
@@ -188,6 +140,158 @@ __global__ __launch_bounds__(1024) void k_scan_small(const int *in, int *out, in
     }
     if (threadIdx.x == 0 && total)
         *total = carry;
+}
+
+// Single-pass exclusive scan with decoupled look-back, for counts that live on the device.
+// A fixed grid of blocks claims tiles of kLbTile ints in increasing order from a device counter;
+// each tile publishes its own sum at once (flag 1) and, after looking back over its
+// predecessors until one carries an inclusive prefix (flag 2), its inclusive prefix. A tile waits
+// only for tiles claimed before it by blocks that are running, and those publish their sums
+// without waiting on anything, so the look-back always ends; a spin limit turns a missing
+// status into the timeout flag. Status words: epoch (30 bits) | flag (2) | value (32). The epoch
+// is read from ctl[2] at the start of a launch and advanced by the launch's last block, which
+// also resets the tile and exit counters (ctl[0], ctl[1]), so no host-side clearing (and no
+// memset) is needed between scans and the launch can be replayed from a graph.
+constexpr int kLbThreads = 256;
+constexpr int kLbItems = 8;
+constexpr int kLbTile = kLbThreads * kLbItems;
+constexpr int kLbBlocks = 128;
+constexpr int kLbSpin = 1 << 22;
+
+__device__ __forceinline__ void lb_publish(unsigned long long *w, unsigned epoch, unsigned flag, int value)
+{
+    unsigned long long v = ((unsigned long long)epoch << 34) | ((unsigned long long)flag << 32) | (unsigned)value;
+    __hip_atomic_store(w, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kLbThreads) void k_scan_lookback(const int *in, int *out, const int *nptr, int nconst,
+                                                              int *total, unsigned long long *status, unsigned *ctl,
+                                                              int maxTiles, int *timeout)
+{
+    __shared__ int lds[kLbThreads / 64];
+    __shared__ int sTile, sPrefix;
+    const int n = nptr ? *nptr : nconst;
+    const int numTiles = min((max(n, 0) + kLbTile - 1) / kLbTile, maxTiles);
+    const unsigned epoch = __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0x3FFFFFFFu;
+    if (numTiles == 0 && blockIdx.x == 0 && threadIdx.x == 0 && total)
+        *total = 0;
+    const bool vec = ((((uintptr_t)in) | ((uintptr_t)out)) & 15) == 0;
+    for (;;)
+    {
+        if (threadIdx.x == 0)
+            sTile = (int)atomicAdd(&ctl[0], 1u);
+        __syncthreads();
+        const int t = sTile;
+        __syncthreads();
+        if (t >= numTiles)
+            break;
+        const int base = t * kLbTile + (int)threadIdx.x * kLbItems;
+        int v[kLbItems];
+        if (vec && base + kLbItems <= n)
+        {
+            int4 a = *(const int4 *)(in + base), b = *(const int4 *)(in + base + 4);
+            v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < kLbItems; ++q)
+                v[q] = base + q < n ? in[base + q] : 0;
+        }
+        int sum = 0;
+#pragma unroll
+        for (int q = 0; q < kLbItems; ++q)
+            sum += v[q];
+        int tot;
+        const int ex = block_exclusive_scan(sum, lds, tot);
+        if (threadIdx.x < 64)
+        {
+            // wave 0 looks back over 64 predecessors per round: lane j reads tile p - j
+            const int lane = threadIdx.x;
+            int prefix = 0;
+            if (t == 0)
+            {
+                if (lane == 0)
+                    lb_publish(&status[0], epoch, 2u, tot);
+            }
+            else
+            {
+                if (lane == 0)
+                    lb_publish(&status[t], epoch, 1u, tot);
+                int p = t - 1, spins = 0;
+                for (;;)
+                {
+                    const int q = p - lane;
+                    unsigned fl = 2u;
+                    int val = 0;
+                    if (q >= 0)
+                    {
+                        unsigned long long w = __hip_atomic_load(&status[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                        fl = (unsigned)(w >> 34) == epoch ? (unsigned)(w >> 32) & 3u : 0u;
+                        val = (int)(unsigned)w;
+                    }
+                    const unsigned long long incl = __ballot(fl == 2u);
+                    const unsigned long long wait = __ballot(fl == 0u);
+                    const int first = incl ? __ffsll((long long)incl) - 1 : 63;   // nearest inclusive prefix
+                    const unsigned long long upto = first == 63 ? ~0ull : ((2ull << first) - 1ull);
+                    if (wait & upto)
+                    {
+                        __builtin_amdgcn_s_sleep(1);
+                        if (++spins > kLbSpin || __hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                        {
+                            if (lane == 0)
+                                atomicOr(timeout, 2);
+                            break;
+                        }
+                        continue;
+                    }
+                    prefix += wave_sum(lane <= first ? val : 0);
+                    if (incl)
+                        break;
+                    p -= 64;
+                }
+                if (lane == 0)
+                    lb_publish(&status[t], epoch, 2u, prefix + tot);
+            }
+            if (lane == 0)
+            {
+                sPrefix = prefix;
+                if (t == numTiles - 1 && total)
+                    *total = prefix + tot;
+            }
+        }
+        __syncthreads();
+        int run = sPrefix + ex;
+        int o[kLbItems];
+#pragma unroll
+        for (int q = 0; q < kLbItems; ++q)
+        {
+            o[q] = run;
+            run += v[q];
+        }
+        if (vec && base + kLbItems <= n)
+        {
+            *(int4 *)(out + base) = make_int4(o[0], o[1], o[2], o[3]);
+            *(int4 *)(out + base + 4) = make_int4(o[4], o[5], o[6], o[7]);
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < kLbItems; ++q)
+                if (base + q < n)
+                    out[base + q] = o[q];
+        }
+    }
+    if (threadIdx.x == 0)
+    {
+        __threadfence();
+        if (atomicAdd(&ctl[1], 1u) == gridDim.x - 1)
+        {
+            atomicExch(&ctl[0], 0u);
+            atomicExch(&ctl[1], 0u);
+            atomicAdd(&ctl[2], 1u);
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -292,6 +396,7 @@ __global__ void k_step_begin(Counters *c)
     c->contactEndpoints = 0;
     c->jointLive = 0;
     c->jointEndpoints = 0;
+    c->maxStampCount = 0;
     c->colorChanged[0] = c->colorChanged[1] = c->colorChanged[2] = 0u;
 }
 
@@ -1174,14 +1279,23 @@ __global__ void k_color(const int *deg, const int *ignoreDeg, const int *adjStar
                 continue;
             unsigned long long used[4] = {0ull, 0ull, 0ull, 0ull};
             int b = adjStart[i], e = b + deg[i];
-            for (int x = b; x < e; ++x)
+            // partners in batches of 8: all partner indices, then all their colours, are in flight
+            // together instead of one dependent round trip per partner
+            for (int x0 = b; x0 < e; x0 += 8)
             {
-                int o = adjPartner[x];
-                if (o < 0 || o >= i)
-                    continue;
-                int c = __hip_atomic_load(&color[o], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (c >= 0 && c < 256)
-                    used[c >> 6] |= 1ull << (c & 63);
+                int o[8], c[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    o[q] = x0 + q < e ? adjPartner[x0 + q] : -1;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    c[q] = (o[q] >= 0 && o[q] < i)
+                               ? __hip_atomic_load(&color[o[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : -1;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (c[q] >= 0 && c[q] < 256)
+                        used[c[q] >> 6] |= 1ull << (c[q] & 63);
             }
             int c = 0;
             for (int w = 0; w < 4; ++w)
@@ -1623,20 +1737,43 @@ __device__ __forceinline__ void spring_record(float *r, int i, const Spring &sp,
 // Per-body stamp lists, built once per step from the force lists: each manifold contributes
 // one record per contact, each joint three, each spring one, in force order.
 __global__ void k_stamp_count(const int *deg, const int *adjStart, const int *adj, const int *mcount, int n,
-                              int *stampCount)
+                              int *stampCount, int *maxStampCount)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
     int s = 0;
-    int b = adjStart[i], e = b + deg[i];
-    for (int x = b; x < e; ++x)
+    if (i < n)
     {
-        int en = adj[x];
-        int kind = adjKind(en);
-        s += kind == kAdjManifold ? mcount[adjIndex(en)] : (kind == kAdjJoint ? 3 : 1);
+        int b = adjStart[i], e = b + deg[i];
+        for (int x = b; x < e; ++x)
+        {
+            int en = adj[x];
+            int kind = adjKind(en);
+            s += kind == kAdjManifold ? mcount[adjIndex(en)] : (kind == kAdjJoint ? 3 : 1);
+        }
+        stampCount[i] = s;
     }
-    stampCount[i] = s;
+    int m = s;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        m = max(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0)
+        atomicMax(maxStampCount, m);
+}
+
+// Stamps in colour-sorted task order at a fixed stride S per task (S a power of two at least the
+// largest stamp count): the primal then reads task k's stamps at k * S + t, an address that does
+// not wait for the task record. Entries past a task's count are never read.
+__global__ void k_pad_stamps(const int4 *tasks, const int4 *stamps, int numTasks, int shiftS, int4 *padded)
+{
+    const int S = 1 << shiftS;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < (long long)numTasks << shiftS;
+         idx += (long long)gridDim.x * blockDim.x)
+    {
+        int k = (int)(idx >> shiftS), t = (int)(idx & (S - 1));
+        int4 task = tasks[k];
+        if (t < task.z)
+            padded[idx] = stamps[task.y + t];
+    }
 }
 
 __global__ void k_stamp_fill(const int *deg, const int *adjStart, const int *adj, const int *adjPartner,
@@ -1716,11 +1853,14 @@ __device__ __forceinline__ void wave_lds_sync()
 // stride), the owner lanes add each system entry in stamp order, lane 0 solves and stores.
 // Every (L, T) gives the same sums. Co: the dataflow form (coherent partner poses and pose
 // stores).
+// pad (may be null): the stamps at stride 2^padShift per task, task k = kpos; st0 is this lane's
+// first padded stamp, loaded by the caller next to the task record.
 template <int L, int T, bool Co>
 __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool valid, int4 task, const Bodies &B,
                                             const Manifolds &M, const Joint *joints, const Spring *springs,
                                             const int4 *stamps, const Params &P, float4 *sysTape,
-                                            unsigned char *flagTape)
+                                            unsigned char *flagTape, const int4 *pad = nullptr, int padShift = 0,
+                                            int kpos = 0, int4 st0 = make_int4(0, 0, 0, 0))
 {
     static_assert(L >= 4 && L <= 64 && (64 % L) == 0 && (T % 64) == 0, "a body's lanes must sit in one wave");
     constexpr int E = (kRec + L - 1) / L;  // accumulated entries per lane
@@ -1752,7 +1892,13 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
         float *row = rec + (g * L + lane) * kRow;
         if (valid && t < ns)
         {
-            int4 st = stamps[s0 + t];
+            int4 st;
+            if (!pad)
+                st = stamps[s0 + t];
+            else if (r == 0)
+                st = st0;
+            else
+                st = pad[((size_t)kpos << padShift) + t];
             int kind = adjKind(st.x), idx = adjIndex(st.x);
             if (kind == kAdjManifold)
             {
@@ -1858,10 +2004,13 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
 // Bodies of a colour are sorted by stamp count, so the groups of a wave need similar round counts.
 // sysTape / flagTape (adjoint recording, null otherwise): per body the accumulated system and
 // dx (9 float4), per stamp the contact's branch flags.
+// pad (may be null): the stamps in task order at stride 2^padShift (k_pad_stamps); a lane's first
+// stamp is then loaded together with the task record instead of after it.
 template <int L, int T>
 __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const Joint *joints, const Spring *springs,
                                                     const int4 *stamps, const int4 *tasks, int begin, int end, Params P,
-                                                    float4 *sysTape, unsigned char *flagTape)
+                                                    float4 *sysTape, unsigned char *flagTape, const int4 *pad,
+                                                    int padShift)
 {
     constexpr int G = T / L;
     __shared__ float rec[T * (kRec + 2)];
@@ -1869,7 +2018,9 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
     const int k = begin + blockIdx.x * G + g;
     const bool valid = k < end;
     const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
-    primal_task<L, T, false>(rec, g, lane, valid, task, B, M, joints, springs, stamps, P, sysTape, flagTape);
+    const int4 st0 = (valid && pad) ? pad[((size_t)k << padShift) + lane] : make_int4(0, 0, 0, 0);
+    primal_task<L, T, false>(rec, g, lane, valid, task, B, M, joints, springs, stamps, P, sysTape, flagTape, pad,
+                             padShift, k, st0);
 }
 
 // Dataflow form of one iteration's primal (contact-only worlds): one launch covers all colours.
