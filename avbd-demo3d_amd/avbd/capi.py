@@ -182,10 +182,12 @@ class HipWorld:
         _check(self._lib.avbd_get_launch_config(self._h, out.ctypes.data_as(_p)))
         return {k: int(out[k]) for k in LAUNCH.names}
 
-    def set_launch_config(self, primal_lanes=16, primal_threads=256, dual_threads=256, use_graph=0, dataflow=0):
+    def set_launch_config(self, primal_lanes=16, primal_threads=256, dual_threads=256, use_graph=0, dataflow=0,
+                          lazy_dual=1):
         c = np.zeros((), LAUNCH)
         c["primal_lanes"], c["primal_threads"] = primal_lanes, primal_threads
         c["dual_threads"], c["use_graph"], c["dataflow"] = dual_threads, int(use_graph), int(dataflow)
+        c["lazy_dual"] = int(lazy_dual)
         _check(self._lib.avbd_set_launch_config(self._h, c.ctypes.data_as(_p)))
 
     def autotune(self, trial_steps: int = 3, allow_graph: bool = True) -> dict:

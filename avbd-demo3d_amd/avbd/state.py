@@ -186,7 +186,7 @@ def split_state(state: State, offsets) -> list:
 
 
 LAUNCH = np.dtype([("primal_lanes", "<i4"), ("primal_threads", "<i4"), ("dual_threads", "<i4"),
-                   ("use_graph", "<i4"), ("dataflow", "<i4")], align=True)
+                   ("use_graph", "<i4"), ("dataflow", "<i4"), ("lazy_dual", "<i4")], align=True)
 
 
 def check_layouts(sizeof, records: int = 7) -> None:

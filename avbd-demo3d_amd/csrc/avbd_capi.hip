@@ -96,7 +96,7 @@ struct avbd_handle
     int *worldStart = nullptr;
     std::vector<int> worldStartHost;
     // launch shapes of the iteration kernels, and the iteration graph built from them
-    avbd_launch_config launch{16, 256, 256, 0, 0};
+    avbd_launch_config launch{16, 256, 256, 0, 0, 1};
     int *dfDone = nullptr;  // dataflow primal: last iteration id each body finished
     int4 *dfTasks = nullptr;  // colour-sorted tasks, each colour padded to kDataflowPad
     int dfIter = 0;
@@ -156,7 +156,7 @@ int alloc_manifolds(avbd_handle *h, Manifolds &m, int cap)
 int grid_for(int n) { return std::max(1, std::min(ceil_div(n, 256), 4096)); }
 
 using PrimalKernel = void (*)(Bodies, Manifolds, const Joint *, const Spring *, const int4 *, const int4 *, int, int,
-                              Params, float4 *, unsigned char *, const int4 *, int);
+                              Params, float4 *, unsigned char *, const int4 *, int, int);
 
 // The primal kernel's launch shapes: lanes per body x threads per block.
 PrimalKernel primal_kernel(int lanes, int threads)
@@ -204,7 +204,8 @@ bool valid_launch(const avbd_launch_config &c)
 {
     auto threads = [](int t) { return t == 64 || t == 128 || t == 256; };
     return primal_kernel(c.primal_lanes, c.primal_threads) != nullptr && threads(c.dual_threads) &&
-           (c.use_graph == 0 || c.use_graph == 1) && (c.dataflow == 0 || c.dataflow == 1);
+           (c.use_graph == 0 || c.use_graph == 1) && (c.dataflow == 0 || c.dataflow == 1) &&
+           (c.lazy_dual == 0 || c.lazy_dual == 1);
 }
 
 // Kernel argument blocks (hipLaunchKernel / graph kernel nodes take an array of argument pointers).
@@ -222,9 +223,10 @@ struct PrimalArgs
     unsigned char *flagTape;
     const int4 *pad;
     int padShift;
-    void *ptr[13];
+    int lazyDual;
+    void *ptr[14];
     PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys = nullptr, unsigned char *flags = nullptr,
-               const int4 *pad_ = nullptr, int padShift_ = 0);
+               const int4 *pad_ = nullptr, int padShift_ = 0, int lazyDual_ = 0);
 };
 
 struct DualArgs
@@ -238,8 +240,10 @@ struct DualArgs
     Params P;
     unsigned char *flagTape;
     unsigned *claimReset;
-    void *ptr[9];
-    DualArgs(const avbd_handle *h, int nc_, unsigned char *flags = nullptr, unsigned *claim = nullptr);
+    int staticRepeat;
+    void *ptr[10];
+    DualArgs(const avbd_handle *h, int nc_, unsigned char *flags = nullptr, unsigned *claim = nullptr, int nj_ = -1,
+             int staticRepeat_ = 1);
 };
 
 // Colouring runs as one launch of a few large blocks (at most 64, one per compute unit, every
@@ -462,19 +466,20 @@ int enqueue_prepare(avbd_handle *h)
 }
 
 PrimalArgs::PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys, unsigned char *flags, const int4 *pad_,
-                       int padShift_)
+                       int padShift_, int lazyDual_)
     : B(h->B), M(h->cur), joints(h->joints), springs(h->springs), stamps(h->S.stamps), tasks(h->S.tasks), begin(b),
-      end(e), P(h->P), sysTape(sys), flagTape(flags), pad(pad_), padShift(padShift_)
+      end(e), P(h->P), sysTape(sys), flagTape(flags), pad(pad_), padShift(padShift_), lazyDual(lazyDual_)
 {
-    void *p[13] = {&B, &M, &joints, &springs, &stamps, &tasks, &begin, &end, &P, &sysTape, &flagTape, &pad, &padShift};
+    void *p[14] = {&B, &M, &joints, &springs, &stamps, &tasks, &begin, &end, &P, &sysTape, &flagTape, &pad, &padShift,
+                   &lazyDual};
     std::memcpy(ptr, p, sizeof(p));
 }
 
-DualArgs::DualArgs(const avbd_handle *h, int nc_, unsigned char *flags, unsigned *claim)
-    : B(h->B), M(h->cur), joints(h->joints), nj(h->nj), contactRec(h->S.contactRec), nc(nc_), P(h->P),
-      flagTape(flags), claimReset(claim)
+DualArgs::DualArgs(const avbd_handle *h, int nc_, unsigned char *flags, unsigned *claim, int nj_, int staticRepeat_)
+    : B(h->B), M(h->cur), joints(h->joints), nj(nj_ < 0 ? h->nj : nj_), contactRec(h->S.contactRec), nc(nc_), P(h->P),
+      flagTape(flags), claimReset(claim), staticRepeat(staticRepeat_)
 {
-    void *p[9] = {&B, &M, &joints, &nj, &contactRec, &nc, &P, &flagTape, &claimReset};
+    void *p[10] = {&B, &M, &joints, &nj, &contactRec, &nc, &P, &flagTape, &claimReset, &staticRepeat};
     std::memcpy(ptr, p, sizeof(p));
 }
 
@@ -635,8 +640,13 @@ int enqueue_iterate(avbd_handle *h)
                 ++padShift;
             const long long work = (long long)numTasks << padShift;
             k_pad_stamps<<<(int)std::min<long long>(ceil_div(work, 256LL), 8192), 256, 0, s>>>(
-                h->S.tasks, h->S.stamps, numTasks, padShift, h->padStamps);
+                h->S.tasks, h->S.stamps, h->S.color, numTasks, padShift, h->padStamps);
         }
+        // lazy dual: contact duals move into the next iteration's primal (owner stamps), joints keep
+        // their own dual launch per iteration, and one contact dual follows the last iteration
+        const bool lazy = lc.lazy_dual && padShift >= 0 && c.contactCount > 0 && h->P.iterations > 0;
+        if (lazy)
+            k_lazy_bytes<<<1, 1, 0, s>>>(h->ctr, h->P.iterations);
         if (dataflow)
         {
             k_tasks_pad_clear<<<grid_for(numPadded), 256, 0, s>>>(h->dfTasks, numPadded);
@@ -662,13 +672,28 @@ int enqueue_iterate(avbd_handle *h)
                 for (const auto &rg : ranges)
                 {
                     PrimalArgs a(h, rg.first, rg.second, sys, sfl, padShift >= 0 ? h->padStamps : nullptr,
-                                 std::max(padShift, 0));
+                                 std::max(padShift, 0), lazy && it > 0 ? 1 : 0);
                     HIP_TRY(hipLaunchKernel(pfn, dim3(ceil_div(rg.second - rg.first, groups)),
                                             dim3(lc.primal_threads), a.ptr, 0, s));
                 }
             if (mark && it == last)
                 HIP_TRY(hipEventRecord(mark[5], s));
-            if (dualLanes > 0)
+            if (lazy)
+            {
+                if (h->nj > 0)
+                {
+                    DualArgs a(h, 0, nullptr, nullptr);
+                    HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(h->nj, lc.dual_threads)),
+                                            dim3(lc.dual_threads), a.ptr, 0, s));
+                }
+                if (it == last)
+                {
+                    DualArgs a(h, c.contactCount, nullptr, nullptr, 0, h->P.iterations);
+                    HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(c.contactCount, lc.dual_threads)),
+                                            dim3(lc.dual_threads), a.ptr, 0, s));
+                }
+            }
+            else if (dualLanes > 0)
             {
                 DualArgs a(h, c.contactCount, dfl, dataflow ? &h->ctr->dfClaim : nullptr);
                 HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(dualLanes, lc.dual_threads)),
@@ -1853,6 +1878,19 @@ int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launc
     {
         avbd_launch_config c = best;
         c.dual_threads = threads;
+        float ms = 0.0f;
+        if ((r = trial(c, ms)))
+            return r;
+        if (ms < bestMs)
+        {
+            bestMs = ms;
+            best = c;
+        }
+    }
+    for (int lz : {0, 1})
+    {
+        avbd_launch_config c = best;
+        c.lazy_dual = lz;
         float ms = 0.0f;
         if ((r = trial(c, ms)))
             return r;

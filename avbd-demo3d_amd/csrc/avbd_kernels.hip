@@ -1117,6 +1117,21 @@ __global__ void k_step_bytes(Counters *c, int nBodies, int iterations)
     c->cumSteps += 1;
 }
 
+// Byte accounting of the lazy-dual form, applied to the step's figures once it is chosen: in
+// iterations 1.. the contact duals ride on the primal's loads and add only their penalty/lambda
+// stores (32 B per contact point) to the primal, and one dual launch remains.
+__global__ void k_lazy_bytes(Counters *c, int iterations)
+{
+    double moved = (double)(iterations - 1) * (double)c->contactCount;
+    double saved = moved * (248.0 - 32.0);
+    c->cumPrimalBytes += moved * 32.0;
+    c->cumDualBytes -= moved * 248.0;
+    c->lastIterBytes -= saved;
+    c->lastStepBytes -= saved;
+    c->cumIterBytes -= saved;
+    c->cumStepBytes -= saved;
+}
+
 // ------------------------------------------------------------------------------------------
 // per-body force lists, descending creation serial (the order force.cpp:25-36 leaves them in)
 // ------------------------------------------------------------------------------------------
@@ -1505,12 +1520,14 @@ __device__ __forceinline__ void contact_anchors(float4 rA4, float4 rB4, float4 c
 
 // Co: the partner's current pose is read coherently (dataflow primal); the own body's rows and
 // the constant rows (x-) are ordinary loads.
+// dualP (lazy-dual form, null otherwise): this body owns the contact point's dual update, which
+// is applied first.
 template <bool Co = false>
 __device__ __forceinline__ int contact_record(float *r, int i, int partner, bool isA, int sl, const Manifolds &M,
-                                              const Bodies &B, float oma)
+                                              const Bodies &B, float oma, const Params *dualP = nullptr)
 {
     int2 ab = isA ? make_int2(i, partner) : make_int2(partner, i);
-    const float4 *cr = M.cr + (size_t)sl * kCR;
+    float4 *cr = M.cr + (size_t)sl * kCR;
     float4 rA4 = cr[CR_RA], rB4 = cr[CR_RB], c04 = cr[CR_C0], bb0 = cr[CR_N], bb1 = cr[CR_T1], bb2 = cr[CR_T2];
     float4 pen4 = cr[CR_PEN], lam4 = cr[CR_LAM];
     auto cur = [&](int b, int row) { return (Co && b == partner) ? ld_coherent(&B.pose[4 * b + row]) : B.pose[4 * b + row]; };
@@ -1530,6 +1547,28 @@ __device__ __forceinline__ int contact_record(float *r, int i, int partner, bool
     V3 pen = ld3(pen4);
     ContactEval ev = contact_eval(ld3(c04), pen, ld3(lam4), rA4.w, oma, b0, b1, b2, nb0, nb1, nb2, ja0, ja1, ja2, jb0,
                                   jb1, jb2, dqAL, dqBL, dqAA, dqBA);
+    if (dualP)
+    {
+        // the previous iteration's dual update of this contact point (Manifold::updateDual,
+        // manifold.cpp:231-280), done here because neither body has moved since that iteration
+        // ended; then the record from the updated penalty and lambda
+        const Params &P = *dualP;
+        float stick = lam4.w;
+        if (ev.F0 < 0.0f)
+            pen.x = fminr(pen.x + P.betaLin * fabsf(ev.C0), kPenaltyMax);
+        if (ev.fs <= ev.bounds)
+        {
+            float k1 = pen.y + P.betaLin * fabsf(ev.C1), k2 = pen.z + P.betaLin * fabsf(ev.C2);
+            pen.y = fminr(k1, kPenaltyMax);
+            pen.z = fminr(k2, kPenaltyMax);
+            stick = sqrtf(ev.C1 * ev.C1 + ev.C2 * ev.C2) < kStickThresh ? 1.0f : 0.0f;
+        }
+        lam4 = make_float4(ev.F0, ev.F1, ev.F2, stick);
+        cr[CR_LAM] = lam4;
+        cr[CR_PEN] = st3(pen, 0.0f);
+        ev = contact_eval(ld3(c04), pen, ld3(lam4), rA4.w, oma, b0, b1, b2, nb0, nb1, nb2, ja0, ja1, ja2, jb0, jb1, jb2,
+                          dqAL, dqBL, dqAA, dqBA);
+    }
     if (isA)
         rows_record(r, b0, b1, b2, ja0, ja1, ja2, pen, ev.F0, ev.F1, ev.F2);
     else
@@ -1762,8 +1801,12 @@ __global__ void k_stamp_count(const int *deg, const int *adjStart, const int *ad
 
 // Stamps in colour-sorted task order at a fixed stride S per task (S a power of two at least the
 // largest stamp count): the primal then reads task k's stamps at k * S + t, an address that does
-// not wait for the task record. Entries past a task's count are never read.
-__global__ void k_pad_stamps(const int4 *tasks, const int4 *stamps, int numTasks, int shiftS, int4 *padded)
+// not wait for the task record. Entries past a task's count are never read. A contact stamp gets
+// kStampDualOwner when its body updates the contact's dual in the lazy-dual form: the
+// lower-colour body of the pair, or the only solved one (colour -1 marks an unsolved partner).
+constexpr int kStampDualOwner = 1 << 17;
+__global__ void k_pad_stamps(const int4 *tasks, const int4 *stamps, const int *color, int numTasks, int shiftS,
+                             int4 *padded)
 {
     const int S = 1 << shiftS;
     for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < (long long)numTasks << shiftS;
@@ -1772,7 +1815,16 @@ __global__ void k_pad_stamps(const int4 *tasks, const int4 *stamps, int numTasks
         int k = (int)(idx >> shiftS), t = (int)(idx & (S - 1));
         int4 task = tasks[k];
         if (t < task.z)
-            padded[idx] = stamps[task.y + t];
+        {
+            int4 st = stamps[task.y + t];
+            if (adjKind(st.x) == kAdjManifold)
+            {
+                int cp = color[st.z], ci = color[task.x];
+                if (cp < 0 || ci < cp)
+                    st.y |= kStampDualOwner;
+            }
+            padded[idx] = st;
+        }
     }
 }
 
@@ -1860,7 +1912,7 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
                                             const Manifolds &M, const Joint *joints, const Spring *springs,
                                             const int4 *stamps, const Params &P, float4 *sysTape,
                                             unsigned char *flagTape, const int4 *pad = nullptr, int padShift = 0,
-                                            int kpos = 0, int4 st0 = make_int4(0, 0, 0, 0))
+                                            int kpos = 0, int4 st0 = make_int4(0, 0, 0, 0), bool lazyDual = false)
 {
     static_assert(L >= 4 && L <= 64 && (64 % L) == 0 && (T % 64) == 0, "a body's lanes must sit in one wave");
     constexpr int E = (kRec + L - 1) / L;  // accumulated entries per lane
@@ -1902,7 +1954,9 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
             int kind = adjKind(st.x), idx = adjIndex(st.x);
             if (kind == kAdjManifold)
             {
-                int fl = contact_record<Co>(row, i, st.z, (st.y >> 16) != 0, st.w, M, B, oma);
+                const bool owner = lazyDual && (st.y & kStampDualOwner) != 0;
+                int fl = contact_record<Co>(row, i, st.z, ((st.y >> 16) & 1) != 0, st.w, M, B, oma,
+                                            owner ? &P : nullptr);
                 if (flagTape)
                     flagTape[s0 + t] = (unsigned char)fl;
             }
@@ -2005,12 +2059,14 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
 // sysTape / flagTape (adjoint recording, null otherwise): per body the accumulated system and
 // dx (9 float4), per stamp the contact's branch flags.
 // pad (may be null): the stamps in task order at stride 2^padShift (k_pad_stamps); a lane's first
-// stamp is then loaded together with the task record instead of after it.
+// stamp is then loaded together with the task record instead of after it. lazyDual (with pad only):
+// a stamp marked kStampDualOwner applies its contact point's dual update of the previous iteration
+// before building its record.
 template <int L, int T>
 __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const Joint *joints, const Spring *springs,
                                                     const int4 *stamps, const int4 *tasks, int begin, int end, Params P,
                                                     float4 *sysTape, unsigned char *flagTape, const int4 *pad,
-                                                    int padShift)
+                                                    int padShift, int lazyDual)
 {
     constexpr int G = T / L;
     __shared__ float rec[T * (kRec + 2)];
@@ -2020,7 +2076,7 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
     const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
     const int4 st0 = (valid && pad) ? pad[((size_t)k << padShift) + lane] : make_int4(0, 0, 0, 0);
     primal_task<L, T, false>(rec, g, lane, valid, task, B, M, joints, springs, stamps, P, sysTape, flagTape, pad,
-                             padShift, k, st0);
+                             padShift, k, st0, lazyDual != 0);
 }
 
 // Dataflow form of one iteration's primal (contact-only worlds): one launch covers all colours.
@@ -2231,15 +2287,24 @@ __device__ __forceinline__ void ctr_reset_claim(unsigned *claim)
 }
 
 // claimReset (may be null): the dataflow primal's claim counter, cleared for the next iteration.
+// staticRepeat (lazy-dual form's closing launch, 1 otherwise): a contact between two bodies that
+// are never solved (both static) has no owner stamp to carry its per-iteration dual updates; its
+// bodies never move, so its updates of all iterations are applied here one after another.
 __global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int4 *contactRec, int nc, Params P,
-                       unsigned char *flagTape, unsigned *claimReset)
+                       unsigned char *flagTape, unsigned *claimReset, int staticRepeat)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0)
         ctr_reset_claim(claimReset);
     if (t < nc)
     {
-        int fl = dual_contact(contactRec[t], M, B, P);
+        const int4 cr = contactRec[t];
+        int reps = 1;
+        if (staticRepeat > 1 && !(B.massMoment[cr.z].x > 0.0f) && !(B.massMoment[cr.w].x > 0.0f))
+            reps = staticRepeat;
+        int fl = 0;
+        for (int r = 0; r < reps; ++r)
+            fl = dual_contact(cr, M, B, P);
         if (flagTape)
             flagTape[t] = (unsigned char)fl;
     }

@@ -182,7 +182,9 @@ def main():
     achieved = pr_bytes / (pr_ms * 1e-3) / 1e9
     du_launches = max(int(st["dual_launches"]), 1)
     du_ms = float(st["dual_ms_total"]) / du_launches
-    du_bytes = float(st["dual_bytes_total"]) / steps_counted / iters / (du_launches / timed)
+    # dual launches per step: one per iteration, or (lazy dual, contact-only worlds) one per step
+    du_per_step = 1 if launch.get("lazy_dual") else iters
+    du_bytes = float(st["dual_bytes_total"]) / steps_counted / du_per_step / (du_launches / timed)
     loop_gbs = iter_bytes / (iter_ms * 1e-3) / 1e9
     log(f"k_primal_color {pr_ms * 1e3:.2f} us/launch {pr_bytes / 1e6:.2f} MB/launch -> {achieved:.0f} GB/s; "
         f"k_dual {du_ms * 1e3:.2f} us/launch {du_bytes / 1e6:.2f} MB/launch -> "

@@ -348,6 +348,10 @@ typedef struct avbd_launch_config {
     int32_t dataflow;        /* 1: one primal launch per iteration for all colours, bodies
                                 starting as their lower-colour neighbours finish
                                 (contact-only worlds; otherwise ignored) */
+    int32_t lazy_dual;       /* 1: each contact point's dual update is applied at the start of
+                                the next iteration's primal by the first of its bodies to be
+                                solved, and one dual launch after the last iteration covers
+                                them all (direct launches only; results are identical) */
 } avbd_launch_config;
 int avbd_set_launch_config(avbd_handle *h, const avbd_launch_config *cfg);
 int avbd_get_launch_config(avbd_handle *h, avbd_launch_config *out);

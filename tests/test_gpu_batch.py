@@ -78,17 +78,40 @@ def test_every_launch_shape_gives_the_same_state(gpu):
     ref.step(40)
     base = ref.state()
     ref.close()
-    shapes = [(l, t, d, gr, 0) for l in (8, 16, 32) for t in (64, 128, 256) for d, gr in ((64, 0), (256, 1))]
-    shapes += [(l, t, 128, 0, 1) for l, t in ((8, 128), (16, 64), (16, 256), (32, 256))]
-    for l, t, d, gr, df in shapes:
+    shapes = [(l, t, d, gr, 0, 1) for l in (8, 16, 32) for t in (64, 128, 256) for d, gr in ((64, 0), (256, 1))]
+    shapes += [(l, t, 128, 0, 1, 1) for l, t in ((8, 128), (16, 64), (16, 256), (32, 256))]
+    shapes += [(l, t, 256, 0, 0, 0) for l, t in ((8, 64), (16, 128), (32, 256))]
+    for l, t, d, gr, df, lz in shapes:
         w = gpu.HipWorld.for_state(st, prm)
-        w.set_launch_config(l, t, d, use_graph=gr, dataflow=df)
+        w.set_launch_config(l, t, d, use_graph=gr, dataflow=df, lazy_dual=lz)
         assert w.launch_config() == {"primal_lanes": l, "primal_threads": t, "dual_threads": d, "use_graph": gr,
-                                     "dataflow": df}
+                                     "dataflow": df, "lazy_dual": lz}
         w.step(40)
         w.synchronize()
-        assert_states_equal(w.state(), base, f"lanes {l} threads {t} dual {d} graph {gr} dataflow {df}")
+        assert_states_equal(w.state(), base, f"lanes {l} threads {t} dual {d} graph {gr} dataflow {df} lazy {lz}")
         w.close()
+
+
+def test_lazy_dual_equals_eager_with_joints_and_springs(gpu):
+    """The lazy-dual form (contact duals applied by their owner body at the next iteration's
+    primal, joint duals per iteration, one contact dual at the end) equals the per-iteration dual
+    launches bit for bit, on a pyramid with a chain of joints and springs and on the 8x8x8 soft
+    body (joints, IgnoreCollision pairs, ground contacts)."""
+    from avbd import named_scenes, scenes
+    from avbd.state import default_params
+    cases = [(scenes.pyramid_chain(base=12, links=200), 60), (named_scenes.build("Soft Body 8x8x8"), 40)]
+    for sc, steps in cases:
+        prm = default_params(gravity=sc.gravity)
+        st = scenes.build_state(sc, gpu.Maker())
+        out = []
+        for lz in (0, 1):
+            w = gpu.HipWorld.for_state(st, prm)
+            w.set_launch_config(16, 64, 256, 0, 0, lazy_dual=lz)
+            w.step(steps)
+            w.synchronize()
+            out.append(w.state())
+            w.close()
+        assert_states_equal(out[1], out[0], f"{sc.name}: lazy vs eager dual")
 
 
 def test_dataflow_primal_at_headline_size(gpu):

@@ -72,16 +72,17 @@ def test_round_every_launch_shape_identical(gpu):
     prm = default_params(gravity=sc.gravity)
     st = scenes.build_state(sc, gpu.Maker())
     base = None
-    for lanes, threads, dual, graph in [(16, 256, 256, 0), (8, 64, 64, 0), (32, 128, 128, 1), (16, 64, 256, 1)]:
+    for lanes, threads, dual, graph, lazy in [(16, 256, 256, 0, 1), (8, 64, 64, 0, 1), (32, 128, 128, 1, 1),
+                                              (16, 64, 256, 1, 1), (16, 256, 256, 0, 0), (8, 64, 64, 0, 0)]:
         w = gpu.HipWorld.for_state(st, prm)
-        w.set_launch_config(lanes, threads, dual, graph, 0)
+        w.set_launch_config(lanes, threads, dual, graph, 0, lazy)
         w.step(60)
         out = w.state()
         w.close()
         if base is None:
             base = out
         else:
-            assert_states_equal(out, base, f"shape L{lanes} T{threads} D{dual} G{graph}")
+            assert_states_equal(out, base, f"shape L{lanes} T{threads} D{dual} G{graph} lazy {lazy}")
 
 
 def test_round_worlds_batched_equal_reference(gpu):
