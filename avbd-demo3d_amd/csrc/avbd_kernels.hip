@@ -1272,68 +1272,88 @@ __global__ void k_warmstart(Bodies B, Params P, const int *deg, const int *ignor
 // (avbd_gpu_solver.cpp:1812-1865); iterating it from any start converges to that unique fixpoint
 // in at most (longest index-increasing path) rounds. Warm-started from the previous step's colours.
 // ------------------------------------------------------------------------------------------
+// One launch of a few large blocks (grid barrier between rounds). Block b evaluates the bodies
+// b*blockDim + t (+ k*gridDim*blockDim): in a global round it re-evaluates its own bodies up to
+// kColorLocalPasses times, with block barriers only, until none of them changes, so colour
+// changes that chain through neighbouring indices settle inside the block instead of costing a
+// grid-wide round each. The fixpoint is unique (each body's colour depends only on lower-index
+// neighbours), so the order of evaluation does not change the result; the colouring ends after
+// a global round in which no block changed any colour.
+constexpr int kColorLocalPasses = 8;
 __global__ void k_color(const int *deg, const int *ignoreDeg, const int *adjStart, const int *adjPartner,
                         const float4 *massMoment, int n, int *color, Counters *ctr, GridBarrier *gb, int maxRounds)
 {
     const unsigned nblocks = gridDim.x;
-    __shared__ int anyChange;
+    __shared__ int localChange, roundChange;
     int rounds = 0;
     for (int r = 0; r < maxRounds; ++r)
     {
         if (threadIdx.x == 0)
         {
-            anyChange = 0;
+            roundChange = 0;
             if (blockIdx.x == 0)
                 ctr->colorChanged[(r + 1) % 3] = 0u;
         }
-        __syncthreads();
-        int changed = 0;
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nblocks * blockDim.x)
+        for (int pass = 0; pass < kColorLocalPasses; ++pass)
         {
-            if (!(massMoment[i].x > 0.0f) || (deg[i] == 0 && !(ignoreDeg && ignoreDeg[i] > 0)))
-                continue;
-            unsigned long long used[4] = {0ull, 0ull, 0ull, 0ull};
-            int b = adjStart[i], e = b + deg[i];
-            // partners in batches of 8: all partner indices, then all their colours, are in flight
-            // together instead of one dependent round trip per partner
-            for (int x0 = b; x0 < e; x0 += 8)
+            if (threadIdx.x == 0)
+                localChange = 0;
+            __syncthreads();
+            int changed = 0;
+            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nblocks * blockDim.x)
             {
-                int o[8], c[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    o[q] = x0 + q < e ? adjPartner[x0 + q] : -1;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    c[q] = (o[q] >= 0 && o[q] < i)
-                               ? __hip_atomic_load(&color[o[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : -1;
-#pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (c[q] >= 0 && c[q] < 256)
-                        used[c[q] >> 6] |= 1ull << (c[q] & 63);
-            }
-            int c = 0;
-            for (int w = 0; w < 4; ++w)
-            {
-                if (~used[w])
+                if (!(massMoment[i].x > 0.0f) || (deg[i] == 0 && !(ignoreDeg && ignoreDeg[i] > 0)))
+                    continue;
+                unsigned long long used[4] = {0ull, 0ull, 0ull, 0ull};
+                int b = adjStart[i], e = b + deg[i];
+                // partners in batches of 8: all partner indices, then all their colours, are in
+                // flight together instead of one dependent round trip per partner
+                for (int x0 = b; x0 < e; x0 += 8)
                 {
-                    c = w * 64 + __ffsll((long long)~used[w]) - 1;
-                    break;
+                    int o[8], c[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        o[q] = x0 + q < e ? adjPartner[x0 + q] : -1;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        c[q] = (o[q] >= 0 && o[q] < i)
+                                   ? __hip_atomic_load(&color[o[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                   : -1;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (c[q] >= 0 && c[q] < 256)
+                            used[c[q] >> 6] |= 1ull << (c[q] & 63);
                 }
-                c = 256;
+                int c = 0;
+                for (int w = 0; w < 4; ++w)
+                {
+                    if (~used[w])
+                    {
+                        c = w * 64 + __ffsll((long long)~used[w]) - 1;
+                        break;
+                    }
+                    c = 256;
+                }
+                if (c >= kMaxColors)
+                    atomicOr(&ctr->overflow, 8);
+                if (color[i] != c)
+                {
+                    __hip_atomic_store(&color[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    changed = 1;
+                }
             }
-            if (c >= kMaxColors)
-                atomicOr(&ctr->overflow, 8);
-            if (color[i] != c)
+            if (changed)
             {
-                __hip_atomic_store(&color[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                changed = 1;
+                localChange = 1;
+                roundChange = 1;
             }
+            __syncthreads();
+            const bool settled = localChange == 0;
+            __syncthreads();
+            if (settled)
+                break;
         }
-        if (changed)
-            anyChange = 1;
-        __syncthreads();
-        if (threadIdx.x == 0 && anyChange)
+        if (threadIdx.x == 0 && roundChange)
             atomicOr(&ctr->colorChanged[r % 3], 1u);
         grid_sync(gb, nblocks, &ctr->barrierTimeout);
         rounds = r + 1;
