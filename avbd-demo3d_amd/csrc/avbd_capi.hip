@@ -392,13 +392,18 @@ int enqueue_prepare(avbd_handle *h)
         }
         scan(h, S.chunkCount, S.chunkStart, nullptr, N, &ctr->numChunks);
         k_clamp_count<<<1, 1, 0, s>>>(&ctr->numChunks, h->maxChunks, &ctr->overflow, 32);
-        k_sweep_count<<<G, 256, 0, s>>>(S.chunkStart, h->sweepEnd, N, ctr, S.sPos, S.hitCount, S.hitMask);
+        // chunk owners (one wave per interval), then groups of chunks per wave
+        k_chunk_owner<<<std::max(1, std::min(ceil_div(N, 4), 16384)), 256, 0, s>>>(S.chunkStart, S.chunkCount, N, ctr,
+                                                                                   S.chunkOwner);
+        const int sweepBlocks = std::max(1, std::min(ceil_div(h->maxChunks, 4 * kSweepGroup), 4096));
+        k_sweep_count_g<<<sweepBlocks, 256, 0, s>>>(S.chunkOwner, S.chunkStart, h->sweepEnd, ctr, S.sPos, S.hitCount,
+                                                    S.hitMask);
         scan(h, S.hitCount, S.hitStart, &ctr->numChunks, 0, &ctr->numHits);
         k_fill_u64<<<grid_for(h->hashSize), 256, 0, s>>>(S.hash, h->hashSize, ~0ull);
         k_hash_insert<<<G, 256, 0, s>>>(h->cur.ab, ctr, h->joints, h->nj, h->springs, h->ns, h->ignorePairs,
                                         h->numIgnore, S.hash, S.hashMask, ctr);
-        k_sweep_emit<<<G, 256, 0, s>>>(S.chunkStart, N, ctr, S.hitMask, S.sIdx, S.hitStart, S.hash, S.hashMask, S.hits,
-                                       S.hitNew, h->maxPairs);
+        k_sweep_emit_g<<<sweepBlocks, 256, 0, s>>>(S.chunkOwner, S.chunkStart, ctr, S.hitMask, S.sIdx, S.hitStart,
+                                                   S.hash, S.hashMask, S.hits, S.hitNew, h->maxPairs);
         k_clamp_count<<<1, 1, 0, s>>>(&ctr->numHits, h->maxPairs, &ctr->overflow, 1);
         scan(h, S.hitNew, S.hitRank, &ctr->numHits, 0, &ctr->numCreated);
         k_create<<<G, 256, 0, s>>>(S.hits, S.hitNew, S.hitRank, ctr, h->cur, h->maxManifolds, h->maxPairs);
@@ -427,7 +432,8 @@ int enqueue_prepare(avbd_handle *h)
     scan(h, S.deg, S.adjStart, nullptr, N, h->adjTotal);
     k_adj<true><<<G, 256, 0, s>>>(h->cur, ctr, h->joints, h->nj, h->springs, h->ns, h->B.massMoment, S.deg, S.adjStart,
                                   S.adjCursor, S.adj, S.adjSerial, S.adjPartner, h->maxAdj, &ctr->overflow);
-    k_adj_sort<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjSerial, S.adjPartner, N, h->maxAdj);
+    k_adj_sort<<<std::max(1, std::min(ceil_div(N, 4), 16384)), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjSerial,
+                                                                          S.adjPartner, N, h->maxAdj);
     k_stamp_count<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, h->cur.count, N, S.stampCount,
                                                    &ctr->maxStampCount);
     scan(h, S.stampCount, S.stampStart, nullptr, N, &ctr->numStamps);
@@ -962,7 +968,8 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         (r = h->alloc(h->radixHistScan, 256 * tiles)) || (r = h->alloc(S.sLo, N)) || (r = h->alloc(S.sHi, N)) ||
         (r = h->alloc(S.sPos, N)) || (r = h->alloc(S.sIdx, N)) || (r = h->alloc(S.chunkCount, N)) ||
         (r = h->alloc(S.chunkStart, N)) || (r = h->alloc(h->sweepEnd, N)) || (r = h->alloc(S.hitCount, h->maxChunks)) ||
-        (r = h->alloc(S.hitStart, h->maxChunks)) || (r = h->alloc(S.hitMask, h->maxChunks)) || (r = h->alloc(S.hits, h->maxPairs)) ||
+        (r = h->alloc(S.hitStart, h->maxChunks)) || (r = h->alloc(S.hitMask, h->maxChunks)) ||
+        (r = h->alloc(S.chunkOwner, h->maxChunks)) || (r = h->alloc(S.hits, h->maxPairs)) ||
         (r = h->alloc(S.hitNew, h->maxPairs)) || (r = h->alloc(S.hitRank, h->maxPairs)) ||
         (r = h->alloc(S.hash, h->hashSize)) || (r = h->alloc(S.keep, h->maxManifolds)) ||
         (r = h->alloc(S.keepRank, h->maxManifolds)) || (r = h->alloc(S.deg, N)) || (r = h->alloc(S.adjStart, N)) ||

@@ -159,6 +159,7 @@ struct Scratch
     int *chunkStart;        // [N]
     int *hitCount;          // [maxChunks] hits per work item
     unsigned long long *hitMask; // [maxChunks] bit k: the k-th interval visited by the work item is a hit
+    int *chunkOwner;             // [maxChunks] the interval each sweep chunk belongs to
     int *hitStart;
     int2 *hits;             // [maxPairs]
     int *hitNew;            // [maxPairs] 1 = unconstrained

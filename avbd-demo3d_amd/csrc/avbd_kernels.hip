@@ -700,20 +700,6 @@ __global__ __launch_bounds__(256) void k_world_sap(const float4 *pose, const flo
         atomicAdd(&ctr->pairChecks, sum);
 }
 
-__device__ __forceinline__ int chunk_owner(const int *chunkStart, int n, int t)
-{
-    int lo = 0, hi = n;
-    while (lo < hi)
-    {
-        int mid = (lo + hi) >> 1;
-        if (chunkStart[mid] > t)
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    return lo - 1;
-}
-
 // Bounding-sphere test of the reference sweep (solver.cpp:558-561).
 __device__ __forceinline__ bool sphere_hit(float4 a, float4 b)
 {
@@ -722,22 +708,75 @@ __device__ __forceinline__ bool sphere_hit(float4 a, float4 b)
     return !(dot(dp, dp) > r * r);
 }
 
-__global__ void k_sweep_count(const int *chunkStart, const int *sweepEnd, int n, const Counters *ctr,
-                              const float4 *sPos, int *hitCount, unsigned long long *hitMask)
+// The sweep (solver.cpp:525-577) by chunks of 64 consecutive candidates. k_chunk_owner writes each
+// chunk's interval (one wave per interval, lanes over its chunks). The count and emission kernels
+// then give each wave kSweepGroup consecutive chunks: lanes 0..kSweepGroup-1 load their chunk's
+// interval data at once, and for each chunk lane l tests candidate l (coalesced loads), the
+// chunk's mask being the wave's ballot.
+constexpr int kSweepGroup = 16;
+
+__global__ void k_chunk_owner(const int *chunkStart, const int *chunkCount, int n, const Counters *ctr, int *owner)
 {
-    int T = ctr->numChunks;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x)
+    const int T = ctr->numChunks;
+    const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
+    for (int i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += gridDim.x * wpb)
     {
-        int i = chunk_owner(chunkStart, n, t);
-        int j0 = i + 1 + (t - chunkStart[i]) * kSweepChunk;
-        int j1 = min(j0 + kSweepChunk, sweepEnd[i]);
-        float4 a = sPos[i];
-        unsigned long long mask = 0ull;
-        for (int j = j0; j < j1; ++j)
-            if (sphere_hit(a, sPos[j]))
-                mask |= 1ull << (j - j0);
-        hitCount[t] = __popcll(mask);
-        hitMask[t] = mask;
+        const int c0 = chunkStart[i], nc = chunkCount[i];
+        for (int c = lane; c < nc && c0 + c < T; c += 64)
+            owner[c0 + c] = i;
+    }
+}
+
+__device__ __forceinline__ float4 shfl4(float4 v, int src)
+{
+    return make_float4(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64), __shfl(v.w, src, 64));
+}
+
+__global__ void k_sweep_count_g(const int *owner, const int *chunkStart, const int *sweepEnd, const Counters *ctr,
+                                const float4 *sPos, int *hitCount, unsigned long long *hitMask)
+{
+    const int T = ctr->numChunks;
+    const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
+    const int waves = gridDim.x * wpb;
+    for (int g = blockIdx.x * wpb + (threadIdx.x >> 6); g * kSweepGroup < T; g += waves)
+    {
+        const int t0 = g * kSweepGroup;
+        int pj0 = 0, pend = 0;
+        float4 pa = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (lane < kSweepGroup && t0 + lane < T)
+        {
+            const int i = owner[t0 + lane];
+            pj0 = i + 1 + (t0 + lane - chunkStart[i]) * kSweepChunk;
+            pend = sweepEnd[i];
+            pa = sPos[i];
+        }
+        const int kn = min(kSweepGroup, T - t0);
+        for (int k0 = 0; k0 < kn; k0 += 4)
+        {
+            // four chunks' candidate loads in flight together
+            float4 b[4];
+            bool in[4];
+            float4 a[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const int k = min(k0 + q, kn - 1);
+                const int j = __shfl(pj0, k, 64) + lane;
+                a[q] = shfl4(pa, k);
+                in[q] = k0 + q < kn && j < __shfl(pend, k, 64);
+                b[q] = in[q] ? sPos[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+            {
+                const unsigned long long m = __ballot(in[q] && sphere_hit(a[q], b[q]));
+                if (lane == 0 && k0 + q < kn)
+                {
+                    hitMask[t0 + k0 + q] = m;
+                    hitCount[t0 + k0 + q] = __popcll(m);
+                }
+            }
+        }
     }
 }
 
@@ -826,33 +865,52 @@ __global__ void k_hash_insert(const int2 *mab, const Counters *ctr, const Joint 
     }
 }
 
-__global__ void k_sweep_emit(const int *chunkStart, int n, Counters *ctr, const unsigned long long *hitMask,
-                             const int *sIdx, const int *hitStart, const unsigned long long *h, int mask, int2 *hits,
-                             int *hitNew, int maxPairs)
+// Emission by the same chunk groups: the lanes whose bit is set in a chunk's mask write their pair
+// at the chunk's start plus the count of set bits below them (the order of a serial walk over the
+// mask), each checking the constrained-pair hash on its own.
+__global__ void k_sweep_emit_g(const int *owner, const int *chunkStart, Counters *ctr, const unsigned long long *hitMask,
+                               const int *sIdx, const int *hitStart, const unsigned long long *h, int mask, int2 *hits,
+                               int *hitNew, int maxPairs)
 {
-    int T = ctr->numChunks;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x)
+    const int T = ctr->numChunks;
+    const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
+    const int waves = gridDim.x * wpb;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int g = blockIdx.x * wpb + (threadIdx.x >> 6); g * kSweepGroup < T; g += waves)
     {
-        unsigned long long bits = hitMask[t];
-        if (!bits)
-            continue;
-        int i = chunk_owner(chunkStart, n, t);
-        int j0 = i + 1 + (t - chunkStart[i]) * kSweepChunk;
-        int ia = sIdx[i];
-        int out = hitStart[t];
-        while (bits)
+        const int t0 = g * kSweepGroup;
+        unsigned long long pbits = 0ull;
+        int pj0 = 0, pstart = 0, pia = 0;
+        if (lane < kSweepGroup && t0 + lane < T)
         {
-            int j = j0 + __ffsll((long long)bits) - 1;
-            bits &= bits - 1ull;
+            pbits = hitMask[t0 + lane];
+            if (pbits)
+            {
+                const int i = owner[t0 + lane];
+                pj0 = i + 1 + (t0 + lane - chunkStart[i]) * kSweepChunk;
+                pstart = hitStart[t0 + lane];
+                pia = sIdx[i];
+            }
+        }
+        const int kn = min(kSweepGroup, T - t0);
+        for (int k = 0; k < kn; ++k)
+        {
+            // every shuffle runs with the whole wave active; only the stores diverge
+            const unsigned long long bits = __shfl(pbits, k, 64);
+            if (!bits)
+                continue;
+            const int start = __shfl(pstart, k, 64), ia = __shfl(pia, k, 64), j0 = __shfl(pj0, k, 64);
+            if (!((bits >> lane) & 1ull))
+                continue;
+            const int out = start + __popcll(bits & below);
             if (out < maxPairs)
             {
-                int ib = sIdx[j];
+                const int ib = sIdx[j0 + lane];
                 hits[out] = make_int2(ia, ib);
                 hitNew[out] = hash_contains(h, mask, pair_key(ia, ib)) ? 0 : 1;
             }
             else
                 atomicOr(&ctr->overflow, 1);
-            ++out;
         }
     }
 }
@@ -1199,28 +1257,61 @@ __global__ void k_adj(Manifolds M, const Counters *ctr, const Joint *joints, int
     }
 }
 
+// One wave per body: its force list in descending creation serial. With at most 64 entries, lane l
+// holds entry l and moves it to the number of entries with a larger serial (earlier entries first
+// among equal serials, as a stable insertion sort leaves them); longer lists are insertion-sorted
+// by lane 0.
 __global__ void k_adj_sort(const int *deg, const int *adjStart, int *adj, long long *adjSerial, int *adjPartner, int n,
                            int maxAdj)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    int b = adjStart[i], e = min(b + deg[i], maxAdj);
-    for (int x = b + 1; x < e; ++x)
+    const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
+    for (int i = blockIdx.x * wpb + (threadIdx.x >> 6); i < n; i += gridDim.x * wpb)
     {
-        long long s = adjSerial[x];
-        int v = adj[x], pv = adjPartner[x];
-        int y = x - 1;
-        while (y >= b && adjSerial[y] < s)
+        const int b = adjStart[i], e = min(b + deg[i], maxAdj);
+        const int d = e - b;
+        if (d <= 1)
+            continue;
+        if (d <= 64)
         {
-            adjSerial[y + 1] = adjSerial[y];
-            adj[y + 1] = adj[y];
-            adjPartner[y + 1] = adjPartner[y];
-            --y;
+            long long s = 0;
+            int v = 0, pv = 0;
+            if (lane < d)
+            {
+                s = adjSerial[b + lane];
+                v = adj[b + lane];
+                pv = adjPartner[b + lane];
+            }
+            int rank = 0;
+            for (int m = 0; m < d; ++m)
+            {
+                const long long sm = __shfl(s, m, 64);
+                rank += (sm > s || (sm == s && m < lane)) ? 1 : 0;
+            }
+            if (lane < d)
+            {
+                adjSerial[b + rank] = s;
+                adj[b + rank] = v;
+                adjPartner[b + rank] = pv;
+            }
+            continue;
         }
-        adjSerial[y + 1] = s;
-        adj[y + 1] = v;
-        adjPartner[y + 1] = pv;
+        if (lane == 0)
+            for (int x = b + 1; x < e; ++x)
+            {
+                long long s = adjSerial[x];
+                int v = adj[x], pv = adjPartner[x];
+                int y = x - 1;
+                while (y >= b && adjSerial[y] < s)
+                {
+                    adjSerial[y + 1] = adjSerial[y];
+                    adj[y + 1] = adj[y];
+                    adjPartner[y + 1] = adjPartner[y];
+                    --y;
+                }
+                adjSerial[y + 1] = s;
+                adj[y + 1] = v;
+                adjPartner[y + 1] = pv;
+            }
     }
 }
 
