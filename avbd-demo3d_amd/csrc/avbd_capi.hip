@@ -413,7 +413,10 @@ int enqueue_prepare(avbd_handle *h)
         HIP_TRY(hipEventRecord(h->ev[1], s));
 
     // ---- force initialisation (manifold.cpp:100-174, joint.cpp:35-54) ----
-    k_narrowphase<<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick);
+    if (h->numRound > 0)
+        k_narrowphase<true><<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick);
+    else
+        k_narrowphase<false><<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick);
     scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numKept);
     k_compact_manifolds<<<G, 256, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc);
     k_compact_finish<<<1, 1, 0, s>>>(ctr);

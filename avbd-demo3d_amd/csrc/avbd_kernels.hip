@@ -963,6 +963,9 @@ __global__ void k_create_finish(Counters *ctr, int maxManifolds, int maxPairs)
 // in it, and re-takes its anchors and friction rows at every match.
 // matchOld / stickUsed (adjoint recording, may be null): per new contact slot, the old contact it
 // continues (-1: none) and whether its anchors were kept by the stick rule.
+// Round: the world holds round bodies (otherwise only the box-box path is compiled, which keeps
+// the box worlds' kernel small).
+template <bool Round>
 __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, Counters *ctr, int *keep,
                               int *matchOld, int *stickUsed)
 {
@@ -984,7 +987,7 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
         NewContactsT<float> nc;
         V3T<float> t0{0, 0, 0}, t1{0, 0, 0}, t2{0, 0, 0};
         int nn;
-        if (shA == kShapeBox && shB == kShapeBox)
+        if (!Round || (shA == kShapeBox && shB == kShapeBox))
             nn = collide_boxes<float>(tv(pA), tq(qA), tv(ld3(sA4)), tv(pB), tq(qB), tv(ld3(sB4)), nc, t0, t1, t2);
         else
         {
@@ -1034,7 +1037,7 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
                 matchOld[m * kMaxContacts + i] = match;
                 stickUsed[m * kMaxContacts + i] = kept;
             }
-            int code = shape_code(shA, shB, nc.feature[i]);
+            int code = Round ? shape_code(shA, shB, nc.feature[i]) : 0;
             V3 xA, xB;
             if (code == 0)
             {
