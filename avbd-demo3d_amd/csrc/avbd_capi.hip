@@ -448,7 +448,11 @@ int enqueue_prepare(avbd_handle *h)
 
     // ---- colouring (avbd_gpu_solver.cpp:1812-1865) ----
     {
-        k_color<<<h->coopColor, kColorThreads, 0, s>>>(S.deg, ignoreDeg, S.adjStart, S.adjPartner, h->B.massMoment, N,
+        if (h->numWorlds > 1)
+            k_color_worlds<<<h->numWorlds, 64, 0, s>>>(h->worldStart, S.deg, ignoreDeg, S.adjStart, S.adjPartner,
+                                                       h->B.massMoment, S.color, ctr);
+        else
+            k_color<<<h->coopColor, kColorThreads, 0, s>>>(S.deg, ignoreDeg, S.adjStart, S.adjPartner, h->B.massMoment, N,
                                                         S.color, ctr, h->gb, N + 2);
         HIP_TRY(hipGetLastError());
     }

@@ -1459,6 +1459,100 @@ __global__ void k_color(const int *deg, const int *ignoreDeg, const int *adjStar
         ctr->coloringRounds = rounds;
 }
 
+// Batched worlds: each world (at most kWorldSapMax bodies, no force across worlds) is coloured by
+// one workgroup of one wave. Its bodies' colours, force-list starts, degrees (-1 for a body that
+// is not solved) and, when it fits, its partner slice are staged in LDS; then the bodies are
+// visited in index order and each takes the smallest colour none of its lower-index partners
+// holds (the lanes read the partners' colours, ballots find the smallest free colour). That is
+// the sequential greedy colouring itself, the fixpoint k_color converges to, with no rounds: a
+// colour change that runs along a long chain costs one visit per body instead of a grid round.
+constexpr int kColorWorldParts = 4096;
+__global__ __launch_bounds__(64) void k_color_worlds(const int *worldStart, const int *deg, const int *ignoreDeg,
+                                                     const int *adjStart, const int *adjPartner,
+                                                     const float4 *massMoment, int *color, Counters *ctr)
+{
+    __shared__ int col[kWorldSapMax];
+    __shared__ int st[kWorldSapMax];
+    __shared__ int dg[kWorldSapMax];
+    __shared__ int part[kColorWorldParts];
+    const int lane = threadIdx.x;
+    const int ws = worldStart[blockIdx.x], n = worldStart[blockIdx.x + 1] - ws;
+    if (n <= 0)
+        return;
+    const int pb = adjStart[ws];
+    for (int l = lane; l < n; l += 64)
+    {
+        const int i = ws + l;
+        const bool solved = massMoment[i].x > 0.0f && (deg[i] > 0 || (ignoreDeg && ignoreDeg[i] > 0));
+        col[l] = color[i];
+        st[l] = adjStart[i] - pb;
+        dg[l] = solved ? deg[i] : -1;
+    }
+    const int np = adjStart[ws + n - 1] + deg[ws + n - 1] - pb;
+    const bool fits = np <= kColorWorldParts;
+    if (fits)
+        for (int x = lane; x < np; x += 64)
+            part[x] = adjPartner[pb + x];
+    __syncthreads();
+    for (int l = 0; l < n; ++l)
+    {
+        const int d = dg[l];
+        if (d < 0)
+            continue;
+        const int i = ws + l, b = st[l];
+        int c = 0;
+        if (d <= 64)
+        {
+            int pc = -1;
+            if (lane < d)
+            {
+                const int o = fits ? part[b + lane] : adjPartner[pb + b + lane];
+                if (o >= 0 && o < i)
+                    pc = col[o - ws];
+            }
+            // smallest colour in [0, 256) that no lane holds
+            c = 256;
+            for (int k = 0; k < 256; ++k)
+                if (__ballot(pc == k) == 0ull)
+                {
+                    c = k;
+                    break;
+                }
+        }
+        else if (lane == 0)
+        {
+            unsigned long long used[4] = {0ull, 0ull, 0ull, 0ull};
+            for (int x = 0; x < d; ++x)
+            {
+                const int o = fits ? part[b + x] : adjPartner[pb + b + x];
+                if (o < 0 || o >= i)
+                    continue;
+                const int pcx = col[o - ws];
+                if (pcx >= 0 && pcx < 256)
+                    used[pcx >> 6] |= 1ull << (pcx & 63);
+            }
+            c = 256;
+            for (int w = 0; w < 4; ++w)
+                if (~used[w])
+                {
+                    c = w * 64 + __ffsll((long long)~used[w]) - 1;
+                    break;
+                }
+        }
+        if (lane == 0)
+        {
+            if (c >= kMaxColors)
+                atomicOr(&ctr->overflow, 8);
+            col[l] = c;
+        }
+        __syncthreads();
+    }
+    for (int l = lane; l < n; l += 64)
+        color[ws + l] = col[l];
+    if (blockIdx.x == 0 && lane == 0)
+        ctr->coloringRounds = 1;
+}
+
 // Sort keys of the colour buckets: colour in bits 8-15, and 255 - stamp count in bits 0-7, so
 // that inside a colour the bodies with the most stamps come first (the order inside a colour
 // does not change any result: its bodies share no force).
