@@ -253,6 +253,7 @@ struct DualArgs
 // with hipErrorIllegalState in a second process while another process uses cooperative launches
 // on the same GPU, which would keep two programs from stepping on one device.
 constexpr int kColorThreads = 1024;
+constexpr int kColorSmallRounds = 6;  // round limit of the fixpoint for a world of at most kWorldSapMax bodies
 
 // Exclusive scan with the count either on the device (nptr) or given (nconst).
 void scan(avbd_handle *h, const int *in, int *out, const int *nptr, int nconst, int *total)
@@ -448,12 +449,21 @@ int enqueue_prepare(avbd_handle *h)
 
     // ---- colouring (avbd_gpu_solver.cpp:1812-1865) ----
     {
+        // batched worlds: the sequential greedy colouring per world, one wave each. One world: the
+        // parallel fixpoint; a world small enough for one workgroup gets kColorSmallRounds rounds,
+        // after which (colours still changing along a long chain) the sequential rule finishes.
         if (h->numWorlds > 1)
             k_color_worlds<<<h->numWorlds, 64, 0, s>>>(h->worldStart, S.deg, ignoreDeg, S.adjStart, S.adjPartner,
-                                                       h->B.massMoment, S.color, ctr);
+                                                       h->B.massMoment, S.color, ctr, 0);
         else
+        {
+            const bool small = N <= kWorldSapMax;
             k_color<<<h->coopColor, kColorThreads, 0, s>>>(S.deg, ignoreDeg, S.adjStart, S.adjPartner, h->B.massMoment, N,
-                                                        S.color, ctr, h->gb, N + 2);
+                                                        S.color, ctr, h->gb, small ? kColorSmallRounds : N + 2);
+            if (small)
+                k_color_worlds<<<1, 64, 0, s>>>(h->worldStart, S.deg, ignoreDeg, S.adjStart, S.adjPartner,
+                                                h->B.massMoment, S.color, ctr, 1);
+        }
         HIP_TRY(hipGetLastError());
     }
     {
@@ -1249,6 +1259,9 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     std::vector<int> negs(std::max(N, 1), -1);
     if ((e = up(h->S.color, negs.data(), sizeof(int) * N)))
         return fail(AVBD_ERR_HIP, std::string("upload colours: ") + hipGetErrorString(e));
+    const int oneWorld[2] = {0, N};  // the whole state is world 0
+    if ((e = up(h->worldStart, oneWorld, sizeof(oneWorld))))
+        return fail(AVBD_ERR_HIP, std::string("upload world range: ") + hipGetErrorString(e));
     if ((e = hipStreamSynchronize(s)) != hipSuccess)
         return fail(AVBD_ERR_HIP, std::string("upload sync: ") + hipGetErrorString(e));
     h->nb = N;

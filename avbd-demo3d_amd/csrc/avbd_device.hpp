@@ -119,6 +119,7 @@ struct Counters
     int contactEndpoints;   // sum over contacts of their dynamic endpoints
     int numStamps;
     int maxStampCount;      // the most stamps of one body this step
+    int colorPending;       // k_color stopped at its round limit with colours still changing
     int jointLive;
     int jointEndpoints;
     int springEndpoints;    // fixed by the uploaded springs

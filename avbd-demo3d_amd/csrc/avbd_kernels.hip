@@ -1456,7 +1456,15 @@ __global__ void k_color(const int *deg, const int *ignoreDeg, const int *adjStar
             break;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
         ctr->coloringRounds = rounds;
+        // still changing after the last allowed round: k_color_worlds finishes the colouring
+        ctr->colorPending = (rounds == maxRounds && !ctr->barrierTimeout &&
+                             __hip_atomic_load(&ctr->colorChanged[(rounds - 1) % 3], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT) != 0u)
+                                ? 1
+                                : 0;
+    }
 }
 
 // Batched worlds: each world (at most kWorldSapMax bodies, no force across worlds) is coloured by
@@ -1467,10 +1475,15 @@ __global__ void k_color(const int *deg, const int *ignoreDeg, const int *adjStar
 // the sequential greedy colouring itself, the fixpoint k_color converges to, with no rounds: a
 // colour change that runs along a long chain costs one visit per body instead of a grid round.
 constexpr int kColorWorldParts = 4096;
+// onlyIfPending: run only when k_color left the colouring unfinished (a small single world whose
+// round limit was reached); the greedy rule does not depend on the colours it starts from.
 __global__ __launch_bounds__(64) void k_color_worlds(const int *worldStart, const int *deg, const int *ignoreDeg,
                                                      const int *adjStart, const int *adjPartner,
-                                                     const float4 *massMoment, int *color, Counters *ctr)
+                                                     const float4 *massMoment, int *color, Counters *ctr,
+                                                     int onlyIfPending)
 {
+    if (onlyIfPending && !ctr->colorPending)
+        return;
     __shared__ int col[kWorldSapMax];
     __shared__ int st[kWorldSapMax];
     __shared__ int dg[kWorldSapMax];
@@ -1549,7 +1562,7 @@ __global__ __launch_bounds__(64) void k_color_worlds(const int *worldStart, cons
     }
     for (int l = lane; l < n; l += 64)
         color[ws + l] = col[l];
-    if (blockIdx.x == 0 && lane == 0)
+    if (blockIdx.x == 0 && lane == 0 && !onlyIfPending)
         ctr->coloringRounds = 1;
 }
 

@@ -144,7 +144,7 @@ def main():
     launch = world.launch_config()
     if not args.no_tune:
         # direct launches only: the per-kernel timing marks need them
-        launch = world.autotune(trial_steps=3, allow_graph=False)
+        launch = world.autotune(trial_steps=8, allow_graph=False)
         log(f"autotuned launch shape {launch}")
         world.step(2)
         world.synchronize()
@@ -378,7 +378,7 @@ def pour_leg(args):
     world = capi.HipWorld.for_state(start, prm)
     world.step(300)
     world.synchronize()
-    launch = world.autotune(trial_steps=3, allow_graph=False)
+    launch = world.autotune(trial_steps=6, allow_graph=False)
     world.upload(start)
     world.step(5)
     world.synchronize()
@@ -434,7 +434,7 @@ def batched(args, params):
     world = capi.HipWorld.for_batch(states, params)
     world.step(2)
     world.synchronize()
-    launch = world.autotune(trial_steps=3, allow_graph=True)
+    launch = world.autotune(trial_steps=6, allow_graph=True)
     log(f"batched: {n_bodies} bodies ({n_dyn} dynamic); autotuned launch shape {launch}")
     world.step(args.batch_warmup)
     world.synchronize()
