@@ -2106,6 +2106,14 @@ __device__ __forceinline__ float init_entry(int e, float ml, float ax, float ay,
     }
 }
 
+// Offset in a body's 27-float system record of entry (r, k), k <= r, of the 6x6 matrix solve6
+// factors (rows 0-2 linear, 3-5 angular; the coupling block is the record's lhsCross).
+__device__ __forceinline__ int sys_entry(int r, int k)
+{
+    return r < 3 ? R_L00 + r * (r + 1) / 2 + k
+                 : (k < 3 ? R_C + 3 * (r - 3) + k : R_A00 + (r - 3) * (r - 2) / 2 + (k - 3));
+}
+
 // LDS hand-off between the lanes of one wave: a group's 16 lanes always sit in one wave, so
 // ordering the wave's own LDS writes before its reads is enough (no block barrier).
 __device__ __forceinline__ void wave_lds_sync()
@@ -2223,34 +2231,87 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
         if (lane + e * L < kRec)
             rec[g * L * kRow + lane + e * L] = acc[e];
     wave_lds_sync();
+    // The 6x6 LDL^T solve of solve6 (maths.h:481-547) spread over lanes 0..5 of the group: lane r
+    // owns row r + 1, the columns are factored in order (the column's owner broadcasts its
+    // diagonal and its row of L, every lower row forms its entry with solve6's own expression),
+    // the forward substitution broadcasts each y_k, each lane divides its own z; lane 0 then runs
+    // the back substitution from L and z staged in LDS. Every value is computed by the same
+    // operations in the same order as solve6, so the result is bit-identical. All lanes of the
+    // wave run this part (invalid groups on junk, never stored): the shuffles need every lane.
+    V3 dxL, dxA;
+    {
+        const float *a = rec + g * L * kRow;
+        const int r = lane < 6 ? lane : 5;
+        const int base = (int)(threadIdx.x & 63) - lane;
+        float arow[6], lrow[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, d[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            arow[k] = k <= r ? a[sys_entry(r, k)] : 0.0f;
+        float myD = 0.0f;
+        d[0] = __shfl(arow[0], base, 64);  // D1 = A11
+        myD = r == 0 ? d[0] : myD;
+        lrow[0] = arow[0] / d[0];          // L_i1 = A_i1 / A11
+#pragma unroll
+        for (int j = 1; j < 6; ++j)
+        {
+            // row j + 1 of L, from its owner
+            float lj[6];
+#pragma unroll
+            for (int k = 0; k < j; ++k)
+                lj[k] = __shfl(lrow[k], base + j, 64);
+            // D_j = A_jj - (sum_k (L_jk L_jk) D_k), summed left to right (computed by its owner)
+            float sum = (lrow[0] * lrow[0]) * d[0];
+#pragma unroll
+            for (int k = 1; k < j; ++k)
+                sum = sum + (lrow[k] * lrow[k]) * d[k];
+            d[j] = __shfl(arow[j] - sum, base + j, 64);
+            myD = r == j ? d[j] : myD;
+            // L_ij = (A_ij - L_j1 L_i1 D_1 - ... ) / D_j for the rows below
+            float t = arow[j];
+#pragma unroll
+            for (int k = 0; k < j; ++k)
+                t = t - (lj[k] * lrow[k]) * d[k];
+            if (r > j)
+                lrow[j] = t / d[j];
+        }
+        // forward substitution: y_i = b_i - L_i1 y_1 - ... - L_i(i-1) y_(i-1)
+        float y = -(r < 3 ? a[R_RL + r] : a[R_RA + r - 3]);
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+        {
+            const float yk = __shfl(y, base + k, 64);
+            if (r > k)
+                y = y - lrow[k] * yk;
+        }
+        const float z = y / myD;
+        // stage L (row-major, 6 per row) and z in the group's second LDS row for lane 0
+        float *m = rec + (g * L + 1) * kRow;
+        wave_lds_sync();
+        if (lane < 6)
+        {
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                if (k < r)
+                    m[6 * r + k] = lrow[k];
+            m[6 * r + 5] = z;
+        }
+        wave_lds_sync();
+        if (lane == 0)
+        {
+            auto Lm = [&](int i, int k) { return m[6 * (i - 1) + (k - 1)]; };  // L_ik, 1-based
+            const float z1 = m[5], z2 = m[11], z3 = m[17], z4 = m[23], z5 = m[29], z6 = m[35];
+            dxA.z = z6;
+            dxA.y = z5 - Lm(6, 5) * dxA.z;
+            dxA.x = z4 - Lm(5, 4) * dxA.y - Lm(6, 4) * dxA.z;
+            dxL.z = z3 - Lm(4, 3) * dxA.x - Lm(5, 3) * dxA.y - Lm(6, 3) * dxA.z;
+            dxL.y = z2 - Lm(3, 2) * dxL.z - Lm(4, 2) * dxA.x - Lm(5, 2) * dxA.y - Lm(6, 2) * dxA.z;
+            dxL.x = z1 - Lm(2, 1) * dxL.y - Lm(3, 1) * dxL.z - Lm(4, 1) * dxA.x - Lm(5, 1) * dxA.y -
+                    Lm(6, 1) * dxA.z;
+        }
+    }
     if (valid && lane == 0)
     {
         const float *a = rec + g * L * kRow;
-        Sys6 sys;
-        sys.l00 = a[R_L00];
-        sys.l10 = a[R_L10];
-        sys.l11 = a[R_L11];
-        sys.l20 = a[R_L20];
-        sys.l21 = a[R_L21];
-        sys.l22 = a[R_L22];
-        sys.a00 = a[R_A00];
-        sys.a10 = a[R_A10];
-        sys.a11 = a[R_A11];
-        sys.a20 = a[R_A20];
-        sys.a21 = a[R_A21];
-        sys.a22 = a[R_A22];
-        sys.c00 = a[R_C + 0];
-        sys.c01 = a[R_C + 1];
-        sys.c02 = a[R_C + 2];
-        sys.c10 = a[R_C + 3];
-        sys.c11 = a[R_C + 4];
-        sys.c12 = a[R_C + 5];
-        sys.c20 = a[R_C + 6];
-        sys.c21 = a[R_C + 7];
-        sys.c22 = a[R_C + 8];
-        V3 rl{a[R_RL], a[R_RL + 1], a[R_RL + 2]}, ra{a[R_RA], a[R_RA + 1], a[R_RA + 2]};
-        V3 dxL, dxA;
-        solve6(sys, -rl, -ra, dxL, dxA);
         float4 np = st3(pos + dxL, 0.0f), nq = stq(qintegrate(rot, dxA));
         if (Co)
         {
