@@ -417,7 +417,8 @@ int enqueue_prepare(avbd_handle *h)
     if (h->numRound > 0)
         k_narrowphase<true><<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick);
     else
-        k_narrowphase<false><<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick);
+        k_narrowphase_boxes<<<4 * G, kNpBoxThreads, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch,
+                                                            h->recStick);
     scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numKept);
     k_compact_manifolds<<<G, 256, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc);
     k_compact_finish<<<1, 1, 0, s>>>(ctr);

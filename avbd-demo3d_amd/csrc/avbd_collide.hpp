@@ -117,7 +117,14 @@ template <class T> struct ObbT
     V3T<T> c, h, ax0, ax1, ax2;
 };
 
-template <class T> AVBD_HD V3T<T> obb_ax(const ObbT<T> &o, int k) { return k == 0 ? o.ax0 : (k == 1 ? o.ax1 : o.ax2); }
+// Component-wise value selects (a conditional of lvalues selects an address, which forces the
+// operands into private memory).
+template <class T> AVBD_HD V3T<T> sel3(bool c, V3T<T> a, V3T<T> b)
+{
+    return V3T<T>{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z};
+}
+
+template <class T> AVBD_HD V3T<T> obb_ax(const ObbT<T> &o, int k) { return sel3(k == 0, o.ax0, sel3(k == 1, o.ax1, o.ax2)); }
 
 template <class T> AVBD_HD ObbT<T> make_obb(V3T<T> pos, Q4T<T> rot, V3T<T> size)
 {
@@ -185,6 +192,17 @@ template <class T> struct NewContactsT
     V3T<T> mid[kMaxNewContacts];
 };
 
+// Clip polygon of up to 16 vertices in the lane's private memory (the generic storage).
+template <class T> struct LocalPoly
+{
+    V3T<T> v[16];
+    AVBD_HD V3T<T> get(int i) const { return v[i]; }
+    AVBD_HD void set(int i, V3T<T> x) { v[i] = x; }
+};
+
+// The clip polygon storage a contact sink brings: private arrays for NewContactsT.
+template <class T> AVBD_HD LocalPoly<T> make_poly(NewContactsT<T> &, int) { return LocalPoly<T>{}; }
+
 template <class T>
 AVBD_HD void add_contact(NewContactsT<T> &nc, V3T<T> posA, Q4T<T> rotA, V3T<T> posB, Q4T<T> rotB, V3T<T> xA,
                          V3T<T> xB, int key)
@@ -203,16 +221,16 @@ AVBD_HD void add_contact(NewContactsT<T> &nc, V3T<T> posA, Q4T<T> rotA, V3T<T> p
     nc.count = c + 1;
 }
 
-template <class T> AVBD_HD int clip_plane(const V3T<T> *in, int n, V3T<T> pn, T off, V3T<T> *out)
+template <class T, class PI, class PO> AVBD_HD int clip_plane(const PI &in, int n, V3T<T> pn, T off, PO &out)
 {
     if (n <= 0)
         return 0;
     int m = 0;
-    V3T<T> a = in[n - 1];
+    V3T<T> a = in.get(n - 1);
     T da = dot(pn, a) - off;
     for (int i = 0; i < n; ++i)
     {
-        V3T<T> b = in[i];
+        V3T<T> b = in.get(i);
         T db = dot(pn, b) - off;
         bool ain = da <= kPlaneEps;
         bool bin = db <= kPlaneEps;
@@ -223,10 +241,10 @@ template <class T> AVBD_HD int clip_plane(const V3T<T> *in, int n, V3T<T> pn, T 
             if (absv(den) > kSatEps)
                 t = clampv(da / den, T(0.0f), T(1.0f));
             if (m < 16)
-                out[m++] = a + (b - a) * t;
+                out.set(m++, a + (b - a) * t);
         }
         if (bin && m < 16)
-            out[m++] = b;
+            out.set(m++, b);
         a = b;
         da = db;
     }
@@ -235,35 +253,33 @@ template <class T> AVBD_HD int clip_plane(const V3T<T> *in, int n, V3T<T> pn, T 
 
 template <class T> AVBD_HD void face_axes(const ObbT<T> &o, int k, V3T<T> &u, V3T<T> &v, T &eu, T &ev)
 {
-    if (k == 0)
-    {
-        u = o.ax1;
-        v = o.ax2;
-        eu = o.h.y;
-        ev = o.h.z;
-    }
-    else if (k == 1)
-    {
-        u = o.ax0;
-        v = o.ax2;
-        eu = o.h.x;
-        ev = o.h.z;
-    }
-    else
-    {
-        u = o.ax0;
-        v = o.ax1;
-        eu = o.h.x;
-        ev = o.h.y;
-    }
+    // k = 0: (ax1, ax2, h.y, h.z); k = 1: (ax0, ax2, h.x, h.z); k = 2: (ax0, ax1, h.x, h.y)
+    const T hx = o.h.x, hy = o.h.y, hz = o.h.z;
+    u = sel3(k == 0, o.ax1, o.ax0);
+    v = sel3(k == 2, o.ax1, o.ax2);
+    eu = k == 0 ? hy : hx;
+    ev = k == 2 ? hy : hz;
 }
 
-template <class T>
-AVBD_HD void face_manifold(NewContactsT<T> &nc, V3T<T> posA, Q4T<T> rotA, V3T<T> posB, Q4T<T> rotB,
-                           const ObbT<T> &A, const ObbT<T> &B, bool refA, int refAxis, V3T<T> nAB)
+// Component-wise select of two boxes (a select of references, or of whole aggregates, would
+// force both boxes into private memory).
+template <class T> AVBD_HD ObbT<T> obb_sel(bool c, const ObbT<T> &a, const ObbT<T> &b)
 {
-    const ObbT<T> &ref = refA ? A : B;
-    const ObbT<T> &inc = refA ? B : A;
+    ObbT<T> o;
+    o.c = sel3(c, a.c, b.c);
+    o.h = sel3(c, a.h, b.h);
+    o.ax0 = sel3(c, a.ax0, b.ax0);
+    o.ax1 = sel3(c, a.ax1, b.ax1);
+    o.ax2 = sel3(c, a.ax2, b.ax2);
+    return o;
+}
+
+template <class T, class NC>
+AVBD_HD void face_manifold(NC &nc, V3T<T> posA, Q4T<T> rotA, V3T<T> posB, Q4T<T> rotB, const ObbT<T> &A,
+                           const ObbT<T> &B, bool refA, int refAxis, V3T<T> nAB)
+{
+    const ObbT<T> ref = obb_sel(refA, A, B);
+    const ObbT<T> inc = obb_sel(refA, B, A);
     V3T<T> outward = refA ? nAB : -nAB;
     V3T<T> rax = obb_ax(ref, refAxis);
     float fs = dot(outward, rax) >= 0.0f ? 1.0f : -1.0f;
@@ -284,7 +300,8 @@ AVBD_HD void face_manifold(NewContactsT<T> &nc, V3T<T> posA, Q4T<T> rotA, V3T<T>
             incAxis = i;
         }
     }
-    V3T<T> p0[16], p1[16];
+    auto p0 = make_poly(nc, 0);
+    auto p1 = make_poly(nc, 1);
     {
         V3T<T> iax = obb_ax(inc, incAxis);
         float s = dot(iax, fn) > 0.0f ? -1.0f : 1.0f;
@@ -293,10 +310,10 @@ AVBD_HD void face_manifold(NewContactsT<T> &nc, V3T<T> posA, Q4T<T> rotA, V3T<T>
         V3T<T> u, v;
         T ue, ve;
         face_axes(inc, incAxis, u, v, ue, ve);
-        p0[0] = c + u * ue + v * ve;
-        p0[1] = c - u * ue + v * ve;
-        p0[2] = c - u * ue - v * ve;
-        p0[3] = c + u * ue - v * ve;
+        p0.set(0, c + u * ue + v * ve);
+        p0.set(1, c - u * ue + v * ve);
+        p0.set(2, c - u * ue - v * ve);
+        p0.set(3, c + u * ue - v * ve);
     }
     int n = 4;
     n = clip_plane(p0, n, fu, dot(fu, fc) + eu, p1);
@@ -316,7 +333,7 @@ AVBD_HD void face_manifold(NewContactsT<T> &nc, V3T<T> posA, Q4T<T> rotA, V3T<T>
     int prefix = ((refA ? 0 : 1) << 24) | ((refAxis & 0xFF) << 16) | ((incAxis & 0xFF) << 8);
     for (int i = 0; i < n && nc.count < kMaxNewContacts; ++i)
     {
-        V3T<T> pi = p0[i];
+        V3T<T> pi = p0.get(i);
         T depth = dot(pi - fc, fn);
         if (depth > kPlaneEps)
             continue;
@@ -381,9 +398,11 @@ template <class T> AVBD_HD void closest_seg(V3T<T> p0, V3T<T> p1, V3T<T> q0, V3T
 
 // Box-box collide: returns the contact count; the basis rows are written only when the boxes
 // overlap on all 15 axes.
-template <class T>
+// NC: the contact sink (NewContactsT<T>, or a sink that takes each contact as it is found and
+// supplies add_contact / make_poly overloads of its own).
+template <class T, class NC>
 AVBD_HD int collide_boxes(V3T<T> posA, Q4T<T> rotA, V3T<T> sizeA, V3T<T> posB, Q4T<T> rotB, V3T<T> sizeB,
-                          NewContactsT<T> &nc, V3T<T> &b0, V3T<T> &b1, V3T<T> &b2)
+                          NC &nc, V3T<T> &b0, V3T<T> &b1, V3T<T> &b2)
 {
     nc.count = 0;
     ObbT<T> A = make_obb(posA, rotA, sizeA), B = make_obb(posB, rotB, sizeB);

@@ -158,10 +158,13 @@ constexpr int kLbTile = kLbThreads * kLbItems;
 constexpr int kLbBlocks = 128;
 constexpr int kLbSpin = 1 << 22;
 
+// The status word carries its value, so publishing and reading it need no ordering with other
+// memory: relaxed device-scope atomics, which skip the cache write-back and invalidation that
+// release and acquire semantics cost.
 __device__ __forceinline__ void lb_publish(unsigned long long *w, unsigned epoch, unsigned flag, int value)
 {
     unsigned long long v = ((unsigned long long)epoch << 34) | ((unsigned long long)flag << 32) | (unsigned)value;
-    __hip_atomic_store(w, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ __launch_bounds__(kLbThreads) void k_scan_lookback(const int *in, int *out, const int *nptr, int nconst,
@@ -226,7 +229,7 @@ __global__ __launch_bounds__(kLbThreads) void k_scan_lookback(const int *in, int
                     int val = 0;
                     if (q >= 0)
                     {
-                        unsigned long long w = __hip_atomic_load(&status[q], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                        unsigned long long w = __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         fl = (unsigned)(w >> 34) == epoch ? (unsigned)(w >> 32) & 3u : 0u;
                         val = (int)(unsigned)w;
                     }
@@ -282,9 +285,11 @@ __global__ __launch_bounds__(kLbThreads) void k_scan_lookback(const int *in, int
                     out[base + q] = o[q];
         }
     }
+    // A block's exit count follows its last tile claim, whose returned value it waited for, so
+    // the last block to exit resets the counters after every claim of this launch. The scanned
+    // output is published to later kernels by the launch boundary.
     if (threadIdx.x == 0)
     {
-        __threadfence();
         if (atomicAdd(&ctl[1], 1u) == gridDim.x - 1)
         {
             atomicExch(&ctl[0], 0u);
@@ -963,8 +968,7 @@ __global__ void k_create_finish(Counters *ctr, int maxManifolds, int maxPairs)
 // in it, and re-takes its anchors and friction rows at every match.
 // matchOld / stickUsed (adjoint recording, may be null): per new contact slot, the old contact it
 // continues (-1: none) and whether its anchors were kept by the stick rule.
-// Round: the world holds round bodies (otherwise only the box-box path is compiled, which keeps
-// the box worlds' kernel small).
+// Round: the world holds round bodies (worlds of boxes only run k_narrowphase_boxes below).
 template <bool Round>
 __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, Counters *ctr, int *keep,
                               int *matchOld, int *stickUsed)
@@ -1069,6 +1073,176 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
             o[CR_PEN] = st3(k, 0.0f);
             o[CR_LAM] = st3(l, lam.w);
         }
+    }
+}
+
+// ---- box-only narrowphase (worlds without round bodies) ----
+// The same Manifold::initialize as k_narrowphase<false>, with no private arrays: each contact's
+// record is written as collide_boxes finds it (only the merge midpoints stay, in registers), and
+// the two clip polygons sit in LDS. Private arrays would go to scratch memory, whose latency and
+// per-wave allocation throttled the kernel.
+constexpr int kNpBoxThreads = 64;
+
+// One lane's 16-vertex clip polygon in LDS, component-major with the lane index fastest.
+struct LdsPoly
+{
+    float *p;
+    AVBD_HD V3T<float> get(int i) const
+    {
+        return {p[(3 * i) * kNpBoxThreads], p[(3 * i + 1) * kNpBoxThreads], p[(3 * i + 2) * kNpBoxThreads]};
+    }
+    AVBD_HD void set(int i, V3T<float> x)
+    {
+        p[(3 * i) * kNpBoxThreads] = x.x;
+        p[(3 * i + 1) * kNpBoxThreads] = x.y;
+        p[(3 * i + 2) * kNpBoxThreads] = x.z;
+    }
+};
+
+// Contact sink of the box-only narrowphase: the manifold's context and the contacts' midpoints.
+struct BoxRecordSink
+{
+    int count;
+    V3T<float> mid[kMaxNewContacts];
+    V3T<float> b0, b1, b2; // the manifold basis, set by collide_boxes before the first contact
+    float *poly;           // this lane's two clip polygons in LDS
+    int m, nOld;
+    bool refresh;
+    float friction, alpha, gamma, hAx, hAy, hBx, hBy;
+    V3 pA, pB;
+    Q4 qA, qB;
+    const int *curFeature;
+    const float4 *curCr;
+    int *tmpFeature;
+    float4 *tmpCr;
+    int *matchOld, *stickUsed;
+};
+
+AVBD_HD LdsPoly make_poly(BoxRecordSink &nc, int k) { return LdsPoly{nc.poly + k * 48 * kNpBoxThreads}; }
+
+// ContactBuilder::add (collide.cpp:30-58), then the contact's part of Manifold::initialize
+// (manifold.cpp:100-174): warm start from the old contact with the same feature key, the stick
+// rule, and the record. The merge test and the cap run over the midpoints as add_contact's do.
+AVBD_HD void add_contact(BoxRecordSink &nc, V3T<float> posA, Q4T<float> rotA, V3T<float> posB, Q4T<float> rotB,
+                         V3T<float> xA, V3T<float> xB, int key)
+{
+    V3T<float> mp = (xA + xB) * 0.5f;
+    bool merged = false;
+#pragma unroll
+    for (int i = 0; i < kMaxNewContacts; ++i)
+        if (i < nc.count && lensq(mp - nc.mid[i]) < kMergeDistSq)
+            merged = true;
+    if (merged || nc.count >= kMaxNewContacts)
+        return;
+    const int c = nc.count;
+#pragma unroll
+    for (int i = 0; i < kMaxNewContacts; ++i)
+        if (i == c)
+            nc.mid[i] = mp;
+    nc.count = c + 1;
+    V3 rA = fv(rotate(qconj(rotA), xA - posA)), rB = fv(rotate(qconj(rotB), xB - posB));
+    float4 pen = make_float4(0, 0, 0, 0), lam = make_float4(0, 0, 0, 0);
+    int match = -1, kept = 0;
+    for (int j = 0; j < nc.nOld; ++j)
+    {
+        if (nc.curFeature[nc.m * kMaxContacts + j] == key)
+        {
+            const float4 *o = nc.curCr + (size_t)(nc.m * kMaxContacts + j) * kCR;
+            pen = o[CR_PEN];
+            lam = o[CR_LAM];
+            match = j;
+            if (!nc.refresh && lam.w != 0.0f)
+            {
+                rA = ld3(o[CR_RA]);
+                rB = ld3(o[CR_RB]);
+                kept = 1;
+            }
+            if (nc.refresh)
+            {
+                lam = make_float4(lam.x, 0.0f, 0.0f, 0.0f);
+                pen = make_float4(pen.x, kPenaltyMin, kPenaltyMin, pen.w);
+            }
+            break;
+        }
+    }
+    const int sl = nc.m * kMaxContacts + c;
+    if (nc.matchOld)
+    {
+        nc.matchOld[sl] = match;
+        nc.stickUsed[sl] = kept;
+    }
+    V3 b0 = fv(nc.b0), b1 = fv(nc.b1), b2 = fv(nc.b2);
+    V3 wA = nc.pA + rotate(nc.qA, rA);
+    V3 wB = nc.pB + rotate(nc.qB, rB);
+    V3 d = wA - wB;
+    V3 c0 = V3{dot(b0, d), dot(b1, d), dot(b2, d)} + V3{kMargin, 0.0f, 0.0f};
+    V3 l = V3{lam.x, lam.y, lam.z} * nc.alpha * nc.gamma;
+    V3 k = V3{pen.x, pen.y, pen.z} * nc.gamma;
+    k = V3{clampr(k.x, kPenaltyMin, kPenaltyMax), clampr(k.y, kPenaltyMin, kPenaltyMax),
+           clampr(k.z, kPenaltyMin, kPenaltyMax)};
+    nc.tmpFeature[sl] = key;
+    float4 *o = nc.tmpCr + (size_t)sl * kCR;
+    o[CR_RA] = st3(rA, nc.friction);
+    o[CR_RB] = st3(rB, __int_as_float(0));
+    o[CR_C0] = st3(c0, nc.hBy);
+    o[CR_N] = st3(b0, nc.hAx);
+    o[CR_T1] = st3(b1, nc.hBx);
+    o[CR_T2] = st3(b2, nc.hAy);
+    o[CR_PEN] = st3(k, 0.0f);
+    o[CR_LAM] = st3(l, lam.w);
+}
+
+// Launched with kNpBoxThreads threads per block, grid-stride over the manifolds.
+__global__ __launch_bounds__(kNpBoxThreads) void k_narrowphase_boxes(Manifolds cur, Manifolds tmp, Bodies B,
+                                                                     Params P, Counters *ctr, int *keep,
+                                                                     int *matchOld, int *stickUsed)
+{
+    __shared__ float polyLds[2 * 48 * kNpBoxThreads];
+    const int total = ctr->numTotal;
+    const int old = ctr->numManifolds;
+    for (int m = blockIdx.x * kNpBoxThreads + threadIdx.x; m < total; m += gridDim.x * kNpBoxThreads)
+    {
+        int2 ab = cur.ab[m];
+        BoxRecordSink nc;
+        nc.count = 0;
+        nc.poly = polyLds + threadIdx.x;
+        nc.m = m;
+        nc.nOld = m < old ? cur.count[m] : 0;
+        nc.pA = ld3(B.pose[4 * (ab.x) + P_POS]);
+        nc.pB = ld3(B.pose[4 * (ab.y) + P_POS]);
+        nc.qA = ldq(B.pose[4 * (ab.x) + P_ROT]);
+        nc.qB = ldq(B.pose[4 * (ab.y) + P_ROT]);
+        float4 sA4 = B.sizeRad[ab.x], sB4 = B.sizeRad[ab.y];
+        float4 hA = B.shape[ab.x], hB = B.shape[ab.y];
+        int shA = (int)hA.z, shB = (int)hB.z;
+        nc.friction = sqrtf(B.friction[ab.x] * B.friction[ab.y]);
+        if (shA == kShapeSphere || shB == kShapeSphere)
+            nc.friction = 0.0f;
+        nc.refresh = any_round(shA) || any_round(shB);
+        nc.alpha = P.alpha;
+        nc.gamma = P.gamma;
+        nc.hAx = hA.x;
+        nc.hAy = hA.y;
+        nc.hBx = hB.x;
+        nc.hBy = hB.y;
+        nc.curFeature = cur.feature;
+        nc.curCr = cur.cr;
+        nc.tmpFeature = tmp.feature;
+        nc.tmpCr = tmp.cr;
+        nc.matchOld = matchOld;
+        nc.stickUsed = stickUsed;
+        nc.b0 = nc.b1 = nc.b2 = V3T<float>{0, 0, 0};
+        int nn = collide_boxes<float>(tv(nc.pA), tq(nc.qA), tv(ld3(sA4)), tv(nc.pB), tq(nc.qB), tv(ld3(sB4)), nc,
+                                      nc.b0, nc.b1, nc.b2);
+        if (nn == 0)
+        {
+            keep[m] = 0;
+            continue;
+        }
+        keep[m] = 1;
+        tmp.ab[m] = ab;
+        tmp.serial[m] = cur.serial[m];
+        tmp.count[m] = nn;
     }
 }
 
