@@ -440,14 +440,25 @@ __global__ void k_axis_partial(const float4 *pose, int n, double *partials)
 
 // Decision on the summed statistics; flags the exact serial pass when two variances are within
 // the error bound of the reference's n-term serial float sums.
+// One wave: lane b loads block b's six partial sums and the wave adds them by butterfly (any
+// order of the double sums is within the tolerance band below).
 __global__ void k_axis_decide(const double *partials, int nparts, int n, Counters *ctr)
 {
-    if (threadIdx.x != 0)
-        return;
-    double t[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < nparts; ++b)
+    const int lane = threadIdx.x;
+    double t[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        t[k] = 0.0;
+    for (int b = lane; b < nparts; b += 64)
+#pragma unroll
         for (int k = 0; k < 6; ++k)
             t[k] += partials[b * 6 + k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        for (int off = 32; off > 0; off >>= 1)
+            t[k] += __shfl_xor(t[k], off, 64);
+    if (lane != 0)
+        return;
     double v[3];
     for (int k = 0; k < 3; ++k)
     {
@@ -1548,11 +1559,80 @@ __global__ void k_warmstart(Bodies B, Params P, const int *deg, const int *ignor
 // neighbours), so the order of evaluation does not change the result; the colouring ends after
 // a global round in which no block changed any colour.
 constexpr int kColorLocalPasses = 8;
+constexpr int kColorCachedPartners = 16;
+
+// Marks in `used` the colours (< 256) of body i's partners below i in adjPartner[b, e), reading
+// them in batches of 8: all partner indices, then all their colours, are in flight together.
+__device__ __forceinline__ void color_used_list(const int *adjPartner, int b, int e, int i, const int *color,
+                                                unsigned long long used[4])
+{
+    for (int x0 = b; x0 < e; x0 += 8)
+    {
+        int o[8], c[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            o[q] = x0 + q < e ? adjPartner[x0 + q] : -1;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            c[q] = (o[q] >= 0 && o[q] < i) ? __hip_atomic_load(&color[o[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                           : -1;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            if (c[q] >= 0 && c[q] < 256)
+                used[c[q] >> 6] |= 1ull << (c[q] & 63);
+    }
+}
+
+// Smallest colour not in `used` (256 when all 256 are taken).
+__device__ __forceinline__ int color_mex(const unsigned long long used[4])
+{
+    int c = 0;
+    for (int w = 0; w < 4; ++w)
+    {
+        if (~used[w])
+        {
+            c = w * 64 + __ffsll((long long)~used[w]) - 1;
+            break;
+        }
+        c = 256;
+    }
+    return c;
+}
+
+// When every thread holds at most one body (n <= grid threads), the body's status, its own
+// colour and its lower-index partners (up to kColorCachedPartners) are loaded once and kept in
+// registers through all passes and rounds; a pass then costs one round trip for the partners'
+// colours. Only the owning thread writes a body's colour, so the register copy stays exact.
 __global__ void k_color(const int *deg, const int *ignoreDeg, const int *adjStart, const int *adjPartner,
                         const float4 *massMoment, int n, int *color, Counters *ctr, GridBarrier *gb, int maxRounds)
 {
     const unsigned nblocks = gridDim.x;
     __shared__ int localChange, roundChange;
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool oneEach = (long long)n <= (long long)nblocks * blockDim.x;
+    bool cAct = false, cFit = false;
+    int cOwn = 0, cb = 0, ce = 0;
+    int cp[kColorCachedPartners];
+#pragma unroll
+    for (int q = 0; q < kColorCachedPartners; ++q)
+        cp[q] = -1;
+    if (oneEach && i0 < n)
+    {
+        cAct = (massMoment[i0].x > 0.0f) && !(deg[i0] == 0 && !(ignoreDeg && ignoreDeg[i0] > 0));
+        if (cAct)
+        {
+            cb = adjStart[i0];
+            ce = cb + deg[i0];
+            cFit = ce - cb <= kColorCachedPartners;
+#pragma unroll
+            for (int q = 0; q < kColorCachedPartners; ++q)
+            {
+                const int o = (cFit && cb + q < ce) ? adjPartner[cb + q] : -1;
+                cp[q] = (o >= 0 && o < i0) ? o : -1;
+            }
+            cOwn = color[i0];
+        }
+    }
     int rounds = 0;
     for (int r = 0; r < maxRounds; ++r)
     {
@@ -1568,46 +1648,53 @@ __global__ void k_color(const int *deg, const int *ignoreDeg, const int *adjStar
                 localChange = 0;
             __syncthreads();
             int changed = 0;
-            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += nblocks * blockDim.x)
+            if (oneEach)
             {
-                if (!(massMoment[i].x > 0.0f) || (deg[i] == 0 && !(ignoreDeg && ignoreDeg[i] > 0)))
-                    continue;
-                unsigned long long used[4] = {0ull, 0ull, 0ull, 0ull};
-                int b = adjStart[i], e = b + deg[i];
-                // partners in batches of 8: all partner indices, then all their colours, are in
-                // flight together instead of one dependent round trip per partner
-                for (int x0 = b; x0 < e; x0 += 8)
+                if (cAct)
                 {
-                    int o[8], c[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        o[q] = x0 + q < e ? adjPartner[x0 + q] : -1;
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        c[q] = (o[q] >= 0 && o[q] < i)
-                                   ? __hip_atomic_load(&color[o[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                   : -1;
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        if (c[q] >= 0 && c[q] < 256)
-                            used[c[q] >> 6] |= 1ull << (c[q] & 63);
-                }
-                int c = 0;
-                for (int w = 0; w < 4; ++w)
-                {
-                    if (~used[w])
+                    unsigned long long used[4] = {0ull, 0ull, 0ull, 0ull};
+                    if (cFit)
                     {
-                        c = w * 64 + __ffsll((long long)~used[w]) - 1;
-                        break;
+                        int c[kColorCachedPartners];
+#pragma unroll
+                        for (int q = 0; q < kColorCachedPartners; ++q)
+                            c[q] = cp[q] >= 0 ? __hip_atomic_load(&color[cp[q]], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT)
+                                              : -1;
+#pragma unroll
+                        for (int q = 0; q < kColorCachedPartners; ++q)
+                            if (c[q] >= 0 && c[q] < 256)
+                                used[c[q] >> 6] |= 1ull << (c[q] & 63);
                     }
-                    c = 256;
+                    else
+                        color_used_list(adjPartner, cb, ce, i0, color, used);
+                    const int c = color_mex(used);
+                    if (c >= kMaxColors)
+                        atomicOr(&ctr->overflow, 8);
+                    if (cOwn != c)
+                    {
+                        __hip_atomic_store(&color[i0], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        cOwn = c;
+                        changed = 1;
+                    }
                 }
-                if (c >= kMaxColors)
-                    atomicOr(&ctr->overflow, 8);
-                if (color[i] != c)
+            }
+            else
+            {
+                for (int i = i0; i < n; i += nblocks * blockDim.x)
                 {
-                    __hip_atomic_store(&color[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    changed = 1;
+                    if (!(massMoment[i].x > 0.0f) || (deg[i] == 0 && !(ignoreDeg && ignoreDeg[i] > 0)))
+                        continue;
+                    unsigned long long used[4] = {0ull, 0ull, 0ull, 0ull};
+                    color_used_list(adjPartner, adjStart[i], adjStart[i] + deg[i], i, color, used);
+                    const int c = color_mex(used);
+                    if (c >= kMaxColors)
+                        atomicOr(&ctr->overflow, 8);
+                    if (color[i] != c)
+                    {
+                        __hip_atomic_store(&color[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        changed = 1;
+                    }
                 }
             }
             if (changed)
