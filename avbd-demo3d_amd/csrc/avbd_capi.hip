@@ -389,7 +389,7 @@ int enqueue_prepare(avbd_handle *h)
             radix_sort(h, keys, vals, keysAlt, valsAlt, N, 4);
             k_sap_gather<<<ceil_div(N, 256), 256, 0, s>>>(vals, h->B.pose, h->B.sizeRad, N, ctr, S.sLo, S.sHi, S.sPos,
                                                           S.sIdx);
-            k_sweep_len<<<ceil_div(N, 256), 256, 0, s>>>(S.sLo, S.sHi, N, h->sweepEnd, S.chunkCount, ctr);
+            k_sweep_len<<<ceil_div(N, 1024), 1024, 0, s>>>(S.sLo, S.sHi, N, h->sweepEnd, S.chunkCount, ctr);
         }
         scan(h, S.chunkCount, S.chunkStart, nullptr, N, &ctr->numChunks);
         k_clamp_count<<<1, 1, 0, s>>>(&ctr->numChunks, h->maxChunks, &ctr->overflow, 32);
@@ -420,12 +420,12 @@ int enqueue_prepare(avbd_handle *h)
         k_narrowphase_boxes<<<4 * G, kNpBoxThreads, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch,
                                                             h->recStick);
     scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numKept);
-    k_compact_manifolds<<<G, 256, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc);
+    k_compact_manifolds<<<G / 4, 1024, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc);
     k_compact_finish<<<1, 1, 0, s>>>(ctr);
     scan(h, h->cur.count, S.contactStart, &ctr->numManifolds, 0, &ctr->contactCount);
     k_contact_records<<<G, 256, 0, s>>>(h->cur.count, h->cur.ab, S.contactStart, ctr, S.contactRec);
     if (h->nj > 0)
-        k_joint_init<<<ceil_div(h->nj, 256), 256, 0, s>>>(h->joints, h->nj, h->B, h->P, ctr);
+        k_joint_init<<<ceil_div(h->nj, 1024), 1024, 0, s>>>(h->joints, h->nj, h->B, h->P, ctr);
     if (h->timing)
         HIP_TRY(hipEventRecord(h->ev[2], s));
 
@@ -439,13 +439,13 @@ int enqueue_prepare(avbd_handle *h)
                                   S.adjCursor, S.adj, S.adjSerial, S.adjPartner, h->maxAdj, &ctr->overflow);
     k_adj_sort<<<std::max(1, std::min(ceil_div(N, 4), 16384)), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjSerial,
                                                                           S.adjPartner, N, h->maxAdj);
-    k_stamp_count<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, h->cur.count, N, S.stampCount,
+    k_stamp_count<<<ceil_div(N, 1024), 1024, 0, s>>>(S.deg, S.adjStart, S.adj, h->cur.count, N, S.stampCount,
                                                    &ctr->maxStampCount);
     scan(h, S.stampCount, S.stampStart, nullptr, N, &ctr->numStamps);
     k_stamp_fill<<<ceil_div(N, 256), 256, 0, s>>>(S.deg, S.adjStart, S.adj, S.adjPartner, h->cur.count, h->cur.ab, N,
                                                   S.stampStart, S.stamps, h->maxStamps, &ctr->overflow);
     const int *ignoreDeg = h->numIgnore > 0 ? h->ignoreDeg : nullptr;
-    k_warmstart<<<ceil_div(N, 256), 256, 0, s>>>(h->B, h->P, S.deg, ignoreDeg, N, S.color, ctr);
+    k_warmstart<<<ceil_div(N, 1024), 1024, 0, s>>>(h->B, h->P, S.deg, ignoreDeg, N, S.color, ctr);
     k_step_bytes<<<1, 1, 0, s>>>(ctr, N, h->P.iterations);
 
     // ---- colouring (avbd_gpu_solver.cpp:1812-1865) ----
@@ -470,7 +470,7 @@ int enqueue_prepare(avbd_handle *h)
     {
         unsigned *keys = S.keys, *keysAlt = S.keysAlt;
         int *vals = S.vals, *valsAlt = S.valsAlt;
-        k_color_keys<<<ceil_div(N, 256), 256, 0, s>>>(S.color, S.stampCount, N, keys, vals, ctr);
+        k_color_keys<<<ceil_div(N, 1024), 1024, 0, s>>>(S.color, S.stampCount, N, keys, vals, ctr);
         int tiles = ceil_div(N, kRadixTile);
         k_radix_hist<<<tiles, 256, 0, s>>>(keys, N, 0, S.radixHist, tiles);
         scan(h, S.radixHist, h->radixHistScan, nullptr, 256 * tiles, nullptr);
@@ -1886,22 +1886,41 @@ int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launc
     float bestMs = 0.0f;
     if ((r = trial(best, bestMs)))
         return r;
-    // primal shape first (with the dual at the current shape), then the dual, then the graph form
+    // primal shape first (with the dual at the current shape), then the dual, then the graph form.
+    // Every primal shape is timed twice, the second pass in reverse order, and keeps its faster
+    // time, so that a clock change during the sweep does not decide the shape.
+    struct Shape
+    {
+        int lanes, threads;
+        float ms;
+    };
+    std::vector<Shape> shapes;
     for (int lanes : {8, 16, 32})
         for (int threads : {64, 128, 256})
+            shapes.push_back({lanes, threads, 3.0e38f});
+    for (int pass = 0; pass < 2; ++pass)
+        for (size_t q = 0; q < shapes.size(); ++q)
         {
+            Shape &sh = shapes[pass == 0 ? q : shapes.size() - 1 - q];
             avbd_launch_config c = best;
-            c.primal_lanes = lanes;
-            c.primal_threads = threads;
+            c.primal_lanes = sh.lanes;
+            c.primal_threads = sh.threads;
             float ms = 0.0f;
             if ((r = trial(c, ms)))
                 return r;
-            if (ms < bestMs)
-            {
-                bestMs = ms;
-                best = c;
-            }
+            sh.ms = std::min(sh.ms, ms);
         }
+    {
+        const avbd_launch_config base = best;
+        for (const Shape &sh : shapes)
+            if (sh.ms < bestMs)
+            {
+                bestMs = sh.ms;
+                best = base;
+                best.primal_lanes = sh.lanes;
+                best.primal_threads = sh.threads;
+            }
+    }
     for (int threads : {64, 128, 256})
     {
         avbd_launch_config c = best;

@@ -54,6 +54,58 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
+// Block-wide sums / maximum handed to one same-address device atomic per block. Device atomics
+// on one address from many CUs (and XCDs) serialize, about 10 ns each, so one per wave costs
+// tens of microseconds on large grids. Every thread of the block must call these (no early
+// return before them); blocks of at most 1024 threads.
+__device__ __forceinline__ void block_add2(int a, int *da, int b, int *db)
+{
+    __shared__ int part[2][16];
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((threadIdx.x & 63) == 0)
+    {
+        part[0][threadIdx.x >> 6] = a;
+        part[1][threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int sa = 0, sb = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w)
+        {
+            sa += part[0][w];
+            sb += part[1][w];
+        }
+        if (sa && da)
+            atomicAdd(da, sa);
+        if (sb && db)
+            atomicAdd(db, sb);
+    }
+}
+
+__device__ __forceinline__ void block_add(int a, int *da) { block_add2(a, da, 0, nullptr); }
+
+// Block maximum of v, applied with atomicMax when above `floor`.
+__device__ __forceinline__ void block_max(int v, int *dst, int floor)
+{
+    __shared__ int part[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = max(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0)
+        part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int m = part[0];
+        for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w)
+            m = max(m, part[w]);
+        if (m > floor)
+            atomicMax(dst, m);
+    }
+}
+
 // Exclusive scan over the block; lds needs blockDim/64 ints.
 __device__ __forceinline__ int block_exclusive_scan(int v, int *lds, int &total)
 {
@@ -558,9 +610,7 @@ __global__ void k_sweep_len(const float *sLo, const float *sHi, int n, int *swee
         len = lo_ - i - 1;
         chunkCount[i] = (len + kSweepChunk - 1) / kSweepChunk;
     }
-    int s = wave_sum(len);
-    if ((threadIdx.x & 63) == 0 && s)
-        atomicAdd(&ctr->pairChecks, s);
+    block_add(len, &ctr->pairChecks);
 }
 
 // Batched worlds (each an independent reference Solver, bodies stored world by world): one
@@ -711,9 +761,7 @@ __global__ __launch_bounds__(256) void k_world_sap(const float4 *pose, const flo
         chunkCount[ws + k] = (len + kSweepChunk - 1) / kSweepChunk;
         checks += len;
     }
-    int sum = wave_sum(checks);
-    if ((t & 63) == 0 && sum)
-        atomicAdd(&ctr->pairChecks, sum);
+    block_add(checks, &ctr->pairChecks);
 }
 
 // Bounding-sphere test of the reference sweep (solver.cpp:558-561).
@@ -1288,21 +1336,16 @@ __global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *kee
                 dst[q] = src[q];
         }
     }
-    contacts = wave_sum(contacts);
-    endpoints = wave_sum(endpoints);
     (void)contacts;
-    if ((threadIdx.x & 63) == 0 && endpoints)
-        atomicAdd(&ctr->contactEndpoints, endpoints);
+    block_add(endpoints, &ctr->contactEndpoints);
 }
 
 __global__ void k_compact_finish(Counters *ctr) { ctr->numManifolds = ctr->numKept; }
 
 // Joint::initialize (joint.cpp:35-54); a broken joint is removed (dead) here.
-__global__ void k_joint_init(Joint *joints, int nj, Bodies B, Params P, Counters *ctr)
+// One joint of k_joint_init; live / ends receive the joint's live count and solved endpoints.
+__device__ __forceinline__ void joint_init_one(Joint *joints, int k, Bodies &B, const Params &P, int &live, int &ends)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nj)
-        return;
     Joint j = joints[k];
     if (j.dead)
         return;
@@ -1333,9 +1376,18 @@ __global__ void k_joint_init(Joint *joints, int nj, Bodies B, Params P, Counters
     else
     {
         int e = (j.a >= 0 && B.massMoment[j.a].x > 0.0f ? 1 : 0) + (B.massMoment[j.b].x > 0.0f ? 1 : 0);
-        atomicAdd(&ctr->jointLive, 1);
-        atomicAdd(&ctr->jointEndpoints, e);
+        live = 1;
+        ends = e;
     }
+}
+
+__global__ void k_joint_init(Joint *joints, int nj, Bodies B, Params P, Counters *ctr)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int live = 0, ends = 0;
+    if (k < nj)
+        joint_init_one(joints, k, B, P, live, ends);
+    block_add2(live, &ctr->jointLive, ends, &ctr->jointEndpoints);
 }
 
 // Algorithmic HBM bytes of this step from its own counts, with the reference's
@@ -1505,11 +1557,10 @@ __global__ void k_adj_sort(const int *deg, const int *adjStart, int *adj, long l
 
 // Body warm start (solver.cpp:1134-1169).
 // ignoreDeg (may be null): per body, its IgnoreCollision forces; they make a body constrained.
-__global__ void k_warmstart(Bodies B, Params P, const int *deg, const int *ignoreDeg, int n, int *color, Counters *ctr)
+// One body of k_warmstart; returns 1 when the body is solved this step.
+__device__ __forceinline__ int warmstart_one(Bodies &B, const Params &P, const int *deg, const int *ignoreDeg, int i,
+                                             int *color)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
     float mass = B.massMoment[i].x;
     V3 pos = ld3(B.pose[4 * (i) + P_POS]), vel = ld3(B.vel[i]), w = ld3(B.angVel[i]);
     Q4 rot = ldq(B.pose[4 * (i) + P_ROT]);
@@ -1542,8 +1593,14 @@ __global__ void k_warmstart(Bodies B, Params P, const int *deg, const int *ignor
     }
     if (!solved)
         color[i] = -1;
-    else
-        atomicAdd(&ctr->solvedCount, 1);
+    return solved ? 1 : 0;
+}
+
+__global__ void k_warmstart(Bodies B, Params P, const int *deg, const int *ignoreDeg, int n, int *color, Counters *ctr)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int solved = i < n ? warmstart_one(B, P, deg, ignoreDeg, i, color) : 0;
+    block_add(solved, &ctr->solvedCount);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1833,20 +1890,16 @@ __global__ __launch_bounds__(64) void k_color_worlds(const int *worldStart, cons
 __global__ void k_color_keys(const int *color, const int *stampCount, int n, unsigned *keys, int *vals, Counters *ctr)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int mx = -1;
+    int mx = 0;  // colour count seen: max colour + 1, capped
     if (i < n)
     {
         int c = color[i];
         unsigned col = c < 0 ? 255u : (unsigned)min(c, kMaxColors - 1);
         keys[i] = (col << 8) | (unsigned)(255 - min(stampCount[i], 255));
         vals[i] = i;
-        mx = c;
+        mx = c >= 0 ? min(c + 1, kMaxColors) : 0;
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        mx = max(mx, __shfl_xor(mx, off, 64));
-    if ((threadIdx.x & 63) == 0 && mx >= 0)
-        atomicMax(&ctr->colorCount, min(mx + 1, kMaxColors));
+    block_max(mx, &ctr->colorCount, 0);
 }
 
 __global__ void k_color_offsets(const int *scannedHist, int tiles, int n, int *colorOffsets)
@@ -2273,12 +2326,7 @@ __global__ void k_stamp_count(const int *deg, const int *adjStart, const int *ad
         }
         stampCount[i] = s;
     }
-    int m = s;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-        m = max(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0 && m > 0)
-        atomicMax(maxStampCount, m);
+    block_max(s, maxStampCount, 0);
 }
 
 // Stamps in colour-sorted task order at a fixed stride S per task (S a power of two at least the
