@@ -420,7 +420,7 @@ int enqueue_prepare(avbd_handle *h)
         k_narrowphase_boxes<<<4 * G, kNpBoxThreads, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch,
                                                             h->recStick);
     scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numKept);
-    k_compact_manifolds<<<G / 4, 1024, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc);
+    k_compact_manifolds<<<G / 2, 1024, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc);
     k_compact_finish<<<1, 1, 0, s>>>(ctr);
     scan(h, h->cur.count, S.contactStart, &ctr->numManifolds, 0, &ctr->contactCount);
     k_contact_records<<<G, 256, 0, s>>>(h->cur.count, h->cur.ab, S.contactStart, ctr, S.contactRec);

@@ -1306,37 +1306,37 @@ __global__ __launch_bounds__(kNpBoxThreads) void k_narrowphase_boxes(Manifolds c
 }
 
 // srcSlot (adjoint recording, may be null): the pre-compaction slot of each kept manifold.
+// Eight lanes per manifold: lane q copies float4 q of each live contact record (and feature q),
+// so a manifold's records move as whole 128-byte rows.
 __global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *keep, const int *keepRank,
                                     const float4 *massMoment, Counters *ctr, int *srcSlot)
 {
-    int total = ctr->numTotal;
-    int contacts = 0, endpoints = 0;
-    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < total; m += gridDim.x * blockDim.x)
+    static_assert(kCR == 8 && kMaxContacts == 8, "one lane per record float4 and per feature slot");
+    const int total = ctr->numTotal;
+    const int q = threadIdx.x & 7;
+    int endpoints = 0;
+    for (int m = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 3); m < total;
+         m += (int)((gridDim.x * blockDim.x) >> 3))
     {
         if (!keep[m])
             continue;
-        int d = keepRank[m];
-        int n = tmp.count[m];
-        int2 ab = tmp.ab[m];
-        if (srcSlot)
-            srcSlot[d] = m;
-        contacts += n;
-        endpoints += n * ((massMoment[ab.x].x > 0.0f ? 1 : 0) + (massMoment[ab.y].x > 0.0f ? 1 : 0));
-        cur.ab[d] = ab;
-        cur.serial[d] = tmp.serial[m];
-        cur.count[d] = n;
-        for (int i = 0; i < n; ++i)
+        const int d = keepRank[m];
+        const int n = tmp.count[m];
+        if (q == 0)
         {
-            int s = m * kMaxContacts + i, o = d * kMaxContacts + i;
-            cur.feature[o] = tmp.feature[s];
-            const float4 *src = tmp.cr + (size_t)s * kCR;
-            float4 *dst = cur.cr + (size_t)o * kCR;
-#pragma unroll
-            for (int q = 0; q < kCR; ++q)
-                dst[q] = src[q];
+            int2 ab = tmp.ab[m];
+            if (srcSlot)
+                srcSlot[d] = m;
+            endpoints += n * ((massMoment[ab.x].x > 0.0f ? 1 : 0) + (massMoment[ab.y].x > 0.0f ? 1 : 0));
+            cur.ab[d] = ab;
+            cur.serial[d] = tmp.serial[m];
+            cur.count[d] = n;
         }
+        if (q < n)
+            cur.feature[d * kMaxContacts + q] = tmp.feature[m * kMaxContacts + q];
+        for (int i = 0; i < n; ++i)
+            cur.cr[(size_t)(d * kMaxContacts + i) * kCR + q] = tmp.cr[(size_t)(m * kMaxContacts + i) * kCR + q];
     }
-    (void)contacts;
     block_add(endpoints, &ctr->contactEndpoints);
 }
 

@@ -143,8 +143,9 @@ def main():
     world.synchronize()
     launch = world.launch_config()
     if not args.no_tune:
-        # direct launches only: the per-kernel timing marks need them
-        launch = world.autotune(trial_steps=8, allow_graph=False)
+        # direct launches only: the per-kernel timing marks need them; 20 steps per trial so
+        # that the steps right after each trial's state restore do not decide the shape
+        launch = world.autotune(trial_steps=20, allow_graph=False)
         log(f"autotuned launch shape {launch}")
         world.step(2)
         world.synchronize()
