@@ -370,12 +370,31 @@ __global__ void k_radix_hist(const unsigned *keys, int n, int shift, int *hist, 
     hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
+// offs: the exclusive scan of the digit-major histogram, or, with fused != 0, the histogram
+// itself: each block then forms its own offsets (digit d's total over all tiles and its count in
+// the tiles before this one, then a block scan over the digits), which saves the separate scan
+// launch when the tiles are few.
 __global__ void k_radix_scatter(const unsigned *keys, const int *vals, int n, int shift, const int *offs, int tiles,
-                                unsigned *keysOut, int *valsOut)
+                                unsigned *keysOut, int *valsOut, int fused)
 {
     __shared__ int base[256];
     __shared__ int waveCnt[4][256];
-    base[threadIdx.x] = offs[threadIdx.x * tiles + blockIdx.x];
+    if (fused)
+    {
+        __shared__ int lds[4];
+        const int *h = offs + threadIdx.x * tiles;
+        int tot = 0, pre = 0;
+        for (int t = 0; t < tiles; ++t)
+        {
+            const int c = h[t];
+            tot += c;
+            pre += t < (int)blockIdx.x ? c : 0;
+        }
+        int all;
+        base[threadIdx.x] = block_exclusive_scan(tot, lds, all) + pre;
+    }
+    else
+        base[threadIdx.x] = offs[threadIdx.x * tiles + blockIdx.x];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     for (int r = 0; r < kRadixTile / 256; ++r)
