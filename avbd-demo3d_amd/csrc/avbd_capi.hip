@@ -9,6 +9,7 @@
 #include "avbd_kernels.hip"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -256,16 +257,19 @@ constexpr int kColorThreads = 1024;
 constexpr int kColorSmallRounds = 6;  // round limit of the fixpoint for a world of at most kWorldSapMax bodies
 
 // Exclusive scan with the count either on the device (nptr) or given (nconst).
-void scan(avbd_handle *h, const int *in, int *out, const int *nptr, int nconst, int *total)
+// total (may be null) receives the sum, clamped to cap (a clamp sets capBit in the overflow flags).
+void scan(avbd_handle *h, const int *in, int *out, const int *nptr, int nconst, int *total, int cap = INT_MAX,
+          int capBit = 0)
 {
     hipStream_t s = h->stream;
     if (!nptr && nconst <= kScanSmallMax)
     {
-        k_scan_small<<<1, 1024, 0, s>>>(in, out, nconst, total);
+        k_scan_small<<<1, 1024, 0, s>>>(in, out, nconst, total, cap, &h->ctr->overflow, capBit);
         return;
     }
     k_scan_lookback<<<kLbBlocks, kLbThreads, 0, s>>>(in, out, nptr, nconst, total, h->scanStatus, h->scanCtl,
-                                                    h->scanMaxTiles, &h->ctr->barrierTimeout);
+                                                    h->scanMaxTiles, &h->ctr->barrierTimeout, cap,
+                                                    &h->ctr->overflow, capBit);
 }
 
 // Stable LSD radix sort of (key, value); `passes` 8-bit digits from bit 0. Result in keys/vals.
@@ -290,15 +294,6 @@ void radix_sort(avbd_handle *h, unsigned *&keys, int *&vals, unsigned *&keysAlt,
         }
         std::swap(keys, keysAlt);
         std::swap(vals, valsAlt);
-    }
-}
-
-__global__ void k_clamp_count(int *count, int cap, int *overflow, int bit)
-{
-    if (*count > cap)
-    {
-        *count = cap;
-        atomicOr(overflow, bit);
     }
 }
 
@@ -401,21 +396,20 @@ int enqueue_prepare(avbd_handle *h)
                                                           S.sIdx);
             k_sweep_len<<<ceil_div(N, 1024), 1024, 0, s>>>(S.sLo, S.sHi, N, h->sweepEnd, S.chunkCount, ctr);
         }
-        scan(h, S.chunkCount, S.chunkStart, nullptr, N, &ctr->numChunks);
-        k_clamp_count<<<1, 1, 0, s>>>(&ctr->numChunks, h->maxChunks, &ctr->overflow, 32);
+        scan(h, S.chunkCount, S.chunkStart, nullptr, N, &ctr->numChunks, h->maxChunks, 32);
         // chunk owners (one wave per interval), then groups of chunks per wave
         k_chunk_owner<<<std::max(1, std::min(ceil_div(N, 4), 16384)), 256, 0, s>>>(S.chunkStart, S.chunkCount, N, ctr,
                                                                                    S.chunkOwner);
         const int sweepBlocks = std::max(1, std::min(ceil_div(h->maxChunks, 4 * kSweepGroup), 4096));
         k_sweep_count_g<<<sweepBlocks, 256, 0, s>>>(S.chunkOwner, S.chunkStart, h->sweepEnd, ctr, S.sPos, S.hitCount,
                                                     S.hitMask);
-        scan(h, S.hitCount, S.hitStart, &ctr->numChunks, 0, &ctr->numHits);
+        // numHits is clamped to the pair capacity here (k_sweep_emit_g bounds its writes itself)
+        scan(h, S.hitCount, S.hitStart, &ctr->numChunks, 0, &ctr->numHits, h->maxPairs, 1);
         k_fill_u64<<<grid_for(h->hashSize), 256, 0, s>>>(S.hash, h->hashSize, ~0ull);
         k_hash_insert<<<G, 256, 0, s>>>(h->cur.ab, ctr, h->joints, h->nj, h->springs, h->ns, h->ignorePairs,
                                         h->numIgnore, S.hash, S.hashMask, ctr);
         k_sweep_emit_g<<<sweepBlocks, 256, 0, s>>>(S.chunkOwner, S.chunkStart, ctr, S.hitMask, S.sIdx, S.hitStart,
                                                    S.hash, S.hashMask, S.hits, S.hitNew, h->maxPairs);
-        k_clamp_count<<<1, 1, 0, s>>>(&ctr->numHits, h->maxPairs, &ctr->overflow, 1);
         scan(h, S.hitNew, S.hitRank, &ctr->numHits, 0, &ctr->numCreated);
         k_create<<<G, 256, 0, s>>>(S.hits, S.hitNew, S.hitRank, ctr, h->cur, h->maxManifolds, h->maxPairs);
     }
@@ -429,9 +423,9 @@ int enqueue_prepare(avbd_handle *h)
     else
         k_narrowphase_boxes<<<4 * G, kNpBoxThreads, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch,
                                                             h->recStick);
-    scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numKept);
+    // the kept count is the new manifold count (nothing until the compaction reads the old one)
+    scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numManifolds);
     k_compact_manifolds<<<G / 2, 1024, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc);
-    k_compact_finish<<<1, 1, 0, s>>>(ctr);
     scan(h, h->cur.count, S.contactStart, &ctr->numManifolds, 0, &ctr->contactCount);
     k_contact_records<<<G, 256, 0, s>>>(h->cur.count, h->cur.ab, S.contactStart, ctr, S.contactRec);
     if (h->nj > 0)
@@ -440,8 +434,7 @@ int enqueue_prepare(avbd_handle *h)
         HIP_TRY(hipEventRecord(h->ev[2], s));
 
     // ---- per-body force lists, body warm start (solver.cpp:1134-1169) ----
-    k_fill_int<<<grid_for(N), 256, 0, s>>>(S.deg, N, 0);
-    k_fill_int<<<grid_for(N), 256, 0, s>>>(S.adjCursor, N, 0);
+    k_fill_int2<<<grid_for(N), 256, 0, s>>>(S.deg, S.adjCursor, N, 0);
     k_adj<false><<<G, 256, 0, s>>>(h->cur, ctr, h->joints, h->nj, h->springs, h->ns, h->B.massMoment, S.deg, S.adjStart,
                                    S.adjCursor, S.adj, S.adjSerial, S.adjPartner, h->maxAdj, &ctr->overflow);
     scan(h, S.deg, S.adjStart, nullptr, N, h->adjTotal);

@@ -168,7 +168,20 @@ __device__ __forceinline__ void grid_sync(GridBarrier *gb, unsigned nblocks, int
 // (4 consecutive elements per thread, coalesced), each scanned across the block and carried
 // into the next. One launch instead of three for the small histogram scans.
 constexpr int kScanSmallMax = 8192;
-__global__ __launch_bounds__(1024) void k_scan_small(const int *in, int *out, int n, int *total)
+
+// Stores a scan's total clamped to cap; a clamped total sets `bit` in *overflow.
+__device__ __forceinline__ void put_total(int *total, int v, int cap, int *overflow, int bit)
+{
+    if (v > cap)
+    {
+        v = cap;
+        atomicOr(overflow, bit);
+    }
+    *total = v;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_small(const int *in, int *out, int n, int *total, int cap,
+                                                     int *overflow, int bit)
 {
     __shared__ int lds[16];
     int carry = 0;
@@ -191,7 +204,7 @@ __global__ __launch_bounds__(1024) void k_scan_small(const int *in, int *out, in
         carry += tot;
     }
     if (threadIdx.x == 0 && total)
-        *total = carry;
+        put_total(total, carry, cap, overflow, bit);
 }
 
 // Single-pass exclusive scan with decoupled look-back, for counts that live on the device.
@@ -221,7 +234,8 @@ __device__ __forceinline__ void lb_publish(unsigned long long *w, unsigned epoch
 
 __global__ __launch_bounds__(kLbThreads) void k_scan_lookback(const int *in, int *out, const int *nptr, int nconst,
                                                               int *total, unsigned long long *status, unsigned *ctl,
-                                                              int maxTiles, int *timeout)
+                                                              int maxTiles, int *timeout, int cap, int *overflow,
+                                                              int bit)
 {
     __shared__ int lds[kLbThreads / 64];
     __shared__ int sTile, sPrefix;
@@ -229,7 +243,7 @@ __global__ __launch_bounds__(kLbThreads) void k_scan_lookback(const int *in, int
     const int numTiles = min((max(n, 0) + kLbTile - 1) / kLbTile, maxTiles);
     const unsigned epoch = __hip_atomic_load(&ctl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0x3FFFFFFFu;
     if (numTiles == 0 && blockIdx.x == 0 && threadIdx.x == 0 && total)
-        *total = 0;
+        put_total(total, 0, cap, overflow, bit);
     const bool vec = ((((uintptr_t)in) | ((uintptr_t)out)) & 15) == 0;
     for (;;)
     {
@@ -312,7 +326,7 @@ __global__ __launch_bounds__(kLbThreads) void k_scan_lookback(const int *in, int
             {
                 sPrefix = prefix;
                 if (t == numTiles - 1 && total)
-                    *total = prefix + tot;
+                    put_total(total, prefix + tot, cap, overflow, bit);
             }
         }
         __syncthreads();
@@ -448,6 +462,15 @@ __global__ void k_fill_int(int *p, int n, int v)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+__global__ void k_fill_int2(int *p, int *q, int n, int v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        p[i] = v;
+        q[i] = v;
+    }
 }
 
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
@@ -1358,8 +1381,6 @@ __global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *kee
     }
     block_add(endpoints, &ctr->contactEndpoints);
 }
-
-__global__ void k_compact_finish(Counters *ctr) { ctr->numManifolds = ctr->numKept; }
 
 // Joint::initialize (joint.cpp:35-54); a broken joint is removed (dead) here.
 // One joint of k_joint_init; live / ends receive the joint's live count and solved endpoints.
