@@ -96,6 +96,7 @@ struct avbd_handle
     int numWorlds = 1;
     int *worldStart = nullptr;
     std::vector<int> worldStartHost;
+    int maxWorldBodies = 0;  // largest world of the batch (sizes k_color_worlds' LDS)
     // launch shapes of the iteration kernels, and the iteration graph built from them
     avbd_launch_config launch{16, 256, 256, 0, 0, 1};
     int *dfDone = nullptr;  // dataflow primal: last iteration id each body finished
@@ -456,17 +457,21 @@ int enqueue_prepare(avbd_handle *h)
         // batched worlds: the sequential greedy colouring per world, one wave each. One world: the
         // parallel fixpoint; a world small enough for one workgroup gets kColorSmallRounds rounds,
         // after which (colours still changing along a long chain) the sequential rule finishes.
+        // k_color_worlds' LDS: starts (int) + colours and degrees (short) per body of the largest
+        // world, and the partner slice (short)
+        const int cap = std::max(1, std::min(h->maxWorldBodies, kWorldSapMax));
+        const size_t worldLds = (size_t)cap * 8 + (size_t)kColorWorldParts * 2;
         if (h->numWorlds > 1)
-            k_color_worlds<<<h->numWorlds, 64, 0, s>>>(h->worldStart, S.deg, ignoreDeg, S.adjStart, S.adjPartner,
-                                                       h->B.massMoment, S.color, ctr, 0);
+            k_color_worlds<<<h->numWorlds, 64, worldLds, s>>>(h->worldStart, S.deg, ignoreDeg, S.adjStart,
+                                                              S.adjPartner, h->B.massMoment, S.color, ctr, 0, cap);
         else
         {
             const bool small = N <= kWorldSapMax;
             k_color<<<h->coopColor, kColorThreads, 0, s>>>(S.deg, ignoreDeg, S.adjStart, S.adjPartner, h->B.massMoment, N,
                                                         S.color, ctr, h->gb, small ? kColorSmallRounds : N + 2);
             if (small)
-                k_color_worlds<<<1, 64, 0, s>>>(h->worldStart, S.deg, ignoreDeg, S.adjStart, S.adjPartner,
-                                                h->B.massMoment, S.color, ctr, 1);
+                k_color_worlds<<<1, 64, worldLds, s>>>(h->worldStart, S.deg, ignoreDeg, S.adjStart, S.adjPartner,
+                                                       h->B.massMoment, S.color, ctr, 1, cap);
         }
         HIP_TRY(hipGetLastError());
     }
@@ -1284,6 +1289,7 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     h->lastNumManifolds = M;
     h->numWorlds = 1;
     h->worldStartHost.assign({0, N});
+    h->maxWorldBodies = N;
     return AVBD_OK;
 }
 
@@ -1817,6 +1823,9 @@ int avbd_batch_upload(avbd_handle *h, const int32_t *world_body_offsets, int32_t
                       hipMemcpyHostToDevice));
     h->numWorlds = num_worlds;
     h->worldStartHost.assign(world_body_offsets, world_body_offsets + num_worlds + 1);
+    h->maxWorldBodies = 0;
+    for (int w = 0; w < num_worlds; ++w)
+        h->maxWorldBodies = std::max(h->maxWorldBodies, world_body_offsets[w + 1] - world_body_offsets[w]);
     return AVBD_OK;
 }
 
@@ -1856,7 +1865,7 @@ int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launc
     int r = avbd_synchronize(h);
     if (r)
         return r;
-    // snapshot of the resident state through the public tables, restored before every trial
+    // snapshot of the resident state through the public tables, restored after the trials
     int32_t nb = 0, nj = 0, ns = 0, nm = 0;
     int64_t next = 0;
     if ((r = avbd_state_counts(h, &nb, &nj, &ns, &nm, &next)))
@@ -1875,10 +1884,11 @@ int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launc
     };
     const avbd_launch_config start = h->launch;
     hipEvent_t t0 = h->ev[6], t1 = h->ev[7];
+    // Trials continue from the state the previous trial left (every shape steps to the same bits);
+    // the snapshot is restored once at the end. Restoring before each trial would put a
+    // host-to-device upload, and an idle GPU, in front of every timed window.
     auto trial = [&](const avbd_launch_config &c, float &ms) -> int {
         int rr;
-        if ((rr = restore()))
-            return rr;
         h->launch = c;
         if ((rr = avbd_step(h, 1)) || (rr = avbd_synchronize(h)))
             return rr;

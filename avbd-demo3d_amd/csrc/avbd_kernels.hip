@@ -1838,32 +1838,38 @@ constexpr int kColorWorldParts = 4096;
 __global__ __launch_bounds__(64) void k_color_worlds(const int *worldStart, const int *deg, const int *ignoreDeg,
                                                      const int *adjStart, const int *adjPartner,
                                                      const float4 *massMoment, int *color, Counters *ctr,
-                                                     int onlyIfPending)
+                                                     int onlyIfPending, int cap)
 {
     if (onlyIfPending && !ctr->colorPending)
         return;
-    __shared__ int col[kWorldSapMax];
-    __shared__ int st[kWorldSapMax];
-    __shared__ int dg[kWorldSapMax];
-    __shared__ int part[kColorWorldParts];
+    // dynamic LDS sized for the batch's largest world (cap bodies): force-list starts (int), then
+    // colours and degrees (short), then the partner slice as world-local indices (short)
+    extern __shared__ int colorLds[];
+    int *st = colorLds;
+    short *col = (short *)(st + cap);
+    short *dg = col + cap;
+    short *part = dg + cap;
     const int lane = threadIdx.x;
     const int ws = worldStart[blockIdx.x], n = worldStart[blockIdx.x + 1] - ws;
-    if (n <= 0)
+    if (n <= 0 || n > cap)
         return;
     const int pb = adjStart[ws];
     for (int l = lane; l < n; l += 64)
     {
         const int i = ws + l;
         const bool solved = massMoment[i].x > 0.0f && (deg[i] > 0 || (ignoreDeg && ignoreDeg[i] > 0));
-        col[l] = color[i];
+        col[l] = (short)color[i];
         st[l] = adjStart[i] - pb;
-        dg[l] = solved ? deg[i] : -1;
+        dg[l] = (short)(solved ? min(deg[i], 32767) : -1);
     }
     const int np = adjStart[ws + n - 1] + deg[ws + n - 1] - pb;
     const bool fits = np <= kColorWorldParts;
     if (fits)
         for (int x = lane; x < np; x += 64)
-            part[x] = adjPartner[pb + x];
+        {
+            const int o = adjPartner[pb + x];
+            part[x] = (short)(o >= 0 ? o - ws : -1);
+        }
     __syncthreads();
     for (int l = 0; l < n; ++l)
     {
@@ -1877,7 +1883,7 @@ __global__ __launch_bounds__(64) void k_color_worlds(const int *worldStart, cons
             int pc = -1;
             if (lane < d)
             {
-                const int o = fits ? part[b + lane] : adjPartner[pb + b + lane];
+                const int o = fits ? (part[b + lane] >= 0 ? ws + part[b + lane] : -1) : adjPartner[pb + b + lane];
                 if (o >= 0 && o < i)
                     pc = col[o - ws];
             }
@@ -1895,7 +1901,7 @@ __global__ __launch_bounds__(64) void k_color_worlds(const int *worldStart, cons
             unsigned long long used[4] = {0ull, 0ull, 0ull, 0ull};
             for (int x = 0; x < d; ++x)
             {
-                const int o = fits ? part[b + x] : adjPartner[pb + b + x];
+                const int o = fits ? (part[b + x] >= 0 ? ws + part[b + x] : -1) : adjPartner[pb + b + x];
                 if (o < 0 || o >= i)
                     continue;
                 const int pcx = col[o - ws];

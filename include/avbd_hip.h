@@ -357,8 +357,9 @@ int avbd_set_launch_config(avbd_handle *h, const avbd_launch_config *cfg);
 int avbd_get_launch_config(avbd_handle *h, avbd_launch_config *out);
 #define AVBD_TUNE_NO_GRAPH 1   /* keep direct launches (per-kernel timing marks stay exact) */
 /* Times every launch shape on the resident state for trial_steps steps each and
- * keeps the fastest. The state is restored before each trial and at the end,
- * so tuning changes speed only; stats sums are reset. out may be NULL. */
+ * keeps the fastest (primal shapes are timed twice). The trials step on from
+ * one another; the state is restored at the end, so tuning changes speed only;
+ * stats sums are reset. out may be NULL. */
 int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launch_config *out);
 
 /* ---- Adjoint of the step (reverse-mode gradients) ----
