@@ -420,10 +420,16 @@ int enqueue_prepare(avbd_handle *h)
 
     // ---- force initialisation (manifold.cpp:100-174, joint.cpp:35-54) ----
     if (h->numRound > 0)
-        k_narrowphase<true><<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick);
+        k_narrowphase<true><<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick,
+                                              nullptr, nullptr);
     else
+    {
         k_narrowphase_boxes<<<4 * G, kNpBoxThreads, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch,
-                                                            h->recStick);
+                                                            h->recStick, S.npRedo);
+        // the rare manifolds whose clip polygon outgrew the LDS capacity, at full capacity
+        k_narrowphase<true><<<64, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick,
+                                               S.npRedo, &ctr->npRedoCount);
+    }
     // the kept count is the new manifold count (nothing until the compaction reads the old one)
     scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numManifolds);
     k_compact_manifolds<<<G / 2, 1024, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc);
@@ -1004,7 +1010,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         (r = h->alloc(S.hitStart, h->maxChunks)) || (r = h->alloc(S.hitMask, h->maxChunks)) ||
         (r = h->alloc(S.chunkOwner, h->maxChunks)) || (r = h->alloc(S.hits, h->maxPairs)) ||
         (r = h->alloc(S.hitNew, h->maxPairs)) || (r = h->alloc(S.hitRank, h->maxPairs)) ||
-        (r = h->alloc(S.hash, h->hashSize)) || (r = h->alloc(S.keep, h->maxManifolds)) ||
+        (r = h->alloc(S.hash, h->hashSize)) || (r = h->alloc(S.keep, h->maxManifolds)) || (r = h->alloc(S.npRedo, h->maxManifolds)) ||
         (r = h->alloc(S.keepRank, h->maxManifolds)) || (r = h->alloc(S.deg, N)) || (r = h->alloc(S.adjStart, N)) ||
         (r = h->alloc(S.adjCursor, N)) || (r = h->alloc(S.adj, h->maxAdj)) || (r = h->alloc(S.adjSerial, h->maxAdj)) ||
         (r = h->alloc(S.color, N)) || (r = h->alloc(S.colorOffsets, 257)) || (r = h->alloc(S.colorBodies, N)) ||

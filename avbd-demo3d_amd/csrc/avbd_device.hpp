@@ -120,6 +120,7 @@ struct Counters
     int numStamps;
     int maxStampCount;      // the most stamps of one body this step
     int colorPending;       // k_color stopped at its round limit with colours still changing
+    int npRedoCount;      // box-only narrowphase: manifolds whose clip polygon outgrew LDS
     int jointLive;
     int jointEndpoints;
     int springEndpoints;    // fixed by the uploaded springs
@@ -161,6 +162,7 @@ struct Scratch
     int *hitCount;          // [maxChunks] hits per work item
     unsigned long long *hitMask; // [maxChunks] bit k: the k-th interval visited by the work item is a hit
     int *chunkOwner;             // [maxChunks] the interval each sweep chunk belongs to
+    int *npRedo;          // their indices, for the full-capacity narrowphase
     int *hitStart;
     int2 *hits;             // [maxPairs]
     int *hitNew;            // [maxPairs] 1 = unconstrained

@@ -496,6 +496,7 @@ __global__ void k_step_begin(Counters *c)
     c->jointLive = 0;
     c->jointEndpoints = 0;
     c->maxStampCount = 0;
+    c->npRedoCount = 0;
     c->colorChanged[0] = c->colorChanged[1] = c->colorChanged[2] = 0u;
 }
 
@@ -1071,13 +1072,15 @@ __global__ void k_create_finish(Counters *ctr, int maxManifolds, int maxPairs)
 // continues (-1: none) and whether its anchors were kept by the stick rule.
 // Round: the world holds round bodies (worlds of boxes only run k_narrowphase_boxes below).
 template <bool Round>
+// list / listCount (may be null): process only the listed manifolds.
 __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, Counters *ctr, int *keep,
-                              int *matchOld, int *stickUsed)
+                              int *matchOld, int *stickUsed, const int *list, const int *listCount)
 {
-    int total = ctr->numTotal;
-    int old = ctr->numManifolds;
-    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < total; m += gridDim.x * blockDim.x)
+    const int total = list ? *listCount : ctr->numTotal;
+    const int old = ctr->numManifolds;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x)
     {
+        const int m = list ? list[k] : k;
         int2 ab = cur.ab[m];
         int nOld = m < old ? cur.count[m] : 0;
         V3 pA = ld3(B.pose[4 * (ab.x) + P_POS]), pB = ld3(B.pose[4 * (ab.y) + P_POS]);
@@ -1183,17 +1186,29 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
 // the two clip polygons sit in LDS. Private arrays would go to scratch memory, whose latency and
 // per-wave allocation throttled the kernel.
 constexpr int kNpBoxThreads = 64;
+// LDS vertices per clip polygon. A quad clipped by four planes keeps at most 8 vertices; a
+// polygon that would grow past this (rounding near a plane can add vertices; the reference
+// allows 16) raises the lane's overflow flag, and the manifold is redone by the full-capacity
+// narrowphase.
+constexpr int kNpPolyCap = 8;
 
-// One lane's 16-vertex clip polygon in LDS, component-major with the lane index fastest.
+// One lane's clip polygon in LDS, component-major with the lane index fastest.
 struct LdsPoly
 {
     float *p;
+    int *ovf;  // the lane's overflow flag (LDS)
     AVBD_HD V3T<float> get(int i) const
     {
+        i = min(i, kNpPolyCap - 1);  // only read after an overflow, whose result is discarded
         return {p[(3 * i) * kNpBoxThreads], p[(3 * i + 1) * kNpBoxThreads], p[(3 * i + 2) * kNpBoxThreads]};
     }
     AVBD_HD void set(int i, V3T<float> x)
     {
+        if (i >= kNpPolyCap)
+        {
+            *ovf = 1;
+            return;
+        }
         p[(3 * i) * kNpBoxThreads] = x.x;
         p[(3 * i + 1) * kNpBoxThreads] = x.y;
         p[(3 * i + 2) * kNpBoxThreads] = x.z;
@@ -1207,6 +1222,7 @@ struct BoxRecordSink
     V3T<float> mid[kMaxNewContacts];
     V3T<float> b0, b1, b2; // the manifold basis, set by collide_boxes before the first contact
     float *poly;           // this lane's two clip polygons in LDS
+    int *ovf;              // this lane's polygon overflow flag in LDS
     int m, nOld;
     bool refresh;
     float friction, alpha, gamma, hAx, hAy, hBx, hBy;
@@ -1219,7 +1235,10 @@ struct BoxRecordSink
     int *matchOld, *stickUsed;
 };
 
-AVBD_HD LdsPoly make_poly(BoxRecordSink &nc, int k) { return LdsPoly{nc.poly + k * 48 * kNpBoxThreads}; }
+AVBD_HD LdsPoly make_poly(BoxRecordSink &nc, int k)
+{
+    return LdsPoly{nc.poly + k * 3 * kNpPolyCap * kNpBoxThreads, nc.ovf};
+}
 
 // ContactBuilder::add (collide.cpp:30-58), then the contact's part of Manifold::initialize
 // (manifold.cpp:100-174): warm start from the old contact with the same feature key, the stick
@@ -1293,12 +1312,14 @@ AVBD_HD void add_contact(BoxRecordSink &nc, V3T<float> posA, Q4T<float> rotA, V3
     o[CR_LAM] = st3(l, lam.w);
 }
 
-// Launched with kNpBoxThreads threads per block, grid-stride over the manifolds.
+// Launched with kNpBoxThreads threads per block, grid-stride over the manifolds. A manifold whose
+// clip polygon overflowed is listed in redo (its partial records are rewritten there).
 __global__ __launch_bounds__(kNpBoxThreads) void k_narrowphase_boxes(Manifolds cur, Manifolds tmp, Bodies B,
                                                                      Params P, Counters *ctr, int *keep,
-                                                                     int *matchOld, int *stickUsed)
+                                                                     int *matchOld, int *stickUsed, int *redo)
 {
-    __shared__ float polyLds[2 * 48 * kNpBoxThreads];
+    __shared__ float polyLds[2 * 3 * kNpPolyCap * kNpBoxThreads];
+    __shared__ int polyOvf[kNpBoxThreads];
     const int total = ctr->numTotal;
     const int old = ctr->numManifolds;
     for (int m = blockIdx.x * kNpBoxThreads + threadIdx.x; m < total; m += gridDim.x * kNpBoxThreads)
@@ -1307,6 +1328,8 @@ __global__ __launch_bounds__(kNpBoxThreads) void k_narrowphase_boxes(Manifolds c
         BoxRecordSink nc;
         nc.count = 0;
         nc.poly = polyLds + threadIdx.x;
+        nc.ovf = polyOvf + threadIdx.x;
+        polyOvf[threadIdx.x] = 0;
         nc.m = m;
         nc.nOld = m < old ? cur.count[m] : 0;
         nc.pA = ld3(B.pose[4 * (ab.x) + P_POS]);
@@ -1335,6 +1358,11 @@ __global__ __launch_bounds__(kNpBoxThreads) void k_narrowphase_boxes(Manifolds c
         nc.b0 = nc.b1 = nc.b2 = V3T<float>{0, 0, 0};
         int nn = collide_boxes<float>(tv(nc.pA), tq(nc.qA), tv(ld3(sA4)), tv(nc.pB), tq(nc.qB), tv(ld3(sB4)), nc,
                                       nc.b0, nc.b1, nc.b2);
+        if (polyOvf[threadIdx.x])
+        {
+            redo[atomicAdd(&ctr->npRedoCount, 1)] = m;
+            continue;
+        }
         if (nn == 0)
         {
             keep[m] = 0;
