@@ -54,6 +54,7 @@ def load(path: str | None = None):
         lib.avbd_get_colors.argtypes = [_p, _p, C.c_int32]
         lib.avbd_get_hits.argtypes = [_p, _p, C.c_int32, _p]
         lib.avbd_debug_inject_fault.argtypes = [_p, C.c_int32]
+        lib.avbd_debug_narrowphase_capacity.argtypes = [_p, C.c_int32]
         lib.avbd_default_params.argtypes = [_p]
         lib.avbd_make_box_body.argtypes = [_p, C.c_float, C.c_float, _p, _p, _p, _p, _p]
         lib.avbd_make_round_body.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_float, C.c_float, _p, _p, _p, _p, _p]
@@ -314,3 +315,7 @@ class HipWorld:
 
     def inject_fault(self, code: int = 1):
         _check(self._lib.avbd_debug_inject_fault(self._h, code))
+
+    def debug_narrowphase_capacity(self, vertices):
+        """Clip-polygon vertices the box-only narrowphase keeps in LDS (1..8); results do not change."""
+        _check(self._lib.avbd_debug_narrowphase_capacity(self._h, int(vertices)))

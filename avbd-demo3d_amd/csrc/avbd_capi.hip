@@ -97,6 +97,7 @@ struct avbd_handle
     int *worldStart = nullptr;
     std::vector<int> worldStartHost;
     int maxWorldBodies = 0;  // largest world of the batch (sizes k_color_worlds' LDS)
+    int npPolyCap = kNpPolyCap;  // clip-polygon vertices the box-only narrowphase keeps in LDS
     // launch shapes of the iteration kernels, and the iteration graph built from them
     avbd_launch_config launch{16, 256, 256, 0, 0, 1};
     int *dfDone = nullptr;  // dataflow primal: last iteration id each body finished
@@ -425,7 +426,7 @@ int enqueue_prepare(avbd_handle *h)
     else
     {
         k_narrowphase_boxes<<<4 * G, kNpBoxThreads, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch,
-                                                            h->recStick, S.npRedo);
+                                                            h->recStick, S.npRedo, h->npPolyCap);
         // the rare manifolds whose clip polygon outgrew the LDS capacity, at full capacity
         k_narrowphase<true><<<64, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick,
                                                S.npRedo, &ctr->npRedoCount);
@@ -2004,6 +2005,16 @@ int avbd_debug_inject_fault(avbd_handle *h, int32_t code)
     if (!h)
         return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_debug_inject_fault: null handle");
     h->injectFault = code;
+    return AVBD_OK;
+}
+
+int avbd_debug_narrowphase_capacity(avbd_handle *h, int32_t vertices)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_debug_narrowphase_capacity: null handle");
+    if (vertices < 1 || vertices > kNpPolyCap)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_debug_narrowphase_capacity: vertices must be in 1..8");
+    h->npPolyCap = vertices;
     return AVBD_OK;
 }
 

@@ -1197,14 +1197,15 @@ struct LdsPoly
 {
     float *p;
     int *ovf;  // the lane's overflow flag (LDS)
+    int cap;   // vertices kept (at most kNpPolyCap)
     AVBD_HD V3T<float> get(int i) const
     {
-        i = min(i, kNpPolyCap - 1);  // only read after an overflow, whose result is discarded
+        i = min(i, cap - 1);  // only read past cap after an overflow, whose result is discarded
         return {p[(3 * i) * kNpBoxThreads], p[(3 * i + 1) * kNpBoxThreads], p[(3 * i + 2) * kNpBoxThreads]};
     }
     AVBD_HD void set(int i, V3T<float> x)
     {
-        if (i >= kNpPolyCap)
+        if (i >= cap)
         {
             *ovf = 1;
             return;
@@ -1223,6 +1224,7 @@ struct BoxRecordSink
     V3T<float> b0, b1, b2; // the manifold basis, set by collide_boxes before the first contact
     float *poly;           // this lane's two clip polygons in LDS
     int *ovf;              // this lane's polygon overflow flag in LDS
+    int polyCap;           // vertices per polygon kept in LDS
     int m, nOld;
     bool refresh;
     float friction, alpha, gamma, hAx, hAy, hBx, hBy;
@@ -1237,7 +1239,7 @@ struct BoxRecordSink
 
 AVBD_HD LdsPoly make_poly(BoxRecordSink &nc, int k)
 {
-    return LdsPoly{nc.poly + k * 3 * kNpPolyCap * kNpBoxThreads, nc.ovf};
+    return LdsPoly{nc.poly + k * 3 * kNpPolyCap * kNpBoxThreads, nc.ovf, nc.polyCap};
 }
 
 // ContactBuilder::add (collide.cpp:30-58), then the contact's part of Manifold::initialize
@@ -1316,7 +1318,8 @@ AVBD_HD void add_contact(BoxRecordSink &nc, V3T<float> posA, Q4T<float> rotA, V3
 // clip polygon overflowed is listed in redo (its partial records are rewritten there).
 __global__ __launch_bounds__(kNpBoxThreads) void k_narrowphase_boxes(Manifolds cur, Manifolds tmp, Bodies B,
                                                                      Params P, Counters *ctr, int *keep,
-                                                                     int *matchOld, int *stickUsed, int *redo)
+                                                                     int *matchOld, int *stickUsed, int *redo,
+                                                                     int polyCap)
 {
     __shared__ float polyLds[2 * 3 * kNpPolyCap * kNpBoxThreads];
     __shared__ int polyOvf[kNpBoxThreads];
@@ -1329,6 +1332,7 @@ __global__ __launch_bounds__(kNpBoxThreads) void k_narrowphase_boxes(Manifolds c
         nc.count = 0;
         nc.poly = polyLds + threadIdx.x;
         nc.ovf = polyOvf + threadIdx.x;
+        nc.polyCap = polyCap;
         polyOvf[threadIdx.x] = 0;
         nc.m = m;
         nc.nOld = m < old ? cur.count[m] : 0;

@@ -386,6 +386,11 @@ int avbd_adjoint_backward(avbd_handle *h, const double *d_pos, const double *d_r
 /* Test hook: make the next avbd_step on this handle fail with AVBD_ERR_INJECTED
  * before any work is enqueued (code 1), or clear it (code 0). */
 int avbd_debug_inject_fault(avbd_handle *h, int32_t code);
+/* Test hook (no counterpart in the reference): the number of clip-polygon vertices
+ * (1..8, default 8) the box-only narrowphase keeps in LDS. Manifolds whose polygon
+ * outgrows it are redone by the full-capacity narrowphase, so results do not change;
+ * a small value sends most manifolds down that path. */
+int avbd_debug_narrowphase_capacity(avbd_handle *h, int32_t vertices);
 
 /* sizeof of the ABI records (0 body, 1 contact, 2 manifold, 3 joint, 4 spring,
  * 5 params, 6 launch config), so bindings can verify their layouts; -1 for

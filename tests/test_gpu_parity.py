@@ -168,6 +168,29 @@ def test_injected_fault_and_capacity_errors(gpu):
     assert "max_pairs" in str(e.value)
 
 
+def test_narrowphase_lds_capacity_redo_is_exact(gpu):
+    """The box-only narrowphase keeps its clip polygons in LDS up to a vertex capacity; a manifold
+    whose polygon outgrows it is listed and redone by the full-capacity narrowphase. With the
+    capacity cut to 3 most face manifolds take that path: the state must stay bit-identical to
+    the default capacity's."""
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scenes.pile(2000, seed=5)
+    prm = default_params(gravity=sc.gravity)
+    st = scenes.build_state(sc, gpu.Maker())
+    a = gpu.HipWorld.for_state(st, prm)
+    b = gpu.HipWorld.for_state(st, prm)
+    b.debug_narrowphase_capacity(3)
+    for _ in range(3):
+        a.step(40)
+        b.step(40)
+        sa, sb = a.state(), b.state()
+        assert_states_equal(sa, sb, "LDS capacity 3 vs 8")
+    assert len(sa.manifolds) > 500
+    with pytest.raises(gpu.AvbdError):
+        b.debug_narrowphase_capacity(9)
+
+
 def test_headline_size_single_step_vs_oracle(gpu, oracle):
     """50,000-box pile at its full size: after 120 GPU steps (falling, first contacts with the ground),
     one more step from that state equals the oracle's step from the same state, including the SAP pair
