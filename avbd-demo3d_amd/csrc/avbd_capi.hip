@@ -284,14 +284,14 @@ void radix_sort(avbd_handle *h, unsigned *&keys, int *&vals, unsigned *&keysAlt,
     for (int p = 0; p < passes; ++p)
     {
         int shift = 8 * p;
-        k_radix_hist<<<tiles, 256, 0, h->stream>>>(keys, n, shift, h->S.radixHist, tiles);
+        k_radix_hist<<<tiles, kRadixThreads, 0, h->stream>>>(keys, n, shift, h->S.radixHist, tiles);
         if (tiles <= kRadixFusedTiles)
-            k_radix_scatter<<<tiles, 256, 0, h->stream>>>(keys, vals, n, shift, h->S.radixHist, tiles, keysAlt,
+            k_radix_scatter<<<tiles, kRadixThreads, 0, h->stream>>>(keys, vals, n, shift, h->S.radixHist, tiles, keysAlt,
                                                           valsAlt, 1);
         else
         {
             scan(h, h->S.radixHist, h->radixHistScan, nullptr, 256 * tiles, nullptr);
-            k_radix_scatter<<<tiles, 256, 0, h->stream>>>(keys, vals, n, shift, h->radixHistScan, tiles, keysAlt,
+            k_radix_scatter<<<tiles, kRadixThreads, 0, h->stream>>>(keys, vals, n, shift, h->radixHistScan, tiles, keysAlt,
                                                           valsAlt, 0);
         }
         std::swap(keys, keysAlt);
@@ -487,18 +487,18 @@ int enqueue_prepare(avbd_handle *h)
         int *vals = S.vals, *valsAlt = S.valsAlt;
         k_color_keys<<<ceil_div(N, 1024), 1024, 0, s>>>(S.color, S.stampCount, N, keys, vals, ctr);
         int tiles = ceil_div(N, kRadixTile);
-        k_radix_hist<<<tiles, 256, 0, s>>>(keys, N, 0, S.radixHist, tiles);
+        k_radix_hist<<<tiles, kRadixThreads, 0, s>>>(keys, N, 0, S.radixHist, tiles);
         if (tiles <= kRadixFusedTiles)
-            k_radix_scatter<<<tiles, 256, 0, s>>>(keys, vals, N, 0, S.radixHist, tiles, keysAlt, valsAlt, 1);
+            k_radix_scatter<<<tiles, kRadixThreads, 0, s>>>(keys, vals, N, 0, S.radixHist, tiles, keysAlt, valsAlt, 1);
         else
         {
             scan(h, S.radixHist, h->radixHistScan, nullptr, 256 * tiles, nullptr);
-            k_radix_scatter<<<tiles, 256, 0, s>>>(keys, vals, N, 0, h->radixHistScan, tiles, keysAlt, valsAlt, 0);
+            k_radix_scatter<<<tiles, kRadixThreads, 0, s>>>(keys, vals, N, 0, h->radixHistScan, tiles, keysAlt, valsAlt, 0);
         }
         // the second pass keeps its scanned histogram: the colour offsets are read from it
-        k_radix_hist<<<tiles, 256, 0, s>>>(keysAlt, N, 8, S.radixHist, tiles);
+        k_radix_hist<<<tiles, kRadixThreads, 0, s>>>(keysAlt, N, 8, S.radixHist, tiles);
         scan(h, S.radixHist, h->radixHistScan, nullptr, 256 * tiles, nullptr);
-        k_radix_scatter<<<tiles, 256, 0, s>>>(keysAlt, valsAlt, N, 8, h->radixHistScan, tiles, keys, S.colorBodies,
+        k_radix_scatter<<<tiles, kRadixThreads, 0, s>>>(keysAlt, valsAlt, N, 8, h->radixHistScan, tiles, keys, S.colorBodies,
                                               0);
         k_color_offsets<<<1, 256, 0, s>>>(h->radixHistScan, tiles, N, S.colorOffsets);
         k_tasks<<<ceil_div(N, 256), 256, 0, s>>>(S.colorBodies, S.stampStart, S.stampCount, N, S.tasks);

@@ -368,55 +368,70 @@ __global__ __launch_bounds__(kLbThreads) void k_scan_lookback(const int *in, int
 // ------------------------------------------------------------------------------------------
 // stable LSD radix sort, 8 bits per pass, wave-ballot ranking
 // ------------------------------------------------------------------------------------------
-__global__ void k_radix_hist(const unsigned *keys, int n, int shift, int *hist, int tiles)
+// One block of kRadixThreads per tile of kRadixTile keys (two rounds of 1024).
+constexpr int kRadixThreads = 1024;
+static_assert(kRadixTile % kRadixThreads == 0, "whole rounds per tile");
+
+__global__ __launch_bounds__(kRadixThreads) void k_radix_hist(const unsigned *keys, int n, int shift, int *hist,
+                                                              int tiles)
 {
     __shared__ int cnt[256];
-    cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 256)
+        cnt[threadIdx.x] = 0;
     __syncthreads();
     int base = blockIdx.x * kRadixTile;
-    for (int r = 0; r < kRadixTile / 256; ++r)
+    for (int r = 0; r < kRadixTile / kRadixThreads; ++r)
     {
-        int i = base + r * 256 + threadIdx.x;
+        int i = base + r * kRadixThreads + threadIdx.x;
         if (i < n)
             atomicAdd(&cnt[(keys[i] >> shift) & 255u], 1);
     }
     __syncthreads();
-    hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+    if (threadIdx.x < 256)
+        hist[threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
 // offs: the exclusive scan of the digit-major histogram, or, with fused != 0, the histogram
 // itself: each block then forms its own offsets (digit d's total over all tiles and its count in
 // the tiles before this one, then a block scan over the digits), which saves the separate scan
-// launch when the tiles are few.
-__global__ void k_radix_scatter(const unsigned *keys, const int *vals, int n, int shift, const int *offs, int tiles,
-                                unsigned *keysOut, int *valsOut, int fused)
+// launch when the tiles are few. Keys are ranked per round by (wave, lane), i.e. in index order,
+// so the sort is stable.
+__global__ __launch_bounds__(kRadixThreads) void k_radix_scatter(const unsigned *keys, const int *vals, int n,
+                                                                 int shift, const int *offs, int tiles,
+                                                                 unsigned *keysOut, int *valsOut, int fused)
 {
+    constexpr int W = kRadixThreads / 64;
     __shared__ int base[256];
-    __shared__ int waveCnt[4][256];
+    __shared__ int waveCnt[W][256];
     if (fused)
     {
-        __shared__ int lds[4];
-        const int *h = offs + threadIdx.x * tiles;
+        __shared__ int lds[W];
         int tot = 0, pre = 0;
-        for (int t = 0; t < tiles; ++t)
+        if (threadIdx.x < 256)
         {
-            const int c = h[t];
-            tot += c;
-            pre += t < (int)blockIdx.x ? c : 0;
+            const int *h = offs + threadIdx.x * tiles;
+            for (int t = 0; t < tiles; ++t)
+            {
+                const int c = h[t];
+                tot += c;
+                pre += t < (int)blockIdx.x ? c : 0;
+            }
         }
         int all;
-        base[threadIdx.x] = block_exclusive_scan(tot, lds, all) + pre;
+        const int ex = block_exclusive_scan(tot, lds, all);
+        if (threadIdx.x < 256)
+            base[threadIdx.x] = ex + pre;
     }
-    else
+    else if (threadIdx.x < 256)
         base[threadIdx.x] = offs[threadIdx.x * tiles + blockIdx.x];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-    for (int r = 0; r < kRadixTile / 256; ++r)
+    for (int r = 0; r < kRadixTile / kRadixThreads; ++r)
     {
-        for (int t = threadIdx.x; t < 4 * 256; t += 256)
+        for (int t = threadIdx.x; t < W * 256; t += kRadixThreads)
             (&waveCnt[0][0])[t] = 0;
         __syncthreads();
-        int i = blockIdx.x * kRadixTile + r * 256 + threadIdx.x;
+        int i = blockIdx.x * kRadixTile + r * kRadixThreads + threadIdx.x;
         bool valid = i < n;
         unsigned k = valid ? keys[i] : 0u;
         int v = valid ? vals[i] : 0;
@@ -433,10 +448,11 @@ __global__ void k_radix_scatter(const unsigned *keys, const int *vals, int n, in
         if (valid && rank == 0)
             waveCnt[wave][d] = __popcll(peers);
         __syncthreads();
+        if (threadIdx.x < 256)
         {
             int dd = threadIdx.x;
             int run = base[dd];
-            for (int w = 0; w < 4; ++w)
+            for (int w = 0; w < W; ++w)
             {
                 int c = waveCnt[w][dd];
                 waveCnt[w][dd] = run;
