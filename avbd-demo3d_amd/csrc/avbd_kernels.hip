@@ -2412,11 +2412,20 @@ __global__ void k_stamp_count(const int *deg, const int *adjStart, const int *ad
     if (i < n)
     {
         int b = adjStart[i], e = b + deg[i];
-        for (int x = b; x < e; ++x)
+        // entries in batches of 8, then their contact counts, each batch in flight together
+        for (int x0 = b; x0 < e; x0 += 8)
         {
-            int en = adj[x];
-            int kind = adjKind(en);
-            s += kind == kAdjManifold ? mcount[adjIndex(en)] : (kind == kAdjJoint ? 3 : 1);
+            int en[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                en[q] = x0 + q < e ? adj[x0 + q] : -1;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (x0 + q < e)
+                {
+                    const int kind = adjKind(en[q]);
+                    s += kind == kAdjManifold ? mcount[adjIndex(en[q])] : (kind == kAdjJoint ? 3 : 1);
+                }
         }
         stampCount[i] = s;
     }
@@ -2461,20 +2470,41 @@ __global__ void k_stamp_fill(const int *deg, const int *adjStart, const int *adj
         return;
     int o = stampStart[i];
     int b = adjStart[i], e = b + deg[i];
-    for (int x = b; x < e; ++x)
+    // force entries in batches of 8: the entries and partners, then their manifolds' contact
+    // counts and body pairs, are in flight together; stamps are written in force order
+    for (int x0 = b; x0 < e; x0 += 8)
     {
-        int en = adj[x];
-        int kind = adjKind(en), idx = adjIndex(en);
-        int parts = kind == kAdjManifold ? mcount[idx] : (kind == kAdjJoint ? 3 : 1);
-        int isA = kind == kAdjManifold ? (mab[idx].x == i ? 1 : 0) : 0;
-        int partner = adjPartner[x];
-        for (int p = 0; p < parts; ++p, ++o)
+        int en[8], partner[8], parts[8], isA[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
         {
-            if (o < maxStamps)
-                stamps[o] = make_int4(en, p | (isA << 16), partner,
-                                      kind == kAdjManifold ? idx * kMaxContacts + p : -1);
-            else
-                atomicOr(overflow, 64);
+            en[q] = x0 + q < e ? adj[x0 + q] : -1;
+            partner[q] = x0 + q < e ? adjPartner[x0 + q] : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+        {
+            parts[q] = 0;
+            isA[q] = 0;
+            if (x0 + q < e)
+            {
+                const int kind = adjKind(en[q]), idx = adjIndex(en[q]);
+                parts[q] = kind == kAdjManifold ? mcount[idx] : (kind == kAdjJoint ? 3 : 1);
+                isA[q] = kind == kAdjManifold ? (mab[idx].x == i ? 1 : 0) : 0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+        {
+            const int kind = adjKind(en[q]), idx = adjIndex(en[q]);
+            for (int p = 0; p < parts[q]; ++p, ++o)
+            {
+                if (o < maxStamps)
+                    stamps[o] = make_int4(en[q], p | (isA[q] << 16), partner[q],
+                                          kind == kAdjManifold ? idx * kMaxContacts + p : -1);
+                else
+                    atomicOr(overflow, 64);
+            }
         }
     }
 }
