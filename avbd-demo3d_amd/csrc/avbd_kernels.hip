@@ -82,6 +82,7 @@ __device__ __forceinline__ void block_add2(int a, int *da, int b, int *db)
         if (sb && db)
             atomicAdd(db, sb);
     }
+    __syncthreads();  // part[] may be reused by a later call in the same kernel
 }
 
 __device__ __forceinline__ void block_add(int a, int *da) { block_add2(a, da, 0, nullptr); }
@@ -104,6 +105,7 @@ __device__ __forceinline__ void block_max(int v, int *dst, int floor)
         if (m > floor)
             atomicMax(dst, m);
     }
+    __syncthreads();  // part[] may be reused by a later call in the same kernel
 }
 
 // Exclusive scan over the block; lds needs blockDim/64 ints.
@@ -1899,7 +1901,14 @@ __global__ __launch_bounds__(64) void k_color_worlds(const int *worldStart, cons
     short *part = dg + cap;
     const int lane = threadIdx.x;
     const int ws = worldStart[blockIdx.x], n = worldStart[blockIdx.x + 1] - ws;
-    if (n <= 0 || n > cap)
+    if (n > cap)
+    {
+        // cap is the batch's largest world, so this cannot happen; report it rather than skip
+        if (lane == 0)
+            atomicOr(&ctr->overflow, 8);
+        return;
+    }
+    if (n <= 0)
         return;
     const int pb = adjStart[ws];
     for (int l = lane; l < n; l += 64)
