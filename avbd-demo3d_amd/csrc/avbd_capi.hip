@@ -354,7 +354,8 @@ int check_device_errors(avbd_handle *h)
                            : (c.overflow & 8) ? "colour count (more than 254 colours)"
                            : (c.overflow & 16) ? "constrained-pair hash"
                            : (c.overflow & 64) ? "primal stamp buffer"
-                                               : "sweep work-item buffer";
+                           : (c.overflow & 128) ? "per-world LDS (a world larger than the batch's largest)"
+                                                : "sweep work-item buffer";
         return fail((c.overflow & 8) ? AVBD_ERR_UNSUPPORTED : AVBD_ERR_CAPACITY,
                     std::string("capacity exceeded: ") + what);
     }

@@ -1905,7 +1905,7 @@ __global__ __launch_bounds__(64) void k_color_worlds(const int *worldStart, cons
     {
         // cap is the batch's largest world, so this cannot happen; report it rather than skip
         if (lane == 0)
-            atomicOr(&ctr->overflow, 8);
+            atomicOr(&ctr->overflow, 128);
         return;
     }
     if (n <= 0)
