@@ -693,9 +693,15 @@ __global__ __launch_bounds__(256) void k_world_sap(const float4 *pose, const flo
     const int w = blockIdx.x, t = threadIdx.x;
     const int ws = worldStart[w], n = worldStart[w + 1] - ws;
     double a[6] = {0, 0, 0, 0, 0, 0};
+    // the centres are also staged in LDS (x in lo_s, y in hi_s, z over key, all free until the
+    // sort) so that an exact serial pass below reads LDS instead of making n dependent loads
+    float *zs = (float *)key;
     for (int l = t; l < n; l += 256)
     {
         float4 p = pose[4 * (ws + l) + P_POS];
+        lo_s[l] = p.x;
+        hi_s[l] = p.y;
+        zs[l] = p.z;
         a[0] += p.x;
         a[1] += p.y;
         a[2] += p.z;
@@ -735,12 +741,12 @@ __global__ __launch_bounds__(256) void k_world_sap(const float4 *pose, const flo
         {
             V3 mean{0.0f, 0.0f, 0.0f};
             for (int l = 0; l < n; ++l)
-                mean += ld3(pose[4 * (ws + l) + P_POS]);
+                mean += V3{lo_s[l], hi_s[l], zs[l]};
             mean = mean / (float)n;
             V3 var{0.0f, 0.0f, 0.0f};
             for (int l = 0; l < n; ++l)
             {
-                V3 d = ld3(pose[4 * (ws + l) + P_POS]) - mean;
+                V3 d = V3{lo_s[l], hi_s[l], zs[l]} - mean;
                 var += V3{d.x * d.x, d.y * d.y, d.z * d.z};
             }
             best = 0;
