@@ -6,8 +6,11 @@ GPU is missing, construction raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -18,6 +21,16 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libavbd_hip.so")
 
 _p = C.c_void_p
 _LIB = None
+# Every open HipWorld, closed by an exit hook while the HIP runtime is still whole: a handle
+# left to the garbage collector during interpreter shutdown would otherwise free device memory
+# and destroy its stream after the runtime (or a profiler attached to it) has begun to tear down.
+_LIVE = weakref.WeakSet()
+
+
+def close_all():
+    """Close every open HipWorld (installed as an exit hook by load())."""
+    for w in list(_LIVE):
+        w.close()
 
 AVBD_OK = 0
 ERRORS = {-1: "invalid argument", -2: "capacity", -3: "hip", -4: "no device", -5: "unsupported", -6: "injected"}
@@ -76,6 +89,7 @@ def load(path: str | None = None):
         lib.avbd_adjoint_backward.argtypes = [_p] + [_p] * 10
         check_layouts(lib.avbd_sizeof)
         _LIB = lib
+        atexit.register(close_all)
     return _LIB
 
 
@@ -142,6 +156,7 @@ class HipWorld:
         h = _p()
         _check(lib.avbd_create(cfg.ctypes.data_as(_p), C.byref(h)))
         self._h = h
+        _LIVE.add(self)
 
     @classmethod
     def for_state(cls, state: State, params, **kw) -> "HipWorld":
@@ -221,8 +236,13 @@ class HipWorld:
         if getattr(self, "_h", None):
             self._lib.avbd_destroy(self._h)
             self._h = None
+        _LIVE.discard(self)
 
     def __del__(self):
+        # during interpreter shutdown the exit hook has already closed every handle; a handle
+        # still open here is left to the process exit rather than torn down out of order
+        if sys.is_finalizing():
+            return
         self.close()
 
     def set_params(self, params):

@@ -50,6 +50,7 @@ struct HipAvbdBackend : PhysicsBackend
     avbd_handle *h = nullptr;
     int capBodies = 0, capJoints = 0, capSprings = 0, capManifolds = 0;
     bool broken = false;
+    bool warnedMode = false;
 
     explicit HipAvbdBackend(int dev) : device(dev) {}
     ~HipAvbdBackend() override
@@ -86,10 +87,39 @@ struct HipAvbdBackend : PhysicsBackend
         solver.stepCpuReference();
     }
 
+    // The GPU step reproduces the sweep-and-prune broadphase (solver.cpp:488-577). Any other
+    // pair source the Solver is set to -- the uniform grid or all pairs (solver.cpp:1077-1082),
+    // external pairs or contacts (solver.h:336-343, 364-365) or the GPU-narrowphase pair sink
+    // (solver.h:379-383) -- gives a different pair order or contact set, so those steps run on the
+    // CPU reference, counted as fallbacks, as the reference's own GPU lane does for steps it cannot
+    // take (avbd_gpu_solver.cpp:2152-2170).
+    const char *unsupported(const Solver &solver) const
+    {
+        if (solver.useExternalBroadphasePairs)
+            return "external broadphase pairs";
+        if (solver.useExternalManifoldContacts)
+            return "external manifold contacts";
+        if (solver.spherePairSink)
+            return "a GPU-narrowphase pair sink";
+        if (solver.broadphaseMode != BROADPHASE_SWEEP_AND_PRUNE)
+            return solver.broadphaseMode == BROADPHASE_UNIFORM_GRID ? "the uniform-grid broadphase"
+                                                                     : "the all-pairs broadphase";
+        return nullptr;
+    }
+
     void step(Solver &solver) override
     {
         if (broken)
         {
+            g_stats.cpuIterateFallbacks++;
+            solver.stepCpuReference();
+            return;
+        }
+        if (const char *why = unsupported(solver))
+        {
+            if (!warnedMode)
+                std::fprintf(stderr, "hip-avbd: the Solver is set to %s; stepping the CPU reference\n", why);
+            warnedMode = true;
             g_stats.cpuIterateFallbacks++;
             solver.stepCpuReference();
             return;

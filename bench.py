@@ -16,8 +16,12 @@ Printed JSON line:
                average HIP-event duration, against 8 TB/s HBM; the whole
                iteration loop and k_dual are reported beside it
   cpu_baseline the reference CPU solver (oracle/_ref, its own WorkerPool) on a
-               bounded sample of the same workload: the GPU's settled state
-               after the timed region, stepped for ~15 s of CPU time
+               bounded sample of the same workload: the settled state the timed
+               window starts from, stepped for ~25 s of CPU time
+
+The pile is first stepped --settle untimed steps (default 900) so that the
+timed window is the settled pile (about 240k contacts, 7 colours) and not the
+free fall of the first steps; the run fails if the timed window has no colours.
 """
 import argparse
 import json
@@ -50,7 +54,7 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(state, params, target_s):
+def cpu_baseline(state, params, target_s, first_step):
     import oracle.bindings as ob
     n_dyn = state.num_dynamic
     if ob.ref_available():
@@ -70,9 +74,9 @@ def cpu_baseline(state, params, target_s):
     n = max(1, min(200, int(target_s / max(t1, 1e-6))))
     t = run(n)
     return {"value": n_dyn * n / t, "unit": "body-steps/s", "cores": cores, "kind": kind,
-            "sample": f"{what}: {n} steps of the {len(state.bodies)}-body pile from the GPU state after the "
-                      f"timed region ({t:.1f} s; broadphase/force init on {cores} threads, primal/dual loops "
-                      f"single-threaded as in the reference)"}
+            "sample": f"{what}: {n} steps of the {len(state.bodies)}-body pile from the settled state the GPU's "
+                      f"timed window starts from (step {first_step}; {t:.1f} s; broadphase/force init on {cores} "
+                      f"threads, primal/dual loops single-threaded as in the reference)"}
 
 
 def calibrate_copy(mib):
@@ -94,7 +98,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--bodies", type=int, default=50000)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--settle", type=int, default=900,
+                    help="untimed steps before the warm-up, so that the timed window is the settled pile")
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
@@ -138,6 +144,17 @@ def main():
     world = capi.HipWorld.for_state(state, params, timing=True)
     n_dyn = state.num_dynamic
 
+    t_settle = time.perf_counter()
+    done = 0
+    while done < args.settle:
+        k = min(100, args.settle - done)
+        world.step(k)
+        done += k
+    world.synchronize()
+    st = world.stats()
+    log(f"settled {args.settle} untimed steps in {time.perf_counter() - t_settle:.2f} s: "
+        f"{int(st['manifold_count'])} manifolds, {int(st['contact_count'])} contacts, "
+        f"{int(st['color_count'])} colours")
     log(f"warmup {args.warmup} steps")
     world.step(args.warmup)
     world.synchronize()
@@ -150,8 +167,12 @@ def main():
         world.step(2)
         world.synchronize()
     torch.cuda.synchronize()
+    # the CPU baseline steps this same state (downloaded outside the timed region)
+    start_state = None if args.no_cpu_baseline else world.state()
     world.reset_stats()
-    log(f"timing {args.steps} steps")
+    # the autotuner restores the state it started from; two steps follow it
+    first = args.settle + args.warmup + (0 if args.no_tune else 2) + 1
+    log(f"timing {args.steps} steps ({first}-{first + args.steps - 1})")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     done = 0
@@ -169,6 +190,10 @@ def main():
         f"({int(st['coloring_rounds'])} rounds), {int(st['sphere_hits'])} hits, {int(st['pair_checks'])} checks; "
         f"phase ms: broadphase {st['broadphase_ms']:.3f} init {st['narrowphase_ms']:.3f} "
         f"colour {st['coloring_ms']:.3f} iterate {st['iterate_ms']:.3f} finish {st['finish_ms']:.3f}")
+
+    if int(st["color_count"]) == 0 or int(st["contact_count"]) == 0:
+        raise SystemExit(f"bench: the timed window ({first}-{first + args.steps - 1}) has no coloured bodies "
+                         f"({int(st['contact_count'])} contacts): the pile is not in contact; raise --settle")
 
     steps_counted = max(int(st["counted_steps"]), 1)
     iter_ms = float(st["iterate_ms_total"]) / max(int(st["timed_steps"]), 1)
@@ -207,8 +232,14 @@ def main():
         "config": {
             "workload": f"pile{args.bodies}: {args.bodies} boxes (half-extents U[0.25,1] m, random orientations, "
                         f"v~N(0,0.5) m/s) on a jittered 50x40x25 grid dropping onto ground inside 4 walls; "
-                        f"dt=1/60, 10 iterations, friction 0.5, gravity -9.81; steps {args.warmup + 1}-"
-                        f"{args.warmup + args.steps} timed",
+                        f"dt=1/60, 10 iterations, friction 0.5, gravity -9.81; {args.settle} untimed settling steps and "
+                        f"{args.warmup} warm-up steps, then steps {first}-{first + args.steps - 1} timed "
+                        f"(last timed step: {int(st['manifold_count'])} manifolds, {int(st['contact_count'])} "
+                        f"contacts, {int(st['color_count'])} colours)",
+            "timed_steps": [first, first + args.steps - 1],
+            "manifolds": int(st["manifold_count"]),
+            "contacts": int(st["contact_count"]),
+            "colours": int(st["color_count"]),
             "bodies": int(n_dyn),
             "iterations": int(params["iterations"]),
             "parallelism": "single GPU",
@@ -237,8 +268,7 @@ def main():
     }
     if not args.no_cpu_baseline:
         log("CPU baseline")
-        final = world.state()
-        result["cpu_baseline"] = cpu_baseline(final, params, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(start_state, params, args.cpu_seconds, first)
     world.close()
     torch.cuda.synchronize()
     if args.batch_worlds > 0:

@@ -6,9 +6,11 @@ CPU-baseline leg of bench.py. The product path never imports this module.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
 import sys
+import weakref
 
 import numpy as np
 
@@ -38,6 +40,14 @@ def ref_path() -> str:
 
 _ORC = None
 _REF = None
+# Reference solvers that may hold the hip-avbd backend (a device handle): destroyed by an exit
+# hook while the HIP runtime is whole, never by the garbage collector during interpreter shutdown.
+_REF_LIVE = weakref.WeakSet()
+
+
+def _close_refs():
+    for w in list(_REF_LIVE):
+        w.close()
 
 
 def load_oracle():
@@ -111,7 +121,10 @@ def load_ref():
         lib.ref_collide_pair.argtypes = [_p, _p, C.c_float, _p, _p]
         lib.ref_joint_spring_derived.argtypes = [_p, _p, _p, _p, _p, _p]
         lib.ref_use_hip.argtypes = [_p, _i]
+        lib.ref_set_broadphase.argtypes = [_p, _i]
+        lib.ref_set_broadphase.restype = _i
         _REF = lib
+        atexit.register(_close_refs)
     return _REF
 
 
@@ -373,11 +386,25 @@ class RefWorld:
             ig = np.ascontiguousarray(state.ignores, np.int32)
             if len(ig):
                 lib.ref_add_ignores(self._h, _ptr(ig), len(ig))
+        _REF_LIVE.add(self)
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "_h", None):
             self._lib.ref_destroy(self._h)
             self._h = None
+        _REF_LIVE.discard(self)
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        self.close()
+
+    BROADPHASE_ALL_PAIRS, BROADPHASE_GRID, BROADPHASE_SAP, BROADPHASE_DEFAULT = 0, 1, 2, -1
+
+    def set_broadphase(self, mode: int) -> int:
+        """Solver::broadphaseMode (0 all pairs, 1 uniform grid, 2 SAP; -1 the default-constructed
+        Solver's mode). Returns the mode now set."""
+        return self._lib.ref_set_broadphase(self._h, int(mode))
 
     def use_hip(self, device: int = 0) -> bool:
         """Install the hip-avbd PhysicsBackend adapter (Solver::step then runs on the GPU)."""

@@ -520,7 +520,23 @@ int ref_get_colors(void *h, int32_t *out, int cap)
 
 int ref_color_count(void *h) { return ((RefWorld *)h)->colorCount; }
 
-/* SAP pair emission of the current state: a clone runs
+/* Solver::broadphaseMode (solver.h:39-45): 0 all pairs, 1 uniform grid, 2 sweep and prune (the
+ * harness default), or -1 for the mode a default-constructed Solver has (solver.cpp:676-680).
+ * Returns the mode now set. */
+int ref_set_broadphase(void *h, int mode)
+{
+    RefWorld *w = (RefWorld *)h;
+    if (mode < 0)
+    {
+        Solver fresh;
+        w->solver.broadphaseMode = fresh.broadphaseMode;
+    }
+    else if (mode < BROADPHASE_COUNT)
+        w->solver.broadphaseMode = (BroadphaseMode)mode;
+    return (int)w->solver.broadphaseMode;
+}
+
+/* Pair emission of the current state in the world's broadphase mode: a clone runs
  * Solver::benchmarkBroadphaseOnly (solver.cpp:1010-1034), which drops the
  * manifolds and creates one per unconstrained bounding-sphere hit in sweep
  * order. Returns the number of (a, b) pairs written (list indices). */
@@ -537,6 +553,7 @@ int ref_broadphase_pairs(void *h, int32_t *out, int cap, int32_t *checks, int32_
     avbd_params p{w->solver.dt, w->solver.gravity, w->solver.iterations, w->solver.alpha, w->solver.betaLin,
                   w->solver.betaAng, w->solver.gamma};
     RefWorld *c = fromState(&p, b.data(), nb, j.data(), nj, s.data(), ns, nullptr, 0);
+    c->solver.broadphaseMode = w->solver.broadphaseMode;
     c->solver.benchmarkBroadphaseOnly();
     if (checks)
         *checks = c->solver.stats.pairChecks;

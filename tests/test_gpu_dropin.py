@@ -43,3 +43,23 @@ def test_hip_backend_steps_like_reference(ref, name):
         done = s
         assert_states_equal(gpu.state(), cpu.state(), f"{name} step {s}")
     assert ref.load_ref().ref_hip_fallbacks() == 0
+
+
+def test_hip_backend_all_pairs_falls_back(ref):
+    """A Solver set to a broadphase the GPU step does not reproduce (here all pairs,
+    solver.cpp:165-176) steps the CPU reference through the adapter, counted as fallbacks,
+    so its trajectory is the CPU backend's own."""
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scenes.pile(200, seed=13)
+    prm = default_params(gravity=sc.gravity)
+    gpu = ref.RefWorld(prm, scene=sc)
+    cpu = ref.RefWorld(prm, scene=sc)
+    for w in (gpu, cpu):
+        assert w.set_broadphase(ref.RefWorld.BROADPHASE_ALL_PAIRS) == 0
+    assert gpu.use_hip(0)
+    before = ref.load_ref().ref_hip_fallbacks()
+    gpu.step(30, colored=False)
+    cpu.step(30, colored=False)   # Solver::step with the CPU backend, list order
+    assert_states_equal(gpu.state(), cpu.state(), "all-pairs solver through hip-avbd")
+    assert ref.load_ref().ref_hip_fallbacks() - before == 30
