@@ -75,6 +75,8 @@ def load(path: str | None = None):
         lib.avbd_snapshot_avbs_size.argtypes = [_p, C.c_int32]
         lib.avbd_snapshot_avbs.restype = C.c_int64
         lib.avbd_snapshot_avbs.argtypes = [_p, C.c_uint64, C.c_char_p, _p, C.c_int64]
+        lib.avbd_snapshot_avbs_static.restype = C.c_int64
+        lib.avbd_snapshot_avbs_static.argtypes = [_p, C.c_uint64, C.c_char_p, _p, C.c_int64]
         lib.avbd_make_joint.argtypes = [_p, _p, C.c_int32, C.c_int32, C.c_int64, _p, _p, C.c_float, C.c_float,
                                         C.c_float, _p]
         lib.avbd_make_spring.argtypes = [_p, _p, C.c_int32, C.c_int32, C.c_int64, _p, _p, C.c_float, C.c_float, _p]

@@ -13,10 +13,19 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace avbd;
+
+// Staging of the allocation-free snapshot (avbd_snapshot_avbs_static): the 64-byte AVBS body
+// records of at most kStaticSnapshotBodies bodies (4 MiB). A file-scope device array, so each GPU
+// holds its own copy, at its own address, from the module's load until the process ends. In host
+// code `scratch` names a host-side stand-in, never the device memory: host code uses only the
+// address hipGetSymbolAddress returns for the current device (g_scratchPtr, bound in avbd_create).
+constexpr int kStaticSnapshotBodies = 65536;
+static __device__ __align__(256) uint4 scratch[4 * kStaticSnapshotBodies];
 
 namespace
 {
@@ -38,6 +47,31 @@ int fail(int code, const std::string &msg)
     } while (0)
 
 int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Per device ordinal: the device address of that GPU's copy of `scratch` (null until bound).
+constexpr int kMaxStaticDevices = 64;
+std::mutex g_scratchMutex;
+uint4 *g_scratchPtr[kMaxStaticDevices] = {};
+
+// With `device` current: looks up the address of its copy of `scratch` once and keeps it.
+int bind_static_scratch(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (device < 0 || device >= kMaxStaticDevices || g_scratchPtr[device])
+        return AVBD_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(AVBD_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratchPtr[device] = (uint4 *)p;
+    return AVBD_OK;
+}
+
+uint4 *static_scratch(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    return device >= 0 && device < kMaxStaticDevices ? g_scratchPtr[device] : nullptr;
+}
 
 } // namespace
 
@@ -981,7 +1015,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         avbd_destroy(h);
         return code;
     };
-    if ((r = pick_device(h)))
+    if ((r = pick_device(h)) || (r = bind_static_scratch(h->device)))
         return bail(r);
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess)
@@ -1747,6 +1781,54 @@ int64_t avbd_snapshot_avbs(avbd_handle *h, uint64_t frame, const char *scene, vo
     {
         k_avbs_pack<<<ceil_div(h->nb, 256), 256, 0, h->stream>>>(h->B, h->avbsRank, h->nb, h->avbsBuf);
         HIP_TRY(hipMemcpyAsync(o + AVBD_AVBS_HEADER_BYTES + sb, h->avbsBuf, 64 * (size_t)h->nb, hipMemcpyDeviceToHost,
+                               h->stream));
+    }
+    return need;
+}
+
+int64_t avbd_snapshot_avbs_static(avbd_handle *h, uint64_t frame, const char *scene, void *out, int64_t capacity)
+{
+    if (!h || !out)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_snapshot_avbs_static: null argument");
+    size_t sb = scene ? std::strlen(scene) : 0;
+    if (sb > 0xFFFFFFFFu)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_snapshot_avbs_static: scene name too long");
+    if (h->nb > kStaticSnapshotBodies)
+        return fail(AVBD_ERR_CAPACITY, "avbd_snapshot_avbs_static: " + std::to_string(h->nb) +
+                                           " bodies, more than the static staging holds (" +
+                                           std::to_string(kStaticSnapshotBodies) + "); use avbd_snapshot_avbs");
+    int64_t need = AVBD_AVBS_HEADER_BYTES + (int64_t)sb + 64 * (int64_t)h->nb;
+    if (capacity < need)
+        return fail(AVBD_ERR_CAPACITY, "avbd_snapshot_avbs_static: output buffer smaller than avbd_snapshot_avbs_size");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    uint4 *stage = static_scratch(h->device);
+    if (!stage)
+        return fail(AVBD_ERR_UNSUPPORTED, "avbd_snapshot_avbs_static: no static staging on device ordinal " +
+                                              std::to_string(h->device));
+    // the copy out of the staging must not need a runtime staging buffer of its own: pinned host memory
+    hipPointerAttribute_t attr{};
+    hipError_t e = hipPointerGetAttributes(&attr, out);
+    if (e != hipSuccess)
+        (void)hipGetLastError();
+    if (e != hipSuccess || attr.type != hipMemoryTypeHost)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_snapshot_avbs_static: out must be pinned host memory (hipHostMalloc)");
+    unsigned char *o = (unsigned char *)out;
+    uint32_t magic = AVBD_AVBS_MAGIC, version = 1, sceneBytes = (uint32_t)sb, bodies = (uint32_t)h->nb;
+    std::memcpy(o, &magic, 4);
+    std::memcpy(o + 4, &version, 4);
+    std::memcpy(o + 8, &frame, 8);
+    std::memcpy(o + 16, &sceneBytes, 4);
+    std::memcpy(o + 20, &bodies, 4);
+    if (sb)
+        std::memcpy(o + AVBD_AVBS_HEADER_BYTES, scene, sb);
+    if (h->nb > 0)
+    {
+        // every record in [0, nb) is written (the ranks are a permutation), so nothing is cleared
+        k_avbs_pack<<<ceil_div(h->nb, 256), 256, 0, h->stream>>>(h->B, h->avbsRank, h->nb, stage);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(o + AVBD_AVBS_HEADER_BYTES + sb, stage, 64 * (size_t)h->nb, hipMemcpyDeviceToHost,
                                h->stream));
     }
     return need;

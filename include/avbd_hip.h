@@ -313,6 +313,16 @@ int64_t avbd_snapshot_avbs_size(avbd_handle *h, int32_t scene_bytes);
  * (avbd_synchronize). Pinned out (hipHostMalloc) makes the copy asynchronous,
  * so a viewer can stream frames while the next steps run. scene may be NULL. */
 int64_t avbd_snapshot_avbs(avbd_handle *h, uint64_t frame, const char *scene, void *out, int64_t capacity);
+/* Allocation-free form of avbd_snapshot_avbs. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving or streaming
+ * loops, harnesses that forbid hipMalloc). avbd_snapshot_avbs allocates its device staging on its
+ * first call; this form stages the packed records in a fixed 4 MiB device array of the library
+ * instead (one per GPU, held until the process ends) and calls no allocating routine. Limits: the
+ * handle holds at most 65536 bodies (more: AVBD_ERR_CAPACITY), and out must be pinned host memory
+ * (hipHostMalloc; anything else: AVBD_ERR_INVALID_ARGUMENT). Output and return value are those of
+ * avbd_snapshot_avbs. All handles on one GPU share the staging array: calls that overlap on one GPU
+ * (different handles or threads, before the earlier call's stream has passed its copy) race. */
+int64_t avbd_snapshot_avbs_static(avbd_handle *h, uint64_t frame, const char *scene, void *out, int64_t capacity);
 
 /* ---- Batched worlds ----
  * Many independent solvers in one handle. The reference runs one Solver per
