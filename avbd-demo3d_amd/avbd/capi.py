@@ -240,10 +240,10 @@ class HipWorld:
             self._h = None
         _LIVE.discard(self)
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # during interpreter shutdown the exit hook has already closed every handle; a handle
         # still open here is left to the process exit rather than torn down out of order
-        if sys.is_finalizing():
+        if _finalizing():
             return
         self.close()
 

@@ -56,7 +56,8 @@ STATS = np.dtype(
      ("iterate_bytes_total", "<f8"), ("step_bytes_total", "<f8"), ("iterate_ms_total", "<f8"),
      ("step_ms_total", "<f8"), ("counted_steps", "<i8"), ("timed_steps", "<i8"),
      ("primal_bytes_total", "<f8"), ("dual_bytes_total", "<f8"), ("primal_ms_total", "<f8"),
-     ("dual_ms_total", "<f8"), ("primal_launches", "<i8"), ("dual_launches", "<i8")], align=True)
+     ("dual_ms_total", "<f8"), ("primal_launches", "<i8"), ("dual_launches", "<i8"),
+     ("primal_iterations", "<i8")], align=True)
 
 CONFIG = np.dtype(
     [("params", PARAMS), ("max_bodies", "<i4"), ("max_joints", "<i4"), ("max_springs", "<i4"),
@@ -189,11 +190,13 @@ LAUNCH = np.dtype([("primal_lanes", "<i4"), ("primal_threads", "<i4"), ("dual_th
                    ("use_graph", "<i4"), ("dataflow", "<i4"), ("lazy_dual", "<i4")], align=True)
 
 
-def check_layouts(sizeof, records: int = 7) -> None:
+def check_layouts(sizeof, records: int = 9) -> None:
     """Compare dtype sizes with a library's sizeof(which) for which in [0, records)
-    (0 body, 1 contact, 2 manifold, 3 joint, 4 spring, 5 params, 6 launch config)."""
-    names = ["body", "contact", "manifold", "joint", "spring", "params", "launch config"][:records]
-    dts = [BODY, CONTACT, MANIFOLD, JOINT, SPRING, PARAMS, LAUNCH][:records]
+    (0 body, 1 contact, 2 manifold, 3 joint, 4 spring, 5 params, 6 launch config, 7 stats,
+    8 config)."""
+    names = ["body", "contact", "manifold", "joint", "spring", "params", "launch config", "stats",
+             "config"][:records]
+    dts = [BODY, CONTACT, MANIFOLD, JOINT, SPRING, PARAMS, LAUNCH, STATS, CONFIG][:records]
     for i, (n, dt) in enumerate(zip(names, dts)):
         got = sizeof(i)
         if got != dt.itemsize:

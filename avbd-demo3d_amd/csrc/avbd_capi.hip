@@ -121,6 +121,7 @@ struct avbd_handle
     std::vector<hipEvent_t> ring;
     std::vector<int> ringIters;                    // iterations marked in each ring step
     std::vector<int> ringPrimalLaunches;           // primal launches in each ring step
+    std::vector<int> ringPrimalIters;              // solver iterations those launches cover
     long long primalLaunches = 0, dualLaunches = 0;
     long long ringSteps = 0;   // steps recorded into the ring since the last reset
     avbd_stats stats{};
@@ -134,9 +135,9 @@ struct avbd_handle
     int npPolyCap = kNpPolyCap;  // clip-polygon vertices the box-only narrowphase keeps in LDS
     // launch shapes of the iteration kernels, and the iteration graph built from them
     avbd_launch_config launch{16, 256, 256, 0, 0, 1};
-    int *dfDone = nullptr;  // dataflow primal: last iteration id each body finished
-    int4 *dfTasks = nullptr;  // colour-sorted tasks, each colour padded to kDataflowPad
-    int dfIter = 0;
+    int *dfDone = nullptr;  // flow primal: iterations each solved body has finished in this step
+    int numCUs = 0;
+    int flowBlocks[3][3] = {};  // flow primal grid per (lanes, threads) shape: every wave resident
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
     std::vector<hipGraphNode_t> graphPrimal;
@@ -214,16 +215,14 @@ PrimalKernel primal_kernel(int lanes, int threads)
     return nullptr;
 }
 
-using DataflowKernel = void (*)(Bodies, Manifolds, const int4 *, const int4 *, int, const int *, int *, int, Params,
-                                Counters *);
+using FlowKernel = void (*)(Bodies, Manifolds, const int4 *, const int *, int, const int *, int *, Params,
+                            const int4 *, int, Counters *);
 
-PrimalKernel primal_kernel(int lanes, int threads);
-
-DataflowKernel dataflow_kernel(int lanes, int threads)
+FlowKernel flow_kernel(int lanes, int threads)
 {
 #define AVBD_DATAFLOW_SHAPE(l, t)                                                                          \
     if (lanes == l && threads == t)                                                                        \
-        return &k_primal_dataflow<l, t>;
+        return &k_primal_flow<l, t>;
     AVBD_DATAFLOW_SHAPE(8, 64)
     AVBD_DATAFLOW_SHAPE(8, 128)
     AVBD_DATAFLOW_SHAPE(8, 256)
@@ -276,11 +275,9 @@ struct DualArgs
     int nc;
     Params P;
     unsigned char *flagTape;
-    unsigned *claimReset;
     int staticRepeat;
-    void *ptr[10];
-    DualArgs(const avbd_handle *h, int nc_, unsigned char *flags = nullptr, unsigned *claim = nullptr, int nj_ = -1,
-             int staticRepeat_ = 1);
+    void *ptr[9];
+    DualArgs(const avbd_handle *h, int nc_, unsigned char *flags = nullptr, int nj_ = -1, int staticRepeat_ = 1);
 };
 
 // Colouring runs as one launch of a few large blocks (at most 64, one per compute unit, every
@@ -536,7 +533,7 @@ int enqueue_prepare(avbd_handle *h)
         k_radix_scatter<<<tiles, kRadixThreads, 0, s>>>(keysAlt, valsAlt, N, 8, h->radixHistScan, tiles, keys, S.colorBodies,
                                               0);
         k_color_offsets<<<1, 256, 0, s>>>(h->radixHistScan, tiles, N, S.colorOffsets);
-        k_tasks<<<ceil_div(N, 256), 256, 0, s>>>(S.colorBodies, S.stampStart, S.stampCount, N, S.tasks);
+        k_tasks<<<ceil_div(N, 256), 256, 0, s>>>(S.colorBodies, S.stampStart, S.stampCount, N, S.tasks, h->dfDone);
     }
     if (h->timing)
     {
@@ -556,11 +553,11 @@ PrimalArgs::PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys, unsigned
     std::memcpy(ptr, p, sizeof(p));
 }
 
-DualArgs::DualArgs(const avbd_handle *h, int nc_, unsigned char *flags, unsigned *claim, int nj_, int staticRepeat_)
+DualArgs::DualArgs(const avbd_handle *h, int nc_, unsigned char *flags, int nj_, int staticRepeat_)
     : B(h->B), M(h->cur), joints(h->joints), nj(nj_ < 0 ? h->nj : nj_), contactRec(h->S.contactRec), nc(nc_), P(h->P),
-      flagTape(flags), claimReset(claim), staticRepeat(staticRepeat_)
+      flagTape(flags), staticRepeat(staticRepeat_)
 {
-    void *p[10] = {&B, &M, &joints, &nj, &contactRec, &nc, &P, &flagTape, &claimReset, &staticRepeat};
+    void *p[9] = {&B, &M, &joints, &nj, &contactRec, &nc, &P, &flagTape, &staticRepeat};
     std::memcpy(ptr, p, sizeof(p));
 }
 
@@ -681,6 +678,11 @@ int enqueue_iterate(avbd_handle *h)
     };
     if (tape)
         tapeAt(0);
+    if (h->timing)
+    {
+        h->ringPrimalLaunches[h->ringSteps % avbd_handle::kRing] = primalPerIter;
+        h->ringPrimalIters[h->ringSteps % avbd_handle::kRing] = 1;
+    }
     if (lc.use_graph && !tape && h->P.iterations > 0 && (primalPerIter > 0 || dualLanes > 0))
     {
         int r = iterate_graph(h, ranges, c.contactCount, dualLanes);
@@ -705,16 +707,10 @@ int enqueue_iterate(avbd_handle *h)
         const void *pfn = (const void *)primal_kernel(lc.primal_lanes, lc.primal_threads);
         const int groups = lc.primal_threads / lc.primal_lanes;
         const int last = h->P.iterations - 1;
-        // dataflow form: one primal launch per iteration (contact-only worlds, not while taping)
-        const bool dataflow = lc.dataflow && h->nj == 0 && h->ns == 0 && !tape && primalPerIter > 0;
         const int numTasks = h->hostOffsets[colors];
-        int numPadded = 0;
-        for (int col = 0; col < colors; ++col)
-            numPadded += ceil_div(h->hostOffsets[col + 1] - h->hostOffsets[col], kDataflowPad) * kDataflowPad;
-        const int dfBlocks = std::max(1, std::min(ceil_div(numPadded, groups), 2048));
-        // task-ordered stamps at a power-of-two stride (direct per-colour launches, no adjoint tape)
+        // task-ordered stamps at a power-of-two stride (direct launches, no adjoint tape)
         int padShift = -1;
-        if (!dataflow && !tape && numTasks > 0 && c.maxStampCount <= kPadStampsMax)
+        if (!tape && numTasks > 0 && c.maxStampCount <= kPadStampsMax)
         {
             padShift = 0;
             while ((1 << padShift) < std::max(c.maxStampCount, lc.primal_lanes))
@@ -728,28 +724,48 @@ int enqueue_iterate(avbd_handle *h)
         const bool lazy = lc.lazy_dual && padShift >= 0 && c.contactCount > 0 && h->P.iterations > 0;
         if (lazy)
             k_lazy_bytes<<<1, 1, 0, s>>>(h->ctr, h->P.iterations);
-        if (dataflow)
+        // flow form: one primal launch for every iteration and colour (contact-only worlds with the
+        // lazy dual), its grid the number of blocks that are resident at once
+        int flowGrid = 0;
+        if (lc.dataflow && lazy && h->nj == 0 && h->ns == 0 && primalPerIter > 0)
         {
-            k_tasks_pad_clear<<<grid_for(numPadded), 256, 0, s>>>(h->dfTasks, numPadded);
-            k_tasks_padded_fill<<<ceil_div(numTasks, 256), 256, 0, s>>>(h->S.tasks, numTasks, h->S.colorOffsets,
-                                                                        colors, h->dfTasks);
-        }
-        for (int it = 0; it < h->P.iterations; ++it)
-        {
-            if (mark && it == last)
-                HIP_TRY(hipEventRecord(mark[4], s));
-            float4 *sys = tape ? h->tapeSys + (size_t)it * 9 * h->maxBodies : nullptr;
-            unsigned char *sfl = tape ? h->tapeStampFlags + (size_t)it * h->maxStamps : nullptr;
-            unsigned char *dfl = tape ? h->tapeDualFlags + (size_t)it * kMaxContacts * h->maxManifolds : nullptr;
-            if (dataflow)
+            int &fb = h->flowBlocks[lc.primal_lanes == 8 ? 0 : lc.primal_lanes == 16 ? 1 : 2]
+                                   [lc.primal_threads == 64 ? 0 : lc.primal_threads == 128 ? 1 : 2];
+            if (fb == 0)
             {
-                const int iter = ++h->dfIter;
-                dataflow_kernel(lc.primal_lanes, lc.primal_threads)<<<dfBlocks, lc.primal_threads, 0, s>>>(
-                    h->B, h->cur, h->S.stamps, h->dfTasks, numPadded, h->S.color, h->dfDone, iter, h->P, h->ctr);
-                if (dualLanes == 0)
-                    k_claim_reset<<<1, 1, 0, s>>>(h->ctr);
+                int per = 0;
+                HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per, (const void *)flow_kernel(lc.primal_lanes, lc.primal_threads), lc.primal_threads, 0));
+                fb = std::max(per, 0) * h->numCUs;
+                if (fb <= 0)
+                    fb = -1;  // does not fit: per-colour launches
             }
-            else
+            flowGrid = std::max(fb, 0);
+        }
+        if (flowGrid > 0)
+        {
+            if (mark)
+                HIP_TRY(hipEventRecord(mark[4], s));
+            flow_kernel(lc.primal_lanes, lc.primal_threads)<<<flowGrid, lc.primal_threads, 0, s>>>(
+                h->B, h->cur, h->S.tasks, h->S.colorOffsets, colors, h->S.color, h->dfDone, h->P, h->padStamps,
+                padShift, h->ctr);
+            HIP_TRY(hipGetLastError());
+            if (mark)
+                HIP_TRY(hipEventRecord(mark[5], s));
+            DualArgs a(h, c.contactCount, nullptr, 0, h->P.iterations);
+            HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(c.contactCount, lc.dual_threads)),
+                                    dim3(lc.dual_threads), a.ptr, 0, s));
+            if (mark)
+                HIP_TRY(hipEventRecord(mark[6], s));
+        }
+        else
+            for (int it = 0; it < h->P.iterations; ++it)
+            {
+                if (mark && it == last)
+                    HIP_TRY(hipEventRecord(mark[4], s));
+                float4 *sys = tape ? h->tapeSys + (size_t)it * 9 * h->maxBodies : nullptr;
+                unsigned char *sfl = tape ? h->tapeStampFlags + (size_t)it * h->maxStamps : nullptr;
+                unsigned char *dfl = tape ? h->tapeDualFlags + (size_t)it * kMaxContacts * h->maxManifolds : nullptr;
                 for (const auto &rg : ranges)
                 {
                     PrimalArgs a(h, rg.first, rg.second, sys, sfl, padShift >= 0 ? h->padStamps : nullptr,
@@ -757,33 +773,38 @@ int enqueue_iterate(avbd_handle *h)
                     HIP_TRY(hipLaunchKernel(pfn, dim3(ceil_div(rg.second - rg.first, groups)),
                                             dim3(lc.primal_threads), a.ptr, 0, s));
                 }
-            if (mark && it == last)
-                HIP_TRY(hipEventRecord(mark[5], s));
-            if (lazy)
-            {
-                if (h->nj > 0)
+                if (mark && it == last)
+                    HIP_TRY(hipEventRecord(mark[5], s));
+                if (lazy)
                 {
-                    DualArgs a(h, 0, nullptr, nullptr);
-                    HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(h->nj, lc.dual_threads)),
+                    if (h->nj > 0)
+                    {
+                        DualArgs a(h, 0, nullptr);
+                        HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(h->nj, lc.dual_threads)),
+                                                dim3(lc.dual_threads), a.ptr, 0, s));
+                    }
+                    if (it == last)
+                    {
+                        DualArgs a(h, c.contactCount, nullptr, 0, h->P.iterations);
+                        HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(c.contactCount, lc.dual_threads)),
+                                                dim3(lc.dual_threads), a.ptr, 0, s));
+                    }
+                }
+                else if (dualLanes > 0)
+                {
+                    DualArgs a(h, c.contactCount, dfl);
+                    HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(dualLanes, lc.dual_threads)),
                                             dim3(lc.dual_threads), a.ptr, 0, s));
                 }
-                if (it == last)
-                {
-                    DualArgs a(h, c.contactCount, nullptr, nullptr, 0, h->P.iterations);
-                    HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(c.contactCount, lc.dual_threads)),
-                                            dim3(lc.dual_threads), a.ptr, 0, s));
-                }
+                if (mark && it == last)
+                    HIP_TRY(hipEventRecord(mark[6], s));
+                if (tape)
+                    tapeAt(it + 1);
             }
-            else if (dualLanes > 0)
-            {
-                DualArgs a(h, c.contactCount, dfl, dataflow ? &h->ctr->dfClaim : nullptr);
-                HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(dualLanes, lc.dual_threads)),
-                                        dim3(lc.dual_threads), a.ptr, 0, s));
-            }
-            if (mark && it == last)
-                HIP_TRY(hipEventRecord(mark[6], s));
-            if (tape)
-                tapeAt(it + 1);
+        if (h->timing && flowGrid > 0)
+        {
+            h->ringPrimalLaunches[h->ringSteps % avbd_handle::kRing] = 1;
+            h->ringPrimalIters[h->ringSteps % avbd_handle::kRing] = h->P.iterations;
         }
     }
     if (h->timing)
@@ -791,7 +812,6 @@ int enqueue_iterate(avbd_handle *h)
         HIP_TRY(hipEventRecord(h->ev[4], s));
         HIP_TRY(hipEventRecord(mark[2], s));
         h->ringIters[h->ringSteps % avbd_handle::kRing] = h->P.iterations > 0 ? 1 : 0;
-        h->ringPrimalLaunches[h->ringSteps % avbd_handle::kRing] = primalPerIter;
     }
     return AVBD_OK;
 }
@@ -1053,8 +1073,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         (r = h->alloc(h->ctr, 1)) || (r = h->alloc(h->gb, 1)) ||
         (r = h->alloc(h->scanStatus, (size_t)h->scanMaxTiles)) || (r = h->alloc(h->scanCtl, 4)) ||
         (r = h->alloc(h->padStamps, (size_t)N * kPadStampsMax)) ||
-        (r = h->alloc(h->adjTotal, 1)) || (r = h->alloc(h->worldStart, (size_t)N + 1)) || (r = h->alloc(h->dfDone, N)) ||
-        (r = h->alloc(h->dfTasks, (size_t)N + (size_t)kDataflowPad * kMaxColors)))
+        (r = h->alloc(h->adjTotal, 1)) || (r = h->alloc(h->worldStart, (size_t)N + 1)) || (r = h->alloc(h->dfDone, N)))
         return bail(r);
     S.hashMask = h->hashSize - 1;
     if ((e = hipHostMalloc((void **)&h->hostCtr, sizeof(Counters))) != hipSuccess ||
@@ -1078,6 +1097,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     if (perCuCol < 1)
         return bail(fail(AVBD_ERR_HIP, "the colouring kernel does not fit on a compute unit"));
     h->coopColor = std::min(prop.multiProcessorCount, 64);
+    h->numCUs = prop.multiProcessorCount;
     for (int k = 0; k < 8; ++k)
         if ((e = hipEventCreate(&h->ev[k])) != hipSuccess)
             return bail(fail(AVBD_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e)));
@@ -1086,6 +1106,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         h->ring.assign((size_t)avbd_handle::kSlots * avbd_handle::kRing, nullptr);
         h->ringIters.assign(avbd_handle::kRing, 0);
         h->ringPrimalLaunches.assign(avbd_handle::kRing, 0);
+        h->ringPrimalIters.assign(avbd_handle::kRing, 0);
         for (hipEvent_t &ev : h->ring)
             if ((e = hipEventCreate(&ev)) != hipSuccess)
                 return bail(fail(AVBD_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e)));
@@ -1626,7 +1647,7 @@ int avbd_get_stats(avbd_handle *h, avbd_stats *out)
     {
         long long n = std::min<long long>(h->ringSteps, avbd_handle::kRing);
         double itTot = 0.0, stTot = 0.0, prTot = 0.0, duTot = 0.0;
-        long long prLaunch = 0, duLaunch = 0;
+        long long prLaunch = 0, duLaunch = 0, prIters = 0;
         for (long long k = h->ringSteps - n; k < h->ringSteps; ++k)
         {
             hipEvent_t *e = &h->ring[avbd_handle::kSlots * (k % avbd_handle::kRing)];
@@ -1646,6 +1667,8 @@ int avbd_get_stats(avbd_handle *h, avbd_stats *out)
                 duTot += d;
             }
             prLaunch += h->ringPrimalLaunches[k % avbd_handle::kRing];
+            if (iters > 0)
+                prIters += h->ringPrimalIters[k % avbd_handle::kRing];
             duLaunch += iters;
         }
         out->iterate_ms_total = itTot;
@@ -1654,6 +1677,7 @@ int avbd_get_stats(avbd_handle *h, avbd_stats *out)
         out->primal_ms_total = prTot;
         out->dual_ms_total = duTot;
         out->primal_launches = prLaunch;
+        out->primal_iterations = prIters;
         out->dual_launches = duLaunch;
     }
     if (c.barrierTimeout || c.overflow)
@@ -2057,9 +2081,33 @@ int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launc
             best = c;
         }
     }
+    // the flow form (one primal launch per step) for contact-only worlds, at the tuned shape and at
+    // two shapes that suit its resident grid
+    if (h->nj == 0 && h->ns == 0 && best.lazy_dual == 1)
+    {
+        const avbd_launch_config base = best;
+        const int flowShapes[3][2] = {{base.primal_lanes, base.primal_threads}, {16, 64}, {16, 128}};
+        for (int q = 0; q < 3; ++q)
+        {
+            if (q > 0 && flowShapes[q][0] == base.primal_lanes && flowShapes[q][1] == base.primal_threads)
+                continue;
+            avbd_launch_config c = base;
+            c.dataflow = 1;
+            c.primal_lanes = flowShapes[q][0];
+            c.primal_threads = flowShapes[q][1];
+            float ms = 0.0f;
+            if ((r = trial(c, ms)))
+                return r;
+            if (ms < bestMs)
+            {
+                bestMs = ms;
+                best = c;
+            }
+        }
+    }
     for (int g : {0, 1})
     {
-        if (g == 1 && !allowGraph)
+        if (g == 1 && (!allowGraph || best.dataflow))
             continue;
         avbd_launch_config c = best;
         c.use_graph = g;
@@ -2112,6 +2160,8 @@ int avbd_sizeof(int which)
     case 4: return (int)sizeof(avbd_spring);
     case 5: return (int)sizeof(avbd_params);
     case 6: return (int)sizeof(avbd_launch_config);
+    case 7: return (int)sizeof(avbd_stats);
+    case 8: return (int)sizeof(avbd_config);
     default: return -1;
     }
 }

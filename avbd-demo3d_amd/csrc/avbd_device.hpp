@@ -133,7 +133,6 @@ struct Counters
     double cumPrimalBytes;
     double cumDualBytes;
     unsigned int colorChanged[3];
-    unsigned int dfClaim; // next colour-sorted task for the dataflow primal kernel
     unsigned int pad_;
 };
 

@@ -500,7 +500,6 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 // Per-step counter reset (a kernel instead of a memset so the step can be graph-captured).
 __global__ void k_step_begin(Counters *c)
 {
-    c->dfClaim = 0u;
     c->numChunks = 0;
     c->numHits = 0;
     c->numCreated = 0;
@@ -2173,8 +2172,8 @@ __device__ __forceinline__ int contact_record(float *r, int i, int partner, bool
     int2 ab = isA ? make_int2(i, partner) : make_int2(partner, i);
     float4 *cr = M.cr + (size_t)sl * kCR;
     float4 rA4 = cr[CR_RA], rB4 = cr[CR_RB], c04 = cr[CR_C0], bb0 = cr[CR_N], bb1 = cr[CR_T1], bb2 = cr[CR_T2];
-    float4 pen4 = cr[CR_PEN], lam4 = cr[CR_LAM];
-    auto cur = [&](int b, int row) { return (Co && b == partner) ? ld_coherent(&B.pose[4 * b + row]) : B.pose[4 * b + row]; };
+    float4 pen4 = Co ? ld_coherent(&cr[CR_PEN]) : cr[CR_PEN], lam4 = Co ? ld_coherent(&cr[CR_LAM]) : cr[CR_LAM];
+    auto cur = [&](int b, int row) { return Co ? ld_coherent(&B.pose[4 * b + row]) : B.pose[4 * b + row]; };
     float4 pA4 = cur(ab.x, P_POS), pB4 = cur(ab.y, P_POS), qA4 = cur(ab.x, P_ROT), qB4 = cur(ab.y, P_ROT);
     float4 iA4 = B.pose[4 * (ab.x) + P_ILIN], iB4 = B.pose[4 * (ab.y) + P_ILIN], jA4 = B.pose[4 * (ab.x) + P_IANG], jB4 = B.pose[4 * (ab.y) + P_IANG];
     Q4 qA = ldq(qA4), qB = ldq(qB4);
@@ -2208,8 +2207,16 @@ __device__ __forceinline__ int contact_record(float *r, int i, int partner, bool
             stick = sqrtf(ev.C1 * ev.C1 + ev.C2 * ev.C2) < kStickThresh ? 1.0f : 0.0f;
         }
         lam4 = make_float4(ev.F0, ev.F1, ev.F2, stick);
-        cr[CR_LAM] = lam4;
-        cr[CR_PEN] = st3(pen, 0.0f);
+        if (Co)
+        {
+            st_coherent(&cr[CR_LAM], lam4);
+            st_coherent(&cr[CR_PEN], st3(pen, 0.0f));
+        }
+        else
+        {
+            cr[CR_LAM] = lam4;
+            cr[CR_PEN] = st3(pen, 0.0f);
+        }
         ev = contact_eval(ld3(c04), pen, ld3(lam4), rA4.w, oma, b0, b1, b2, nb0, nb1, nb2, ja0, ja1, ja2, jb0, jb1, jb2,
                           dqAL, dqBL, dqAA, dqBA);
     }
@@ -2525,13 +2532,16 @@ __global__ void k_stamp_fill(const int *deg, const int *adjStart, const int *adj
 }
 
 // Colour-sorted primal tasks: (body, first stamp, stamp count).
-__global__ void k_tasks(const int *colorBodies, const int *stampStart, const int *stampCount, int n, int4 *tasks)
+// done: the flow primal's per-body count of finished iterations, zeroed for every solved body.
+__global__ void k_tasks(const int *colorBodies, const int *stampStart, const int *stampCount, int n, int4 *tasks,
+                        int *done)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n)
         return;
     int i = colorBodies[k];
     tasks[k] = make_int4(i, stampStart[i], stampCount[i], 0);
+    done[i] = 0;
 }
 
 __device__ __forceinline__ float init_entry(int e, float ml, float ax, float ay, float az, V3 dl, V3 da)
@@ -2602,8 +2612,8 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
     for (int off = 32; off >= L; off >>= 1)
         rounds = max(rounds, __shfl_xor(rounds, off, 64));
     float4 mm = B.massMoment[i];
-    V3 pos = ld3(B.pose[4 * i + P_POS]);
-    Q4 rot = ldq(B.pose[4 * i + P_ROT]);
+    V3 pos = ld3(Co ? ld_coherent(&B.pose[4 * i + P_POS]) : B.pose[4 * i + P_POS]);
+    Q4 rot = ldq(Co ? ld_coherent(&B.pose[4 * i + P_ROT]) : B.pose[4 * i + P_ROT]);
     float dt2 = P.dt * P.dt;
     float ml = mm.x / dt2, ax = mm.y / dt2, ay = mm.z / dt2, az = mm.w / dt2;
     V3 dl = pos - ld3(B.inerLin[i]);
@@ -2809,93 +2819,102 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
                              padShift, k, st0, lazyDual != 0);
 }
 
-// Dataflow form of one iteration's primal (contact-only worlds): one launch covers all colours.
-// Each wave claims the next 64/L colour-sorted tasks; a body starts once every lower-colour
-// neighbour has published this iteration (done[j] == iter), so colours overlap instead of
-// waiting for kernel boundaries. Claiming in order keeps the wait deadlock-free: a body waits
-// only on bodies claimed before it, and the earliest unfinished one has nothing left to wait on.
-// Partner poses are read coherently; a body's new pose is stored coherently and its done flag
-// is published with release order after it. A bounded spin turns a missing flag into the
-// timeout error instead of a hang. Results equal the per-colour launches bit for bit.
-constexpr int kDataflowSpin = 1 << 18;
-constexpr int kDataflowPad = 8;  // a colour's tasks start on a multiple of this (the most groups a wave claims)
+// Flow form of a step's solver iterations (contact-only worlds, lazy dual): ONE launch runs every
+// iteration and colour. Instead of a kernel boundary after each colour (70 per step at 7 colours),
+// each body waits only for the partners it reads: done[j] counts the iterations body j has finished
+// in this step (k_tasks sets it to 0), and body i of colour c starts iteration `it` once every
+// partner j of a lower colour has done[j] >= it + 1 and every partner of a higher colour has
+// done[j] >= it (its pose and the dual updates i applies are then those the per-colour sweep sees).
+// The dependencies are exactly the coloured Gauss-Seidel order, so results equal the per-colour
+// launches bit for bit.
+//
+// Schedule: the colour-sorted tasks of each colour are cut into batches of one wave (64 / L
+// bodies); batch b of the step's batch list goes to wave b mod NW, every wave visits its batches in
+// (iteration, colour) order. The grid is sized so that every wave is resident at once (the
+// occupancy limit), so the wave holding the earliest unfinished batch always has its dependencies
+// met and the grid drains; a wait that outlives kFlowSpin polls raises the timeout flag and every
+// later wait gives up at once (the step's results are then void and the host reports the error).
+//
+// Coherence without cache flushes: poses and each contact's penalty/lambda (the data one body
+// writes and another reads) are stored and loaded with agent-scope atomics, which bypass the
+// per-XCD L2 slices; a body publishes done[i] only after s_waitcnt vmcnt(0) has seen its wave's
+// stores complete. Everything else the primal reads is fixed for the step and cached normally.
+constexpr int kFlowSpin = 1 << 18;
+// s_waitcnt with vmcnt = 0 and expcnt / lgkmcnt at their maxima (GFX9 encoding): wait until every
+// vector memory operation of the wave has completed.
+constexpr int kWaitVmcnt0 = 0x0F70;
 
-// The colour-sorted tasks with every colour padded to a multiple of kDataflowPad (padding tasks
-// have body -1), so that one wave's claim never holds two colours: a body never waits on a
-// lower-colour neighbour its own wave processes in lockstep with it.
-__global__ void k_tasks_pad_clear(int4 *padded, int numPadded)
+__device__ __forceinline__ bool flow_timed_out(Counters *ctr)
 {
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < numPadded; k += gridDim.x * blockDim.x)
-        padded[k] = make_int4(-1, 0, 0, 0);
+    return __hip_atomic_load(&ctr->barrierTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
-__global__ void k_tasks_padded_fill(const int4 *tasks, int numTasks, const int *colorOffsets, int colors,
-                                    int4 *padded)
+// Waits until body `i` (task position k, colour c) may run iteration `it`: lane l checks the
+// partners of stamps l, l + L, ...
+template <int L>
+__device__ __forceinline__ void flow_wait(int lane, int4 task, int k, int c, int it, const int4 *pad, int padShift,
+                                          const int *color, const int *done, Counters *ctr)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= numTasks)
-        return;
-    int shift = 0, c = 0;
-    for (; c < colors; ++c)
+    for (int t = lane; t < task.z; t += L)
     {
-        int b = colorOffsets[c], e = colorOffsets[c + 1];
-        if (k < e)
-            break;
-        int n = e - b;
-        shift += (n + kDataflowPad - 1) / kDataflowPad * kDataflowPad - n;
-    }
-    padded[k + shift] = tasks[k];
-}
-template <int L, int T>
-__global__ __launch_bounds__(T) void k_primal_dataflow(Bodies B, Manifolds M, const int4 *stamps, const int4 *tasks,
-                                                       int numTasks, const int *color, int *done, int iter, Params P,
-                                                       Counters *ctr)
-{
-    constexpr int G = T / L, WG = 64 / L;  // groups per block, per wave
-    __shared__ float rec[T * (kRec + 2)];
-    const int lane = threadIdx.x % L, g = threadIdx.x / L;
-    const int wl = threadIdx.x & 63;
-    for (;;)
-    {
-        int k0 = 0;
-        if (wl == 0)
-            k0 = (int)atomicAdd(&ctr->dfClaim, (unsigned)WG);
-        k0 = __shfl(k0, 0, 64);
-        if (k0 >= numTasks)
-            break;
-        const int k = k0 + (g % WG);
-        int4 task = k < numTasks ? tasks[k] : make_int4(-1, 0, 0, 0);
-        const bool valid = task.x >= 0;
-        if (!valid)
-            task = make_int4(0, 0, 0, 0);
-        // wait for the lower-colour neighbours of this body
-        if (valid)
+        const int j = pad[((size_t)k << padShift) + t].z;
+        if (j < 0)
+            continue;
+        const int cj = color[j];
+        if (cj < 0)
+            continue;  // never solved in this step: its pose does not change
+        const int need = cj < c ? it + 1 : it;
+        int spins = 0;
+        while (__hip_atomic_load(&done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need)
         {
-            const int c = color[task.x];
-            for (int t = lane; t < task.z; t += L)
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins >= kFlowSpin || ((spins & 255) == 0 && flow_timed_out(ctr)))
             {
-                int j = stamps[task.y + t].z;
-                int cj = j >= 0 ? color[j] : -1;
-                if (cj < 0 || cj >= c)
-                    continue;
-                int spins = 0;
-                while (__hip_atomic_load(&done[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != iter)
-                {
-                    __builtin_amdgcn_s_sleep(1);
-                    // once any wait has timed out the results are void: stop waiting at once
-                    if (++spins >= kDataflowSpin ||
-                        ((spins & 1023) == 0 &&
-                         __hip_atomic_load(&ctr->barrierTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-                    {
-                        atomicOr(&ctr->barrierTimeout, 1);
-                        break;
-                    }
-                }
+                atomicOr(&ctr->barrierTimeout, 1);
+                break;
             }
         }
-        primal_task<L, T, true>(rec, g, lane, valid, task, B, M, nullptr, nullptr, stamps, P, nullptr, nullptr);
-        if (valid && lane == 0)
-            __hip_atomic_store(&done[task.x], iter, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // the partner loads that follow are issued after the polls returned (no compiler reordering)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+template <int L, int T>
+__global__ __launch_bounds__(T) void k_primal_flow(Bodies B, Manifolds M, const int4 *tasks, const int *colorOffsets,
+                                                   int colors, const int *color, int *done, Params P, const int4 *pad,
+                                                   int padShift, Counters *ctr)
+{
+    constexpr int G = 64 / L;  // bodies per wave batch
+    __shared__ float rec[T * (kRec + 2)];
+    const int lane = threadIdx.x % L, g = threadIdx.x / L, gw = (int)(threadIdx.x & 63) / L;
+    const int wave = (int)blockIdx.x * (T / 64) + (int)(threadIdx.x >> 6);
+    const int numWaves = (int)gridDim.x * (T / 64);
+    for (int it = 0; it < P.iterations; ++it)
+    {
+        int batchBase = 0;  // batches of the colours before c
+        for (int c = 0; c < colors; ++c)
+        {
+            const int cb = colorOffsets[c], ce = colorOffsets[c + 1];
+            const int nb = (ce - cb + G - 1) / G;
+            for (int m = ((wave - batchBase) % numWaves + numWaves) % numWaves; m < nb; m += numWaves)
+            {
+                const int k = cb + m * G + gw;
+                const bool valid = k < ce;
+                const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
+                const int4 st0 = valid ? pad[((size_t)k << padShift) + lane] : make_int4(0, 0, 0, 0);
+                if (valid && !flow_timed_out(ctr))
+                    flow_wait<L>(lane, task, k, c, it, pad, padShift, color, done, ctr);
+                primal_task<L, T, true>(rec, g, lane, valid, task, B, M, nullptr, nullptr, nullptr, P, nullptr, nullptr,
+                                        pad, padShift, k, st0, it > 0);
+                // publish only once this wave's pose and penalty/lambda stores have completed
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                if (valid && lane == 0)
+                    __hip_atomic_store(&done[task.x], it + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            batchBase += nb;
+        }
     }
 }
 
@@ -3010,22 +3029,13 @@ __device__ void dual_joint(Joint *jp, const Bodies &B, const Params &P)
 
 // Dual pass of one iteration: one lane per contact, then one per joint.
 // flagTape (adjoint recording, may be null): per contact the dual's branch flags.
-__device__ __forceinline__ void ctr_reset_claim(unsigned *claim)
-{
-    if (claim)
-        *claim = 0u;
-}
-
-// claimReset (may be null): the dataflow primal's claim counter, cleared for the next iteration.
 // staticRepeat (lazy-dual form's closing launch, 1 otherwise): a contact between two bodies that
 // are never solved (both static) has no owner stamp to carry its per-iteration dual updates; its
 // bodies never move, so its updates of all iterations are applied here one after another.
 __global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int4 *contactRec, int nc, Params P,
-                       unsigned char *flagTape, unsigned *claimReset, int staticRepeat)
+                       unsigned char *flagTape, int staticRepeat)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t == 0)
-        ctr_reset_claim(claimReset);
     if (t < nc)
     {
         const int4 cr = contactRec[t];
@@ -3065,7 +3075,6 @@ __global__ void k_tape_contacts(const int4 *contactRec, const Counters *ctr, con
     }
 }
 
-__global__ void k_claim_reset(Counters *c) { c->dfClaim = 0u; }
 
 // Solver::finishStep for boxes (solver.cpp:1275-1288).
 // applySphereRollingFriction (solver.cpp:628-660), per dynamic sphere after k_finish: its

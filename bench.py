@@ -35,17 +35,17 @@ for _p in (ROOT, os.path.join(ROOT, "avbd-demo3d_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_primal_color_pile50k_traffic.txt")
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per k_primal_color launch from the committed rocprofv3 PMC summary, if present."""
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from its committed rocprofv3 PMC summary, if present."""
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(os.path.join(PROFILES, f"{kernel}_pile50k_traffic.txt")) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -204,7 +204,12 @@ def main():
     timed = max(int(st["timed_steps"]), 1)
     pr_launches = max(int(st["primal_launches"]), 1)
     pr_ms = float(st["primal_ms_total"]) / pr_launches
-    pr_bytes = float(st["primal_bytes_total"]) / steps_counted / iters / (pr_launches / timed)
+    # the spans cover the last iteration's colour launches, or (flow form) the one launch per step
+    # that runs every iteration
+    pr_iters = max(int(st["primal_iterations"]), 1)
+    flow = pr_iters > timed
+    pr_kernel = "k_primal_flow" if flow else "k_primal_color"
+    pr_bytes = float(st["primal_bytes_total"]) / steps_counted / iters * pr_iters / pr_launches
     achieved = pr_bytes / (pr_ms * 1e-3) / 1e9
     du_launches = max(int(st["dual_launches"]), 1)
     du_ms = float(st["dual_ms_total"]) / du_launches
@@ -212,7 +217,7 @@ def main():
     du_per_step = 1 if launch.get("lazy_dual") else iters
     du_bytes = float(st["dual_bytes_total"]) / steps_counted / du_per_step / (du_launches / timed)
     loop_gbs = iter_bytes / (iter_ms * 1e-3) / 1e9
-    log(f"k_primal_color {pr_ms * 1e3:.2f} us/launch {pr_bytes / 1e6:.2f} MB/launch -> {achieved:.0f} GB/s; "
+    log(f"{pr_kernel} {pr_ms * 1e3:.2f} us/launch {pr_bytes / 1e6:.2f} MB/launch -> {achieved:.0f} GB/s; "
         f"k_dual {du_ms * 1e3:.2f} us/launch {du_bytes / 1e6:.2f} MB/launch -> "
         f"{du_bytes / (du_ms * 1e-3) / 1e9:.0f} GB/s; iteration loop {iter_ms:.3f} ms -> {loop_gbs:.0f} GB/s")
 
@@ -250,15 +255,16 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_primal_color",
+            "kernel": pr_kernel,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(),
+            "traffic": measured_traffic(pr_kernel),
             "bytes_per_launch": pr_bytes,
             "launch_us": pr_ms * 1e3,
-            "iteration_loop": {"kernels": "k_primal_color x colours + k_dual, x10", "ms_per_step": iter_ms,
+            "iteration_loop": {"kernels": ("k_primal_flow (all iterations) + k_dual" if flow else
+                                           "k_primal_color x colours + k_dual, x10"), "ms_per_step": iter_ms,
                                "bytes_per_step": iter_bytes, "achieved": loop_gbs,
                                "frac": loop_gbs / HBM_PEAK_GBS},
             "k_dual": {"launch_us": du_ms * 1e3, "bytes_per_launch": du_bytes,

@@ -207,6 +207,10 @@ typedef struct avbd_stats {
     double dual_ms_total;
     int64_t primal_launches;
     int64_t dual_launches;
+    /* Solver iterations the primal spans cover: one per timed step with per-colour launches (the
+     * last iteration), all of them with the flow form (avbd_launch_config.dataflow), whose one
+     * primal launch per step runs every iteration. */
+    int64_t primal_iterations;
 } avbd_stats;
 
 typedef struct avbd_handle avbd_handle;
@@ -355,9 +359,10 @@ typedef struct avbd_launch_config {
     int32_t primal_threads;  /* threads per primal block: 64, 128 or 256 */
     int32_t dual_threads;    /* threads per dual block: 64, 128 or 256 */
     int32_t use_graph;       /* 1: each solver iteration is one hipGraph launch */
-    int32_t dataflow;        /* 1: one primal launch per iteration for all colours, bodies
-                                starting as their lower-colour neighbours finish
-                                (contact-only worlds; otherwise ignored) */
+    int32_t dataflow;        /* 1: the flow form: one primal launch per step runs every iteration
+                                and colour, each body starting as soon as the partners it reads
+                                have finished the iterations it needs (contact-only worlds with
+                                lazy_dual and direct launches; otherwise ignored) */
     int32_t lazy_dual;       /* 1: each contact point's dual update is applied at the start of
                                 the next iteration's primal by the first of its bodies to be
                                 solved, and one dual launch after the last iteration covers
@@ -403,8 +408,8 @@ int avbd_debug_inject_fault(avbd_handle *h, int32_t code);
 int avbd_debug_narrowphase_capacity(avbd_handle *h, int32_t vertices);
 
 /* sizeof of the ABI records (0 body, 1 contact, 2 manifold, 3 joint, 4 spring,
- * 5 params, 6 launch config), so bindings can verify their layouts; -1 for
- * other values. */
+ * 5 params, 6 launch config, 7 stats, 8 config), so bindings can verify their
+ * layouts; -1 for other values. */
 int avbd_sizeof(int which);
 
 #ifdef __cplusplus

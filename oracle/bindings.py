@@ -394,8 +394,8 @@ class RefWorld:
             self._h = None
         _REF_LIVE.discard(self)
 
-    def __del__(self):
-        if sys.is_finalizing():
+    def __del__(self, _finalizing=sys.is_finalizing):
+        if _finalizing():
             return
         self.close()
 

@@ -114,25 +114,32 @@ def test_lazy_dual_equals_eager_with_joints_and_springs(gpu):
         assert_states_equal(out[1], out[0], f"{sc.name}: lazy vs eager dual")
 
 
-def test_dataflow_primal_at_headline_size(gpu):
-    """The dataflow primal (all colours in one launch per iteration, bodies waiting on their
-    lower-colour neighbours) equals the per-colour launches bit for bit on the 50,000-box pile."""
+@pytest.mark.parametrize("lanes,threads", [(16, 64), (8, 128)])
+def test_flow_primal_on_settled_headline_pile(gpu, lanes, threads):
+    """The flow primal (one launch per step for every iteration and colour, each body waiting on
+    the progress counters of its partners) equals the per-colour launches bit for bit on the
+    settled 50,000-box pile (about 240k contacts, 7 colours)."""
     from avbd import scenes
     from avbd.state import default_params
     sc = scenes.pile(50000)
     prm = default_params(gravity=sc.gravity)
     st = scenes.build_state(sc, gpu.Maker())
     a = gpu.HipWorld.for_state(st, prm)
-    a.step(80)
+    a.set_launch_config(16, 64, 128, use_graph=0, dataflow=0)
+    a.step(600)
     mid = a.state()
-    a.step(3)
+    a.step(4)
     ref = a.state()
+    stats = a.stats()
     a.close()
+    assert int(stats["color_count"]) >= 5 and int(stats["contact_count"]) > 150000
+    print(f"flow L{lanes} T{threads}: {int(stats['contact_count'])} contacts, {int(stats['color_count'])} colours",
+          flush=True)
     b = gpu.HipWorld.for_state(mid, prm)
-    b.set_launch_config(16, 64, 128, use_graph=0, dataflow=1)
-    b.step(3)
+    b.set_launch_config(lanes, threads, 128, use_graph=0, dataflow=1)
+    b.step(4)
     b.synchronize()
-    assert_states_equal(b.state(), ref, "dataflow, 50k pile, steps 81-83")
+    assert_states_equal(b.state(), ref, f"flow L{lanes} T{threads}, 50k pile, steps 601-604")
     b.close()
 
 
