@@ -84,6 +84,8 @@ def load(path: str | None = None):
         lib.avbd_batch_upload.argtypes = [_p, _p, C.c_int32, _p, C.c_int32, _p, C.c_int32, _p, C.c_int32, _p,
                                           C.c_int32, C.c_int64]
         lib.avbd_batch_num_worlds.argtypes = [_p, _p]
+        lib.avbd_set_broadphase.argtypes = [_p, C.c_int32]
+        lib.avbd_get_broadphase.argtypes = [_p, _p]
         lib.avbd_set_launch_config.argtypes = [_p, _p]
         lib.avbd_get_launch_config.argtypes = [_p, _p]
         lib.avbd_autotune.argtypes = [_p, C.c_int32, C.c_int32, _p]
@@ -194,6 +196,19 @@ class HipWorld:
         n = C.c_int32()
         _check(self._lib.avbd_batch_num_worlds(self._h, C.byref(n)))
         return n.value
+
+    BROADPHASE_ALL_PAIRS, BROADPHASE_GRID, BROADPHASE_SAP = 0, 1, 2
+
+    def set_broadphase(self, mode: int):
+        """Solver::broadphaseMode: BROADPHASE_SAP (default), _GRID (the reference Solver's default)
+        or _ALL_PAIRS."""
+        _check(self._lib.avbd_set_broadphase(self._h, int(mode)))
+
+    @property
+    def broadphase(self) -> int:
+        m = C.c_int32()
+        _check(self._lib.avbd_get_broadphase(self._h, C.byref(m)))
+        return m.value
 
     def launch_config(self) -> dict:
         out = np.zeros((), LAUNCH)

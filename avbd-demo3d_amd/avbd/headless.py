@@ -236,12 +236,14 @@ class _Stat:
 class Host:
     """SimulationHost for the GPU step: a named scene resident on the GPU, reloadable."""
 
-    def __init__(self, scene_index: int, iterations: int = 0, gravity=None, timing: bool = False):
+    def __init__(self, scene_index: int, iterations: int = 0, gravity=None, timing: bool = False,
+                 broadphase: str | None = None):
         from . import capi
         self._capi = capi
         self.scene_index = scene_index
         self.scene_name = named_scenes.NAMES[scene_index]
         self.iterations, self.gravity, self.timing = iterations, gravity, timing
+        self.broadphase = broadphase_mode(broadphase)
         self.world = None
         self.load()
 
@@ -259,6 +261,7 @@ class Host:
             self.world.close()
         self.initial = state
         self.world = self._capi.HipWorld.for_state(state, self.params(), timing=self.timing)
+        self.world.set_broadphase(self.broadphase)
         self.snapshot_frame = 0
         self.constraints = state.next_serial
 
@@ -281,13 +284,14 @@ class Host:
 
 def run_benchmark(scene: str | None, frames: int = 600, warmup_frames: int = 0, reset_after_warmup: bool = False,
                   no_stream: bool = True, iterations: int = 0, gravity=None, avbs_out: str | None = None,
-                  phase_timing: bool = False, host: Host | None = None) -> dict:
-    """runBenchmark (headless_server.cpp:605-852) on the GPU step. Returns (fields, JSON line)."""
+                  phase_timing: bool = False, host: Host | None = None, broadphase: str | None = None) -> dict:
+    """runBenchmark (headless_server.cpp:605-852) on the GPU step. Returns (fields, JSON line).
+    broadphase: a --broadphase name (None: the reference Solver's default, the uniform grid)."""
     frames = frames if frames > 0 else 600
     warmup_frames = max(warmup_frames, 0)
     idx = named_scenes.index_of(scene) if scene else 1
     own = host is None
-    host = host or Host(idx, iterations, gravity, phase_timing)
+    host = host or Host(idx, iterations, gravity, phase_timing, broadphase)
     for _ in range(warmup_frames):
         host.step_frame()
     if reset_after_warmup and warmup_frames > 0:
@@ -396,13 +400,15 @@ def benchmark_fields(scene_name, frames, warmup_frames, reset_after_warmup, no_s
 
 
 def run_backend_comparison(scene: str | None, frames: int, reference, iterations: int = 0,
-                           reference_name: str = "CPU Reference", host: Host | None = None):
+                           reference_name: str = "CPU Reference", host: Host | None = None,
+                           broadphase: str | None = None):
     """runBackendComparison (headless_server.cpp:852-943): the GPU step against `reference`, a solver
-    on the same scene with step(n) and state(). Returns (fields, JSON line)."""
+    on the same scene with step(n) and state(), both in the broadphase mode `broadphase` (None:
+    the reference Solver's default, the uniform grid). Returns (fields, JSON line)."""
     frames = frames if frames > 0 else 120
     idx = named_scenes.index_of(scene) if scene else 1
     own = host is None
-    host = host or Host(idx, iterations)
+    host = host or Host(idx, iterations, broadphase=broadphase)
     rstat, cstat = _Stat(), _Stat()
     for _ in range(frames):
         t0 = time.perf_counter()
@@ -451,6 +457,23 @@ def comparison_fields(scene_name, frames, iterations, reference_name, reference_
 
 # ---- command line (headless_server.cpp:946-985) ----
 
+# applyBroadphaseMode (headless_server.cpp:579-591): names of Solver::broadphaseMode; without
+# --broadphase the Solver keeps its default, the uniform grid (solver.cpp:676-680).
+BROADPHASE_NAMES = {"allpairs": 0, "grid": 1, "uniformgrid": 1, "spatialhash": 1, "sap": 2, "sweepandprune": 2}
+DEFAULT_BROADPHASE = 1
+
+
+def broadphase_mode(name: str | None) -> int:
+    """The mode for a --broadphase name; None or an unknown name (reported on stderr, as the
+    reference does) keeps the default."""
+    if name is None:
+        return DEFAULT_BROADPHASE
+    if name not in BROADPHASE_NAMES:
+        print(f"Unknown broadphase '{name}' (expected allpairs, grid, or sap)", file=sys.stderr)
+        return DEFAULT_BROADPHASE
+    return BROADPHASE_NAMES[name]
+
+
 def _arg(argv, name):
     for i, a in enumerate(argv[:-1]):
         if a == name:
@@ -474,9 +497,6 @@ def main(argv=None) -> int:
     if backend is not None and backend not in BACKEND_NAMES:
         print(f"Unknown physics backend '{backend}' (this library runs hip-avbd only)", file=sys.stderr)
         return 2
-    if broadphase is not None and broadphase not in ("sap", "sweepandprune"):
-        print(f"Broadphase '{broadphase}' is not available (the GPU step uses sweep and prune)", file=sys.stderr)
-        return 2
     if "--collision-only" in argv:
         print("--collision-only is not available on hip-avbd", file=sys.stderr)
         return 2
@@ -490,7 +510,7 @@ def main(argv=None) -> int:
                 return 2
         _, line = run_benchmark(name, frames, warmup, "--reset-after-warmup" in argv, "--no-stream" in argv,
                                 iterations, float(np.float32(float(g))) if g is not None else None,
-                                _arg(argv, "--avbs-out"), "--phase-timing" in argv)
+                                _arg(argv, "--avbs-out"), "--phase-timing" in argv, broadphase=broadphase)
         print(line, flush=True)
         return 0
     if compare_scene or compare_frames > 0:
@@ -498,8 +518,8 @@ def main(argv=None) -> int:
               "call avbd.headless.run_backend_comparison with a reference solver", file=sys.stderr)
         return 2
     print("usage: avbd_headless.py --benchmark-scene NAME [--benchmark-frames N] [--warmup-frames W] "
-          "[--reset-after-warmup] [--no-stream] [--iterations I] [--gravity G] [--avbs-out FILE] [--phase-timing]",
-          file=sys.stderr)
+          "[--reset-after-warmup] [--no-stream] [--iterations I] [--gravity G] [--broadphase allpairs|grid|sap] "
+          "[--avbs-out FILE] [--phase-timing]", file=sys.stderr)
     return 2
 
 

@@ -247,6 +247,23 @@ def pile(count: int = 50000, seed: int = 3, grid=None, pitch: float = 3.6, steps
     return sb.scene(f"pile{count}", -9.81, steps)
 
 
+def spread(count: int = 2000, seed: int = 9, extent: float = 400.0, steps: int = 120) -> Scene:
+    """Small boxes (edges U[0.05, 0.2] m) scattered over an ``extent`` x ``extent`` m ground slab at
+    heights U[0.3, 12] m. Sparse enough that the uniform-grid broadphase's 2^22-cell limit grows
+    its cells (solver.cpp:336-344)."""
+    rng = _rng(seed)
+    sb = _Builder()
+    sb.body((extent + 20.0, extent + 20.0, 2.0), 0.0, 0.5, (0.0, 0.0, -1.0))
+    size = rng.uniform(0.05, 0.2, (count, 3))
+    quats = random_quaternions(rng, count)
+    xy = rng.uniform(-extent / 2.0, extent / 2.0, (count, 2))
+    z = rng.uniform(0.3, 12.0, count)
+    vel = rng.normal(0.0, 0.5, (count, 3))
+    for i in range(count):
+        sb.body(tuple(size[i]), 1.0, 0.5, (xy[i, 0], xy[i, 1], z[i]), tuple(quats[i]), tuple(vel[i]))
+    return sb.scene(f"spread{count}", -9.81, steps)
+
+
 def stacks(count: int, seed: int = 1, height: int = 10, jitter: float = 0.01, spacing: float = 1.5,
            steps: int = 300) -> Scene:
     """Columns of up to ``height`` unit boxes on a static ground box, horizontal jitter +-0.01 m."""

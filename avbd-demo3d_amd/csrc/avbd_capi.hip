@@ -7,6 +7,7 @@
 // sequence of launches.
 #include "../../include/avbd_hip.h"
 #include "avbd_kernels.hip"
+#include "avbd_grid.hip"
 
 #include <algorithm>
 #include <climits>
@@ -135,6 +136,12 @@ struct avbd_handle
     int npPolyCap = kNpPolyCap;  // clip-polygon vertices the box-only narrowphase keeps in LDS
     // launch shapes of the iteration kernels, and the iteration graph built from them
     avbd_launch_config launch{16, 256, 256, 0, 0, 1};
+    // broadphase mode (Solver::broadphaseMode, solver.h:366) and the grid's buffers, allocated
+    // when a mode other than sweep and prune is first set
+    int broadphaseMode = AVBD_BROADPHASE_SAP;
+    GridInfo *gridInfo = nullptr, *hostGridInfo = nullptr;
+    unsigned *cellStart = nullptr;  // kGridMaxCells entries, 0xFFFFFFFF when empty
+    float4 *largeLo = nullptr, *largeHi = nullptr, *largeC = nullptr;
     int *dfDone = nullptr;  // flow primal: iterations each solved body has finished in this step
     int numCUs = 0;
     int flowBlocks[3][3] = {};  // flow primal grid per (lanes, threads) shape: every wave resident
@@ -393,6 +400,92 @@ int check_device_errors(avbd_handle *h)
     return AVBD_OK;
 }
 
+// Uniform-grid or all-pairs broadphase of a single world (solver.cpp:176-185, 284-486): the hits in
+// the reference's emission order into S.hits / S.hitNew, their count into ctr->numHits and the
+// checks into ctr->pairChecks, like the sweep-and-prune launches. The grid's cell size needs
+// the host: the radius select and the bounds are read back, and the cell size, dimensions and
+// the coarsening loop (with the host's cbrtf) are computed here exactly as the reference does.
+int enqueue_grid_pairs(avbd_handle *h)
+{
+    hipStream_t s = h->stream;
+    const int N = h->nb;
+    Scratch &S = h->S;
+    Counters *ctr = h->ctr;
+    const bool grid = h->broadphaseMode == AVBD_BROADPHASE_GRID;
+    GridParams g{};
+    unsigned *keys = S.keys, *keysAlt = S.keysAlt;
+    int *vals = S.vals, *valsAlt = S.valsAlt;
+    if (grid)
+    {
+        const int rank = (int)std::min<long long>((long long)N - 1, (long long)N * 95 / 100);
+        k_grid_select<<<1, 1024, 0, s>>>(h->B.sizeRad, N, rank, h->gridInfo);
+        k_grid_bounds<<<std::max(1, std::min(ceil_div(N, 256), 1024)), 256, 0, s>>>(h->B.pose, h->B.sizeRad, N,
+                                                                                   h->gridInfo);
+        HIP_TRY(hipMemcpyAsync(h->hostGridInfo, h->gridInfo, sizeof(GridInfo), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        const GridInfo &info = *h->hostGridInfo;
+        g.binnedRadius = info.binnedRadius;
+        float cellSize = info.binnedRadius * 2.0f;
+        int nx = 1, ny = 1, nz = 1;
+        g.numBinned = info.numBinned;
+        if (g.numBinned > 0)
+        {
+            const float lo[3] = {grid_unord(info.bmin[0]), grid_unord(info.bmin[1]), grid_unord(info.bmin[2])};
+            const float hi[3] = {grid_unord(info.bmax[0]), grid_unord(info.bmax[1]), grid_unord(info.bmax[2])};
+            const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+            auto dims = [&](float cs) -> unsigned long long {
+                nx = (int)(ex / cs) + 1;
+                ny = (int)(ey / cs) + 1;
+                nz = (int)(ez / cs) + 1;
+                return (unsigned long long)nx * (unsigned long long)ny * (unsigned long long)nz;
+            };
+            unsigned long long cells = dims(cellSize);
+            while (cells > kGridMaxCells)
+            {
+                const float grow = std::cbrt((float)cells / (float)kGridMaxCells);
+                cellSize *= grow > 1.1f ? grow : 1.1f;
+                cells = dims(cellSize);
+            }
+            g.bminx = lo[0];
+            g.bminy = lo[1];
+            g.bminz = lo[2];
+        }
+        g.cellSize = cellSize;
+        g.nx = nx;
+        g.ny = ny;
+        g.nz = nz;
+        g.numLarge = N - g.numBinned;
+        k_grid_keys<<<ceil_div(N, 256), 256, 0, s>>>(h->B.pose, h->B.sizeRad, N, g, keys, vals);
+        radix_sort(h, keys, vals, keysAlt, valsAlt, N, 3);
+        k_grid_gather<<<ceil_div(N, 256), 256, 0, s>>>(vals, keys, N, g, h->B, h->P.dt, S.sPos, S.sLo, h->cellStart,
+                                                      h->largeLo, h->largeHi, h->largeC);
+    }
+    else
+    {
+        // all pairs: every body is a large body, in list order, tested by the bounding spheres
+        g.numBinned = 0;
+        g.numLarge = N;
+        g.nx = g.ny = g.nz = 1;
+        k_grid_gather<<<ceil_div(N, 256), 256, 0, s>>>(nullptr, keys, N, g, h->B, h->P.dt, S.sPos, S.sLo, h->cellStart,
+                                                      h->largeLo, h->largeHi, h->largeC);
+    }
+    const int *order = grid ? vals : nullptr;
+    const int blocks = std::max(1, std::min(ceil_div(N, 256), 8192));
+    k_grid_pairs<false><<<blocks, 256, 0, s>>>(g, S.sPos, S.sLo, keys, order, h->cellStart, h->largeLo, h->largeHi,
+                                               h->largeC, S.hitCount, nullptr, nullptr, 0, nullptr, nullptr, 0, ctr);
+    scan(h, S.hitCount, S.hitStart, nullptr, N, &ctr->numHits, h->maxPairs, 1);
+    k_fill_u64<<<grid_for(h->hashSize), 256, 0, s>>>(S.hash, h->hashSize, ~0ull);
+    k_hash_insert<<<1024, 256, 0, s>>>(h->cur.ab, ctr, h->joints, h->nj, h->springs, h->ns, h->ignorePairs, h->numIgnore,
+                                       S.hash, S.hashMask, ctr);
+    k_grid_pairs<true><<<blocks, 256, 0, s>>>(g, S.sPos, S.sLo, keys, order, h->cellStart, h->largeLo, h->largeHi,
+                                              h->largeC, nullptr, S.hitStart, S.hash, S.hashMask, S.hits, S.hitNew,
+                                              h->maxPairs, ctr);
+    if (grid && g.numBinned > 0)
+        k_grid_clear<<<ceil_div(g.numBinned, 256), 256, 0, s>>>(keys, g.numBinned, h->cellStart);
+    HIP_TRY(hipGetLastError());
+    return AVBD_OK;
+}
+
 int enqueue_prepare(avbd_handle *h)
 {
     hipStream_t s = h->stream;
@@ -408,8 +501,16 @@ int enqueue_prepare(avbd_handle *h)
     k_step_begin<<<1, 1, 0, s>>>(ctr);
     k_barrier_reset<<<1, 1, 0, s>>>(h->gb);
 
-    // ---- broadphase: sweep and prune (solver.cpp:488-577) ----
-    if (N >= 2)
+    // ---- broadphase: sweep and prune (solver.cpp:488-577), or the grid / all pairs ----
+    if (N >= 2 && h->broadphaseMode != AVBD_BROADPHASE_SAP)
+    {
+        int r = enqueue_grid_pairs(h);
+        if (r)
+            return r;
+        scan(h, S.hitNew, S.hitRank, &ctr->numHits, 0, &ctr->numCreated);
+        k_create<<<G, 256, 0, s>>>(S.hits, S.hitNew, S.hitRank, ctr, h->cur, h->maxManifolds, h->maxPairs);
+    }
+    else if (N >= 2)
     {
         if (h->numWorlds > 1)
         {
@@ -1131,6 +1232,8 @@ int avbd_destroy(avbd_handle *h)
         (void)hipHostFree(h->hostCtr);
     if (h->hostOffsets)
         (void)hipHostFree(h->hostOffsets);
+    if (h->hostGridInfo)
+        (void)hipHostFree(h->hostGridInfo);
     for (int k = 0; k < 8; ++k)
         if (h->ev[k])
             (void)hipEventDestroy(h->ev[k]);
@@ -1898,6 +2001,9 @@ int avbd_batch_upload(avbd_handle *h, const int32_t *world_body_offsets, int32_t
         return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_batch_upload: world count must be in [1, num_bodies]");
     if (world_body_offsets[0] != 0 || world_body_offsets[num_worlds] != num_bodies)
         return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_batch_upload: offsets must run from 0 to num_bodies");
+    if (num_worlds > 1 && h->broadphaseMode != AVBD_BROADPHASE_SAP)
+        return fail(AVBD_ERR_UNSUPPORTED, "avbd_batch_upload: batched worlds use sweep and prune "
+                                          "(avbd_set_broadphase(h, AVBD_BROADPHASE_SAP) first)");
     std::vector<int> worldOf(std::max(num_bodies, 1), 0);
     for (int w = 0; w < num_worlds; ++w)
     {
@@ -1940,6 +2046,39 @@ int avbd_batch_upload(avbd_handle *h, const int32_t *world_body_offsets, int32_t
     h->maxWorldBodies = 0;
     for (int w = 0; w < num_worlds; ++w)
         h->maxWorldBodies = std::max(h->maxWorldBodies, world_body_offsets[w + 1] - world_body_offsets[w]);
+    return AVBD_OK;
+}
+
+int avbd_set_broadphase(avbd_handle *h, int32_t mode)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_broadphase: null handle");
+    if (mode != AVBD_BROADPHASE_ALL_PAIRS && mode != AVBD_BROADPHASE_GRID && mode != AVBD_BROADPHASE_SAP)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_broadphase: unknown mode " + std::to_string(mode));
+    if (mode != AVBD_BROADPHASE_SAP && h->numWorlds > 1)
+        return fail(AVBD_ERR_UNSUPPORTED, "avbd_set_broadphase: batched worlds use sweep and prune");
+    int r = pick_device(h);
+    if (r)
+        return r;
+    if (mode != AVBD_BROADPHASE_SAP && !h->gridInfo)
+    {
+        if ((r = h->alloc(h->gridInfo, 1)) || (r = h->alloc(h->cellStart, (size_t)kGridMaxCells)) ||
+            (r = h->alloc(h->largeLo, h->maxBodies)) || (r = h->alloc(h->largeHi, h->maxBodies)) ||
+            (r = h->alloc(h->largeC, h->maxBodies)))
+            return r;
+        HIP_TRY(hipHostMalloc((void **)&h->hostGridInfo, sizeof(GridInfo)));
+        HIP_TRY(hipMemsetAsync(h->cellStart, 0xFF, sizeof(unsigned) * kGridMaxCells, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    h->broadphaseMode = mode;
+    return AVBD_OK;
+}
+
+int avbd_get_broadphase(avbd_handle *h, int32_t *mode)
+{
+    if (!h || !mode)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_get_broadphase: null argument");
+    *mode = h->broadphaseMode;
     return AVBD_OK;
 }
 

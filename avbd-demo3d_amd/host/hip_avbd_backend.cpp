@@ -87,12 +87,12 @@ struct HipAvbdBackend : PhysicsBackend
         solver.stepCpuReference();
     }
 
-    // The GPU step reproduces the sweep-and-prune broadphase (solver.cpp:488-577). Any other
-    // pair source the Solver is set to -- the uniform grid or all pairs (solver.cpp:1077-1082),
-    // external pairs or contacts (solver.h:336-343, 364-365) or the GPU-narrowphase pair sink
-    // (solver.h:379-383) -- gives a different pair order or contact set, so those steps run on the
-    // CPU reference, counted as fallbacks, as the reference's own GPU lane does for steps it cannot
-    // take (avbd_gpu_solver.cpp:2152-2170).
+    // The GPU step runs the Solver's own broadphase mode (sweep and prune, the uniform grid or all
+    // pairs; solver.cpp:1077-1082), each with the reference's pair order. The other pair sources a
+    // Solver can be set to -- external pairs or contacts (solver.h:336-343, 364-365) or the
+    // GPU-narrowphase pair sink (solver.h:379-383) -- give a different contact set, so those steps
+    // run on the CPU reference, counted as fallbacks, as the reference's own GPU lane does for
+    // steps it cannot take (avbd_gpu_solver.cpp:2152-2170).
     const char *unsupported(const Solver &solver) const
     {
         if (solver.useExternalBroadphasePairs)
@@ -101,9 +101,9 @@ struct HipAvbdBackend : PhysicsBackend
             return "external manifold contacts";
         if (solver.spherePairSink)
             return "a GPU-narrowphase pair sink";
-        if (solver.broadphaseMode != BROADPHASE_SWEEP_AND_PRUNE)
-            return solver.broadphaseMode == BROADPHASE_UNIFORM_GRID ? "the uniform-grid broadphase"
-                                                                     : "the all-pairs broadphase";
+        if (solver.broadphaseMode != BROADPHASE_SWEEP_AND_PRUNE && solver.broadphaseMode != BROADPHASE_UNIFORM_GRID &&
+            solver.broadphaseMode != BROADPHASE_ALL_PAIRS)
+            return "an unknown broadphase mode";
         return nullptr;
     }
 
@@ -260,6 +260,7 @@ struct HipAvbdBackend : PhysicsBackend
         if (avbd_upload_state(h, tb.data(), (int)tb.size(), tj.data(), (int)tj.size(), ts.data(), (int)ts.size(),
                               tm.data(), (int)tm.size(), (int64_t)created.size()) != AVBD_OK ||
             avbd_set_ignore_pairs(h, ti.data(), (int32_t)(ti.size() / 2)) != AVBD_OK ||
+            avbd_set_broadphase(h, (int32_t)solver.broadphaseMode) != AVBD_OK ||  // enum values shared
             avbd_step(h, 1) != AVBD_OK || avbd_synchronize(h) != AVBD_OK ||
             avbd_state_counts(h, &nb, &nj, &ns, &nm, &nextSerial) != AVBD_OK)
         {

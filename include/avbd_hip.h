@@ -288,6 +288,19 @@ int avbd_get_colors(avbd_handle *h, int32_t *colors, int32_t capacity);
  * (index_a, index_b, created) triples; created = 1 when a new manifold was made. */
 int avbd_get_hits(avbd_handle *h, int32_t *triples, int32_t capacity, int32_t *count);
 
+/* Broadphase mode: Solver::broadphaseMode (solver.h:39-45, 366). AVBD_BROADPHASE_SAP is sweep
+ * and prune (solver.cpp:488-577), the default of a new handle; AVBD_BROADPHASE_GRID is the uniform
+ * grid (solver.cpp:284-486), the default of a default-constructed reference Solver
+ * (solver.cpp:676-680); AVBD_BROADPHASE_ALL_PAIRS tests every pair in list order
+ * (solver.cpp:176-185). The modes emit different pair sequences, hence different manifold creation
+ * orders and trajectories; each reproduces its reference mode bit for bit. The grid's cell size is
+ * computed on the host from two values read back per step (the reference's own float arithmetic,
+ * cbrtf included). Batched worlds (more than one) use sweep and prune only. The first switch to a
+ * mode other than sweep and prune allocates the grid's buffers (a 16 MiB cell table). */
+enum avbd_broadphase { AVBD_BROADPHASE_ALL_PAIRS = 0, AVBD_BROADPHASE_GRID = 1, AVBD_BROADPHASE_SAP = 2 };
+int avbd_set_broadphase(avbd_handle *h, int32_t mode);
+int avbd_get_broadphase(avbd_handle *h, int32_t *mode);
+
 /* IgnoreCollision forces (solver.h:283-290, force.cpp:75-79) of the uploaded
  * state: body pairs (list indices, 2 * count values) the broadphase treats as
  * already joined (Rigid::constrainedTo, rigid.cpp:105-116), so no manifold is

@@ -148,6 +148,10 @@ template <class V> inline DualT<V> sqrt_(DualT<V> x)
     return mk<V>(sqrt_(x.v), x.d / (2.0 * std::sqrt((double)x.v)));
 }
 template <class V> inline DualT<V> abs_(DualT<V> x) { return x.v < V(0) ? -x : x; }
+// The plain value of a scalar: the broadphase's discrete decisions use the value alone.
+inline float plain_(float x) { return x; }
+inline double plain_(double x) { return x; }
+template <class V> inline V plain_(DualT<V> x) { return x.v; }
 template <class V> inline bool isfinite_(DualT<V> x) { return std::isfinite(x.v); }
 template <class V> inline bool isinf_(DualT<V> x) { return std::isinf(x.v); }
 
@@ -379,6 +383,15 @@ struct Hit
 {
     int a, b;
     int created;
+};
+
+// Solver::broadphaseMode (solver.h:39-45): all pairs, uniform grid (the reference's default,
+// solver.cpp:676-680) or sweep and prune.
+enum BroadphaseMode
+{
+    BP_ALL_PAIRS = 0,
+    BP_GRID = 1,
+    BP_SAP = 2
 };
 
 enum OrderMode
@@ -1355,6 +1368,7 @@ struct World
     std::vector<std::pair<int, int>> ignores; // IgnoreCollision pairs (force.cpp:75-79)
     int64_t nextSerial = 0;
     int orderMode = ORDER_COLOR;
+    int broadphaseMode = BP_SAP;
 
     // introspection of the last step
     std::vector<int> colors;
@@ -1396,15 +1410,212 @@ struct World
             add(g.first, g.second);
     }
 
-    // Sweep and prune (solver.cpp:488-577) followed by the serial, ordered apply
-    // of new manifolds (solver.cpp:220-248).
+    // The broadphase of the world's mode, then the serial, ordered apply of new manifolds
+    // (solver.cpp:220-248): new manifolds get consecutive serials in emission order.
     void broadphase()
     {
         hits.clear();
         pairChecks = 0;
+        std::vector<std::pair<int, int>> created;
+        if (bodies.size() >= 2)
+        {
+            std::vector<std::vector<int>> partners;
+            partnerLists(partners);
+            if (broadphaseMode == BP_GRID)
+                broadphaseGrid(partners, created);
+            else if (broadphaseMode == BP_ALL_PAIRS)
+                broadphaseAllPairs(partners, created);
+            else
+                broadphaseSap(partners, created);
+        }
+        for (auto &pr : created)
+        {
+            Manifold<R> m{};
+            m.a = pr.first;
+            m.b = pr.second;
+            m.serial = nextSerial++;
+            m.n = 0;
+            manifolds.push_back(m);
+        }
+    }
+
+    // A bounding-sphere hit (a, b): skipped if a force already joins the pair (processSphereHit /
+    // classifyHit, solver.cpp:131-164, 202-216), else a new manifold (a, b) in emission order.
+    void classify(int a, int b, const std::vector<std::vector<int>> &partners,
+                  std::vector<std::pair<int, int>> &created)
+    {
+        bool con = partners[a].size() <= partners[b].size() ? constrained(a, b, partners) : constrained(b, a, partners);
+        hits.push_back(Hit{a, b, con ? 0 : 1});
+        if (!con)
+            created.push_back({a, b});
+    }
+
+    // generatePair (solver.cpp:166-174): one bounding-sphere test.
+    void spherePair(int a, int b, const std::vector<std::vector<int>> &partners,
+                    std::vector<std::pair<int, int>> &created)
+    {
+        ++pairChecks;
+        V3<R> dp = bodies[a].pos - bodies[b].pos;
+        R r = bodies[a].radius + bodies[b].radius;
+        if (dot(dp, dp) <= r * r)
+            classify(a, b, partners, created);
+    }
+
+    // broadphaseAllPairs (solver.cpp:176-185): every pair in list order.
+    void broadphaseAllPairs(const std::vector<std::vector<int>> &partners, std::vector<std::pair<int, int>> &created)
+    {
+        const int n = (int)bodies.size();
+        for (int a = 0; a < n; ++a)
+            for (int b = a + 1; b < n; ++b)
+                spherePair(a, b, partners, created);
+    }
+
+    // broadphaseUniformGrid (solver.cpp:284-486). Cell size twice the 95th-percentile bounding
+    // radius (at least 1e-3), grown until the grid has at most 2^22 cells; bodies no larger than
+    // half a cell are binned by centre, sorted by (cell key, list index) and tested within their
+    // own cell and the 13 forward neighbour cells; larger bodies carry a world AABB (boxes) or
+    // their bounding cube (round shapes) plus a |v| dt slack and are tested against every binned
+    // body, then against each other by the plain sphere test. Emission order: binned bodies in
+    // sorted order, each with its own-cell, forward-cell and large-body hits, then the large pairs.
+    void broadphaseGrid(const std::vector<std::vector<int>> &partners, std::vector<std::pair<int, int>> &created)
+    {
+        using S = decltype(plain_(R()));
+        const int n = (int)bodies.size();
+        std::vector<S> radii(n);
+        for (int i = 0; i < n; ++i)
+            radii[i] = plain_(bodies[i].radius);
+        const size_t pct = std::min<size_t>((size_t)n - 1, (size_t)n * 95 / 100);
+        std::nth_element(radii.begin(), radii.begin() + pct, radii.end());
+        const S binnedRadius = fmax_(radii[pct], S(1e-3f));
+        S cellSize = binnedRadius * S(2.0f);
+        struct Entry
+        {
+            uint32_t key;
+            int idx;
+            V3<S> c;
+            S r, slack;
+        };
+        struct Large
+        {
+            int idx;
+            V3<S> lo, hi;
+            S slack;
+        };
+        std::vector<Entry> entries;
+        std::vector<Large> large;
+        V3<S> bmin{S(FLT_MAX), S(FLT_MAX), S(FLT_MAX)}, bmax{S(-FLT_MAX), S(-FLT_MAX), S(-FLT_MAX)};
+        for (int i = 0; i < n; ++i)
+        {
+            const Body<R> &b = bodies[i];
+            V3<S> v{plain_(b.vel.x), plain_(b.vel.y), plain_(b.vel.z)};
+            const S slack = sqrt_(dot(v, v)) * S(plain_(p.dt));
+            const V3<S> c{plain_(b.pos.x), plain_(b.pos.y), plain_(b.pos.z)};
+            const S r = plain_(b.radius);
+            if (r > binnedRadius)
+            {
+                V3<S> h{r, r, r};
+                if (b.shape == SHAPE_BOX)
+                {
+                    const V3<S> e = V3<S>{plain_(b.size.x), plain_(b.size.y), plain_(b.size.z)} * S(0.5f);
+                    const Q<S> q{plain_(b.rot.x), plain_(b.rot.y), plain_(b.rot.z), plain_(b.rot.w)};
+                    const V3<S> cx = rotate(q, V3<S>{1, 0, 0}), cy = rotate(q, V3<S>{0, 1, 0}),
+                                cz = rotate(q, V3<S>{0, 0, 1});
+                    h = {std::fabs(cx.x) * e.x + std::fabs(cy.x) * e.y + std::fabs(cz.x) * e.z,
+                         std::fabs(cx.y) * e.x + std::fabs(cy.y) * e.y + std::fabs(cz.y) * e.z,
+                         std::fabs(cx.z) * e.x + std::fabs(cy.z) * e.y + std::fabs(cz.z) * e.z};
+                }
+                large.push_back(Large{i, c - h, c + h, slack});
+                continue;
+            }
+            bmin = {fmin_(bmin.x, c.x), fmin_(bmin.y, c.y), fmin_(bmin.z, c.z)};
+            bmax = {fmax_(bmax.x, c.x), fmax_(bmax.y, c.y), fmax_(bmax.z, c.z)};
+            entries.push_back(Entry{0u, i, c, r, slack});
+        }
+        const size_t binned = entries.size();
+        int nx = 1, ny = 1, nz = 1;
+        std::vector<uint32_t> cellStart;
+        if (binned > 0)
+        {
+            const uint64_t maxCells = 1ull << 22;
+            const V3<S> extent = bmax - bmin;
+            auto dimsFor = [&](S cs) -> uint64_t {
+                nx = (int)(extent.x / cs) + 1;
+                ny = (int)(extent.y / cs) + 1;
+                nz = (int)(extent.z / cs) + 1;
+                return (uint64_t)nx * (uint64_t)ny * (uint64_t)nz;
+            };
+            uint64_t numCells = dimsFor(cellSize);
+            while (numCells > maxCells)
+            {
+                cellSize *= fmax_(S(std::cbrt((S)numCells / (S)maxCells)), S(1.1f));
+                numCells = dimsFor(cellSize);
+            }
+            for (Entry &e : entries)
+            {
+                const int ix = (int)clampv((e.c.x - bmin.x) / cellSize, S(0.0f), (S)(nx - 1));
+                const int iy = (int)clampv((e.c.y - bmin.y) / cellSize, S(0.0f), (S)(ny - 1));
+                const int iz = (int)clampv((e.c.z - bmin.z) / cellSize, S(0.0f), (S)(nz - 1));
+                e.key = (uint32_t)(ix + nx * (iy + ny * iz));
+            }
+            std::sort(entries.begin(), entries.end(), [](const Entry &a, const Entry &b) {
+                return a.key == b.key ? a.idx < b.idx : a.key < b.key;
+            });
+            cellStart.assign((size_t)numCells, UINT32_MAX);
+            for (size_t i = 0; i < binned; ++i)
+                if (i == 0 || entries[i].key != entries[i - 1].key)
+                    cellStart[entries[i].key] = (uint32_t)i;
+        }
+        static const int forward[13][3] = {{1, 0, 0},   {-1, 1, 0}, {0, 1, 0},  {1, 1, 0},  {-1, -1, 1},
+                                           {0, -1, 1},  {1, -1, 1}, {-1, 0, 1}, {0, 0, 1},  {1, 0, 1},
+                                           {-1, 1, 1},  {0, 1, 1},  {1, 1, 1}};
+        const S reachMargin = S(4.0f * 0.01f);  // 4 * COLLISION_MARGIN (solver.h:23)
+        for (size_t i = 0; i < binned; ++i)
+        {
+            const Entry &a = entries[i];
+            auto test = [&](const Entry &b) {
+                ++pairChecks;
+                const V3<S> dp = a.c - b.c;
+                const S r = a.r + b.r;
+                if (!(dot(dp, dp) > r * r))
+                    classify(a.idx, b.idx, partners, created);
+            };
+            for (size_t j = i + 1; j < binned && entries[j].key == a.key; ++j)
+                test(entries[j]);
+            const int ix = (int)(a.key % (uint32_t)nx);
+            const uint32_t t = a.key / (uint32_t)nx;
+            const int iy = (int)(t % (uint32_t)ny), iz = (int)(t / (uint32_t)ny);
+            for (const int *d : forward)
+            {
+                const int cx = ix + d[0], cy = iy + d[1], cz = iz + d[2];
+                if (cx < 0 || cx >= nx || cy < 0 || cy >= ny || cz < 0 || cz >= nz)
+                    continue;
+                const uint32_t nkey = (uint32_t)(cx + nx * (cy + ny * cz));
+                const uint32_t s0 = cellStart[nkey];
+                if (s0 == UINT32_MAX)
+                    continue;
+                for (size_t j = s0; j < binned && entries[j].key == nkey; ++j)
+                    test(entries[j]);
+            }
+            for (const Large &l : large)
+            {
+                ++pairChecks;
+                const V3<S> q{clampv(a.c.x, l.lo.x, l.hi.x), clampv(a.c.y, l.lo.y, l.hi.y),
+                              clampv(a.c.z, l.lo.z, l.hi.z)};
+                const V3<S> dp = a.c - q;
+                const S reach = a.r + a.slack + l.slack + reachMargin;
+                if (!(dot(dp, dp) > reach * reach))
+                    classify(a.idx, l.idx, partners, created);
+            }
+        }
+        for (size_t i = 0; i < large.size(); ++i)
+            for (size_t j = i + 1; j < large.size(); ++j)
+                spherePair(large[i].idx, large[j].idx, partners, created);
+    }
+
+    // Sweep and prune (solver.cpp:488-577).
+    void broadphaseSap(const std::vector<std::vector<int>> &partners, std::vector<std::pair<int, int>> &created)
+    {
         size_t n = bodies.size();
-        if (n < 2)
-            return;
         V3<R> mean{0, 0, 0};
         for (const auto &b : bodies)
             mean += b.pos;
@@ -1439,9 +1650,6 @@ struct World
             return a.lo < b.lo;
         });
 
-        std::vector<std::vector<int>> partners;
-        partnerLists(partners);
-        std::vector<std::pair<int, int>> created;
         for (size_t i = 0; i < n; ++i)
         {
             const Iv &a = iv[i];
@@ -1455,22 +1663,8 @@ struct World
                 R r = bodies[a.idx].radius + bodies[b.idx].radius;
                 if (dot(dp, dp) > r * r)
                     continue;
-                bool con = partners[a.idx].size() <= partners[b.idx].size()
-                               ? constrained(a.idx, b.idx, partners)
-                               : constrained(b.idx, a.idx, partners);
-                hits.push_back(Hit{a.idx, b.idx, con ? 0 : 1});
-                if (!con)
-                    created.push_back({a.idx, b.idx});
+                classify(a.idx, b.idx, partners, created);
             }
-        }
-        for (auto &pr : created)
-        {
-            Manifold<R> m{};
-            m.a = pr.first;
-            m.b = pr.second;
-            m.serial = nextSerial++;
-            m.n = 0;
-            manifolds.push_back(m);
         }
     }
 

@@ -60,6 +60,7 @@ def load_oracle():
             getattr(lib, n).argtypes = [_p]
         lib.orc_set_ignores.argtypes = [_p, _p, _i]
         lib.orc_set_order.argtypes = [_p, _i]
+        lib.orc_set_broadphase.argtypes = [_p, _i]
         lib.orc_step.argtypes = [_p, _i]
         lib.orc_counts.argtypes = [_p, _p, _p, _p, _p, _p]
         lib.orc_get_state.argtypes = [_p, _p, _p, _p, _p]
@@ -171,7 +172,9 @@ class OracleWorld:
     ORDER_COLOR = 0
     ORDER_LIST = 1
 
-    def __init__(self, state: State, params, fp64: bool = False, order: int = 0):
+    BROADPHASE_ALL_PAIRS, BROADPHASE_GRID, BROADPHASE_SAP = 0, 1, 2
+
+    def __init__(self, state: State, params, fp64: bool = False, order: int = 0, broadphase: int = 2):
         lib = load_oracle()
         self._lib = lib
         self._keep = (state.bodies, state.joints, state.springs, state.manifolds)
@@ -181,6 +184,7 @@ class OracleWorld:
                                        _ptr(state.springs), len(state.springs), _ptr(state.manifolds),
                                        len(state.manifolds), C.c_int64(state.next_serial))
         lib.orc_set_order(self._h, order)
+        lib.orc_set_broadphase(self._h, broadphase)
         ig = np.ascontiguousarray(getattr(state, "ignores", np.zeros((0, 2), np.int32)), np.int32)
         self._ignores = ig
         lib.orc_set_ignores(self._h, _ptr(ig), len(ig))

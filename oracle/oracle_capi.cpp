@@ -36,6 +36,7 @@ struct WorldBase
     virtual void iterate() = 0;
     virtual void finish() = 0;
     virtual void setOrder(int mode) = 0;
+    virtual void setBroadphase(int mode) = 0;
     virtual void setIgnores(const int32_t *pairs, int n) = 0;
     virtual void counts(int32_t *nb, int32_t *nj, int32_t *ns, int32_t *nm, int64_t *serial) = 0;
     virtual void get(avbd_body *b, avbd_joint *j, avbd_spring *s, avbd_manifold *m) = 0;
@@ -62,6 +63,7 @@ struct WorldImpl : WorldBase
     void iterate() override { w.iterate(); }
     void finish() override { w.finish(); }
     void setOrder(int mode) override { w.orderMode = mode; }
+    void setBroadphase(int mode) override { w.broadphaseMode = mode; }
     void setIgnores(const int32_t *pairs, int n) override
     {
         w.ignores.clear();
@@ -432,6 +434,8 @@ void orc_f64_run(const avbd_params *p, const avbd_body *bodies, int nb, const av
     impl.getBodiesF64(out13);
 }
 void orc_set_order(void *w, int mode) { ((WorldBase *)w)->setOrder(mode); }
+/* Solver::broadphaseMode (solver.h:39-45): 0 all pairs, 1 uniform grid, 2 sweep and prune (default). */
+void orc_set_broadphase(void *w, int mode) { ((WorldBase *)w)->setBroadphase(mode); }
 /* IgnoreCollision pairs (list indices), replacing the previous set. */
 void orc_set_ignores(void *w, const int32_t *pairs, int n) { ((WorldBase *)w)->setIgnores(pairs, n); }
 void orc_step(void *w, int n) { ((WorldBase *)w)->step(n); }

@@ -23,6 +23,10 @@ Fixtures
   round_scenes.npz    the round-shape scenes of avbd.scenes (branch cases, stacks, ramps,
                       a 300-sphere pour, a 200-body mixed pile): bodies after every step
                       (every 10th step for the pour and the mixed pile)
+  grid.npz            the uniform-grid broadphase (the reference Solver's default) and the
+                      all-pairs broadphase: a box pile, a sparse field whose grid is coarsened
+                      below 2^22 cells, a mixed round-shape pile; bodies every 10th step and the
+                      pair emission (pairs, checks, hits) of the state before selected steps
 """
 from __future__ import annotations
 
@@ -251,10 +255,45 @@ def make_pile(path, count=1000, pre_steps=(60, 140, 240)):
     print(f"{path}: {count} boxes")
 
 
+def grid_scene_list():
+    """(key, scene, broadphase mode, steps, steps whose pair emission is recorded)."""
+    return [("pile", scenes.pile(800, seed=7), RefWorld.BROADPHASE_GRID, 150, (0, 60, 149)),
+            ("spread", scenes.spread(1500, seed=9), RefWorld.BROADPHASE_GRID, 90, (0, 45, 89)),
+            ("mixed", scenes.mixed_pile(250, seed=13), RefWorld.BROADPHASE_GRID, 100, (0, 50, 99)),
+            ("allpairs", scenes.mixed_pile(120, seed=14), RefWorld.BROADPHASE_ALL_PAIRS, 60, (0, 30, 59))]
+
+
+def make_grid(path):
+    out = {}
+    for key, sc, mode, steps, emit_at in grid_scene_list():
+        prm = default_params(gravity=sc.gravity)
+        w = RefWorld(prm, scene=sc)
+        assert w.set_broadphase(mode) == mode
+        traj = []
+        for k in range(steps):
+            if k in emit_at:
+                pairs, checks, hits = w.broadphase_pairs()
+                out[f"{key}.pairs{k}"] = pairs
+                out[f"{key}.checks{k}"] = np.int64(checks)
+                out[f"{key}.hits{k}"] = np.int64(hits)
+            w.step(1, colored=True)
+            if (k + 1) % 10 == 0:
+                traj.append(body_block(w.state()))
+        for name, v in scene_arrays(sc).items():
+            out[f"{key}.{name}"] = v
+        out[f"{key}.mode"] = np.int32(mode)
+        out[f"{key}.steps"] = np.int32(steps)
+        out[f"{key}.emit_at"] = np.array(emit_at, np.int32)
+        out[f"{key}.traj"] = np.stack(traj)
+        print(f"  {key}: {sc.num_bodies} bodies, mode {mode}, {steps} steps, final manifolds {len(w.state().manifolds)}")
+    np.savez_compressed(path, **out)
+    print(f"{path}: {len(grid_scene_list())} scenes")
+
+
 def main(which=None):
     load_ref()
     jobs = {"collide_boxes": make_collide, "stack": make_stack, "pyramid_chain": make_pyramid_chain, "pile": make_pile,
-            "collide_round": make_collide_round, "round_scenes": make_round_scenes}
+            "collide_round": make_collide_round, "round_scenes": make_round_scenes, "grid": make_grid}
     for name, fn in jobs.items():
         if not which or name in which:
             fn(os.path.join(HERE, f"{name}.npz"))

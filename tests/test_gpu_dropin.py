@@ -45,21 +45,27 @@ def test_hip_backend_steps_like_reference(ref, name):
     assert ref.load_ref().ref_hip_fallbacks() == 0
 
 
-def test_hip_backend_all_pairs_falls_back(ref):
-    """A Solver set to a broadphase the GPU step does not reproduce (here all pairs,
-    solver.cpp:165-176) steps the CPU reference through the adapter, counted as fallbacks,
-    so its trajectory is the CPU backend's own."""
+@pytest.mark.parametrize("mode", ["default", "all_pairs"])
+def test_hip_backend_follows_solver_broadphase_mode(ref, mode):
+    """A default-constructed reference Solver uses the uniform grid (solver.cpp:676-680); the
+    drop-in runs the Solver's own mode on the GPU (grid, or all pairs) and steps exactly like the
+    reference CPU solver in that mode (colour order), with no CPU fallback."""
     from avbd import scenes
     from avbd.state import default_params
-    sc = scenes.pile(200, seed=13)
+    sc = scenes.pile(400, seed=13) if mode == "default" else scenes.mixed_pile(150, seed=15)
     prm = default_params(gravity=sc.gravity)
     gpu = ref.RefWorld(prm, scene=sc)
     cpu = ref.RefWorld(prm, scene=sc)
+    want = ref.RefWorld.BROADPHASE_DEFAULT if mode == "default" else ref.RefWorld.BROADPHASE_ALL_PAIRS
     for w in (gpu, cpu):
-        assert w.set_broadphase(ref.RefWorld.BROADPHASE_ALL_PAIRS) == 0
+        got = w.set_broadphase(want)
+    assert got == (ref.RefWorld.BROADPHASE_GRID if mode == "default" else ref.RefWorld.BROADPHASE_ALL_PAIRS)
     assert gpu.use_hip(0)
     before = ref.load_ref().ref_hip_fallbacks()
-    gpu.step(30, colored=False)
-    cpu.step(30, colored=False)   # Solver::step with the CPU backend, list order
-    assert_states_equal(gpu.state(), cpu.state(), "all-pairs solver through hip-avbd")
-    assert ref.load_ref().ref_hip_fallbacks() - before == 30
+    done = 0
+    for s in (1, 5, 30, 80):
+        gpu.step(s - done, colored=False)   # Solver::step -> hip-avbd backend
+        cpu.step(s - done, colored=True)    # reference CPU solver, colour order
+        done = s
+        assert_states_equal(gpu.state(), cpu.state(), f"{mode} broadphase through hip-avbd, step {s}")
+    assert ref.load_ref().ref_hip_fallbacks() == before

@@ -72,7 +72,7 @@ def test_reference_benchmark_gates(hl, scene):
     want = GATE_REF[scene]
     avg_bound, gates = GATES[scene]
     print(f"headless gate {scene}: {want['warmup']} + {want['frames']} frames", flush=True)
-    host = hl.Host(named_scenes.index_of(scene))
+    host = hl.Host(named_scenes.index_of(scene), broadphase="sap")
     f, line = hl.run_benchmark(scene, frames=want["frames"], warmup_frames=want["warmup"], no_stream=True, host=host)
     state = host.world.state()
     host.close()
@@ -99,12 +99,14 @@ def ref():
 
 
 class _RefBackend:
-    """The compiled reference solver stepping in colour order (the GPU step's order)."""
+    """The compiled reference solver stepping in colour order (the GPU step's order), in the
+    broadphase mode `mode` (-1: a default-constructed Solver's, the uniform grid)."""
 
-    def __init__(self, ob, name):
+    def __init__(self, ob, name, mode=2):
         from avbd import named_scenes
         from avbd.state import default_params
         self.w = ob.RefWorld(default_params(), ref_scene=named_scenes.index_of(name))
+        self.w.set_broadphase(mode)
 
     def step(self, n):
         self.w.step(n, colored=True)
@@ -113,10 +115,14 @@ class _RefBackend:
         return self.w.state()
 
 
-@pytest.mark.parametrize("scene", ["Pyramid", "Rope", "Soft Body"])
-def test_backend_comparison_gate(hl, ref, scene):
-    print(f"backend comparison {scene}: 120 frames", flush=True)
-    f, line = hl.run_backend_comparison(scene, 120, _RefBackend(ref, scene))
+@pytest.mark.parametrize("scene,broadphase", [("Pyramid", "sap"), ("Rope", "sap"), ("Soft Body", "sap"),
+                                              ("Pyramid", None), ("Soft Body", None)])
+def test_backend_comparison_gate(hl, ref, scene, broadphase):
+    """The reference's comparison gate (zero error) with sweep and prune, and with no --broadphase,
+    where both solvers keep the reference Solver's default uniform grid."""
+    print(f"backend comparison {scene} ({broadphase or 'default grid'}): 120 frames", flush=True)
+    f, line = hl.run_backend_comparison(scene, 120, _RefBackend(ref, scene, 2 if broadphase == "sap" else -1),
+                                        broadphase=broadphase)
     print(line, flush=True)
     assert list(json.loads(line)) == KEYS["backendComparison"]
     assert f["bodyCountMismatch"] == 0 and f["comparedBodies"] == f["referenceBodies"] > 0
@@ -162,6 +168,7 @@ def test_cli_line_and_avbs_stream(hl, tmp_path):
     # the last frame is the final state: compare with the same 30 steps stepped here
     st = build_state(named_scenes.build("Newton's Cradle"), capi.Maker())
     w = capi.HipWorld.for_state(st, default_params())
+    w.set_broadphase(capi.HipWorld.BROADPHASE_GRID)  # the command line's default, as the reference's
     w.step(30)
     w.synchronize()
     expect = avbs.encode_avbs(w.state().bodies, 29, "Newton's Cradle")

@@ -137,7 +137,14 @@ def test_compare_worlds():
 def test_cli_errors():
     assert hl.main(["--benchmark-scene", "no such scene", "--no-stream"]) == 2
     assert hl.main(["--benchmark-scene", "Pyramid", "--physics-backend", "webgpu-avbd"]) == 2
-    assert hl.main(["--benchmark-scene", "Pyramid", "--broadphase", "grid"]) == 2
     assert hl.main(["--benchmark-scene", "Pyramid", "--collision-only"]) == 2
     assert hl.main(["--compare-scene", "Rope"]) == 2
     assert hl.main([]) == 2
+
+
+def test_broadphase_names_follow_the_reference():
+    """applyBroadphaseMode (headless_server.cpp:579-591); no flag keeps the Solver's default grid."""
+    assert hl.broadphase_mode(None) == 1
+    assert [hl.broadphase_mode(n) for n in ("allpairs", "grid", "uniformgrid", "spatialhash", "sap",
+                                             "sweepandprune")] == [0, 1, 1, 1, 2, 2]
+    assert hl.broadphase_mode("octree") == 1

@@ -117,3 +117,29 @@ def test_record_builders_match_reference_constructors(oracle):
         np.testing.assert_array_equal(a.bodies[f], b.bodies[f], err_msg=f)
     np.testing.assert_array_equal(a.joints, b.joints)
     np.testing.assert_array_equal(a.springs, b.springs)
+
+
+@pytest.mark.parametrize("key", ["pile", "spread", "mixed", "allpairs"])
+def test_grid_and_all_pairs_broadphase_match_reference(oracle, key):
+    """The uniform-grid broadphase (the reference Solver's default mode, solver.cpp:284-486) and the
+    all-pairs broadphase (solver.cpp:176-185): the pair emission (pairs, checks, hits) before the
+    recorded steps and the bodies every 10th step equal the reference's. `spread` is sparse enough
+    that the grid's cell size is grown below 2^22 cells (solver.cpp:336-344)."""
+    z = load_golden("grid.npz")
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scene_from_golden(z, f"{key}.")
+    prm = default_params(gravity=sc.gravity)
+    w = oracle.OracleWorld(scenes.build_state(sc, oracle.OracleMaker()), prm, broadphase=int(z[f"{key}.mode"]))
+    emit_at = set(int(k) for k in z[f"{key}.emit_at"])
+    traj = z[f"{key}.traj"]
+    for k in range(int(z[f"{key}.steps"])):
+        w.step(1)
+        if k in emit_at:
+            hits = w.hits()
+            np.testing.assert_array_equal(hits[:, :2], z[f"{key}.pairs{k}"], err_msg=f"{key} step {k + 1}: pairs")
+            assert w.pair_checks == int(z[f"{key}.checks{k}"]), f"{key} step {k + 1}: checks"
+            assert len(hits) == int(z[f"{key}.hits{k}"]), f"{key} step {k + 1}: hits"
+        if (k + 1) % 10 == 0:
+            np.testing.assert_array_equal(body_block(w.state()), traj[(k + 1) // 10 - 1],
+                                          err_msg=f"{key} step {k + 1}: bodies")
