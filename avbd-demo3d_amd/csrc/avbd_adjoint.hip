@@ -159,27 +159,43 @@ struct COut
     double mu;
 };
 
+// The contact's forward values at which the reversal linearises: exactly those the FP32 step
+// computed (contact_eval, manifold.cpp:176-229), promoted to FP64. The inputs are FP32 values held
+// in FP64, so converting back is exact. Recomputing them in FP64 instead would not be the step's
+// linearisation point: pose differences such as x - x- are small against a position's rounding,
+// and F = k C + lambda multiplies a one-ulp change of C by penalties of up to 1e10, so the force
+// values that scale the friction cone and the friction derivative would differ by percents.
+__device__ __forceinline__ V3 f3(DV v) { return V3{(float)v.x, (float)v.y, (float)v.z}; }
+__device__ __forceinline__ Q4 f4(DQ q) { return Q4{(float)q.x, (float)q.y, (float)q.z, (float)q.w}; }
 __device__ __forceinline__ void cfwd(const CIn &in, CFwd &f)
 {
-    f.rAW = drot(in.qA, in.rA);
-    f.rBW = drot(in.qB, in.rB);
-    f.dqAL = in.pA - in.iLA;
-    f.dqBL = in.pB - in.iLB;
-    f.dqAA = dqdiff(in.qA, in.iAA);
-    f.dqBA = dqdiff(in.qB, in.iAB);
-    for (int k = 0; k < 3; ++k)
-    {
-        DV n = in.n[k];
-        f.C[k] = dcomp(in.C0, k) * in.oma + ddot(n, f.dqAL) - ddot(n, f.dqBL) + ddot(dcross(f.rAW, n), f.dqAA) -
-                 ddot(dcross(f.rBW, n), f.dqBA);
-        f.F0[k] = dcomp(in.K, k) * f.C[k] + dcomp(in.lam, k);
-    }
+    const Q4 qA = f4(in.qA), qB = f4(in.qB);
+    const V3 rAW = rotate(qA, f3(in.rA)), rBW = rotate(qB, f3(in.rB));
+    const V3 dqAL = f3(in.pA) - f3(in.iLA), dqBL = f3(in.pB) - f3(in.iLB);
+    const V3 dqAA = qdiff(qA, f4(in.iAA)), dqBA = qdiff(qB, f4(in.iAB));
+    const V3 b0 = f3(in.n[0]), b1 = f3(in.n[1]), b2 = f3(in.n[2]);
+    const V3 pen = f3(in.K), lam = f3(in.lam);
+    const ContactEval ev = contact_eval(f3(in.C0), pen, lam, (float)in.mu, (float)in.oma, b0, b1, b2, -b0, -b1, -b2,
+                                        cross(rAW, b0), cross(rAW, b1), cross(rAW, b2), cross(rBW, -b0),
+                                        cross(rBW, -b1), cross(rBW, -b2), dqAL, dqBL, dqAA, dqBA);
+    f.rAW = dv(rAW);
+    f.rBW = dv(rBW);
+    f.dqAL = dv(dqAL);
+    f.dqBL = dv(dqBL);
+    f.dqAA = dv(dqAA);
+    f.dqBA = dv(dqBA);
+    f.C[0] = ev.C0;
+    f.C[1] = ev.C1;
+    f.C[2] = ev.C2;
+    // unclamped, unscaled forces as the step formed them
+    f.F0[0] = pen.x * ev.C0 + lam.x;
+    f.F0[1] = pen.y * ev.C1 + lam.y;
+    f.F0[2] = pen.z * ev.C2 + lam.z;
     f.F[0] = in.clamp0 ? 0.0 : f.F0[0];
-    f.bounds = fabs(f.F[0]) * in.mu;
-    f.fs = sqrt(f.F0[1] * f.F0[1] + f.F0[2] * f.F0[2]);
-    double sc = in.scaled ? f.bounds / f.fs : 1.0;
-    f.F[1] = f.F0[1] * sc;
-    f.F[2] = f.F0[2] * sc;
+    f.bounds = ev.bounds;
+    f.fs = ev.fs;
+    f.F[1] = in.scaled ? (double)ev.F1 : f.F0[1];
+    f.F[2] = in.scaled ? (double)ev.F2 : f.F0[2];
 }
 
 // Given F_bar, an extra C_bar and pre-accumulated rAW_bar / rBW_bar / n_bar (Jacobian terms of a
@@ -612,9 +628,13 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
         S.rAb[2] = rAb.z;
         // inertia: H_LL += m/dt^2 I, H_AA += diag(I)/dt^2, r_L += m/dt^2 (x - xt), r_A += I/dt^2 o (q - qt)
         double idt2 = 1.0 / ((double)P.dt * (double)P.dt);
-        DV dld = dv(pos) - dv4(B.inerLin[i]);
+        // the inertial residuals with the values the primal used: x - xt and q (-) qt rounded in
+        // FP32 (for a resting body x - xt is ~1e-4 m against a position ulp of ~5e-7 m, so the
+        // exact difference of the FP32 poses would differ from the primal's by ~1e-2 relative in
+        // the mass derivative, which these residuals multiply)
+        DV dld = dv(pos - ld3(B.inerLin[i]));
         DQ inA = dq4(B.inerAng[i]);
-        DV dad = dqdiff(q, inA);
+        DV dad = dv(qdiff(ldq(rot4), ldq(B.inerAng[i])));
         double mbar = idt2 * (-(w6[0] * dx6[0] + w6[1] * dx6[1] + w6[2] * dx6[2]) + ddot(rLb, dld));
         double Ib[3];
         for (int c = 0; c < 3; ++c)
@@ -985,6 +1005,28 @@ struct Snapshot
 
 } // namespace
 
+// One step of a checkpoint segment, as its reversal reads it: the tape (poses and penalty/lambda
+// after every iteration, each body's system and dx, branch flags), the narrowphase/compaction
+// recording, and copies of the handle's arrays as the step left them before finishStep (bodies,
+// manifolds, the flat contact list, stamps, colour-sorted tasks, colours, degrees) with the
+// per-step counters and colour offsets the host read. Filled while the segment is replayed, so
+// each step is computed once in the backward pass.
+struct StepStore
+{
+    float4 *tapePose = nullptr, *tapeContact = nullptr, *tapeSys = nullptr;
+    unsigned char *tapeStampFlags = nullptr, *tapeDualFlags = nullptr;
+    int *recMatch = nullptr, *recStick = nullptr, *recSrc = nullptr;
+    float4 *pose = nullptr, *vel = nullptr, *angVel = nullptr, *prevVel = nullptr, *inerLin = nullptr, *inerAng = nullptr;
+    int2 *ab = nullptr;
+    long long *serial = nullptr;
+    int *count = nullptr, *feature = nullptr;
+    float4 *cr = nullptr;
+    int4 *contactRec = nullptr, *stamps = nullptr, *tasks = nullptr;
+    int *color = nullptr, *deg = nullptr;
+    Counters ctr{};
+    std::vector<int> offsets;
+};
+
 struct AdjointState
 {
     int steps = 0, every = 1;
@@ -993,11 +1035,8 @@ struct AdjointState
     Snapshot end;
     AdjBuf A{};
     int slotCap = 0;
-    // tape of the step being reversed and the narrowphase/compaction recording (the handle's
-    // pointers of the same names are set only while that step is recomputed)
-    float4 *tapePose = nullptr, *tapeContact = nullptr, *tapeSys = nullptr;
-    unsigned char *tapeStampFlags = nullptr, *tapeDualFlags = nullptr;
-    int *recMatch = nullptr, *recStick = nullptr, *recSrc = nullptr;
+    int tapeIters = 0;            // iterations the step stores' tapes hold
+    std::vector<StepStore> store; // one per step of a checkpoint segment
 };
 
 namespace
@@ -1061,17 +1100,39 @@ int adj_alloc(avbd_handle *h, int steps, int every)
     }
     if (!S.end.pose && (r = snap_alloc(h, S.end)))
         return r;
-    if (!S.tapePose)
+    const int N = h->maxBodies, M = h->maxManifolds, I = std::max(h->P.iterations, 1);
+    S.slotCap = M * kMaxContacts;
+    const size_t C = (size_t)S.slotCap;
+    // step stores, sized for the current iteration count (more iterations later: a larger set)
+    if (I > S.tapeIters)
     {
-        int N = h->maxBodies, M = h->maxManifolds, I = 64;
-        S.slotCap = M * kMaxContacts;
-        size_t C = (size_t)S.slotCap;
+        S.store.clear();
+        S.tapeIters = I;
+    }
+    while ((int)S.store.size() < every)
+    {
+        S.store.emplace_back();
+        StepStore &t = S.store.back();
+        t.offsets.assign(257, 0);
+        if ((r = h->alloc(t.tapePose, (size_t)(I + 1) * 2 * N)) || (r = h->alloc(t.tapeSys, (size_t)I * 9 * N)) ||
+            (r = h->alloc(t.tapeStampFlags, (size_t)I * h->maxStamps)) ||
+            (r = h->alloc(t.tapeDualFlags, (size_t)I * C)) || (r = h->alloc(t.tapeContact, (size_t)(I + 1) * 2 * C)) ||
+            (r = h->alloc(t.recMatch, C)) || (r = h->alloc(t.recStick, C)) || (r = h->alloc(t.recSrc, M)) ||
+            (r = h->alloc(t.pose, 4 * (size_t)N)) || (r = h->alloc(t.vel, N)) || (r = h->alloc(t.angVel, N)) ||
+            (r = h->alloc(t.prevVel, N)) || (r = h->alloc(t.inerLin, N)) || (r = h->alloc(t.inerAng, N)) ||
+            (r = h->alloc(t.ab, M)) || (r = h->alloc(t.serial, M)) || (r = h->alloc(t.count, M)) ||
+            (r = h->alloc(t.feature, C)) || (r = h->alloc(t.cr, C * kCR)) || (r = h->alloc(t.contactRec, C)) ||
+            (r = h->alloc(t.stamps, h->maxStamps)) || (r = h->alloc(t.tasks, N)) || (r = h->alloc(t.color, N)) ||
+            (r = h->alloc(t.deg, N)))
+        {
+            S.store.pop_back();
+            return r;
+        }
+    }
+    if (!S.A.xb)
+    {
         AdjBuf &A = S.A;
-        if ((r = h->alloc(S.tapePose, (size_t)(I + 1) * 2 * N)) || (r = h->alloc(S.tapeSys, (size_t)I * 9 * N)) ||
-            (r = h->alloc(S.tapeStampFlags, (size_t)I * h->maxStamps)) ||
-            (r = h->alloc(S.tapeDualFlags, (size_t)I * C)) ||
-            (r = h->alloc(S.tapeContact, (size_t)(I + 1) * 2 * C)) || (r = h->alloc(S.recMatch, C)) ||
-            (r = h->alloc(S.recStick, C)) || (r = h->alloc(S.recSrc, M)) || (r = h->alloc(A.xb, 3 * (size_t)N)) ||
+        if ((r = h->alloc(A.xb, 3 * (size_t)N)) ||
             (r = h->alloc(A.qb, 4 * (size_t)N)) || (r = h->alloc(A.vb, 3 * (size_t)N)) ||
             (r = h->alloc(A.wb, 3 * (size_t)N)) || (r = h->alloc(A.pvb, 3 * (size_t)N)) ||
             (r = h->alloc(A.ilb, 3 * (size_t)N)) || (r = h->alloc(A.iab, 4 * (size_t)N)) ||
@@ -1082,73 +1143,106 @@ int adj_alloc(avbd_handle *h, int steps, int every)
             (r = h->alloc(A.ckb, 3 * C)) || (r = h->alloc(A.clb, 3 * C)) || (r = h->alloc(A.crAb, 3 * C)) ||
             (r = h->alloc(A.crBb, 3 * C)))
         {
-            S.tapePose = nullptr;
+            A.xb = nullptr;
             return r;
         }
     }
     return AVBD_OK;
 }
 
-// One step recomputed with its iterations taped (no finishStep: the reversal reads the step's
-// start velocities, which finishStep would overwrite).
-int taped_recompute(avbd_handle *h)
+// One step computed with its iterations taped into the store t, which then receives copies of the
+// arrays its reversal reads as the step left them before finishStep (finishStep overwrites the
+// start velocities the reversal needs); the caller runs finishStep afterwards to go on.
+int taped_step(avbd_handle *h, StepStore &t)
 {
-    AdjointState &S = *h->adj;
-    h->tapePose = S.tapePose;
-    h->tapeContact = S.tapeContact;
-    h->tapeSys = S.tapeSys;
-    h->tapeStampFlags = S.tapeStampFlags;
-    h->tapeDualFlags = S.tapeDualFlags;
-    h->recMatch = S.recMatch;
-    h->recStick = S.recStick;
-    h->recSrc = S.recSrc;
+    h->tapePose = t.tapePose;
+    h->tapeContact = t.tapeContact;
+    h->tapeSys = t.tapeSys;
+    h->tapeStampFlags = t.tapeStampFlags;
+    h->tapeDualFlags = t.tapeDualFlags;
+    h->recMatch = t.recMatch;
+    h->recStick = t.recStick;
+    h->recSrc = t.recSrc;
     int r = enqueue_prepare(h);
     if (!r)
         r = enqueue_iterate(h);
     h->tapePose = h->tapeContact = h->tapeSys = nullptr;
     h->tapeStampFlags = h->tapeDualFlags = nullptr;
     h->recMatch = h->recStick = h->recSrc = nullptr;
-    return r;
+    if (r)
+        return r;
+    // enqueue_iterate read the counters and colour offsets back before its launches
+    t.ctr = *h->hostCtr;
+    std::memcpy(t.offsets.data(), h->hostOffsets, sizeof(int) * 257);
+    hipStream_t st = h->stream;
+    const size_t N = h->nb, M = t.ctr.numManifolds, C = M * kMaxContacts;
+    auto cp = [&](void *dst, const void *src, size_t bytes) {
+        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st) : hipSuccess;
+    };
+    hipError_t e;
+    if ((e = cp(t.pose, h->B.pose, 4 * N * sizeof(float4))) || (e = cp(t.vel, h->B.vel, N * sizeof(float4))) ||
+        (e = cp(t.angVel, h->B.angVel, N * sizeof(float4))) || (e = cp(t.prevVel, h->B.prevVel, N * sizeof(float4))) ||
+        (e = cp(t.inerLin, h->B.inerLin, N * sizeof(float4))) || (e = cp(t.inerAng, h->B.inerAng, N * sizeof(float4))) ||
+        (e = cp(t.ab, h->cur.ab, M * sizeof(int2))) || (e = cp(t.serial, h->cur.serial, M * sizeof(long long))) ||
+        (e = cp(t.count, h->cur.count, M * sizeof(int))) || (e = cp(t.feature, h->cur.feature, C * sizeof(int))) ||
+        (e = cp(t.cr, h->cur.cr, C * kCR * sizeof(float4))) ||
+        (e = cp(t.contactRec, h->S.contactRec, (size_t)t.ctr.contactCount * sizeof(int4))) ||
+        (e = cp(t.stamps, h->S.stamps, (size_t)std::min(t.ctr.numStamps, h->maxStamps) * sizeof(int4))) ||
+        (e = cp(t.tasks, h->S.tasks, N * sizeof(int4))) || (e = cp(t.color, h->S.color, N * sizeof(int))) ||
+        (e = cp(t.deg, h->S.deg, N * sizeof(int))))
+        return fail(AVBD_ERR_HIP, std::string("adjoint step store: ") + hipGetErrorString(e));
+    return AVBD_OK;
 }
 
-int reverse_step(avbd_handle *h, const Snapshot &start)
+// The reverse sweep of one step from its store t; start: the step's start state (the previous
+// step's final manifolds, which the force init's warm start read).
+int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
 {
     hipStream_t s = h->stream;
     AdjointState &S = *h->adj;
     AdjBuf &A = S.A;
-    const Counters &c = *h->hostCtr;
+    const Counters &c = t.ctr;
     const int nb = h->nb, nc = c.contactCount, nm = c.numManifolds, I = h->P.iterations;
     const int colors = std::min(c.colorCount, kMaxColors);
     const size_t ps = 2 * (size_t)h->maxBodies, cs = 2 * (size_t)S.slotCap;
-    const float4 *tp = S.tapePose, *tc = S.tapeContact;
+    const float4 *tp = t.tapePose, *tc = t.tapeContact;
+    // the handle's views with the step's arrays in place of the live ones
+    Bodies B = h->B;
+    B.pose = t.pose;
+    B.vel = t.vel;
+    B.angVel = t.angVel;
+    B.prevVel = t.prevVel;
+    B.inerLin = t.inerLin;
+    B.inerAng = t.inerAng;
+    Manifolds M{t.ab, t.serial, t.count, t.feature, t.cr};
     int n0 = std::max(std::max(nb, nc), std::max(nm, 1));
-    k_adj_begin<<<ceil_div(n0, 256), 256, 0, s>>>(h->S.contactRec, nc, nm, nb, A);
+    k_adj_begin<<<ceil_div(n0, 256), 256, 0, s>>>(t.contactRec, nc, nm, nb, A);
     k_adj_clear_carry<<<ceil_div(3 * S.slotCap, 256), 256, 0, s>>>(3 * S.slotCap, A);
-    k_adj_finish<<<ceil_div(nb, 256), 256, 0, s>>>(h->B, h->P, nb, A);
+    k_adj_finish<<<ceil_div(nb, 256), 256, 0, s>>>(B, h->P, nb, A);
     for (int it = I - 1; it >= 0; --it)
     {
         const float4 *prev = tp + it * ps, *cur = tp + (it + 1) * ps, *lam = tc + it * cs;
-        const float4 *sys = S.tapeSys + (size_t)it * 9 * h->maxBodies;
-        const unsigned char *sfl = S.tapeStampFlags + (size_t)it * h->maxStamps;
-        const unsigned char *dfl = S.tapeDualFlags + (size_t)it * S.slotCap;
+        const float4 *sys = t.tapeSys + (size_t)it * 9 * h->maxBodies;
+        const unsigned char *sfl = t.tapeStampFlags + (size_t)it * h->maxStamps;
+        const unsigned char *dfl = t.tapeDualFlags + (size_t)it * S.slotCap;
         if (nc > 0)
-            k_adj_dual<<<ceil_div(nc, 64), 64, 0, s>>>(h->B, h->cur, h->S.contactRec, nc, cur, lam, dfl, h->P, A);
+            k_adj_dual<<<ceil_div(nc, 64), 64, 0, s>>>(B, M, t.contactRec, nc, cur, lam, dfl, h->P, A);
         for (int col = colors - 1; col >= 0; --col)
         {
-            int b = h->hostOffsets[col], e = h->hostOffsets[col + 1];
+            int b = t.offsets[col], e = t.offsets[col + 1];
             if (e > b)
                 k_adj_primal<<<ceil_div(e - b, kAdjThreads / kAdjLanes), kAdjThreads, 0, s>>>(
-                    h->B, h->cur, h->S.stamps, h->S.tasks, b, e, h->S.color, col, prev, cur, lam, sys, sfl, h->P, A);
+                    B, M, t.stamps, t.tasks, b, e, t.color, col, prev, cur, lam, sys, sfl, h->P, A);
         }
     }
-    k_adj_warmstart<<<ceil_div(nb, 256), 256, 0, s>>>(h->B, h->P, h->S.deg, nb, A);
+    k_adj_warmstart<<<ceil_div(nb, 256), 256, 0, s>>>(B, h->P, t.deg, nb, A);
     if (nc > 0)
-        k_adj_init<<<ceil_div(nc, 256), 256, 0, s>>>(h->B, h->cur, h->S.contactRec, nc, S.recSrc, S.recMatch,
-                                                     S.recStick, start.cr, start.numManifolds, h->P, A);
+        k_adj_init<<<ceil_div(nc, 256), 256, 0, s>>>(B, M, t.contactRec, nc, t.recSrc, t.recMatch, t.recStick,
+                                                     start.cr, start.numManifolds, h->P, A);
     if (nm > 0)
     {
-        k_adj_collide<<<ceil_div(nm * 14, 64), 64, 0, s>>>(h->B, h->cur, nm, A);
-        k_adj_friction<<<ceil_div(nm, 256), 256, 0, s>>>(h->B, h->cur, nm, A);
+        k_adj_collide<<<ceil_div(nm * 14, 64), 64, 0, s>>>(B, M, nm, A);
+        k_adj_friction<<<ceil_div(nm, 256), 256, 0, s>>>(B, M, nm, A);
     }
     HIP_TRY(hipGetLastError());
     return AVBD_OK;
@@ -1227,24 +1321,24 @@ int avbd_adjoint_backward(avbd_handle *h, const double *d_pos, const double *d_r
         return fail(AVBD_ERR_HIP, std::string("adjoint init: ") + hipGetErrorString(e));
     const int K = S.every, steps = S.steps;
     const int nseg = (steps + K - 1) / K;
+    if (h->P.iterations > S.tapeIters && (r = adj_alloc(h, steps, K)))
+        return r;
     for (int sg = nseg - 1; sg >= 0; --sg)
     {
         const int s0 = sg * K, s1 = std::min(s0 + K, steps);
-        // replay the segment from its checkpoint, keeping every step's start state
-        if ((r = snap_copy(h, false, S.ck[sg])) || (r = snap_copy(h, true, S.seg[0])))
+        // replay the segment from its checkpoint once, taping every step into its store and
+        // keeping its start state (the previous step's manifolds) for the force init's reversal
+        if ((r = snap_copy(h, false, S.ck[sg])))
             return r;
-        for (int st = s0; st < s1 - 1; ++st)
+        for (int st = s0; st < s1; ++st)
         {
-            if ((r = enqueue_prepare(h)) || (r = enqueue_iterate(h)) || (r = enqueue_finish(h)) ||
-                (r = snap_copy(h, true, S.seg[st - s0 + 1])))
+            if ((r = snap_copy(h, true, S.seg[st - s0])) || (r = taped_step(h, S.store[st - s0])) ||
+                (st + 1 < s1 && (r = enqueue_finish(h))))
                 return r;
         }
         for (int st = s1 - 1; st >= s0; --st)
-        {
-            Snapshot &start = S.seg[st - s0];
-            if ((r = snap_copy(h, false, start)) || (r = taped_recompute(h)) || (r = reverse_step(h, start)))
+            if ((r = reverse_step(h, S.store[st - s0], S.seg[st - s0])))
                 return r;
-        }
     }
     // leave the handle at the forward's final state
     if ((r = snap_copy(h, false, S.end)))

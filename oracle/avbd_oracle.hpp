@@ -155,6 +155,66 @@ template <class V> inline V plain_(DualT<V> x) { return x.v; }
 template <class V> inline bool isfinite_(DualT<V> x) { return std::isfinite(x.v); }
 template <class V> inline bool isinf_(DualT<V> x) { return std::isinf(x.v); }
 
+// ---- frozen-branch scalars: an FP32 shadow that decides, an FP64 value that computes ----
+// Every arithmetic operation is carried out twice: in FP32 on the shadow s (exactly the operations
+// of Real = float) and in FP64 on the value v; every comparison, clamp, sign and stick decision
+// reads the shadow. A World<ShadowT> therefore takes the FP32 trajectory's branches at every step
+// (contact sets, features, clamps, penalty saturation, stick flags, colours) while v is the FP64
+// evaluation of that same piecewise-smooth map. Perturbing only v (the initial velocities, masses
+// or friction) leaves s and so every branch unchanged: central differences of v are then
+// differences of one smooth map -- the map the adjoint differentiates -- and converge as h -> 0.
+struct ShadowT
+{
+    float s;
+    double v;
+    ShadowT() = default;
+    constexpr ShadowT(float x) : s(x), v((double)x) {}
+    constexpr ShadowT(double x) : s((float)x), v(x) {}
+    constexpr ShadowT(int x) : s((float)x), v((double)x) {}
+    constexpr ShadowT(size_t x) : s((float)x), v((double)x) {}
+    constexpr ShadowT(float sx, double vx, int) : s(sx), v(vx) {}
+};
+inline ShadowT operator+(ShadowT a, ShadowT b) { return ShadowT(a.s + b.s, a.v + b.v, 0); }
+inline ShadowT operator-(ShadowT a, ShadowT b) { return ShadowT(a.s - b.s, a.v - b.v, 0); }
+inline ShadowT operator-(ShadowT a) { return ShadowT(-a.s, -a.v, 0); }
+inline ShadowT operator*(ShadowT a, ShadowT b) { return ShadowT(a.s * b.s, a.v * b.v, 0); }
+inline ShadowT operator/(ShadowT a, ShadowT b) { return ShadowT(a.s / b.s, a.v / b.v, 0); }
+inline ShadowT &operator+=(ShadowT &a, ShadowT b) { return a = a + b; }
+inline ShadowT &operator-=(ShadowT &a, ShadowT b) { return a = a - b; }
+inline ShadowT &operator*=(ShadowT &a, ShadowT b) { return a = a * b; }
+inline ShadowT &operator/=(ShadowT &a, ShadowT b) { return a = a / b; }
+inline bool operator<(ShadowT a, ShadowT b) { return a.s < b.s; }
+inline bool operator>(ShadowT a, ShadowT b) { return a.s > b.s; }
+inline bool operator<=(ShadowT a, ShadowT b) { return a.s <= b.s; }
+inline bool operator>=(ShadowT a, ShadowT b) { return a.s >= b.s; }
+inline bool operator==(ShadowT a, ShadowT b) { return a.s == b.s; }
+inline bool operator!=(ShadowT a, ShadowT b) { return a.s != b.s; }
+#define AVBD_SHADOW_MIXED(OP)                                                                              \
+    template <class S, class = typename std::enable_if<std::is_arithmetic<S>::value>::type>                  \
+    inline auto operator OP(ShadowT a, S b)->decltype(a OP ShadowT(b))                                     \
+    {                                                                                                      \
+        return a OP ShadowT(b);                                                                            \
+    }                                                                                                      \
+    template <class S, class = typename std::enable_if<std::is_arithmetic<S>::value>::type>                  \
+    inline auto operator OP(S a, ShadowT b)->decltype(ShadowT(a) OP b)                                     \
+    {                                                                                                      \
+        return ShadowT(a) OP b;                                                                            \
+    }
+AVBD_SHADOW_MIXED(+)
+AVBD_SHADOW_MIXED(-)
+AVBD_SHADOW_MIXED(*)
+AVBD_SHADOW_MIXED(/)
+AVBD_SHADOW_MIXED(<)
+AVBD_SHADOW_MIXED(>)
+AVBD_SHADOW_MIXED(<=)
+AVBD_SHADOW_MIXED(>=)
+#undef AVBD_SHADOW_MIXED
+inline ShadowT sqrt_(ShadowT x) { return ShadowT(sqrtf(x.s), std::sqrt(x.v), 0); }
+inline ShadowT abs_(ShadowT x) { return x.s < 0.0f ? -x : x; }
+inline float plain_(ShadowT x) { return x.s; }
+inline bool isfinite_(ShadowT x) { return std::isfinite(x.s); }
+inline bool isinf_(ShadowT x) { return std::isinf(x.s); }
+
 // ---- vector algebra (maths.h:117-165, 317-406) ----
 template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
@@ -281,6 +341,52 @@ inline void solve6(const M3<R> &aLin, const M3<R> &aAng, const M3<R> &aCross, V3
     xLin[2] = z3 - L43 * xAng[0] - L53 * xAng[1] - L63 * xAng[2];
     xLin[1] = z2 - L32 * xLin[2] - L42 * xAng[0] - L52 * xAng[1] - L62 * xAng[2];
     xLin[0] = z1 - L21 * xLin[1] - L31 * xLin[2] - L41 * xAng[0] - L51 * xAng[1] - L61 * xAng[2];
+}
+
+// Forward mode through the solve: the values are solve6's own (the scalar's value operations),
+// the tangent is that of the exact solution x = H^-1 b at those values, dx = H^-1 (db - dH x),
+// evaluated in FP64. Differentiating the FP32 elimination operation by operation instead would
+// add the FP32 factors' rounding times the condition number of H (1e6 and more for stiff contacts)
+// to the tangent; the adjoint's transposed solve is the exact H^-1 as well.
+template <class V>
+inline void solve6(const M3<DualT<V>> &aLin, const M3<DualT<V>> &aAng, const M3<DualT<V>> &aCross,
+                   V3<DualT<V>> bLin, V3<DualT<V>> bAng, V3<DualT<V>> &xLin, V3<DualT<V>> &xAng)
+{
+    M3<V> vl, va, vc;
+    M3<double> H[3], dH[3];  // lin, ang, cross in FP64: values and tangents
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+        {
+            vl[i][j] = aLin[i][j].v, va[i][j] = aAng[i][j].v, vc[i][j] = aCross[i][j].v;
+            H[0][i][j] = (double)aLin[i][j].v, H[1][i][j] = (double)aAng[i][j].v, H[2][i][j] = (double)aCross[i][j].v;
+            dH[0][i][j] = aLin[i][j].d, dH[1][i][j] = aAng[i][j].d, dH[2][i][j] = aCross[i][j].d;
+        }
+    V3<V> xl, xa;
+    solve6<V>(vl, va, vc, V3<V>{bLin.x.v, bLin.y.v, bLin.z.v}, V3<V>{bAng.x.v, bAng.y.v, bAng.z.v}, xl, xa);
+    // the symmetric 6x6 matrix from the lower triangle solve6 reads: rows 0-2 linear, 3-5 angular
+    auto entry = [](const M3<double> *m, int r, int c) {
+        if (c > r)
+        {
+            int t = r;
+            r = c;
+            c = t;
+        }
+        return r < 3 ? m[0][r][c] : (c < 3 ? m[2][r - 3][c] : m[1][r - 3][c - 3]);
+    };
+    const double x[6] = {(double)xl.x, (double)xl.y, (double)xl.z, (double)xa.x, (double)xa.y, (double)xa.z};
+    const double db[6] = {bLin.x.d, bLin.y.d, bLin.z.d, bAng.x.d, bAng.y.d, bAng.z.d};
+    double rhs[6];
+    for (int r = 0; r < 6; ++r)
+    {
+        double acc = db[r];
+        for (int c = 0; c < 6; ++c)
+            acc -= entry(dH, r, c) * x[c];
+        rhs[r] = acc;
+    }
+    V3<double> dxl, dxa;
+    solve6<double>(H[0], H[1], H[2], V3<double>{rhs[0], rhs[1], rhs[2]}, V3<double>{rhs[3], rhs[4], rhs[5]}, dxl, dxa);
+    xLin = V3<DualT<V>>{mk<V>(xl.x, dxl.x), mk<V>(xl.y, dxl.y), mk<V>(xl.z, dxl.z)};
+    xAng = V3<DualT<V>>{mk<V>(xa.x, dxa.x), mk<V>(xa.y, dxa.y), mk<V>(xa.z, dxa.z)};
 }
 
 // ---- constants (solver.h:21-24, collide.cpp:18-22) ----

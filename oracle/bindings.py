@@ -61,6 +61,7 @@ def load_oracle():
         lib.orc_set_ignores.argtypes = [_p, _p, _i]
         lib.orc_set_order.argtypes = [_p, _i]
         lib.orc_set_broadphase.argtypes = [_p, _i]
+        lib.orc_shadow_run.argtypes = [_p, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 6
         lib.orc_step.argtypes = [_p, _i]
         lib.orc_counts.argtypes = [_p, _p, _p, _p, _p, _p]
         lib.orc_get_state.argtypes = [_p, _p, _p, _p, _p]
@@ -495,6 +496,25 @@ def oracle_tangent_run(state: State, params, steps: int, dvel=None, dangvel=None
                         C.c_int64(state.next_serial), steps, *[_ptr(a) for a in args], val.ctypes.data_as(_p),
                         tan.ctypes.data_as(_p))
     return val, tan
+
+
+def oracle_shadow_run(state: State, params, steps: int, vel=None, angvel=None, mass=None, friction=None):
+    """Frozen-branch FP64 run: every decision (contact sets, clamps, stick flags, colours) is the
+    FP32 trajectory's of the unperturbed state, the arithmetic is FP64 with the given overrides.
+    Returns (values, shadows), each (bodies, 13): pos3 rot4 vel3 angvel3; the shadows are the FP32
+    trajectory of the unperturbed state."""
+    lib = load_oracle()
+    nb = len(state.bodies)
+    if len(state.joints) or len(state.springs):
+        raise ValueError("the frozen-branch run covers contact-only worlds")
+    prm = np.ascontiguousarray(params)
+    out = np.zeros((nb, 13))
+    sh = np.zeros((nb, 13))
+    args = [_f64(vel, 3 * nb), _f64(angvel, 3 * nb), _f64(mass, nb), _f64(friction, nb)]
+    lib.orc_shadow_run(prm.ctypes.data_as(_p), _ptr(state.bodies), nb, _ptr(state.manifolds), len(state.manifolds),
+                       C.c_int64(state.next_serial), steps, *[_ptr(a) for a in args], out.ctypes.data_as(_p),
+                       sh.ctypes.data_as(_p))
+    return out, sh
 
 
 def oracle_f64_run(state: State, params, steps: int, vel=None, angvel=None, mass=None, friction=None):

@@ -433,6 +433,51 @@ void orc_f64_run(const avbd_params *p, const avbd_body *bodies, int nb, const av
     impl.step(steps);
     impl.getBodiesF64(out13);
 }
+/* Frozen-branch run: a World<ShadowT> from the state, with FP64 overrides of the initial
+ * velocities, angular velocities, masses (moments scale with the mass) and friction applied to the
+ * FP64 values only. Its shadows step the FP32 trajectory of the unperturbed state and take every
+ * decision; out13 receives the FP64 values (as orc_f64_run), shadow13 (may be NULL) the shadows. */
+void orc_shadow_run(const avbd_params *p, const avbd_body *bodies, int nb, const avbd_manifold *manifolds, int nm,
+                    int64_t nextSerial, int steps, const double *vel, const double *angvel, const double *mass,
+                    const double *friction, double *out13, double *shadow13)
+{
+    World<ShadowT> w;
+    fill(w, p, bodies, nb, nullptr, 0, nullptr, 0, manifolds, nm, nextSerial);
+    for (int i = 0; i < nb; ++i)
+    {
+        Body<ShadowT> &b = w.bodies[i];
+        for (int k = 0; k < 3; ++k)
+        {
+            if (vel)
+                b.vel[k].v = vel[3 * i + k];
+            if (angvel)
+                b.angVel[k].v = angvel[3 * i + k];
+        }
+        if (mass && bodies[i].mass > 0.0f)
+        {
+            b.mass.v = mass[i];
+            for (int k = 0; k < 3; ++k)
+                b.moment[k].v = mass[i] * ((double)bodies[i].moment[k] / (double)bodies[i].mass);
+        }
+        if (friction)
+            b.friction.v = friction[i];
+    }
+    for (int s = 0; s < steps; ++s)
+        w.step();
+    for (int i = 0; i < nb; ++i)
+    {
+        const Body<ShadowT> &b = w.bodies[i];
+        const ShadowT f[13] = {b.pos.x, b.pos.y, b.pos.z, b.rot.x, b.rot.y, b.rot.z, b.rot.w,
+                               b.vel.x, b.vel.y, b.vel.z, b.angVel.x, b.angVel.y, b.angVel.z};
+        for (int k = 0; k < 13; ++k)
+        {
+            out13[13 * i + k] = f[k].v;
+            if (shadow13)
+                shadow13[13 * i + k] = (double)f[k].s;
+        }
+    }
+}
+
 void orc_set_order(void *w, int mode) { ((WorldBase *)w)->setOrder(mode); }
 /* Solver::broadphaseMode (solver.h:39-45): 0 all pairs, 1 uniform grid, 2 sweep and prune (default). */
 void orc_set_broadphase(void *w, int mode) { ((WorldBase *)w)->setBroadphase(mode); }

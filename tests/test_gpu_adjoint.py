@@ -126,16 +126,19 @@ def _com_loss_parts(pos, mass, pile, target):
     return com, ((com - target) ** 2).sum()
 
 
-def test_rollout_autograd_function_matches_forward_mode(gpu, oracle):
-    """torch.autograd.Function over the C ABI: the projectile workload at small size, loss =
-    |COM(block) - target|^2 with the COM weighted by the (differentiable) masses."""
+@pytest.mark.parametrize("count,projectiles,seed,steps,tol", [(64, 4, 9, 90, 1e-4), (1024, 64, 5, 120, 1e-3)],
+                         ids=["small", "full_C5"])
+def test_rollout_autograd_function_matches_forward_mode(gpu, oracle, count, projectiles, seed, steps, tol):
+    """torch.autograd.Function over the C ABI: the projectile workload, loss = |COM(block) - target|^2
+    with the COM weighted by the (differentiable) masses; at small size and at the differentiable
+    benchmark configuration itself (1024-box block + 64 projectiles, 120 steps, checkpoints every 10)."""
     import torch
     from avbd import adjoint, scenes
     from avbd.state import default_params
-    sc = scenes.projectile_pile(count=64, projectiles=4, seed=9)
-    st, pile, proj, target = scenes.projectile_state(sc, gpu.Maker(), seed=9)
+    sc = scenes.projectile_pile(count=count, projectiles=projectiles, seed=seed)
+    st, pile, proj, target = scenes.projectile_state(sc, gpu.Maker(), seed=seed)
     prm = default_params(gravity=sc.gravity)
-    steps = 90
+    print(f"rollout {count}+{projectiles} boxes, {steps} steps", flush=True)
     sim = adjoint.Rollout(st, prm, steps, checkpoint_every=10)
     b = st.bodies
     t = lambda a: torch.tensor(np.asarray(a, np.float64), requires_grad=True)
@@ -167,5 +170,53 @@ def test_rollout_autograd_function_matches_forward_mode(gpu, oracle):
         got = float((grads[k] * u).sum())
         rel = abs(got - ref) / max(abs(ref), 1e-12)
         print(f"rollout {k:9s} autograd {got: .10e}  forward-mode {ref: .10e}  rel {rel:.2e}", flush=True)
-        assert rel < 1e-4, f"{k}: {rel:.2e}"
+        assert rel < tol, f"{k}: {rel:.2e}"
     sim.close()
+
+
+def test_adjoint_against_frozen_branch_central_differences(gpu, oracle):
+    """The GPU adjoint against FP64 central differences (h = 1e-4) of the frozen-branch run
+    (orc_shadow_run: the FP32 trajectory's contact sets, clamps and stick decisions, FP64
+    arithmetic), on a 30-box pile falling and settling over 120 steps with checkpoints every 10:
+    the directional derivatives of L = sum c . pos_final agree for initial velocities, angular
+    velocities, masses and friction within 1e-3 of sum |c_i . dpos_i| (the size of the terms the
+    projection sums; measured 3e-5 to 3e-4) and within 3e-3 of the projection itself (measured
+    4e-4 to 1.5e-3). The adjoint is the derivative of the FP32 trajectory and the differences are
+    the derivative of the FP64 map along the same branches: they differ by the FP32 rounding of
+    the linearisation point, about 1e-3 after 120 steps, while the adjoint agrees with the FP32
+    forward-mode tangent to about 1e-6. (On the 1024+64 projectile workload no finite
+    difference converges: the FP32 and FP64 trajectories part by metres and the frozen-branch
+    FP64 run leaves the FP32 path at the impacts; that workload is checked against the forward-mode
+    reference above.)"""
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scenes.pile(30, seed=5)
+    st = scenes.build_state(sc, gpu.Maker())
+    prm = default_params(gravity=sc.gravity)
+    steps = 120
+    nb = len(st.bodies)
+    dyn = st.bodies["mass"] > 0
+    rng = np.random.default_rng(7)
+    c = rng.normal(size=(nb, 3)) * dyn[:, None]
+    w = gpu.HipWorld.for_state(st, prm)
+    w.adjoint_forward(steps, 10)
+    g = w.adjoint_backward(d_pos=c)
+    w.close()
+    u = _directions(st, 3)
+    keys = {"vel": "vel", "ang_vel": "angvel", "mass": "mass", "friction": "friction"}
+    base = {"vel": st.bodies["vel"], "ang_vel": st.bodies["ang_vel"], "mass": st.bodies["mass"],
+            "friction": st.bodies["friction"]}
+    h = 1e-5
+    for k in KINDS:
+        x0 = base[k].astype(np.float64)
+        p, _ = oracle.oracle_shadow_run(st, prm, steps, **{keys[k]: x0 + h * u[k]})
+        m, _ = oracle.oracle_shadow_run(st, prm, steps, **{keys[k]: x0 - h * u[k]})
+        terms = c * (p[:, 0:3] - m[:, 0:3]) / (2 * h)
+        fd = float(terms.sum())
+        got = float((g[k] * u[k]).sum())
+        rel = abs(got - fd) / max(abs(fd), 1e-12)
+        rel_terms = abs(got - fd) / max(float(np.abs(terms).sum()), 1e-12)
+        print(f"pile30 frozen-branch FD {k:8s} adjoint {got: .10e}  FD64 {fd: .10e}  rel {rel:.2e}  "
+              f"rel to sum|terms| {rel_terms:.2e}", flush=True)
+        assert rel_terms < 1e-3, f"{k}: adjoint vs frozen-branch FD {rel_terms:.2e} of sum|terms|"
+        assert rel < 3e-3, f"{k}: adjoint vs frozen-branch FD {rel:.2e}"

@@ -90,3 +90,40 @@ def test_fp32_branch_tangent_follows_the_fp32_trajectory(oracle):
     np.testing.assert_array_equal(val[:, 0:3].astype(np.float32), b["pos"])
     np.testing.assert_array_equal(val[:, 3:7].astype(np.float32), b["rot"])
     np.testing.assert_array_equal(val[:, 7:10].astype(np.float32), b["vel"])
+
+
+def test_frozen_branch_run_keeps_the_fp32_trajectory_and_converges(oracle):
+    """orc_shadow_run: its FP32 shadows are the FP32 oracle's trajectory bit for bit (every branch
+    the frozen-branch FP64 run takes is the FP32 run's), and on a 30-box pile falling and settling
+    over 120 steps its central differences converge (h = 1e-4 and 1e-6 agree) to within 1e-3 of
+    the FP32-path tangent, the derivative the GPU adjoint computes."""
+    from avbd import scenes
+    from avbd.state import default_params
+    from conftest import body_block
+    sc = scenes.pile(30, seed=5)
+    st = scenes.build_state(sc, oracle.OracleMaker())
+    prm = default_params(gravity=sc.gravity)
+    steps = 120
+    val, sh = oracle.oracle_shadow_run(st, prm, steps)
+    w = oracle.OracleWorld(st, prm)
+    w.step(steps)
+    np.testing.assert_array_equal(sh.astype(np.float32), body_block(w.state()))
+    assert np.abs(val - sh).max() < 1e-2
+    nb = len(st.bodies)
+    dyn = st.bodies["mass"] > 0
+    rng = np.random.default_rng(1)
+    for kind, key in (("dvel", "vel"), ("dangvel", "angvel"), ("dmass", "mass"), ("dfriction", "friction")):
+        u = {"dvel": rng.normal(size=(nb, 3)) * dyn[:, None], "dangvel": rng.normal(size=(nb, 3)) * dyn[:, None],
+             "dmass": rng.normal(size=nb) * dyn * st.bodies["mass"], "dfriction": rng.normal(size=nb) * 0.1}[kind]
+        base = {"dvel": st.bodies["vel"], "dangvel": st.bodies["ang_vel"], "dmass": st.bodies["mass"],
+                "dfriction": st.bodies["friction"]}[kind].astype(np.float64)
+        _, t32 = oracle.oracle_tangent_run(st, prm, steps, **{kind: u})
+        fds = []
+        for h in (1e-4, 1e-6):
+            p, _ = oracle.oracle_shadow_run(st, prm, steps, **{key: base + h * u})
+            m, _ = oracle.oracle_shadow_run(st, prm, steps, **{key: base - h * u})
+            fds.append((p - m) / (2 * h))
+        scale = max(np.abs(fds[0][dyn, 0:7]).max(), 1e-12)
+        assert np.abs(fds[0] - fds[1])[dyn, 0:7].max() / scale < 1e-5, f"{kind}: frozen FD not converged"
+        rel = np.abs(t32[dyn, 0:7] - fds[0][dyn, 0:7]).max() / scale
+        assert rel < 1e-3, f"{kind}: FP32-path tangent vs frozen-branch FD {rel:.2e}"
