@@ -104,6 +104,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--launch", default="", metavar="KEY=V,...",
+                    help="set launch-shape fields (primal_lanes, primal_threads, dual_threads, use_graph, "
+                         "dataflow, lazy_dual) instead of autotuning, e.g. primal_lanes=16,primal_threads=64")
     ap.add_argument("--batch-worlds", type=int, default=4096, help="0 skips the batched measurement")
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--batch-warmup", type=int, default=20)
@@ -159,7 +162,14 @@ def main():
     world.step(args.warmup)
     world.synchronize()
     launch = world.launch_config()
-    if not args.no_tune:
+    if args.launch:
+        for kv in args.launch.split(","):
+            key, val = kv.split("=")
+            launch[key.strip()] = int(val)
+        world.set_launch_config(**launch)
+        launch = world.launch_config()
+        log(f"launch shape set to {launch}")
+    elif not args.no_tune:
         # direct launches only: the per-kernel timing marks need them; 20 steps per trial so
         # that the steps right after each trial's state restore do not decide the shape
         launch = world.autotune(trial_steps=20, allow_graph=False)
@@ -171,7 +181,7 @@ def main():
     start_state = None if args.no_cpu_baseline else world.state()
     world.reset_stats()
     # the autotuner restores the state it started from; two steps follow it
-    first = args.settle + args.warmup + (0 if args.no_tune else 2) + 1
+    first = args.settle + args.warmup + (0 if args.no_tune or args.launch else 2) + 1
     log(f"timing {args.steps} steps ({first}-{first + args.steps - 1})")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -290,7 +300,8 @@ def adjoint_leg(args):
     """The differentiable rollout: a 1024-box block plus 64 projectiles over 120 steps, loss =
     |COM(block) - target|^2, gradients w.r.t. projectile velocities and angular velocities, all
     masses and friction coefficients; cost against the plain forward, and the gradient checked
-    against the CPU forward-mode reference (same FP32 trajectory) and FP64 central differences."""
+    against the CPU forward-mode reference (same FP32 trajectory) and against central differences
+    of the frozen-branch FP64 run where those exist (adjoint_checks)."""
     import numpy as np
     import torch
     from avbd import adjoint, capi, scenes
@@ -351,8 +362,20 @@ def adjoint_leg(args):
 
 
 def adjoint_checks(sim, st, prm, steps, pile, proj, target, pos, grads):
-    """Directional derivatives of the loss against the CPU oracle: the forward-mode run on the
-    same FP32 trajectory, and FP64 central differences (h = 1e-4) on the FP64 trajectory."""
+    """Directional derivatives of the loss against the CPU oracle.
+
+    - forward_mode: the oracle's forward-mode run on the same FP32 trajectory (the derivative the
+      adjoint computes); the error is given relative to the derivative and relative to the sum of
+      the absolute per-body terms it adds up (the projection can cancel: mass, 0.07 out of terms
+      near 1).
+    - fd_fp64_free: FP64 central differences (h = 1e-4) of the FP64 trajectory, with how far the
+      FP32 and FP64 trajectories end apart: through the impacts they part by metres, so these
+      differences are of another trajectory and are printed, not compared.
+    - frozen-branch run (the FP32 trajectory's contact sets, clamps and stick decisions in FP64
+      arithmetic): how far its unperturbed run ends from the FP32 trajectory (at the impacts the
+      frozen map leaves it, so no finite difference of it exists here), and on a 30-box pile
+      falling and settling for 120 steps, where it does, the GPU adjoint against its central
+      differences (h = 1e-5)."""
     import numpy as np
     import oracle.bindings as ob
     b = st.bodies
@@ -378,12 +401,22 @@ def adjoint_checks(sim, st, prm, steps, pile, proj, target, pos, grads):
 
     out = {}
     t0 = time.perf_counter()
+    p64 = ob.oracle_f64_run(init, prm, steps)
+    fr, sh = ob.oracle_shadow_run(init, prm, steps)
+    gap = np.abs(fr[:, 0:3] - sh[:, 0:3])
+    out["trajectories"] = {
+        "fp32_vs_fp64_max_pos_m": float(np.abs(p64[:, 0:3] - pos).max()),
+        "frozen_branch_vs_fp32_max_pos_m": float(np.nanmax(gap)) if np.isfinite(gap).any() else None,
+        "frozen_branch_finite": bool(np.isfinite(fr).all()),
+    }
+    log(f"adjoint check trajectories: {out['trajectories']}")
     for k, u in dirs.items():
         got = float((grads[k] * u).sum())
         _, tan = ob.oracle_tangent_run(init, prm, steps, **{seed[k]: u})
-        ref = float(dcom @ ((tan[pile, 0:3] * w[:, None]).sum(0) / w.sum()))
+        terms = (tan[pile, 0:3] * w[:, None]) @ dcom / w.sum()
         if k == "mass":
-            ref += float(dcom @ (((pos[pile] - com) * u[pile, None]).sum(0) / w.sum()))
+            terms = np.concatenate([terms, ((pos[pile] - com) * u[pile, None]) @ dcom / w.sum()])
+        ref = float(terms.sum())
         h = 1e-4
         x0 = base[k].astype(np.float64)
         lp, lm = [], []
@@ -392,11 +425,54 @@ def adjoint_checks(sim, st, prm, steps, pile, proj, target, pos, grads):
             r = ob.oracle_f64_run(init, prm, steps, **{fdkey[k]: xs})
             acc.append(loss_of(r[:, 0:3], xs if k == "mass" else b["mass"].astype(np.float64)))
         fd = (lp[0] - lm[0]) / (2 * h)
-        out[k] = {"adjoint": got, "forward_mode_fp32_path": ref, "rel_vs_forward_mode": abs(got - ref) / max(abs(ref), 1e-30),
-                  "fd_fp64_h1e-4": fd, "rel_vs_fd": abs(got - fd) / max(abs(fd), 1e-30)}
+        out[k] = {"adjoint": got, "forward_mode_fp32_path": ref,
+                  "rel_vs_forward_mode": abs(got - ref) / max(abs(ref), 1e-30),
+                  "rel_vs_forward_mode_terms": abs(got - ref) / max(float(np.abs(terms).sum()), 1e-30),
+                  "fd_fp64_free_h1e-4": fd}
         log(f"adjoint check {k:8s}: adjoint {got: .6e} forward-mode {ref: .6e} "
-            f"(rel {out[k]['rel_vs_forward_mode']:.1e}) FD64 {fd: .6e} (rel {out[k]['rel_vs_fd']:.1e})")
+            f"(rel {out[k]['rel_vs_forward_mode']:.1e}, of sum|terms| {out[k]['rel_vs_forward_mode_terms']:.1e}) "
+            f"free FD64 {fd: .6e}")
+    out["pile30_frozen_branch_fd"] = pile30_frozen_check()
     out["seconds"] = time.perf_counter() - t0
+    return out
+
+
+def pile30_frozen_check(steps=120, h=1e-5):
+    """GPU adjoint of L = sum c . pos_final on a 30-box pile (falls and settles over 120 steps,
+    checkpoints every 10) against central differences of the oracle's frozen-branch FP64 run."""
+    import numpy as np
+    import oracle.bindings as ob
+    from avbd import capi, scenes
+    from avbd.state import default_params
+    sc = scenes.pile(30, seed=5)
+    st = scenes.build_state(sc, capi.Maker())
+    prm = default_params(gravity=sc.gravity)
+    nb = len(st.bodies)
+    dyn = (st.bodies["mass"] > 0).astype(np.float64)
+    rng = np.random.default_rng(7)
+    c = rng.normal(size=(nb, 3)) * dyn[:, None]
+    w = capi.HipWorld.for_state(st, prm)
+    w.adjoint_forward(steps, 10)
+    g = w.adjoint_backward(d_pos=c)
+    w.close()
+    rng = np.random.default_rng(3)
+    u = {"vel": rng.normal(size=(nb, 3)) * dyn[:, None], "ang_vel": rng.normal(size=(nb, 3)) * dyn[:, None],
+         "mass": rng.normal(size=nb) * dyn * st.bodies["mass"], "friction": rng.normal(size=nb) * 0.1}
+    keys = {"vel": "vel", "ang_vel": "angvel", "mass": "mass", "friction": "friction"}
+    base = {"vel": st.bodies["vel"], "ang_vel": st.bodies["ang_vel"], "mass": st.bodies["mass"],
+            "friction": st.bodies["friction"]}
+    out = {"workload": f"30-box pile, {steps} steps, checkpoints every 10, L = sum c . pos_final, h = {h:g}"}
+    for k in ("vel", "ang_vel", "mass", "friction"):
+        x0 = base[k].astype(np.float64)
+        p, _ = ob.oracle_shadow_run(st, prm, steps, **{keys[k]: x0 + h * u[k]})
+        m, _ = ob.oracle_shadow_run(st, prm, steps, **{keys[k]: x0 - h * u[k]})
+        terms = c * (p[:, 0:3] - m[:, 0:3]) / (2 * h)
+        fd = float(terms.sum())
+        got = float((g[k] * u[k]).sum())
+        out[k] = {"adjoint": got, "fd": fd, "rel": abs(got - fd) / max(abs(fd), 1e-30),
+                  "rel_terms": abs(got - fd) / max(float(np.abs(terms).sum()), 1e-30)}
+        log(f"adjoint check pile30 frozen-branch FD {k:8s}: adjoint {got: .6e} FD {fd: .6e} "
+            f"(rel {out[k]['rel']:.1e}, of sum|terms| {out[k]['rel_terms']:.1e})")
     return out
 
 
