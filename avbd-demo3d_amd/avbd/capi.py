@@ -14,7 +14,8 @@ import weakref
 
 import numpy as np
 
-from .state import (BODY, CONFIG, JOINT, LAUNCH, MANIFOLD, SPRING, STATS, State, batch_states, check_layouts)
+from .state import (BODY, CONFIG, EXTERNAL_CONTACT, JOINT, LAUNCH, MANIFOLD, SPRING, STATS, State, batch_states,
+                    check_layouts)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "libavbd_hip.so")
@@ -86,6 +87,9 @@ def load(path: str | None = None):
         lib.avbd_batch_num_worlds.argtypes = [_p, _p]
         lib.avbd_set_broadphase.argtypes = [_p, C.c_int32]
         lib.avbd_get_broadphase.argtypes = [_p, _p]
+        lib.avbd_set_external_pairs.argtypes = [_p, _p, C.c_int32]
+        lib.avbd_clear_external_pairs.argtypes = [_p]
+        lib.avbd_set_external_contacts.argtypes = [_p, _p, C.c_int32]
         lib.avbd_set_launch_config.argtypes = [_p, _p]
         lib.avbd_get_launch_config.argtypes = [_p, _p]
         lib.avbd_autotune.argtypes = [_p, C.c_int32, C.c_int32, _p]
@@ -203,6 +207,22 @@ class HipWorld:
         """Solver::broadphaseMode: BROADPHASE_SAP (default), _GRID (the reference Solver's default)
         or _ALL_PAIRS."""
         _check(self._lib.avbd_set_broadphase(self._h, int(mode)))
+
+    def set_external_pairs(self, pairs):
+        """Following steps take their candidate pairs from `pairs` ((n, 2) body indices) instead of
+        the broadphase (Solver::stepCpuReferenceWithExternalBroadphase), until
+        clear_external_pairs."""
+        p = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+        _check(self._lib.avbd_set_external_pairs(self._h, p.ctypes.data_as(_p) if len(p) else None, len(p)))
+
+    def clear_external_pairs(self):
+        _check(self._lib.avbd_clear_external_pairs(self._h))
+
+    def set_external_contacts(self, contacts):
+        """The external contact table (EXTERNAL_CONTACT records) manifold initialisation reads
+        (Solver::setExternalManifoldContacts); an empty table clears it."""
+        c = np.ascontiguousarray(np.zeros(0, EXTERNAL_CONTACT) if contacts is None else contacts, EXTERNAL_CONTACT)
+        _check(self._lib.avbd_set_external_contacts(self._h, c.ctypes.data_as(_p) if len(c) else None, len(c)))
 
     @property
     def broadphase(self) -> int:

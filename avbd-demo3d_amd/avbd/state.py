@@ -33,6 +33,10 @@ MANIFOLD = np.dtype(
     [("body_a", "<i4"), ("body_b", "<i4"), ("serial", "<i8"), ("num_contacts", "<i4"),
      ("friction", "<f4"), ("basis", "<f4", 9), ("contacts", CONTACT, MAX_CONTACTS)], align=True)
 
+EXTERNAL_CONTACT = np.dtype(
+    [("body_a", "<i4"), ("body_b", "<i4"), ("num_contacts", "<i4"), ("pad_", "<i4"), ("basis", "<f4", 9),
+     ("contacts", CONTACT, MAX_CONTACTS)], align=True)
+
 JOINT = np.dtype(
     [("body_a", "<i4"), ("body_b", "<i4"), ("serial", "<i8"), ("r_a", "<f4", 3), ("r_b", "<f4", 3),
      ("c0_lin", "<f4", 3), ("c0_ang", "<f4", 3), ("penalty_lin", "<f4", 3), ("penalty_ang", "<f4", 3),
@@ -190,13 +194,13 @@ LAUNCH = np.dtype([("primal_lanes", "<i4"), ("primal_threads", "<i4"), ("dual_th
                    ("use_graph", "<i4"), ("dataflow", "<i4"), ("lazy_dual", "<i4")], align=True)
 
 
-def check_layouts(sizeof, records: int = 9) -> None:
+def check_layouts(sizeof, records: int = 10) -> None:
     """Compare dtype sizes with a library's sizeof(which) for which in [0, records)
     (0 body, 1 contact, 2 manifold, 3 joint, 4 spring, 5 params, 6 launch config, 7 stats,
-    8 config)."""
+    8 config, 9 external contact)."""
     names = ["body", "contact", "manifold", "joint", "spring", "params", "launch config", "stats",
-             "config"][:records]
-    dts = [BODY, CONTACT, MANIFOLD, JOINT, SPRING, PARAMS, LAUNCH, STATS, CONFIG][:records]
+             "config", "external contact"][:records]
+    dts = [BODY, CONTACT, MANIFOLD, JOINT, SPRING, PARAMS, LAUNCH, STATS, CONFIG, EXTERNAL_CONTACT][:records]
     for i, (n, dt) in enumerate(zip(names, dts)):
         got = sizeof(i)
         if got != dt.itemsize:

@@ -1252,6 +1252,8 @@ int check_adjoint_scene(avbd_handle *h)
 {
     if (h->nj > 0 || h->ns > 0)
         return fail(AVBD_ERR_UNSUPPORTED, "adjoint: contact-only worlds (no joints or springs)");
+    if (h->extPairsOn || !h->extContacts.empty())
+        return fail(AVBD_ERR_UNSUPPORTED, "adjoint: worlds without external pairs or contacts");
     if (h->numIgnore > 0)
         return fail(AVBD_ERR_UNSUPPORTED, "adjoint: worlds without IgnoreCollision pairs");
     if (h->numRound > 0)

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 using namespace avbd;
@@ -142,6 +143,19 @@ struct avbd_handle
     GridInfo *gridInfo = nullptr, *hostGridInfo = nullptr;
     unsigned *cellStart = nullptr;  // kGridMaxCells entries, 0xFFFFFFFF when empty
     float4 *largeLo = nullptr, *largeHi = nullptr, *largeC = nullptr;
+    // external pair and contact sources (avbd_set_external_pairs / _contacts): the host copies, the
+    // bodies' dense ids (from the last upload) and the device lists built from them before the next
+    // step once either changed (extDirty); these buffers are freed by avbd_destroy
+    bool extPairsOn = false, extDirty = false;
+    std::vector<int2> extPairs;
+    std::vector<avbd_external_contact> extContacts;
+    std::vector<int> hostDenseIds;
+    int numExtEntries = 0, extEntryCap = 0, extTableCap = 0, extMask = -1;
+    int4 *extEntries = nullptr;            // (a, b, flags, 0): valid pairs, then valid table entries
+    int *extHit = nullptr, *extHitStart = nullptr;
+    avbd_external_contact *extTable = nullptr;
+    unsigned long long *extKeys = nullptr;  // ordered pair keys of the table, kHashEmpty when free
+    int *extIdx = nullptr;                  // per key: the last table entry with that key
     int *dfDone = nullptr;  // flow primal: iterations each solved body has finished in this step
     int numCUs = 0;
     int flowBlocks[3][3] = {};  // flow primal grid per (lanes, threads) shape: every wave resident
@@ -486,6 +500,145 @@ int enqueue_grid_pairs(avbd_handle *h)
     return AVBD_OK;
 }
 
+// (Re)allocates an external-source buffer to hold `count` items when it holds fewer.
+template <class T>
+int ext_buffer(T *&p, int &cap, int count)
+{
+    if (p && cap >= count)
+        return AVBD_OK;
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+    cap = std::max(count, 1);
+    hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)cap);
+    if (e != hipSuccess)
+    {
+        p = nullptr;
+        cap = 0;
+        return fail(AVBD_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    return AVBD_OK;
+}
+
+// Device lists of the external sources (avbd_external.hip) from the host copies and the bodies'
+// dense ids: the entries of broadphaseExternalPairs (solver.cpp:579-626) with their flags, and the
+// contact table with its ordered-key index, where a later entry replaces an earlier one with the
+// same key (Solver::setExternalManifoldContacts, solver.cpp:958-967). Table entries naming a body
+// outside the state never match a manifold and are left out of the index.
+int ext_rebuild(avbd_handle *h)
+{
+    const int N = h->nb;
+    auto inRange = [&](int i) { return i >= 0 && i < N; };
+    auto unordered = [](int a, int b) {
+        return ((unsigned long long)(unsigned)std::min(a, b) << 32) | (unsigned)std::max(a, b);
+    };
+    std::vector<int4> entries;
+    if (h->extPairsOn)
+    {
+        std::unordered_set<unsigned long long> pairSeen, entrySeen;
+        for (const int2 &p : h->extPairs)
+        {
+            if (!inRange(p.x) || !inRange(p.y) || p.x == p.y)
+                continue;
+            const unsigned long long k = unordered(p.x, p.y);
+            entries.push_back(make_int4(p.x, p.y, pairSeen.count(k) ? kExtAfterPair : 0, 0));
+            pairSeen.insert(k);
+        }
+        for (const avbd_external_contact &e : h->extContacts)
+        {
+            const int a = e.body_a, b = e.body_b;
+            if (!inRange(a) || !inRange(b) || (unsigned)h->hostDenseIds[a] >= (unsigned)h->hostDenseIds[b])
+                continue;
+            const unsigned long long k = unordered(a, b);
+            const int flags =
+                kExtContact | (pairSeen.count(k) ? kExtAfterPair : 0) | (entrySeen.count(k) ? kExtAfterEntry : 0);
+            entries.push_back(make_int4(a, b, flags, 0));
+            entrySeen.insert(k);
+        }
+    }
+    if ((long long)entries.size() > h->maxPairs)
+        return fail(AVBD_ERR_CAPACITY, "external pairs and contact entries exceed the handle's pair capacity");
+    const int n = (int)entries.size();
+    int hitCap = h->extEntryCap, startCap = h->extEntryCap, r;
+    if ((r = ext_buffer(h->extEntries, h->extEntryCap, n)) || (r = ext_buffer(h->extHit, hitCap, n)) ||
+        (r = ext_buffer(h->extHitStart, startCap, n)))
+        return r;
+    hipStream_t s = h->stream;
+    if (n)
+        HIP_TRY(hipMemcpyAsync(h->extEntries, entries.data(), sizeof(int4) * n, hipMemcpyHostToDevice, s));
+    h->numExtEntries = n;
+    const int nt = (int)h->extContacts.size();
+    h->extMask = -1;
+    if (nt > 0)
+    {
+        int size = 16;
+        while (size < 2 * nt)
+            size *= 2;
+        std::vector<unsigned long long> keys(size, ~0ull);
+        std::vector<int> idx(size, -1);
+        for (int k = 0; k < nt; ++k)
+        {
+            const avbd_external_contact &e = h->extContacts[k];
+            if (!inRange(e.body_a) || !inRange(e.body_b))
+                continue;
+            const unsigned long long key = ext_key(e.body_a, e.body_b);
+            unsigned slot = ext_slot(key, size - 1);
+            while (keys[slot] != ~0ull && keys[slot] != key)
+                slot = (slot + 1) & (unsigned)(size - 1);
+            keys[slot] = key;
+            idx[slot] = k;
+        }
+        int tableCap = h->extTableCap, keyCap = h->extTableCap, idxCap = h->extTableCap;
+        if ((r = ext_buffer(h->extTable, tableCap, nt)) || (r = ext_buffer(h->extKeys, keyCap, size)) ||
+            (r = ext_buffer(h->extIdx, idxCap, size)))
+            return r;
+        h->extTableCap = std::min({tableCap, keyCap, idxCap});
+        HIP_TRY(hipMemcpyAsync(h->extTable, h->extContacts.data(), sizeof(avbd_external_contact) * nt,
+                               hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(h->extKeys, keys.data(), sizeof(unsigned long long) * size, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(h->extIdx, idx.data(), sizeof(int) * size, hipMemcpyHostToDevice, s));
+        h->extMask = size - 1;
+    }
+    // the host vectors above are copied before they go out of scope
+    HIP_TRY(hipStreamSynchronize(s));
+    h->extDirty = false;
+    return AVBD_OK;
+}
+
+// External pairs instead of the broadphase (broadphaseExternalPairs, solver.cpp:579-626): hit flags,
+// their scan, then the hits in entry order with their new-manifold flags, into S.hits / S.hitNew and
+// ctr->numHits like the other broadphases.
+int enqueue_external_pairs(avbd_handle *h)
+{
+    hipStream_t s = h->stream;
+    Scratch &S = h->S;
+    Counters *ctr = h->ctr;
+    const int n = h->numExtEntries;
+    const int blocks = std::max(1, std::min(ceil_div(n, 256), 1024));
+    k_ext_hits<<<blocks, 256, 0, s>>>(h->extEntries, n, h->B.pose, h->B.sizeRad, h->extHit, ctr);
+    scan(h, h->extHit, h->extHitStart, nullptr, n, &ctr->numHits, h->maxPairs, 1);
+    k_fill_u64<<<grid_for(h->hashSize), 256, 0, s>>>(S.hash, h->hashSize, ~0ull);
+    k_hash_insert<<<1024, 256, 0, s>>>(h->cur.ab, ctr, h->joints, h->nj, h->springs, h->ns, h->ignorePairs, h->numIgnore,
+                                       S.hash, S.hashMask, ctr);
+    k_ext_emit<<<blocks, 256, 0, s>>>(h->extEntries, n, h->B.pose, h->B.sizeRad, h->extHit, h->extHitStart, S.hash,
+                                      S.hashMask, S.hits, S.hitNew, h->maxPairs, ctr);
+    HIP_TRY(hipGetLastError());
+    return AVBD_OK;
+}
+
+ExtTable ext_table(const avbd_handle *h)
+{
+    ExtTable X;
+    if (h->extMask >= 0)
+    {
+        X.entries = h->extTable;
+        X.keys = h->extKeys;
+        X.index = h->extIdx;
+        X.mask = h->extMask;
+    }
+    return X;
+}
+
 int enqueue_prepare(avbd_handle *h)
 {
     hipStream_t s = h->stream;
@@ -501,8 +654,23 @@ int enqueue_prepare(avbd_handle *h)
     k_step_begin<<<1, 1, 0, s>>>(ctr);
     k_barrier_reset<<<1, 1, 0, s>>>(h->gb);
 
-    // ---- broadphase: sweep and prune (solver.cpp:488-577), or the grid / all pairs ----
-    if (N >= 2 && h->broadphaseMode != AVBD_BROADPHASE_SAP)
+    // ---- broadphase: sweep and prune (solver.cpp:488-577), or the grid / all pairs, or the
+    // external pairs (solver.cpp:1075-1076) ----
+    if (h->extDirty)
+    {
+        int r = ext_rebuild(h);
+        if (r)
+            return r;
+    }
+    if (N >= 2 && h->extPairsOn)
+    {
+        int r = enqueue_external_pairs(h);
+        if (r)
+            return r;
+        scan(h, S.hitNew, S.hitRank, &ctr->numHits, 0, &ctr->numCreated);
+        k_create<<<G, 256, 0, s>>>(S.hits, S.hitNew, S.hitRank, ctr, h->cur, h->maxManifolds, h->maxPairs);
+    }
+    else if (N >= 2 && h->broadphaseMode != AVBD_BROADPHASE_SAP)
     {
         int r = enqueue_grid_pairs(h);
         if (r)
@@ -553,9 +721,9 @@ int enqueue_prepare(avbd_handle *h)
         HIP_TRY(hipEventRecord(h->ev[1], s));
 
     // ---- force initialisation (manifold.cpp:100-174, joint.cpp:35-54) ----
-    if (h->numRound > 0)
+    if (h->numRound > 0 || h->extMask >= 0)
         k_narrowphase<true><<<G, 256, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch, h->recStick,
-                                              nullptr, nullptr);
+                                              nullptr, nullptr, ext_table(h));
     else
     {
         k_narrowphase_boxes<<<4 * G, kNpBoxThreads, 0, s>>>(h->cur, h->tmp, h->B, h->P, ctr, S.keep, h->recMatch,
@@ -1228,6 +1396,10 @@ int avbd_destroy(avbd_handle *h)
     destroy_graph(h);
     for (void *p : h->allocs)
         (void)hipFree(p);
+    for (void *p : {(void *)h->extEntries, (void *)h->extHit, (void *)h->extHitStart, (void *)h->extTable,
+                    (void *)h->extKeys, (void *)h->extIdx})
+        if (p)
+            (void)hipFree(p);
     if (h->hostCtr)
         (void)hipHostFree(h->hostCtr);
     if (h->hostOffsets)
@@ -1347,6 +1519,8 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
         (e = up(h->B.friction, fr.data(), sizeof(float) * N)) ||
         (e = up(h->B.denseId, ids.data(), sizeof(int) * N)))
         return fail(AVBD_ERR_HIP, std::string("upload bodies: ") + hipGetErrorString(e));
+    h->hostDenseIds = ids;
+    h->extDirty = true;
     // joints / springs
     std::vector<Joint> jt(num_joints);
     for (int i = 0; i < num_joints; ++i)
@@ -2004,6 +2178,9 @@ int avbd_batch_upload(avbd_handle *h, const int32_t *world_body_offsets, int32_t
     if (num_worlds > 1 && h->broadphaseMode != AVBD_BROADPHASE_SAP)
         return fail(AVBD_ERR_UNSUPPORTED, "avbd_batch_upload: batched worlds use sweep and prune "
                                           "(avbd_set_broadphase(h, AVBD_BROADPHASE_SAP) first)");
+    if (num_worlds > 1 && (h->extPairsOn || !h->extContacts.empty()))
+        return fail(AVBD_ERR_UNSUPPORTED, "avbd_batch_upload: external pairs or contacts are for a single world "
+                                          "(clear them first)");
     std::vector<int> worldOf(std::max(num_bodies, 1), 0);
     for (int w = 0; w < num_worlds; ++w)
     {
@@ -2071,6 +2248,45 @@ int avbd_set_broadphase(avbd_handle *h, int32_t mode)
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
     h->broadphaseMode = mode;
+    return AVBD_OK;
+}
+
+int avbd_set_external_pairs(avbd_handle *h, const int32_t *pairs, int32_t count)
+{
+    if (!h || count < 0 || (count > 0 && !pairs))
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_external_pairs: null handle, bad count or null pairs");
+    if (h->numWorlds > 1)
+        return fail(AVBD_ERR_UNSUPPORTED, "avbd_set_external_pairs: external pairs are for a single world");
+    if (count > h->maxPairs)
+        return fail(AVBD_ERR_CAPACITY, "avbd_set_external_pairs: more pairs than the handle's pair capacity");
+    h->extPairs.resize(count);
+    for (int k = 0; k < count; ++k)
+        h->extPairs[k] = make_int2(pairs[2 * k], pairs[2 * k + 1]);
+    h->extPairsOn = true;
+    h->extDirty = true;
+    return AVBD_OK;
+}
+
+int avbd_clear_external_pairs(avbd_handle *h)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_clear_external_pairs: null handle");
+    h->extPairs.clear();
+    h->extPairsOn = false;
+    h->extDirty = true;
+    return AVBD_OK;
+}
+
+int avbd_set_external_contacts(avbd_handle *h, const avbd_external_contact *contacts, int32_t count)
+{
+    if (!h || count < 0 || (count > 0 && !contacts))
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_external_contacts: null handle, bad count or null table");
+    if (count > 0 && h->numWorlds > 1)
+        return fail(AVBD_ERR_UNSUPPORTED, "avbd_set_external_contacts: external contacts are for a single world");
+    if (count > h->maxPairs)
+        return fail(AVBD_ERR_CAPACITY, "avbd_set_external_contacts: more entries than the handle's pair capacity");
+    h->extContacts.assign(contacts, contacts + count);
+    h->extDirty = true;
     return AVBD_OK;
 }
 
@@ -2301,6 +2517,7 @@ int avbd_sizeof(int which)
     case 6: return (int)sizeof(avbd_launch_config);
     case 7: return (int)sizeof(avbd_stats);
     case 8: return (int)sizeof(avbd_config);
+    case 9: return (int)sizeof(avbd_external_contact);
     default: return -1;
     }
 }

@@ -995,6 +995,8 @@ __global__ void k_hash_insert(const int2 *mab, const Counters *ctr, const Joint 
     }
 }
 
+#include "avbd_external.hip"
+
 // Emission by the same chunk groups: the lanes whose bit is set in a chunk's mask write their pair
 // at the chunk's start plus the count of set bits below them (the order of a serial walk over the
 // mask), each checking the constrained-pair hash on its own.
@@ -1094,10 +1096,14 @@ __global__ void k_create_finish(Counters *ctr, int maxManifolds, int maxPairs)
 // matchOld / stickUsed (adjoint recording, may be null): per new contact slot, the old contact it
 // continues (-1: none) and whether its anchors were kept by the stick rule.
 // Round: the world holds round bodies (worlds of boxes only run k_narrowphase_boxes below).
-template <bool Round>
 // list / listCount (may be null): process only the listed manifolds.
+// X: the external contact table (none by default); a manifold with an entry takes the entry's
+// basis and contacts instead of colliding (manifold.cpp:110-121): its unmatched contacts keep the
+// entry's penalty, lambda and stick flag, where colliding starts them at zero.
+template <bool Round>
 __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, Counters *ctr, int *keep,
-                              int *matchOld, int *stickUsed, const int *list, const int *listCount)
+                              int *matchOld, int *stickUsed, const int *list, const int *listCount,
+                              ExtTable X = ExtTable())
 {
     const int total = list ? *listCount : ctr->numTotal;
     const int old = ctr->numManifolds;
@@ -1118,7 +1124,23 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
         NewContactsT<float> nc;
         V3T<float> t0{0, 0, 0}, t1{0, 0, 0}, t2{0, 0, 0};
         int nn;
-        if (!Round || (shA == kShapeBox && shB == kShapeBox))
+        const avbd_external_contact *xe = ext_find(X, ab.x, ab.y);
+        if (xe)
+        {
+            nn = min(xe->num_contacts, kMaxContacts);
+            const float *bs = xe->basis;
+            t0 = tv(V3{bs[0], bs[1], bs[2]});
+            t1 = tv(V3{bs[3], bs[4], bs[5]});
+            t2 = tv(V3{bs[6], bs[7], bs[8]});
+            for (int i = 0; i < nn; ++i)
+            {
+                const avbd_contact &xc = xe->contacts[i];
+                nc.rA[i] = tv(V3{xc.r_a[0], xc.r_a[1], xc.r_a[2]});
+                nc.rB[i] = tv(V3{xc.r_b[0], xc.r_b[1], xc.r_b[2]});
+                nc.feature[i] = xc.feature;
+            }
+        }
+        else if (!Round || (shA == kShapeBox && shB == kShapeBox))
             nn = collide_boxes<float>(tv(pA), tq(qA), tv(ld3(sA4)), tv(pB), tq(qB), tv(ld3(sB4)), nc, t0, t1, t2);
         else
         {
@@ -1127,7 +1149,7 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
             nn = collide_pair(A, Bb, P.dt, nc, t0, t1, t2);
         }
         V3 b0 = fv(t0), b1 = fv(t1), b2 = fv(t2);
-        if (nn == 0)
+        if (nn <= 0)
         {
             keep[m] = 0;
             continue;
@@ -1140,6 +1162,12 @@ __global__ void k_narrowphase(Manifolds cur, Manifolds tmp, Bodies B, Params P, 
         {
             V3 rA = fv(nc.rA[i]), rB = fv(nc.rB[i]);
             float4 pen = make_float4(0, 0, 0, 0), lam = make_float4(0, 0, 0, 0);
+            if (xe)
+            {
+                const avbd_contact &xc = xe->contacts[i];
+                pen = make_float4(xc.penalty[0], xc.penalty[1], xc.penalty[2], 0.0f);
+                lam = make_float4(xc.lambda[0], xc.lambda[1], xc.lambda[2], xc.stick ? 1.0f : 0.0f);
+            }
             int match = -1, kept = 0;
             for (int j = 0; j < nOld; ++j)
             {

@@ -88,23 +88,85 @@ struct HipAvbdBackend : PhysicsBackend
     }
 
     // The GPU step runs the Solver's own broadphase mode (sweep and prune, the uniform grid or all
-    // pairs; solver.cpp:1077-1082), each with the reference's pair order. The other pair sources a
-    // Solver can be set to -- external pairs or contacts (solver.h:336-343, 364-365) or the
-    // GPU-narrowphase pair sink (solver.h:379-383) -- give a different contact set, so those steps
-    // run on the CPU reference, counted as fallbacks, as the reference's own GPU lane does for
-    // steps it cannot take (avbd_gpu_solver.cpp:2152-2170).
+    // pairs; solver.cpp:1077-1082), each with the reference's pair order, or its external pairs and
+    // external contact table (solver.h:336-343, 361-365; sent to the handle every step). The
+    // GPU-narrowphase pair sink (solver.h:379-383) gives a different contact set, and an external
+    // contact list the Solver's lookup does not see (filled without setExternalManifoldContacts, so
+    // broadphaseExternalPairs creates manifolds from it that Manifold::initialize then collides) has
+    // no counterpart on the handle; those steps run on the CPU reference, counted as fallbacks, as the
+    // reference's own GPU lane does for steps it cannot take (avbd_gpu_solver.cpp:2152-2170).
     const char *unsupported(const Solver &solver) const
     {
-        if (solver.useExternalBroadphasePairs)
-            return "external broadphase pairs";
-        if (solver.useExternalManifoldContacts)
-            return "external manifold contacts";
+        if (solver.useExternalBroadphasePairs && !solver.useExternalManifoldContacts &&
+            !solver.externalManifoldContacts.empty())
+            return "an external contact list not installed by setExternalManifoldContacts";
         if (solver.spherePairSink)
             return "a GPU-narrowphase pair sink";
         if (solver.broadphaseMode != BROADPHASE_SWEEP_AND_PRUNE && solver.broadphaseMode != BROADPHASE_UNIFORM_GRID &&
             solver.broadphaseMode != BROADPHASE_ALL_PAIRS)
             return "an unknown broadphase mode";
         return nullptr;
+    }
+
+    // The Solver's external pairs and contact table, by dense id, as handle body indices (an id with
+    // no body in the Solver's list becomes -1, which the handle skips as the reference skips an id
+    // without a source body, solver.cpp:583-596).
+    bool sendExternal(const Solver &solver, const std::vector<Rigid *> &bodies)
+    {
+        std::unordered_map<BodyId, int> byId;
+        if (solver.useExternalBroadphasePairs || solver.useExternalManifoldContacts)
+        {
+            byId.reserve(bodies.size() * 2);
+            for (int i = 0; i < (int)bodies.size(); ++i)
+                byId[bodies[i]->denseId] = i;
+        }
+        auto idx = [&](BodyId id) -> int32_t {
+            auto it = byId.find(id);
+            return it == byId.end() ? -1 : it->second;
+        };
+        if (solver.useExternalBroadphasePairs)
+        {
+            std::vector<int32_t> pairs;
+            pairs.reserve(2 * solver.externalBroadphasePairs.size());
+            for (const BroadphasePair &q : solver.externalBroadphasePairs)
+            {
+                pairs.push_back(idx(q.bodyA));
+                pairs.push_back(idx(q.bodyB));
+            }
+            if (avbd_set_external_pairs(h, pairs.data(), (int32_t)(pairs.size() / 2)) != AVBD_OK)
+                return false;
+        }
+        else if (avbd_clear_external_pairs(h) != AVBD_OK)
+            return false;
+        std::vector<avbd_external_contact> table;
+        if (solver.useExternalManifoldContacts)
+        {
+            table.resize(solver.externalManifoldContacts.size());
+            for (size_t k = 0; k < table.size(); ++k)
+            {
+                const ExternalManifoldContact &e = solver.externalManifoldContacts[k];
+                avbd_external_contact &o = table[k];
+                std::memset(&o, 0, sizeof(o));
+                o.body_a = idx(e.bodyA);
+                o.body_b = idx(e.bodyB);
+                o.num_contacts = e.numContacts;
+                for (int r = 0; r < 3; ++r)
+                    put3(o.basis + 3 * r, e.basis[r]);
+                for (int c = 0; c < AVBD_MAX_CONTACTS; ++c)
+                {
+                    const Manifold::Contact &ci = e.contacts[c];
+                    avbd_contact &co = o.contacts[c];
+                    co.feature = ci.feature.key;
+                    put3(co.r_a, ci.rA);
+                    put3(co.r_b, ci.rB);
+                    put3(co.c0, ci.C0);
+                    put3(co.penalty, ci.penalty);
+                    put3(co.lambda, ci.lambda);
+                    co.stick = ci.stick ? 1 : 0;
+                }
+            }
+        }
+        return avbd_set_external_contacts(h, table.data(), (int32_t)table.size()) == AVBD_OK;
     }
 
     void step(Solver &solver) override
@@ -261,7 +323,7 @@ struct HipAvbdBackend : PhysicsBackend
                               tm.data(), (int)tm.size(), (int64_t)created.size()) != AVBD_OK ||
             avbd_set_ignore_pairs(h, ti.data(), (int32_t)(ti.size() / 2)) != AVBD_OK ||
             avbd_set_broadphase(h, (int32_t)solver.broadphaseMode) != AVBD_OK ||  // enum values shared
-            avbd_step(h, 1) != AVBD_OK || avbd_synchronize(h) != AVBD_OK ||
+            !sendExternal(solver, bodies) || avbd_step(h, 1) != AVBD_OK || avbd_synchronize(h) != AVBD_OK ||
             avbd_state_counts(h, &nb, &nj, &ns, &nm, &nextSerial) != AVBD_OK)
         {
             broken = true;

@@ -111,6 +111,17 @@ typedef struct avbd_manifold {
 } avbd_manifold;
 
 /* Ball-socket joint with optional angular row and fracture: Joint (solver.h:251-264). */
+/* One entry of the external contact table (ExternalManifoldContact, solver.h:336-343): a body pair
+ * by handle body index (-1: none), its basis (row 0 the normal, B to A) and its contacts. */
+typedef struct avbd_external_contact {
+    int32_t body_a;
+    int32_t body_b;
+    int32_t num_contacts;   /* the first min(num_contacts, 8) contacts are used */
+    int32_t pad_;
+    float basis[9];
+    avbd_contact contacts[AVBD_MAX_CONTACTS];
+} avbd_external_contact;
+
 typedef struct avbd_joint {
     int32_t body_a;         /* -1: anchored to the world point r_a */
     int32_t body_b;
@@ -301,6 +312,32 @@ enum avbd_broadphase { AVBD_BROADPHASE_ALL_PAIRS = 0, AVBD_BROADPHASE_GRID = 1, 
 int avbd_set_broadphase(avbd_handle *h, int32_t mode);
 int avbd_get_broadphase(avbd_handle *h, int32_t *mode);
 
+/* External pair and contact sources (replace Solver::stepCpuReferenceWithExternalBroadphase and
+ * Solver::setExternalManifoldContacts / clearExternalManifoldContacts, solver.h:403-406,
+ * solver.cpp:939-974; the state they set is solver.h:361-365).
+ *
+ * avbd_set_external_pairs: every following step takes its candidate pairs from `pairs` (count
+ * (index_a, index_b) pairs, handle body indices) instead of the broadphase mode, as
+ * broadphaseExternalPairs does (solver.cpp:579-626): in order, each valid pair (both indices in
+ * range and different) is one check and gets the bounding-sphere test; then every contact-table
+ * entry whose bodies are valid and in ascending dense-id order (avbd_body.dense_id) is a check and
+ * a hit with no test. Each hit is classified like a broadphase hit (a pair already joined by a
+ * force, or by a manifold an earlier entry of the same step created, is a constrained hit), so new
+ * manifolds are created in entry order. count may be 0 (only contact entries then create
+ * manifolds). The list is copied; it stays in force until avbd_clear_external_pairs.
+ *
+ * avbd_set_external_contacts: the contact table Manifold::initialize consults (manifold.cpp:110-121):
+ * a manifold (a, b) whose pair has an entry -- the last entry for (a, b), else the last for (b, a)
+ * -- takes the entry's basis and contacts (penalty, lambda and stick included) instead of the
+ * narrowphase, then merges with its old contacts by feature key as usual. count 0 clears the table.
+ * The table is copied and used by every following step.
+ *
+ * A single world only (AVBD_ERR_UNSUPPORTED on batched handles); the adjoint rejects handles with
+ * an external source. */
+int avbd_set_external_pairs(avbd_handle *h, const int32_t *pairs, int32_t count);
+int avbd_clear_external_pairs(avbd_handle *h);
+int avbd_set_external_contacts(avbd_handle *h, const avbd_external_contact *contacts, int32_t count);
+
 /* IgnoreCollision forces (solver.h:283-290, force.cpp:75-79) of the uploaded
  * state: body pairs (list indices, 2 * count values) the broadphase treats as
  * already joined (Rigid::constrainedTo, rigid.cpp:105-116), so no manifold is
@@ -421,7 +458,7 @@ int avbd_debug_inject_fault(avbd_handle *h, int32_t code);
 int avbd_debug_narrowphase_capacity(avbd_handle *h, int32_t vertices);
 
 /* sizeof of the ABI records (0 body, 1 contact, 2 manifold, 3 joint, 4 spring,
- * 5 params, 6 launch config, 7 stats, 8 config), so bindings can verify their
+ * 5 params, 6 launch config, 7 stats, 8 config, 9 external contact), so bindings can verify their
  * layouts; -1 for other values. */
 int avbd_sizeof(int which);
 

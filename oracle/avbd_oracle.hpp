@@ -444,6 +444,16 @@ struct Manifold
     Contact<R> c[8];
 };
 
+// One entry of the external contact table (ExternalManifoldContact, solver.h:336-343), by body
+// list index (-1: no body).
+template <class R>
+struct ExternalContact
+{
+    int a, b, n;
+    M3<R> basis;
+    Contact<R> c[8];
+};
+
 template <class R>
 struct Joint
 {
@@ -1475,6 +1485,12 @@ struct World
     int64_t nextSerial = 0;
     int orderMode = ORDER_COLOR;
     int broadphaseMode = BP_SAP;
+    // External pair and contact sources (Solver::externalBroadphasePairs / externalManifoldContacts,
+    // solver.h:361-365): with externalPairsOn the broadphase is broadphaseExternal; a non-empty
+    // contact table feeds initManifold (Solver::setExternalManifoldContacts, solver.cpp:958-967).
+    bool externalPairsOn = false;
+    std::vector<std::pair<int, int>> externalPairs;
+    std::vector<ExternalContact<R>> externalContacts;
 
     // introspection of the last step
     std::vector<int> colors;
@@ -1527,7 +1543,9 @@ struct World
         {
             std::vector<std::vector<int>> partners;
             partnerLists(partners);
-            if (broadphaseMode == BP_GRID)
+            if (externalPairsOn)
+                broadphaseExternal(partners, created);
+            else if (broadphaseMode == BP_GRID)
                 broadphaseGrid(partners, created);
             else if (broadphaseMode == BP_ALL_PAIRS)
                 broadphaseAllPairs(partners, created);
@@ -1565,6 +1583,58 @@ struct World
         R r = bodies[a].radius + bodies[b].radius;
         if (dot(dp, dp) <= r * r)
             classify(a, b, partners, created);
+    }
+
+    // broadphaseExternalPairs (solver.cpp:579-626): the external pairs in their order, each with the
+    // bounding-sphere test of generatePair (invalid ids, a body paired with itself: skipped), then
+    // every external contact entry whose bodies are valid and in ascending dense-id order, counted
+    // as a check and a hit with no sphere test. A new manifold joins its bodies at once
+    // (Manifold's constructor), so a later entry for the same two bodies is a constrained hit.
+    void broadphaseExternal(std::vector<std::vector<int>> &partners, std::vector<std::pair<int, int>> &created)
+    {
+        const int n = (int)bodies.size();
+        auto valid = [&](int i) { return i >= 0 && i < n; };
+        auto hit = [&](int a, int b) {
+            const size_t before = created.size();
+            classify(a, b, partners, created);
+            if (created.size() != before)
+            {
+                partners[a].push_back(b);
+                partners[b].push_back(a);
+            }
+        };
+        for (const auto &pr : externalPairs)
+        {
+            const int a = pr.first, b = pr.second;
+            if (!valid(a) || !valid(b) || a == b)
+                continue;
+            ++pairChecks;
+            V3<R> dp = bodies[a].pos - bodies[b].pos;
+            R r = bodies[a].radius + bodies[b].radius;
+            if (dot(dp, dp) <= r * r)
+                hit(a, b);
+        }
+        for (const auto &e : externalContacts)
+        {
+            if (!valid(e.a) || !valid(e.b) || (uint32_t)bodies[e.a].denseId >= (uint32_t)bodies[e.b].denseId)
+                continue;
+            ++pairChecks;
+            hit(e.a, e.b);
+        }
+    }
+
+    // Solver::findExternalManifoldContact (solver.cpp:976-987): the table's last entry for (a, b),
+    // else its last entry for (b, a); null when the table is empty or has neither.
+    const ExternalContact<R> *findExternal(int a, int b) const
+    {
+        for (int pass = 0; pass < 2; ++pass)
+        {
+            const int x = pass == 0 ? a : b, y = pass == 0 ? b : a;
+            for (int k = (int)externalContacts.size() - 1; k >= 0; --k)
+                if (externalContacts[k].a == x && externalContacts[k].b == y)
+                    return &externalContacts[k];
+        }
+        return nullptr;
     }
 
     // broadphaseAllPairs (solver.cpp:176-185): every pair in list order.
@@ -1786,7 +1856,18 @@ struct World
             m.friction = R(0);
         Contact<R> nc[8];
         std::memset((void *)nc, 0, sizeof(nc));
-        int nn = collidePair(A, B, p.dt, nc, m.basis);
+        int nn;
+        // an external contact entry for the pair replaces collide (manifold.cpp:110-121): its basis
+        // and its first min(numContacts, 8) contacts, penalties, lambdas and stick flags included
+        if (const ExternalContact<R> *e = findExternal(m.a, m.b))
+        {
+            m.basis = e->basis;
+            nn = std::min(e->n, 8);
+            for (int i = 0; i < nn; ++i)
+                nc[i] = e->c[i];
+        }
+        else
+            nn = collidePair(A, B, p.dt, nc, m.basis);
         bool refresh = anyRound(A) || anyRound(B);
         for (int i = 0; i < nn; ++i)
         {

@@ -19,7 +19,7 @@ _PKG = os.path.join(os.path.dirname(_HERE), "avbd-demo3d_amd")
 if _PKG not in sys.path:
     sys.path.insert(0, _PKG)
 
-from avbd.state import BODY, CONTACT, JOINT, MANIFOLD, SPRING, State, check_layouts  # noqa: E402
+from avbd.state import BODY, CONTACT, EXTERNAL_CONTACT, JOINT, MANIFOLD, SPRING, State, check_layouts  # noqa: E402
 
 _p = C.c_void_p
 _i = C.c_int
@@ -84,7 +84,10 @@ def load_oracle():
         lib.orc_sizeof.argtypes = [_i]
         lib.orc_tangent_run.argtypes = [_i, _p, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 6
         lib.orc_f64_run.argtypes = [_p, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 5
+        lib.orc_set_external.argtypes = [_p, _i, _p, _i, _p, _i]
         check_layouts(lib.orc_sizeof, records=6)
+        if lib.orc_sizeof(9) != EXTERNAL_CONTACT.itemsize:
+            raise RuntimeError("layout mismatch for external contact")
         _ORC = lib
     return _ORC
 
@@ -125,6 +128,8 @@ def load_ref():
         lib.ref_use_hip.argtypes = [_p, _i]
         lib.ref_set_broadphase.argtypes = [_p, _i]
         lib.ref_set_broadphase.restype = _i
+        lib.ref_step_external.argtypes = [_p, _i, _i, _i, _p, _i, _p, _i]
+        lib.ref_step_counts.argtypes = [_p, _p]
         _REF = lib
         atexit.register(_close_refs)
     return _REF
@@ -197,6 +202,15 @@ class OracleWorld:
 
     def step(self, n: int = 1):
         self._lib.orc_step(self._h, n)
+
+    def set_external(self, pairs=None, contacts=None, pairs_on: bool = True):
+        """External pair and contact sources for the following steps: `pairs` ((n, 2) body indices)
+        replace the broadphase when pairs_on, `contacts` (EXTERNAL_CONTACT records) is the contact
+        table; replaces the previous sources (pairs_on=False with no contacts clears them)."""
+        p = np.ascontiguousarray(np.zeros((0, 2), np.int32) if pairs is None else pairs, np.int32).reshape(-1, 2)
+        c = np.ascontiguousarray(np.zeros(0, EXTERNAL_CONTACT) if contacts is None else contacts, EXTERNAL_CONTACT)
+        self._external = (p, c)
+        self._lib.orc_set_external(self._h, int(pairs_on), _ptr(p), len(p), _ptr(c), len(c))
 
     def prepare(self):
         self._lib.orc_prepare(self._h)
@@ -418,6 +432,17 @@ class RefWorld:
     def step(self, n: int = 1, colored: bool = True):
         self._lib.ref_step(self._h, n, int(colored))
 
+    def step_external(self, n: int = 1, pairs=None, contacts=None, pairs_on: bool = True, colored: bool = True,
+                      backend: bool = False):
+        """n steps with external pairs ((n, 2) list indices) and contact-table entries
+        (EXTERNAL_CONTACT records), as Solver::stepCpuReferenceWithExternalBroadphase; pairs_on=False
+        sets only the contact table. backend=True steps through Solver::step (the installed
+        PhysicsBackend) instead of the CPU reference."""
+        p = np.ascontiguousarray(np.zeros((0, 2), np.int32) if pairs is None else pairs, np.int32).reshape(-1, 2)
+        c = np.ascontiguousarray(np.zeros(0, EXTERNAL_CONTACT) if contacts is None else contacts, EXTERNAL_CONTACT)
+        mode = 2 if backend else int(colored)
+        self._lib.ref_step_external(self._h, n, mode, int(pairs_on), _ptr(p), len(p), _ptr(c), len(c))
+
     def time_steps(self, n: int, colored: bool = False) -> float:
         return self._lib.ref_time_steps(self._h, n, int(colored))
 
@@ -445,6 +470,12 @@ class RefWorld:
         out = np.zeros(n, np.int32)
         self._lib.ref_get_colors(self._h, _ptr(out), n)
         return out
+
+    def step_counts(self):
+        """The last step's (pair checks, sphere hits, constrained hits, manifolds created)."""
+        out = np.zeros(4, np.int32)
+        self._lib.ref_step_counts(self._h, out.ctypes.data_as(_p))
+        return tuple(int(x) for x in out)
 
     def broadphase_pairs(self):
         checks, hits = C.c_int32(), C.c_int32()

@@ -38,6 +38,8 @@ struct WorldBase
     virtual void setOrder(int mode) = 0;
     virtual void setBroadphase(int mode) = 0;
     virtual void setIgnores(const int32_t *pairs, int n) = 0;
+    virtual void setExternal(int pairsOn, const int32_t *pairs, int np, const avbd_external_contact *contacts,
+                             int nc) = 0;
     virtual void counts(int32_t *nb, int32_t *nj, int32_t *ns, int32_t *nm, int64_t *serial) = 0;
     virtual void get(avbd_body *b, avbd_joint *j, avbd_spring *s, avbd_manifold *m) = 0;
     virtual void getBodiesF64(double *out) = 0;
@@ -48,6 +50,20 @@ struct WorldBase
     virtual int pairChecks() = 0;
     virtual int colorCount() = 0;
 };
+
+template <class R>
+Contact<R> contact_of(const avbd_contact &sc)
+{
+    Contact<R> ci;
+    ci.feature = sc.feature;
+    ci.rA = v3<R>(sc.r_a);
+    ci.rB = v3<R>(sc.r_b);
+    ci.C0 = v3<R>(sc.c0);
+    ci.penalty = v3<R>(sc.penalty);
+    ci.lambda = v3<R>(sc.lambda);
+    ci.stick = sc.stick != 0;
+    return ci;
+}
 
 template <class R>
 struct WorldImpl : WorldBase
@@ -64,6 +80,28 @@ struct WorldImpl : WorldBase
     void finish() override { w.finish(); }
     void setOrder(int mode) override { w.orderMode = mode; }
     void setBroadphase(int mode) override { w.broadphaseMode = mode; }
+    void setExternal(int pairsOn, const int32_t *pairs, int np, const avbd_external_contact *contacts,
+                     int nc) override
+    {
+        w.externalPairsOn = pairsOn != 0;
+        w.externalPairs.clear();
+        for (int k = 0; k < np; ++k)
+            w.externalPairs.push_back({pairs[2 * k], pairs[2 * k + 1]});
+        w.externalContacts.clear();
+        for (int k = 0; k < nc; ++k)
+        {
+            const avbd_external_contact &e = contacts[k];
+            ExternalContact<R> x{};
+            x.a = e.body_a;
+            x.b = e.body_b;
+            x.n = e.num_contacts;
+            for (int r = 0; r < 3; ++r)
+                x.basis[r] = v3<R>(e.basis + 3 * r);
+            for (int c = 0; c < 8; ++c)
+                x.c[c] = contact_of<R>(e.contacts[c]);
+            w.externalContacts.push_back(x);
+        }
+    }
     void setIgnores(const int32_t *pairs, int n) override
     {
         w.ignores.clear();
@@ -300,17 +338,7 @@ void fill(World<R> &w, const avbd_params *p, const avbd_body *bodies, int nb, co
         for (int r = 0; r < 3; ++r)
             m.basis[r] = v3<R>(s.basis + 3 * r);
         for (int c = 0; c < 8; ++c)
-        {
-            const avbd_contact &sc = s.contacts[c];
-            Contact<R> &ci = m.c[c];
-            ci.feature = sc.feature;
-            ci.rA = v3<R>(sc.r_a);
-            ci.rB = v3<R>(sc.r_b);
-            ci.C0 = v3<R>(sc.c0);
-            ci.penalty = v3<R>(sc.penalty);
-            ci.lambda = v3<R>(sc.lambda);
-            ci.stick = sc.stick != 0;
-        }
+            m.c[c] = contact_of<R>(s.contacts[c]);
         w.manifolds.push_back(m);
     }
     auto bySerial = [](const auto &x, const auto &y) { return x.serial < y.serial; };
@@ -481,6 +509,13 @@ void orc_shadow_run(const avbd_params *p, const avbd_body *bodies, int nb, const
 void orc_set_order(void *w, int mode) { ((WorldBase *)w)->setOrder(mode); }
 /* Solver::broadphaseMode (solver.h:39-45): 0 all pairs, 1 uniform grid, 2 sweep and prune (default). */
 void orc_set_broadphase(void *w, int mode) { ((WorldBase *)w)->setBroadphase(mode); }
+/* External pair and contact sources (Solver::externalBroadphasePairs, useExternalBroadphasePairs and
+ * the external contact table, solver.h:361-365): pairs_on selects broadphaseExternalPairs, `pairs`
+ * holds np (a, b) list-index pairs, `contacts` nc table entries; replaces the previous sources. */
+void orc_set_external(void *w, int pairs_on, const int32_t *pairs, int np, const avbd_external_contact *contacts, int nc)
+{
+    ((WorldBase *)w)->setExternal(pairs_on, pairs, np, contacts, nc);
+}
 /* IgnoreCollision pairs (list indices), replacing the previous set. */
 void orc_set_ignores(void *w, const int32_t *pairs, int n) { ((WorldBase *)w)->setIgnores(pairs, n); }
 void orc_step(void *w, int n) { ((WorldBase *)w)->step(n); }
@@ -710,6 +745,7 @@ int orc_sizeof(int which)
     case 3: return (int)sizeof(avbd_joint);
     case 4: return (int)sizeof(avbd_spring);
     case 5: return (int)sizeof(avbd_params);
+    case 9: return (int)sizeof(avbd_external_contact);
     default: return -1;
     }
 }

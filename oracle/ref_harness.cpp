@@ -536,6 +536,86 @@ int ref_set_broadphase(void *h, int mode)
     return (int)w->solver.broadphaseMode;
 }
 
+/* n steps with external pair and contact sources, as Solver::stepCpuReferenceWithExternalBroadphase
+ * (solver.cpp:939-957) sets them around each step: `pairs` (np (a, b) pairs) and the contact table
+ * entries by list index, turned into the bodies' dense ids (an index outside the list becomes
+ * INVALID_BODY_ID). mode 0: Solver::stepCpuReference; 1: in colour order like ref_step; 2:
+ * Solver::step, the installed PhysicsBackend (ref_use_hip). pairs_on = 0 sets only the contact table
+ * (Solver::setExternalManifoldContacts) and runs the world's own broadphase. */
+void ref_step_external(void *h, int n, int mode, int pairs_on, const int32_t *pairs, int np,
+                       const avbd_external_contact *contacts, int nc)
+{
+    RefWorld *w = (RefWorld *)h;
+    Solver &s = w->solver;
+    std::vector<Rigid *> list = bodyList(s);
+    auto id = [&](int32_t i) -> BodyId {
+        return i >= 0 && i < (int32_t)list.size() ? list[i]->denseId : INVALID_BODY_ID;
+    };
+    std::vector<BroadphasePair> bp;
+    for (int k = 0; k < np; ++k)
+    {
+        BroadphasePair q{};
+        q.bodyA = id(pairs[2 * k]);
+        q.bodyB = id(pairs[2 * k + 1]);
+        bp.push_back(q);
+    }
+    std::vector<ExternalManifoldContact> ec;
+    for (int k = 0; k < nc; ++k)
+    {
+        const avbd_external_contact &e = contacts[k];
+        ExternalManifoldContact x{};
+        x.bodyA = id(e.body_a);
+        x.bodyB = id(e.body_b);
+        x.numContacts = e.num_contacts;
+        for (int r = 0; r < 3; ++r)
+            x.basis[r] = f3(e.basis + 3 * r);
+        for (int c = 0; c < 8; ++c)
+        {
+            const avbd_contact &sc = e.contacts[c];
+            Manifold::Contact &oc = x.contacts[c];
+            oc.feature.key = sc.feature;
+            oc.rA = f3(sc.r_a);
+            oc.rB = f3(sc.r_b);
+            oc.C0 = f3(sc.c0);
+            oc.penalty = f3(sc.penalty);
+            oc.lambda = f3(sc.lambda);
+            oc.stick = sc.stick != 0;
+        }
+        ec.push_back(x);
+    }
+    for (int i = 0; i < n; ++i)
+    {
+        if (pairs_on)
+        {
+            s.setExternalManifoldContacts(ec);
+            s.externalBroadphasePairs = bp;
+            s.useExternalBroadphasePairs = true;
+        }
+        else
+            s.setExternalManifoldContacts(ec);
+        if (mode == 1)
+            coloredStep(*w);
+        else if (mode == 2)
+            s.step();
+        else
+            s.stepCpuReference();
+        s.useExternalBroadphasePairs = false;
+        s.externalBroadphasePairs.clear();
+        s.clearExternalManifoldContacts();
+    }
+}
+
+/* The last step's broadphase counters (SolverStats, solver.h:120-130): pair checks, sphere hits,
+ * constrained hits, manifolds created. */
+void ref_step_counts(void *h, int32_t out[4])
+{
+    const SolverStats &st = ((RefWorld *)h)->solver.stats;
+    out[0] = st.pairChecks;
+    out[1] = st.sphereHits;
+    out[2] = st.constrainedHits;
+    out[3] = st.manifoldsCreated;
+}
+
 /* Pair emission of the current state in the world's broadphase mode: a clone runs
  * Solver::benchmarkBroadphaseOnly (solver.cpp:1010-1034), which drops the
  * manifolds and creates one per unconstrained bounding-sphere hit in sweep

@@ -27,6 +27,9 @@ Fixtures
                       all-pairs broadphase: a box pile, a sparse field whose grid is coarsened
                       below 2^22 cells, a mixed round-shape pile; bodies every 10th step and the
                       pair emission (pairs, checks, hits) of the state before selected steps
+  external.npz        external pairs and a contact table (Solver::stepCpuReferenceWithExternalBroadphase,
+                      setExternalManifoldContacts) on a box pile and a mixed round-shape pile: the
+                      inputs, and the full state and broadphase counts after every step
 """
 from __future__ import annotations
 
@@ -290,10 +293,105 @@ def make_grid(path):
     print(f"{path}: {len(grid_scene_list())} scenes")
 
 
+def external_scene_list():
+    """(key, scene, steps before the external steps)."""
+    return [("boxes", scenes.pile(150, seed=3), 90), ("mixed", scenes.mixed_pile(100, seed=15), 80)]
+
+
+def external_inputs(state, pairs, seed):
+    """External sources for a state: the broadphase's pairs plus half of the state's manifold pairs,
+    shuffled, with random orientation, reversed duplicates, random (mostly separated) pairs and
+    invalid entries; and a contact table of collide results for some of those pairs (perturbed
+    anchors, some with a zero count, some with preset penalty / lambda / stick), entries for
+    random body pairs with a made-up basis and contacts, and a repeated key."""
+    from avbd.state import EXTERNAL_CONTACT
+    rng = np.random.default_rng(seed)
+    nb = len(state.bodies)
+    mp = np.stack([state.manifolds["body_a"], state.manifolds["body_b"]], 1)
+    P = np.concatenate([pairs, mp[: len(mp) // 2]])
+    P = P[rng.permutation(len(P))]
+    flip = rng.random(len(P)) < 0.5
+    P[flip] = P[flip][:, ::-1]
+    dup = P[rng.choice(len(P), 20)][:, ::-1]
+    far = rng.integers(0, nb, size=(30, 2))
+    bad = np.array([[-1, 3], [5, 5], [nb, 2], [2, nb + 7]])
+    P = np.concatenate([P, dup, far, bad]).astype(np.int32)
+    P = P[rng.permutation(len(P))]
+    C = []
+    for k in range(40):
+        a, b = (int(x) for x in P[rng.integers(len(P))])
+        if a < 0 or b < 0 or a >= nb or b >= nb or a == b:
+            continue
+        n, con, basis = ref_collide_pair(state.bodies[a], state.bodies[b])
+        e = np.zeros((), EXTERNAL_CONTACT)
+        e["body_a"], e["body_b"] = a, b
+        e["num_contacts"] = n if k % 7 else 0
+        e["basis"] = basis
+        e["contacts"][:len(con)] = con
+        e["contacts"]["r_a"] += (rng.normal(size=(8, 3)) * 1e-3).astype(np.float32)
+        if k % 5 == 0:
+            e["contacts"]["penalty"] = 1000.0
+            e["contacts"]["lambda"] = -1.0
+            e["contacts"]["stick"] = 1
+        C.append(e)
+    for k in range(10):
+        a, b = (int(x) for x in rng.integers(0, nb, 2))
+        if a == b:
+            continue
+        e = np.zeros((), EXTERNAL_CONTACT)
+        e["body_a"], e["body_b"] = a, b
+        e["num_contacts"] = 1 + k % 3
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        t1 = np.cross(v, [0.0, 0.0, 1.0])
+        t1 /= np.linalg.norm(t1)
+        e["basis"] = np.concatenate([v, t1, np.cross(v, t1)]).astype(np.float32)
+        e["contacts"]["feature"] = np.arange(8) + 100
+        e["contacts"]["r_a"] = rng.normal(size=(8, 3)) * 0.3
+        e["contacts"]["r_b"] = rng.normal(size=(8, 3)) * 0.3
+        C.append(e)
+    C.append(C[3].copy())
+    C[-1]["contacts"]["r_b"] += np.float32(0.01)
+    return P, np.array(C, EXTERNAL_CONTACT)
+
+
+def make_external(path, pair_steps=5, table_steps=3):
+    """External pairs and contacts (Solver::stepCpuReferenceWithExternalBroadphase): from a state
+    stepped in SAP mode, `pair_steps` steps with the external pairs and contact table, then
+    `table_steps` steps with the contact table only (the world's own SAP broadphase); the full
+    state and the step's (checks, hits, constrained hits, created) after every step."""
+    out = {}
+    for key, sc, pre in external_scene_list():
+        prm = default_params(gravity=sc.gravity)
+        w = RefWorld(prm, scene=sc)
+        w.step(pre, colored=True)
+        st = w.state()
+        pairs, _, _ = w.broadphase_pairs()
+        P, C = external_inputs(st, pairs, seed=len(key))
+        for name, v in state_arrays(f"{key}.start", st).items():
+            out[name] = v
+        out[f"{key}.pairs"] = P
+        out[f"{key}.contacts"] = C
+        counts = []
+        for k in range(pair_steps + table_steps):
+            w.step_external(1, P, C, pairs_on=k < pair_steps, colored=True)
+            counts.append(w.step_counts())
+            for name, v in state_arrays(f"{key}.s{k}", w.state()).items():
+                out[name] = v
+        out[f"{key}.counts"] = np.array(counts, np.int32)
+        out[f"{key}.pair_steps"] = np.int32(pair_steps)
+        out[f"{key}.gravity"] = np.float32(sc.gravity)
+        out[f"{key}.steps"] = np.int32(pair_steps + table_steps)
+        print(f"  {key}: {len(st.bodies)} bodies, {len(P)} pairs, {len(C)} table entries, counts {counts}")
+    np.savez_compressed(path, **out)
+    print(f"{path}: {len(external_scene_list())} cases")
+
+
 def main(which=None):
     load_ref()
     jobs = {"collide_boxes": make_collide, "stack": make_stack, "pyramid_chain": make_pyramid_chain, "pile": make_pile,
-            "collide_round": make_collide_round, "round_scenes": make_round_scenes, "grid": make_grid}
+            "collide_round": make_collide_round, "round_scenes": make_round_scenes, "grid": make_grid,
+            "external": make_external}
     for name, fn in jobs.items():
         if not which or name in which:
             fn(os.path.join(HERE, f"{name}.npz"))

@@ -143,3 +143,29 @@ def test_grid_and_all_pairs_broadphase_match_reference(oracle, key):
         if (k + 1) % 10 == 0:
             np.testing.assert_array_equal(body_block(w.state()), traj[(k + 1) // 10 - 1],
                                           err_msg=f"{key} step {k + 1}: bodies")
+
+
+@pytest.mark.parametrize("key", ["boxes", "mixed"])
+def test_external_pairs_and_contacts_match_reference(oracle, key):
+    """External pairs and an external contact table (Solver::stepCpuReferenceWithExternalBroadphase,
+    solver.cpp:939-957; broadphaseExternalPairs, solver.cpp:579-626; the table's use in
+    Manifold::initialize, manifold.cpp:110-121): shuffled and reversed pairs, duplicates, invalid and
+    separated pairs; table entries with perturbed anchors, zero counts, preset penalties, lambdas and
+    stick flags, made-up contacts for separated pairs, a repeated key, entries whose dense ids are
+    descending (lookup only). The full state and the step's checks, hits and creations after every
+    step equal the reference's, with the pairs and with the table alone."""
+    z = load_golden("external.npz")
+    from avbd.state import default_params
+    st = state_from_golden(z, f"{key}.start")
+    prm = default_params(gravity=float(z[f"{key}.gravity"]))
+    P, Cx = z[f"{key}.pairs"], z[f"{key}.contacts"]
+    w = oracle.OracleWorld(st, prm)
+    pair_steps = int(z[f"{key}.pair_steps"])
+    counts = z[f"{key}.counts"]
+    for k in range(int(z[f"{key}.steps"])):
+        w.set_external(P, Cx, pairs_on=k < pair_steps)
+        w.step(1)
+        hits = w.hits()
+        assert (w.pair_checks, len(hits), int(len(hits) - hits[:, 2].sum()), int(hits[:, 2].sum())) == \
+            tuple(int(x) for x in counts[k]), f"{key} step {k}: counts"
+        assert_states_equal(w.state(), state_from_golden(z, f"{key}.s{k}"), f"{key} step {k}")
