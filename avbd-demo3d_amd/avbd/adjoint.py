@@ -51,7 +51,9 @@ class _RolloutFn:
 
 class Rollout:
     """`steps` GPU steps of a contact-only world as a differentiable function of the initial
-    velocities, angular velocities, masses and friction coefficients (all bodies, list order)."""
+    velocities, angular velocities, masses and friction coefficients (all bodies, list order).
+    checkpoint_every: checkpoints every that many steps, each segment recomputed by the backward;
+    0 tapes every step in the forward (more HBM, no recompute)."""
 
     def __init__(self, state: State, params, steps: int, checkpoint_every: int = 10, **world_kw):
         if len(state.joints) or len(state.springs):

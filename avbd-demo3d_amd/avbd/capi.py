@@ -250,7 +250,8 @@ class HipWorld:
         return {k: int(out[k]) for k in LAUNCH.names}
 
     def adjoint_forward(self, steps: int, checkpoint_every: int = 10):
-        """Step `steps` times keeping checkpoints for avbd_adjoint_backward."""
+        """Step `steps` times keeping checkpoints for avbd_adjoint_backward (every
+        `checkpoint_every` steps; 0: tape every step now, so the backward only sweeps)."""
         _check(self._lib.avbd_adjoint_forward(self._h, steps, checkpoint_every))
 
     def adjoint_backward(self, d_pos=None, d_rot=None, d_vel=None, d_ang_vel=None) -> dict:

@@ -82,14 +82,15 @@ __device__ __forceinline__ DV dqdiff(DQ a, DQ b)
 __device__ __forceinline__ void dqdiff_adj(DQ a, DQ b, DV db, DQ &ab, DQ &bb)
 {
     double s = dqdot(b, b);
-    DQ inv = dqconj(b) * (1.0 / s);
+    const double rs = 1.0 / s;  // one division; the rest multiply by it
+    DQ inv = dqconj(b) * rs;
     DQ pb{2.0 * db.x, 2.0 * db.y, 2.0 * db.z, 0.0};
     ab += dqmul(pb, dqconj(inv));
     DQ ib = dqmul(dqconj(a), pb);
     // inv = c o b / s with c = (-1,-1,-1,1)
     double cib = -b.x * ib.x - b.y * ib.y - b.z * ib.z + b.w * ib.w;
-    double k2 = 2.0 * cib / (s * s);
-    bb += DQ{-ib.x / s - k2 * b.x, -ib.y / s - k2 * b.y, -ib.z / s - k2 * b.z, ib.w / s - k2 * b.w};
+    double k2 = 2.0 * cib * (rs * rs);
+    bb += DQ{-ib.x * rs - k2 * b.x, -ib.y * rs - k2 * b.y, -ib.z * rs - k2 * b.z, ib.w * rs - k2 * b.w};
 }
 // q' = normalize(u), u = q + ((w,0) q) / 2: accumulate q_bar, w_bar from q'_bar
 __device__ __forceinline__ void dqint_adj(DQ q, DV w, DQ qpb, DQ &qb, DV &wb)
@@ -324,7 +325,7 @@ __device__ __forceinline__ bool eval_scaled(const ContactEval &ev) { return ev.f
 // ---- 6x6 symmetric solve in FP64 (the LDL^T of maths.h:481-547, unpivoted) ----
 __device__ __forceinline__ void solve6d(double H[6][6], const double b[6], double x[6])
 {
-    double L[6][6], D[6];
+    double L[6][6], D[6], rD[6];  // rD: the six reciprocals, the only divisions
     for (int i = 0; i < 6; ++i)
     {
         for (int j = 0; j < i; ++j)
@@ -332,12 +333,13 @@ __device__ __forceinline__ void solve6d(double H[6][6], const double b[6], doubl
             double s = H[i][j];
             for (int k = 0; k < j; ++k)
                 s -= L[i][k] * L[j][k] * D[k];
-            L[i][j] = s / D[j];
+            L[i][j] = s * rD[j];
         }
         double d = H[i][i];
         for (int k = 0; k < i; ++k)
             d -= L[i][k] * L[i][k] * D[k];
         D[i] = d;
+        rD[i] = 1.0 / d;
     }
     double y[6];
     for (int i = 0; i < 6; ++i)
@@ -349,7 +351,7 @@ __device__ __forceinline__ void solve6d(double H[6][6], const double b[6], doubl
     }
     for (int i = 5; i >= 0; --i)
     {
-        double s = y[i] / D[i];
+        double s = y[i] * rD[i];
         for (int k = i + 1; k < 6; ++k)
             s -= L[k][i] * x[k];
         x[i] = s;
@@ -360,9 +362,32 @@ __device__ __forceinline__ void solve6d(double H[6][6], const double b[6], doubl
 // reverse kernels
 // ------------------------------------------------------------------------------------------
 
+// finishStep (solver.cpp:1275-1288): prevVel' = v, and for dynamic bodies v' = (x - x0)/dt,
+// w' = (q - q0)/dt.
+__device__ __forceinline__ void adj_finish_body(const Bodies &B, const Params &P, int i, AdjBuf &A)
+{
+    DV vb = ld_dv(A.vb + 3 * i), wb = ld_dv(A.wb + 3 * i), pvb = ld_dv(A.pvb + 3 * i);
+    if (B.massMoment[i].x > 0.0f)
+    {
+        double idt = 1.0 / (double)P.dt;
+        add_dv(A.xb + 3 * i, vb * idt);
+        add_dv(A.ilb + 3 * i, vb * (-idt));
+        DQ qb = ld_dq(A.qb + 4 * i), iab = ld_dq(A.iab + 4 * i);
+        dqdiff_adj(dq4(B.pose[4 * i + P_ROT]), dq4(B.pose[4 * i + P_IANG]), wb * idt, qb, iab);
+        st_dq(A.qb + 4 * i, qb);
+        st_dq(A.iab + 4 * i, iab);
+        st_dv(A.vb + 3 * i, pvb);
+        st_dv(A.wb + 3 * i, DV{0, 0, 0});
+    }
+    else
+        st_dv(A.vb + 3 * i, vb + pvb);
+    st_dv(A.pvb + 3 * i, DV{0, 0, 0});
+}
+
 // Start of a step's reversal: contact adjoints of the step's final contacts come from the carry
-// (written by the reversal of the next step); everything per step is cleared.
-__global__ void k_adj_begin(const int4 *contactRec, int nc, int nm, int nb, AdjBuf A)
+// (written by the reversal of the next step, only ever at slots of these contacts, and cleared here
+// as they are taken); everything per step is cleared; then finishStep is reversed per body.
+__global__ void k_adj_begin(Bodies B, Params P, const int4 *contactRec, int nc, int nm, int nb, AdjBuf A)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nc)
@@ -376,6 +401,10 @@ __global__ void k_adj_begin(const int4 *contactRec, int nc, int nm, int nb, AdjB
             A.rAb[3 * sl + k] = A.crAb[3 * sl + k];
             A.rBb[3 * sl + k] = A.crBb[3 * sl + k];
             A.c0b[3 * sl + k] = 0.0;
+            A.ckb[3 * sl + k] = 0.0;
+            A.clb[3 * sl + k] = 0.0;
+            A.crAb[3 * sl + k] = 0.0;
+            A.crBb[3 * sl + k] = 0.0;
         }
     }
     if (t < nm)
@@ -396,45 +425,10 @@ __global__ void k_adj_begin(const int4 *contactRec, int nc, int nm, int nb, AdjB
             A.iab[4 * t + k] = 0.0;
             A.nab[4 * t + k] = 0.0;
         }
+        adj_finish_body(B, P, t, A);
     }
 }
 
-__global__ void k_adj_clear_carry(int n3, AdjBuf A)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n3)
-    {
-        A.ckb[t] = 0.0;
-        A.clb[t] = 0.0;
-        A.crAb[t] = 0.0;
-        A.crBb[t] = 0.0;
-    }
-}
-
-// finishStep (solver.cpp:1275-1288): prevVel' = v, and for dynamic bodies v' = (x - x0)/dt,
-// w' = (q - q0)/dt.
-__global__ void k_adj_finish(Bodies B, Params P, int n, AdjBuf A)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    DV vb = ld_dv(A.vb + 3 * i), wb = ld_dv(A.wb + 3 * i), pvb = ld_dv(A.pvb + 3 * i);
-    if (B.massMoment[i].x > 0.0f)
-    {
-        double idt = 1.0 / (double)P.dt;
-        add_dv(A.xb + 3 * i, vb * idt);
-        add_dv(A.ilb + 3 * i, vb * (-idt));
-        DQ qb = ld_dq(A.qb + 4 * i), iab = ld_dq(A.iab + 4 * i);
-        dqdiff_adj(dq4(B.pose[4 * i + P_ROT]), dq4(B.pose[4 * i + P_IANG]), wb * idt, qb, iab);
-        st_dq(A.qb + 4 * i, qb);
-        st_dq(A.iab + 4 * i, iab);
-        st_dv(A.vb + 3 * i, pvb);
-        st_dv(A.wb + 3 * i, DV{0, 0, 0});
-    }
-    else
-        st_dv(A.vb + 3 * i, vb + pvb);
-    st_dv(A.pvb + 3 * i, DV{0, 0, 0});
-}
 
 // Pushes a contact's input adjoints out: bodies and init poses (atomic), the contact's own slot
 // (exclusive) and its manifold (atomic unless `exclusive`).
@@ -473,8 +467,9 @@ __device__ __forceinline__ void cout_store(const COut &o, int a, int b, int sl, 
 }
 
 // Dual update of one contact, reversed (manifold.cpp:231-280). Poses after this iteration's
-// primal (tapeCur), penalty/lambda before the dual (tapeLamPrev).
-__global__ void k_adj_dual(Bodies B, Manifolds M, const int4 *contactRec, int nc, const float4 *tapeCur,
+// primal (tapeCur), penalty/lambda before the dual (tapeLamPrev), branch flags by contact slot.
+// 64-thread blocks (the launch's shape): the bound lets the compiler keep the FP64 chain in registers.
+__global__ __launch_bounds__(64) void k_adj_dual(Bodies B, Manifolds M, const int4 *contactRec, int nc, const float4 *tapeCur,
                            const float4 *tapeLamPrev, const unsigned char *flags, Params P, AdjBuf A)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -488,7 +483,7 @@ __global__ void k_adj_dual(Bodies B, Manifolds M, const int4 *contactRec, int nc
     float4 iA = B.pose[4 * a + P_ILIN], jA = B.pose[4 * a + P_IANG], iB = B.pose[4 * b + P_ILIN],
            jB = B.pose[4 * b + P_IANG];
     float oma = 1.0f - P.alpha;
-    const int fl = flags[t];
+    const int fl = flags[sl];
     CIn in;
     cin_load(in, cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma);
     in.clamp0 = (fl & kFlagClamp0) != 0;
@@ -501,20 +496,16 @@ __global__ void k_adj_dual(Bodies B, Manifolds M, const int4 *contactRec, int nc
     double Cx[3] = {0.0, 0.0, 0.0};
     double kold[3] = {kbn.x, kbn.y, kbn.z};
     const double beta = P.betaLin;
+    // unrolled, so the three rows stay in registers (a rolled loop indexes them from scratch memory)
+#pragma unroll
     for (int k = 0; k < 3; ++k)
     {
-        bool upd = (fl & (k == 0 ? kFlagUpd0 : kFlagUpd12)) != 0;
-        if (!upd)
-            continue;
-        bool sat = (fl & (kFlagSat0 << k)) != 0;
-        double kb = k == 0 ? kbn.x : (k == 1 ? kbn.y : kbn.z);
-        if (sat)
-            kold[k] = 0.0;
-        else
-        {
-            double sg = f.C[k] < 0.0 ? -1.0 : (f.C[k] > 0.0 ? 1.0 : 0.0);
-            Cx[k] = kb * beta * sg;
-        }
+        const bool upd = (fl & (k == 0 ? kFlagUpd0 : kFlagUpd12)) != 0;
+        const bool sat = (fl & (kFlagSat0 << k)) != 0;
+        const double kb = k == 0 ? kbn.x : (k == 1 ? kbn.y : kbn.z);
+        const double sg = f.C[k] < 0.0 ? -1.0 : (f.C[k] > 0.0 ? 1.0 : 0.0);
+        kold[k] = upd && sat ? 0.0 : kold[k];
+        Cx[k] = upd && !sat ? kb * beta * sg : 0.0;
     }
     COut o;
     cout_zero(o);
@@ -528,19 +519,22 @@ __global__ void k_adj_dual(Bodies B, Manifolds M, const int4 *contactRec, int nc
 }
 
 // Primal update of the bodies of colour `col`, reversed (solver.cpp:1184-1240,
-// manifold.cpp:176-229); kAdjLanes lanes per body. tapePrev: poses before this iteration,
-// tapeCur: after it (lower colours already updated); tapeLamPrev: penalty/lambda the iteration's
-// primal read. Phase 1 rebuilds the body's FP32 system exactly as the step did (records of one
-// round in LDS rows, each entry added by its owner lane in stamp order) so the solve gives the
-// step's own dx; lane 0 then solves H w = dx_bar in FP64. Phase 2: every lane reverses its own
-// stamps; own-pose adjoints are summed over the lanes, partner ones go out atomically.
+// manifold.cpp:176-229); kAdjLanes lanes per body, two waves per block. tapePrev: poses before
+// this iteration, tapeCur: after it (lower colours already updated); tapeLamPrev: penalty/lambda
+// the iteration's primal read. The solver wave's lane 0 of each group solves H w = dx_bar in FP64
+// with the step's taped system and dx, while the stamp wave loads its first round of stamps and
+// re-evaluates their forces (the two do not depend on each other, so their latencies overlap);
+// after the block barrier the stamp wave reverses every stamp: own-pose adjoints are summed over
+// the lanes, partner ones go out atomically.
 constexpr int kAdjLanes = 16;
-constexpr int kAdjThreads = 64;
+constexpr int kAdjBodies = 64 / kAdjLanes;  // bodies per block
+constexpr int kAdjThreads = 128;            // a solver wave and a stamp wave
 
 struct AdjBodyShared
 {
     double G[27];      // G_LL (9), G_AA (9), G_AL (9)
     double rLb[3], rAb[3];
+    double xb[3], qb[4];  // the body's pose adjoints through its update and inertia terms
 };
 
 __device__ __forceinline__ void wave_sum_group(double &v)
@@ -556,9 +550,10 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
                                                             const float4 *tapeLamPrev, const float4 *sysTape,
                                                             const unsigned char *stampFlags, Params P, AdjBuf A)
 {
-    constexpr int L = kAdjLanes, G = kAdjThreads / kAdjLanes;
+    constexpr int L = kAdjLanes, G = kAdjBodies;
     __shared__ AdjBodyShared sh[G];
-    const int lane = threadIdx.x % L, g = threadIdx.x / L;
+    const bool solver = threadIdx.x < 64;
+    const int lane = (threadIdx.x & 63) % L, g = (threadIdx.x & 63) / L;
     const int k = begin + blockIdx.x * G + g;
     const bool valid = k < end;
     const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
@@ -568,14 +563,15 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
         return (j != i && cj >= 0 && cj < col) ? tapeCur : tapePrev;
     };
     const float oma = 1.0f - P.alpha;
-    float4 mm = B.massMoment[i];
-    float4 pos4 = tapePrev[2 * i], rot4 = tapePrev[2 * i + 1];
-    V3 pos = ld3(pos4);
-    // ---- lane 0: the step's dx, the FP64 solve of H w = dx_bar, inertia terms ----
-    DV xb{0, 0, 0};
-    DQ qb{0, 0, 0, 0};
-    if (valid && lane == 0)
+    AdjBodyShared &S = sh[g];
+    const float4 mm = B.massMoment[i];
+    const float4 rot4 = tapePrev[2 * i + 1];
+    const V3 pos = ld3(tapePrev[2 * i]);
+    // ---- solver wave, lane 0 of each group: the step's dx, the FP64 solve of H w = dx_bar, inertia ----
+    if (solver && valid && lane == 0)
     {
+        DV xb{0, 0, 0};
+        DQ qb{0, 0, 0, 0};
         // the step's accumulated system and dx, as its primal kernel taped them
         float a[36];
         const float4 *src = sysTape + 9 * (size_t)i;
@@ -611,7 +607,6 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
         double g6[6] = {xbo.x, xbo.y, xbo.z, dxAb.x, dxAb.y, dxAb.z}, w6[6];
         solve6d(H, g6, w6);
         const double dx6[6] = {dxL.x, dxL.y, dxL.z, dxA.x, dxA.y, dxA.z};
-        AdjBodyShared &S = sh[g];
         for (int r = 0; r < 3; ++r)
             for (int c = 0; c < 3; ++c)
             {
@@ -651,10 +646,42 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
         if (mm.x > 0.0f)
             mbar += (Ib[0] * (double)mm.y + Ib[1] * (double)mm.z + Ib[2] * (double)mm.w) / (double)mm.x;
         A.mb[i] += mbar;
+        S.xb[0] = xb.x;
+        S.xb[1] = xb.y;
+        S.xb[2] = xb.z;
+        S.qb[0] = qb.x;
+        S.qb[1] = qb.y;
+        S.qb[2] = qb.z;
+        S.qb[3] = qb.w;
     }
-    wave_lds_sync();
-    // ---- phase 2: every lane reverses its own stamps ----
-    const AdjBodyShared &S = sh[g];
+    // ---- stamp wave: its stamps' forward values (the first round before the barrier) ----
+    int4 st = make_int4(0, 0, 0, 0);
+    int fl = 0;
+    CIn in;
+    CFwd f;
+    auto load_stamp = [&](int t) {
+        st = stamps[s0 + t];
+        fl = stampFlags[s0 + t];
+        const bool isA = (st.y >> 16) != 0;
+        const int partner = st.z, sl = st.w;
+        const int a = isA ? i : partner, b = isA ? partner : i;
+        const float4 *cr = M.cr + (size_t)sl * kCR;
+        const float4 *sa = a == i ? tapePrev : seen(a), *sb = b == i ? tapePrev : seen(b);
+        float4 pen4 = tapeLamPrev[2 * sl], lam4 = tapeLamPrev[2 * sl + 1];
+        float4 pA = sa[2 * a], qA = sa[2 * a + 1], pB = sb[2 * b], qB = sb[2 * b + 1];
+        float4 iA = B.pose[4 * a + P_ILIN], jA = B.pose[4 * a + P_IANG], iB = B.pose[4 * b + P_ILIN],
+               jB = B.pose[4 * b + P_IANG];
+        cin_load(in, cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma);
+        in.clamp0 = (fl & kFlagClamp0) != 0;
+        in.scaled = (fl & kFlagScaled) != 0;
+        cfwd(in, f);
+    };
+    if (!solver && valid && lane < ns)
+        load_stamp(lane);
+    __syncthreads();
+    if (solver)
+        return;
+    // ---- stamp wave: every lane reverses its own stamps ----
     DV rLb{S.rLb[0], S.rLb[1], S.rLb[2]}, rAb{S.rAb[0], S.rAb[1], S.rAb[2]};
     DV oxb{0, 0, 0};
     DQ oqb{0, 0, 0, 0};
@@ -662,23 +689,11 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
     DQ oia{0, 0, 0, 0};
     for (int t = lane; valid && t < ns; t += L)
     {
-        int4 st = stamps[s0 + t];
-        const int fl = stampFlags[s0 + t];
-        bool isA = (st.y >> 16) != 0;
-        int partner = st.z, sl = st.w, m = adjIndex(st.x);
-        int a = isA ? i : partner, b = isA ? partner : i;
-        const float4 *cr = M.cr + (size_t)sl * kCR;
-        const float4 *sa = a == i ? tapePrev : seen(a), *sb = b == i ? tapePrev : seen(b);
-        float4 pen4 = tapeLamPrev[2 * sl], lam4 = tapeLamPrev[2 * sl + 1];
-        float4 pA = sa[2 * a], qA = sa[2 * a + 1], pB = sb[2 * b], qB = sb[2 * b + 1];
-        float4 iA = B.pose[4 * a + P_ILIN], jA = B.pose[4 * a + P_IANG], iB = B.pose[4 * b + P_ILIN],
-               jB = B.pose[4 * b + P_IANG];
-        CIn in;
-        cin_load(in, cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma);
-        in.clamp0 = (fl & kFlagClamp0) != 0;
-        in.scaled = (fl & kFlagScaled) != 0;
-        CFwd f;
-        cfwd(in, f);
+        if (t != lane)
+            load_stamp(t);
+        const bool isA = (st.y >> 16) != 0;
+        const int partner = st.z, sl = st.w, m = adjIndex(st.x);
+        const int a = isA ? i : partner, b = isA ? partner : i;
         const double sgn = isA ? 1.0 : -1.0;
         DV rW = isA ? f.rAW : f.rBW;
         COut o;
@@ -767,8 +782,8 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
     wave_sum_group(oia.w);
     if (valid && lane == 0)
     {
-        st_dv(A.xb + 3 * i, xb + oxb);
-        st_dq(A.qb + 4 * i, qb + oqb);
+        st_dv(A.xb + 3 * i, DV{S.xb[0], S.xb[1], S.xb[2]} + oxb);
+        st_dq(A.qb + 4 * i, DQ{S.qb[0], S.qb[1], S.qb[2], S.qb[3]} + oqb);
         // no other thread of this launch touches body i (neighbours have other colours)
         add_dv(A.ilb + 3 * i, oil);
         DQ ia = ld_dq(A.iab + 4 * i);
@@ -914,8 +929,9 @@ __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc
 }
 
 // Collide pose Jacobian (collide.cpp:898-955), one thread per (manifold, pose component): the
-// tangent of the basis and fresh anchors along that component, dotted with their adjoints.
-__global__ void k_adj_collide(Bodies B, Manifolds M, int nm, AdjBuf A)
+// tangent of the basis and fresh anchors along that component, dotted with their adjoints. The
+// thread of component 0 also reverses the manifold friction sqrt(mu_A mu_B) (manifold.cpp:104).
+__global__ __launch_bounds__(64) void k_adj_collide(Bodies B, Manifolds M, int nm, AdjBuf A)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nm * 14)
@@ -923,6 +939,16 @@ __global__ void k_adj_collide(Bodies B, Manifolds M, int nm, AdjBuf A)
     int m = t / 14, comp_ = t % 14;
     int2 ab = M.ab[m];
     int a = ab.x, b = ab.y;
+    if (comp_ == 0)
+    {
+        const double fa = B.friction[a], fb = B.friction[b];
+        const double mub = A.mumb[m], sq = sqrt(fa * fb);
+        if (sq > 0.0)
+        {
+            aadd(A.mub + a, mub * fb / (2.0 * sq));
+            aadd(A.mub + b, mub * fa / (2.0 * sq));
+        }
+    }
     bool onA = comp_ < 7;
     if (!(B.massMoment[onA ? a : b].x > 0.0f))
         return;
@@ -962,24 +988,6 @@ __global__ void k_adj_collide(Bodies B, Manifolds M, int nm, AdjBuf A)
     else
         aadd(A.qb + 4 * body + (c - 3), g);
 }
-
-// Manifold friction sqrt(mu_A mu_B) (manifold.cpp:104) reversed.
-__global__ void k_adj_friction(Bodies B, Manifolds M, int nm, AdjBuf A)
-{
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nm)
-        return;
-    int2 ab = M.ab[m];
-    double fa = B.friction[ab.x], fb = B.friction[ab.y];
-    double mub = A.mumb[m];
-    double s = sqrt(fa * fb);
-    if (s > 0.0)
-    {
-        aadd(A.mub + ab.x, mub * fb / (2.0 * s));
-        aadd(A.mub + ab.y, mub * fa / (2.0 * s));
-    }
-}
-
 
 } // namespace avbd
 
@@ -1030,6 +1038,7 @@ struct StepStore
 struct AdjointState
 {
     int steps = 0, every = 1;
+    bool tapeAll = false;  // the forward taped every step (checkpoint_every = 0)
     bool recorded = false;
     std::vector<Snapshot> ck, seg;
     Snapshot end;
@@ -1054,38 +1063,69 @@ int snap_alloc(avbd_handle *h, Snapshot &s)
     return AVBD_OK;
 }
 
+// Device-to-device copies gathered into one k_copy_many launch (sizes are whole 4-byte words).
+struct CopyBatch
+{
+    CopyList L{};
+    CopyBatch() { L.start[0] = 0; }
+    void add(void *dst, const void *src, size_t bytes)
+    {
+        if (!bytes)
+            return;
+        L.src[L.n] = (const unsigned *)src;
+        L.dst[L.n] = (unsigned *)dst;
+        L.start[L.n + 1] = L.start[L.n] + (long long)(bytes / 4);
+        ++L.n;
+    }
+    int launch(hipStream_t s, const char *what)
+    {
+        const long long total = L.start[L.n];
+        if (total > 0)
+            k_copy_many<<<(int)std::min<long long>((total + 255) / 256, 4096), 256, 0, s>>>(L);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return fail(AVBD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+        return AVBD_OK;
+    }
+};
+
 int snap_copy(avbd_handle *h, bool save, Snapshot &s)
 {
-    hipStream_t st = h->stream;
     const size_t N = h->nb;
     if (save)
         s.numManifolds = h->lastNumManifolds;
     const size_t M = s.numManifolds, C = M * kMaxContacts;
-    auto cp = [&](void *snap, void *live, size_t bytes) -> hipError_t {
-        if (!bytes)
-            return hipSuccess;
-        return save ? hipMemcpyAsync(snap, live, bytes, hipMemcpyDeviceToDevice, st)
-                    : hipMemcpyAsync(live, snap, bytes, hipMemcpyDeviceToDevice, st);
-    };
-    hipError_t e;
-    if ((e = cp(s.pose, h->B.pose, 4 * N * sizeof(float4))) || (e = cp(s.vel, h->B.vel, N * sizeof(float4))) ||
-        (e = cp(s.angVel, h->B.angVel, N * sizeof(float4))) || (e = cp(s.prevVel, h->B.prevVel, N * sizeof(float4))) ||
-        (e = cp(s.ab, h->cur.ab, M * sizeof(int2))) || (e = cp(s.serial, h->cur.serial, M * sizeof(long long))) ||
-        (e = cp(s.count, h->cur.count, M * sizeof(int))) || (e = cp(s.feature, h->cur.feature, C * sizeof(int))) ||
-        (e = cp(s.cr, h->cur.cr, C * kCR * sizeof(float4))) || (e = cp(s.ctr, h->ctr, sizeof(Counters))))
-        return fail(AVBD_ERR_HIP, std::string("adjoint snapshot copy: ") + hipGetErrorString(e));
+    CopyBatch b;
+    auto cp = [&](void *snap, void *live, size_t bytes) { save ? b.add(snap, live, bytes) : b.add(live, snap, bytes); };
+    cp(s.pose, h->B.pose, 4 * N * sizeof(float4));
+    cp(s.vel, h->B.vel, N * sizeof(float4));
+    cp(s.angVel, h->B.angVel, N * sizeof(float4));
+    cp(s.prevVel, h->B.prevVel, N * sizeof(float4));
+    cp(s.ab, h->cur.ab, M * sizeof(int2));
+    cp(s.serial, h->cur.serial, M * sizeof(long long));
+    cp(s.count, h->cur.count, M * sizeof(int));
+    cp(s.feature, h->cur.feature, C * sizeof(int));
+    cp(s.cr, h->cur.cr, C * kCR * sizeof(float4));
+    cp(s.ctr, h->ctr, sizeof(Counters));
+    int r = b.launch(h->stream, "adjoint snapshot copy");
+    if (r)
+        return r;
     if (!save)
         h->lastNumManifolds = s.numManifolds;
     return AVBD_OK;
 }
 
+// every = 0: a store and a start snapshot for every step (the forward tapes them all)
 int adj_alloc(avbd_handle *h, int steps, int every)
 {
     if (!h->adj)
         h->adj = new AdjointState();
     AdjointState &S = *h->adj;
     int r;
-    int nck = steps / every + 1;
+    const bool all = every == 0;
+    if (all)
+        every = steps;
+    int nck = all ? 1 : steps / every + 1;
     while ((int)S.ck.size() < nck)
     {
         S.ck.emplace_back();
@@ -1174,24 +1214,25 @@ int taped_step(avbd_handle *h, StepStore &t)
     // enqueue_iterate read the counters and colour offsets back before its launches
     t.ctr = *h->hostCtr;
     std::memcpy(t.offsets.data(), h->hostOffsets, sizeof(int) * 257);
-    hipStream_t st = h->stream;
     const size_t N = h->nb, M = t.ctr.numManifolds, C = M * kMaxContacts;
-    auto cp = [&](void *dst, const void *src, size_t bytes) {
-        return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st) : hipSuccess;
-    };
-    hipError_t e;
-    if ((e = cp(t.pose, h->B.pose, 4 * N * sizeof(float4))) || (e = cp(t.vel, h->B.vel, N * sizeof(float4))) ||
-        (e = cp(t.angVel, h->B.angVel, N * sizeof(float4))) || (e = cp(t.prevVel, h->B.prevVel, N * sizeof(float4))) ||
-        (e = cp(t.inerLin, h->B.inerLin, N * sizeof(float4))) || (e = cp(t.inerAng, h->B.inerAng, N * sizeof(float4))) ||
-        (e = cp(t.ab, h->cur.ab, M * sizeof(int2))) || (e = cp(t.serial, h->cur.serial, M * sizeof(long long))) ||
-        (e = cp(t.count, h->cur.count, M * sizeof(int))) || (e = cp(t.feature, h->cur.feature, C * sizeof(int))) ||
-        (e = cp(t.cr, h->cur.cr, C * kCR * sizeof(float4))) ||
-        (e = cp(t.contactRec, h->S.contactRec, (size_t)t.ctr.contactCount * sizeof(int4))) ||
-        (e = cp(t.stamps, h->S.stamps, (size_t)std::min(t.ctr.numStamps, h->maxStamps) * sizeof(int4))) ||
-        (e = cp(t.tasks, h->S.tasks, N * sizeof(int4))) || (e = cp(t.color, h->S.color, N * sizeof(int))) ||
-        (e = cp(t.deg, h->S.deg, N * sizeof(int))))
-        return fail(AVBD_ERR_HIP, std::string("adjoint step store: ") + hipGetErrorString(e));
-    return AVBD_OK;
+    CopyBatch b;
+    b.add(t.pose, h->B.pose, 4 * N * sizeof(float4));
+    b.add(t.vel, h->B.vel, N * sizeof(float4));
+    b.add(t.angVel, h->B.angVel, N * sizeof(float4));
+    b.add(t.prevVel, h->B.prevVel, N * sizeof(float4));
+    b.add(t.inerLin, h->B.inerLin, N * sizeof(float4));
+    b.add(t.inerAng, h->B.inerAng, N * sizeof(float4));
+    b.add(t.ab, h->cur.ab, M * sizeof(int2));
+    b.add(t.serial, h->cur.serial, M * sizeof(long long));
+    b.add(t.count, h->cur.count, M * sizeof(int));
+    b.add(t.feature, h->cur.feature, C * sizeof(int));
+    b.add(t.cr, h->cur.cr, C * kCR * sizeof(float4));
+    b.add(t.contactRec, h->S.contactRec, (size_t)t.ctr.contactCount * sizeof(int4));
+    b.add(t.stamps, h->S.stamps, (size_t)std::min(t.ctr.numStamps, h->maxStamps) * sizeof(int4));
+    b.add(t.tasks, h->S.tasks, N * sizeof(int4));
+    b.add(t.color, h->S.color, N * sizeof(int));
+    b.add(t.deg, h->S.deg, N * sizeof(int));
+    return b.launch(h->stream, "adjoint step store");
 }
 
 // The reverse sweep of one step from its store t; start: the step's start state (the previous
@@ -1216,9 +1257,7 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
     B.inerAng = t.inerAng;
     Manifolds M{t.ab, t.serial, t.count, t.feature, t.cr};
     int n0 = std::max(std::max(nb, nc), std::max(nm, 1));
-    k_adj_begin<<<ceil_div(n0, 256), 256, 0, s>>>(t.contactRec, nc, nm, nb, A);
-    k_adj_clear_carry<<<ceil_div(3 * S.slotCap, 256), 256, 0, s>>>(3 * S.slotCap, A);
-    k_adj_finish<<<ceil_div(nb, 256), 256, 0, s>>>(B, h->P, nb, A);
+    k_adj_begin<<<ceil_div(n0, 256), 256, 0, s>>>(B, h->P, t.contactRec, nc, nm, nb, A);
     for (int it = I - 1; it >= 0; --it)
     {
         const float4 *prev = tp + it * ps, *cur = tp + (it + 1) * ps, *lam = tc + it * cs;
@@ -1231,7 +1270,7 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
         {
             int b = t.offsets[col], e = t.offsets[col + 1];
             if (e > b)
-                k_adj_primal<<<ceil_div(e - b, kAdjThreads / kAdjLanes), kAdjThreads, 0, s>>>(
+                k_adj_primal<<<ceil_div(e - b, kAdjBodies), kAdjThreads, 0, s>>>(
                     B, M, t.stamps, t.tasks, b, e, t.color, col, prev, cur, lam, sys, sfl, h->P, A);
         }
     }
@@ -1240,10 +1279,7 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
         k_adj_init<<<ceil_div(nc, 256), 256, 0, s>>>(B, M, t.contactRec, nc, t.recSrc, t.recMatch, t.recStick,
                                                      start.cr, start.numManifolds, h->P, A);
     if (nm > 0)
-    {
         k_adj_collide<<<ceil_div(nm * 14, 64), 64, 0, s>>>(B, M, nm, A);
-        k_adj_friction<<<ceil_div(nm, 256), 256, 0, s>>>(B, M, nm, A);
-    }
     HIP_TRY(hipGetLastError());
     return AVBD_OK;
 }
@@ -1271,20 +1307,29 @@ int avbd_adjoint_forward(avbd_handle *h, int32_t num_steps, int32_t checkpoint_e
 {
     if (!h)
         return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_forward: null handle");
-    if (num_steps < 1 || checkpoint_every < 1)
-        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_forward: steps and checkpoint interval must be >= 1");
+    if (num_steps < 1 || checkpoint_every < 0)
+        return fail(AVBD_ERR_INVALID_ARGUMENT,
+                    "avbd_adjoint_forward: steps must be >= 1 and the checkpoint interval >= 0");
     int r;
     if ((r = pick_device(h)) || (r = check_adjoint_scene(h)) || (r = adj_alloc(h, num_steps, checkpoint_every)))
         return r;
     AdjointState &S = *h->adj;
     S.recorded = false;
     S.steps = num_steps;
-    S.every = checkpoint_every;
+    S.tapeAll = checkpoint_every == 0;
+    S.every = S.tapeAll ? num_steps : checkpoint_every;
     HIP_TRY(hipStreamSynchronize(h->stream));
     if ((r = snap_copy(h, true, S.ck[0])))
         return r;
     for (int k = 0; k < num_steps; ++k)
     {
+        if (S.tapeAll)
+        {
+            // every step taped as it is taken: its start state and store, then finishStep
+            if ((r = snap_copy(h, true, S.seg[k])) || (r = taped_step(h, S.store[k])) || (r = enqueue_finish(h)))
+                return r;
+            continue;
+        }
         if ((r = enqueue_prepare(h)) || (r = enqueue_iterate(h)) || (r = enqueue_finish(h)))
             return r;
         if ((k + 1) % checkpoint_every == 0 && (k + 1) < num_steps && (r = snap_copy(h, true, S.ck[(k + 1) / checkpoint_every])))
@@ -1323,9 +1368,17 @@ int avbd_adjoint_backward(avbd_handle *h, const double *d_pos, const double *d_r
         return fail(AVBD_ERR_HIP, std::string("adjoint init: ") + hipGetErrorString(e));
     const int K = S.every, steps = S.steps;
     const int nseg = (steps + K - 1) / K;
-    if (h->P.iterations > S.tapeIters && (r = adj_alloc(h, steps, K)))
+    if (S.tapeAll)
+    {
+        if (h->P.iterations > S.tapeIters)
+            return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_backward: iterations changed since the taped forward");
+        for (int st = steps - 1; st >= 0; --st)
+            if ((r = reverse_step(h, S.store[st], S.seg[st])))
+                return r;
+    }
+    else if (h->P.iterations > S.tapeIters && (r = adj_alloc(h, steps, K)))
         return r;
-    for (int sg = nseg - 1; sg >= 0; --sg)
+    for (int sg = S.tapeAll ? -1 : nseg - 1; sg >= 0; --sg)
     {
         const int s0 = sg * K, s1 = std::min(s0 + K, steps);
         // replay the segment from its checkpoint once, taping every step into its store and

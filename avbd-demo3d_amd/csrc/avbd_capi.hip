@@ -297,8 +297,10 @@ struct DualArgs
     Params P;
     unsigned char *flagTape;
     int staticRepeat;
-    void *ptr[9];
-    DualArgs(const avbd_handle *h, int nc_, unsigned char *flags = nullptr, int nj_ = -1, int staticRepeat_ = 1);
+    DualTape tape;
+    void *ptr[10];
+    DualArgs(const avbd_handle *h, int nc_, unsigned char *flags = nullptr, int nj_ = -1, int staticRepeat_ = 1,
+             DualTape tape_ = DualTape());
 };
 
 // Colouring runs as one launch of a few large blocks (at most 64, one per compute unit, every
@@ -822,11 +824,11 @@ PrimalArgs::PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys, unsigned
     std::memcpy(ptr, p, sizeof(p));
 }
 
-DualArgs::DualArgs(const avbd_handle *h, int nc_, unsigned char *flags, int nj_, int staticRepeat_)
+DualArgs::DualArgs(const avbd_handle *h, int nc_, unsigned char *flags, int nj_, int staticRepeat_, DualTape tape_)
     : B(h->B), M(h->cur), joints(h->joints), nj(nj_ < 0 ? h->nj : nj_), contactRec(h->S.contactRec), nc(nc_), P(h->P),
-      flagTape(flags), staticRepeat(staticRepeat_)
+      flagTape(flags), staticRepeat(staticRepeat_), tape(tape_)
 {
-    void *p[9] = {&B, &M, &joints, &nj, &contactRec, &nc, &P, &flagTape, &staticRepeat};
+    void *p[10] = {&B, &M, &joints, &nj, &contactRec, &nc, &P, &flagTape, &staticRepeat, &tape};
     std::memcpy(ptr, p, sizeof(p));
 }
 
@@ -941,9 +943,9 @@ int enqueue_iterate(avbd_handle *h)
     const bool tape = h->tapePose != nullptr;
     const size_t tapePoseStride = 2 * (size_t)h->maxBodies, tapeContactStride = 2 * (size_t)kMaxContacts * h->maxManifolds;
     auto tapeAt = [&](int slot) {
-        k_tape_pose<<<ceil_div(std::max(h->nb, 1), 256), 256, 0, s>>>(h->B.pose, h->nb, h->tapePose + slot * tapePoseStride);
-        k_tape_contacts<<<grid_for(std::max(c.contactCount, 1)), 256, 0, s>>>(h->S.contactRec, h->ctr, h->cur.cr,
-                                                                            h->tapeContact + slot * tapeContactStride);
+        const int blocks = ceil_div(h->nb, 256) + std::max(1, std::min(ceil_div(c.contactCount, 256), 256));
+        k_tape<<<blocks, 256, 0, s>>>(h->B.pose, h->nb, h->tapePose + slot * tapePoseStride, h->S.contactRec, h->ctr,
+                                      h->cur.cr, h->tapeContact + slot * tapeContactStride);
     };
     if (tape)
         tapeAt(0);
@@ -977,9 +979,9 @@ int enqueue_iterate(avbd_handle *h)
         const int groups = lc.primal_threads / lc.primal_lanes;
         const int last = h->P.iterations - 1;
         const int numTasks = h->hostOffsets[colors];
-        // task-ordered stamps at a power-of-two stride (direct launches, no adjoint tape)
+        // task-ordered stamps at a power-of-two stride (direct launches)
         int padShift = -1;
-        if (!tape && numTasks > 0 && c.maxStampCount <= kPadStampsMax)
+        if (numTasks > 0 && c.maxStampCount <= kPadStampsMax)
         {
             padShift = 0;
             while ((1 << padShift) < std::max(c.maxStampCount, lc.primal_lanes))
@@ -990,7 +992,8 @@ int enqueue_iterate(avbd_handle *h)
         }
         // lazy dual: contact duals move into the next iteration's primal (owner stamps), joints keep
         // their own dual launch per iteration, and one contact dual follows the last iteration
-        const bool lazy = lc.lazy_dual && padShift >= 0 && c.contactCount > 0 && h->P.iterations > 0;
+        // (a taped step keeps one dual launch per iteration: the tape records each dual's flags)
+        const bool lazy = lc.lazy_dual && !tape && padShift >= 0 && c.contactCount > 0 && h->P.iterations > 0;
         if (lazy)
             k_lazy_bytes<<<1, 1, 0, s>>>(h->ctr, h->P.iterations);
         // flow form: one primal launch for every iteration and colour (contact-only worlds with the
@@ -1059,16 +1062,23 @@ int enqueue_iterate(avbd_handle *h)
                                                 dim3(lc.dual_threads), a.ptr, 0, s));
                     }
                 }
-                else if (dualLanes > 0)
+                else if (dualLanes > 0 || tape)
                 {
-                    DualArgs a(h, c.contactCount, dfl);
-                    HIP_TRY(hipLaunchKernel((const void *)k_dual, dim3(ceil_div(dualLanes, lc.dual_threads)),
+                    // a taped step's tape slot after this iteration is written by its dual launch
+                    DualTape dt;
+                    if (tape)
+                    {
+                        dt.pose = h->tapePose + (it + 1) * tapePoseStride;
+                        dt.contact = h->tapeContact + (it + 1) * tapeContactStride;
+                        dt.nb = h->nb;
+                    }
+                    DualArgs a(h, c.contactCount, dfl, -1, 1, dt);
+                    HIP_TRY(hipLaunchKernel((const void *)k_dual,
+                                            dim3(std::max(1, ceil_div(dualLanes + dt.nb, lc.dual_threads))),
                                             dim3(lc.dual_threads), a.ptr, 0, s));
                 }
                 if (mark && it == last)
                     HIP_TRY(hipEventRecord(mark[6], s));
-                if (tape)
-                    tapeAt(it + 1);
             }
         if (h->timing && flowGrid > 0)
         {

@@ -3056,12 +3056,21 @@ __device__ void dual_joint(Joint *jp, const Bodies &B, const Params &P)
 }
 
 // Dual pass of one iteration: one lane per contact, then one per joint.
-// flagTape (adjoint recording, may be null): per contact the dual's branch flags.
+// flagTape (adjoint recording, may be null): per contact slot the dual's branch flags.
 // staticRepeat (lazy-dual form's closing launch, 1 otherwise): a contact between two bodies that
 // are never solved (both static) has no owner stamp to carry its per-iteration dual updates; its
 // bodies never move, so its updates of all iterations are applied here one after another.
+// The adjoint tape's slot after this iteration, written by the dual launch (null pointers when no
+// step is taped): each contact's penalty and lambda after its update, and threads past the joints
+// copy the bodies' poses (the primal's results; the dual does not move them).
+struct DualTape
+{
+    float4 *pose = nullptr, *contact = nullptr;
+    int nb = 0;
+};
+
 __global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int4 *contactRec, int nc, Params P,
-                       unsigned char *flagTape, int staticRepeat)
+                       unsigned char *flagTape, int staticRepeat, DualTape tape)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nc)
@@ -3073,33 +3082,70 @@ __global__ void k_dual(Bodies B, Manifolds M, Joint *joints, int nj, const int4 
         int fl = 0;
         for (int r = 0; r < reps; ++r)
             fl = dual_contact(cr, M, B, P);
+        const int sl = cr.x * kMaxContacts + cr.y;
         if (flagTape)
-            flagTape[t] = (unsigned char)fl;
+            flagTape[sl] = (unsigned char)fl;
+        if (tape.contact)
+        {
+            tape.contact[2 * sl] = M.cr[(size_t)sl * kCR + CR_PEN];
+            tape.contact[2 * sl + 1] = M.cr[(size_t)sl * kCR + CR_LAM];
+        }
     }
     else if (t - nc < nj)
         dual_joint(&joints[t - nc], B, P);
+    else if (tape.pose && t - nc - nj < tape.nb)
+    {
+        const int i = t - nc - nj;
+        tape.pose[2 * i] = B.pose[4 * i + P_POS];
+        tape.pose[2 * i + 1] = B.pose[4 * i + P_ROT];
+    }
 }
 
 // Adjoint tape of the step being reversed: poses (pos, rot per body) and contact penalty/lambda
 // (per contact slot) after the warm start / force init and after every iteration's primal and dual.
-__global__ void k_tape_pose(const float4 *pose, int n, float4 *dst)
+// One launch: threads [0, n) the poses, the rest (grid-stride) the contacts.
+__global__ void k_tape(const float4 *pose, int n, float4 *poseDst, const int4 *contactRec, const Counters *ctr,
+                       const float4 *cr, float4 *contactDst)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n)
+    {
+        poseDst[2 * t] = pose[4 * t + P_POS];
+        poseDst[2 * t + 1] = pose[4 * t + P_ROT];
         return;
-    dst[2 * i] = pose[4 * i + P_POS];
-    dst[2 * i + 1] = pose[4 * i + P_ROT];
-}
-
-__global__ void k_tape_contacts(const int4 *contactRec, const Counters *ctr, const float4 *cr, float4 *dst)
-{
-    int nc = ctr->contactCount;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+    }
+    const int nc = ctr->contactCount;
+    for (int c = t - n; c < nc; c += gridDim.x * blockDim.x - n)
     {
         int4 r = contactRec[c];
         int sl = r.x * kMaxContacts + r.y;
-        dst[2 * sl] = cr[(size_t)sl * kCR + CR_PEN];
-        dst[2 * sl + 1] = cr[(size_t)sl * kCR + CR_LAM];
+        contactDst[2 * sl] = cr[(size_t)sl * kCR + CR_PEN];
+        contactDst[2 * sl + 1] = cr[(size_t)sl * kCR + CR_LAM];
+    }
+}
+
+// Device-to-device copies in one launch (the adjoint's checkpoints and step stores): entry k
+// copies words[k] 4-byte words from src[k] to dst[k]; start[k] is the running word total.
+constexpr int kCopyMax = 32;
+struct CopyList
+{
+    const unsigned *src[kCopyMax];
+    unsigned *dst[kCopyMax];
+    long long start[kCopyMax + 1];
+    int n;
+};
+
+__global__ void k_copy_many(CopyList L)
+{
+    const long long total = L.start[L.n];
+    for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < total;
+         w += (long long)gridDim.x * blockDim.x)
+    {
+        int k = 0;
+        while (w >= L.start[k + 1])
+            ++k;
+        const long long o = w - L.start[k];
+        L.dst[k][o] = L.src[k][o];
     }
 }
 

@@ -434,7 +434,11 @@ int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launc
  * then returns, for a loss L of the final state, dL/d(initial pos, rot, vel,
  * ang_vel) and dL/d(mass) (each body's moments scale with its mass) and
  * dL/d(friction) per body. It replays each checkpoint segment, recomputes every
- * step with its iterations taped, and sweeps the step in reverse: finishStep,
+ * step with its iterations taped, and sweeps the step in reverse. With
+ * checkpoint_every = 0 the forward itself tapes every step (tapes and step
+ * copies of all num_steps steps stay in HBM, sized by the handle's capacities:
+ * about (iterations + 1) * (32 B per body + 256 B per contact slot) per step)
+ * and the backward only sweeps, with the same result. The sweep: finishStep,
  * then per iteration the dual update and the primal colours in reverse order
  * (one transposed 6x6 LDL^T solve per body), the warm start and the force init
  * (collide's pose Jacobian included). The gradient is that of the FP32
