@@ -311,6 +311,7 @@ struct DualArgs
 // on the same GPU, which would keep two programs from stepping on one device.
 constexpr int kColorThreads = 1024;
 constexpr int kColorSmallRounds = 6;  // round limit of the fixpoint for a world of at most kWorldSapMax bodies
+constexpr long long kMaxSweepItems = 1LL << 26;  // sweep work-item capacity for the all-pairs worst case
 
 // Exclusive scan with the count either on the device (nptr) or given (nconst).
 // total (may be null) receives the sum, clamped to cap (a clamp sets capBit in the overflow flags).
@@ -1301,8 +1302,16 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
     h->maxSprings = std::max(config->max_springs, 0);
     h->maxManifolds = config->max_manifolds > 0 ? config->max_manifolds : 12 * N + 1024;
     h->maxPairs = config->max_pairs > 0 ? config->max_pairs : 16 * N + 4096;
-    // one work item per 64 interval visits: room for ~40 * 64 = 2560 sweep checks per body
-    h->maxChunks = 41 * N + h->maxPairs / 8 + 65536;
+    // One work item per 64 interval visits. The sweep axis is the axis of largest centre variance,
+    // so a single body far from the rest (one that fell through the ground, say) can turn it to an
+    // axis along which every interval overlaps every other: the sweep then visits all n(n-1)/2
+    // pairs, as the reference's does. Room for that worst case, up to kMaxSweepItems work items
+    // (20 bytes each); beyond it ~2560 checks per body and the capacity error.
+    {
+        const long long worst = (long long)N * (N - 1) / 2 / kSweepChunk + N + 1;
+        const long long typical = 41LL * N + h->maxPairs / 8 + 65536;
+        h->maxChunks = (int)std::max(typical, std::min(worst, kMaxSweepItems));
+    }
     h->maxAdj = 2 * (h->maxManifolds + h->maxJoints + h->maxSprings) + 16;
     h->maxStamps = 2 * kMaxContacts * h->maxManifolds + 6 * h->maxJoints + 2 * h->maxSprings + 16;
     int hs = 1024;

@@ -57,6 +57,33 @@ def test_settled_pile_step_equals_oracle(gpu, oracle, settled_pile):
     g.close()
 
 
+def test_pile_axis_flip_step_equals_oracle(gpu, oracle):
+    """Step 1909 of the 50,000-box pile: one box that fell through the ground (about 4 km below it
+    by then) makes z the axis of largest centre variance, and along z every interval of the flat
+    pile overlaps most others, so the sweep visits ~7.5e8 pairs (solver.cpp:499-556), 24 times the
+    settled pile's 31 M. The GPU's sweep work items must hold that, and the step must equal the
+    oracle's."""
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scenes.pile(50000)
+    prm = default_params(gravity=sc.gravity)
+    g = gpu.HipWorld.for_state(scenes.build_state(sc, gpu.Maker()), prm)
+    g.step(1908)
+    assert int(g.stats()["sweep_axis"]) == 0
+    pre = g.state()
+    g.step(1)
+    st = g.stats()
+    c = oracle.OracleWorld(pre, prm)
+    c.step(1)
+    print(f"pile50k step 1909: axis {int(st['sweep_axis'])}, {int(st['pair_checks'])} checks, "
+          f"{int(st['sphere_hits'])} hits", flush=True)
+    assert int(st["sweep_axis"]) == 2 and c.sweep_axis == 2
+    assert int(st["pair_checks"]) == c.pair_checks > 5 * 10 ** 8
+    np.testing.assert_array_equal(g.hits(), c.hits(), err_msg="SAP pair emission")
+    assert_states_equal(g.state(), c.state(), "pile50k, step 1909 (sweep along z)")
+    g.close()
+
+
 def _fp64_errors(oracle, pre, prm, post):
     """Per-field max relative error (scale max(1, |x|)) of the FP32 step `post` against the FP64
     oracle's step from `pre`; contact lambdas also against the largest |lambda| of the step, and
