@@ -257,10 +257,24 @@ FlowKernel flow_kernel(int lanes, int threads)
     return nullptr;
 }
 
+using AdaptiveKernel = void (*)(Bodies, Manifolds, const Joint *, const Spring *, const int4 *, const int4 *, int4,
+                                int, int4, Params, float4 *, unsigned char *, const int4 *, int, int);
+
+// primal_lanes = 0: lanes per body chosen per body from its stamp count (k_primal_adaptive).
+AdaptiveKernel adaptive_kernel(int threads)
+{
+    return threads == 64 ? &k_primal_adaptive<64>
+           : threads == 128 ? &k_primal_adaptive<128>
+           : threads == 256 ? &k_primal_adaptive<256>
+                            : nullptr;
+}
+
 bool valid_launch(const avbd_launch_config &c)
 {
     auto threads = [](int t) { return t == 64 || t == 128 || t == 256; };
-    return primal_kernel(c.primal_lanes, c.primal_threads) != nullptr && threads(c.dual_threads) &&
+    const bool primal = c.primal_lanes == 0 ? adaptive_kernel(c.primal_threads) != nullptr
+                                            : primal_kernel(c.primal_lanes, c.primal_threads) != nullptr;
+    return primal && threads(c.dual_threads) &&
            (c.use_graph == 0 || c.use_graph == 1) && (c.dataflow == 0 || c.dataflow == 1) &&
            (c.lazy_dual == 0 || c.lazy_dual == 1);
 }
@@ -284,6 +298,31 @@ struct PrimalArgs
     void *ptr[14];
     PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys = nullptr, unsigned char *flags = nullptr,
                const int4 *pad_ = nullptr, int padShift_ = 0, int lazyDual_ = 0);
+};
+
+// Adaptive-lanes launch of one colour: its segments (from the per-step readback of the colour
+// offsets) and the blocks each one takes.
+struct AdaptiveArgs
+{
+    Bodies B;
+    Manifolds M;
+    const Joint *joints;
+    const Spring *springs;
+    const int4 *stamps;
+    const int4 *tasks;
+    int4 seg;
+    int end;
+    int4 blk;
+    Params P;
+    float4 *sysTape;
+    unsigned char *flagTape;
+    const int4 *pad;
+    int padShift;
+    int lazyDual;
+    void *ptr[15];
+    AdaptiveArgs(const avbd_handle *h, int color, int threads, float4 *sys = nullptr, unsigned char *flags = nullptr,
+                 const int4 *pad_ = nullptr, int padShift_ = 0, int lazyDual_ = 0);
+    int grid() const { return blk.w; }
 };
 
 struct DualArgs
@@ -805,7 +844,8 @@ int enqueue_prepare(avbd_handle *h)
         k_radix_scatter<<<tiles, kRadixThreads, 0, s>>>(keysAlt, valsAlt, N, 8, h->radixHistScan, tiles, keys, S.colorBodies,
                                               0);
         k_color_offsets<<<1, 256, 0, s>>>(h->radixHistScan, tiles, N, S.colorOffsets);
-        k_tasks<<<ceil_div(N, 256), 256, 0, s>>>(S.colorBodies, S.stampStart, S.stampCount, N, S.tasks, h->dfDone);
+        k_tasks<<<ceil_div(N, 256), 256, 0, s>>>(S.colorBodies, S.stampStart, S.stampCount, S.color, N, S.tasks, h->dfDone,
+                                                 S.colorOffsets);
     }
     if (h->timing)
     {
@@ -822,6 +862,26 @@ PrimalArgs::PrimalArgs(const avbd_handle *h, int b, int e, float4 *sys, unsigned
 {
     void *p[14] = {&B, &M, &joints, &springs, &stamps, &tasks, &begin, &end, &P, &sysTape, &flagTape, &pad, &padShift,
                    &lazyDual};
+    std::memcpy(ptr, p, sizeof(p));
+}
+
+AdaptiveArgs::AdaptiveArgs(const avbd_handle *h, int color, int threads, float4 *sys, unsigned char *flags,
+                           const int4 *pad_, int padShift_, int lazyDual_)
+    : B(h->B), M(h->cur), joints(h->joints), springs(h->springs), stamps(h->S.stamps), tasks(h->S.tasks), P(h->P),
+      sysTape(sys), flagTape(flags), pad(pad_), padShift(padShift_), lazyDual(lazyDual_)
+{
+    const int *o = h->hostOffsets;
+    const int cb = o[color], ce = o[color + 1];
+    const int *b = o + kSegBase + kSegCount * color;
+    const int b32 = std::min(std::max(b[0], cb), ce), b16 = std::min(std::max(b[1], b32), ce),
+              b8 = std::min(std::max(b[2], b16), ce);
+    seg = make_int4(cb, b32, b16, b8);
+    end = ce;
+    const int n64 = ceil_div(b32 - cb, threads / 64), n32 = ceil_div(b16 - b32, threads / 32),
+              n16 = ceil_div(b8 - b16, threads / 16), n8 = ceil_div(ce - b8, threads / 8);
+    blk = make_int4(n64, n64 + n32, n64 + n32 + n16, n64 + n32 + n16 + n8);
+    void *p[15] = {&B, &M, &joints, &springs, &stamps, &tasks, &seg, &end, &blk, &P, &sysTape, &flagTape, &pad,
+                   &padShift, &lazyDual};
     std::memcpy(ptr, p, sizeof(p));
 }
 
@@ -860,11 +920,27 @@ hipKernelNodeParams node_params(const void *fn, int grid, int threads, void **ar
 // One solver iteration as a graph: the colour launches in order, then the dual launch. The
 // graph is rebuilt when its node count or launch shape changes; otherwise only the nodes'
 // grids and ranges are updated for the step.
-int iterate_graph(avbd_handle *h, const std::vector<std::pair<int, int>> &ranges, int nc, int dualLanes)
+int iterate_graph(avbd_handle *h, const std::vector<std::pair<int, int>> &ranges, const std::vector<int> &colours,
+                  int nc, int dualLanes)
 {
     const avbd_launch_config &lc = h->launch;
-    const void *pfn = (const void *)primal_kernel(lc.primal_lanes, lc.primal_threads);
-    const int groups = lc.primal_threads / lc.primal_lanes;
+    const bool adaptive = lc.primal_lanes == 0;
+    const void *pfn = adaptive ? (const void *)adaptive_kernel(lc.primal_threads)
+                               : (const void *)primal_kernel(lc.primal_lanes, lc.primal_threads);
+    const int groups = adaptive ? 1 : lc.primal_threads / lc.primal_lanes;
+    // the kernel node of colour range k (its argument block lives for the call that copies it)
+    auto withNode = [&](size_t k, auto &&call) -> hipError_t {
+        if (adaptive)
+        {
+            AdaptiveArgs a(h, colours[k], lc.primal_threads);
+            hipKernelNodeParams kp = node_params(pfn, a.grid(), lc.primal_threads, a.ptr);
+            return call(kp);
+        }
+        PrimalArgs a(h, ranges[k].first, ranges[k].second);
+        hipKernelNodeParams kp = node_params(pfn, ceil_div(ranges[k].second - ranges[k].first, groups),
+                                             lc.primal_threads, a.ptr);
+        return call(kp);
+    };
     const bool dual = dualLanes > 0;
     bool rebuild = !h->graphExec || h->graphPrimal.size() != ranges.size() || (h->graphDual != nullptr) != dual ||
                    h->graphLaunch.primal_lanes != lc.primal_lanes ||
@@ -874,12 +950,12 @@ int iterate_graph(avbd_handle *h, const std::vector<std::pair<int, int>> &ranges
         destroy_graph(h);
         HIP_TRY(hipGraphCreate(&h->graph, 0));
         hipGraphNode_t prev = nullptr;
-        for (const auto &r : ranges)
+        for (size_t k = 0; k < ranges.size(); ++k)
         {
-            PrimalArgs a(h, r.first, r.second);
-            hipKernelNodeParams kp = node_params(pfn, ceil_div(r.second - r.first, groups), lc.primal_threads, a.ptr);
             hipGraphNode_t node;
-            HIP_TRY(hipGraphAddKernelNode(&node, h->graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp));
+            HIP_TRY(withNode(k, [&](hipKernelNodeParams &kp) {
+                return hipGraphAddKernelNode(&node, h->graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
+            }));
             h->graphPrimal.push_back(node);
             prev = node;
         }
@@ -896,12 +972,9 @@ int iterate_graph(avbd_handle *h, const std::vector<std::pair<int, int>> &ranges
     else
     {
         for (size_t k = 0; k < ranges.size(); ++k)
-        {
-            PrimalArgs a(h, ranges[k].first, ranges[k].second);
-            hipKernelNodeParams kp = node_params(pfn, ceil_div(ranges[k].second - ranges[k].first, groups),
-                                                 lc.primal_threads, a.ptr);
-            HIP_TRY(hipGraphExecKernelNodeSetParams(h->graphExec, h->graphPrimal[k], &kp));
-        }
+            HIP_TRY(withNode(k, [&](hipKernelNodeParams &kp) {
+                return hipGraphExecKernelNodeSetParams(h->graphExec, h->graphPrimal[k], &kp);
+            }));
         if (dual)
         {
             DualArgs a(h, nc);
@@ -919,7 +992,7 @@ int enqueue_iterate(avbd_handle *h)
 {
     hipStream_t s = h->stream;
     HIP_TRY(hipMemcpyAsync(h->hostCtr, h->ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(h->hostOffsets, h->S.colorOffsets, sizeof(int) * 257, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h->hostOffsets, h->S.colorOffsets, sizeof(int) * kOffsetsLen, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     const Counters &c = *h->hostCtr;
     if (c.overflow || c.barrierTimeout)
@@ -933,9 +1006,13 @@ int enqueue_iterate(avbd_handle *h)
     // the unmeasured one.
     hipEvent_t *mark = h->timing ? &h->ring[avbd_handle::kSlots * (h->ringSteps % avbd_handle::kRing)] : nullptr;
     std::vector<std::pair<int, int>> ranges;
+    std::vector<int> rangeColours;
     for (int col = 0; col < colors; ++col)
         if (h->hostOffsets[col + 1] > h->hostOffsets[col])
+        {
             ranges.emplace_back(h->hostOffsets[col], h->hostOffsets[col + 1]);
+            rangeColours.push_back(col);
+        }
     const int primalPerIter = (int)ranges.size();
     const avbd_launch_config &lc = h->launch;
     h->lastNumManifolds = c.numManifolds;
@@ -957,7 +1034,7 @@ int enqueue_iterate(avbd_handle *h)
     }
     if (lc.use_graph && !tape && h->P.iterations > 0 && (primalPerIter > 0 || dualLanes > 0))
     {
-        int r = iterate_graph(h, ranges, c.contactCount, dualLanes);
+        int r = iterate_graph(h, ranges, rangeColours, c.contactCount, dualLanes);
         if (r)
             return r;
         // in graph form the last iteration is timed as a whole (its primal figure includes the dual)
@@ -976,8 +1053,10 @@ int enqueue_iterate(avbd_handle *h)
     }
     else
     {
-        const void *pfn = (const void *)primal_kernel(lc.primal_lanes, lc.primal_threads);
-        const int groups = lc.primal_threads / lc.primal_lanes;
+        const bool adaptive = lc.primal_lanes == 0;
+        const void *pfn = adaptive ? (const void *)adaptive_kernel(lc.primal_threads)
+                                   : (const void *)primal_kernel(lc.primal_lanes, lc.primal_threads);
+        const int groups = adaptive ? 1 : lc.primal_threads / lc.primal_lanes;
         const int last = h->P.iterations - 1;
         const int numTasks = h->hostOffsets[colors];
         // task-ordered stamps at a power-of-two stride (direct launches)
@@ -985,7 +1064,8 @@ int enqueue_iterate(avbd_handle *h)
         if (numTasks > 0 && c.maxStampCount <= kPadStampsMax)
         {
             padShift = 0;
-            while ((1 << padShift) < std::max(c.maxStampCount, lc.primal_lanes))
+            // the stride holds every lane of a body's group: adaptive groups have 8 lanes or more
+            while ((1 << padShift) < std::max(c.maxStampCount, adaptive ? 8 : lc.primal_lanes))
                 ++padShift;
             const long long work = (long long)numTasks << padShift;
             k_pad_stamps<<<(int)std::min<long long>(ceil_div(work, 256LL), 8192), 256, 0, s>>>(
@@ -1000,15 +1080,16 @@ int enqueue_iterate(avbd_handle *h)
         // flow form: one primal launch for every iteration and colour (contact-only worlds with the
         // lazy dual), its grid the number of blocks that are resident at once
         int flowGrid = 0;
+        const int flowLanes = adaptive ? 16 : lc.primal_lanes;  // the flow form has fixed lanes per body
         if (lc.dataflow && lazy && h->nj == 0 && h->ns == 0 && primalPerIter > 0)
         {
-            int &fb = h->flowBlocks[lc.primal_lanes == 8 ? 0 : lc.primal_lanes == 16 ? 1 : 2]
+            int &fb = h->flowBlocks[flowLanes == 8 ? 0 : flowLanes == 16 ? 1 : 2]
                                    [lc.primal_threads == 64 ? 0 : lc.primal_threads == 128 ? 1 : 2];
             if (fb == 0)
             {
                 int per = 0;
                 HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &per, (const void *)flow_kernel(lc.primal_lanes, lc.primal_threads), lc.primal_threads, 0));
+                    &per, (const void *)flow_kernel(flowLanes, lc.primal_threads), lc.primal_threads, 0));
                 fb = std::max(per, 0) * h->numCUs;
                 if (fb <= 0)
                     fb = -1;  // does not fit: per-colour launches
@@ -1019,7 +1100,7 @@ int enqueue_iterate(avbd_handle *h)
         {
             if (mark)
                 HIP_TRY(hipEventRecord(mark[4], s));
-            flow_kernel(lc.primal_lanes, lc.primal_threads)<<<flowGrid, lc.primal_threads, 0, s>>>(
+            flow_kernel(flowLanes, lc.primal_threads)<<<flowGrid, lc.primal_threads, 0, s>>>(
                 h->B, h->cur, h->S.tasks, h->S.colorOffsets, colors, h->S.color, h->dfDone, h->P, h->padStamps,
                 padShift, h->ctr);
             HIP_TRY(hipGetLastError());
@@ -1039,12 +1120,21 @@ int enqueue_iterate(avbd_handle *h)
                 float4 *sys = tape ? h->tapeSys + (size_t)it * 9 * h->maxBodies : nullptr;
                 unsigned char *sfl = tape ? h->tapeStampFlags + (size_t)it * h->maxStamps : nullptr;
                 unsigned char *dfl = tape ? h->tapeDualFlags + (size_t)it * kMaxContacts * h->maxManifolds : nullptr;
-                for (const auto &rg : ranges)
+                for (size_t q = 0; q < ranges.size(); ++q)
                 {
-                    PrimalArgs a(h, rg.first, rg.second, sys, sfl, padShift >= 0 ? h->padStamps : nullptr,
-                                 std::max(padShift, 0), lazy && it > 0 ? 1 : 0);
-                    HIP_TRY(hipLaunchKernel(pfn, dim3(ceil_div(rg.second - rg.first, groups)),
-                                            dim3(lc.primal_threads), a.ptr, 0, s));
+                    const int4 *pad = padShift >= 0 ? h->padStamps : nullptr;
+                    const int lz = lazy && it > 0 ? 1 : 0;
+                    if (adaptive)
+                    {
+                        AdaptiveArgs a(h, rangeColours[q], lc.primal_threads, sys, sfl, pad, std::max(padShift, 0), lz);
+                        HIP_TRY(hipLaunchKernel(pfn, dim3(a.grid()), dim3(lc.primal_threads), a.ptr, 0, s));
+                    }
+                    else
+                    {
+                        PrimalArgs a(h, ranges[q].first, ranges[q].second, sys, sfl, pad, std::max(padShift, 0), lz);
+                        HIP_TRY(hipLaunchKernel(pfn, dim3(ceil_div(ranges[q].second - ranges[q].first, groups)),
+                                                dim3(lc.primal_threads), a.ptr, 0, s));
+                    }
                 }
                 if (mark && it == last)
                     HIP_TRY(hipEventRecord(mark[5], s));
@@ -1357,7 +1447,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         (r = h->alloc(S.hash, h->hashSize)) || (r = h->alloc(S.keep, h->maxManifolds)) || (r = h->alloc(S.npRedo, h->maxManifolds)) ||
         (r = h->alloc(S.keepRank, h->maxManifolds)) || (r = h->alloc(S.deg, N)) || (r = h->alloc(S.adjStart, N)) ||
         (r = h->alloc(S.adjCursor, N)) || (r = h->alloc(S.adj, h->maxAdj)) || (r = h->alloc(S.adjSerial, h->maxAdj)) ||
-        (r = h->alloc(S.color, N)) || (r = h->alloc(S.colorOffsets, 257)) || (r = h->alloc(S.colorBodies, N)) ||
+        (r = h->alloc(S.color, N)) || (r = h->alloc(S.colorOffsets, kOffsetsLen)) || (r = h->alloc(S.colorBodies, N)) ||
         (r = h->alloc(h->ctr, 1)) || (r = h->alloc(h->gb, 1)) ||
         (r = h->alloc(h->scanStatus, (size_t)h->scanMaxTiles)) || (r = h->alloc(h->scanCtl, 4)) ||
         (r = h->alloc(h->padStamps, (size_t)N * kPadStampsMax)) ||
@@ -1365,7 +1455,7 @@ int avbd_create(const avbd_config *config, avbd_handle **out)
         return bail(r);
     S.hashMask = h->hashSize - 1;
     if ((e = hipHostMalloc((void **)&h->hostCtr, sizeof(Counters))) != hipSuccess ||
-        (e = hipHostMalloc((void **)&h->hostOffsets, sizeof(int) * 257)) != hipSuccess)
+        (e = hipHostMalloc((void **)&h->hostOffsets, sizeof(int) * kOffsetsLen)) != hipSuccess)
         return bail(fail(AVBD_ERR_HIP, std::string("hipHostMalloc: ") + hipGetErrorString(e)));
     if ((e = hipMemsetAsync(h->ctr, 0, sizeof(Counters), h->stream)) != hipSuccess ||
         (e = hipMemsetAsync(h->gb, 0, sizeof(GridBarrier), h->stream)) != hipSuccess ||
@@ -2403,7 +2493,7 @@ int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launc
         float ms;
     };
     std::vector<Shape> shapes;
-    for (int lanes : {8, 16, 32})
+    for (int lanes : {0, 8, 16, 32})  // 0: lanes per body by stamp count
         for (int threads : {64, 128, 256})
             shapes.push_back({lanes, threads, 3.0e38f});
     for (int pass = 0; pass < 2; ++pass)

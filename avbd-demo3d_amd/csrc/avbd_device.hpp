@@ -13,6 +13,12 @@ namespace avbd
 
 constexpr int kMaxContacts = 8;
 constexpr int kMaxColors = 255;      // colour 255 is the "unsolved" bucket of the colour sort
+// Adaptive primal lanes: segment q + 1 of a colour holds the bodies with at most seg_stamps(q)
+// (32, 16, 8) stamps; the segments' starts follow the 257 colour offsets in one device array.
+constexpr int kSegCount = 3;
+__host__ __device__ constexpr int seg_stamps(int q) { return 32 >> q; }
+constexpr int kSegBase = 257;
+constexpr int kOffsetsLen = kSegBase + kSegCount * 256;
 constexpr int kSweepChunk = 64;      // sorted intervals visited per sweep work item
 constexpr int kPadStampsMax = 64;  // largest per-task stride of the task-ordered stamp copy
 constexpr int kRadixTile = 2048;     // keys per radix-sort block (256 threads x 8)

@@ -2038,11 +2038,20 @@ __global__ void k_color_keys(const int *color, const int *stampCount, int n, uns
     block_max(mx, &ctr->colorCount, 0);
 }
 
+// colorOffsets[0..256]: the colour buckets' starts (and n); colorOffsets[kSegBase + 3 c + q]: where
+// the bodies of colour c with at most seg_stamps(q) stamps start (k_tasks writes the boundaries
+// that exist; a colour with no such body keeps its end, set here).
 __global__ void k_color_offsets(const int *scannedHist, int tiles, int n, int *colorOffsets)
 {
     int d = threadIdx.x;
     if (d < 256)
+    {
         colorOffsets[d] = scannedHist[d * tiles];
+        const int end = d + 1 < 256 ? scannedHist[(d + 1) * tiles] : n;
+#pragma unroll
+        for (int q = 0; q < kSegCount; ++q)
+            colorOffsets[kSegBase + kSegCount * d + q] = end;
+    }
     if (d == 0)
         colorOffsets[256] = n;
 }
@@ -2561,15 +2570,32 @@ __global__ void k_stamp_fill(const int *deg, const int *adjStart, const int *adj
 
 // Colour-sorted primal tasks: (body, first stamp, stamp count).
 // done: the flow primal's per-body count of finished iterations, zeroed for every solved body.
-__global__ void k_tasks(const int *colorBodies, const int *stampStart, const int *stampCount, int n, int4 *tasks,
-                        int *done)
+// Inside a colour the tasks run by descending stamp count; the task where the count first drops
+// to seg_stamps(q) or below starts segment q + 1 of the adaptive primal's lane counts.
+__global__ void k_tasks(const int *colorBodies, const int *stampStart, const int *stampCount, const int *color, int n,
+                        int4 *tasks, int *done, int *colorOffsets)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n)
         return;
     int i = colorBodies[k];
-    tasks[k] = make_int4(i, stampStart[i], stampCount[i], 0);
+    const int cnt = stampCount[i];
+    tasks[k] = make_int4(i, stampStart[i], cnt, 0);
     done[i] = 0;
+    const int c = color[i];
+    if (c < 0 || c >= kMaxColors)
+        return;
+    int cp = -1, cntp = 0;
+    if (k > 0)
+    {
+        const int ip = colorBodies[k - 1];
+        cp = color[ip];
+        cntp = stampCount[ip];
+    }
+#pragma unroll
+    for (int q = 0; q < kSegCount; ++q)
+        if (cnt <= seg_stamps(q) && (cp != c || cntp > seg_stamps(q)))
+            colorOffsets[kSegBase + kSegCount * c + q] = k;
 }
 
 __device__ __forceinline__ float init_entry(int e, float ml, float ax, float ay, float az, V3 dl, V3 da)
@@ -2845,6 +2871,53 @@ __global__ __launch_bounds__(T) void k_primal_color(Bodies B, Manifolds M, const
     const int4 st0 = (valid && pad) ? pad[((size_t)k << padShift) + lane] : make_int4(0, 0, 0, 0);
     primal_task<L, T, false>(rec, g, lane, valid, task, B, M, joints, springs, stamps, P, sysTape, flagTape, pad,
                              padShift, k, st0, lazyDual != 0);
+}
+
+// Adaptive lanes per body (launch config primal_lanes = 0): each body gets the fewest of 64, 32,
+// 16 or 8 lanes that hold all its stamps in one round, so no wave waits on a second round of
+// stamp loads and records. A colour's tasks (descending stamp count) are cut into four segments
+// at counts 32, 16 and 8 (k_tasks); the launch's blocks serve the segments in order, block b of
+// segment q running T / L_q bodies. Every body's sums are the same as with any fixed L.
+template <int L, int T>
+__device__ __forceinline__ void primal_segment(float *rec, int bb, int begin, int end, const Bodies &B,
+                                               const Manifolds &M, const Joint *joints, const Spring *springs,
+                                               const int4 *stamps, const int4 *tasks, const Params &P,
+                                               float4 *sysTape, unsigned char *flagTape, const int4 *pad,
+                                               int padShift, int lazyDual)
+{
+    constexpr int G = T / L;
+    const int lane = threadIdx.x % L, g = threadIdx.x / L;
+    const int k = begin + bb * G + g;
+    const bool valid = k < end;
+    const int4 task = valid ? tasks[k] : make_int4(0, 0, 0, 0);
+    const int4 st0 = (valid && pad) ? pad[((size_t)k << padShift) + lane] : make_int4(0, 0, 0, 0);
+    primal_task<L, T, false>(rec, g, lane, valid, task, B, M, joints, springs, stamps, P, sysTape, flagTape, pad,
+                             padShift, k, st0, lazyDual != 0);
+}
+
+// seg: the starts of the 64-, 32-, 16- and 8-lane segments of the colour; blk: the first block
+// of the 32-, 16- and 8-lane segments and the block count.
+template <int T>
+__global__ __launch_bounds__(T) void k_primal_adaptive(Bodies B, Manifolds M, const Joint *joints,
+                                                       const Spring *springs, const int4 *stamps, const int4 *tasks,
+                                                       int4 seg, int end, int4 blk, Params P, float4 *sysTape,
+                                                       unsigned char *flagTape, const int4 *pad, int padShift,
+                                                       int lazyDual)
+{
+    __shared__ float rec[T * (kRec + 2)];
+    const int b = blockIdx.x;
+    if (b < blk.x)
+        primal_segment<64, T>(rec, b, seg.x, seg.y, B, M, joints, springs, stamps, tasks, P, sysTape, flagTape, pad,
+                              padShift, lazyDual);
+    else if (b < blk.y)
+        primal_segment<32, T>(rec, b - blk.x, seg.y, seg.z, B, M, joints, springs, stamps, tasks, P, sysTape,
+                              flagTape, pad, padShift, lazyDual);
+    else if (b < blk.z)
+        primal_segment<16, T>(rec, b - blk.y, seg.z, seg.w, B, M, joints, springs, stamps, tasks, P, sysTape,
+                              flagTape, pad, padShift, lazyDual);
+    else
+        primal_segment<8, T>(rec, b - blk.z, seg.w, end, B, M, joints, springs, stamps, tasks, P, sysTape, flagTape,
+                             pad, padShift, lazyDual);
 }
 
 // Flow form of a step's solver iterations (contact-only worlds, lazy dual): ONE launch runs every

@@ -405,7 +405,9 @@ int avbd_batch_num_worlds(avbd_handle *h, int32_t *num_worlds);
  * results; shapes differ only in speed. Defaults: 16 lanes, 256 threads,
  * dual 256 threads, no graph, no dataflow. */
 typedef struct avbd_launch_config {
-    int32_t primal_lanes;    /* lanes per body in the primal kernel: 8, 16 or 32 */
+    int32_t primal_lanes;    /* lanes per body in the primal kernel: 8, 16 or 32; 0: per body the
+                                fewest of 8, 16, 32, 64 that hold its stamps (the flow form then
+                                uses 16) */
     int32_t primal_threads;  /* threads per primal block: 64, 128 or 256 */
     int32_t dual_threads;    /* threads per dual block: 64, 128 or 256 */
     int32_t use_graph;       /* 1: each solver iteration is one hipGraph launch */

@@ -78,9 +78,9 @@ def test_every_launch_shape_gives_the_same_state(gpu):
     ref.step(40)
     base = ref.state()
     ref.close()
-    shapes = [(l, t, d, gr, 0, 1) for l in (8, 16, 32) for t in (64, 128, 256) for d, gr in ((64, 0), (256, 1))]
+    shapes = [(l, t, d, gr, 0, 1) for l in (0, 8, 16, 32) for t in (64, 128, 256) for d, gr in ((64, 0), (256, 1))]
     shapes += [(l, t, 128, 0, 1, 1) for l, t in ((8, 128), (16, 64), (16, 256), (32, 256))]
-    shapes += [(l, t, 256, 0, 0, 0) for l, t in ((8, 64), (16, 128), (32, 256))]
+    shapes += [(l, t, 256, 0, 0, 0) for l, t in ((8, 64), (16, 128), (32, 256), (0, 64))]
     for l, t, d, gr, df, lz in shapes:
         w = gpu.HipWorld.for_state(st, prm)
         w.set_launch_config(l, t, d, use_graph=gr, dataflow=df, lazy_dual=lz)
@@ -96,7 +96,8 @@ def test_lazy_dual_equals_eager_with_joints_and_springs(gpu):
     """The lazy-dual form (contact duals applied by their owner body at the next iteration's
     primal, joint duals per iteration, one contact dual at the end) equals the per-iteration dual
     launches bit for bit, on a pyramid with a chain of joints and springs and on the 8x8x8 soft
-    body (joints, IgnoreCollision pairs, ground contacts)."""
+    body (joints, IgnoreCollision pairs, ground contacts); so do both with adaptive lanes per body
+    (primal_lanes 0)."""
     from avbd import named_scenes, scenes
     from avbd.state import default_params
     cases = [(scenes.pyramid_chain(base=12, links=200), 60), (named_scenes.build("Soft Body 8x8x8"), 40)]
@@ -104,14 +105,16 @@ def test_lazy_dual_equals_eager_with_joints_and_springs(gpu):
         prm = default_params(gravity=sc.gravity)
         st = scenes.build_state(sc, gpu.Maker())
         out = []
-        for lz in (0, 1):
+        for lanes, lz in ((16, 0), (16, 1), (0, 1), (0, 0)):
             w = gpu.HipWorld.for_state(st, prm)
-            w.set_launch_config(16, 64, 256, 0, 0, lazy_dual=lz)
+            w.set_launch_config(lanes, 64, 256, 0, 0, lazy_dual=lz)
             w.step(steps)
             w.synchronize()
             out.append(w.state())
             w.close()
         assert_states_equal(out[1], out[0], f"{sc.name}: lazy vs eager dual")
+        assert_states_equal(out[2], out[0], f"{sc.name}: adaptive lanes, lazy dual")
+        assert_states_equal(out[3], out[0], f"{sc.name}: adaptive lanes, eager dual")
 
 
 @pytest.mark.parametrize("lanes,threads", [(16, 64), (8, 128)])
@@ -154,7 +157,7 @@ def test_autotune_keeps_state_and_picks_a_valid_shape(gpu):
     tuned = gpu.HipWorld.for_batch(states, prm)
     tuned.step(2)
     cfg = tuned.autotune(trial_steps=2)
-    assert cfg["primal_lanes"] in (8, 16, 32) and cfg["primal_threads"] in (64, 128, 256)
+    assert cfg["primal_lanes"] in (0, 8, 16, 32) and cfg["primal_threads"] in (64, 128, 256)
     assert cfg["dual_threads"] in (64, 128, 256) and cfg["use_graph"] in (0, 1)
     assert tuned.launch_config() == cfg
     tuned.step(3)

@@ -45,9 +45,13 @@ def settled_pile(gpu):
     return prm, pre
 
 
-def test_settled_pile_step_equals_oracle(gpu, oracle, settled_pile):
+@pytest.mark.parametrize("lanes,threads", [(16, 256), (0, 64)])
+def test_settled_pile_step_equals_oracle(gpu, oracle, settled_pile, lanes, threads):
+    """One settled step at the default launch shape and with adaptive lanes per body (the settled
+    pile has bodies in all four lane segments: more than 32 stamps, 17-32, 9-16, at most 8)."""
     prm, pre = settled_pile
     g = gpu.HipWorld.for_state(pre, prm)
+    g.set_launch_config(lanes, threads, 256, use_graph=0, dataflow=0, lazy_dual=1)
     c = oracle.OracleWorld(pre, prm)
     g.step(1)
     c.step(1)

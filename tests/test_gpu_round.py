@@ -73,7 +73,8 @@ def test_round_every_launch_shape_identical(gpu):
     st = scenes.build_state(sc, gpu.Maker())
     base = None
     for lanes, threads, dual, graph, lazy in [(16, 256, 256, 0, 1), (8, 64, 64, 0, 1), (32, 128, 128, 1, 1),
-                                              (16, 64, 256, 1, 1), (16, 256, 256, 0, 0), (8, 64, 64, 0, 0)]:
+                                              (16, 64, 256, 1, 1), (16, 256, 256, 0, 0), (8, 64, 64, 0, 0),
+                                              (0, 64, 256, 0, 1), (0, 128, 128, 1, 1), (0, 256, 64, 0, 0)]:
         w = gpu.HipWorld.for_state(st, prm)
         w.set_launch_config(lanes, threads, dual, graph, 0, lazy)
         w.step(60)
