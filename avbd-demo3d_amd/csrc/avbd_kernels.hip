@@ -2626,6 +2626,41 @@ __device__ __forceinline__ int sys_entry(int r, int k)
                  : (k < 3 ? R_C + 3 * (r - 3) + k : R_A00 + (r - 3) * (r - 2) / 2 + (k - 3));
 }
 
+// Value of lane j (0..15) of this lane's row of 16 lanes (a DPP row broadcast). Every lane of the
+// wave must be active; j must fold to a constant (it selects the DPP control code).
+__device__ __forceinline__ float row_bcast(float x, int j)
+{
+    const int v = __float_as_int(x);
+    int out = v;
+    switch (j)
+    {
+#define AVBD_ROW_BCAST(n)                                                                                  \
+    case n: out = __builtin_amdgcn_update_dpp(0, v, 0x150 + n, 0xF, 0xF, false); break;
+        AVBD_ROW_BCAST(0) AVBD_ROW_BCAST(1) AVBD_ROW_BCAST(2) AVBD_ROW_BCAST(3) AVBD_ROW_BCAST(4)
+        AVBD_ROW_BCAST(5) AVBD_ROW_BCAST(6) AVBD_ROW_BCAST(7) AVBD_ROW_BCAST(8) AVBD_ROW_BCAST(9)
+        AVBD_ROW_BCAST(10) AVBD_ROW_BCAST(11) AVBD_ROW_BCAST(12) AVBD_ROW_BCAST(13) AVBD_ROW_BCAST(14)
+        AVBD_ROW_BCAST(15)
+#undef AVBD_ROW_BCAST
+    default: break;
+    }
+    return __int_as_float(out);
+}
+
+// Value of lane j of this lane's group of L lanes (groups are aligned to L): a row broadcast for
+// L >= 16; for L = 8, two groups share a row and each half takes its own broadcast.
+template <int L>
+__device__ __forceinline__ float group_bcast(float x, int j)
+{
+    static_assert(L >= 8 && L % 8 == 0, "groups of 8 lanes or more");
+    if constexpr (L >= 16)
+        return row_bcast(x, j);
+    else
+    {
+        const float lo = row_bcast(x, j), hi = row_bcast(x, 8 + j);
+        return (threadIdx.x & 8) ? hi : lo;
+    }
+}
+
 // LDS hand-off between the lanes of one wave: a group's 16 lanes always sit in one wave, so
 // ordering the wave's own LDS writes before its reads is enough (no block barrier).
 __device__ __forceinline__ void wave_lds_sync()
@@ -2747,22 +2782,22 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
     // owns row r + 1, the columns are factored in order (the column's owner broadcasts its
     // diagonal and its row of L, every lower row forms its entry with solve6's own expression),
     // the forward substitution broadcasts each y_k, each lane divides its own z; lane 0 then runs
-    // the back substitution from L and z staged in LDS. Every value is computed by the same
+    // the back substitution from L and z broadcast to it. Broadcasts inside the group are DPP row
+    // broadcasts (a group's lanes 0..5 sit in one row of 16). Every value is computed by the same
     // operations in the same order as solve6, so the result is bit-identical. All lanes of the
-    // wave run this part (invalid groups on junk, never stored): the shuffles need every lane.
+    // wave run this part (invalid groups on junk, never stored): the broadcasts need every lane.
     V3 dxL, dxA;
     {
         const float *a = rec + g * L * kRow;
         const int r = lane < 6 ? lane : 5;
-        const int base = (int)(threadIdx.x & 63) - lane;
         float arow[6], lrow[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, d[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k)
             arow[k] = k <= r ? a[sys_entry(r, k)] : 0.0f;
         float myD = 0.0f;
-        d[0] = __shfl(arow[0], base, 64);  // D1 = A11
+        d[0] = group_bcast<L>(arow[0], 0);  // D1 = A11
         myD = r == 0 ? d[0] : myD;
-        lrow[0] = arow[0] / d[0];          // L_i1 = A_i1 / A11
+        lrow[0] = arow[0] / d[0];           // L_i1 = A_i1 / A11
 #pragma unroll
         for (int j = 1; j < 6; ++j)
         {
@@ -2770,13 +2805,13 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
             float lj[6];
 #pragma unroll
             for (int k = 0; k < j; ++k)
-                lj[k] = __shfl(lrow[k], base + j, 64);
+                lj[k] = group_bcast<L>(lrow[k], j);
             // D_j = A_jj - (sum_k (L_jk L_jk) D_k), summed left to right (computed by its owner)
             float sum = (lrow[0] * lrow[0]) * d[0];
 #pragma unroll
             for (int k = 1; k < j; ++k)
                 sum = sum + (lrow[k] * lrow[k]) * d[k];
-            d[j] = __shfl(arow[j] - sum, base + j, 64);
+            d[j] = group_bcast<L>(arow[j] - sum, j);
             myD = r == j ? d[j] : myD;
             // L_ij = (A_ij - L_j1 L_i1 D_1 - ... ) / D_j for the rows below
             float t = arow[j];
@@ -2791,27 +2826,25 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
 #pragma unroll
         for (int k = 0; k < 5; ++k)
         {
-            const float yk = __shfl(y, base + k, 64);
+            const float yk = group_bcast<L>(y, k);
             if (r > k)
                 y = y - lrow[k] * yk;
         }
         const float z = y / myD;
-        // stage L (row-major, 6 per row) and z in the group's second LDS row for lane 0
-        float *m = rec + (g * L + 1) * kRow;
-        wave_lds_sync();
-        if (lane < 6)
-        {
+        // L (1-based L_ik = lrow[k - 1] of lane i - 1) and z for lane 0
+        float Lg[6][6], zg[6];
 #pragma unroll
-            for (int k = 0; k < 5; ++k)
-                if (k < r)
-                    m[6 * r + k] = lrow[k];
-            m[6 * r + 5] = z;
+        for (int i = 0; i < 6; ++i)
+        {
+            zg[i] = group_bcast<L>(z, i);
+#pragma unroll
+            for (int k = 0; k < i; ++k)
+                Lg[i][k] = group_bcast<L>(lrow[k], i);
         }
-        wave_lds_sync();
         if (lane == 0)
         {
-            auto Lm = [&](int i, int k) { return m[6 * (i - 1) + (k - 1)]; };  // L_ik, 1-based
-            const float z1 = m[5], z2 = m[11], z3 = m[17], z4 = m[23], z5 = m[29], z6 = m[35];
+            auto Lm = [&](int i, int k) { return Lg[i - 1][k - 1]; };  // L_ik, 1-based
+            const float z1 = zg[0], z2 = zg[1], z3 = zg[2], z4 = zg[3], z5 = zg[4], z6 = zg[5];
             dxA.z = z6;
             dxA.y = z5 - Lm(6, 5) * dxA.z;
             dxA.x = z4 - Lm(5, 4) * dxA.y - Lm(6, 4) * dxA.z;
