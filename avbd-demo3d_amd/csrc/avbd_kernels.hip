@@ -2198,26 +2198,57 @@ __device__ __forceinline__ void contact_anchors(float4 rA4, float4 rB4, float4 c
     rBW = contact_offset((code >> 2) & 3, bb1.w, c04.w, cap, qB, ld3(rB4), n, false);
 }
 
-// Co: the partner's current pose is read coherently (dataflow primal); the own body's rows and
-// the constant rows (x-) are ordinary loads.
-// dualP (lazy-dual form, null otherwise): this body owns the contact point's dual update, which
-// is applied first.
-template <bool Co = false>
-__device__ __forceinline__ int contact_record(float *r, int i, int partner, bool isA, int sl, const Manifolds &M,
-                                              const Bodies &B, float oma, const Params *dualP = nullptr)
+// What one contact stamp reads: the contact record's rows and both bodies' pose records.
+// Co: the current poses and the penalty / lambda row are read coherently (dataflow primal); the
+// constant rows (x-) are ordinary loads.
+struct ContactIn
 {
-    int2 ab = isA ? make_int2(i, partner) : make_int2(partner, i);
+    float4 rA4, rB4, c04, bb0, bb1, bb2, pen4, lam4, pA4, pB4, qA4, qB4, iA4, iB4, jA4, jB4;
+};
+
+template <bool Co = false>
+__device__ __forceinline__ ContactIn contact_load(int i, int partner, bool isA, int sl, const Manifolds &M,
+                                                  const Bodies &B)
+{
+    ContactIn in;
+    const int2 ab = isA ? make_int2(i, partner) : make_int2(partner, i);
+    const float4 *cr = M.cr + (size_t)sl * kCR;
+    in.rA4 = cr[CR_RA];
+    in.rB4 = cr[CR_RB];
+    in.c04 = cr[CR_C0];
+    in.bb0 = cr[CR_N];
+    in.bb1 = cr[CR_T1];
+    in.bb2 = cr[CR_T2];
+    in.pen4 = Co ? ld_coherent(&cr[CR_PEN]) : cr[CR_PEN];
+    in.lam4 = Co ? ld_coherent(&cr[CR_LAM]) : cr[CR_LAM];
+    const float4 *pa = B.pose + 4 * ab.x, *pb = B.pose + 4 * ab.y;
+    in.pA4 = Co ? ld_coherent(pa + P_POS) : pa[P_POS];
+    in.pB4 = Co ? ld_coherent(pb + P_POS) : pb[P_POS];
+    in.qA4 = Co ? ld_coherent(pa + P_ROT) : pa[P_ROT];
+    in.qB4 = Co ? ld_coherent(pb + P_ROT) : pb[P_ROT];
+    in.iA4 = B.pose[4 * ab.x + P_ILIN];
+    in.iB4 = B.pose[4 * ab.y + P_ILIN];
+    in.jA4 = B.pose[4 * ab.x + P_IANG];
+    in.jB4 = B.pose[4 * ab.y + P_IANG];
+    return in;
+}
+
+// One contact of a manifold as seen by one of its bodies, from its loaded rows `in`.
+// dualP (lazy-dual form, null otherwise): this body owns the contact point's dual update, which
+// is applied first (and stored to contact slot sl).
+template <bool Co = false>
+__device__ __forceinline__ int contact_record(float *r, const ContactIn &in, bool isA, int sl, const Manifolds &M,
+                                              float oma, const Params *dualP = nullptr)
+{
     float4 *cr = M.cr + (size_t)sl * kCR;
-    float4 rA4 = cr[CR_RA], rB4 = cr[CR_RB], c04 = cr[CR_C0], bb0 = cr[CR_N], bb1 = cr[CR_T1], bb2 = cr[CR_T2];
-    float4 pen4 = Co ? ld_coherent(&cr[CR_PEN]) : cr[CR_PEN], lam4 = Co ? ld_coherent(&cr[CR_LAM]) : cr[CR_LAM];
-    auto cur = [&](int b, int row) { return Co ? ld_coherent(&B.pose[4 * b + row]) : B.pose[4 * b + row]; };
-    float4 pA4 = cur(ab.x, P_POS), pB4 = cur(ab.y, P_POS), qA4 = cur(ab.x, P_ROT), qB4 = cur(ab.y, P_ROT);
-    float4 iA4 = B.pose[4 * (ab.x) + P_ILIN], iB4 = B.pose[4 * (ab.y) + P_ILIN], jA4 = B.pose[4 * (ab.x) + P_IANG], jB4 = B.pose[4 * (ab.y) + P_IANG];
-    Q4 qA = ldq(qA4), qB = ldq(qB4);
-    V3 dqAL = ld3(pA4) - ld3(iA4);
-    V3 dqAA = qdiff(qA, ldq(jA4));
-    V3 dqBL = ld3(pB4) - ld3(iB4);
-    V3 dqBA = qdiff(qB, ldq(jB4));
+    const float4 rA4 = in.rA4, rB4 = in.rB4, c04 = in.c04, bb0 = in.bb0, bb1 = in.bb1, bb2 = in.bb2;
+    float4 lam4 = in.lam4;
+    const float4 pen4 = in.pen4;
+    Q4 qA = ldq(in.qA4), qB = ldq(in.qB4);
+    V3 dqAL = ld3(in.pA4) - ld3(in.iA4);
+    V3 dqAA = qdiff(qA, ldq(in.jA4));
+    V3 dqBL = ld3(in.pB4) - ld3(in.iB4);
+    V3 dqBA = qdiff(qB, ldq(in.jB4));
     V3 b0 = ld3(bb0), b1 = ld3(bb1), b2 = ld3(bb2);
     V3 nb0 = -b0, nb1 = -b1, nb2 = -b2;
     V3 rAW, rBW;
@@ -2696,43 +2727,35 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
     const int i = task.x;
     const int s0 = task.y;
     const int ns = valid ? task.z : 0;
-    int rounds = (ns + L - 1) / L;
-#pragma unroll
-    for (int off = 32; off >= L; off >>= 1)
-        rounds = max(rounds, __shfl_xor(rounds, off, 64));
-    float4 mm = B.massMoment[i];
-    V3 pos = ld3(Co ? ld_coherent(&B.pose[4 * i + P_POS]) : B.pose[4 * i + P_POS]);
-    Q4 rot = ldq(Co ? ld_coherent(&B.pose[4 * i + P_ROT]) : B.pose[4 * i + P_ROT]);
-    float dt2 = P.dt * P.dt;
-    float ml = mm.x / dt2, ax = mm.y / dt2, ay = mm.z / dt2, az = mm.w / dt2;
-    V3 dl = pos - ld3(B.inerLin[i]);
-    V3 da = qdiff(rot, ldq(B.inerAng[i]));
-    float acc[E];
-#pragma unroll
-    for (int q = 0; q < E; ++q)
-        acc[q] = lane + q * L < kRec ? init_entry(lane + q * L, ml, ax, ay, az, dl, da) : 0.0f;
+    const float4 mm = B.massMoment[i];
+    const float4 pos4 = Co ? ld_coherent(&B.pose[4 * i + P_POS]) : B.pose[4 * i + P_POS];
+    const float4 rot4 = Co ? ld_coherent(&B.pose[4 * i + P_ROT]) : B.pose[4 * i + P_ROT];
+    const float4 il4 = B.inerLin[i], ia4 = B.inerAng[i];
+    V3 pos = ld3(pos4);
+    Q4 rot = ldq(rot4);
     const float oma = 1.0f - P.alpha;
-    for (int r = 0; r < rounds; ++r)
-    {
+    // Round r's records (stamps r L + lane) straight into this lane's LDS row; a contact writes
+    // all of it, the other kinds write part of it and the row is cleared first.
+    auto writeRound = [&](int r) {
         int t = r * L + lane;
-        // the record is written straight into this lane's LDS row; a contact writes all of it,
-        // the other kinds write part of it and the row is cleared first
         float *row = rec + (g * L + lane) * kRow;
         if (valid && t < ns)
         {
+            // the flow form (Co) always has task-ordered stamps and only contacts
             int4 st;
-            if (!pad)
+            if (!Co && !pad)
                 st = stamps[s0 + t];
             else if (r == 0)
                 st = st0;
             else
                 st = pad[((size_t)kpos << padShift) + t];
             int kind = adjKind(st.x), idx = adjIndex(st.x);
-            if (kind == kAdjManifold)
+            if (Co || kind == kAdjManifold)
             {
                 const bool owner = lazyDual && (st.y & kStampDualOwner) != 0;
-                int fl = contact_record<Co>(row, i, st.z, ((st.y >> 16) & 1) != 0, st.w, M, B, oma,
-                                            owner ? &P : nullptr);
+                const bool isA = ((st.y >> 16) & 1) != 0;
+                const ContactIn in = contact_load<Co>(i, st.z, isA, st.w, M, B);
+                int fl = contact_record<Co>(row, in, isA, st.w, M, oma, owner ? &P : nullptr);
                 if (flagTape)
                     flagTape[s0 + t] = (unsigned char)fl;
             }
@@ -2753,6 +2776,10 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
             for (int q = 0; q < kRec; ++q)
                 row[q] = 0.0f;
         }
+    };
+    // The records of round r added entry by entry in stamp order.
+    float acc[E];
+    auto sumRound = [&](int r) {
         wave_lds_sync();
         int cnt = valid ? min(L, ns - r * L) : 0;
         float v[E][L];
@@ -2772,6 +2799,28 @@ __device__ __forceinline__ void primal_task(float *rec, int g, int lane, bool va
             }
         }
         wave_lds_sync();
+    };
+    // Round 0 first: its stamp loads go out next to the body's own rows, and the inertia terms
+    // (divisions by dt^2, the quaternion difference) are formed while they are in flight.
+    writeRound(0);
+    {
+        float dt2 = P.dt * P.dt;
+        float ml = mm.x / dt2, ax = mm.y / dt2, ay = mm.z / dt2, az = mm.w / dt2;
+        V3 dl = pos - ld3(il4);
+        V3 da = qdiff(rot, ldq(ia4));
+#pragma unroll
+        for (int q = 0; q < E; ++q)
+            acc[q] = lane + q * L < kRec ? init_entry(lane + q * L, ml, ax, ay, az, dl, da) : 0.0f;
+    }
+    sumRound(0);
+    int rounds = (ns + L - 1) / L;
+#pragma unroll
+    for (int off = 32; off >= L; off >>= 1)
+        rounds = max(rounds, __shfl_xor(rounds, off, 64));
+    for (int r = 1; r < rounds; ++r)
+    {
+        writeRound(r);
+        sumRound(r);
     }
 #pragma unroll
     for (int e = 0; e < E; ++e)
