@@ -10,8 +10,9 @@ is K steps after W warm-up steps, bracketed by device synchronisation.
 
 Printed JSON line:
   value        dynamic bodies x K / wall seconds (body-steps/s)
-  roofline     the dominant kernel, k_primal_color (one launch per colour per
-               iteration): algorithmic bytes per launch (from the step's own
+  roofline     the dominant kernel, the primal (k_primal_adaptive or k_primal_color:
+               one launch per colour per iteration; k_primal_flow: one per
+               step): algorithmic bytes per launch (from the step's own
                counts: solved bodies, contact/joint/spring endpoints) over its
                average HIP-event duration, against 8 TB/s HBM; the whole
                iteration loop and k_dual are reported beside it
@@ -222,7 +223,9 @@ def main():
     # that runs every iteration
     pr_iters = max(int(st["primal_iterations"]), 1)
     flow = pr_iters > timed
-    pr_kernel = "k_primal_flow" if flow else "k_primal_color"
+    # per-colour launches: k_primal_adaptive (lanes per body by stamp count) or k_primal_color
+    pr_kernel = "k_primal_flow" if flow else ("k_primal_adaptive" if launch.get("primal_lanes") == 0
+                                              else "k_primal_color")
     pr_bytes = float(st["primal_bytes_total"]) / steps_counted / iters * pr_iters / pr_launches
     achieved = pr_bytes / (pr_ms * 1e-3) / 1e9
     du_launches = max(int(st["dual_launches"]), 1)
