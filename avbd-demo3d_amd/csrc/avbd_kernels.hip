@@ -522,6 +522,43 @@ __global__ void k_step_begin(Counters *c)
 // (solver.cpp:499-515). The decision is taken in double; only when two variances lie within
 // the float sums' error bound of each other is the exact serial float pass run.
 // ------------------------------------------------------------------------------------------
+// Sweep axis from double sums s = (sum x, sum y, sum z, sum x^2, sum y^2, sum z^2) of n centres:
+// the axis of largest variance, and whether the reference's own float computation could pick
+// another (solver.cpp:499-515: a serial float sum for the mean, then a serial float sum of the
+// squared deviations, first maximum wins). Per axis, the float variance lies within
+//   [V (1 - t) - e, (V + n d^2)(1 + t) + e]
+// of the true one V: t = 4 (n + 16) 2^-24 bounds the rounding of the n-term serial sums (four
+// times the textbook bound); d bounds the float mean's error, (n + 1) 2^-24 sum|c| / n with
+// sum|c| <= sqrt(n sum c^2), and shifting every deviation by d adds n d^2 to their squares'
+// sum (which matters for a world far from the origin, where sum c^2 >> V); e covers this double
+// estimate's own rounding. Overlapping intervals make the decision ambiguous: the caller then
+// runs the reference's float computation serially.
+__device__ __forceinline__ int axis_decide(const double s[6], int n, bool &amb)
+{
+    const double u = 5.960464477539063e-08, ud = 1.1102230246251565e-16;
+    const double tol = 4.0 * ((double)n + 16.0) * u + 1e-9;
+    const double g = ((double)n + 1.0) * u * 1.01;
+    double v[3], lo[3], hi[3];
+    for (int k = 0; k < 3; ++k)
+    {
+        const double mean = s[k] / n;
+        v[k] = s[k + 3] - n * mean * mean;
+        const double e = ((double)n + 8.0) * 2.0 * ud * s[k + 3];
+        lo[k] = (v[k] - e) * (1.0 - tol);
+        hi[k] = (v[k] + e + g * g * s[k + 3]) * (1.0 + tol);
+    }
+    int best = 0;
+    if (v[1] > v[best])
+        best = 1;
+    if (v[2] > v[best])
+        best = 2;
+    amb = false;
+    for (int k = 0; k < 3; ++k)
+        if (k != best && hi[k] >= lo[best])
+            amb = true;
+    return best;
+}
+
 __global__ void k_axis_partial(const float4 *pose, int n, double *partials)
 {
     __shared__ double red[6][256];
@@ -550,10 +587,10 @@ __global__ void k_axis_partial(const float4 *pose, int n, double *partials)
         partials[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Decision on the summed statistics; flags the exact serial pass when two variances are within
-// the error bound of the reference's n-term serial float sums.
+// Decision on the summed statistics (axis_decide); flags the exact serial pass when the
+// reference's float computation could pick another axis.
 // One wave: lane b loads block b's six partial sums and the wave adds them by butterfly (any
-// order of the double sums is within the tolerance band below).
+// order of the double sums is within axis_decide's bound on this estimate's rounding).
 __global__ void k_axis_decide(const double *partials, int nparts, int n, Counters *ctr)
 {
     const int lane = threadIdx.x;
@@ -571,24 +608,10 @@ __global__ void k_axis_decide(const double *partials, int nparts, int n, Counter
             t[k] += __shfl_xor(t[k], off, 64);
     if (lane != 0)
         return;
-    double v[3];
-    for (int k = 0; k < 3; ++k)
-    {
-        double mean = t[k] / n;
-        v[k] = t[k + 3] - n * mean * mean;
-    }
-    int best = 0;
-    if (v[1] > v[best])
-        best = 1;
-    if (v[2] > v[best])
-        best = 2;
-    double tol = 4.0 * ((double)n + 16.0) * 5.960464477539063e-08 + 1e-9;
-    int amb = 0;
-    for (int k = 0; k < 3; ++k)
-        if (k != best && v[k] >= v[best] * (1.0 - tol))
-            amb = 1;
+    bool amb = false;
+    const int best = axis_decide(t, n, amb);
     ctr->sweepAxis = best;
-    ctr->axisAmbiguous = amb;
+    ctr->axisAmbiguous = amb ? 1 : 0;
 }
 
 __global__ void k_axis_serial(const float4 *pose, int n, Counters *ctr)
@@ -720,22 +743,11 @@ __global__ __launch_bounds__(256) void k_world_sap(const float4 *pose, const flo
     }
     if (t == 0)
     {
-        double v[3];
-        for (int k = 0; k < 3; ++k)
-        {
-            double mean = red[k][0] / n;
-            v[k] = red[k + 3][0] - n * mean * mean;
-        }
-        int best = 0;
-        if (v[1] > v[best])
-            best = 1;
-        if (v[2] > v[best])
-            best = 2;
-        double tol = 4.0 * ((double)n + 16.0) * 5.960464477539063e-08 + 1e-9;
+        double sums[6];
+        for (int k = 0; k < 6; ++k)
+            sums[k] = red[k][0];
         bool amb = false;
-        for (int k = 0; k < 3; ++k)
-            if (k != best && v[k] >= v[best] * (1.0 - tol))
-                amb = true;
+        int best = axis_decide(sums, n, amb);
         if (amb && n > 0)
         {
             V3 mean{0.0f, 0.0f, 0.0f};

@@ -192,3 +192,75 @@ def test_batch_upload_rejects_bad_worlds(gpu):
     w.upload_batch(batch, off)
     w.step(2)
     w.synchronize()
+
+
+def _grid_world(n_side, pitch_x, pitch_y, offset=(0.0, 0.0), name="grid"):
+    """Unit boxes on an n_side x n_side grid resting on a ground slab centred under it: with equal
+    pitches the centres' x and y variances are equal, so the sweep axis is decided by the float
+    rounding of the reference's serial sums."""
+    from avbd.scenes import _Builder
+    sb = _Builder()
+    ox, oy = offset
+    span = n_side * max(pitch_x, pitch_y) + 4.0
+    sb.body((span, span, 1.0), 0.0, 0.5, (ox, oy, -0.5))
+    for i in range(n_side):
+        for j in range(n_side):
+            sb.body((1.0, 1.0, 1.0), 1.0, 0.5,
+                    (ox + (i - (n_side - 1) / 2.0) * pitch_x, oy + (j - (n_side - 1) / 2.0) * pitch_y, 0.5))
+    return sb.scene(name, -9.81, 10)
+
+
+@pytest.mark.parametrize("batched", [False, True], ids=["single", "batched"])
+def test_sweep_axis_ties_and_far_worlds_take_the_serial_pass(gpu, oracle, batched):
+    """Worlds whose two largest centre variances are equal (a square grid) or within the float
+    rounding of the reference's serial mean for a world 10 km from the origin (pitches 0.02 %
+    apart): the double-precision estimate cannot tell which axis the reference's float sums pick,
+    so the exact serial pass runs (axis_fallbacks > 0) and every step equals the oracle."""
+    from avbd import scenes
+    from avbd.state import default_params, split_state
+    prm = default_params(gravity=-9.81)
+    ws = [_grid_world(20, 1.5, 1.5, name="tie"), _grid_world(20, 1.5, 1.5003, (1.0e4, -1.0e4), name="far")]
+    cpu = [oracle.OracleWorld(scenes.build_state(w, oracle.OracleMaker()), prm) for w in ws]
+    if batched:
+        handles = [gpu.HipWorld.for_batch([scenes.build_state(w, gpu.Maker()) for w in ws], prm)]
+    else:
+        handles = [gpu.HipWorld.for_state(scenes.build_state(w, gpu.Maker()), prm) for w in ws]
+    for s in range(4):
+        for h in handles:
+            h.step(1)
+        for c in cpu:
+            c.step(1)
+        got = split_state(handles[0].state(), handles[0].world_offsets) if batched else [h.state() for h in handles]
+        for w, (a, c) in enumerate(zip(got, cpu)):
+            assert_states_equal(a, c.state(), f"{ws[w].name} step {s + 1}")
+    for w, h in enumerate(handles):
+        fb = int(h.stats()["axis_fallbacks"])
+        print(f"{'batch' if batched else ws[w].name}: axis fallbacks {fb}, axis {int(h.stats()['sweep_axis'])}",
+              flush=True)
+        assert fb > 0
+        h.close()
+
+
+def test_batch_with_a_world_of_the_largest_size(gpu, oracle):
+    """A world of 2048 bodies, the per-world broadphase's LDS capacity (bitonic sort of 2048 keys),
+    batched with small worlds: equal to one handle per world, and the large world to the oracle."""
+    from avbd import scenes
+    from avbd.state import default_params, split_state
+    big = scenes.pile(2043, seed=41)
+    assert big.num_bodies == 2048
+    ws = [big, scenes.stacks(2, seed=42), scenes.pile(300, seed=43)]
+    prm = default_params(gravity=-9.81)
+    states = [scenes.build_state(w, gpu.Maker()) for w in ws]
+    g = gpu.HipWorld.for_batch(states, prm)
+    cpu = oracle.OracleWorld(scenes.build_state(big, oracle.OracleMaker()), prm)
+    g.step(150)
+    cpu.step(150)
+    got = split_state(g.state(), g.world_offsets)
+    assert_states_equal(got[0], cpu.state(), "2048-body world, step 150")
+    assert len(got[0].manifolds) > 500
+    for w, st in enumerate(states):
+        one = gpu.HipWorld.for_state(st, prm)
+        one.step(150)
+        assert_states_equal(got[w], one.state(), f"world {w} ({ws[w].name}) vs its own handle")
+        one.close()
+    g.close()
