@@ -529,6 +529,44 @@ def batch(num_worlds: int = 4096, seed: int = 4, lo: int = 8, hi: int = 512, ste
     return worlds
 
 
+def spring_chain(links: int = 6, seed: int = 12, ground: bool = False, steps: int = 60) -> Scene:
+    """Unit boxes in a row 1.2 m apart, each tied to the next by a spring (rest 1.0 m, stiffness
+    log-uniform in [1e2, 1e3], anchors on the facing faces), with seeded velocities and spins;
+    free flight, or dropped onto a ground slab. Without the ground the motion is smooth: no
+    contact or clamp decision changes under small perturbations."""
+    rng = _rng(seed)
+    sb = _Builder()
+    if ground:
+        sb.body((40.0, 40.0, 1.0), 0.0, 0.5, (0.0, 0.0, -0.5))
+    first = len(sb.b["size"])
+    for i in range(links):
+        sb.body((1.0, 1.0, 1.0), 1.0, 0.5, (1.2 * i, 0.0, 2.0 + 0.1 * i), (0, 0, 0, 1),
+                tuple(rng.normal(0.0, 0.5, 3)), tuple(rng.normal(0.0, 0.3, 3)))
+    for i in range(links - 1):
+        sb.spring(first + i, first + i + 1, (0.5, 0.0, 0.0), (-0.5, 0.0, 0.0),
+                  float(np.exp(rng.uniform(np.log(1e2), np.log(1e3)))), 1.0)
+    return sb.scene(f"spring_chain{links}", -9.81, steps)
+
+
+def spring_drop(count: int = 60, links: int = 8, seed: int = 13, steps: int = 90) -> Scene:
+    """A jittered pile of ``count`` boxes with a chain of ``links`` boxes tied by springs (rest
+    0.3 m, stiffness log-uniform in [1e3, 1e4]) dropped onto it: contacts and springs together."""
+    rng = _rng(seed)
+    base = pile(count, seed=seed, pitch=2.2)
+    sb = _Builder()
+    for i in range(base.num_bodies):
+        sb.body(tuple(base.size[i]), float(base.density[i]), float(base.friction[i]), tuple(base.pos[i]),
+                tuple(base.rot[i]), tuple(base.vel[i]), tuple(base.angvel[i]))
+    top = float(base.pos[:, 2].max()) + 3.0
+    first = len(sb.b["size"])
+    for i in range(links):
+        sb.body((0.4, 0.4, 0.4), 1.0, 0.5, (-1.4 + 0.4 * i, 0.0, top), (0, 0, 0, 1), tuple(rng.normal(0.0, 0.3, 3)))
+    for i in range(links - 1):
+        sb.spring(first + i, first + i + 1, (0.2, 0.0, 0.0), (-0.2, 0.0, 0.0),
+                  float(np.exp(rng.uniform(np.log(1e3), np.log(1e4)))), 0.3)
+    return sb.scene(f"spring_drop{count}+{links}", -9.81, steps)
+
+
 def projectile_pile(count: int = 1024, projectiles: int = 64, seed: int = 5, steps: int = 120) -> Scene:
     """The differentiable workload: a jittered block of unit boxes resting on a ground box
     (16 x 16 x 4 for 1024) and 0.5 m projectile boxes launched from a ring of radius 20 m so

@@ -537,6 +537,64 @@ struct AdjBodyShared
     double xb[3], qb[4];  // the body's pose adjoints through its update and inertia terms
 };
 
+// One spring stamp of body i reversed (Spring::updatePrimal, spring.cpp:27-69): with rW the own
+// anchor, s = +1 for body A and -1 for body B, the record is H_LL += k gl gl^T, H_AA += k ga ga^T,
+// H_AL += k ga gl^T, r_L += f gl, r_A += f ga for gl = s n, ga = s (rW x n), n = d / |d|,
+// f = k (|d| - rest), d = (rotate(qA, rA) + xA) - (rotate(qB, rB) + xB). G: the symmetrised
+// adjoint of the body's 6x6 matrix (G_LL, G_AA, G_AL), rLb / rAb: of its right-hand side. The
+// forward values are the FP32 step's own (as spring_record computes them); the pose adjoints of
+// body A and B come out in xAb, qAb, xBb, qBb.
+__device__ __forceinline__ void spring_adj(const Spring &sp, bool ownA, float4 pA4, float4 qA4, float4 pB4,
+                                           float4 qB4, const double *G, DV rLb, DV rAb, DV &xAb, DQ &qAb, DV &xBb,
+                                           DQ &qBb)
+{
+    const Q4 qA = ldq(qA4), qB = ldq(qB4);
+    const V3 rAw = rotate(qA, ld3(sp.rA)), rBw = rotate(qB, ld3(sp.rB));
+    const V3 d = (rAw + ld3(pA4)) - (rBw + ld3(pB4));
+    const float dl = len(d);
+    if (dl <= 1.0e-6f)
+        return;
+    const V3 n = d / dl;
+    const float k = sp.rB.w;
+    const float f = k * (dl - sp.rA.w);
+    const V3 rW = ownA ? rAw : rBw;
+    const double sg = ownA ? 1.0 : -1.0;
+    const DV nd = dv(n), rWd = dv(rW);
+    const DV gl = nd * sg, ga = dv(ownA ? cross(rW, n) : -cross(rW, n));
+    const double *GLL = G, *GAA = G + 9, *GAL = G + 18;
+    auto mv = [](const double *M, DV v) {
+        return DV{M[0] * v.x + M[1] * v.y + M[2] * v.z, M[3] * v.x + M[4] * v.y + M[5] * v.z,
+                  M[6] * v.x + M[7] * v.y + M[8] * v.z};
+    };
+    auto mtv = [](const double *M, DV v) {
+        return DV{M[0] * v.x + M[3] * v.y + M[6] * v.z, M[1] * v.x + M[4] * v.y + M[7] * v.z,
+                  M[2] * v.x + M[5] * v.y + M[8] * v.z};
+    };
+    const double kd = k, fd = f;
+    const double fb = ddot(gl, rLb) + ddot(ga, rAb);
+    const DV glb = rLb * fd + (mv(GLL, gl) + mtv(GAL, ga)) * kd;
+    const DV gab = rAb * fd + (mv(GAA, ga) + mv(GAL, gl)) * kd;
+    // gl = s n, ga = s (rW x n)
+    DV nb = glb * sg + dcross(gab, rWd) * sg;
+    const DV rWb = dcross(nd, gab) * sg;
+    // f = k (|d| - rest), n = d / |d|
+    const double dlb = kd * fb;
+    const double dld = dl;
+    // n = d / |d| and |d| = sqrt(d . d) differentiated at the FP32 d and |d| (as the step's values)
+    const DV dd = dv(d);
+    const DV db = nb * (1.0 / dld) - dd * (ddot(dd, nb) / (dld * dld * dld)) + dd * (dlb / dld);
+    DV rAwb = db, rBwb = -db;
+    if (ownA)
+        rAwb += rWb;
+    else
+        rBwb += rWb;
+    xAb += db;
+    xBb -= db;
+    DV unusedA{0, 0, 0}, unusedB{0, 0, 0};
+    drot_adj(dq4(qA4), dv4(sp.rA), rAwb, qAb, unusedA);
+    drot_adj(dq4(qB4), dv4(sp.rB), rBwb, qBb, unusedB);
+}
+
 __device__ __forceinline__ void wave_sum_group(double &v)
 {
 #pragma unroll
@@ -544,8 +602,9 @@ __device__ __forceinline__ void wave_sum_group(double &v)
         v += __shfl_xor(v, off, kAdjLanes);
 }
 
-__global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds M, const int4 *stamps,
-                                                            const int4 *tasks, int begin, int end, const int *color,
+__global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds M, const Spring *springs,
+                                                            const int4 *stamps, const int4 *tasks, int begin, int end,
+                                                            const int *color,
                                                             int col, const float4 *tapePrev, const float4 *tapeCur,
                                                             const float4 *tapeLamPrev, const float4 *sysTape,
                                                             const unsigned char *stampFlags, Params P, AdjBuf A)
@@ -661,6 +720,8 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
     CFwd f;
     auto load_stamp = [&](int t) {
         st = stamps[s0 + t];
+        if (adjKind(st.x) != kAdjManifold)
+            return;  // springs are evaluated where they are reversed
         fl = stampFlags[s0 + t];
         const bool isA = (st.y >> 16) != 0;
         const int partner = st.z, sl = st.w;
@@ -691,6 +752,28 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
     {
         if (t != lane)
             load_stamp(t);
+        if (adjKind(st.x) == kAdjSpring)
+        {
+            const Spring &sp = springs[adjIndex(st.x)];
+            const bool ownA = sp.a == i;
+            const int j = ownA ? sp.b : sp.a;
+            const float4 *sj = seen(j);
+            const float4 pI = tapePrev[2 * i], qI = tapePrev[2 * i + 1], pJ = sj[2 * j], qJ = sj[2 * j + 1];
+            DV xAb{0, 0, 0}, xBb{0, 0, 0};
+            DQ qAb{0, 0, 0, 0}, qBb{0, 0, 0, 0};
+            if (ownA)
+                spring_adj(sp, true, pI, qI, pJ, qJ, S.G, rLb, rAb, xAb, qAb, xBb, qBb);
+            else
+                spring_adj(sp, false, pJ, qJ, pI, qI, S.G, rLb, rAb, xAb, qAb, xBb, qBb);
+            oxb += ownA ? xAb : xBb;
+            oqb += ownA ? qAb : qBb;
+            if (B.massMoment[j].x > 0.0f)
+            {
+                atomic_add3(A.xb + 3 * j, ownA ? xBb : xAb);
+                atomic_add4(A.qb + 4 * j, ownA ? qBb : qAb);
+            }
+            continue;
+        }
         const bool isA = (st.y >> 16) != 0;
         const int partner = st.z, sl = st.w, m = adjIndex(st.x);
         const int a = isA ? i : partner, b = isA ? partner : i;
@@ -1271,7 +1354,7 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
             int b = t.offsets[col], e = t.offsets[col + 1];
             if (e > b)
                 k_adj_primal<<<ceil_div(e - b, kAdjBodies), kAdjThreads, 0, s>>>(
-                    B, M, t.stamps, t.tasks, b, e, t.color, col, prev, cur, lam, sys, sfl, h->P, A);
+                    B, M, h->springs, t.stamps, t.tasks, b, e, t.color, col, prev, cur, lam, sys, sfl, h->P, A);
         }
     }
     k_adj_warmstart<<<ceil_div(nb, 256), 256, 0, s>>>(B, h->P, t.deg, nb, A);
@@ -1286,8 +1369,8 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
 
 int check_adjoint_scene(avbd_handle *h)
 {
-    if (h->nj > 0 || h->ns > 0)
-        return fail(AVBD_ERR_UNSUPPORTED, "adjoint: contact-only worlds (no joints or springs)");
+    if (h->nj > 0)
+        return fail(AVBD_ERR_UNSUPPORTED, "adjoint: worlds of contacts and springs (no joints)");
     if (h->extPairsOn || !h->extContacts.empty())
         return fail(AVBD_ERR_UNSUPPORTED, "adjoint: worlds without external pairs or contacts");
     if (h->numIgnore > 0)

@@ -82,8 +82,8 @@ def load_oracle():
         lib.orc_quat_ops.argtypes = [_p] * 7
         lib.orc_joint_spring_derived.argtypes = [_p, _p, _p, _p, _p, _p]
         lib.orc_sizeof.argtypes = [_i]
-        lib.orc_tangent_run.argtypes = [_i, _p, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 6
-        lib.orc_f64_run.argtypes = [_p, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 5
+        lib.orc_tangent_run.argtypes = [_i, _p, _p, _i, _p, _i, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 6
+        lib.orc_f64_run.argtypes = [_p, _p, _i, _p, _i, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 5
         lib.orc_set_external.argtypes = [_p, _i, _p, _i, _p, _i]
         check_layouts(lib.orc_sizeof, records=6)
         if lib.orc_sizeof(9) != EXTERNAL_CONTACT.itemsize:
@@ -518,14 +518,13 @@ def oracle_tangent_run(state: State, params, steps: int, dvel=None, dangvel=None
     pos3 rot4 vel3 angvel3."""
     lib = load_oracle()
     nb = len(state.bodies)
-    if len(state.joints) or len(state.springs):
-        raise ValueError("the forward-mode reference covers contact-only worlds")
     prm = np.ascontiguousarray(params)
     val = np.zeros((nb, 13)); tan = np.zeros((nb, 13))
     args = [_f64(dvel, 3 * nb), _f64(dangvel, 3 * nb), _f64(dmass, nb), _f64(dfriction, nb)]
-    lib.orc_tangent_run(int(fp64), prm.ctypes.data_as(_p), _ptr(state.bodies), nb, _ptr(state.manifolds), len(state.manifolds),
-                        C.c_int64(state.next_serial), steps, *[_ptr(a) for a in args], val.ctypes.data_as(_p),
-                        tan.ctypes.data_as(_p))
+    lib.orc_tangent_run(int(fp64), prm.ctypes.data_as(_p), _ptr(state.bodies), nb, _ptr(state.joints),
+                        len(state.joints), _ptr(state.springs), len(state.springs), _ptr(state.manifolds),
+                        len(state.manifolds), C.c_int64(state.next_serial), steps, *[_ptr(a) for a in args],
+                        val.ctypes.data_as(_p), tan.ctypes.data_as(_p))
     return val, tan
 
 
@@ -554,11 +553,10 @@ def oracle_f64_run(state: State, params, steps: int, vel=None, angvel=None, mass
     Returns (bodies, 13): pos3 rot4 vel3 angvel3."""
     lib = load_oracle()
     nb = len(state.bodies)
-    if len(state.joints) or len(state.springs):
-        raise ValueError("the FP64 override run covers contact-only worlds")
     prm = np.ascontiguousarray(params)
     out = np.zeros((nb, 13))
     args = [_f64(vel, 3 * nb), _f64(angvel, 3 * nb), _f64(mass, nb), _f64(friction, nb)]
-    lib.orc_f64_run(prm.ctypes.data_as(_p), _ptr(state.bodies), nb, _ptr(state.manifolds), len(state.manifolds),
+    lib.orc_f64_run(prm.ctypes.data_as(_p), _ptr(state.bodies), nb, _ptr(state.joints), len(state.joints),
+                    _ptr(state.springs), len(state.springs), _ptr(state.manifolds), len(state.manifolds),
                     C.c_int64(state.next_serial), steps, *[_ptr(a) for a in args], out.ctypes.data_as(_p))
     return out
