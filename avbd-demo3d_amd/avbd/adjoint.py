@@ -1,7 +1,7 @@
 """Differentiable rollouts of the resident AVBD step as a torch.autograd.Function.
 
 A thin layer over the C ABI's adjoint (avbd_adjoint_forward / avbd_adjoint_backward in
-include/avbd_hip.h): the forward uploads a state of contacts and springs whose initial velocities, angular
+include/avbd_hip.h): the forward uploads a state of boxes, contacts, joints and springs whose initial velocities, angular
 velocities, masses (moments scale with the mass) and friction coefficients come from tensors,
 steps it on the GPU keeping checkpoints, and returns the final positions; the backward runs
 the hand-written reverse sweep on the GPU and returns the gradients of those inputs. There is
@@ -50,14 +50,12 @@ class _RolloutFn:
 
 
 class Rollout:
-    """`steps` GPU steps of a world of contacts and springs as a differentiable function of the initial
+    """`steps` GPU steps of a world of boxes (contacts, joints, springs) as a differentiable function of the initial
     velocities, angular velocities, masses and friction coefficients (all bodies, list order).
     checkpoint_every: checkpoints every that many steps, each segment recomputed by the backward;
     0 tapes every step in the forward (more HBM, no recompute)."""
 
     def __init__(self, state: State, params, steps: int, checkpoint_every: int = 10, **world_kw):
-        if len(state.joints):
-            raise ValueError("Rollout: worlds of contacts and springs (no joints)")
         self.state = state.copy()
         self.params = params
         self.steps = int(steps)

@@ -548,6 +548,28 @@ def spring_chain(links: int = 6, seed: int = 12, ground: bool = False, steps: in
     return sb.scene(f"spring_chain{links}", -9.81, steps)
 
 
+def joint_chain(links: int = 6, seed: int = 14, k_ang: float = 0.0, pile_count: int = 0, steps: int = 60) -> Scene:
+    """Boxes of 0.4 m in a row, each joined to the next at their facing faces (infinite linear
+    stiffness; angular stiffness ``k_ang``, 0 for ball joints), with seeded velocities and spins;
+    free flight, or (pile_count > 0) dropped onto a jittered pile of that many boxes."""
+    rng = _rng(seed)
+    sb = _Builder()
+    top = 3.0
+    if pile_count > 0:
+        base = pile(pile_count, seed=seed, pitch=2.2)
+        for i in range(base.num_bodies):
+            sb.body(tuple(base.size[i]), float(base.density[i]), float(base.friction[i]), tuple(base.pos[i]),
+                    tuple(base.rot[i]), tuple(base.vel[i]), tuple(base.angvel[i]))
+        top = float(base.pos[:, 2].max()) + 3.0
+    first = len(sb.b["size"])
+    for i in range(links):
+        sb.body((0.4, 0.4, 0.4), 1.0, 0.5, (-0.5 * (links - 1) * 0.5 + 0.5 * i, 0.0, top + 0.05 * i), (0, 0, 0, 1),
+                tuple(rng.normal(0.0, 0.4, 3)), tuple(rng.normal(0.0, 0.4, 3)))
+    for i in range(links - 1):
+        sb.joint(first + i, first + i + 1, (0.25, 0.0, 0.0), (-0.25, 0.0, 0.0), k_lin=math.inf, k_ang=k_ang)
+    return sb.scene(f"joint_chain{links}" + (f"+pile{pile_count}" if pile_count else ""), -9.81, steps)
+
+
 def spring_drop(count: int = 60, links: int = 8, seed: int = 13, steps: int = 90) -> Scene:
     """A jittered pile of ``count`` boxes with a chain of ``links`` boxes tied by springs (rest
     0.3 m, stiffness log-uniform in [1e3, 1e4]) dropped onto it: contacts and springs together."""

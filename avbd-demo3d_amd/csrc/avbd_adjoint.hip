@@ -1,5 +1,5 @@
-// avbd_adjoint.hip -- hand-written adjoint (reverse mode) of the resident AVBD step for
-// contact-only box worlds. Included by avbd_capi.hip after the handle definition.
+// avbd_adjoint.hip -- hand-written adjoint (reverse mode) of the resident AVBD step for worlds of
+// boxes with contacts, joints and springs. Included by avbd_capi.hip after the handle definition.
 //
 // The forward step runs in FP32 (bit-exact with the reference, solver.cpp:1036-1041); the
 // adjoint differentiates that FP32 trajectory, with every branch the step took (contact set,
@@ -8,10 +8,14 @@
 //   finishStep (solver.cpp:1275-1288)            v = (x - x0)/dt, w = (q - q0)/dt
 //   iterations (solver.cpp:1172-1273), reversed iteration by iteration:
 //     dual (manifold.cpp:231-280)                 lambda = F(C), k += beta |C| (unsaturated rows)
+//     joint dual (joint.cpp:125-182)              lambda += K C (infinite stiffness), k += beta |C|
 //     primal colours in reverse (manifold.cpp:176-229, solver.cpp:1184-1240): the 6x6 solve's
 //       adjoint is one more LDL^T solve with the same (symmetric) matrix, w = H^-1 dx_bar, then
-//       r_bar = -w, H_bar = -w dx^T are pushed through every contact stamp and the inertia terms
+//       r_bar = -w, H_bar = -w dx^T are pushed through every contact, joint (linear rows,
+//       geometric-stiffness diagonal, angular rows; joint.cpp:56-123) and spring (spring.cpp:27-69)
+//       stamp and the inertia terms
 //   warm start (solver.cpp:1134-1169)             inertial targets, adaptive gravity weight
+//   joint init (joint.cpp:35-54)                  C0 from the start poses, lambda*alpha*gamma, clamped k*gamma
 //   force init (manifold.cpp:100-174)             C0 from the start poses, lambda*alpha*gamma,
 //       clamp(k*gamma); anchors kept by the stick rule pass their adjoint to the previous step,
 //       fresh anchors and the basis go through the collide pose Jacobian, evaluated by running
@@ -137,7 +141,13 @@ struct AdjBuf
     double *xb, *qb, *vb, *wb, *pvb, *ilb, *iab, *nlb, *nab, *mb, *mub;
     double *kb, *lb, *c0b, *rAb, *rBb, *basisb, *mumb;
     double *ckb, *clb, *crAb, *crBb;
+    // joints, kJointAdj per joint: penalty lin, penalty ang, lambda lin, lambda ang (the state's
+    // adjoint at the current point of the sweep), its carry to the previous step's final state,
+    // and the C0 lin / ang adjoints (kJointC0 per joint) the joint init's reversal takes
+    double *jb, *cjb, *jc0b;
 };
+constexpr int kJointAdj = 12, kJointC0 = 6;
+enum { JA_PL = 0, JA_PA = 3, JA_LL = 6, JA_LA = 9 };
 
 // ---- one contact's force and its adjoint (manifold.cpp:176-229 / 231-280) ----
 struct CIn
@@ -595,6 +605,323 @@ __device__ __forceinline__ void spring_adj(const Spring &sp, bool ownA, float4 p
     drot_adj(dq4(qB4), dv4(sp.rB), rBwb, qBb, unusedB);
 }
 
+// ---- joints (joint.cpp:35-182) ----
+// A joint's poses as the reversal sees them: body A (or the world anchor for a = -1) and body B.
+struct JointPoses
+{
+    float4 pA, qA, pB, qB;
+};
+
+// Linear part of a joint at poses `jp` (FP32, as joint_record / dual_joint compute it):
+// rAW = rotate(qA, rA) (0 for a world anchor), rBW = rotate(qB, rB), C = (xA + rAW) - (xB + rBW),
+// stabilised by -C0 alpha for an infinite stiffness.
+__device__ __forceinline__ void joint_lin(const Joint &j, const JointPoses &jp, float alpha, V3 &rAW, V3 &rBW, V3 &C)
+{
+    const Q4 qB = ldq(jp.qB);
+    rAW = j.a >= 0 ? rotate(ldq(jp.qA), ld3(j.rA)) : V3{0.0f, 0.0f, 0.0f};
+    rBW = rotate(qB, ld3(j.rB));
+    C = (j.a >= 0 ? ld3(jp.pA) + rAW : ld3(j.rA)) - (ld3(jp.pB) + rBW);
+    if (isinf(j.rA.w))
+        C -= ld3(j.c0Lin) * alpha;
+}
+// Angular part: C = (qA - qB) arm, stabilised likewise.
+__device__ __forceinline__ V3 joint_ang(const Joint &j, const JointPoses &jp, float alpha)
+{
+    const Q4 qa = j.a >= 0 ? ldq(jp.qA) : Q4{0.0f, 0.0f, 0.0f, 1.0f};
+    V3 C = qdiff(qa, ldq(jp.qB)) * j.c0Lin.w;
+    if (isinf(j.rB.w))
+        C -= ld3(j.c0Ang) * alpha;
+    return C;
+}
+
+// Pose and C0 adjoints of a joint's constraint values: Clb of the linear C, Cab of the angular C.
+__device__ __forceinline__ void joint_c_adj(const Joint &j, const JointPoses &jp, float alpha, DV Clb, DV Cab,
+                                            DV rAWb, DV rBWb, DV &xAb, DQ &qAb, DV &xBb, DQ &qBb, DV &c0lb, DV &c0ab)
+{
+    if (j.a >= 0)
+    {
+        xAb += Clb;
+        rAWb += Clb;
+    }
+    xBb -= Clb;
+    rBWb -= Clb;
+    if (isinf(j.rA.w))
+        c0lb -= Clb * (double)alpha;
+    if (isinf(j.rB.w))
+        c0ab -= Cab * (double)alpha;
+    DV unused{0, 0, 0};
+    if (j.a >= 0)
+        drot_adj(dq4(jp.qA), dv4(j.rA), rAWb, qAb, unused);
+    drot_adj(dq4(jp.qB), dv4(j.rB), rBWb, qBb, unused);
+    const DQ qa = j.a >= 0 ? dq4(jp.qA) : DQ{0, 0, 0, 1};
+    DQ qab{0, 0, 0, 0};
+    dqdiff_adj(qa, dq4(jp.qB), Cab * (double)j.c0Lin.w, qab, qBb);
+    if (j.a >= 0)
+        qAb += qab;
+}
+
+// One joint stamp of body i reversed (Joint::updatePrimal, joint.cpp:56-123), part 0 (the linear
+// rows: jL = +-I, jA = skew(-rAW) or skew(rBW), K = diag(penalty), F = K C + lambda), part 1 (the
+// ball joint's geometric-stiffness diagonal: diag |column b of sum_k F_k G_k(rr)|) or part 2 (the
+// angular rows, jA = +-arm I). G / rLb / rAb: the body system's adjoints; the joint state's
+// adjoints go to jsb (kJointAdj), the C0 adjoints to jc0 (kJointC0), the poses' to xAb ... qBb.
+__device__ __forceinline__ void joint_stamp_adj(const Joint &j, int part, bool isA, const JointPoses &jp, float alpha,
+                                                const double *G, DV rLb, DV rAb, double *jsb, double *jc0,
+                                                DV &xAb, DQ &qAb, DV &xBb, DQ &qBb)
+{
+    const double *GLL = G, *GAA = G + 9, *GAL = G + 18;
+    const double sg = isA ? 1.0 : -1.0;
+    DV Clb{0, 0, 0}, Cab{0, 0, 0}, rAWb{0, 0, 0}, rBWb{0, 0, 0}, c0lb{0, 0, 0}, c0ab{0, 0, 0};
+    if (part < 2)
+    {
+        const V3 penL = ld3(j.penLin);
+        if (!(lensq(penL) > 0.0f))
+            return;
+        V3 rAW, rBW, C;
+        joint_lin(j, jp, alpha, rAW, rBW, C);
+        const V3 F = V3{penL.x * C.x, penL.y * C.y, penL.z * C.z} + ld3(j.lamLin);
+        const double K[3] = {penL.x, penL.y, penL.z}, Cd[3] = {C.x, C.y, C.z}, Fd[3] = {F.x, F.y, F.z};
+        double Fb[3] = {0, 0, 0}, Kb[3] = {0, 0, 0};
+        if (part == 0)
+        {
+            const DV rW = dv(isA ? rAW : rBW);
+            DV rWb{0, 0, 0};
+            for (int k = 0; k < 3; ++k)
+            {
+                const DV n{k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+                const DV av = dcross(rW, n) * sg;
+                const DV GLLn{GLL[k], GLL[3 + k], GLL[6 + k]};
+                const DV GAAa{GAA[0] * av.x + GAA[1] * av.y + GAA[2] * av.z, GAA[3] * av.x + GAA[4] * av.y + GAA[5] * av.z,
+                              GAA[6] * av.x + GAA[7] * av.y + GAA[8] * av.z};
+                const DV GALn{GAL[k], GAL[3 + k], GAL[6 + k]};
+                Kb[k] += 0.5 * ddot(n, GLLn) + 0.5 * ddot(av, GAAa) + sg * ddot(av, GALn);
+                Fb[k] += sg * dcomp(rLb, k) + ddot(av, rAb);
+                const DV ab = rAb * Fd[k] + GAAa * K[k] + GALn * (K[k] * sg);
+                rWb += dcross(n, ab) * sg;
+            }
+            if (isA)
+                rAWb += rWb;
+            else
+                rBWb += rWb;
+        }
+        else
+        {
+            // column b of H = sum_k F_k G_k(rr): c_b[a] = -(F . rr) [a == b] + rr_a F_b, as FP32
+            const V3 rr = isA ? rAW : -rBW;
+#define AVBD_GS_H(a, b) ((gs_entry<0, a, b>(rr) * F.x + gs_entry<1, a, b>(rr) * F.y) + gs_entry<2, a, b>(rr) * F.z)
+            const V3 c0{AVBD_GS_H(0, 0), AVBD_GS_H(1, 0), AVBD_GS_H(2, 0)};
+            const V3 c1{AVBD_GS_H(0, 1), AVBD_GS_H(1, 1), AVBD_GS_H(2, 1)};
+            const V3 c2{AVBD_GS_H(0, 2), AVBD_GS_H(1, 2), AVBD_GS_H(2, 2)};
+#undef AVBD_GS_H
+            const V3 cols[3] = {c0, c1, c2};
+            DV cb[3];
+            for (int b = 0; b < 3; ++b)
+            {
+                const float lb = len(cols[b]);
+                cb[b] = lb > 0.0f ? dv(cols[b]) * (0.5 * GAA[4 * b] / (double)lb) : DV{0, 0, 0};
+            }
+            const DV rrd = dv(rr);
+            const double trace = cb[0].x + cb[1].y + cb[2].z;
+            DV rrb = DV{Fd[0], Fd[1], Fd[2]} * (-trace);
+            for (int b = 0; b < 3; ++b)
+                rrb += cb[b] * Fd[b];
+            for (int k = 0; k < 3; ++k)
+                Fb[k] += -dcomp(rrd, k) * trace + ddot(cb[k], rrd);
+            if (isA)
+                rAWb += rrb;
+            else
+                rBWb -= rrb;
+        }
+        for (int k = 0; k < 3; ++k)
+        {
+            Kb[k] += Fb[k] * Cd[k];
+            jsb[JA_PL + k] += Kb[k];
+            jsb[JA_LL + k] += Fb[k];
+        }
+        Clb = DV{K[0] * Fb[0], K[1] * Fb[1], K[2] * Fb[2]};
+    }
+    else
+    {
+        const V3 penA = ld3(j.penAng);
+        if (!(lensq(penA) > 0.0f))
+            return;
+        const V3 C = joint_ang(j, jp, alpha);
+        const double arm = j.c0Lin.w, K[3] = {penA.x, penA.y, penA.z}, Cd[3] = {C.x, C.y, C.z};
+        for (int k = 0; k < 3; ++k)
+        {
+            const double Fb = sg * arm * dcomp(rAb, k);
+            jsb[JA_PA + k] += 0.5 * GAA[4 * k] * arm * arm + Fb * Cd[k];
+            jsb[JA_LA + k] += Fb;
+            if (k == 0) Cab.x = K[0] * Fb;
+            if (k == 1) Cab.y = K[1] * Fb;
+            if (k == 2) Cab.z = K[2] * Fb;
+        }
+    }
+    joint_c_adj(j, jp, alpha, Clb, Cab, rAWb, rBWb, xAb, qAb, xBb, qBb, c0lb, c0ab);
+    for (int k = 0; k < 3; ++k)
+    {
+        jc0[k] += dcomp(c0lb, k);
+        jc0[3 + k] += dcomp(c0ab, k);
+    }
+}
+
+// Joint dual of one iteration reversed (Joint::updateDual, joint.cpp:125-182): before / after its
+// update (tape slots it and it + 1), poses after the iteration's primal. With an infinite
+// stiffness lambda' = K C + lambda; penalty' = min(penalty + beta |C|, cap) where not capped; a
+// joint that fractured in this update has all-zero outputs and passes nothing back.
+__global__ void k_adj_joint_dual(const Joint *before, const Joint *after, int nj, const float4 *pose, Bodies B,
+                                 Params P, AdjBuf A)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nj)
+        return;
+    const Joint j = before[q];
+    if (j.dead)
+        return;
+    double *jb = A.jb + (size_t)kJointAdj * q;
+    if (after[q].broken && !j.broken)
+    {
+        for (int k = 0; k < kJointAdj; ++k)
+            jb[k] = 0.0;
+        return;
+    }
+    JointPoses jp;
+    jp.pA = j.a >= 0 ? pose[2 * j.a] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    jp.qA = j.a >= 0 ? pose[2 * j.a + 1] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    jp.pB = pose[2 * j.b];
+    jp.qB = pose[2 * j.b + 1];
+    DV Clb{0, 0, 0}, Cab{0, 0, 0};
+    // linear rows, then angular rows: penalty and lambda adjoints before the update
+    for (int part = 0; part < 2; ++part)
+    {
+        const V3 pen = part == 0 ? ld3(j.penLin) : ld3(j.penAng);
+        if (!(lensq(pen) > 0.0f))
+            continue;
+        V3 C;
+        if (part == 0)
+        {
+            V3 rAW, rBW;
+            joint_lin(j, jp, P.alpha, rAW, rBW, C);
+        }
+        else
+            C = joint_ang(j, jp, P.alpha);
+        const float stiff = part == 0 ? j.rA.w : j.rB.w, beta = part == 0 ? P.betaLin : P.betaAng;
+        const int op = part == 0 ? JA_PL : JA_PA, ol = part == 0 ? JA_LL : JA_LA;
+        const float cap = fminr(stiff, kPenaltyMax);
+        const float Cf[3] = {C.x, C.y, C.z}, pf[3] = {pen.x, pen.y, pen.z};
+        double Cb[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k)
+        {
+            const double lb = jb[ol + k], pb = jb[op + k];
+            double penb = 0.0;
+            if (isinf(stiff))
+            {
+                penb += lb * (double)Cf[k];
+                Cb[k] += (double)pf[k] * lb;
+            }
+            const float t = pf[k] + fabsf(Cf[k]) * beta;
+            if (t < cap)
+            {
+                penb += pb;
+                Cb[k] += (double)beta * (Cf[k] < 0.0f ? -1.0 : (Cf[k] > 0.0f ? 1.0 : 0.0)) * pb;
+            }
+            jb[op + k] = penb;
+            jb[ol + k] = lb;
+        }
+        if (part == 0)
+            Clb = DV{Cb[0], Cb[1], Cb[2]};
+        else
+            Cab = DV{Cb[0], Cb[1], Cb[2]};
+    }
+    DV xAb{0, 0, 0}, xBb{0, 0, 0}, rAWb{0, 0, 0}, rBWb{0, 0, 0}, c0lb{0, 0, 0}, c0ab{0, 0, 0};
+    DQ qAb{0, 0, 0, 0}, qBb{0, 0, 0, 0};
+    joint_c_adj(j, jp, P.alpha, Clb, Cab, rAWb, rBWb, xAb, qAb, xBb, qBb, c0lb, c0ab);
+    if (j.a >= 0 && B.massMoment[j.a].x > 0.0f)
+    {
+        atomic_add3(A.xb + 3 * j.a, xAb);
+        atomic_add4(A.qb + 4 * j.a, qAb);
+    }
+    if (B.massMoment[j.b].x > 0.0f)
+    {
+        atomic_add3(A.xb + 3 * j.b, xBb);
+        atomic_add4(A.qb + 4 * j.b, qBb);
+    }
+    atomic_add3(A.jc0b + kJointC0 * q, c0lb);
+    atomic_add3(A.jc0b + kJointC0 * q + 3, c0ab);
+}
+
+// Start of a step's reversal for the joints: the state adjoint of the step's final joints is the
+// carry the next step's init reversal left; the carry and the C0 adjoints are cleared.
+__global__ void k_adj_joint_begin(int nj, AdjBuf A)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nj)
+        return;
+    for (int k = 0; k < kJointAdj; ++k)
+    {
+        A.jb[(size_t)kJointAdj * q + k] = A.cjb[(size_t)kJointAdj * q + k];
+        A.cjb[(size_t)kJointAdj * q + k] = 0.0;
+    }
+    for (int k = 0; k < kJointC0; ++k)
+        A.jc0b[(size_t)kJointC0 * q + k] = 0.0;
+}
+
+// Joint init reversed (Joint::initialize, joint.cpp:35-54): C0 lin = (xA + rotate(qA, rA)) -
+// (xB + rotate(qB, rB)) and C0 ang = (qA - qB) arm at the step's start poses; lambda = lambda_prev
+// alpha gamma; penalty = min(clamp(penalty_prev gamma, min, max), stiffness). The previous step's
+// final state receives its adjoint as the carry; joints removed or fractured before this step pass
+// nothing.
+__global__ void k_adj_joint_init(const Joint *start, int nj, Bodies B, Params P, AdjBuf A)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nj)
+        return;
+    const Joint j = start[q];
+    if (j.dead || j.broken)
+        return;
+    const double *jb = A.jb + (size_t)kJointAdj * q, *jc = A.jc0b + (size_t)kJointC0 * q;
+    double *cj = A.cjb + (size_t)kJointAdj * q;
+    const double ag = (double)P.alpha * (double)P.gamma;
+    for (int k = 0; k < 3; ++k)
+    {
+        cj[JA_LL + k] = jb[JA_LL + k] * ag;
+        cj[JA_LA + k] = jb[JA_LA + k] * ag;
+    }
+    const float pl[3] = {j.penLin.x, j.penLin.y, j.penLin.z}, pa[3] = {j.penAng.x, j.penAng.y, j.penAng.z};
+    for (int k = 0; k < 3; ++k)
+        for (int part = 0; part < 2; ++part)
+        {
+            const float v = (part == 0 ? pl[k] : pa[k]) * P.gamma, stiff = part == 0 ? j.rA.w : j.rB.w;
+            const float v1 = clampr(v, kPenaltyMin, kPenaltyMax);
+            const bool inside = !(kPenaltyMax < v) && !(kPenaltyMin > v) && v1 < stiff;
+            cj[(part == 0 ? JA_PL : JA_PA) + k] = inside ? jb[(part == 0 ? JA_PL : JA_PA) + k] * (double)P.gamma : 0.0;
+        }
+    // C0 from the start poses (the x- rows of the pose records)
+    JointPoses jp;
+    jp.pA = j.a >= 0 ? B.pose[4 * j.a + P_ILIN] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    jp.qA = j.a >= 0 ? B.pose[4 * j.a + P_IANG] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    jp.pB = B.pose[4 * j.b + P_ILIN];
+    jp.qB = B.pose[4 * j.b + P_IANG];
+    DV Clb{jc[0], jc[1], jc[2]}, Cab{jc[3], jc[4], jc[5]};
+    DV xAb{0, 0, 0}, xBb{0, 0, 0}, rAWb{0, 0, 0}, rBWb{0, 0, 0}, c0lb{0, 0, 0}, c0ab{0, 0, 0};
+    DQ qAb{0, 0, 0, 0}, qBb{0, 0, 0, 0};
+    // C0 has no stabilisation term: a copy of the joint with finite stiffness flags
+    Joint jf = j;
+    jf.rA.w = 0.0f;
+    jf.rB.w = 0.0f;
+    joint_c_adj(jf, jp, P.alpha, Clb, Cab, rAWb, rBWb, xAb, qAb, xBb, qBb, c0lb, c0ab);
+    if (j.a >= 0 && B.massMoment[j.a].x > 0.0f)
+    {
+        atomic_add3(A.xb + 3 * j.a, xAb);
+        atomic_add4(A.qb + 4 * j.a, qAb);
+    }
+    if (B.massMoment[j.b].x > 0.0f)
+    {
+        atomic_add3(A.xb + 3 * j.b, xBb);
+        atomic_add4(A.qb + 4 * j.b, qBb);
+    }
+}
+
 __device__ __forceinline__ void wave_sum_group(double &v)
 {
 #pragma unroll
@@ -603,8 +930,8 @@ __device__ __forceinline__ void wave_sum_group(double &v)
 }
 
 __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds M, const Spring *springs,
-                                                            const int4 *stamps, const int4 *tasks, int begin, int end,
-                                                            const int *color,
+                                                            const Joint *jointsTape, const int4 *stamps,
+                                                            const int4 *tasks, int begin, int end, const int *color,
                                                             int col, const float4 *tapePrev, const float4 *tapeCur,
                                                             const float4 *tapeLamPrev, const float4 *sysTape,
                                                             const unsigned char *stampFlags, Params P, AdjBuf A)
@@ -721,7 +1048,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
     auto load_stamp = [&](int t) {
         st = stamps[s0 + t];
         if (adjKind(st.x) != kAdjManifold)
-            return;  // springs are evaluated where they are reversed
+            return;  // springs and joints are evaluated where they are reversed
         fl = stampFlags[s0 + t];
         const bool isA = (st.y >> 16) != 0;
         const int partner = st.z, sl = st.w;
@@ -752,6 +1079,46 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
     {
         if (t != lane)
             load_stamp(t);
+        if (adjKind(st.x) == kAdjJoint)
+        {
+            // the joint's state as this iteration's primal read it (tape slot of the iteration)
+            const Joint &jt = jointsTape[adjIndex(st.x)];
+            const bool ownA = jt.a == i;
+            const int j = ownA ? jt.b : jt.a;
+            JointPoses jp;
+            const float4 *sj = j >= 0 ? seen(j) : tapePrev;
+            const float4 pI = tapePrev[2 * i], qI = tapePrev[2 * i + 1];
+            const float4 pJ = j >= 0 ? sj[2 * j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float4 qJ = j >= 0 ? sj[2 * j + 1] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            jp.pA = ownA ? pI : pJ;
+            jp.qA = ownA ? qI : qJ;
+            jp.pB = ownA ? pJ : pI;
+            jp.qB = ownA ? qJ : qI;
+            double jsb[kJointAdj], jc0[kJointC0];
+            for (int q = 0; q < kJointAdj; ++q)
+                jsb[q] = 0.0;
+            for (int q = 0; q < kJointC0; ++q)
+                jc0[q] = 0.0;
+            DV xAb{0, 0, 0}, xBb{0, 0, 0};
+            DQ qAb{0, 0, 0, 0}, qBb{0, 0, 0, 0};
+            joint_stamp_adj(jt, st.y & 0xFFFF, ownA, jp, P.alpha, S.G, rLb, rAb, jsb, jc0, xAb, qAb, xBb, qBb);
+            oxb += ownA ? xAb : xBb;
+            oqb += ownA ? qAb : qBb;
+            if (j >= 0 && B.massMoment[j].x > 0.0f)
+            {
+                atomic_add3(A.xb + 3 * j, ownA ? xBb : xAb);
+                atomic_add4(A.qb + 4 * j, ownA ? qBb : qAb);
+            }
+            // a joint's parts sit on several lanes: its adjoints are added atomically
+            double *jb = A.jb + (size_t)kJointAdj * adjIndex(st.x), *jc = A.jc0b + (size_t)kJointC0 * adjIndex(st.x);
+            for (int q = 0; q < kJointAdj; ++q)
+                if (jsb[q] != 0.0)
+                    aadd(jb + q, jsb[q]);
+            for (int q = 0; q < kJointC0; ++q)
+                if (jc0[q] != 0.0)
+                    aadd(jc + q, jc0[q]);
+            continue;
+        }
         if (adjKind(st.x) == kAdjSpring)
         {
             const Spring &sp = springs[adjIndex(st.x)];
@@ -1091,6 +1458,7 @@ struct Snapshot
     long long *serial = nullptr;
     int *count = nullptr, *feature = nullptr;
     Counters *ctr = nullptr;
+    Joint *joints = nullptr;  // the joint records (penalties, lambdas, C0) at the step start
     int numManifolds = 0;
 };
 
@@ -1106,6 +1474,7 @@ struct StepStore
 {
     float4 *tapePose = nullptr, *tapeContact = nullptr, *tapeSys = nullptr;
     unsigned char *tapeStampFlags = nullptr, *tapeDualFlags = nullptr;
+    Joint *tapeJoint = nullptr;  // joint records after the init (slot 0) and after each iteration
     int *recMatch = nullptr, *recStick = nullptr, *recSrc = nullptr;
     float4 *pose = nullptr, *vel = nullptr, *angVel = nullptr, *prevVel = nullptr, *inerLin = nullptr, *inerAng = nullptr;
     int2 *ab = nullptr;
@@ -1141,7 +1510,7 @@ int snap_alloc(avbd_handle *h, Snapshot &s)
     if ((r = h->alloc(s.pose, 4 * (size_t)N)) || (r = h->alloc(s.vel, N)) || (r = h->alloc(s.angVel, N)) ||
         (r = h->alloc(s.prevVel, N)) || (r = h->alloc(s.ab, M)) || (r = h->alloc(s.serial, M)) ||
         (r = h->alloc(s.count, M)) || (r = h->alloc(s.feature, C)) || (r = h->alloc(s.cr, C * kCR)) ||
-        (r = h->alloc(s.ctr, 1)))
+        (r = h->alloc(s.ctr, 1)) || (h->maxJoints > 0 && (r = h->alloc(s.joints, h->maxJoints))))
         return r;
     return AVBD_OK;
 }
@@ -1190,6 +1559,7 @@ int snap_copy(avbd_handle *h, bool save, Snapshot &s)
     cp(s.feature, h->cur.feature, C * sizeof(int));
     cp(s.cr, h->cur.cr, C * kCR * sizeof(float4));
     cp(s.ctr, h->ctr, sizeof(Counters));
+    cp(s.joints, h->joints, (size_t)h->nj * sizeof(Joint));
     int r = b.launch(h->stream, "adjoint snapshot copy");
     if (r)
         return r;
@@ -1246,7 +1616,8 @@ int adj_alloc(avbd_handle *h, int steps, int every)
             (r = h->alloc(t.ab, M)) || (r = h->alloc(t.serial, M)) || (r = h->alloc(t.count, M)) ||
             (r = h->alloc(t.feature, C)) || (r = h->alloc(t.cr, C * kCR)) || (r = h->alloc(t.contactRec, C)) ||
             (r = h->alloc(t.stamps, h->maxStamps)) || (r = h->alloc(t.tasks, N)) || (r = h->alloc(t.color, N)) ||
-            (r = h->alloc(t.deg, N)))
+            (r = h->alloc(t.deg, N)) ||
+            (h->maxJoints > 0 && (r = h->alloc(t.tapeJoint, (size_t)(I + 1) * h->maxJoints))))
         {
             S.store.pop_back();
             return r;
@@ -1264,7 +1635,10 @@ int adj_alloc(avbd_handle *h, int steps, int every)
             (r = h->alloc(A.lb, 3 * C)) || (r = h->alloc(A.c0b, 3 * C)) || (r = h->alloc(A.rAb, 3 * C)) ||
             (r = h->alloc(A.rBb, 3 * C)) || (r = h->alloc(A.basisb, 9 * (size_t)M)) || (r = h->alloc(A.mumb, M)) ||
             (r = h->alloc(A.ckb, 3 * C)) || (r = h->alloc(A.clb, 3 * C)) || (r = h->alloc(A.crAb, 3 * C)) ||
-            (r = h->alloc(A.crBb, 3 * C)))
+            (r = h->alloc(A.crBb, 3 * C)) ||
+            (h->maxJoints > 0 && ((r = h->alloc(A.jb, (size_t)kJointAdj * h->maxJoints)) ||
+                                  (r = h->alloc(A.cjb, (size_t)kJointAdj * h->maxJoints)) ||
+                                  (r = h->alloc(A.jc0b, (size_t)kJointC0 * h->maxJoints)))))
         {
             A.xb = nullptr;
             return r;
@@ -1283,6 +1657,7 @@ int taped_step(avbd_handle *h, StepStore &t)
     h->tapeSys = t.tapeSys;
     h->tapeStampFlags = t.tapeStampFlags;
     h->tapeDualFlags = t.tapeDualFlags;
+    h->tapeJoint = t.tapeJoint;
     h->recMatch = t.recMatch;
     h->recStick = t.recStick;
     h->recSrc = t.recSrc;
@@ -1291,6 +1666,7 @@ int taped_step(avbd_handle *h, StepStore &t)
         r = enqueue_iterate(h);
     h->tapePose = h->tapeContact = h->tapeSys = nullptr;
     h->tapeStampFlags = h->tapeDualFlags = nullptr;
+    h->tapeJoint = nullptr;
     h->recMatch = h->recStick = h->recSrc = nullptr;
     if (r)
         return r;
@@ -1341,6 +1717,9 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
     Manifolds M{t.ab, t.serial, t.count, t.feature, t.cr};
     int n0 = std::max(std::max(nb, nc), std::max(nm, 1));
     k_adj_begin<<<ceil_div(n0, 256), 256, 0, s>>>(B, h->P, t.contactRec, nc, nm, nb, A);
+    const int nj = h->nj;
+    if (nj > 0)
+        k_adj_joint_begin<<<ceil_div(nj, 256), 256, 0, s>>>(nj, A);
     for (int it = I - 1; it >= 0; --it)
     {
         const float4 *prev = tp + it * ps, *cur = tp + (it + 1) * ps, *lam = tc + it * cs;
@@ -1349,15 +1728,20 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
         const unsigned char *dfl = t.tapeDualFlags + (size_t)it * S.slotCap;
         if (nc > 0)
             k_adj_dual<<<ceil_div(nc, 64), 64, 0, s>>>(B, M, t.contactRec, nc, cur, lam, dfl, h->P, A);
+        const Joint *jt = nj > 0 ? t.tapeJoint + (size_t)it * h->maxJoints : nullptr;
+        if (nj > 0)
+            k_adj_joint_dual<<<ceil_div(nj, 64), 64, 0, s>>>(jt, jt + h->maxJoints, nj, cur, B, h->P, A);
         for (int col = colors - 1; col >= 0; --col)
         {
             int b = t.offsets[col], e = t.offsets[col + 1];
             if (e > b)
                 k_adj_primal<<<ceil_div(e - b, kAdjBodies), kAdjThreads, 0, s>>>(
-                    B, M, h->springs, t.stamps, t.tasks, b, e, t.color, col, prev, cur, lam, sys, sfl, h->P, A);
+                    B, M, h->springs, jt, t.stamps, t.tasks, b, e, t.color, col, prev, cur, lam, sys, sfl, h->P, A);
         }
     }
     k_adj_warmstart<<<ceil_div(nb, 256), 256, 0, s>>>(B, h->P, t.deg, nb, A);
+    if (nj > 0)
+        k_adj_joint_init<<<ceil_div(nj, 256), 256, 0, s>>>(start.joints, nj, B, h->P, A);
     if (nc > 0)
         k_adj_init<<<ceil_div(nc, 256), 256, 0, s>>>(B, M, t.contactRec, nc, t.recSrc, t.recMatch, t.recStick,
                                                      start.cr, start.numManifolds, h->P, A);
@@ -1369,8 +1753,6 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
 
 int check_adjoint_scene(avbd_handle *h)
 {
-    if (h->nj > 0)
-        return fail(AVBD_ERR_UNSUPPORTED, "adjoint: worlds of contacts and springs (no joints)");
     if (h->extPairsOn || !h->extContacts.empty())
         return fail(AVBD_ERR_UNSUPPORTED, "adjoint: worlds without external pairs or contacts");
     if (h->numIgnore > 0)
@@ -1447,7 +1829,8 @@ int avbd_adjoint_backward(avbd_handle *h, const double *d_pos, const double *d_r
     hipError_t e;
     if ((e = up(A.xb, d_pos, 3 * N)) || (e = up(A.qb, d_rot, 4 * N)) || (e = up(A.vb, d_vel, 3 * N)) ||
         (e = up(A.wb, d_ang_vel, 3 * N)) || (e = zero(A.pvb, 3 * N)) || (e = zero(A.mb, N)) || (e = zero(A.mub, N)) ||
-        (e = zero(A.ckb, C3)) || (e = zero(A.clb, C3)) || (e = zero(A.crAb, C3)) || (e = zero(A.crBb, C3)))
+        (e = zero(A.ckb, C3)) || (e = zero(A.clb, C3)) || (e = zero(A.crAb, C3)) || (e = zero(A.crBb, C3)) ||
+        (h->nj > 0 && (e = zero(A.cjb, (size_t)kJointAdj * h->nj))))
         return fail(AVBD_ERR_HIP, std::string("adjoint init: ") + hipGetErrorString(e));
     const int K = S.every, steps = S.steps;
     const int nseg = (steps + K - 1) / K;

@@ -169,6 +169,7 @@ struct avbd_handle
     int *recMatch = nullptr, *recStick = nullptr, *recSrc = nullptr;
     float4 *tapePose = nullptr, *tapeContact = nullptr, *tapeSys = nullptr;
     unsigned char *tapeStampFlags = nullptr, *tapeDualFlags = nullptr;
+    Joint *tapeJoint = nullptr;  // joint states after init (slot 0) and after each iteration's dual
     struct AdjointState *adj = nullptr;
     int lastNumManifolds = 0;
 
@@ -1025,8 +1026,17 @@ int enqueue_iterate(avbd_handle *h)
         k_tape<<<blocks, 256, 0, s>>>(h->B.pose, h->nb, h->tapePose + slot * tapePoseStride, h->S.contactRec, h->ctr,
                                       h->cur.cr, h->tapeContact + slot * tapeContactStride);
     };
+    // joint tape slot k: the joint records the primal of iteration k reads
+    auto tapeJoints = [&](int slot) -> hipError_t {
+        return h->nj > 0 && h->tapeJoint ? hipMemcpyAsync(h->tapeJoint + (size_t)slot * h->maxJoints, h->joints,
+                                                          sizeof(Joint) * h->nj, hipMemcpyDeviceToDevice, s)
+                                         : hipSuccess;
+    };
     if (tape)
+    {
         tapeAt(0);
+        HIP_TRY(tapeJoints(0));
+    }
     if (h->timing)
     {
         h->ringPrimalLaunches[h->ringSteps % avbd_handle::kRing] = primalPerIter;
@@ -1167,6 +1177,8 @@ int enqueue_iterate(avbd_handle *h)
                     HIP_TRY(hipLaunchKernel((const void *)k_dual,
                                             dim3(std::max(1, ceil_div(dualLanes + dt.nb, lc.dual_threads))),
                                             dim3(lc.dual_threads), a.ptr, 0, s));
+                    if (tape)
+                        HIP_TRY(tapeJoints(it + 1));
                 }
                 if (mark && it == last)
                     HIP_TRY(hipEventRecord(mark[6], s));

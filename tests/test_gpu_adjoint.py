@@ -135,12 +135,36 @@ def test_adjoint_springs_and_contacts(gpu, oracle):
     _check(gpu, oracle, st, prm, 30, 10, tol=1e-5, label="spring drop", terms=True)
 
 
-def test_adjoint_rejects_joints(gpu, oracle):
-    from avbd import capi, scenes
+@pytest.mark.parametrize("k_ang", [0.0, 1000.0], ids=["ball", "stiff_angular"])
+def test_adjoint_joints_free_chain(gpu, oracle, k_ang):
+    """Joint stamps, duals and inits (joint.cpp:35-182) reversed: a free-flying chain of six boxes
+    joined face to face (infinite linear stiffness; ball joints or angular stiffness 1e3)."""
+    from avbd import scenes
     from avbd.state import default_params
-    st = scenes.build_state(scenes.pyramid_chain(base=0, links=4), gpu.Maker())
+    sc = scenes.joint_chain(6, seed=14, k_ang=k_ang)
+    st = scenes.build_state(sc, oracle.OracleMaker())
+    st.bodies["prev_vel"] = 0.0
+    _check(gpu, oracle, st, default_params(gravity=sc.gravity), 30, 10, tol=1e-5, label=f"joint chain {k_ang:g}",
+           terms=True)
+
+
+def test_adjoint_joints_and_contacts(gpu, oracle):
+    """A chain of eight ball-jointed boxes dropped onto a 60-box pile: joint and contact stamps
+    in the same bodies' systems, with checkpoints."""
+    from avbd import scenes
+    sc = scenes.joint_chain(8, seed=15, pile_count=60)
+    st, prm = _settled(oracle, sc, 40)
+    assert len(st.joints) == 7 and len(st.manifolds) > 0
+    _check(gpu, oracle, st, prm, 30, 10, tol=1e-5, label="joint drop", terms=True)
+
+
+def test_adjoint_rejects_round_shapes_and_ignore_pairs(gpu, oracle):
+    from avbd import named_scenes, scenes
+    from avbd.state import default_params
+    st = scenes.build_state(named_scenes.build("Soft Body"), gpu.Maker())
     w = gpu.HipWorld.for_state(st, default_params())
-    with pytest.raises(gpu.AvbdError, match="no joints"):
+    w.set_ignore_pairs(st.ignores)
+    with pytest.raises(gpu.AvbdError, match="IgnoreCollision"):
         w.adjoint_forward(2, 1)
     with pytest.raises(gpu.AvbdError, match="no recorded forward"):
         w.adjoint_backward()

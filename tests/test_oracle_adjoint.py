@@ -159,3 +159,27 @@ def test_fp64_tangent_with_springs_matches_central_differences(oracle, kind):
     assert rel < 1e-4, f"spring chain {kind}: FP64 tangent vs central differences {rel:.2e}"
     # the springs act: the chain's relative motion depends on them
     assert np.abs(t64[:, 0:3]).max() > 0.0
+
+
+@pytest.mark.parametrize("k_ang", [0.0, 1000.0], ids=["ball", "stiff_angular"])
+@pytest.mark.parametrize("kind", ["dvel", "dangvel", "dmass"])
+def test_fp64_tangent_with_joints_matches_central_differences(oracle, k_ang, kind):
+    """Joint stamps, duals and inits (joint.cpp:35-182) in the forward-mode reference: a free-flying
+    chain of six boxes joined face to face (infinite linear stiffness; ball joints, or an angular
+    stiffness of 1e3 for the angular rows), previous velocities at rest as for the spring chain."""
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = scenes.joint_chain(6, seed=14, k_ang=k_ang)
+    st = scenes.build_state(sc, oracle.OracleMaker())
+    st.bodies["prev_vel"] = 0.0
+    prm = default_params(gravity=sc.gravity)
+    nb = len(st.bodies)
+    rng = np.random.default_rng(3)
+    u = {"dvel": rng.normal(size=(nb, 3)), "dangvel": rng.normal(size=(nb, 3)),
+         "dmass": rng.normal(size=nb) * st.bodies["mass"]}[kind]
+    base = {"dvel": st.bodies["vel"], "dangvel": st.bodies["ang_vel"], "dmass": st.bodies["mass"]}[kind]
+    _, t64 = oracle.oracle_tangent_run(st, prm, 30, fp64=True, **{kind: u})
+    fd = _fd(oracle, st, prm, 30, {"dvel": "vel", "dangvel": "angvel", "dmass": "mass"}[kind],
+             base.astype(np.float64), u, 1e-5)
+    rel = np.abs(t64[:, 0:7] - fd[:, 0:7]).max() / max(np.abs(fd[:, 0:7]).max(), 1e-12)
+    assert rel < 1e-6, f"joint chain {kind}: FP64 tangent vs central differences {rel:.2e}"
