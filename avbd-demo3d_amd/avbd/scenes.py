@@ -548,9 +548,11 @@ def spring_chain(links: int = 6, seed: int = 12, ground: bool = False, steps: in
     return sb.scene(f"spring_chain{links}", -9.81, steps)
 
 
-def joint_chain(links: int = 6, seed: int = 14, k_ang: float = 0.0, pile_count: int = 0, steps: int = 60) -> Scene:
-    """Boxes of 0.4 m in a row, each joined to the next at their facing faces (infinite linear
-    stiffness; angular stiffness ``k_ang``, 0 for ball joints), with seeded velocities and spins;
+def joint_chain(links: int = 6, seed: int = 14, k_ang: float = 0.0, pile_count: int = 0, steps: int = 60,
+                k_lin: float = math.inf) -> Scene:
+    """Boxes of 0.4 m in a row, each joined to the next at their facing faces (linear stiffness
+    ``k_lin``, infinite by default; angular stiffness ``k_ang``, 0 for ball joints), with seeded
+    velocities and spins;
     free flight, or (pile_count > 0) dropped onto a jittered pile of that many boxes."""
     rng = _rng(seed)
     sb = _Builder()
@@ -566,7 +568,7 @@ def joint_chain(links: int = 6, seed: int = 14, k_ang: float = 0.0, pile_count: 
         sb.body((0.4, 0.4, 0.4), 1.0, 0.5, (-0.5 * (links - 1) * 0.5 + 0.5 * i, 0.0, top + 0.05 * i), (0, 0, 0, 1),
                 tuple(rng.normal(0.0, 0.4, 3)), tuple(rng.normal(0.0, 0.4, 3)))
     for i in range(links - 1):
-        sb.joint(first + i, first + i + 1, (0.25, 0.0, 0.0), (-0.25, 0.0, 0.0), k_lin=math.inf, k_ang=k_ang)
+        sb.joint(first + i, first + i + 1, (0.25, 0.0, 0.0), (-0.25, 0.0, 0.0), k_lin=k_lin, k_ang=k_ang)
     return sb.scene(f"joint_chain{links}" + (f"+pile{pile_count}" if pile_count else ""), -9.81, steps)
 
 

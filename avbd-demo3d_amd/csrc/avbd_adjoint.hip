@@ -1242,7 +1242,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
 }
 
 // Warm start (solver.cpp:1134-1169) reversed: the pose adjoints now refer to the step's start.
-__global__ void k_adj_warmstart(Bodies B, Params P, const int *deg, int n, AdjBuf A)
+__global__ void k_adj_warmstart(Bodies B, Params P, const int *deg, const int *ignoreDeg, int n, AdjBuf A)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
@@ -1258,7 +1258,8 @@ __global__ void k_adj_warmstart(Bodies B, Params P, const int *deg, int n, AdjBu
     DV w = dv4(B.angVel[i]);
     DV xb = ilb, qbz{0, 0, 0};
     DQ qb = iab;
-    bool solved = mass > 0.0f && deg[i] > 0;
+    // solved as the forward's warm start decided: forces or IgnoreCollision pairs make a body constrained
+    bool solved = mass > 0.0f && (deg[i] > 0 || (ignoreDeg && ignoreDeg[i] > 0));
     if (mass > 0.0f && !solved)
     {
         // x' = inertial target, q' = inertial rotation target
@@ -1739,7 +1740,8 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
                     B, M, h->springs, jt, t.stamps, t.tasks, b, e, t.color, col, prev, cur, lam, sys, sfl, h->P, A);
         }
     }
-    k_adj_warmstart<<<ceil_div(nb, 256), 256, 0, s>>>(B, h->P, t.deg, nb, A);
+    k_adj_warmstart<<<ceil_div(nb, 256), 256, 0, s>>>(B, h->P, t.deg, h->numIgnore > 0 ? h->ignoreDeg : nullptr,
+                                                      nb, A);
     if (nj > 0)
         k_adj_joint_init<<<ceil_div(nj, 256), 256, 0, s>>>(start.joints, nj, B, h->P, A);
     if (nc > 0)
@@ -1755,8 +1757,6 @@ int check_adjoint_scene(avbd_handle *h)
 {
     if (h->extPairsOn || !h->extContacts.empty())
         return fail(AVBD_ERR_UNSUPPORTED, "adjoint: worlds without external pairs or contacts");
-    if (h->numIgnore > 0)
-        return fail(AVBD_ERR_UNSUPPORTED, "adjoint: worlds without IgnoreCollision pairs");
     if (h->numRound > 0)
         return fail(AVBD_ERR_UNSUPPORTED, "adjoint: box bodies only (the reverse sweep has no round-shape offsets)");
     if (h->P.iterations > 64)

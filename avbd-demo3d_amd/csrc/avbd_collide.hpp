@@ -59,10 +59,11 @@ AVBD_HD float val(DualF x) { return x.v; }
 AVBD_HD float absv(float x) { return fabsf(x); }
 AVBD_HD DualF absv(DualF x) { return x.v < 0.0f ? -x : x; }
 AVBD_HD float sqrtv(float x) { return sqrtf(x); }
+// the derivative at 0 (a zero vector's length, a kink) is taken as 0
 AVBD_HD DualF sqrtv(DualF x)
 {
     float r = sqrtf(x.v);
-    return DualF(r, x.d / (2.0 * sqrt((double)x.v)));
+    return DualF(r, x.v == 0.0f ? 0.0 : x.d / (2.0 * sqrt((double)x.v)));
 }
 template <class T> AVBD_HD T minv(T a, T b) { return a < b ? a : b; }
 template <class T> AVBD_HD T maxv(T a, T b) { return a > b ? a : b; }

@@ -143,11 +143,20 @@ AVBD_DUAL_MIXED(>)
 AVBD_DUAL_MIXED(<=)
 AVBD_DUAL_MIXED(>=)
 #undef AVBD_DUAL_MIXED
+// At 0 the derivative of sqrt is unbounded; the zero arguments met here are lengths of zero
+// vectors (a geometric-stiffness column of a joint that carries no force: joint.cpp:93,
+// maths.h:476), where |h| has a kink and the derivative is taken as 0, the subgradient
+// the adjoint takes too.
 template <class V> inline DualT<V> sqrt_(DualT<V> x)
 {
-    return mk<V>(sqrt_(x.v), x.d / (2.0 * std::sqrt((double)x.v)));
+    return mk<V>(sqrt_(x.v), x.v == V(0) ? 0.0 : x.d / (2.0 * std::sqrt((double)x.v)));
 }
 template <class V> inline DualT<V> abs_(DualT<V> x) { return x.v < V(0) ? -x : x; }
+// |x| where it scales a penalty's growth (k += beta |C|: manifold.cpp:231-280, joint.cpp:125-182):
+// at x == 0 its derivative is taken as 0, the subgradient the adjoint takes there (the
+// narrowphase's |x| keeps abs_).
+template <class R> inline R absk_(R x) { return abs_(x); }
+template <class V> inline DualT<V> absk_(DualT<V> x) { return x.v == V(0) ? mk<V>(abs_(x).v, 0.0) : abs_(x); }
 // The plain value of a scalar: the broadphase's discrete decisions use the value alone.
 inline float plain_(float x) { return x; }
 inline double plain_(double x) { return x; }
@@ -232,6 +241,7 @@ template <class R> inline V3<R> cross(V3<R> a, V3<R> b)
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 template <class R> inline V3<R> absv(V3<R> a) { return {abs_(a.x), abs_(a.y), abs_(a.z)}; }
+template <class R> inline V3<R> absvk(V3<R> a) { return {absk_(a.x), absk_(a.y), absk_(a.z)}; }
 template <class R> inline V3<R> minv(V3<R> a, R b) { return {fmin_(a.x, b), fmin_(a.y, b), fmin_(a.z, b)}; }
 template <class R> inline V3<R> clampv(V3<R> v, R a, R b) { return {clampv(v.x, a, b), clampv(v.y, a, b), clampv(v.z, a, b)}; }
 
@@ -2122,11 +2132,11 @@ struct World
             }
             c.lambda = F;
             if (F[0] < 0)
-                c.penalty[0] = fmin_(c.penalty[0] + p.betaLin * abs_(C[0]), K<R>::penaltyMax);
+                c.penalty[0] = fmin_(c.penalty[0] + p.betaLin * absk_(C[0]), K<R>::penaltyMax);
             if (fs <= bounds)
             {
-                c.penalty[1] = fmin_(c.penalty[1] + p.betaLin * abs_(C[1]), K<R>::penaltyMax);
-                c.penalty[2] = fmin_(c.penalty[2] + p.betaLin * abs_(C[2]), K<R>::penaltyMax);
+                c.penalty[1] = fmin_(c.penalty[1] + p.betaLin * absk_(C[1]), K<R>::penaltyMax);
+                c.penalty[2] = fmin_(c.penalty[2] + p.betaLin * absk_(C[2]), K<R>::penaltyMax);
                 c.stick = sqrt_(C[1] * C[1] + C[2] * C[2]) < K<R>::stickThresh;
             }
         }
@@ -2197,7 +2207,7 @@ struct World
                 C -= j.C0Lin * p.alpha;
                 j.lamLin = Km * C + j.lamLin;
             }
-            j.penLin = minv(j.penLin + absv(C) * p.betaLin, fmin_(j.kLin, K<R>::penaltyMax));
+            j.penLin = minv(j.penLin + absvk(C) * p.betaLin, fmin_(j.kLin, K<R>::penaltyMax));
         }
         if (lengthSq(j.penAng) > 0)
         {
@@ -2209,7 +2219,7 @@ struct World
                 C -= j.C0Ang * p.alpha;
                 j.lamAng = Km * C + j.lamAng;
             }
-            j.penAng = minv(j.penAng + absv(C) * p.betaAng, fmin_(j.kAng, K<R>::penaltyMax));
+            j.penAng = minv(j.penAng + absvk(C) * p.betaAng, fmin_(j.kAng, K<R>::penaltyMax));
         }
         if (lengthSq(j.lamAng) > j.fracture * j.fracture)
         {

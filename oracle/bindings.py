@@ -82,8 +82,8 @@ def load_oracle():
         lib.orc_quat_ops.argtypes = [_p] * 7
         lib.orc_joint_spring_derived.argtypes = [_p, _p, _p, _p, _p, _p]
         lib.orc_sizeof.argtypes = [_i]
-        lib.orc_tangent_run.argtypes = [_i, _p, _p, _i, _p, _i, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 6
-        lib.orc_f64_run.argtypes = [_p, _p, _i, _p, _i, _p, _i, _p, _i, C.c_int64, _i] + [_p] * 5
+        lib.orc_tangent_run.argtypes = [_i, _p, _p, _i, _p, _i, _p, _i, _p, _i, C.c_int64, _p, _i, _i] + [_p] * 6
+        lib.orc_f64_run.argtypes = [_p, _p, _i, _p, _i, _p, _i, _p, _i, C.c_int64, _p, _i, _i] + [_p] * 5
         lib.orc_set_external.argtypes = [_p, _i, _p, _i, _p, _i]
         check_layouts(lib.orc_sizeof, records=6)
         if lib.orc_sizeof(9) != EXTERNAL_CONTACT.itemsize:
@@ -523,8 +523,8 @@ def oracle_tangent_run(state: State, params, steps: int, dvel=None, dangvel=None
     args = [_f64(dvel, 3 * nb), _f64(dangvel, 3 * nb), _f64(dmass, nb), _f64(dfriction, nb)]
     lib.orc_tangent_run(int(fp64), prm.ctypes.data_as(_p), _ptr(state.bodies), nb, _ptr(state.joints),
                         len(state.joints), _ptr(state.springs), len(state.springs), _ptr(state.manifolds),
-                        len(state.manifolds), C.c_int64(state.next_serial), steps, *[_ptr(a) for a in args],
-                        val.ctypes.data_as(_p), tan.ctypes.data_as(_p))
+                        len(state.manifolds), C.c_int64(state.next_serial), _ptr(state.ignores), len(state.ignores),
+                        steps, *[_ptr(a) for a in args], val.ctypes.data_as(_p), tan.ctypes.data_as(_p))
     return val, tan
 
 
@@ -558,5 +558,6 @@ def oracle_f64_run(state: State, params, steps: int, vel=None, angvel=None, mass
     args = [_f64(vel, 3 * nb), _f64(angvel, 3 * nb), _f64(mass, nb), _f64(friction, nb)]
     lib.orc_f64_run(prm.ctypes.data_as(_p), _ptr(state.bodies), nb, _ptr(state.joints), len(state.joints),
                     _ptr(state.springs), len(state.springs), _ptr(state.manifolds), len(state.manifolds),
-                    C.c_int64(state.next_serial), steps, *[_ptr(a) for a in args], out.ctypes.data_as(_p))
+                    C.c_int64(state.next_serial), _ptr(state.ignores), len(state.ignores), steps,
+                    *[_ptr(a) for a in args], out.ctypes.data_as(_p))
     return out

@@ -365,11 +365,13 @@ WorldBase *build(const avbd_params *p, const avbd_body *bodies, int nb, const av
 template <class D>
 void tangent_run(const avbd_params *p, const avbd_body *bodies, int nb, const avbd_joint *joints, int nj,
                  const avbd_spring *springs, int ns, const avbd_manifold *manifolds, int nm, int64_t nextSerial,
-                 int steps, const double *dvel, const double *dangvel, const double *dmass, const double *dfriction,
-                 double *val13, double *tan13)
+                 const int32_t *ignores, int ni, int steps, const double *dvel, const double *dangvel,
+                 const double *dmass, const double *dfriction, double *val13, double *tan13)
 {
     World<D> w;
     fill(w, p, bodies, nb, joints, nj, springs, ns, manifolds, nm, nextSerial);
+    for (int k = 0; k < ni; ++k)
+        w.ignores.push_back({ignores[2 * k], ignores[2 * k + 1]});
     for (int i = 0; i < nb; ++i)
     {
         Body<D> &b = w.bodies[i];
@@ -423,25 +425,28 @@ void orc_world_destroy(void *w) { delete (WorldBase *)w; }
 
 void orc_tangent_run(int fp64, const avbd_params *p, const avbd_body *bodies, int nb, const avbd_joint *joints,
                      int nj, const avbd_spring *springs, int ns, const avbd_manifold *manifolds, int nm,
-                     int64_t nextSerial, int steps, const double *dvel, const double *dangvel, const double *dmass,
-                     const double *dfriction, double *val13, double *tan13)
+                     int64_t nextSerial, const int32_t *ignores, int ni, int steps, const double *dvel,
+                     const double *dangvel, const double *dmass, const double *dfriction, double *val13,
+                     double *tan13)
 {
     if (fp64)
-        tangent_run<DualDD>(p, bodies, nb, joints, nj, springs, ns, manifolds, nm, nextSerial, steps, dvel, dangvel,
-                            dmass, dfriction, val13, tan13);
+        tangent_run<DualDD>(p, bodies, nb, joints, nj, springs, ns, manifolds, nm, nextSerial, ignores, ni, steps,
+                            dvel, dangvel, dmass, dfriction, val13, tan13);
     else
-        tangent_run<DualFD>(p, bodies, nb, joints, nj, springs, ns, manifolds, nm, nextSerial, steps, dvel, dangvel,
-                            dmass, dfriction, val13, tan13);
+        tangent_run<DualFD>(p, bodies, nb, joints, nj, springs, ns, manifolds, nm, nextSerial, ignores, ni, steps,
+                            dvel, dangvel, dmass, dfriction, val13, tan13);
 }
 
 void orc_f64_run(const avbd_params *p, const avbd_body *bodies, int nb, const avbd_joint *joints, int nj,
                  const avbd_spring *springs, int ns, const avbd_manifold *manifolds, int nm, int64_t nextSerial,
-                 int steps, const double *vel, const double *angvel, const double *mass, const double *friction,
-                 double *out13)
+                 const int32_t *ignores, int ni, int steps, const double *vel, const double *angvel,
+                 const double *mass, const double *friction, double *out13)
 {
     WorldImpl<double> impl;
     World<double> &w = impl.w;
     fill(w, p, bodies, nb, joints, nj, springs, ns, manifolds, nm, nextSerial);
+    for (int k = 0; k < ni; ++k)
+        w.ignores.push_back({ignores[2 * k], ignores[2 * k + 1]});
     for (int i = 0; i < nb; ++i)
     {
         Body<double> &b = w.bodies[i];

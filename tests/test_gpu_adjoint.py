@@ -158,13 +158,25 @@ def test_adjoint_joints_and_contacts(gpu, oracle):
     _check(gpu, oracle, st, prm, 30, 10, tol=1e-5, label="joint drop", terms=True)
 
 
-def test_adjoint_rejects_round_shapes_and_ignore_pairs(gpu, oracle):
+def test_adjoint_soft_body_with_ignore_pairs(gpu, oracle):
+    """The reference's Soft Body scene (scenes.h): 192 boxes joined by 432 joints, 648
+    IgnoreCollision pairs, resting on the ground: bodies the pairs make constrained are solved
+    (solver.cpp:1161-1166, 1190-1191), and the reversed warm start follows that decision."""
     from avbd import named_scenes, scenes
+    sc = named_scenes.build("Soft Body")
+    st, prm = _settled(oracle, sc, 20)
+    # the pairs are list indices; stepping keeps the body list, and the oracle's state omits them
+    st.ignores = scenes.build_state(sc, oracle.OracleMaker()).ignores
+    assert len(st.ignores) > 0 and len(st.joints) > 0 and len(st.manifolds) > 0
+    _check(gpu, oracle, st, prm, 20, 5, tol=1e-5, label="soft body", terms=True)
+
+
+def test_adjoint_rejects_round_shapes(gpu, oracle):
+    from avbd import scenes
     from avbd.state import default_params
-    st = scenes.build_state(named_scenes.build("Soft Body"), gpu.Maker())
+    st = scenes.build_state(scenes.round_stack("sphere", count=3), gpu.Maker())
     w = gpu.HipWorld.for_state(st, default_params())
-    w.set_ignore_pairs(st.ignores)
-    with pytest.raises(gpu.AvbdError, match="IgnoreCollision"):
+    with pytest.raises(gpu.AvbdError, match="box bodies only"):
         w.adjoint_forward(2, 1)
     with pytest.raises(gpu.AvbdError, match="no recorded forward"):
         w.adjoint_backward()

@@ -183,3 +183,35 @@ def test_fp64_tangent_with_joints_matches_central_differences(oracle, k_ang, kin
              base.astype(np.float64), u, 1e-5)
     rel = np.abs(t64[:, 0:7] - fd[:, 0:7]).max() / max(np.abs(fd[:, 0:7]).max(), 1e-12)
     assert rel < 1e-6, f"joint chain {kind}: FP64 tangent vs central differences {rel:.2e}"
+
+
+def _joined_pair(oracle, pair):
+    from avbd import named_scenes, scenes
+    from avbd.state import default_params
+    sb = named_scenes.SceneBuilder()
+    a = sb.box((0.8, 0.8, 0.8), 1.0, 0.5, (0.0, 0.0, 5.0))
+    if pair:
+        b = sb.box((0.8, 0.8, 0.8), 1.0, 0.5, (0.8, 0.0, 5.0))
+        sb.joint(a, b, (0.4, 0.0, 0.0), (-0.4, 0.0, 0.0), 5000.0, 1000.0)
+    sc = sb.scene("pair" if pair else "lone", -10.0, 20)
+    return scenes.build_state(sc, oracle.OracleMaker()), default_params(gravity=sc.gravity)
+
+
+@pytest.mark.parametrize("fp64", [False, True], ids=["fp32_values", "fp64_values"])
+def test_zero_force_joint_takes_the_zero_subgradient(oracle, fp64):
+    """Two boxes at rest joined face to face with finite stiffness (the Soft Body scene's joints,
+    joint.cpp:35-182): the joint's C is exactly 0, so the ball joint's geometric-stiffness
+    columns are zero vectors, whose length (maths.h:476) has a kink there, and the dual grows the
+    penalty by beta |C| at |C|'s kink. A uniform velocity seed translates the pair rigidly and
+    keeps C at 0, so each body's tangent is a lone box's: steps * dt * u in position (and finite,
+    where sqrt' at 0 would give NaN). The other seeds must give finite tangents too."""
+    pair, prm = _joined_pair(oracle, True)
+    lone, _ = _joined_pair(oracle, False)
+    u = np.array([0.3, -0.2, 1.0])
+    _, tp = oracle.oracle_tangent_run(pair, prm, 20, dvel=np.tile(u, (2, 1)), fp64=fp64)
+    _, t1 = oracle.oracle_tangent_run(lone, prm, 20, dvel=u[None, :], fp64=fp64)
+    np.testing.assert_allclose(tp, np.tile(t1, (2, 1)), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(tp[:, 0:3], np.tile(u * 20 * float(prm["dt"]), (2, 1)), rtol=1e-6)
+    for kind, seed in (("dangvel", np.ones((2, 3))), ("dmass", np.ones(2)), ("dfriction", np.ones(2))):
+        _, t = oracle.oracle_tangent_run(pair, prm, 20, fp64=fp64, **{kind: seed})
+        assert np.isfinite(t).all(), kind
