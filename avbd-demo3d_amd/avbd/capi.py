@@ -95,6 +95,9 @@ def load(path: str | None = None):
         lib.avbd_autotune.argtypes = [_p, C.c_int32, C.c_int32, _p]
         lib.avbd_adjoint_forward.argtypes = [_p, C.c_int32, C.c_int32]
         lib.avbd_adjoint_backward.argtypes = [_p] + [_p] * 10
+        lib.avbd_set_body_inputs.argtypes = [_p] * 5
+        lib.avbd_get_body_state_f64.argtypes = [_p] * 5
+        lib.avbd_adjoint_rewind.argtypes = [_p]
         check_layouts(lib.avbd_sizeof)
         _LIB = lib
         atexit.register(close_all)
@@ -269,6 +272,28 @@ class HipWorld:
                                                *[out[k].ctypes.data_as(_p) for k in
                                                  ("pos", "rot", "vel", "ang_vel", "mass", "friction")]))
         return out
+
+    # Raw-address forms (host or device memory, e.g. a torch tensor's data_ptr(); 0 or None: the
+    # field is kept / skipped): the C ABI copies with hipMemcpyDefault and is done with the memory
+    # when the call returns, so device inputs must be complete (their stream synchronized) first.
+    def set_body_inputs_ptr(self, vel=None, ang_vel=None, mass=None, friction=None):
+        """avbd_set_body_inputs: FP64 velocities, angular velocities (nb x 3), masses and friction (nb)."""
+        _check(self._lib.avbd_set_body_inputs(self._h, vel or None, ang_vel or None, mass or None,
+                                              friction or None))
+
+    def get_body_state_f64_ptr(self, pos=None, rot=None, vel=None, ang_vel=None):
+        """avbd_get_body_state_f64: FP64 pos, vel, ang_vel (nb x 3) and rot (nb x 4)."""
+        _check(self._lib.avbd_get_body_state_f64(self._h, pos or None, rot or None, vel or None, ang_vel or None))
+
+    def adjoint_rewind(self):
+        """avbd_adjoint_rewind: back to the start state of the recorded adjoint forward."""
+        _check(self._lib.avbd_adjoint_rewind(self._h))
+
+    def adjoint_backward_ptr(self, d_pos=None, d_rot=None, d_vel=None, d_ang_vel=None, g_pos=None, g_rot=None,
+                             g_vel=None, g_ang_vel=None, g_mass=None, g_friction=None):
+        """avbd_adjoint_backward with raw addresses (host or device memory)."""
+        _check(self._lib.avbd_adjoint_backward(self._h, *[a or None for a in (
+            d_pos, d_rot, d_vel, d_ang_vel, g_pos, g_rot, g_vel, g_ang_vel, g_mass, g_friction)]))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -1493,6 +1493,7 @@ struct AdjointState
     int steps = 0, every = 1;
     bool tapeAll = false;  // the forward taped every step (checkpoint_every = 0)
     bool recorded = false;
+    unsigned epoch = 0;  // the handle's upload count when the forward was recorded
     std::vector<Snapshot> ck, seg;
     Snapshot end;
     AdjBuf A{};
@@ -1764,9 +1765,147 @@ int check_adjoint_scene(avbd_handle *h)
     return AVBD_OK;
 }
 
+// Per body {dynamic, moment / mass} of the resident state (a dynamic body's moments scale with
+// its mass when avbd_set_body_inputs changes the mass).
+__global__ void k_mass_ratio(Bodies B, int n, double *ratio)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float4 mm = B.massMoment[i];
+    const bool dyn = mm.x > 0.0f;
+    const double m = dyn ? (double)mm.x : 1.0;
+    ratio[4 * i + 0] = dyn ? 1.0 : 0.0;
+    ratio[4 * i + 1] = dyn ? (double)mm.y / m : 0.0;
+    ratio[4 * i + 2] = dyn ? (double)mm.z / m : 0.0;
+    ratio[4 * i + 3] = dyn ? (double)mm.w / m : 0.0;
+}
+
+// FP64 inputs (staged: vel [3n], angular velocity [3n], mass [n], friction [n]) rounded into the
+// resident bodies; `which` bit k set: input k given. A body uploaded static keeps mass 0.
+__global__ void k_set_inputs(Bodies B, int n, const double *in, const double *ratio, int which)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const double *v = in, *w = in + 3 * (size_t)n, *m = in + 6 * (size_t)n, *f = in + 7 * (size_t)n;
+    if (which & 1)
+    {
+        float4 x = B.vel[i];
+        B.vel[i] = make_float4((float)v[3 * i], (float)v[3 * i + 1], (float)v[3 * i + 2], x.w);
+    }
+    if (which & 2)
+    {
+        float4 x = B.angVel[i];
+        B.angVel[i] = make_float4((float)w[3 * i], (float)w[3 * i + 1], (float)w[3 * i + 2], x.w);
+    }
+    if (which & 4)
+    {
+        const double mi = ratio[4 * i] > 0.0 ? m[i] : 0.0;
+        B.massMoment[i] = make_float4((float)mi, (float)(ratio[4 * i + 1] * mi), (float)(ratio[4 * i + 2] * mi),
+                                      (float)(ratio[4 * i + 3] * mi));
+    }
+    if (which & 8)
+        B.friction[i] = (float)f[i];
+}
+
+// Resident pose, rotation, velocity and angular velocity as FP64: out = pos [3n], rot [4n],
+// vel [3n], angular velocity [3n].
+__global__ void k_get_state_f64(Bodies B, int n, double *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float4 p = B.pose[4 * i + P_POS], q = B.pose[4 * i + P_ROT], v = B.vel[i], w = B.angVel[i];
+    double *pos = out, *rot = out + 3 * (size_t)n, *vel = out + 7 * (size_t)n, *ang = out + 10 * (size_t)n;
+    pos[3 * i] = p.x, pos[3 * i + 1] = p.y, pos[3 * i + 2] = p.z;
+    rot[4 * i] = q.x, rot[4 * i + 1] = q.y, rot[4 * i + 2] = q.z, rot[4 * i + 3] = q.w;
+    vel[3 * i] = v.x, vel[3 * i + 1] = v.y, vel[3 * i + 2] = v.z;
+    ang[3 * i] = w.x, ang[3 * i + 1] = w.y, ang[3 * i + 2] = w.z;
+}
+
+int io_alloc(avbd_handle *h)
+{
+    int r;
+    if (!h->io64 && ((r = h->alloc(h->io64, 13 * (size_t)std::max(h->maxBodies, 1))) ||
+                     (r = h->alloc(h->massRatio, 4 * (size_t)std::max(h->maxBodies, 1)))))
+        return r;
+    return AVBD_OK;
+}
+
 } // namespace
 
 extern "C" {
+
+int avbd_set_body_inputs(avbd_handle *h, const double *vel, const double *ang_vel, const double *mass,
+                         const double *friction)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_set_body_inputs: null handle");
+    int r;
+    if ((r = pick_device(h)) || (r = io_alloc(h)))
+        return r;
+    const int n = h->nb;
+    if (n == 0)
+        return AVBD_OK;
+    hipStream_t s = h->stream;
+    if (!h->ratioValid)
+    {
+        k_mass_ratio<<<ceil_div(n, 256), 256, 0, s>>>(h->B, n, h->massRatio);
+        HIP_TRY(hipGetLastError());
+        h->ratioValid = true;
+    }
+    const double *src[4] = {vel, ang_vel, mass, friction};
+    const size_t off[4] = {0, 3 * (size_t)n, 6 * (size_t)n, 7 * (size_t)n}, len[4] = {3, 3, 1, 1};
+    int which = 0;
+    for (int k = 0; k < 4; ++k)
+        if (src[k])
+        {
+            HIP_TRY(hipMemcpyAsync(h->io64 + off[k], src[k], len[k] * n * sizeof(double), hipMemcpyDefault, s));
+            which |= 1 << k;
+        }
+    if (which)
+    {
+        k_set_inputs<<<ceil_div(n, 256), 256, 0, s>>>(h->B, n, h->io64, h->massRatio, which);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return AVBD_OK;
+}
+
+int avbd_get_body_state_f64(avbd_handle *h, double *pos, double *rot, double *vel, double *ang_vel)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_get_body_state_f64: null handle");
+    int r;
+    if ((r = pick_device(h)) || (r = io_alloc(h)))
+        return r;
+    const int n = h->nb;
+    if (n == 0)
+        return AVBD_OK;
+    hipStream_t s = h->stream;
+    k_get_state_f64<<<ceil_div(n, 256), 256, 0, s>>>(h->B, n, h->io64);
+    HIP_TRY(hipGetLastError());
+    double *dst[4] = {pos, rot, vel, ang_vel};
+    const size_t off[4] = {0, 3 * (size_t)n, 7 * (size_t)n, 10 * (size_t)n}, len[4] = {3, 4, 3, 3};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k])
+            HIP_TRY(hipMemcpyAsync(dst[k], h->io64 + off[k], len[k] * n * sizeof(double), hipMemcpyDefault, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return AVBD_OK;
+}
+
+int avbd_adjoint_rewind(avbd_handle *h)
+{
+    if (!h)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_rewind: null handle");
+    if (!h->adj || !h->adj->recorded || h->adj->epoch != h->uploadEpoch)
+        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_rewind: no recorded forward of the uploaded state");
+    int r;
+    if ((r = pick_device(h)) || (r = snap_copy(h, false, h->adj->ck[0])) || (r = avbd_synchronize(h)))
+        return r;
+    return AVBD_OK;
+}
 
 int avbd_adjoint_forward(avbd_handle *h, int32_t num_steps, int32_t checkpoint_every)
 {
@@ -1780,6 +1919,7 @@ int avbd_adjoint_forward(avbd_handle *h, int32_t num_steps, int32_t checkpoint_e
         return r;
     AdjointState &S = *h->adj;
     S.recorded = false;
+    S.epoch = h->uploadEpoch;
     S.steps = num_steps;
     S.tapeAll = checkpoint_every == 0;
     S.every = S.tapeAll ? num_steps : checkpoint_every;
@@ -1812,8 +1952,9 @@ int avbd_adjoint_backward(avbd_handle *h, const double *d_pos, const double *d_r
 {
     if (!h)
         return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_backward: null handle");
-    if (!h->adj || !h->adj->recorded)
-        return fail(AVBD_ERR_INVALID_ARGUMENT, "avbd_adjoint_backward: no recorded forward (avbd_adjoint_forward)");
+    if (!h->adj || !h->adj->recorded || h->adj->epoch != h->uploadEpoch)
+        return fail(AVBD_ERR_INVALID_ARGUMENT,
+                    "avbd_adjoint_backward: no recorded forward (avbd_adjoint_forward) of the uploaded state");
     int r;
     if ((r = pick_device(h)) || (r = check_adjoint_scene(h)))
         return r;
@@ -1823,7 +1964,7 @@ int avbd_adjoint_backward(avbd_handle *h, const double *d_pos, const double *d_r
     const size_t N = h->nb;
     auto zero = [&](double *p, size_t n) { return hipMemsetAsync(p, 0, n * sizeof(double), s); };
     auto up = [&](double *dst, const double *src, size_t n) {
-        return src ? hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s) : zero(dst, n);
+        return src ? hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDefault, s) : zero(dst, n);
     };
     const size_t C3 = 3 * (size_t)S.slotCap;
     hipError_t e;
@@ -1865,7 +2006,7 @@ int avbd_adjoint_backward(avbd_handle *h, const double *d_pos, const double *d_r
     if ((r = snap_copy(h, false, S.end)))
         return r;
     auto down = [&](double *dst, const double *src, size_t n) {
-        return dst ? hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s) : hipSuccess;
+        return dst ? hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDefault, s) : hipSuccess;
     };
     if ((e = down(g_pos, A.xb, 3 * N)) || (e = down(g_rot, A.qb, 4 * N)) || (e = down(g_vel, A.vb, 3 * N)) ||
         (e = down(g_ang_vel, A.wb, 3 * N)) || (e = down(g_mass, A.mb, N)) || (e = down(g_friction, A.mub, N)) ||

@@ -172,6 +172,12 @@ struct avbd_handle
     Joint *tapeJoint = nullptr;  // joint states after init (slot 0) and after each iteration's dual
     struct AdjointState *adj = nullptr;
     int lastNumManifolds = 0;
+    // FP64 body inputs and outputs in host or device memory (avbd_set_body_inputs,
+    // avbd_get_body_state_f64): a staging buffer, and per body {dynamic, moment / mass} of the
+    // state as last uploaded (taken at the first avbd_set_body_inputs after an upload)
+    double *io64 = nullptr, *massRatio = nullptr;
+    bool ratioValid = false;
+    unsigned uploadEpoch = 0;  // counts uploads: a recorded forward belongs to one of them
 
     template <class T>
     int alloc(T *&p, size_t count)
@@ -1588,6 +1594,8 @@ int avbd_upload_state(avbd_handle *h, const avbd_body *bodies, int32_t num_bodie
     int r = pick_device(h);
     if (r)
         return r;
+    h->ratioValid = false;
+    ++h->uploadEpoch;
     int N = num_bodies;
     std::vector<float4> pos(N), rot(N), il(N), ia(N), nl(N), na(N), vel(N), av(N), pv(N), mm(N), sr(N), sh(N);
     std::vector<float> fr(N);

@@ -334,40 +334,48 @@ def adjoint_leg(args):
     t_plain = min(times)
     sim = adjoint.Rollout(st, prm, steps, checkpoint_every=every)
     b = st.bodies
-    t = lambda a: torch.tensor(np.asarray(a, np.float64), requires_grad=True)
+    # inputs, positions and gradients stay in device tensors (the rollout's device-pointer path)
+    dev = torch.device("cuda", 0)
+    t = lambda a: torch.tensor(np.asarray(a, np.float64), device=dev, requires_grad=True)
+    pile_t, target_t = torch.as_tensor(pile, device=dev), torch.as_tensor(target, device=dev)
     best = None
     for _ in range(3):
         vel, ang, mass, fric = t(b["vel"]), t(b["ang_vel"]), t(b["mass"]), t(b["friction"])
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         pos = sim(vel, ang, mass, fric)
         t1 = time.perf_counter()
-        w = mass[torch.as_tensor(pile)]
-        com = (pos[torch.as_tensor(pile)] * w[:, None]).sum(0) / w.sum()
-        loss = ((com - torch.as_tensor(target)) ** 2).sum()
+        w = mass[pile_t]
+        com = (pos[pile_t] * w[:, None]).sum(0) / w.sum()
+        loss = ((com - target_t) ** 2).sum()
         loss.backward()
+        torch.cuda.synchronize()
         t2 = time.perf_counter()
         if best is None or (t2 - t0) < best[0] + best[1]:
             best = (t1 - t0, t2 - t1)
     t_fwd, t_bwd = best
     log(f"adjoint: plain {t_plain * 1e3:.1f} ms, forward with checkpoints {t_fwd * 1e3:.1f} ms, "
         f"backward {t_bwd * 1e3:.1f} ms -> {(t_fwd + t_bwd) / t_plain:.2f}x the plain forward")
-    grads = {"vel": vel.grad.numpy().copy(), "ang_vel": ang.grad.numpy().copy(), "mass": mass.grad.numpy().copy(),
-             "friction": fric.grad.numpy().copy()}
+    grads = {"vel": vel.grad.cpu().numpy(), "ang_vel": ang.grad.cpu().numpy(), "mass": mass.grad.cpu().numpy(),
+             "friction": fric.grad.cpu().numpy()}
     # the same rollout with every step taped in the forward (checkpoint_every = 0: no recompute)
     sim_all = adjoint.Rollout(st, prm, steps, checkpoint_every=0)
     best_all = None
     for _ in range(3):
         va, aa, ma, fa = t(b["vel"]), t(b["ang_vel"]), t(b["mass"]), t(b["friction"])
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         pa = sim_all(va, aa, ma, fa)
         t1 = time.perf_counter()
-        wa = ma[torch.as_tensor(pile)]
-        ca = (pa[torch.as_tensor(pile)] * wa[:, None]).sum(0) / wa.sum()
-        ((ca - torch.as_tensor(target)) ** 2).sum().backward()
+        wa = ma[pile_t]
+        ca = (pa[pile_t] * wa[:, None]).sum(0) / wa.sum()
+        ((ca - target_t) ** 2).sum().backward()
+        torch.cuda.synchronize()
         t2 = time.perf_counter()
         if best_all is None or (t2 - t0) < best_all[0] + best_all[1]:
             best_all = (t1 - t0, t2 - t1)
-    ga = {"vel": va.grad.numpy(), "ang_vel": aa.grad.numpy(), "mass": ma.grad.numpy(), "friction": fa.grad.numpy()}
+    ga = {"vel": va.grad.cpu().numpy(), "ang_vel": aa.grad.cpu().numpy(), "mass": ma.grad.cpu().numpy(),
+          "friction": fa.grad.cpu().numpy()}
     sim_all.close()
     same = max(float(np.abs(ga[k] - grads[k]).max() / max(np.abs(grads[k]).max(), 1e-300)) for k in grads)
     log(f"adjoint, every step taped in the forward: forward {best_all[0] * 1e3:.1f} ms, backward "
@@ -386,7 +394,7 @@ def adjoint_leg(args):
                             "max_rel_diff_vs_checkpointed": same},
     }
     if not args.no_adjoint_check:
-        out["checks"] = adjoint_checks(sim, st, prm, steps, pile, proj, target, pos.detach().numpy(), grads)
+        out["checks"] = adjoint_checks(sim, st, prm, steps, pile, proj, target, pos.detach().cpu().numpy(), grads)
     sim.close()
     return out
 

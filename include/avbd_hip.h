@@ -456,6 +456,26 @@ int avbd_adjoint_backward(avbd_handle *h, const double *d_pos, const double *d_r
                           double *g_pos, double *g_rot, double *g_vel, double *g_ang_vel,
                           double *g_mass, double *g_friction);
 
+/* FP64 body inputs and outputs for differentiable rollouts driven from device
+ * tensors (no counterpart in the reference). Pointers may be device memory or
+ * host memory (copies use hipMemcpyDefault); each call is done with them when
+ * it returns; d_* and g_* of avbd_adjoint_backward may be either kind as well.
+ * avbd_set_body_inputs overwrites the resident bodies' linear and angular
+ * velocities (nb x 3), masses (nb: a dynamic body's moments scale with its
+ * mass by the moment/mass ratios of the state as uploaded; a body uploaded
+ * static keeps mass 0) and friction coefficients (nb), each rounded to FP32; a
+ * NULL pointer keeps that field. avbd_get_body_state_f64 reads pos (nb x 3),
+ * rot (nb x 4, x y z w), vel and ang_vel (nb x 3) as FP64; NULL outputs are
+ * skipped. avbd_adjoint_rewind returns the handle to the start of the recorded
+ * avbd_adjoint_forward (its first checkpoint: poses, velocities, contacts,
+ * joints; masses and friction are left as they are), so that new inputs can
+ * be set and the forward run again without an upload; AVBD_ERR_INVALID_ARGUMENT
+ * when no forward of the uploaded state is recorded (an upload discards it). */
+int avbd_set_body_inputs(avbd_handle *h, const double *vel, const double *ang_vel, const double *mass,
+                         const double *friction);
+int avbd_get_body_state_f64(avbd_handle *h, double *pos, double *rot, double *vel, double *ang_vel);
+int avbd_adjoint_rewind(avbd_handle *h);
+
 /* Test hook: make the next avbd_step on this handle fail with AVBD_ERR_INJECTED
  * before any work is enqueued (code 1), or clear it (code 0). */
 int avbd_debug_inject_fault(avbd_handle *h, int32_t code);

@@ -236,6 +236,84 @@ def test_rollout_autograd_function_matches_forward_mode(gpu, oracle, count, proj
     sim.close()
 
 
+def test_rollout_device_tensors_equal_the_host_path(gpu):
+    """The Rollout with CUDA tensors (avbd_set_body_inputs / avbd_get_body_state_f64 /
+    avbd_adjoint_rewind, pointers into device memory, no host copies of the tensors) gives the
+    host path's results: the same position bits, and the same gradients up to the order in which
+    FP64 atomics accumulate them (1e-11 of the largest), on a first call, on a second call with
+    other inputs (the rewind to the recorded start), and after a host-path call on the same
+    Rollout (which uploads, so the device path uploads the original state again)."""
+    import torch
+    from avbd import adjoint, scenes
+    from avbd.state import default_params
+    sc = scenes.projectile_pile(count=64, projectiles=4, seed=9)
+    st, pile, proj, target = scenes.projectile_state(sc, gpu.Maker(), seed=9)
+    prm = default_params(gravity=sc.gravity)
+    b = st.bodies
+    rng = np.random.default_rng(2)
+    inputs = [(b["vel"], b["ang_vel"], b["mass"], b["friction"]),
+              (b["vel"] + rng.normal(size=b["vel"].shape) * 0.1, b["ang_vel"] * 0.5,
+               b["mass"] * (1.0 + 0.1 * rng.random(len(b))), b["friction"] * 0.9)]
+
+    # a loss linear in the positions, so that dL/dpos is the same bits on both devices (torch's
+    # own reductions round differently on the CPU and the GPU)
+    c = rng.normal(size=(len(b), 3))
+
+    def run(sim, xs, dev):
+        ts = [torch.tensor(np.asarray(x, np.float64), device=dev, requires_grad=True) for x in xs]
+        pos = sim(*ts)
+        pos.backward(torch.as_tensor(c, device=dev))
+        return [pos.detach().cpu().numpy()] + [t.grad.cpu().numpy() for t in ts]
+
+    host = adjoint.Rollout(st, prm, 60, checkpoint_every=10)
+    dev = adjoint.Rollout(st, prm, 60, checkpoint_every=10)
+    for k, xs in enumerate(inputs + inputs[:1]):
+        if k == 2:
+            run(dev, inputs[1], "cpu")   # a host-path call in between
+        want, got = run(host, xs, "cpu"), run(dev, xs, "cuda")
+        np.testing.assert_array_equal(got[0], want[0], err_msg=f"call {k}: pos")
+        for name, a, g in zip(("vel", "ang_vel", "mass", "friction"), want[1:], got[1:]):
+            err = np.abs(g - a).max() / max(np.abs(a).max(), 1e-300)
+            assert err < 1e-11, f"call {k}: {name} differs by {err:.1e} of the largest"
+    np.testing.assert_array_equal(dev.final.bodies["pos"], host.final.bodies["pos"])
+    host.close()
+    dev.close()
+
+
+def test_body_state_io_through_host_pointers(gpu):
+    """avbd_get_body_state_f64 into host memory equals the downloaded state; avbd_set_body_inputs
+    from host memory sets what a download then shows (masses with their moments scaled, a static
+    body left static); avbd_adjoint_rewind without a recorded forward is refused."""
+    from avbd import scenes
+    from avbd.state import default_params
+    st = scenes.build_state(scenes.pile(20, seed=3), gpu.Maker())
+    w = gpu.HipWorld.for_state(st, default_params())
+    w.step(5)
+    ref = w.state().bodies
+    nb = len(ref)
+    out = {k: np.zeros((nb, n)) for k, n in (("pos", 3), ("rot", 4), ("vel", 3), ("ang_vel", 3))}
+    w.get_body_state_f64_ptr(*[out[k].ctypes.data for k in ("pos", "rot", "vel", "ang_vel")])
+    for k in out:
+        np.testing.assert_array_equal(out[k], ref[k].astype(np.float64), err_msg=k)
+    rng = np.random.default_rng(1)
+    v, a = rng.normal(size=(nb, 3)), rng.normal(size=(nb, 3))
+    m, f = 1.0 + rng.random(nb), rng.random(nb)
+    w.set_body_inputs_ptr(v.ctypes.data, a.ctypes.data, m.ctypes.data, f.ctypes.data)
+    got = w.state().bodies
+    dyn = st.bodies["mass"] > 0
+    assert (~dyn).any() and dyn.any()
+    np.testing.assert_array_equal(got["vel"], v.astype(np.float32))
+    np.testing.assert_array_equal(got["ang_vel"], a.astype(np.float32))
+    np.testing.assert_array_equal(got["friction"], f.astype(np.float32))
+    np.testing.assert_array_equal(got["mass"], np.where(dyn, m, 0.0).astype(np.float32))
+    ratio = st.bodies["moment"][dyn].astype(np.float64) / st.bodies["mass"][dyn, None].astype(np.float64)
+    np.testing.assert_array_equal(got["moment"][dyn], (ratio * m[dyn, None]).astype(np.float32))
+    np.testing.assert_array_equal(got["moment"][~dyn], 0.0)
+    with pytest.raises(gpu.AvbdError, match="no recorded forward"):
+        w.adjoint_rewind()
+    w.close()
+
+
 def test_adjoint_against_frozen_branch_central_differences(gpu, oracle):
     """The GPU adjoint against FP64 central differences (h = 1e-4) of the frozen-branch run
     (orc_shadow_run: the FP32 trajectory's contact sets, clamps and stick decisions, FP64
