@@ -281,7 +281,7 @@ def main():
             "bytes_per_launch": pr_bytes,
             "launch_us": pr_ms * 1e3,
             "iteration_loop": {"kernels": ("k_primal_flow (all iterations) + k_dual" if flow else
-                                           "k_primal_color x colours + k_dual, x10"), "ms_per_step": iter_ms,
+                                           f"{pr_kernel} x colours + k_dual, x10"), "ms_per_step": iter_ms,
                                "bytes_per_step": iter_bytes, "achieved": loop_gbs,
                                "frac": loop_gbs / HBM_PEAK_GBS},
             "k_dual": {"launch_us": du_ms * 1e3, "bytes_per_launch": du_bytes,

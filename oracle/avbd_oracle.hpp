@@ -157,6 +157,14 @@ template <class V> inline DualT<V> abs_(DualT<V> x) { return x.v < V(0) ? -x : x
 // narrowphase's |x| keeps abs_).
 template <class R> inline R absk_(R x) { return abs_(x); }
 template <class V> inline DualT<V> absk_(DualT<V> x) { return x.v == V(0) ? mk<V>(abs_(x).v, 0.0) : abs_(x); }
+// clamp(x, 0, 1) of the warm start's gravity weight (solver.cpp:1145-1149): on a bound its
+// derivative is taken as 0 too, as the adjoint takes it.
+template <class R> inline R clampk_(R x, R a, R b) { return clampv(x, a, b); }
+template <class V> inline DualT<V> clampk_(DualT<V> x, DualT<V> a, DualT<V> b)
+{
+    DualT<V> c = clampv(x, a, b);
+    return x.v > a.v && x.v < b.v ? c : mk<V>(c.v, 0.0);
+}
 // The plain value of a scalar: the broadphase's discrete decisions use the value alone.
 inline float plain_(float x) { return x; }
 inline double plain_(double x) { return x; }
@@ -2047,7 +2055,7 @@ struct World
             b.inerAng = b.rot + b.angVel * p.dt;
             V3<R> accel = (b.vel - b.prevVel) / p.dt;
             R accelExt = accel.z * signv(p.gravity);
-            R w = clampv(accelExt / abs_(p.gravity), R(0), R(1));
+            R w = clampk_(accelExt / abs_(p.gravity), R(0), R(1));
             if (!isfinite_(w))
                 w = 0;
             b.initLin = b.pos;
