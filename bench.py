@@ -113,6 +113,8 @@ def main():
                     help="set launch-shape fields (primal_lanes, primal_threads, dual_threads, use_graph, "
                          "dataflow, lazy_dual) instead of autotuning, e.g. primal_lanes=16,primal_threads=64")
     ap.add_argument("--batch-worlds", type=int, default=4096, help="0 skips the batched measurement")
+    ap.add_argument("--batch-launch", default="", metavar="KEY=V,...",
+                    help="as --launch, for the batched leg (instead of autotuning it)")
     ap.add_argument("--batch-steps", type=int, default=300)
     ap.add_argument("--batch-warmup", type=int, default=20)
     ap.add_argument("--batch-sample", type=int, default=64,
@@ -585,8 +587,17 @@ def batched(args, params):
     world = capi.HipWorld.for_batch(states, params)
     world.step(2)
     world.synchronize()
-    launch = world.autotune(trial_steps=6, allow_graph=True)
-    log(f"batched: {n_bodies} bodies ({n_dyn} dynamic); autotuned launch shape {launch}")
+    if args.batch_launch:
+        launch = world.launch_config()
+        for kv in args.batch_launch.split(","):
+            key, val = kv.split("=")
+            launch[key.strip()] = int(val)
+        world.set_launch_config(**launch)
+        launch = world.launch_config()
+        log(f"batched: {n_bodies} bodies ({n_dyn} dynamic); launch shape set to {launch}")
+    else:
+        launch = world.autotune(trial_steps=6, allow_graph=True)
+        log(f"batched: {n_bodies} bodies ({n_dyn} dynamic); autotuned launch shape {launch}")
     world.step(args.batch_warmup)
     world.synchronize()
     torch.cuda.synchronize()
