@@ -1,8 +1,8 @@
 """Differentiable rollouts of the resident AVBD step as a torch.autograd.Function.
 
 A thin layer over the C ABI's adjoint (avbd_adjoint_forward / avbd_adjoint_backward in
-include/avbd_hip.h): the forward sets a state of boxes, contacts, joints, springs and
-IgnoreCollision pairs whose initial velocities, angular velocities, masses (moments scale with
+include/avbd_hip.h): the forward sets a state of boxes and round shapes with contacts, joints,
+springs and IgnoreCollision pairs whose initial velocities, angular velocities, masses (moments scale with
 the mass) and friction coefficients come from tensors, steps it on the GPU keeping checkpoints,
 and returns the final positions; the backward runs the hand-written reverse sweep on the GPU and
 returns the gradients of those inputs. With CUDA (ROCm) tensors nothing goes through host
@@ -59,7 +59,7 @@ class _RolloutFn:
 
 
 class Rollout:
-    """`steps` GPU steps of a world of boxes (contacts, joints, springs) as a differentiable function of the initial
+    """`steps` GPU steps of a world (boxes and round shapes; contacts, joints, springs) as a differentiable function of the initial
     velocities, angular velocities, masses and friction coefficients (all bodies, list order).
     checkpoint_every: checkpoints every that many steps, each segment recomputed by the backward;
     0 tapes every step in the forward (more HBM, no recompute)."""

@@ -156,6 +156,8 @@ struct CIn
     DQ qA, qB, iAA, iAB;
     double mu, oma;
     bool clamp0, scaled;
+    int code;                        // the record's shape code (0: box pair)
+    float radA, hlA, radB, hlB;      // radii and half lengths of the two shapes
 };
 struct CFwd
 {
@@ -170,6 +172,42 @@ struct COut
     double mu;
 };
 
+// Adjoint of a contact point's world offset (manifold.cpp:29-90). Box sides rotate the stored
+// anchor (drot_adj). A sphere's offset is -+radius n. The other round offsets (capsule: the core
+// point along the axis, clamped; cylinder: rim or cap point) are functions of the rotation, the
+// stored anchor and the manifold normal, reversed through their forward-mode Jacobian: one DualF
+// evaluation per input component at the step's FP32 values, so the branches are the step's.
+__device__ void offset_adj(int code, bool isA, float radius, float hl, DQ q, DV lo, DV n, DV ob, DQ &qb, DV &lob,
+                           DV &nb)
+{
+    const int shape = code == 0 ? kShapeBox : (isA ? (code & 3) : ((code >> 2) & 3));
+    if (shape == kShapeBox)
+    {
+        drot_adj(q, lo, ob, qb, lob);
+        return;
+    }
+    if (shape == kShapeSphere)
+    {
+        nb += ob * (isA ? -(double)radius : (double)radius);
+        return;
+    }
+    const bool cap = (code & kCodeCap) != 0;
+    const float qv[4] = {(float)q.x, (float)q.y, (float)q.z, (float)q.w};
+    const float lv[3] = {(float)lo.x, (float)lo.y, (float)lo.z}, nv[3] = {(float)n.x, (float)n.y, (float)n.z};
+    double g[10];
+    for (int c = 0; c < 10; ++c)
+    {
+        auto sd = [&](float v, int k) { return DualF(v, c == k ? 1.0 : 0.0); };
+        const Q4T<DualF> qd{sd(qv[0], 0), sd(qv[1], 1), sd(qv[2], 2), sd(qv[3], 3)};
+        const V3T<DualF> ld{sd(lv[0], 4), sd(lv[1], 5), sd(lv[2], 6)}, nd{sd(nv[0], 7), sd(nv[1], 8), sd(nv[2], 9)};
+        const V3T<DualF> r = contact_offset_t<DualF>(shape, radius, hl, cap, qd, ld, nd, isA);
+        g[c] = ob.x * r.x.d + ob.y * r.y.d + ob.z * r.z.d;
+    }
+    qb += DQ{g[0], g[1], g[2], g[3]};
+    lob += DV{g[4], g[5], g[6]};
+    nb += DV{g[7], g[8], g[9]};
+}
+
 // The contact's forward values at which the reversal linearises: exactly those the FP32 step
 // computed (contact_eval, manifold.cpp:176-229), promoted to FP64. The inputs are FP32 values held
 // in FP64, so converting back is exact. Recomputing them in FP64 instead would not be the step's
@@ -181,7 +219,19 @@ __device__ __forceinline__ Q4 f4(DQ q) { return Q4{(float)q.x, (float)q.y, (floa
 __device__ __forceinline__ void cfwd(const CIn &in, CFwd &f)
 {
     const Q4 qA = f4(in.qA), qB = f4(in.qB);
-    const V3 rAW = rotate(qA, f3(in.rA)), rBW = rotate(qB, f3(in.rB));
+    V3 rAW, rBW;
+    if (in.code == 0)
+    {
+        rAW = rotate(qA, f3(in.rA));
+        rBW = rotate(qB, f3(in.rB));
+    }
+    else
+    {
+        const bool cap = (in.code & kCodeCap) != 0;
+        const V3 n0 = f3(in.n[0]);
+        rAW = contact_offset(in.code & 3, in.radA, in.hlA, cap, qA, f3(in.rA), n0, true);
+        rBW = contact_offset((in.code >> 2) & 3, in.radB, in.hlB, cap, qB, f3(in.rB), n0, false);
+    }
     const V3 dqAL = f3(in.pA) - f3(in.iLA), dqBL = f3(in.pB) - f3(in.iLB);
     const V3 dqAA = qdiff(qA, f4(in.iAA)), dqBA = qdiff(qB, f4(in.iAB));
     const V3 b0 = f3(in.n[0]), b1 = f3(in.n[1]), b2 = f3(in.n[2]);
@@ -258,8 +308,8 @@ __device__ __forceinline__ void cadj(const CIn &in, const CFwd &f, const double 
     o.iLB += dALb;
     dqdiff_adj(in.qA, in.iAA, dAAb, o.qA, o.iAA);
     dqdiff_adj(in.qB, in.iAB, dBAb, o.qB, o.iAB);
-    drot_adj(in.qA, in.rA, rAWb, o.qA, o.rA);
-    drot_adj(in.qB, in.rB, rBWb, o.qB, o.rB);
+    offset_adj(in.code, true, in.radA, in.hlA, in.qA, in.rA, in.n[0], rAWb, o.qA, o.rA, o.n[0]);
+    offset_adj(in.code, false, in.radB, in.hlB, in.qB, in.rB, in.n[0], rBWb, o.qB, o.rB, o.n[0]);
 }
 
 __device__ __forceinline__ void cout_zero(COut &o)
@@ -295,6 +345,11 @@ __device__ __forceinline__ void cin_load(CIn &in, const float4 *cr, float4 pen, 
     in.lam = dv4(lam);
     in.mu = cr[CR_RA].w;
     in.oma = (double)oma;
+    in.code = __float_as_int(cr[CR_RB].w);
+    in.radA = cr[CR_N].w;
+    in.hlA = cr[CR_T2].w;
+    in.radB = cr[CR_T1].w;
+    in.hlB = cr[CR_C0].w;
 }
 
 // FP32 evaluation of the contact exactly as the step did it: branch flags and the stamp record.
@@ -312,8 +367,8 @@ __device__ __forceinline__ ContactEval c_eval32(const float4 *cr, float4 pen4, f
     b1 = ld3(cr[CR_T1]);
     b2 = ld3(cr[CR_T2]);
     V3 nb0 = -b0, nb1 = -b1, nb2 = -b2;
-    V3 rAW = rotate(qA, ld3(cr[CR_RA]));
-    V3 rBW = rotate(qB, ld3(cr[CR_RB]));
+    V3 rAW, rBW;
+    contact_anchors(cr[CR_RA], cr[CR_RB], cr[CR_C0], cr[CR_N], cr[CR_T1], cr[CR_T2], qA, qB, rAW, rBW);
     ja0 = cross(rAW, b0);
     ja1 = cross(rAW, b1);
     ja2 = cross(rAW, b2);
@@ -396,8 +451,10 @@ __device__ __forceinline__ void adj_finish_body(const Bodies &B, const Params &P
 
 // Start of a step's reversal: contact adjoints of the step's final contacts come from the carry
 // (written by the reversal of the next step, only ever at slots of these contacts, and cleared here
-// as they are taken); everything per step is cleared; then finishStep is reversed per body.
-__global__ void k_adj_begin(Bodies B, Params P, const int4 *contactRec, int nc, int nm, int nb, AdjBuf A)
+// as they are taken); everything per step is cleared; then (finish) finishStep is reversed per
+// body. Worlds with spheres reverse their rolling friction in between (k_adj_rolling, then
+// k_adj_finish).
+__global__ void k_adj_begin(Bodies B, Params P, const int4 *contactRec, int nc, int nm, int nb, bool finish, AdjBuf A)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nc)
@@ -435,8 +492,133 @@ __global__ void k_adj_begin(Bodies B, Params P, const int4 *contactRec, int nc, 
             A.iab[4 * t + k] = 0.0;
             A.nab[4 * t + k] = 0.0;
         }
-        adj_finish_body(B, P, t, A);
+        if (finish)
+            adj_finish_body(B, P, t, A);
     }
+}
+
+__global__ void k_adj_finish(Bodies B, Params P, int nb, AdjBuf A)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nb)
+        adj_finish_body(B, P, t, A);
+}
+
+// applySphereRollingFriction (solver.cpp:628-660) reversed, one thread per solved body. A dynamic
+// sphere's velocities after finishStep (recomputed from its pose records as k_finish forms them)
+// pass, once per contact point of each manifold with a static partner where it is the pair's
+// chosen sphere, through v -= blend slip, w += blend (roll - w), in its stamps' order (the
+// force-list order the forward visits). The end-of-step velocity adjoints go back through those
+// passes (each pass's input recomputed from the start, so nothing is stored), the normal's
+// adjoint to the manifold's basis row 0, and blend = clamp(0.08 sqrt(mu_s mu_o), 0, 0.18) hands
+// its adjoint to both friction coefficients.
+__global__ void k_adj_rolling(Bodies B, Manifolds M, const int4 *stamps, const int4 *tasks, int numTasks, Params P,
+                              AdjBuf A)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= numTasks)
+        return;
+    const int4 task = tasks[k];
+    const int i = task.x;
+    const float4 hs = B.shape[i];
+    if ((int)hs.z != kShapeSphere || !(B.massMoment[i].x > 0.0f))
+        return;
+    const float r = hs.x, fi = B.friction[i];
+    const V3 v0 = (ld3(B.pose[4 * i + P_POS]) - ld3(B.pose[4 * i + P_ILIN])) / P.dt;
+    const V3 w0 = qdiff(ldq(B.pose[4 * i + P_ROT]), ldq(B.pose[4 * i + P_IANG])) / P.dt;
+    // visit(limit, f): the passes in forward order, f(pass, m, isA, o, mu, blend, nrm) for pass < limit;
+    // returns the pass count
+    auto visit = [&](int limit, auto &&f) {
+        int pass = 0, last = -1;
+        for (int t = 0; t < task.z; ++t)
+        {
+            const int4 st = stamps[task.y + t];
+            if (adjKind(st.x) != kAdjManifold)
+                continue;
+            const int m = st.w / kMaxContacts;
+            if (m == last)
+                continue;  // the manifold's other contact points: its passes are counted below
+            last = m;
+            const int2 ab = M.ab[m];
+            const bool isA = ab.x == i;
+            if (!isA && (int)B.shape[ab.x].z == kShapeSphere)
+                continue;
+            const int o = isA ? ab.y : ab.x, nc = M.count[m];
+            if (B.massMoment[o].x > 0.0f || nc <= 0)
+                continue;
+            const float mu = sqrtf(fi * B.friction[o]);
+            if (mu <= 0.0f)
+                continue;
+            const float blend = clampr(mu * 0.08f, 0.0f, 0.18f);
+            const V3 n0 = ld3(M.cr[(size_t)m * kMaxContacts * kCR + CR_N]);
+            const V3 nrm = isA ? n0 : -n0;
+            for (int c = 0; c < nc; ++c, ++pass)
+                if (pass < limit)
+                    f(pass, m, isA, o, mu, blend, nrm);
+        }
+        return pass;
+    };
+    const int passes = visit(0, [](int, int, bool, int, float, float, V3) {});
+    if (passes == 0)
+        return;
+    DV vb = ld_dv(A.vb + 3 * i), wb = ld_dv(A.wb + 3 * i);
+    for (int p = passes - 1; p >= 0; --p)
+    {
+        // this pass's input velocities, and its constants
+        V3 v = v0, w = w0, nrm{0, 0, 0};
+        float blend = 0.0f, mu = 0.0f;
+        int m = 0, o = 0;
+        bool isA = true;
+        visit(p + 1, [&](int q, int mq, bool aq, int oq, float muq, float bq, V3 nq) {
+            if (q < p)
+            {
+                const V3 rW = -nq * r;
+                const V3 cv = v + cross(w, rW);
+                const V3 slip = cv - nq * dot(cv, nq);
+                v = v - slip * bq;
+                const V3 roll = cross(nq, v) / r;
+                w = w + (roll - w) * bq;
+            }
+            else
+            {
+                m = mq, isA = aq, o = oq, mu = muq, blend = bq, nrm = nq;
+            }
+        });
+        const DV n = dv(nrm), vd = dv(v), wd = dv(w);
+        const double bl = blend, rr = r;
+        const DV rW = n * (-rr);
+        const DV cv = vd + dcross(wd, rW);
+        const DV slip = cv - n * ddot(cv, n);
+        const DV v1 = vd - slip * bl;
+        const DV roll = dcross(n, v1) * (1.0 / rr);
+        double blb = ddot(wb, roll - wd);
+        const DV rollb = wb * bl;
+        DV wprev = wb * (1.0 - bl);
+        DV v1b = vb + dcross(rollb, n) * (1.0 / rr);
+        DV nb = dcross(v1, rollb) * (1.0 / rr);
+        DV vprev = v1b;
+        const DV slipb = v1b * (-bl);
+        blb -= ddot(v1b, slip);
+        const double ns = ddot(n, slipb), cn = ddot(cv, n);
+        const DV cvb = slipb - n * ns;
+        nb += slipb * (-cn) - cv * ns;
+        vprev += cvb;
+        wprev += dcross(rW, cvb);
+        const DV rWb = dcross(cvb, wd);
+        nb += rWb * (-rr);
+        vb = vprev;
+        wb = wprev;
+        atomic_add3(A.basisb + 9 * m, isA ? nb : nb * -1.0);
+        const float x = mu * 0.08f;
+        if (x >= 0.0f && x <= 0.18f)
+        {
+            const double mub = blb * 0.08, fo = B.friction[o];
+            aadd(A.mub + i, mub * fo / (2.0 * (double)mu));
+            aadd(A.mub + o, mub * (double)fi / (2.0 * (double)mu));
+        }
+    }
+    st_dv(A.vb + 3 * i, vb);
+    st_dv(A.wb + 3 * i, wb);
 }
 
 
@@ -1307,7 +1489,9 @@ __global__ void k_adj_warmstart(Bodies B, Params P, const int *deg, const int *i
 
 // Force init of one contact reversed (manifold.cpp:100-174): C0 = basis (xA - xB) + margin,
 // lambda = lambda_old alpha gamma, k = clamp(k_old gamma); anchors kept by the stick rule hand
-// their adjoint to the previous step's contact.
+// their adjoint to the previous step's contact. Pairs with a round shape refresh their anchors
+// (manifold.cpp:14-27): a matched contact keeps only its normal lambda and penalty (the
+// tangential rows restart at 0 and the minimum penalty), and its offsets are the shapes'.
 __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc, const int *srcSlot,
                            const int *matchOld, const int *stickUsed, const float4 *oldCr, int numOld, Params P,
                            AdjBuf A)
@@ -1321,7 +1505,17 @@ __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc
     DQ qA = dq4(B.pose[4 * a + P_IANG]), qB = dq4(B.pose[4 * b + P_IANG]);
     DV rA = dv4(cr[CR_RA]), rB = dv4(cr[CR_RB]);
     DV pA = dv4(B.pose[4 * a + P_ILIN]), pB = dv4(B.pose[4 * b + P_ILIN]);
-    DV d = (pA + drot(qA, rA)) - (pB + drot(qB, rB));
+    const int code = __float_as_int(cr[CR_RB].w);
+    DV d;
+    if (code == 0)
+        d = (pA + drot(qA, rA)) - (pB + drot(qB, rB));
+    else
+    {
+        V3 oA, oB;
+        contact_anchors(cr[CR_RA], cr[CR_RB], cr[CR_C0], cr[CR_N], cr[CR_T1], cr[CR_T2], ldq(B.pose[4 * a + P_IANG]),
+                        ldq(B.pose[4 * b + P_IANG]), oA, oB);
+        d = (pA + dv(oA)) - (pB + dv(oB));
+    }
     DV c0b = ld_dv(A.c0b + 3 * sl);
     DV db{0, 0, 0};
     DV n[3] = {dv4(cr[CR_N]), dv4(cr[CR_T1]), dv4(cr[CR_T2])};
@@ -1332,9 +1526,12 @@ __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc
         atomic_add3(A.basisb + 9 * m + 3 * k, d * ck);
     }
     DQ qAb{0, 0, 0, 0}, qBb{0, 0, 0, 0};
-    DV rAb = ld_dv(A.rAb + 3 * sl), rBb = ld_dv(A.rBb + 3 * sl);
-    drot_adj(qA, rA, db, qAb, rAb);
-    drot_adj(qB, rB, -db, qBb, rBb);
+    DV rAb = ld_dv(A.rAb + 3 * sl), rBb = ld_dv(A.rBb + 3 * sl), n0b{0, 0, 0};
+    const DV n0 = dv4(cr[CR_N]);
+    offset_adj(code, true, cr[CR_N].w, cr[CR_T2].w, qA, rA, n0, db, qAb, rAb, n0b);
+    offset_adj(code, false, cr[CR_T1].w, cr[CR_C0].w, qB, rB, n0, -db, qBb, rBb, n0b);
+    if (code != 0)
+        atomic_add3(A.basisb + 9 * m, n0b);
     bool massA = B.massMoment[a].x > 0.0f, massB = B.massMoment[b].x > 0.0f;
     if (massA)
     {
@@ -1349,22 +1546,25 @@ __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc
     int src = srcSlot[m];
     int j = matchOld[src * kMaxContacts + i];
     bool kept = stickUsed[src * kMaxContacts + i] != 0;
+    const bool refresh = any_round((int)B.shape[a].z) || any_round((int)B.shape[b].z);
     if (j >= 0 && src < numOld)
     {
         int os = src * kMaxContacts + j;
         double ag = (double)P.alpha * (double)P.gamma;
         DV lb = ld_dv(A.lb + 3 * sl), kb = ld_dv(A.kb + 3 * sl);
         float4 kold = oldCr[(size_t)os * kCR + CR_PEN];
-        const float ko[3] = {kold.x, kold.y, kold.z};
+        const float ko[3] = {kold.x, refresh ? kPenaltyMin : kold.y, refresh ? kPenaltyMin : kold.z};
         double kbo[3] = {kb.x, kb.y, kb.z};
         for (int k = 0; k < 3; ++k)
         {
             float kg = ko[k] * P.gamma;
-            if (!(kg > kPenaltyMin && kg < kPenaltyMax))
+            if (!(kg > kPenaltyMin && kg < kPenaltyMax) || (refresh && k > 0))
                 kbo[k] = 0.0;
             else
                 kbo[k] *= (double)P.gamma;
         }
+        if (refresh)
+            lb = DV{lb.x, 0.0, 0.0};
         st_dv(A.clb + 3 * os, lb * ag);
         st_dv(A.ckb + 3 * os, DV{kbo[0], kbo[1], kbo[2]});
         if (kept)
@@ -1379,18 +1579,27 @@ __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc
     st_dv(A.rBb + 3 * sl, kept && j >= 0 && src < numOld ? DV{0, 0, 0} : rBb);
 }
 
-// Collide pose Jacobian (collide.cpp:898-955), one thread per (manifold, pose component): the
-// tangent of the basis and fresh anchors along that component, dotted with their adjoints. The
-// thread of component 0 also reverses the manifold friction sqrt(mu_A mu_B) (manifold.cpp:104).
-__global__ __launch_bounds__(64) void k_adj_collide(Bodies B, Manifolds M, int nm, AdjBuf A)
+// Collide pose Jacobian (collide.cpp:435-1181), one thread per (manifold, component): the tangent
+// of the basis and fresh anchors along that component, dotted with their adjoints. Components 0-6
+// and 7-13 are A's and B's pose (x, q) at the step start; 14-16 and 17-19 their velocities, which
+// a sphere's swept test and the push-out face rule read (pairs of boxes skip them). The thread of
+// component 0 also reverses the manifold friction sqrt(mu_A mu_B) (manifold.cpp:104; 0 for pairs
+// with a sphere).
+constexpr int kCollideComps = 20;
+__global__ __launch_bounds__(64) void k_adj_collide(Bodies B, Manifolds M, int nm, float dt, AdjBuf A)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nm * 14)
+    if (t >= nm * kCollideComps)
         return;
-    int m = t / 14, comp_ = t % 14;
+    int m = t / kCollideComps, comp_ = t % kCollideComps;
     int2 ab = M.ab[m];
     int a = ab.x, b = ab.y;
-    if (comp_ == 0)
+    const float4 hA = B.shape[a], hB = B.shape[b];
+    const int shA = (int)hA.z, shB = (int)hB.z;
+    const bool boxes = shA == kShapeBox && shB == kShapeBox;
+    if (comp_ >= 14 && boxes)
+        return;
+    if (comp_ == 0 && shA != kShapeSphere && shB != kShapeSphere)
     {
         const double fa = B.friction[a], fb = B.friction[b];
         const double mub = A.mumb[m], sq = sqrt(fa * fb);
@@ -1400,10 +1609,11 @@ __global__ __launch_bounds__(64) void k_adj_collide(Bodies B, Manifolds M, int n
             aadd(A.mub + b, mub * fa / (2.0 * sq));
         }
     }
-    bool onA = comp_ < 7;
+    // c: 0-2 position, 3-6 rotation, 7-9 velocity of the seeded body
+    const bool onA = comp_ < 7 || (comp_ >= 14 && comp_ < 17);
     if (!(B.massMoment[onA ? a : b].x > 0.0f))
         return;
-    int c = onA ? comp_ : comp_ - 7;
+    const int c = comp_ < 14 ? (onA ? comp_ : comp_ - 7) : 7 + (comp_ - 14) % 3;
     float4 pA = B.pose[4 * a + P_ILIN], qA = B.pose[4 * a + P_IANG], pB = B.pose[4 * b + P_ILIN],
            qB = B.pose[4 * b + P_IANG];
     float4 sA = B.sizeRad[a], sB = B.sizeRad[b];
@@ -1420,7 +1630,15 @@ __global__ __launch_bounds__(64) void k_adj_collide(Bodies B, Manifolds M, int n
     V3T<DualF> SA{DualF(sA.x), DualF(sA.y), DualF(sA.z)}, SB{DualF(sB.x), DualF(sB.y), DualF(sB.z)};
     NewContactsT<DualF> nc;
     V3T<DualF> r0, r1, r2;
-    int nn = collide_boxes<DualF>(PA, QA, SA, PB, QB, SB, nc, r0, r1, r2);
+    int nn;
+    if (boxes)
+        nn = collide_boxes<DualF>(PA, QA, SA, PB, QB, SB, nc, r0, r1, r2);
+    else
+    {
+        const ShapeBodyT<DualF> SBA{PA, SA, seedv(B.vel[a], onA, 7), QA, B.massMoment[a].x, hA.x, hA.y, shA};
+        const ShapeBodyT<DualF> SBB{PB, SB, seedv(B.vel[b], !onA, 7), QB, B.massMoment[b].x, hB.x, hB.y, shB};
+        nn = collide_pair<DualF>(SBA, SBB, dt, nc, r0, r1, r2);
+    }
     int n = M.count[m];
     if (nn != n)
         return;  // a branch the step did not take cannot happen: values are the step's FP32 values
@@ -1436,8 +1654,10 @@ __global__ __launch_bounds__(64) void k_adj_collide(Bodies B, Manifolds M, int n
     int body = onA ? a : b;
     if (c < 3)
         aadd(A.xb + 3 * body + c, g);
-    else
+    else if (c < 7)
         aadd(A.qb + 4 * body + (c - 3), g);
+    else
+        aadd(A.vb + 3 * body + (c - 7), g);
 }
 
 } // namespace avbd
@@ -1718,7 +1938,15 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
     B.inerAng = t.inerAng;
     Manifolds M{t.ab, t.serial, t.count, t.feature, t.cr};
     int n0 = std::max(std::max(nb, nc), std::max(nm, 1));
-    k_adj_begin<<<ceil_div(n0, 256), 256, 0, s>>>(B, h->P, t.contactRec, nc, nm, nb, A);
+    const bool rolling = h->numSpheres > 0;
+    k_adj_begin<<<ceil_div(n0, 256), 256, 0, s>>>(B, h->P, t.contactRec, nc, nm, nb, !rolling, A);
+    if (rolling)
+    {
+        const int numTasks = colors > 0 ? t.offsets[colors] : 0;
+        if (numTasks > 0)
+            k_adj_rolling<<<ceil_div(numTasks, 64), 64, 0, s>>>(B, M, t.stamps, t.tasks, numTasks, h->P, A);
+        k_adj_finish<<<ceil_div(nb, 256), 256, 0, s>>>(B, h->P, nb, A);
+    }
     const int nj = h->nj;
     if (nj > 0)
         k_adj_joint_begin<<<ceil_div(nj, 256), 256, 0, s>>>(nj, A);
@@ -1749,7 +1977,7 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
         k_adj_init<<<ceil_div(nc, 256), 256, 0, s>>>(B, M, t.contactRec, nc, t.recSrc, t.recMatch, t.recStick,
                                                      start.cr, start.numManifolds, h->P, A);
     if (nm > 0)
-        k_adj_collide<<<ceil_div(nm * 14, 64), 64, 0, s>>>(B, M, nm, A);
+        k_adj_collide<<<ceil_div(nm * kCollideComps, 64), 64, 0, s>>>(B, M, nm, h->P.dt, A);
     HIP_TRY(hipGetLastError());
     return AVBD_OK;
 }
@@ -1758,8 +1986,6 @@ int check_adjoint_scene(avbd_handle *h)
 {
     if (h->extPairsOn || !h->extContacts.empty())
         return fail(AVBD_ERR_UNSUPPORTED, "adjoint: worlds without external pairs or contacts");
-    if (h->numRound > 0)
-        return fail(AVBD_ERR_UNSUPPORTED, "adjoint: box bodies only (the reverse sweep has no round-shape offsets)");
     if (h->P.iterations > 64)
         return fail(AVBD_ERR_UNSUPPORTED, "adjoint: at most 64 iterations per step");
     return AVBD_OK;

@@ -431,8 +431,9 @@ int avbd_autotune(avbd_handle *h, int32_t trial_steps, int32_t flags, avbd_launc
 
 /* ---- Adjoint of the step (reverse-mode gradients) ----
  * No counterpart in the reference (its solver is not differentiable). For
- * worlds of boxes with contacts, joints, springs and IgnoreCollision pairs
- * (round shapes and external sources: AVBD_ERR_UNSUPPORTED): avbd_adjoint_forward
+ * single worlds of boxes, spheres, capsules and cylinders with contacts, joints,
+ * springs and IgnoreCollision pairs (external pair or contact sources:
+ * AVBD_ERR_UNSUPPORTED): avbd_adjoint_forward
  * steps num_steps times keeping
  * the state at every checkpoint_every-th step start in HBM; avbd_adjoint_backward
  * then returns, for a loss L of the final state, dL/d(initial pos, rot, vel,

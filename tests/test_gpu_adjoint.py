@@ -36,9 +36,11 @@ def _directions(state, seed):
             "mass": rng.normal(size=nb) * dyn * state.bodies["mass"], "friction": rng.normal(size=nb) * 0.1}
 
 
-def _check(gpu, oracle, state, params, steps, every, seed=0, tol=1e-4, label="", terms=False):
+def _check(gpu, oracle, state, params, steps, every, seed=0, tol=1e-4, label="", terms=False, floor=1e-12):
     """terms: measure the error against the sum of the absolute per-body terms the projection adds
-    up (sum |c_i . dpos_i|) instead of against the projection, for losses whose terms cancel."""
+    up (sum |c_i . dpos_i|) instead of against the projection, for losses whose terms cancel.
+    floor: the smallest scale the error is measured against (derivatives that vanish, e.g. a
+    sphere's spin on a static sphere, are compared in absolute terms)."""
     nb = len(state.bodies)
     rng = np.random.default_rng(seed + 100)
     c = rng.normal(size=(nb, 3)) * (state.bodies["mass"] > 0)[:, None]
@@ -57,7 +59,7 @@ def _check(gpu, oracle, state, params, steps, every, seed=0, tol=1e-4, label="",
         ref = float((c * tan[:, 0:3]).sum())
         got = float((g[k] * u[k]).sum())
         scale = float(np.abs(c * tan[:, 0:3]).sum()) if terms else abs(ref)
-        errs[k] = abs(got - ref) / max(scale, 1e-12)
+        errs[k] = abs(got - ref) / max(scale, floor)
         print(f"{label} {k:8s} adjoint {got: .10e}  forward-mode {ref: .10e}  rel {abs(got - ref) / max(abs(ref), 1e-12):.2e}"
               f"  of sum|terms| {abs(got - ref) / max(float(np.abs(c * tan[:, 0:3]).sum()), 1e-12):.2e}", flush=True)
     for k in KINDS:
@@ -171,15 +173,56 @@ def test_adjoint_soft_body_with_ignore_pairs(gpu, oracle):
     _check(gpu, oracle, st, prm, 20, 5, tol=1e-5, label="soft body", terms=True)
 
 
-def test_adjoint_rejects_round_shapes(gpu, oracle):
+@pytest.mark.parametrize("kind", ["sphere", "capsule", "cylinder"])
+def test_adjoint_round_stacks(gpu, oracle, kind):
+    """Columns of three spheres, crosswise capsules or standing cylinders on a static slab, settled
+    30 steps: round contact offsets (manifold.cpp:29-90) in the reversed stamps, duals and force
+    init, the round narrowphase's pose Jacobian (collide.cpp:435-1181), the spheres' rolling
+    friction against the slab (solver.cpp:628-660) and their frictionless manifolds."""
+    from avbd import scenes
+    st, prm = _settled(oracle, scenes.round_stack(kind, count=3), 30)
+    assert len(st.manifolds) == 3
+    _check(gpu, oracle, st, prm, 20, 5, tol=1e-5, label=f"{kind} stack", terms=True)
+
+
+def _round_case_names():
+    from avbd import scenes
+    return [sc.name for sc in scenes.round_cases()]
+
+
+@pytest.mark.parametrize("name", _round_case_names())
+def test_adjoint_round_branch_cases(gpu, oracle, name):
+    """Every branch case of the round narrowphase and contact offsets (thin static tops, swept
+    spheres whose Jacobian includes the velocities, push-out faces, capsule cores, cylinder caps,
+    rims, sides and crossings, round pairs), 30 steps from rest with checkpoints every 10; error
+    against sum |c_i . dpos_i| with a 1e-6 floor (measured up to 2e-5)."""
+    from avbd import scenes
+    from avbd.state import default_params
+    sc = next(c for c in scenes.round_cases() if c.name == name)
+    st = scenes.build_state(sc, oracle.OracleMaker())
+    _check(gpu, oracle, st, default_params(gravity=sc.gravity), 30, 10, tol=1e-4, label=name, terms=True,
+           floor=1e-6)
+
+
+def test_adjoint_mixed_pile(gpu, oracle):
+    """60 boxes, spheres, capsules and cylinders in four walls, settled 60 steps (every pair kind)."""
+    from avbd import scenes
+    st, prm = _settled(oracle, scenes.mixed_pile(60, seed=3), 60)
+    assert len(st.manifolds) > 60
+    _check(gpu, oracle, st, prm, 30, 10, tol=1e-5, label="mixed pile", terms=True)
+
+
+def test_adjoint_rejects_external_sources(gpu, oracle):
     from avbd import scenes
     from avbd.state import default_params
     st = scenes.build_state(scenes.round_stack("sphere", count=3), gpu.Maker())
     w = gpu.HipWorld.for_state(st, default_params())
-    with pytest.raises(gpu.AvbdError, match="box bodies only"):
+    w.set_external_pairs(np.array([[1, 2]], np.int32))
+    with pytest.raises(gpu.AvbdError, match="external"):
         w.adjoint_forward(2, 1)
     with pytest.raises(gpu.AvbdError, match="no recorded forward"):
         w.adjoint_backward()
+    w.close()
 
 
 def _com_loss_parts(pos, mass, pile, target):

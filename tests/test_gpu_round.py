@@ -105,15 +105,3 @@ def test_round_worlds_batched_equal_reference(gpu):
             for n, part in zip(names, parts):
                 np.testing.assert_array_equal(body_block(part), z[n + ".traj"][k - 1], err_msg=f"{n} step {k}")
     w.close()
-
-
-def test_adjoint_rejects_round_shapes(gpu):
-    from avbd import scenes
-    from avbd.state import default_params
-    sc = scenes.round_stack("sphere", 3)
-    w = gpu.HipWorld.for_state(scenes.build_state(sc, gpu.Maker()), default_params())
-    with pytest.raises(gpu.AvbdError, match="box bodies only"):
-        w.adjoint_forward(2, 1)
-    w.close()
-
-
