@@ -400,7 +400,6 @@ void radix_sort(avbd_handle *h, unsigned *&keys, int *&vals, unsigned *&keysAlt,
     }
 }
 
-__global__ void k_barrier_reset(GridBarrier *gb) { gb->count = 0u; }
 
 
 // Bodies at `dev` -> host records (fields the body table carries).
@@ -700,8 +699,7 @@ int enqueue_prepare(avbd_handle *h)
         HIP_TRY(hipEventRecord(h->ev[0], s));
         HIP_TRY(hipEventRecord(h->ring[avbd_handle::kSlots * (h->ringSteps % avbd_handle::kRing) + 0], s));
     }
-    k_step_begin<<<1, 1, 0, s>>>(ctr);
-    k_barrier_reset<<<1, 1, 0, s>>>(h->gb);
+    k_step_begin<<<1, 1, 0, s>>>(ctr, h->gb);
 
     // ---- broadphase: sweep and prune (solver.cpp:488-577), or the grid / all pairs, or the
     // external pairs (solver.cpp:1075-1076) ----
@@ -807,7 +805,6 @@ int enqueue_prepare(avbd_handle *h)
                                                   S.stampStart, S.stamps, h->maxStamps, &ctr->overflow);
     const int *ignoreDeg = h->numIgnore > 0 ? h->ignoreDeg : nullptr;
     k_warmstart<<<ceil_div(N, 1024), 1024, 0, s>>>(h->B, h->P, S.deg, ignoreDeg, N, S.color, ctr);
-    k_step_bytes<<<1, 1, 0, s>>>(ctr, N, h->P.iterations);
 
     // ---- colouring (avbd_gpu_solver.cpp:1812-1865) ----
     {
@@ -850,7 +847,7 @@ int enqueue_prepare(avbd_handle *h)
         scan(h, S.radixHist, h->radixHistScan, nullptr, 256 * tiles, nullptr);
         k_radix_scatter<<<tiles, kRadixThreads, 0, s>>>(keysAlt, valsAlt, N, 8, h->radixHistScan, tiles, keys, S.colorBodies,
                                               0);
-        k_color_offsets<<<1, 256, 0, s>>>(h->radixHistScan, tiles, N, S.colorOffsets);
+        k_color_offsets<<<1, 256, 0, s>>>(h->radixHistScan, tiles, N, S.colorOffsets, ctr, N, h->P.iterations);
         k_tasks<<<ceil_div(N, 256), 256, 0, s>>>(S.colorBodies, S.stampStart, S.stampCount, S.color, N, S.tasks, h->dfDone,
                                                  S.colorOffsets);
     }

@@ -498,8 +498,10 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 }
 
 // Per-step counter reset (a kernel instead of a memset so the step can be graph-captured).
-__global__ void k_step_begin(Counters *c)
+// Start of a step: the per-step counters, and the colouring's grid barrier.
+__global__ void k_step_begin(Counters *c, GridBarrier *gb)
 {
+    gb->count = 0u;
     c->numChunks = 0;
     c->numHits = 0;
     c->numCreated = 0;
@@ -1531,7 +1533,7 @@ __global__ void k_joint_init(Joint *joints, int nj, Bodies B, Params P, Counters
 //   iteration = Ns*(100+28) + sum_c [e_c*(108+56) + (108+2*56+28)]
 //             + sum_j [e_j*(124+28) + (124+2*28+52)] + sum_s e_s*(40+28)
 //   step      = iters*iteration + N*(148+92+16+64+8) + hits*(80+8) + 36*M + 130*contacts
-__global__ void k_step_bytes(Counters *c, int nBodies, int iterations)
+__device__ void step_bytes(Counters *c, int nBodies, int iterations)
 {
     double ns = c->solvedCount;
     double primal = ns * 128.0 + (double)c->contactEndpoints * 164.0 + (double)c->jointEndpoints * 152.0 +
@@ -2053,9 +2055,13 @@ __global__ void k_color_keys(const int *color, const int *stampCount, int n, uns
 // colorOffsets[0..256]: the colour buckets' starts (and n); colorOffsets[kSegBase + 3 c + q]: where
 // the bodies of colour c with at most seg_stamps(q) stamps start (k_tasks writes the boundaries
 // that exist; a colour with no such body keeps its end, set here).
-__global__ void k_color_offsets(const int *scannedHist, int tiles, int n, int *colorOffsets)
+// Also the step's algorithmic bytes (its counts are final once the warm start has run).
+__global__ void k_color_offsets(const int *scannedHist, int tiles, int n, int *colorOffsets, Counters *c,
+                                int nBodies, int iterations)
 {
     int d = threadIdx.x;
+    if (d == 0)
+        step_bytes(c, nBodies, iterations);
     if (d < 256)
     {
         colorOffsets[d] = scannedHist[d * tiles];
