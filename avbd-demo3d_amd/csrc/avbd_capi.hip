@@ -752,10 +752,10 @@ int enqueue_prepare(avbd_handle *h)
                                                                                    S.chunkOwner);
         const int sweepBlocks = std::max(1, std::min(ceil_div(h->maxChunks, 4 * kSweepGroup), 4096));
         k_sweep_count_g<<<sweepBlocks, 256, 0, s>>>(S.chunkOwner, S.chunkStart, h->sweepEnd, ctr, S.sPos, S.hitCount,
-                                                    S.hitMask);
+                                                    S.hitMask, S.hash,
+                                                    h->hashSize);
         // numHits is clamped to the pair capacity here (k_sweep_emit_g bounds its writes itself)
         scan(h, S.hitCount, S.hitStart, &ctr->numChunks, 0, &ctr->numHits, h->maxPairs, 1);
-        k_fill_u64<<<grid_for(h->hashSize), 256, 0, s>>>(S.hash, h->hashSize, ~0ull);
         k_hash_insert<<<G, 256, 0, s>>>(h->cur.ab, ctr, h->joints, h->nj, h->springs, h->ns, h->ignorePairs,
                                         h->numIgnore, S.hash, S.hashMask, ctr);
         k_sweep_emit_g<<<sweepBlocks, 256, 0, s>>>(S.chunkOwner, S.chunkStart, ctr, S.hitMask, S.sIdx, S.hitStart,
@@ -781,7 +781,8 @@ int enqueue_prepare(avbd_handle *h)
     }
     // the kept count is the new manifold count (nothing until the compaction reads the old one)
     scan(h, S.keep, S.keepRank, &ctr->numTotal, 0, &ctr->numManifolds);
-    k_compact_manifolds<<<G / 2, 1024, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc);
+    k_compact_manifolds<<<G / 2, 1024, 0, s>>>(h->tmp, h->cur, S.keep, S.keepRank, h->B.massMoment, ctr, h->recSrc,
+                                               S.deg, S.adjCursor, N);
     scan(h, h->cur.count, S.contactStart, &ctr->numManifolds, 0, &ctr->contactCount);
     k_contact_records<<<G, 256, 0, s>>>(h->cur.count, h->cur.ab, S.contactStart, ctr, S.contactRec);
     if (h->nj > 0)
@@ -790,7 +791,6 @@ int enqueue_prepare(avbd_handle *h)
         HIP_TRY(hipEventRecord(h->ev[2], s));
 
     // ---- per-body force lists, body warm start (solver.cpp:1134-1169) ----
-    k_fill_int2<<<grid_for(N), 256, 0, s>>>(S.deg, S.adjCursor, N, 0);
     k_adj<false><<<G, 256, 0, s>>>(h->cur, ctr, h->joints, h->nj, h->springs, h->ns, h->B.massMoment, S.deg, S.adjStart,
                                    S.adjCursor, S.adj, S.adjSerial, S.adjPartner, h->maxAdj, &ctr->overflow);
     scan(h, S.deg, S.adjStart, nullptr, N, h->adjTotal);

@@ -482,14 +482,6 @@ __global__ void k_fill_int(int *p, int n, int v)
         p[i] = v;
 }
 
-__global__ void k_fill_int2(int *p, int *q, int n, int v)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    {
-        p[i] = v;
-        q[i] = v;
-    }
-}
 
 __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v)
 {
@@ -876,9 +868,13 @@ __device__ __forceinline__ float4 shfl4(float4 v, int src)
     return make_float4(__shfl(v.x, src, 64), __shfl(v.y, src, 64), __shfl(v.z, src, 64), __shfl(v.w, src, 64));
 }
 
+// Also empties the constrained-pair hash (hashSize entries) that k_hash_insert fills next.
 __global__ void k_sweep_count_g(const int *owner, const int *chunkStart, const int *sweepEnd, const Counters *ctr,
-                                const float4 *sPos, int *hitCount, unsigned long long *hitMask)
+                                const float4 *sPos, int *hitCount, unsigned long long *hitMask,
+                                unsigned long long *hash, int hashSize)
 {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < hashSize; i += gridDim.x * blockDim.x)
+        hash[i] = ~0ull;
     const int T = ctr->numChunks;
     const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
     const int waves = gridDim.x * wpb;
@@ -1447,10 +1443,18 @@ __global__ __launch_bounds__(kNpBoxThreads) void k_narrowphase_boxes(Manifolds c
 // srcSlot (adjoint recording, may be null): the pre-compaction slot of each kept manifold.
 // Eight lanes per manifold: lane q copies float4 q of each live contact record (and feature q),
 // so a manifold's records move as whole 128-byte rows.
+// Also clears the per-body force-list counts and cursors (deg, cursor: nBodies each), which the
+// force lists of this step are built from after the compaction.
 __global__ void k_compact_manifolds(Manifolds tmp, Manifolds cur, const int *keep, const int *keepRank,
-                                    const float4 *massMoment, Counters *ctr, int *srcSlot)
+                                    const float4 *massMoment, Counters *ctr, int *srcSlot, int *deg, int *cursor,
+                                    int nBodies)
 {
     static_assert(kCR == 8 && kMaxContacts == 8, "one lane per record float4 and per feature slot");
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nBodies; i += gridDim.x * blockDim.x)
+    {
+        deg[i] = 0;
+        cursor[i] = 0;
+    }
     const int total = ctr->numTotal;
     const int q = threadIdx.x & 7;
     int endpoints = 0;
