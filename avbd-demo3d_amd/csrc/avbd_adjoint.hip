@@ -216,11 +216,12 @@ __device__ void offset_adj(int code, bool isA, float radius, float hl, DQ q, DV 
 // values that scale the friction cone and the friction derivative would differ by percents.
 __device__ __forceinline__ V3 f3(DV v) { return V3{(float)v.x, (float)v.y, (float)v.z}; }
 __device__ __forceinline__ Q4 f4(DQ q) { return Q4{(float)q.x, (float)q.y, (float)q.z, (float)q.w}; }
-__device__ __forceinline__ void cfwd(const CIn &in, CFwd &f)
+// Round: the world may hold round shapes (without them the shape paths are not compiled in).
+template <bool Round> __device__ __forceinline__ void cfwd(const CIn &in, CFwd &f)
 {
     const Q4 qA = f4(in.qA), qB = f4(in.qB);
     V3 rAW, rBW;
-    if (in.code == 0)
+    if (!Round || in.code == 0)
     {
         rAW = rotate(qA, f3(in.rA));
         rBW = rotate(qB, f3(in.rB));
@@ -261,6 +262,7 @@ __device__ __forceinline__ void cfwd(const CIn &in, CFwd &f)
 
 // Given F_bar, an extra C_bar and pre-accumulated rAW_bar / rBW_bar / n_bar (Jacobian terms of a
 // stamp), accumulate every input adjoint into o.
+template <bool Round>
 __device__ __forceinline__ void cadj(const CIn &in, const CFwd &f, const double Fb[3], const double Cx[3], DV rAWb,
                                      DV rBWb, COut &o)
 {
@@ -308,6 +310,12 @@ __device__ __forceinline__ void cadj(const CIn &in, const CFwd &f, const double 
     o.iLB += dALb;
     dqdiff_adj(in.qA, in.iAA, dAAb, o.qA, o.iAA);
     dqdiff_adj(in.qB, in.iAB, dBAb, o.qB, o.iAB);
+    if (!Round)
+    {
+        drot_adj(in.qA, in.rA, rAWb, o.qA, o.rA);
+        drot_adj(in.qB, in.rB, rBWb, o.qB, o.rB);
+        return;
+    }
     offset_adj(in.code, true, in.radA, in.hlA, in.qA, in.rA, in.n[0], rAWb, o.qA, o.rA, o.n[0]);
     offset_adj(in.code, false, in.radB, in.hlB, in.qB, in.rB, in.n[0], rBWb, o.qB, o.rB, o.n[0]);
 }
@@ -350,33 +358,6 @@ __device__ __forceinline__ void cin_load(CIn &in, const float4 *cr, float4 pen, 
     in.hlA = cr[CR_T2].w;
     in.radB = cr[CR_T1].w;
     in.hlB = cr[CR_C0].w;
-}
-
-// FP32 evaluation of the contact exactly as the step did it: branch flags and the stamp record.
-__device__ __forceinline__ ContactEval c_eval32(const float4 *cr, float4 pen4, float4 lam4, float4 pA4, float4 qA4,
-                                               float4 pB4, float4 qB4, float4 iA4, float4 jA4, float4 iB4,
-                                               float4 jB4, float oma, V3 &b0, V3 &b1, V3 &b2, V3 &ja0, V3 &ja1,
-                                               V3 &ja2, V3 &jb0, V3 &jb1, V3 &jb2)
-{
-    Q4 qA = ldq(qA4), qB = ldq(qB4);
-    V3 dqAL = ld3(pA4) - ld3(iA4);
-    V3 dqAA = qdiff(qA, ldq(jA4));
-    V3 dqBL = ld3(pB4) - ld3(iB4);
-    V3 dqBA = qdiff(qB, ldq(jB4));
-    b0 = ld3(cr[CR_N]);
-    b1 = ld3(cr[CR_T1]);
-    b2 = ld3(cr[CR_T2]);
-    V3 nb0 = -b0, nb1 = -b1, nb2 = -b2;
-    V3 rAW, rBW;
-    contact_anchors(cr[CR_RA], cr[CR_RB], cr[CR_C0], cr[CR_N], cr[CR_T1], cr[CR_T2], qA, qB, rAW, rBW);
-    ja0 = cross(rAW, b0);
-    ja1 = cross(rAW, b1);
-    ja2 = cross(rAW, b2);
-    jb0 = cross(rBW, nb0);
-    jb1 = cross(rBW, nb1);
-    jb2 = cross(rBW, nb2);
-    return contact_eval(ld3(cr[CR_C0]), ld3(pen4), ld3(lam4), cr[CR_RA].w, oma, b0, b1, b2, nb0, nb1, nb2, ja0, ja1,
-                        ja2, jb0, jb1, jb2, dqAL, dqBL, dqAA, dqBA);
 }
 
 __device__ __forceinline__ bool eval_clamp0(const ContactEval &ev, float4 pen, float4 lam)
@@ -661,6 +642,7 @@ __device__ __forceinline__ void cout_store(const COut &o, int a, int b, int sl, 
 // Dual update of one contact, reversed (manifold.cpp:231-280). Poses after this iteration's
 // primal (tapeCur), penalty/lambda before the dual (tapeLamPrev), branch flags by contact slot.
 // 64-thread blocks (the launch's shape): the bound lets the compiler keep the FP64 chain in registers.
+template <bool Round>
 __global__ __launch_bounds__(64) void k_adj_dual(Bodies B, Manifolds M, const int4 *contactRec, int nc, const float4 *tapeCur,
                            const float4 *tapeLamPrev, const unsigned char *flags, Params P, AdjBuf A)
 {
@@ -681,7 +663,7 @@ __global__ __launch_bounds__(64) void k_adj_dual(Bodies B, Manifolds M, const in
     in.clamp0 = (fl & kFlagClamp0) != 0;
     in.scaled = (fl & kFlagScaled) != 0;
     CFwd f;
-    cfwd(in, f);
+    cfwd<Round>(in, f);
     // adjoints of the updated lambda and penalty
     DV lbn = ld_dv(A.lb + 3 * sl), kbn = ld_dv(A.kb + 3 * sl);
     double Fb[3] = {lbn.x, lbn.y, lbn.z};
@@ -701,7 +683,7 @@ __global__ __launch_bounds__(64) void k_adj_dual(Bodies B, Manifolds M, const in
     }
     COut o;
     cout_zero(o);
-    cadj(in, f, Fb, Cx, DV{0, 0, 0}, DV{0, 0, 0}, o);
+    cadj<Round>(in, f, Fb, Cx, DV{0, 0, 0}, DV{0, 0, 0}, o);
     // the dual overwrote lambda and penalty: their old adjoints are what cadj produced, plus the
     // penalty carried through the update
     st_dv(A.lb + 3 * sl, o.lam);
@@ -1111,6 +1093,7 @@ __device__ __forceinline__ void wave_sum_group(double &v)
         v += __shfl_xor(v, off, kAdjLanes);
 }
 
+template <bool Round>
 __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds M, const Spring *springs,
                                                             const Joint *jointsTape, const int4 *stamps,
                                                             const int4 *tasks, int begin, int end, const int *color,
@@ -1244,7 +1227,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
         cin_load(in, cr, pen4, lam4, pA, qA, pB, qB, iA, jA, iB, jB, oma);
         in.clamp0 = (fl & kFlagClamp0) != 0;
         in.scaled = (fl & kFlagScaled) != 0;
-        cfwd(in, f);
+        cfwd<Round>(in, f);
     };
     if (!solver && valid && lane < ns)
         load_stamp(lane);
@@ -1361,7 +1344,7 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_primal(Bodies B, Manifolds 
         }
         double Cx[3] = {0.0, 0.0, 0.0};
         DV rAWb = isA ? rWb : DV{0, 0, 0}, rBWb = isA ? DV{0, 0, 0} : rWb;
-        cadj(in, f, Fb, Cx, rAWb, rBWb, o);
+        cadj<Round>(in, f, Fb, Cx, rAWb, rBWb, o);
         if (isA)
         {
             oxb += o.pA;
@@ -1492,6 +1475,7 @@ __global__ void k_adj_warmstart(Bodies B, Params P, const int *deg, const int *i
 // their adjoint to the previous step's contact. Pairs with a round shape refresh their anchors
 // (manifold.cpp:14-27): a matched contact keeps only its normal lambda and penalty (the
 // tangential rows restart at 0 and the minimum penalty), and its offsets are the shapes'.
+template <bool Round>
 __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc, const int *srcSlot,
                            const int *matchOld, const int *stickUsed, const float4 *oldCr, int numOld, Params P,
                            AdjBuf A)
@@ -1507,7 +1491,7 @@ __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc
     DV pA = dv4(B.pose[4 * a + P_ILIN]), pB = dv4(B.pose[4 * b + P_ILIN]);
     const int code = __float_as_int(cr[CR_RB].w);
     DV d;
-    if (code == 0)
+    if (!Round || code == 0)
         d = (pA + drot(qA, rA)) - (pB + drot(qB, rB));
     else
     {
@@ -1528,10 +1512,17 @@ __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc
     DQ qAb{0, 0, 0, 0}, qBb{0, 0, 0, 0};
     DV rAb = ld_dv(A.rAb + 3 * sl), rBb = ld_dv(A.rBb + 3 * sl), n0b{0, 0, 0};
     const DV n0 = dv4(cr[CR_N]);
-    offset_adj(code, true, cr[CR_N].w, cr[CR_T2].w, qA, rA, n0, db, qAb, rAb, n0b);
-    offset_adj(code, false, cr[CR_T1].w, cr[CR_C0].w, qB, rB, n0, -db, qBb, rBb, n0b);
-    if (code != 0)
+    if (!Round || code == 0)
+    {
+        drot_adj(qA, rA, db, qAb, rAb);
+        drot_adj(qB, rB, -db, qBb, rBb);
+    }
+    else
+    {
+        offset_adj(code, true, cr[CR_N].w, cr[CR_T2].w, qA, rA, n0, db, qAb, rAb, n0b);
+        offset_adj(code, false, cr[CR_T1].w, cr[CR_C0].w, qB, rB, n0, -db, qBb, rBb, n0b);
         atomic_add3(A.basisb + 9 * m, n0b);
+    }
     bool massA = B.massMoment[a].x > 0.0f, massB = B.massMoment[b].x > 0.0f;
     if (massA)
     {
@@ -1546,7 +1537,7 @@ __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc
     int src = srcSlot[m];
     int j = matchOld[src * kMaxContacts + i];
     bool kept = stickUsed[src * kMaxContacts + i] != 0;
-    const bool refresh = any_round((int)B.shape[a].z) || any_round((int)B.shape[b].z);
+    const bool refresh = Round && (any_round((int)B.shape[a].z) || any_round((int)B.shape[b].z));
     if (j >= 0 && src < numOld)
     {
         int os = src * kMaxContacts + j;
@@ -1585,21 +1576,23 @@ __global__ void k_adj_init(Bodies B, Manifolds M, const int4 *contactRec, int nc
 // a sphere's swept test and the push-out face rule read (pairs of boxes skip them). The thread of
 // component 0 also reverses the manifold friction sqrt(mu_A mu_B) (manifold.cpp:104; 0 for pairs
 // with a sphere).
-constexpr int kCollideComps = 20;
+template <bool Round> constexpr int collide_comps() { return Round ? 20 : 14; }
+template <bool Round>
 __global__ __launch_bounds__(64) void k_adj_collide(Bodies B, Manifolds M, int nm, float dt, AdjBuf A)
 {
+    constexpr int kComps = collide_comps<Round>();
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nm * kCollideComps)
+    if (t >= nm * kComps)
         return;
-    int m = t / kCollideComps, comp_ = t % kCollideComps;
+    int m = t / kComps, comp_ = t % kComps;
     int2 ab = M.ab[m];
     int a = ab.x, b = ab.y;
     const float4 hA = B.shape[a], hB = B.shape[b];
     const int shA = (int)hA.z, shB = (int)hB.z;
-    const bool boxes = shA == kShapeBox && shB == kShapeBox;
+    const bool boxes = !Round || (shA == kShapeBox && shB == kShapeBox);
     if (comp_ >= 14 && boxes)
         return;
-    if (comp_ == 0 && shA != kShapeSphere && shB != kShapeSphere)
+    if (comp_ == 0 && (!Round || (shA != kShapeSphere && shB != kShapeSphere)))
     {
         const double fa = B.friction[a], fb = B.friction[b];
         const double mub = A.mumb[m], sq = sqrt(fa * fb);
@@ -1633,7 +1626,7 @@ __global__ __launch_bounds__(64) void k_adj_collide(Bodies B, Manifolds M, int n
     int nn;
     if (boxes)
         nn = collide_boxes<DualF>(PA, QA, SA, PB, QB, SB, nc, r0, r1, r2);
-    else
+    else if (Round)
     {
         const ShapeBodyT<DualF> SBA{PA, SA, seedv(B.vel[a], onA, 7), QA, B.massMoment[a].x, hA.x, hA.y, shA};
         const ShapeBodyT<DualF> SBB{PB, SB, seedv(B.vel[b], !onA, 7), QB, B.massMoment[b].x, hB.x, hB.y, shB};
@@ -1938,7 +1931,7 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
     B.inerAng = t.inerAng;
     Manifolds M{t.ab, t.serial, t.count, t.feature, t.cr};
     int n0 = std::max(std::max(nb, nc), std::max(nm, 1));
-    const bool rolling = h->numSpheres > 0;
+    const bool rolling = h->numSpheres > 0, round = h->numRound > 0;
     k_adj_begin<<<ceil_div(n0, 256), 256, 0, s>>>(B, h->P, t.contactRec, nc, nm, nb, !rolling, A);
     if (rolling)
     {
@@ -1957,7 +1950,12 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
         const unsigned char *sfl = t.tapeStampFlags + (size_t)it * h->maxStamps;
         const unsigned char *dfl = t.tapeDualFlags + (size_t)it * S.slotCap;
         if (nc > 0)
-            k_adj_dual<<<ceil_div(nc, 64), 64, 0, s>>>(B, M, t.contactRec, nc, cur, lam, dfl, h->P, A);
+        {
+            if (round)
+                k_adj_dual<true><<<ceil_div(nc, 64), 64, 0, s>>>(B, M, t.contactRec, nc, cur, lam, dfl, h->P, A);
+            else
+                k_adj_dual<false><<<ceil_div(nc, 64), 64, 0, s>>>(B, M, t.contactRec, nc, cur, lam, dfl, h->P, A);
+        }
         const Joint *jt = nj > 0 ? t.tapeJoint + (size_t)it * h->maxJoints : nullptr;
         if (nj > 0)
             k_adj_joint_dual<<<ceil_div(nj, 64), 64, 0, s>>>(jt, jt + h->maxJoints, nj, cur, B, h->P, A);
@@ -1965,8 +1963,14 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
         {
             int b = t.offsets[col], e = t.offsets[col + 1];
             if (e > b)
-                k_adj_primal<<<ceil_div(e - b, kAdjBodies), kAdjThreads, 0, s>>>(
-                    B, M, h->springs, jt, t.stamps, t.tasks, b, e, t.color, col, prev, cur, lam, sys, sfl, h->P, A);
+            {
+                if (round)
+                    k_adj_primal<true><<<ceil_div(e - b, kAdjBodies), kAdjThreads, 0, s>>>(
+                        B, M, h->springs, jt, t.stamps, t.tasks, b, e, t.color, col, prev, cur, lam, sys, sfl, h->P, A);
+                else
+                    k_adj_primal<false><<<ceil_div(e - b, kAdjBodies), kAdjThreads, 0, s>>>(
+                        B, M, h->springs, jt, t.stamps, t.tasks, b, e, t.color, col, prev, cur, lam, sys, sfl, h->P, A);
+            }
         }
     }
     k_adj_warmstart<<<ceil_div(nb, 256), 256, 0, s>>>(B, h->P, t.deg, h->numIgnore > 0 ? h->ignoreDeg : nullptr,
@@ -1974,10 +1978,21 @@ int reverse_step(avbd_handle *h, const StepStore &t, const Snapshot &start)
     if (nj > 0)
         k_adj_joint_init<<<ceil_div(nj, 256), 256, 0, s>>>(start.joints, nj, B, h->P, A);
     if (nc > 0)
-        k_adj_init<<<ceil_div(nc, 256), 256, 0, s>>>(B, M, t.contactRec, nc, t.recSrc, t.recMatch, t.recStick,
+    {
+        if (round)
+            k_adj_init<true><<<ceil_div(nc, 256), 256, 0, s>>>(B, M, t.contactRec, nc, t.recSrc, t.recMatch, t.recStick,
                                                      start.cr, start.numManifolds, h->P, A);
+        else
+            k_adj_init<false><<<ceil_div(nc, 256), 256, 0, s>>>(B, M, t.contactRec, nc, t.recSrc, t.recMatch, t.recStick,
+                                                     start.cr, start.numManifolds, h->P, A);
+    }
     if (nm > 0)
-        k_adj_collide<<<ceil_div(nm * kCollideComps, 64), 64, 0, s>>>(B, M, nm, h->P.dt, A);
+    {
+        if (round)
+            k_adj_collide<true><<<ceil_div(nm * collide_comps<true>(), 64), 64, 0, s>>>(B, M, nm, h->P.dt, A);
+        else
+            k_adj_collide<false><<<ceil_div(nm * collide_comps<false>(), 64), 64, 0, s>>>(B, M, nm, h->P.dt, A);
+    }
     HIP_TRY(hipGetLastError());
     return AVBD_OK;
 }
