@@ -43,7 +43,7 @@ enum avbd_status {
     AVBD_ERR_CAPACITY = -2,      /* a buffer sized at avbd_create would overflow */
     AVBD_ERR_HIP = -3,           /* a HIP runtime call failed */
     AVBD_ERR_NO_DEVICE = -4,
-    AVBD_ERR_UNSUPPORTED = -5,   /* feature outside the step (shape code, adjoint of a non-box world) */
+    AVBD_ERR_UNSUPPORTED = -5,   /* feature outside the step (shape code; adjoint with external sources) */
     AVBD_ERR_INJECTED = -6       /* avbd_debug_inject_fault armed this failure */
 };
 
