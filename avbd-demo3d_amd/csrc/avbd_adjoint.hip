@@ -699,8 +699,11 @@ __global__ __launch_bounds__(64) void k_adj_dual(Bodies B, Manifolds M, const in
 // with the step's taped system and dx, while the stamp wave loads its first round of stamps and
 // re-evaluates their forces (the two do not depend on each other, so their latencies overlap);
 // after the block barrier the stamp wave reverses every stamp: own-pose adjoints are summed over
-// the lanes, partner ones go out atomically.
-constexpr int kAdjLanes = 16;
+// the lanes, partner ones go out atomically. A whole wave per body: the reversal is bound by the
+// latency of its slowest waves (a colour of a small world is a few hundred bodies, far fewer waves
+// than SIMDs), so one round of stamps for bodies of up to 64 stamps and one body per block took the
+// projectile workload's backward from 289 to 242 ms (16 lanes per body: 4 bodies per block).
+constexpr int kAdjLanes = 64;
 constexpr int kAdjBodies = 64 / kAdjLanes;  // bodies per block
 constexpr int kAdjThreads = 128;            // a solver wave and a stamp wave
 
