@@ -136,7 +136,9 @@ struct avbd_handle
     int maxWorldBodies = 0;  // largest world of the batch (sizes k_color_worlds' LDS)
     int npPolyCap = kNpPolyCap;  // clip-polygon vertices the box-only narrowphase keeps in LDS
     // launch shapes of the iteration kernels, and the iteration graph built from them
-    avbd_launch_config launch{16, 256, 256, 0, 0, 1};
+    // default: adaptive lanes per body in 64-thread blocks (the fastest shape measured on the settled
+    // 50,000-box pile, the batched worlds and the 1,089-body projectile workload)
+    avbd_launch_config launch{0, 64, 256, 0, 0, 1};
     // broadphase mode (Solver::broadphaseMode, solver.h:366) and the grid's buffers, allocated
     // when a mode other than sweep and prune is first set
     int broadphaseMode = AVBD_BROADPHASE_SAP;

@@ -402,8 +402,8 @@ int avbd_batch_num_worlds(avbd_handle *h, int32_t *num_worlds);
 /* ---- Launch shapes and the autotuner ----
  * No counterpart in the reference (its WebGPU lane uses fixed 64-thread
  * workgroups, avbd_gpu_solver.cpp:502-714). Every shape gives bit-identical
- * results; shapes differ only in speed. Defaults: 16 lanes, 256 threads,
- * dual 256 threads, no graph, no dataflow. */
+ * results; shapes differ only in speed. Defaults: adaptive lanes (0), 64
+ * threads, dual 256 threads, no graph, no dataflow, lazy dual. */
 typedef struct avbd_launch_config {
     int32_t primal_lanes;    /* lanes per body in the primal kernel: 8, 16 or 32; 0: per body the
                                 fewest of 8, 16, 32, 64 that hold its stamps (the flow form then
