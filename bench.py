@@ -627,6 +627,23 @@ def batched(args, params):
     sep_ms = float(np.mean(per_world)) * args.batch_worlds
     log(f"one world at a time: {np.mean(per_world):.3f} ms per world-step over {len(sample)} sampled worlds "
         f"-> {sep_ms:.1f} ms per step of all {args.batch_worlds} worlds")
+    # the same window on a fresh handle with the other launch form (iteration graph <-> direct
+    # launches), every other launch field as chosen above, so the line reports both
+    ms_form = {int(launch["use_graph"]): ms}
+    alt = dict(launch, use_graph=1 - int(launch["use_graph"]))
+    other = capi.HipWorld.for_batch(states, params)
+    other.step(2)
+    other.set_launch_config(**alt)
+    other.step(args.batch_warmup)
+    other.synchronize()
+    t2 = time.perf_counter()
+    other.step(args.batch_steps)
+    other.synchronize()
+    ms_form[alt["use_graph"]] = (time.perf_counter() - t2) * 1e3 / args.batch_steps
+    other.close()
+    forms = {"direct_ms_per_step": ms_form[0], "iteration_graph_ms_per_step": ms_form[1],
+             "picked": "iteration_graph" if launch["use_graph"] else "direct"}
+    log(f"batched: direct launches {ms_form[0]:.3f} ms/step, iteration graph {ms_form[1]:.3f} ms/step")
     return {
         "workload": f"{args.batch_worlds} worlds, body counts log-uniform in [8, 512] (stack fields, piles, "
                     f"chains); dt=1/60, 10 iterations; steps {args.batch_warmup + 3}-"
@@ -643,6 +660,7 @@ def batched(args, params):
                           "ms_per_world_step": float(np.mean(per_world)),
                           "ms_per_step_all_worlds": sep_ms},
         "speedup_vs_one_at_a_time": sep_ms / ms,
+        "launch_forms": forms,
     }
 
 
